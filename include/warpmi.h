@@ -1,0 +1,235 @@
+/*
+ * warpmi.h -- C ABI of libwarpmi.so, the MI355X (gfx950) implementation of
+ * WARP's 3-D electrostatic PIC hot path: CIC deposit, multigrid Poisson
+ * solve with subgrid-resolved conductors, field gather and leapfrog push.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative code on error;
+ *    wm_last_error() then returns a message.  Non-convergence of the
+ *    multigrid solve is not an error: it returns 0 with *mgerror > mgtol,
+ *    as the reference does.
+ *  - Grid arrays are device pointers in the reference's Fortran order
+ *    (x fastest): phi is (-1:nx+1,-1:ny+1,-1:nz+1), rho is (0:nx,0:ny,0:nz),
+ *    selfe is (0:2,0:nx,0:ny,0:nz).  Particle arrays are device pointers of
+ *    length np.  Conductor lists passed to wm_mg3d_set_conductors and the
+ *    intercept/conductor set-up routines are host pointers.
+ *  - stream is a hipStream_t (NULL = the null stream).  Calls are
+ *    asynchronous on that stream except wm_mg3d_solve, which returns after
+ *    the solve has finished (mgiters and mgerror are host outputs).
+ *  - bounds[6] are (xm, xp, ym, yp, zm, zp) with 0 = Dirichlet,
+ *    1 = Neumann, 2 = periodic.
+ */
+#ifndef WARPMI_H
+#define WARPMI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message describing the last error on the calling thread. */
+const char *wm_last_error(void);
+
+/* Library and device query: fills name (>= 64 bytes) with the gfx arch of
+ * the current device; returns the number of compute units. */
+int wm_device_info(char *name, int64_t namelen, int64_t *cus);
+
+/* ---------------------------------------------------------------------
+ * Multigrid solver (pywarp90/f3d_mgrid.F)
+ * ------------------------------------------------------------------- */
+typedef struct wm_mg3d wm_mg3d;
+
+/* Creates a solver for an (nx,ny,nz)-cell grid.  The coarsening levels are
+ * those of getmglevels (f3d_mgrid.F:3654-3845); mgmaxlevels caps the
+ * recursion as in vcycle (f3d_mgrid.F:1293).  Replaces the grid set-up that
+ * multigrid3dsolve (f3d_mgrid.F:83) does on every call. */
+int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz,
+                   double dx, double dy, double dz, const int64_t *bounds,
+                   int64_t mgmaxlevels);
+int wm_mg3d_destroy(wm_mg3d *h);
+int wm_mg3d_nlevels(const wm_mg3d *h, int64_t *nlevels);
+int wm_mg3d_level_dims(const wm_mg3d *h, int64_t level, int64_t *nxyz,
+                       double *dxyz);
+
+/* Conductor lists after checkconductors (f3d_mgrid.F:3964-4326), i.e. the
+ * ConductorType of pywarp90/f3d.v:105-164 sorted by level.  Host arrays.
+ * int_indx is (n,3), sub_dels/sub_volt are (n,6) in the order
+ * mx,px,my,py,mz,pz; *_istart has levels+1 entries (0-based). */
+typedef struct {
+  int64_t levels;
+  int64_t int_n;
+  const int64_t *int_indx;
+  const double *int_volt;
+  const int64_t *int_istart;
+  int64_t sub_n[2];
+  const int64_t *sub_indx[2];
+  const double *sub_dels[2];
+  const double *sub_volt[2];
+  const int64_t *sub_istart[2];
+} wm_conductor_lists;
+
+/* Uploads conductor data into the solver (replaces any previous data).
+ * The ConductorType argument of multigrid3dsolve (f3d_mgrid.F:94). */
+int wm_mg3d_set_conductors(wm_mg3d *h, const wm_conductor_lists *c);
+
+/* precision: 0 = FP64 V-cycles (the reference's arithmetic),
+ *            1 = FP32 V-cycles inside an FP64 defect-correction loop,
+ *            2 = as 1 with BF16 storage for the finest-level smoother. */
+#define WM_PREC_FP64 0
+#define WM_PREC_FP32_DC 1
+#define WM_PREC_BF16_DC 2
+
+/* multigrid3dsolve (f3d_mgrid.F:83-500) for iwhich = 0: solves in place on
+ * phi (initial guess in, solution with Dirichlet guard cells filled out),
+ * scaling rho by mgparam/(eps0*2*(dxsqi+dysqi+dzsqi)) and restoring it as
+ * the reference does.  Stops when max|phi_new-phi_old| <= mgtol or after
+ * mgmaxiters V-cycles. */
+int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam,
+                  int64_t mgform, int64_t mgmaxiters, double mgtol,
+                  int64_t downpasses, int64_t uppasses, int64_t lcndbndy,
+                  int64_t icndbndy, int64_t precision, int64_t *mgiters,
+                  double *mgerror, void *stream);
+
+/* Per-kernel entry points on one level, natural-layout device arrays of
+ * that level's size.  sorhalfpass3d (f3d_mgrid.F:2095-2291), residual3d
+ * (3367-3598), restrict3d (1593-1775, res has 1 guard cell, rhocoarse is
+ * the next level's (0:nc)^3), expand3d (1777-1947, phicoarse is the next
+ * level's (-1:nc+1)^3). */
+int wm_mg3d_sorhalfpass(wm_mg3d *h, int64_t level, int64_t parity,
+                        double *phi, const double *rho, double mgparam,
+                        int64_t mgform, int64_t lcndbndy, int64_t icndbndy,
+                        void *stream);
+/* sorpass3d (f3d_mgrid.F:1949-2093): conductor pinning, then each parity's
+ * half sweep followed by applyboundaryconditions3d. */
+int wm_mg3d_sorpass(wm_mg3d *h, int64_t level, double *phi, const double *rho,
+                    double mgparam, int64_t mgform, int64_t lcndbndy,
+                    int64_t icndbndy, void *stream);
+int wm_mg3d_residual(wm_mg3d *h, int64_t level, const double *phi,
+                     const double *rho, double *res, double mgparam,
+                     int64_t mgform, int64_t mgform2init, int64_t lcndbndy,
+                     int64_t icndbndy, void *stream);
+int wm_mg3d_restrict(wm_mg3d *h, int64_t level, const double *res,
+                     double *rhocoarse, void *stream);
+int wm_mg3d_expand(wm_mg3d *h, int64_t level, double *phi,
+                   const double *phicoarse, void *stream);
+
+/* applyboundaryconditions3d (f3d_mgrid.F:3847-3955), ncomp = 1, one guard
+ * cell, on a device array (-1:nx+1)^3. */
+int wm_applyboundaryconditions3d(int64_t nx, int64_t ny, int64_t nz,
+                                 double *u, const int64_t *bounds,
+                                 int64_t lwithdirichlet,
+                                 int64_t lzerodirichlet, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Particles (pywarp90/w3d.F)
+ * ------------------------------------------------------------------- */
+
+/* setrho3d -> setrho3ddirect1 (w3d.F:7199-7301, 7757-7963), no symmetry:
+ * adds the CIC charge of np particles of charge q and weight wght to rho.
+ * Positions must lie in [xmmin, xmmin+nx*dx) etc. (the reference does not
+ * check; here out-of-grid particles are skipped and counted in
+ * *nskipped, which may be NULL). */
+int wm_setrho3d(double *rho, int64_t np, const double *xp, const double *yp,
+                const double *zp, double zgrid, double q, double wght,
+                int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                double dz, double xmmin, double ymmin, double zmmin,
+                int64_t *nskipped, void *stream);
+
+/* Deposit of particles sorted by cell (no reference counterpart as one
+ * call; same sums as setrho3d grouped by cell): computes the cell keys,
+ * sorts a permutation by key and reduces each cell's charge before adding
+ * it to rho.  work must hold wm_setrho3d_sorted_worksize() bytes. */
+int64_t wm_setrho3d_sorted_worksize(int64_t np, int64_t nx, int64_t ny,
+                                    int64_t nz);
+int wm_setrho3d_sorted(double *rho, int64_t np, const double *xp,
+                       const double *yp, const double *zp, double zgrid,
+                       double q, double wght, int64_t nx, int64_t ny,
+                       int64_t nz, double dx, double dy, double dz,
+                       double xmmin, double ymmin, double zmmin, void *work,
+                       void *stream);
+
+/* applyrhoboundaryconditions3d (w3d.F:2660-2750), serial, no guards. */
+int wm_applyrhoboundaryconditions3d(int64_t nx, int64_t ny, int64_t nz,
+                                    double *rho, const int64_t *bounds,
+                                    void *stream);
+
+/* sete3d -> sete_from_phi_linear3d (w3d.F:3656-3777, 3963-4107): adds E
+ * at the particles to ex, ey, ez. */
+int wm_sete3d(const double *phi, int64_t np, const double *xp,
+              const double *yp, const double *zp, double zgrid,
+              double xmmin, double ymmin, double zmmin, double dx, double dy,
+              double dz, int64_t nx, int64_t ny, int64_t nz, double *ex,
+              double *ey, double *ez, void *stream);
+
+/* epush3d (w3d.F:2752-2777): u += E*q*ddt/m. */
+int wm_epush3d(int64_t np, double *uxp, double *uyp, double *uzp,
+               const double *ex, const double *ey, const double *ez, double q,
+               double m, double ddt, void *stream);
+
+/* xpush3d (w3d.F:3070-3092): x += u*gaminv*dtp. */
+int wm_xpush3d(int64_t np, double *xp, double *yp, double *zp,
+               const double *uxp, const double *uyp, const double *uzp,
+               const double *gaminv, double dtp, void *stream);
+
+/* The "fullv" advance of padvnc3d for B = 0 (w3d.F:1816, 1956-1986):
+ * E zeroed and fetched (sete3d), epush3d(dt/2) twice, xpush3d(dt), in one
+ * pass.  ex, ey, ez may be NULL; when given they receive E. */
+int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp,
+                     double *zp, double *uxp, double *uyp, double *uzp,
+                     const double *gaminv, double *ex, double *ey,
+                     double *ez, double zgrid, double xmmin, double ymmin,
+                     double zmmin, double dx, double dy, double dz,
+                     int64_t nx, int64_t ny, int64_t nz, double q, double m,
+                     double dt, void *stream);
+
+/* getselfe3d (w3d.F:6023-6109), nxguarde = 0. */
+int wm_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
+                  double *selfe, double dx, double dy, double dz,
+                  int64_t lzero, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Conductor set-up on the host (pywarp90/f3d_conductors.F)
+ * ------------------------------------------------------------------- */
+
+/* Grid-line intercepts of one object on one MG level
+ * (ConductorInterceptType).  Host arrays: xintercepts (0:nz,0:ny,nxicpt),
+ * yintercepts (0:nz,0:nx,nyicpt), zintercepts (0:ny,0:nx,nzicpt) with the
+ * intercept index fastest; voltages/condids alike. */
+typedef struct {
+  double xmmin, ymmin, zmmin, dx, dy, dz;
+  int64_t nx, ny, nz, ix, iy, iz, mglevel;
+  int64_t nxicpt, nyicpt, nzicpt;
+  double *xintercepts, *yintercepts, *zintercepts;
+  double *xvoltages, *yvoltages, *zvoltages;
+  int64_t *xcondids, *ycondids, *zcondids;
+} wm_intercepts;
+
+/* ZCylinderConductorFnew (f3d_conductors.F:2496-2583, outside = 0) and
+ * ZCylinderOutConductorFnew (3158-3256, outside = 1). */
+int wm_zcylinder_intercepts(double rad, double length, double xcent,
+                            double ycent, double zcent, wm_intercepts *icpt,
+                            double fuzz, int64_t outside);
+
+/* ZSrfrvConductorFnew (f3d_conductors.F:7040-7855) for outlines made of
+ * straight segments (ZAnnulus and polygonal ZSrfrvIn). */
+int wm_zsrfrv_intercepts(int64_t nn, const double *rsrf, const double *zsrf,
+                         double xcent, double ycent, double zcent,
+                         wm_intercepts *icpt);
+
+/* Growable conductor lists owned by the library (host memory). */
+typedef struct wm_condlist wm_condlist;
+int wm_condlist_create(wm_condlist **out);
+int wm_condlist_destroy(wm_condlist *c);
+/* conductordelfromintercepts (f3d_conductors.F:12-728) appending to c. */
+int wm_conductordelfromintercepts(const wm_intercepts *icpt, wm_condlist *c,
+                                  double dfill, double fuzz);
+/* checkconductors (f3d_mgrid.F:3964-4326): levels, clean-up, sort. */
+int wm_checkconductors(wm_condlist *c, int64_t nx, int64_t ny, int64_t nz,
+                       double dx, double dy, double dz);
+/* Read access to the lists (pointers valid until the next modification). */
+int wm_condlist_view(const wm_condlist *c, wm_conductor_lists *view);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

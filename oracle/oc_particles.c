@@ -1,0 +1,186 @@
+/*
+ * oc_particles.c -- TEST INFRASTRUCTURE ONLY (parity oracle and CPU baseline).
+ *
+ * FP64 restatement of the particle side of the reference's electrostatic
+ * PIC step: CIC deposit (setrho3ddirect1), rho boundary folding, field
+ * gather from phi (sete_from_phi_linear3d), grid E (getselfe3d), and the
+ * leapfrog kicks/drift (epush3d, xpush3d), from pywarp90/w3d.F.  No
+ * symmetries (l2symtry = l4symtry = .false.), ny > 0, nxguardphi = 1,
+ * nxguardrho = nxguarde = 0.  Compile with -ffp-contract=off.
+ */
+#include "warp_oracle.h"
+
+#define IDXP(ix, iy, iz) \
+  ((int64_t)((ix) + 1) + (nx + 3) * ((int64_t)((iy) + 1) + (ny + 3) * (int64_t)((iz) + 1)))
+#define IDXR(ix, iy, iz) \
+  ((int64_t)(ix) + (nx + 1) * ((int64_t)(iy) + (ny + 1) * (int64_t)(iz)))
+
+/* setrho3ddirect1, w3d.F:7757-7963, "normal loop" branch (7843-7916).
+ * The one-particle-ahead prefetch of the reference does not change any
+ * value, so each particle's index and weights are computed directly. */
+void oc_setrho3d(double *rho, int64_t np, const double *xp, const double *yp,
+                 const double *zp, double zgrid, double q, double wght,
+                 int64_t nx, int64_t ny, int64_t nz,
+                 double dx, double dy, double dz,
+                 double xmmin, double ymmin, double zmmin) {
+  double g = wght * q;
+  if (nx > 0) g = g / dx;
+  if (ny > 0) g = g / dy;
+  if (nz > 0) g = g / dz;
+  double dxi = 1. / dx, dyi = 1. / dy, dzi = 1. / dz;
+  int64_t ip;
+  for (ip = 0; ip < np; ip++) {
+    int64_t ii = (int64_t)((xp[ip] - xmmin) * dxi);
+    int64_t jj = (int64_t)((yp[ip] - ymmin) * dyi);
+    int64_t kk = (int64_t)((zp[ip] - zgrid - zmmin) * dzi);
+    double u1 = (xp[ip] - xmmin) * dxi - ii;
+    double v1 = (yp[ip] - ymmin) * dyi - jj;
+    double w1 = (zp[ip] - zgrid - zmmin) * dzi - kk;
+    double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+    rho[IDXR(ii, jj, kk)] = rho[IDXR(ii, jj, kk)] + u0 * v0 * w0 * g;
+    rho[IDXR(ii + 1, jj, kk)] = rho[IDXR(ii + 1, jj, kk)] + u1 * v0 * w0 * g;
+    rho[IDXR(ii, jj + 1, kk)] = rho[IDXR(ii, jj + 1, kk)] + u0 * v1 * w0 * g;
+    rho[IDXR(ii + 1, jj + 1, kk)] = rho[IDXR(ii + 1, jj + 1, kk)] + u1 * v1 * w0 * g;
+    rho[IDXR(ii, jj, kk + 1)] = rho[IDXR(ii, jj, kk + 1)] + u0 * v0 * w1 * g;
+    rho[IDXR(ii + 1, jj, kk + 1)] = rho[IDXR(ii + 1, jj, kk + 1)] + u1 * v0 * w1 * g;
+    rho[IDXR(ii, jj + 1, kk + 1)] = rho[IDXR(ii, jj + 1, kk + 1)] + u0 * v1 * w1 * g;
+    rho[IDXR(ii + 1, jj + 1, kk + 1)] = rho[IDXR(ii + 1, jj + 1, kk + 1)] + u1 * v1 * w1 * g;
+  }
+}
+
+/* applyrhoboundaryconditions3d, w3d.F:2660-2750, serial, nxguardrho = 0:
+ * Neumann faces double, periodic faces sum and copy. */
+void oc_applyrhoboundaryconditions3d(int64_t nx, int64_t ny, int64_t nz,
+                                     double *rho, const int64_t *b) {
+  int64_t ix, iy, iz;
+  if (nx > 0) {
+    for (iz = 0; iz <= nz; iz++)
+      for (iy = 0; iy <= ny; iy++) {
+        if (b[0] == 1) rho[IDXR(0, iy, iz)] = rho[IDXR(0, iy, iz)] + rho[IDXR(0, iy, iz)];
+        if (b[1] == 1) rho[IDXR(nx, iy, iz)] = rho[IDXR(nx, iy, iz)] + rho[IDXR(nx, iy, iz)];
+        if (b[0] == 2 || b[1] == 2) {
+          rho[IDXR(0, iy, iz)] = rho[IDXR(0, iy, iz)] + rho[IDXR(nx, iy, iz)];
+          rho[IDXR(nx, iy, iz)] = rho[IDXR(0, iy, iz)];
+        }
+      }
+  }
+  if (ny > 0) {
+    for (iz = 0; iz <= nz; iz++)
+      for (ix = 0; ix <= nx; ix++) {
+        if (b[2] == 1) rho[IDXR(ix, 0, iz)] = rho[IDXR(ix, 0, iz)] + rho[IDXR(ix, 0, iz)];
+        if (b[3] == 1) rho[IDXR(ix, ny, iz)] = rho[IDXR(ix, ny, iz)] + rho[IDXR(ix, ny, iz)];
+        if (b[2] == 2 || b[3] == 2) {
+          rho[IDXR(ix, 0, iz)] = rho[IDXR(ix, 0, iz)] + rho[IDXR(ix, ny, iz)];
+          rho[IDXR(ix, ny, iz)] = rho[IDXR(ix, 0, iz)];
+        }
+      }
+  }
+  if (nz > 0) {
+    for (iy = 0; iy <= ny; iy++)
+      for (ix = 0; ix <= nx; ix++) {
+        if (b[4] == 1) rho[IDXR(ix, iy, 0)] = rho[IDXR(ix, iy, 0)] + rho[IDXR(ix, iy, 0)];
+        if (b[5] == 1) rho[IDXR(ix, iy, nz)] = rho[IDXR(ix, iy, nz)] + rho[IDXR(ix, iy, nz)];
+        if (b[4] == 2 || b[5] == 2) {
+          rho[IDXR(ix, iy, 0)] = rho[IDXR(ix, iy, 0)] + rho[IDXR(ix, iy, nz)];
+          rho[IDXR(ix, iy, nz)] = rho[IDXR(ix, iy, 0)];
+        }
+      }
+  }
+}
+
+/* sete_from_phi_linear3d, w3d.F:3963-4107 (no symmetry).  E is added to
+ * the incoming ex, ey, ez. */
+void oc_sete3d(const double *phi, int64_t np, const double *xp,
+               const double *yp, const double *zp, double zgrid,
+               double xmmin, double ymmin, double zmmin,
+               double dx, double dy, double dz,
+               int64_t nx, int64_t ny, int64_t nz,
+               double *ex, double *ey, double *ez) {
+  double tdxi = 1. / (2. * dx), tdyi = 1. / (2. * dy), tdzi = 1. / (2. * dz);
+  double dxi = 1. / dx, dyi = 1. / dy, dzi = 1. / dz;
+  int64_t ip;
+#define P(a, b, c) phi[IDXP(i + (a), j + (b), k + (c))]
+  for (ip = 0; ip < np; ip++) {
+    int64_t i = (int64_t)((xp[ip] - xmmin) * dxi);
+    int64_t j = (int64_t)((yp[ip] - ymmin) * dyi);
+    int64_t k = (int64_t)((zp[ip] - zgrid - zmmin) * dzi);
+    double u1 = (xp[ip] - xmmin) * dxi - i;
+    double v1 = (yp[ip] - ymmin) * dyi - j;
+    double w1 = (zp[ip] - zgrid - zmmin) * dzi - k;
+    double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+    double ext = tdxi * (u0 * v0 * w0 * (P(-1, 0, 0) - P(1, 0, 0)) +
+                         u1 * v0 * w0 * (P(0, 0, 0) - P(2, 0, 0)) +
+                         u0 * v1 * w0 * (P(-1, 1, 0) - P(1, 1, 0)) +
+                         u1 * v1 * w0 * (P(0, 1, 0) - P(2, 1, 0)) +
+                         u0 * v0 * w1 * (P(-1, 0, 1) - P(1, 0, 1)) +
+                         u1 * v0 * w1 * (P(0, 0, 1) - P(2, 0, 1)) +
+                         u0 * v1 * w1 * (P(-1, 1, 1) - P(1, 1, 1)) +
+                         u1 * v1 * w1 * (P(0, 1, 1) - P(2, 1, 1)));
+    double eyt = tdyi * (u0 * v0 * w0 * (P(0, -1, 0) - P(0, 1, 0)) +
+                         u1 * v0 * w0 * (P(1, -1, 0) - P(1, 1, 0)) +
+                         u0 * v1 * w0 * (P(0, 0, 0) - P(0, 2, 0)) +
+                         u1 * v1 * w0 * (P(1, 0, 0) - P(1, 2, 0)) +
+                         u0 * v0 * w1 * (P(0, -1, 1) - P(0, 1, 1)) +
+                         u1 * v0 * w1 * (P(1, -1, 1) - P(1, 1, 1)) +
+                         u0 * v1 * w1 * (P(0, 0, 1) - P(0, 2, 1)) +
+                         u1 * v1 * w1 * (P(1, 0, 1) - P(1, 2, 1)));
+    double ezt = tdzi * (u0 * v0 * w0 * (P(0, 0, -1) - P(0, 0, 1)) +
+                         u1 * v0 * w0 * (P(1, 0, -1) - P(1, 0, 1)) +
+                         u0 * v1 * w0 * (P(0, 1, -1) - P(0, 1, 1)) +
+                         u1 * v1 * w0 * (P(1, 1, -1) - P(1, 1, 1)) +
+                         u0 * v0 * w1 * (P(0, 0, 0) - P(0, 0, 2)) +
+                         u1 * v0 * w1 * (P(1, 0, 0) - P(1, 0, 2)) +
+                         u0 * v1 * w1 * (P(0, 1, 0) - P(0, 1, 2)) +
+                         u1 * v1 * w1 * (P(1, 1, 0) - P(1, 1, 2)));
+    ex[ip] = ex[ip] + ext;
+    ey[ip] = ey[ip] + eyt;
+    ez[ip] = ez[ip] + ezt;
+  }
+#undef P
+}
+
+/* epush3d, w3d.F:2752-2777. */
+void oc_epush3d(int64_t np, double *uxp, double *uyp, double *uzp,
+                const double *ex, const double *ey, const double *ez,
+                double q, double m, double ddt) {
+  double cst = q * ddt / m;
+  int64_t ip;
+  for (ip = 0; ip < np; ip++) {
+    uxp[ip] = uxp[ip] + ex[ip] * cst;
+    uyp[ip] = uyp[ip] + ey[ip] * cst;
+    uzp[ip] = uzp[ip] + ez[ip] * cst;
+  }
+}
+
+/* xpush3d, w3d.F:3070-3092. */
+void oc_xpush3d(int64_t np, double *xp, double *yp, double *zp,
+                const double *uxp, const double *uyp, const double *uzp,
+                const double *gaminv, double dtp) {
+  int64_t ip;
+  for (ip = 0; ip < np; ip++) {
+    xp[ip] = xp[ip] + uxp[ip] * gaminv[ip] * dtp;
+    yp[ip] = yp[ip] + uyp[ip] * gaminv[ip] * dtp;
+    zp[ip] = zp[ip] + uzp[ip] * gaminv[ip] * dtp;
+  }
+}
+
+/* getselfe3d, w3d.F:6023-6109, nxguarde = 0: selfe(0:2,0:nx,0:ny,0:nz). */
+void oc_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
+                   double *selfe, double dx, double dy, double dz,
+                   int64_t lzero) {
+  double tdxi = 0.5 / dx, tdyi = 0.5 / dy, tdzi = 0.5 / dz;
+  int64_t ix, iy, iz;
+  for (iz = 0; iz <= nz; iz++)
+    for (iy = 0; iy <= ny; iy++)
+      for (ix = 0; ix <= nx; ix++) {
+        double *e = selfe + 3 * IDXR(ix, iy, iz);
+        double gx = tdxi * (phi[IDXP(ix - 1, iy, iz)] - phi[IDXP(ix + 1, iy, iz)]);
+        double gy = tdyi * (phi[IDXP(ix, iy - 1, iz)] - phi[IDXP(ix, iy + 1, iz)]);
+        double gz = tdzi * (phi[IDXP(ix, iy, iz - 1)] - phi[IDXP(ix, iy, iz + 1)]);
+        if (lzero) {
+          e[0] = gx; e[1] = gy; e[2] = gz;
+        } else {
+          e[0] = e[0] + gx; e[1] = e[1] + gy; e[2] = e[2] + gz;
+        }
+      }
+}

@@ -1,0 +1,523 @@
+"""CPU FP64 oracle for the electrostatic PIC hot path -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module.  It wraps liboracle.so (oc_mgrid.c, oc_particles.c,
+oc_conductors.c), a loop-for-loop C restatement of the reference's Fortran
+(pywarp90/f3d_mgrid.F, w3d.F, f3d_conductors.F), and restates the Python
+geometry bookkeeping of scripts/generateconductors.py (Grid.getmeshnew,
+installconductor) that feeds it.
+
+Array conventions are the reference's: phi is Fortran-ordered (-1:nx+1)^3,
+rho (0:nx)^3.  In numpy these are C-contiguous arrays of shape
+(nz+3, ny+3, nx+3) and (nz+1, ny+1, nx+1), which is the same memory layout.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "liboracle.so")
+LARGEPOS = 1.0e36
+MAXLEVELS = 101
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            build()
+        _lib = C.CDLL(_LIBPATH)
+        _declare(_lib)
+    return _lib
+
+
+_i64 = C.c_int64
+_d = C.c_double
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int64)
+
+
+class OcConductors(C.Structure):
+    _fields_ = [
+        ("levels", _i64),
+        ("levelnx", _i64 * MAXLEVELS), ("levelny", _i64 * MAXLEVELS),
+        ("levelnz", _i64 * MAXLEVELS),
+        ("levellx", _d * MAXLEVELS), ("levelly", _d * MAXLEVELS),
+        ("levellz", _d * MAXLEVELS),
+        ("int_n", _i64), ("int_nmax", _i64),
+        ("int_indx", _pi), ("int_volt", _pd), ("int_numb", _pi),
+        ("int_ilevel", _pi), ("int_istart", _i64 * (MAXLEVELS + 1)),
+        ("sub_n", _i64 * 2), ("sub_nmax", _i64 * 2),
+        ("sub_indx", _pi * 2), ("sub_dels", _pd * 2), ("sub_volt", _pd * 2),
+        ("sub_numb", _pi * 2), ("sub_ilevel", _pi * 2),
+        ("sub_prevphi", _pd * 2),
+        ("sub_istart", (_i64 * (MAXLEVELS + 1)) * 2),
+    ]
+
+
+class OcIntercepts(C.Structure):
+    _fields_ = [
+        ("xmmin", _d), ("ymmin", _d), ("zmmin", _d),
+        ("dx", _d), ("dy", _d), ("dz", _d),
+        ("nx", _i64), ("ny", _i64), ("nz", _i64),
+        ("ix", _i64), ("iy", _i64), ("iz", _i64), ("mglevel", _i64),
+        ("nxicpt", _i64), ("nyicpt", _i64), ("nzicpt", _i64),
+        ("xintercepts", _pd), ("yintercepts", _pd), ("zintercepts", _pd),
+        ("xvoltages", _pd), ("yvoltages", _pd), ("zvoltages", _pd),
+        ("xcondids", _pi), ("ycondids", _pi), ("zcondids", _pi),
+    ]
+
+
+def _declare(L):
+    P = C.c_void_p
+    L.oc_eps0.restype = _d
+    L.oc_multigrid3dsolve.restype = _i64
+    L.oc_multigrid3dsolve.argtypes = [
+        _i64, _i64, _i64, _d, _d, _d, P, P, P, _d, _i64, P, _i64, _i64, P,
+        _d, _i64, _i64, _i64, _i64, P]
+    L.oc_sorhalfpass3d.argtypes = [_i64, _i64, _i64, _i64, _i64, P, P, _d, _d,
+                                   _d, P, _d, _i64, _i64, _i64, P]
+    L.oc_sorpass3d.argtypes = [_i64, _i64, _i64, _i64, P, P, _d, _d, _d, P,
+                               _d, _i64, _i64, _i64, P]
+    L.oc_residual3d.argtypes = [_i64, _i64, _i64, _d, _d, _d, P, P, P, _i64, P,
+                                _d, _i64, _i64, _i64, _i64, P]
+    L.oc_restrict3d.argtypes = [_i64, _i64, _i64, P, _i64, _i64, _i64, P, _d, P]
+    L.oc_expand3d.argtypes = [_i64, _i64, _i64, P, _i64, _i64, _i64, P, P]
+    L.oc_applyboundaryconditions3d.argtypes = [_i64, _i64, _i64, _i64, _i64,
+                                               _i64, P, P, _i64, _i64]
+    L.oc_getnextcoarselevel3d.argtypes = [_i64, _i64, _i64, _d, _d, _d, P, P, P,
+                                          P, P, P]
+    L.oc_getmglevels.argtypes = [_i64, _i64, _i64, _d, _d, _d, P]
+    L.oc_checkconductors.argtypes = [_i64, _i64, _i64, _d, _d, _d, P]
+    L.oc_setrho3d.argtypes = [P, _i64, P, P, P, _d, _d, _d, _i64, _i64, _i64,
+                              _d, _d, _d, _d, _d, _d]
+    L.oc_applyrhoboundaryconditions3d.argtypes = [_i64, _i64, _i64, P, P]
+    L.oc_sete3d.argtypes = [P, _i64, P, P, P, _d, _d, _d, _d, _d, _d, _d, _i64,
+                            _i64, _i64, P, P, P]
+    L.oc_epush3d.argtypes = [_i64, P, P, P, P, P, P, _d, _d, _d]
+    L.oc_xpush3d.argtypes = [_i64, P, P, P, P, P, P, P, _d]
+    L.oc_getselfe3d.argtypes = [P, _i64, _i64, _i64, P, _d, _d, _d, _i64]
+    L.oc_conductordelfromintercepts.restype = _i64
+    L.oc_conductordelfromintercepts.argtypes = [P, P, _d, _d, _i64, P]
+    L.oc_zcylinder_intercepts.argtypes = [_d, _d, _d, _d, _d, P, _d, _i64]
+    L.oc_zsrfrv_intercepts.argtypes = [_i64, P, P, _d, _d, _d, P, _i64, P]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def eps0():
+    return lib().oc_eps0()
+
+
+# ---------------------------------------------------------------------------
+# Conductors
+# ---------------------------------------------------------------------------
+class Conductors:
+    """Holds the numpy storage behind an OcConductors struct."""
+
+    def __init__(self):
+        self.s = OcConductors()
+        self.s.levels = 0
+        for l in range(MAXLEVELS):
+            self.s.levellx[l] = self.s.levelly[l] = self.s.levellz[l] = 1.0
+        self.int_indx = np.zeros((0, 3), np.int64)
+        self.int_volt = np.zeros(0)
+        self.int_numb = np.zeros(0, np.int64)
+        self.int_ilevel = np.zeros(0, np.int64)
+        self.sub = [dict(indx=np.zeros((0, 3), np.int64), dels=np.zeros((0, 6)),
+                         volt=np.zeros((0, 6)), numb=np.zeros((0, 6), np.int64),
+                         ilevel=np.zeros(0, np.int64), prevphi=np.zeros(0))
+                    for _ in range(2)]
+        self._bind()
+
+    def _bind(self):
+        s = self.s
+        s.int_nmax = len(self.int_volt)
+        s.int_indx = self.int_indx.ctypes.data_as(_pi)
+        s.int_volt = self.int_volt.ctypes.data_as(_pd)
+        s.int_numb = self.int_numb.ctypes.data_as(_pi)
+        s.int_ilevel = self.int_ilevel.ctypes.data_as(_pi)
+        for p in range(2):
+            d = self.sub[p]
+            s.sub_nmax[p] = len(d["ilevel"])
+            s.sub_indx[p] = d["indx"].ctypes.data_as(_pi)
+            s.sub_dels[p] = d["dels"].ctypes.data_as(_pd)
+            s.sub_volt[p] = d["volt"].ctypes.data_as(_pd)
+            s.sub_numb[p] = d["numb"].ctypes.data_as(_pi)
+            s.sub_ilevel[p] = d["ilevel"].ctypes.data_as(_pi)
+            s.sub_prevphi[p] = d["prevphi"].ctypes.data_as(_pd)
+
+    def grow(self, nc, ne, no):
+        """Make room for nc, ne, no more interior/even/odd points."""
+        n = self.s.int_n
+        def g(a, extra):
+            b = np.zeros((a.shape[0] + extra,) + a.shape[1:], a.dtype)
+            b[:a.shape[0]] = a
+            return b
+        self.int_indx = g(self.int_indx[:n], nc)
+        self.int_volt = g(self.int_volt[:n], nc)
+        self.int_numb = g(self.int_numb[:n], nc)
+        self.int_ilevel = g(self.int_ilevel[:n], nc)
+        for p, extra in ((0, ne), (1, no)):
+            m = self.s.sub_n[p]
+            d = self.sub[p]
+            for k in d:
+                d[k] = g(d[k][:m], extra)
+        self._bind()
+
+    def ptr(self):
+        return C.addressof(self.s)
+
+    def lists(self):
+        """Return copies of the sorted lists (after checkconductors)."""
+        s = self.s
+        out = dict(levels=s.levels,
+                   levelnx=[s.levelnx[l] for l in range(s.levels)],
+                   levelny=[s.levelny[l] for l in range(s.levels)],
+                   levelnz=[s.levelnz[l] for l in range(s.levels)],
+                   int_indx=self.int_indx[:s.int_n].copy(),
+                   int_volt=self.int_volt[:s.int_n].copy(),
+                   int_istart=[s.int_istart[l] for l in range(s.levels + 1)])
+        for p in range(2):
+            m = s.sub_n[p]
+            d = self.sub[p]
+            out["sub%d_indx" % p] = d["indx"][:m].copy()
+            out["sub%d_dels" % p] = d["dels"][:m].copy()
+            out["sub%d_volt" % p] = d["volt"][:m].copy()
+            out["sub%d_istart" % p] = [s.sub_istart[p][l] for l in range(s.levels + 1)]
+        return out
+
+
+def _nint(x):
+    # Fortran NINT: round half away from zero
+    return int(np.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def getmglevels(nx, ny, nz, dx, dy, dz):
+    c = Conductors()
+    lib().oc_getmglevels(nx, ny, nz, dx, dy, dz, c.ptr())
+    s = c.s
+    return [dict(nx=s.levelnx[l], ny=s.levelny[l], nz=s.levelnz[l],
+                 lx=s.levellx[l], ly=s.levelly[l], lz=s.levellz[l])
+            for l in range(s.levels)]
+
+
+class _Geom:
+    """One z-aligned object: kind in ('zcylinder', 'zcylinderout', 'zsrfrv')."""
+
+    def __init__(self, kind, voltage, condid, xcent=0., ycent=0., zcent=0.,
+                 **kw):
+        self.kind, self.voltage, self.condid = kind, voltage, condid
+        self.xcent, self.ycent, self.zcent = xcent, ycent, zcent
+        self.kw = kw
+        if kind == "zcylinder":
+            r, L = kw["radius"], kw["length"]
+            self.extent = ([xcent - r, ycent - r, zcent - L / 2.],
+                           [xcent + r, ycent + r, zcent + L / 2.])
+        elif kind == "zcylinderout":
+            L = kw["length"]
+            self.extent = ([-LARGEPOS, -LARGEPOS, zcent - L / 2.],
+                           [LARGEPOS, LARGEPOS, zcent + L / 2.])
+        else:
+            rmax = max(kw["rsrf"])
+            self.extent = ([xcent - rmax, ycent - rmax, zcent + kw["zmin"]],
+                           [xcent + rmax, ycent + rmax, zcent + kw["zmax"]])
+
+
+def ZCylinder(radius, length, voltage=0., xcent=0., ycent=0., zcent=0., condid=1):
+    """scripts/generateconductors.py:3305 (ZCylinder)."""
+    return _Geom("zcylinder", voltage, condid, xcent, ycent, zcent,
+                 radius=radius, length=length)
+
+
+def ZCylinderOut(radius, length, voltage=0., xcent=0., ycent=0., zcent=0., condid=1):
+    """scripts/generateconductors.py:3494 (ZCylinderOut)."""
+    return _Geom("zcylinderout", voltage, condid, xcent, ycent, zcent,
+                 radius=radius, length=length)
+
+
+def ZAnnulus(rmin, rmax, length, voltage=0., xcent=0., ycent=0., zcent=0., condid=1):
+    """scripts/generateconductors.py:5821-5855: a ZSrfrvIn whose z data is in
+    the lab frame with zcent = 0, and the closed r-z outline of
+    generateconductors.py:5012-5018."""
+    zmin = -length / 2. + zcent
+    zmax = +length / 2. + zcent
+    rsrf = [rmin, rmax, rmax, rmin, rmin]
+    zsrf = [zmin, zmin, zmax, zmax, zmin]
+    return _Geom("zsrfrv", voltage, condid, xcent, ycent, 0.,
+                 rsrf=rsrf, zsrf=zsrf, zmin=zmin, zmax=zmax)
+
+
+def _getmeshnew(g, lev, extent):
+    """Grid.getmeshnew, scripts/generateconductors.py:2592-2635 (serial,
+    zbeam = 0)."""
+    dx = g["dx"] * lev["lx"]
+    dy = g["dy"] * lev["ly"]
+    dz = g["dz"] * lev["lz"]
+    nxl, nyl, nzl = lev["nx"], lev["ny"], lev["nz"]
+    xmin = max(g["xmmin"], g["xmmin"] + 0 * dx)
+    xmax = min(g["xmmax"], g["xmmin"] + (0 + nxl) * dx)
+    ymin = max(g["ymmin"], g["ymmin"] + 0 * dy)
+    ymax = min(g["ymmax"], g["ymmin"] + (0 + nyl) * dy)
+    zmin = max(g["zmmin"], g["zmmin"] + 0 * dz + 0.)
+    zmax = min(g["zmmax"], g["zmmin"] + (0 + nzl) * dz + 0.)
+    mins, maxs = extent
+    if mins[0] - dx > xmin:
+        xmin = int((mins[0] - dx - xmin) / dx) * dx + xmin
+        if xmin >= xmax: xmin = xmax - 2 * dx
+    if mins[1] - dy > ymin:
+        ymin = int((mins[1] - dy - ymin) / dy) * dy + ymin
+        if ymin >= ymax: ymin = ymax - 2 * dy
+    if mins[2] - dz > zmin:
+        zmin = int((mins[2] - dz - zmin) / dz) * dz + zmin
+        if zmin >= zmax: zmin = zmax - 2 * dz
+    if maxs[0] + dx < xmax:
+        xmax = int((maxs[0] + dx - xmin) / dx) * dx + xmin
+        if xmax < maxs[0] + dx: xmax += dx
+        if xmax <= xmin: xmax = xmin + 2 * dx
+    if maxs[1] + dy < ymax:
+        ymax = int((maxs[1] + dy - ymin) / dy) * dy + ymin
+        if ymax < maxs[1] + dy: ymax += dy
+        if ymax <= ymin: ymax = ymin + 2 * dy
+    if maxs[2] + dz < zmax:
+        zmax = int((maxs[2] + dz - zmin) / dz) * dz + zmin
+        if zmax < maxs[2] + dz: zmax += dz
+        if zmax <= zmin: zmax = zmin + 2 * dz
+    nx = _nint((xmax - xmin) / dx)
+    ny = _nint((ymax - ymin) / dy)
+    nz = _nint((zmax - zmin) / dz)
+    ix = _nint((xmin - (g["xmmin"] + 0 * dx)) / dx)
+    iy = _nint((ymin - (g["ymmin"] + 0 * dy)) / dy)
+    iz = _nint((zmin - (g["zmmin"] + 0 * dz + 0.)) / dz)
+    return xmin, ymin, zmin, dx, dy, dz, nx, ny, nz, ix, iy, iz
+
+
+def _checkoverlap(g, lev, extent):
+    """Grid.checkoverlap, generateconductors.py:2637-2660 (serial)."""
+    dx = g["dx"] * lev["lx"]
+    dy = g["dy"] * lev["ly"]
+    dz = g["dz"] * lev["lz"]
+    mins, maxs = extent
+    xmin = max(mins[0], g["xmmin"]); ymin = max(mins[1], g["ymmin"])
+    zmin = max(mins[2], g["zmmin"])
+    xmax = min(maxs[0], g["xmmax"]); ymax = min(maxs[1], g["ymmax"])
+    zmax = min(maxs[2], g["zmmax"])
+    if zmin - dz > zmax or xmin - dx > xmax or ymin - dy > ymax:
+        return False
+    return True
+
+
+def _intercepts(geom, mesh, mglevel):
+    xmin, ymin, zmin, dx, dy, dz, nx, ny, nz, ix, iy, iz = mesh
+    if geom.kind == "zcylinder":
+        nxi = nyi = nzi = 2
+    elif geom.kind == "zcylinderout":
+        nxi = nyi = 4
+        nzi = 2
+    else:
+        nseg = len(geom.kw["rsrf"]) - 1
+        nxi = nyi = nzi = 2 * nseg
+    st = OcIntercepts()
+    st.xmmin, st.ymmin, st.zmmin = xmin, ymin, zmin
+    st.dx, st.dy, st.dz = dx, dy, dz
+    st.nx, st.ny, st.nz = nx, ny, nz
+    st.ix, st.iy, st.iz = ix, iy, iz
+    st.mglevel = mglevel
+    st.nxicpt, st.nyicpt, st.nzicpt = nxi, nyi, nzi
+    arrs = dict(
+        x=np.zeros((nz + 1, ny + 1, nxi)), y=np.zeros((nz + 1, nx + 1, nyi)),
+        z=np.zeros((ny + 1, nx + 1, nzi)))
+    for k in "xyz":
+        arrs[k + "v"] = np.full(arrs[k].shape, float(geom.voltage))
+        arrs[k + "c"] = np.full(arrs[k].shape, int(geom.condid), np.int64)
+    st.xintercepts = arrs["x"].ctypes.data_as(_pd)
+    st.yintercepts = arrs["y"].ctypes.data_as(_pd)
+    st.zintercepts = arrs["z"].ctypes.data_as(_pd)
+    st.xvoltages = arrs["xv"].ctypes.data_as(_pd)
+    st.yvoltages = arrs["yv"].ctypes.data_as(_pd)
+    st.zvoltages = arrs["zv"].ctypes.data_as(_pd)
+    st.xcondids = arrs["xc"].ctypes.data_as(_pi)
+    st.ycondids = arrs["yc"].ctypes.data_as(_pi)
+    st.zcondids = arrs["zc"].ctypes.data_as(_pi)
+    fuzz = 1.e-13
+    if nx >= 0 and ny >= 0 and nz >= 0:
+        if geom.kind in ("zcylinder", "zcylinderout"):
+            lib().oc_zcylinder_intercepts(
+                geom.kw["radius"], geom.kw["length"], geom.xcent, geom.ycent,
+                geom.zcent, C.addressof(st), fuzz,
+                1 if geom.kind == "zcylinderout" else 0)
+        else:
+            r = np.array(geom.kw["rsrf"], np.float64)
+            z = np.array(geom.kw["zsrf"], np.float64)
+            cnt = np.zeros(1, np.int64)
+            lib().oc_zsrfrv_intercepts(len(r), _p(r), _p(z), geom.xcent,
+                                       geom.ycent, geom.zcent, C.addressof(st),
+                                       0, _p(cnt))
+    return st, arrs
+
+
+def install_conductors(geoms, nx, ny, nz, xmmin, xmmax, ymmin, ymmax,
+                       zmmin, zmmax, dfill=2.0, conductors=None):
+    """installconductor + Grid.getdatanew + GridIntercepts.installlist
+    (generateconductors.py:135-215, 2750-2858, 1851-1948), serial."""
+    if conductors is None:
+        conductors = Conductors()
+    g = dict(xmmin=xmmin, xmmax=xmmax, ymmin=ymmin, ymmax=ymmax, zmmin=zmmin,
+             zmmax=zmmax, dx=(xmmax - xmmin) / nx, dy=(ymmax - ymmin) / ny,
+             dz=(zmmax - zmmin) / nz)
+    levels = getmglevels(nx, ny, nz, g["dx"], g["dy"], g["dz"])
+    fuzz = 1.e-13
+    for geom in geoms:
+        if not _checkoverlap(g, levels[-1], geom.extent):
+            continue
+        for mglevel, lev in enumerate(levels):
+            mesh = _getmeshnew(g, lev, geom.extent)
+            st, arrs = _intercepts(geom, mesh, mglevel)
+            cnt = np.zeros(3, np.int64)
+            lib().oc_conductordelfromintercepts(C.addressof(st), conductors.ptr(),
+                                                dfill, fuzz, 1, _p(cnt))
+            conductors.grow(int(cnt[0]), int(cnt[1]), int(cnt[2]))
+            lib().oc_conductordelfromintercepts(C.addressof(st), conductors.ptr(),
+                                                dfill, fuzz, 0, _p(cnt))
+    return conductors
+
+
+def checkconductors(conductors, nx, ny, nz, dx, dy, dz):
+    lib().oc_checkconductors(nx, ny, nz, dx, dy, dz, conductors.ptr())
+    return conductors
+
+
+# ---------------------------------------------------------------------------
+# Solver and particle kernels
+# ---------------------------------------------------------------------------
+def multigrid3dsolve(phi, rho, dx, dy, dz, bounds=(0,) * 6, mgparam=1.2,
+                     mgform=1, mgmaxiters=100, mgmaxlevels=101, mgtol=1e-6,
+                     downpasses=1, uppasses=1, lcndbndy=True, icndbndy=2,
+                     conductors=None):
+    """multigrid3dsolve in place on phi (nz+3,ny+3,nx+3) and rho
+    (nz+1,ny+1,nx+1); returns (mgiters, mgerror)."""
+    nz, ny, nx = rho.shape[0] - 1, rho.shape[1] - 1, rho.shape[2] - 1
+    assert phi.shape == (nz + 3, ny + 3, nx + 3)
+    assert phi.dtype == np.float64 and rho.dtype == np.float64
+    assert phi.flags.c_contiguous and rho.flags.c_contiguous
+    if conductors is None:
+        conductors = Conductors()
+    b = np.array(bounds, np.int64)
+    it = np.zeros(1, np.int64)
+    err = np.zeros(1)
+    lib().oc_multigrid3dsolve(nx, ny, nz, dx, dy, dz, _p(phi), _p(rho), _p(b),
+                              mgparam, mgform, _p(it), mgmaxiters, mgmaxlevels,
+                              _p(err), mgtol, downpasses, uppasses,
+                              int(lcndbndy), icndbndy, conductors.ptr())
+    return int(it[0]), float(err[0])
+
+
+def setrho3d(rho, xp, yp, zp, q, wght, dx, dy, dz, xmmin, ymmin, zmmin,
+             zgrid=0.0):
+    nz, ny, nx = rho.shape[0] - 1, rho.shape[1] - 1, rho.shape[2] - 1
+    xp, yp, zp = _f64(xp), _f64(yp), _f64(zp)
+    lib().oc_setrho3d(_p(rho), len(xp), _p(xp), _p(yp), _p(zp), zgrid, q, wght,
+                      nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin)
+
+
+def applyrhoboundaryconditions3d(rho, bounds):
+    nz, ny, nx = rho.shape[0] - 1, rho.shape[1] - 1, rho.shape[2] - 1
+    b = np.array(bounds, np.int64)
+    lib().oc_applyrhoboundaryconditions3d(nx, ny, nz, _p(rho), _p(b))
+
+
+def sete3d(phi, xp, yp, zp, ex, ey, ez, dx, dy, dz, xmmin, ymmin, zmmin,
+           zgrid=0.0):
+    nz, ny, nx = phi.shape[0] - 3, phi.shape[1] - 3, phi.shape[2] - 3
+    xp, yp, zp = _f64(xp), _f64(yp), _f64(zp)
+    lib().oc_sete3d(_p(phi), len(xp), _p(xp), _p(yp), _p(zp), zgrid, xmmin,
+                    ymmin, zmmin, dx, dy, dz, nx, ny, nz, _p(ex), _p(ey), _p(ez))
+
+
+def epush3d(uxp, uyp, uzp, ex, ey, ez, q, m, ddt):
+    lib().oc_epush3d(len(uxp), _p(uxp), _p(uyp), _p(uzp), _p(ex), _p(ey),
+                     _p(ez), q, m, ddt)
+
+
+def xpush3d(xp, yp, zp, uxp, uyp, uzp, gaminv, dtp):
+    lib().oc_xpush3d(len(xp), _p(xp), _p(yp), _p(zp), _p(uxp), _p(uyp),
+                     _p(uzp), _p(gaminv), dtp)
+
+
+def getselfe3d(phi, dx, dy, dz, selfe=None, lzero=True):
+    nz, ny, nx = phi.shape[0] - 3, phi.shape[1] - 3, phi.shape[2] - 3
+    if selfe is None:
+        selfe = np.zeros((nz + 1, ny + 1, nx + 1, 3))
+    lib().oc_getselfe3d(_p(phi), nx, ny, nz, _p(selfe), dx, dy, dz, int(lzero))
+    return selfe
+
+
+def sorhalfpass3d(parity, phi, rho, dx, dy, dz, bounds, mgparam=1.2, mgform=1,
+                  mglevel=0, conductors=None, lcndbndy=True, icndbndy=2):
+    nz, ny, nx = rho.shape[0] - 1, rho.shape[1] - 1, rho.shape[2] - 1
+    b = np.array(bounds, np.int64)
+    cp = conductors.ptr() if conductors is not None else None
+    lib().oc_sorhalfpass3d(parity, mglevel, nx, ny, nz, _p(phi), _p(rho),
+                           1. / (dx * dx), 1. / (dy * dy), 1. / (dz * dz), _p(b),
+                           mgparam, mgform, int(lcndbndy), icndbndy, cp)
+
+
+def residual3d(phi, rho, dx, dy, dz, bounds, mgparam=1.2, mgform=1, mglevel=0,
+               conductors=None, lcndbndy=True, icndbndy=2, mgform2init=False):
+    nz, ny, nx = rho.shape[0] - 1, rho.shape[1] - 1, rho.shape[2] - 1
+    res = np.zeros((nz + 3, ny + 3, nx + 3))
+    b = np.array(bounds, np.int64)
+    cp = conductors.ptr() if conductors is not None else None
+    lib().oc_residual3d(nx, ny, nz, 1. / (dx * dx), 1. / (dy * dy), 1. / (dz * dz),
+                        _p(phi), _p(rho), _p(res), mglevel, _p(b), mgparam, mgform,
+                        int(mgform2init), int(lcndbndy), icndbndy, cp)
+    return res
+
+
+def getnextcoarselevel3d(nx, ny, nz, dx, dy, dz):
+    n = np.zeros(3, np.int64)
+    d = np.zeros(3)
+    lib().oc_getnextcoarselevel3d(nx, ny, nz, dx, dy, dz, _p(n[0:]), _p(n[1:]),
+                                  _p(n[2:]), _p(d[0:]), _p(d[1:]), _p(d[2:]))
+    return int(n[0]), int(n[1]), int(n[2]), float(d[0]), float(d[1]), float(d[2])
+
+
+def restrict3d(res, nxc, nyc, nzc, ff, bounds):
+    nz, ny, nx = res.shape[0] - 3, res.shape[1] - 3, res.shape[2] - 3
+    rc = np.zeros((nzc + 1, nyc + 1, nxc + 1))
+    b = np.array(bounds, np.int64)
+    lib().oc_restrict3d(nx, ny, nz, _p(res), nxc, nyc, nzc, _p(rc), ff, _p(b))
+    return rc
+
+
+def expand3d(phi, phicoarse, bounds):
+    nz, ny, nx = phi.shape[0] - 3, phi.shape[1] - 3, phi.shape[2] - 3
+    nzc, nyc, nxc = (phicoarse.shape[0] - 3, phicoarse.shape[1] - 3,
+                     phicoarse.shape[2] - 3)
+    b = np.array(bounds, np.int64)
+    lib().oc_expand3d(nx, ny, nz, _p(phi), nxc, nyc, nzc, _p(phicoarse), _p(b))
+
+
+def applyboundaryconditions3d(u, bounds, lwithdirichlet=False,
+                              lzerodirichlet=False):
+    nz, ny, nx = u.shape[0] - 3, u.shape[1] - 3, u.shape[2] - 3
+    b = np.array(bounds, np.int64)
+    lib().oc_applyboundaryconditions3d(nx, ny, nz, 1, 1, 1, _p(u), _p(b),
+                                       int(lwithdirichlet), int(lzerodirichlet))

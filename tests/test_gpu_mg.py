@@ -1,0 +1,211 @@
+"""GPU parity of the multigrid solver (libwarpmi through its C ABI) against
+the FP64 oracle, and the reference's own multigrid known-answer test."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(nx, ny, nz, lo, hi, bounds, **kw):
+    from warp_amd import MultiGrid3D
+    kw.setdefault("mgverbose", -1)
+    return MultiGrid3D(nx=nx, ny=ny, nz=nz, xmmin=lo[0], xmmax=hi[0], ymmin=lo[1],
+                       ymmax=hi[1], zmmin=lo[2], zmmax=hi[2], bounds=list(bounds), **kw)
+
+
+@pytest.mark.parametrize("bounds", H.all_bounds())
+def test_manufactured_kat(device, bounds):
+    """MutltiGrid3D_test.py on the GPU: 24x22x18 on [-1,1]^3, mgtol 1e-12,
+    error < 1e-10 and fewer than mgmaxiters V-cycles; and the same phi,
+    bit for bit, and the same V-cycle count as the oracle."""
+    nx, ny, nz = 24, 22, 18
+    phie, rho, phi0, dx, dy, dz = H.manufactured(nx, ny, nz, bounds)
+    s = _solver(nx, ny, nz, (-1, -1, -1), (1, 1, 1), bounds, mgtol=1e-12)
+    # As the reference test's generate(): an initial (empty) loadrho leaves
+    # the source finalized, so solve() does not fold rho at the faces again.
+    s.loadrho()
+    s.phi = torch.from_numpy(phi0).to(device)
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    phi = s.phi.cpu().numpy()
+    po, ro = phi0.copy(), rho.copy()
+    it, err = O.multigrid3dsolve(po, ro, dx, dy, dz, bounds=bounds, mgtol=1e-12)
+    assert s.mgiters == it
+    assert s.mgerror == err
+    np.testing.assert_array_equal(phi, po)
+    np.testing.assert_array_equal(s.rho.cpu().numpy(), ro)
+    t = phie.copy()
+    p = phi.copy()
+    if 0 not in bounds:
+        p -= p[1:-1, 1:-1, 1:-1].mean()
+        t -= t[1:-1, 1:-1, 1:-1].mean()
+    H.clearcorners(t)
+    H.clearcorners(p)
+    assert np.abs(p - t).max() < 1e-10
+    assert s.mgiters < s.mgmaxiters
+
+
+def _random_level_case(nx, ny, nz, seed):
+    rng = np.random.default_rng(seed)
+    phi = rng.standard_normal((nz + 3, ny + 3, nx + 3))
+    rho = rng.standard_normal((nz + 1, ny + 1, nx + 1))
+    return phi, rho
+
+
+@pytest.mark.parametrize("bounds", [(0,) * 6, (1, 1, 0, 1, 2, 2), (2, 2, 1, 0, 0, 1)])
+@pytest.mark.parametrize("conductors", [False, True])
+def test_kernels_match_oracle(device, bounds, conductors):
+    """Each kernel alone on random data, level 0 and level 1, bitwise:
+    sorhalfpass3d (both parities, with subgrid conductors), residual3d,
+    restrict3d, expand3d."""
+    nx, ny, nz = 32, 32, 64
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    s = _solver(nx, ny, nz, lo, hi, bounds)
+    co = None
+    if conductors:
+        import warp_amd
+        for g in H.pipe_and_annulus(warp_amd):
+            s.installconductor(g)
+        s.getconductorobject()
+        co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+        O.checkconductors(co, nx, ny, nz, s.dx, s.dy, s.dz)
+    from warp_amd import _native as N
+    st = N.stream_of(device)
+    for level in range(2):
+        (lnx, lny, lnz), (ldx, ldy, ldz) = s.level_dims(level)
+        phi, rho = _random_level_case(lnx, lny, lnz, 7 + level)
+        for parity in (0, 1):
+            for mgform in (1, 2):
+                pg = torch.from_numpy(phi).to(device)
+                rg = torch.from_numpy(rho).to(device)
+                N.call("wm_mg3d_sorhalfpass", s._h, level, parity, pg.data_ptr(), rg.data_ptr(),
+                       1.2, mgform, 1, 2, st)
+                po = phi.copy()
+                O.sorhalfpass3d(parity, po, rho, ldx, ldy, ldz, bounds, mgparam=1.2,
+                                mgform=mgform, mglevel=level, conductors=co)
+                np.testing.assert_array_equal(pg.cpu().numpy(), po)
+        for mgform2init in (0, 1):
+            pg = torch.from_numpy(phi).to(device)
+            rg = torch.from_numpy(rho).to(device)
+            res = torch.zeros_like(pg)
+            N.call("wm_mg3d_residual", s._h, level, pg.data_ptr(), rg.data_ptr(), res.data_ptr(),
+                   1.2, 1, mgform2init, 1, 2, st)
+            ro = O.residual3d(phi, rho, ldx, ldy, ldz, bounds, mgparam=1.2, mgform=1,
+                              mglevel=level, conductors=co, mgform2init=bool(mgform2init))
+            np.testing.assert_array_equal(res.cpu().numpy(), ro)
+        (cnx, cny, cnz), _ = s.level_dims(level + 1)
+        ff = ((1. / ldx ** 2 + 1. / ldy ** 2 + 1. / ldz ** 2) /
+              (1. / _[0] ** 2 + 1. / _[1] ** 2 + 1. / _[2] ** 2))
+        resv = torch.from_numpy(ro).to(device)
+        rc = torch.zeros((cnz + 1, cny + 1, cnx + 1), dtype=torch.float64, device=device)
+        N.call("wm_mg3d_restrict", s._h, level, resv.data_ptr(), rc.data_ptr(), st)
+        rco = O.restrict3d(ro, cnx, cny, cnz, ff, bounds)
+        np.testing.assert_array_equal(rc.cpu().numpy(), rco)
+        pc, _r = _random_level_case(cnx, cny, cnz, 11 + level)
+        pf = torch.from_numpy(phi).to(device)
+        N.call("wm_mg3d_expand", s._h, level, pf.data_ptr(),
+               torch.from_numpy(pc).to(device).data_ptr(), st)
+        pfo = phi.copy()
+        O.expand3d(pfo, pc, bounds)
+        np.testing.assert_array_equal(pf.cpu().numpy(), pfo)
+
+
+@pytest.mark.parametrize("lwd,lzd", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("bounds", [(0,) * 6, (1,) * 6, (2,) * 6, (0, 1, 2, 2, 1, 0)])
+def test_boundary_conditions(device, bounds, lwd, lzd):
+    from warp_amd import _native as N
+    nx, ny, nz = 10, 12, 14
+    u = np.random.default_rng(3).standard_normal((nz + 3, ny + 3, nx + 3))
+    ug = torch.from_numpy(u).to(device)
+    b = (C.c_int64 * 6)(*bounds)
+    N.call("wm_applyboundaryconditions3d", nx, ny, nz, ug.data_ptr(), b, lwd, lzd,
+           N.stream_of(device))
+    uo = u.copy()
+    O.applyboundaryconditions3d(uo, bounds, lwd, lzd)
+    np.testing.assert_array_equal(ug.cpu().numpy(), uo)
+
+
+def _c1_like(device, nx, ny, nz, precision="fp64", mgform=1, seed=5):
+    """Pipe + annulus with a Gaussian beam's rho; returns (solver, oracle
+    phi, oracle mgiters)."""
+    import warp_amd
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    s = _solver(nx, ny, nz, lo, hi, (0,) * 6, precision=precision, mgform=mgform)
+    for g in H.pipe_and_annulus(warp_amd):
+        s.installconductor(g)
+    rng = np.random.default_rng(seed)
+    from oracle import oracle as O2
+    x = rng.normal(0, 0.004, 20000)
+    y = rng.normal(0, 0.004, 20000)
+    z = 0.04 + rng.normal(0, 0.015, 20000)
+    k = (np.abs(x) < 0.0199) & (np.abs(y) < 0.0199) & (z > 0) & (z < 0.0799)
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    O2.setrho3d(rho, x[k], y[k], z[k], 1.602176565e-19, 1e5, s.dx, s.dy, s.dz, lo[0], lo[1], lo[2])
+    return s, rho
+
+
+def test_conductor_solve_fp64_bitwise(device):
+    """Full multigrid3dsolve with the pipe and annulus (subgrid conductors),
+    64x64x128: same phi bit for bit and the same V-cycle count."""
+    nx, ny, nz = 64, 64, 128
+    s, rho = _c1_like(device, nx, ny, nz)
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    ro = rho.copy()
+    it, err = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co)
+    assert s.mgiters == it and s.mgerror == err
+    np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+    # warm start from the converged phi: one more V-cycle
+    s.solve()
+    it2, err2 = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co)
+    assert s.mgiters == it2
+    np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+
+
+def test_conductor_solve_mgform2_bitwise(device):
+    nx, ny, nz = 32, 32, 64
+    s, rho = _c1_like(device, nx, ny, nz, mgform=2)
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    it, err = O.multigrid3dsolve(po, rho.copy(), s.dx, s.dy, s.dz, conductors=co, mgform=2)
+    assert s.mgiters == it
+    np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+
+
+def test_conductor_solve_fp32_defect_correction(device):
+    """FP32 V-cycles in an FP64 defect-correction loop still reach mgtol and
+    agree with the FP64 oracle to the tolerance the convergence test
+    implies."""
+    nx, ny, nz = 64, 64, 128
+    s, rho = _c1_like(device, nx, ny, nz, precision="fp32")
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    assert s.mgerror <= s.mgtol
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    O.multigrid3dsolve(po, rho.copy(), s.dx, s.dy, s.dz, conductors=co)
+    d = s.phi.cpu().numpy() - po
+    # both are converged to max|dphi| <= 1e-6 V on a 10 kV problem
+    assert np.abs(d).max() < 1e-4
+    assert np.linalg.norm(d) / np.linalg.norm(po) < 1e-8
+
+
+def test_nonconvergence_reported(device):
+    """mgmaxiters = 1 returns mgerror > mgtol without raising
+    (f3d_mgrid.F:460-463)."""
+    nx, ny, nz = 32, 32, 64
+    s, rho = _c1_like(device, nx, ny, nz)
+    s.mgmaxiters = 1
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    assert s.mgiters == 1 and s.mgerror > s.mgtol

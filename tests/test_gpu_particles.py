@@ -1,0 +1,164 @@
+"""GPU parity of the particle kernels (deposit, gather, push, grid E)
+against the FP64 oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LO = (-0.02, -0.02, 0.0)
+NX, NY, NZ = 64, 64, 64
+DX, DY, DZ = 0.04 / NX, 0.04 / NY, 0.08 / NZ
+
+
+def _beam(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 0.004, n)
+    y = rng.normal(0, 0.004, n)
+    z = 0.04 + rng.normal(0, 0.015, n)
+    k = (x > LO[0]) & (x < -LO[0]) & (y > LO[1]) & (y < -LO[1]) & (z > 0) & (z < 0.08)
+    return x[k], y[k], z[k]
+
+
+def _t(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_deposit_matches_oracle(device, sorted_):
+    """setrho3d: the GPU sums in another order, so rho agrees to rounding
+    (relative 1e-13 of the peak) and the total charge is conserved."""
+    from warp_amd import _native as N
+    x, y, z = _beam(200000, 1)
+    q, w = 1.602176565e-19, 3.7e4
+    rho = torch.zeros((NZ + 1, NY + 1, NX + 1), dtype=torch.float64, device=device)
+    args = [rho.data_ptr(), len(x), _t(x, device).data_ptr(), _t(y, device).data_ptr(),
+            _t(z, device).data_ptr(), 0.0, q, w, NX, NY, NZ, DX, DY, DZ, LO[0], LO[1], LO[2]]
+    keep = [_t(x, device), _t(y, device), _t(z, device)]
+    args[2:5] = [k.data_ptr() for k in keep]
+    st = N.stream_of(device)
+    if sorted_:
+        work = torch.empty(N.lib().wm_setrho3d_sorted_worksize(len(x), NX, NY, NZ),
+                           dtype=torch.uint8, device=device)
+        N.call("wm_setrho3d_sorted", *args, work.data_ptr(), st)
+    else:
+        ns = C.c_int64(-1)
+        N.call("wm_setrho3d", *args, C.byref(ns), st)
+        assert ns.value == 0
+    ro = np.zeros((NZ + 1, NY + 1, NX + 1))
+    O.setrho3d(ro, x, y, z, q, w, DX, DY, DZ, *LO)
+    r = rho.cpu().numpy()
+    assert np.abs(r - ro).max() <= 1e-13 * np.abs(ro).max()
+    total = q * w * len(x) / (DX * DY * DZ)
+    assert abs(r.sum() - total) <= 1e-12 * total
+
+
+def test_deposit_skips_out_of_grid(device):
+    from warp_amd import _native as N
+    x = np.array([0.0, 0.03, -0.025, 0.0])
+    y = np.array([0.0, 0.0, 0.0, 0.0])
+    z = np.array([0.04, 0.04, 0.04, 0.08])   # 0.08 = zmmax is outside [0, 0.08)
+    rho = torch.zeros((NZ + 1, NY + 1, NX + 1), dtype=torch.float64, device=device)
+    keep = [_t(v, device) for v in (x, y, z)]
+    ns = C.c_int64(-1)
+    N.call("wm_setrho3d", rho.data_ptr(), 4, keep[0].data_ptr(), keep[1].data_ptr(),
+           keep[2].data_ptr(), 0.0, 1.0, 1.0, NX, NY, NZ, DX, DY, DZ, *LO, C.byref(ns),
+           N.stream_of(device))
+    assert ns.value == 3
+    assert abs(rho.sum().item() - 1.0 / (DX * DY * DZ)) < 1e-6
+
+
+def _phi(seed):
+    return np.random.default_rng(seed).standard_normal((NZ + 3, NY + 3, NX + 3))
+
+
+def test_gather_bitwise(device):
+    """sete3d: bit-identical E at every particle."""
+    from warp_amd import _native as N
+    x, y, z = _beam(100000, 2)
+    phi = _phi(3)
+    ex, ey, ez = (torch.full((len(x),), 0.5, dtype=torch.float64, device=device) for _ in range(3))
+    keep = [_t(v, device) for v in (x, y, z)]
+    N.call("wm_sete3d", _t(phi, device).data_ptr(), len(x), keep[0].data_ptr(),
+           keep[1].data_ptr(), keep[2].data_ptr(), 0.0, *LO, DX, DY, DZ, NX, NY, NZ,
+           ex.data_ptr(), ey.data_ptr(), ez.data_ptr(), N.stream_of(device))
+    eo = [np.full(len(x), 0.5) for _ in range(3)]
+    O.sete3d(phi, x, y, z, *eo, DX, DY, DZ, *LO)
+    for a, b in zip((ex, ey, ez), eo):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)
+
+
+def test_push_bitwise(device):
+    from warp_amd import _native as N
+    rng = np.random.default_rng(4)
+    n = 50000
+    u = [rng.normal(0, 1e3, n) for _ in range(3)]
+    e = [rng.normal(0, 1e5, n) for _ in range(3)]
+    xx = [rng.normal(0, 1e-2, n) for _ in range(3)]
+    gi = np.ones(n)
+    ug = [_t(a, device) for a in u]
+    eg = [_t(a, device) for a in e]
+    xg = [_t(a, device) for a in xx]
+    st = N.stream_of(device)
+    q, m, dt = 1.602176565e-19, 39 * 1.660538921e-27, 1e-9
+    N.call("wm_epush3d", n, *[a.data_ptr() for a in ug], *[a.data_ptr() for a in eg], q, m,
+           0.5 * dt, st)
+    N.call("wm_xpush3d", n, *[a.data_ptr() for a in xg], *[a.data_ptr() for a in ug],
+           _t(gi, device).data_ptr(), dt, st)
+    uo = [a.copy() for a in u]
+    xo = [a.copy() for a in xx]
+    O.epush3d(*uo, *e, q, m, 0.5 * dt)
+    O.xpush3d(*xo, *uo, gi, dt)
+    for a, b in zip(ug + xg, uo + xo):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)
+
+
+def test_gather_push_matches_sequence(device):
+    """The fused fullv advance equals E=0, sete3d, epush3d(dt/2) twice,
+    xpush3d(dt) of the oracle, bit for bit."""
+    from warp_amd import _native as N
+    x, y, z = _beam(100000, 5)
+    n = len(x)
+    rng = np.random.default_rng(6)
+    u = [rng.normal(0, 1e3, n), rng.normal(0, 1e3, n), 1e6 + rng.normal(0, 1e3, n)]
+    phi = _phi(7) * 100.
+    q, m, dt = 1.602176565e-19, 39 * 1.660538921e-27, 1e-9
+    g = [_t(a, device) for a in (x, y, z)] + [_t(a, device) for a in u]
+    gi = torch.ones(n, dtype=torch.float64, device=device)
+    E = [torch.zeros(n, dtype=torch.float64, device=device) for _ in range(3)]
+    N.call("wm_gather_push3d", _t(phi, device).data_ptr(), n, *[a.data_ptr() for a in g],
+           gi.data_ptr(), *[a.data_ptr() for a in E], 0.0, *LO, DX, DY, DZ, NX, NY, NZ, q, m,
+           dt, N.stream_of(device))
+    xo, uo = [x.copy(), y.copy(), z.copy()], [a.copy() for a in u]
+    eo = [np.zeros(n) for _ in range(3)]
+    O.sete3d(phi, *xo, *eo, DX, DY, DZ, *LO)
+    O.epush3d(*uo, *eo, q, m, 0.5 * (dt * 1 * 1.))
+    O.epush3d(*uo, *eo, q, m, 0.5 * (dt * 1 * 1.))
+    O.xpush3d(*xo, *uo, np.ones(n), dt)
+    for a, b in zip(g + E, xo + uo + eo):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)
+
+
+def test_getselfe_bitwise(device):
+    from warp_amd import _native as N
+    phi = _phi(8)
+    se = torch.zeros((NZ + 1, NY + 1, NX + 1, 3), dtype=torch.float64, device=device)
+    N.call("wm_getselfe3d", _t(phi, device).data_ptr(), NX, NY, NZ, se.data_ptr(), DX, DY, DZ,
+           1, N.stream_of(device))
+    np.testing.assert_array_equal(se.cpu().numpy(), O.getselfe3d(phi, DX, DY, DZ))
+
+
+@pytest.mark.parametrize("bounds", [(1, 1, 1, 1, 1, 1), (2, 2, 2, 2, 2, 2), (0, 1, 2, 2, 1, 0)])
+def test_rho_boundaries_bitwise(device, bounds):
+    from warp_amd import _native as N
+    r = np.random.default_rng(9).standard_normal((NZ + 1, NY + 1, NX + 1))
+    rg = _t(r, device)
+    b = (C.c_int64 * 6)(*bounds)
+    N.call("wm_applyrhoboundaryconditions3d", NX, NY, NZ, rg.data_ptr(), b, N.stream_of(device))
+    ro = r.copy()
+    O.applyrhoboundaryconditions3d(ro, bounds)
+    np.testing.assert_array_equal(rg.cpu().numpy(), ro)

@@ -1,0 +1,1194 @@
+// Multigrid Poisson solver with subgrid-resolved conductors for gfx950:
+// the MI355X implementation of multigrid3dsolve (pywarp90/f3d_mgrid.F).
+//
+// The host side builds the level hierarchy (getmglevels), the transfer
+// tables (restrict3d/expand3d weights), the conductor stencil records and
+// masks, and drives the V-cycle.  One V-cycle plus its convergence test is
+// captured once as a HIP graph and replayed; the convergence kernel sets a
+// device flag that turns every later kernel into a no-op, so several
+// V-cycles can be enqueued before the host looks at the result.
+#include "wm_internal.h"
+#include "mg3d_kernels.h"
+#include "../../include/warpmi.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    WM_HIP(hipMalloc((void **)&p, sizeof(T) * count));
+    WM_HIP(hipMemset(p, 0, sizeof(T) * count));
+    n = count;
+  }
+  void upload(const std::vector<T> &v) {
+    alloc(v.size());
+    if (!v.empty()) WM_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  }
+};
+
+enum { MODE_RAW = SIDE_RAW, MODE_MUL = SIDE_MUL, MODE_DIV = SIDE_DIV, MODE_CONST = SIDE_CONST };
+
+struct Sym {
+  int mode = MODE_RAW;
+  double k = 0.;
+};
+
+static Sym sym_mul(Sym a, double k) {
+  if (a.mode == MODE_RAW) return Sym{MODE_MUL, k};
+  if (a.mode == MODE_CONST) return Sym{MODE_CONST, a.k * k};
+  throw wm_error("subgrid stencil: unexpected double scaling of a neighbour");
+}
+static Sym sym_div(Sym a, double k) {
+  if (a.mode == MODE_RAW) return Sym{MODE_DIV, k};
+  if (a.mode == MODE_CONST) return Sym{MODE_CONST, a.k / k};
+  throw wm_error("subgrid stencil: unexpected double scaling of a neighbour");
+}
+
+// handlesubgrid2 (f3d_mgrid.F:2628-2764) with the neighbour values kept
+// symbolic; every number is produced by the reference's own expression.
+static void handlesubgrid2(int im, int ip, Sym &pm, Sym &pp, double dsqicos,
+                           const double *d, const double *v, double voltfac,
+                           double &denom, double &ppp) {
+  double adel;
+  if (0. < d[im] && d[im] < +1. && 0. < d[ip] && d[ip] < +1.) {
+    pm = Sym{MODE_CONST, voltfac * v[im] * 2. / (d[im] * (d[ip] + d[im]))};
+    pp = Sym{MODE_CONST, voltfac * v[ip] * 2. / (d[ip] * (d[ip] + d[im]))};
+    denom = denom + 2. * (1. - d[ip] * d[im]) / (d[ip] * d[im]) * dsqicos;
+    ppp = std::min(ppp, d[im]);
+    ppp = std::min(ppp, d[ip]);
+  } else if (0. >= d[im] && d[im] > -1. && 0. >= d[ip] && d[ip] > -1.) {
+    pm = Sym{MODE_CONST, 0.};
+    pp = Sym{MODE_CONST, 0.};
+    denom = denom - 2. * dsqicos;
+    if (std::fabs(d[im]) > 0.) ppp = std::min(ppp, std::fabs(d[im]));
+    else ppp = std::min(ppp, 1. - 1.e-9);
+    if (std::fabs(d[ip]) > 0.) ppp = std::min(ppp, std::fabs(d[ip]));
+    else ppp = std::min(ppp, 1. - 1.e-9);
+  } else if (0. < d[im] && d[im] < +1. && 0. >= d[ip] && d[ip] > -1.) {
+    adel = std::fabs(d[ip]);
+    pm = Sym{MODE_CONST, voltfac * v[im] / (d[im] * (adel + 0.5 * d[im]))};
+    pp = Sym{MODE_CONST, 0.};
+    denom = denom + (1. - 2. * d[im] * adel - d[im]) / (d[im] * (adel + 0.5 * d[im])) * dsqicos;
+    ppp = std::min(ppp, d[im]);
+    if (adel > 0.) ppp = std::min(ppp, adel);
+    else ppp = std::min(ppp, 1. - 1.e-9);
+  } else if (0. >= d[im] && d[im] > -1. && 0. < d[ip] && d[ip] < +1.) {
+    adel = std::fabs(d[im]);
+    pm = Sym{MODE_CONST, 0.};
+    pp = Sym{MODE_CONST, voltfac * v[ip] / (d[ip] * (adel + 0.5 * d[ip]))};
+    denom = denom + (1. - 2. * d[ip] * adel - d[ip]) / (d[ip] * (adel + 0.5 * d[ip])) * dsqicos;
+    if (adel > 0.) ppp = std::min(ppp, adel);
+    else ppp = std::min(ppp, 1. - 1.e-9);
+    ppp = std::min(ppp, d[ip]);
+  } else {
+    if (0. < d[im] && d[im] < +1.) {
+      pm = Sym{MODE_CONST, voltfac * v[im] * 2. / (d[im] * (1. + d[im]))};
+      pp = sym_mul(pp, 2. / (1. + d[im]));
+      denom = denom + 2. * (1. - d[im]) / d[im] * dsqicos;
+      ppp = std::min(ppp, d[im]);
+    } else if (0. >= d[im] && d[im] > -1.) {
+      adel = std::fabs(d[im]);
+      pm = Sym{MODE_CONST, 0.};
+      pp = sym_div(pp, 0.5 + adel);
+      denom = denom - 2. * adel / (adel + 0.5) * dsqicos;
+      if (adel > 0.) ppp = std::min(ppp, adel);
+      else ppp = std::min(ppp, 1. - 1.e-9);
+    }
+    if (0. < d[ip] && d[ip] < +1.) {
+      pp = Sym{MODE_CONST, voltfac * v[ip] * 2. / (d[ip] * (1. + d[ip]))};
+      pm = sym_mul(pm, 2. / (1. + d[ip]));
+      denom = denom + 2. * (1. - d[ip]) / d[ip] * dsqicos;
+      ppp = std::min(ppp, d[ip]);
+    } else if (0. >= d[ip] && d[ip] > -1.) {
+      adel = std::fabs(d[ip]);
+      pp = Sym{MODE_CONST, 0.};
+      pm = sym_div(pm, 0.5 + adel);
+      denom = denom - 2. * adel / (adel + 0.5) * dsqicos;
+      if (adel > 0.) ppp = std::min(ppp, adel);
+      else ppp = std::min(ppp, 1. - 1.e-9);
+    }
+  }
+}
+
+// Host image of one colour's subgrid records.
+struct SubHost {
+  std::vector<int> pos;
+  std::vector<uint32_t> modes, zero;
+  std::vector<double> k, denom, fac;
+  int n() const { return (int)modes.size(); }
+};
+
+template <typename T>
+struct SubDev {
+  DevBuf<int> pos;
+  DevBuf<uint32_t> modes, zero;
+  DevBuf<T> k, denom, fac;
+  int n = 0;
+  void upload(const SubHost &h) {
+    n = h.n();
+    pos.upload(h.pos);
+    modes.upload(h.modes);
+    zero.upload(h.zero);
+    std::vector<T> kk(h.k.begin(), h.k.end()), dd(h.denom.begin(), h.denom.end()),
+        ff(h.fac.begin(), h.fac.end());
+    k.upload(kk);
+    denom.upload(dd);
+    fac.upload(ff);
+  }
+  SubList<T> view() const {
+    return SubList<T>{n, pos.p, modes.p, k.p, denom.p, fac.p, zero.p};
+  }
+};
+
+struct RecKey {
+  double mgparam = -1.;
+  int mgform = -1, icndbndy = -1, lcndbndy = -1, mgform2init = -1;
+  bool operator==(const RecKey &o) const {
+    return mgparam == o.mgparam && mgform == o.mgform && icndbndy == o.icndbndy &&
+           lcndbndy == o.lcndbndy && mgform2init == o.mgform2init;
+  }
+};
+
+// Conductor records of one level for one precision.
+template <typename T>
+struct CondDev {
+  RecKey sorkey, reskey[2];
+  SubDev<T> sor[2];
+  SubDev<T> res[2][2];  // [mgform2init][colour]
+  void reset() {
+    sorkey = reskey[0] = reskey[1] = RecKey();
+    for (int p = 0; p < 2; p++) {
+      sor[p].upload(SubHost());
+      res[0][p].upload(SubHost());
+      res[1][p].upload(SubHost());
+    }
+  }
+};
+
+struct Level {
+  int nx, ny, nz;
+  double dx, dy, dz, dxsqi, dysqi, dzsqi;
+  RBGeom g;
+  // buffers (allocated on demand)
+  DevBuf<double> phi64, rho64, res64;
+  DevBuf<float> phi32, rho32, res32;
+  // transfer to the next coarser level
+  bool coarser = false;
+  double ff = 0.;
+  DevBuf<int> rlo[3], rhi[3];
+  DevBuf<double> rw[3];
+  int rcmin[3], rcmax[3];
+  DevBuf<int> ej[3];
+  DevBuf<double> ew[3];
+  int elo[3], ehi[3];
+  // conductors
+  int nint = 0;
+  DevBuf<int> int_pos;
+  DevBuf<double> int_volt;
+  DevBuf<uint32_t> mask;
+  bool hasmask = false;
+  std::vector<int> subpos_h[2];
+  std::vector<double> subdels_h[2], subvolt_h[2];
+  DevBuf<int> subpos_d[2];
+  DevBuf<double> subdels_d[2];
+  bool neumann = false;
+  CondDev<double> c64;
+  CondDev<float> c32;
+  RecKey maskkey;
+};
+
+}  // namespace
+
+struct wm_mg3d {
+  int64_t nx, ny, nz;
+  double dx, dy, dz;
+  int bounds[6];
+  std::vector<std::unique_ptr<Level>> lv;
+  // level-0 extras
+  DevBuf<double> save64, rhosave64;
+  DevBuf<SolveState> state;
+  SolveState *state_h = nullptr;  // pinned
+  hipStream_t s = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // captured V-cycle graphs
+  hipGraphExec_t gexec = nullptr;
+  struct GKey {
+    int prec = -1, mgform = -1, down = -1, up = -1, lcnd = -1, icnd = -1, maxit = -1;
+    double mgparam = -1., mgtol = -1.;
+    bool operator==(const GKey &o) const {
+      return prec == o.prec && mgform == o.mgform && down == o.down && up == o.up &&
+             lcnd == o.lcnd && icnd == o.icnd && maxit == o.maxit && mgparam == o.mgparam &&
+             mgtol == o.mgtol;
+    }
+  } gkey;
+  int last_iters = 0;
+  bool conductors_set = false;
+  ~wm_mg3d() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (state_h) (void)hipHostFree(state_h);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+static int blocks(long long n) { return (int)std::max<long long>(1, cdiv(n, 256)); }
+
+// getnextcoarselevel3d (f3d_mgrid.F:3601-3652), mgcoarsening = 2.
+static void next_coarse(int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                        double dz, int64_t &nxc, int64_t &nyc, int64_t &nzc,
+                        double &dxc, double &dyc, double &dzc) {
+  nxc = (nx + 1) / 2;
+  if (nxc % 2 == 1) nxc = nxc + 1;
+  nyc = (ny + 1) / 2;
+  if (nyc % 2 == 1) nyc = nyc + 1;
+  nzc = (nz + 1) / 2;
+  if (nzc % 2 == 1) nzc = nzc + 1;
+  double mingridsize = std::min(dx, dy);
+  mingridsize = std::min(mingridsize, dz);
+  if (dx > 4. / 3. * mingridsize) nxc = nx;
+  if (dy > 4. / 3. * mingridsize) nyc = ny;
+  if (dz > 4. / 3. * mingridsize) nzc = nz;
+  dxc = dx * nx / nxc;
+  dyc = nyc == 0 ? dxc : dy * ny / nyc;
+  dzc = dz * nz / nzc;
+}
+
+static RBGeom make_geom(int nx, int ny, int nz) {
+  RBGeom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.W = (nx + 4) / 2;
+  g.PY = ny + 3;
+  g.PZ = nz + 3;
+  g.cs = (long long)g.W * g.PY * g.PZ;
+  return g;
+}
+
+// restrict3d's per-axis window tables (f3d_mgrid.F:1654-1688), serial
+// (offset 0), res guard 1.
+static void restrict_axis(int n, int nc, int lbm, int lbp, Level &L, int a) {
+  int cmin = lbm == 0 ? 1 : 0, cmax = lbp == 0 ? nc - 1 : nc;
+  double di = n > 0 ? 1. * nc / n : 1.;
+  std::vector<int> lo(nc + 1, 0), hi(nc + 1, -1);
+  std::vector<double> w(5 * (nc + 1), 0.);
+  for (int ic = cmin; ic <= cmax; ic++) {
+    int imin = ((ic - 1) * n + 4 * nc) / nc - 3;
+    int imax = ((ic + 1) * n - 1) / nc;
+    lo[ic] = std::max(imin, -1);
+    hi[ic] = std::min(imax, n + 1);
+    for (int i = imin; i <= imax; i++) {
+      WM_REQUIRE(i - imin < 5, "restrict3d window wider than 5 points");
+      w[5 * ic + (i - imin)] = 1. - std::fabs(ic - (i + 0.) * di);
+    }
+  }
+  L.rlo[a].upload(lo);
+  L.rhi[a].upload(hi);
+  L.rw[a].upload(w);
+  L.rcmin[a] = cmin;
+  L.rcmax[a] = cmax;
+}
+
+// expand3d's per-axis tables (f3d_mgrid.F:1833-1844).
+static void expand_axis(int n, int nc, int lbm, int lbp, Level &L, int a) {
+  int lo = lbm == 0 ? 1 : 0, hi = lbp == 0 ? n - 1 : n;
+  std::vector<int> j(n + 1, 0);
+  std::vector<double> w(n + 1, 0.);
+  for (int i = lo; i <= hi; i++) {
+    j[i] = (i * nc) / n;
+    w[i] = 1. * (i * nc) / n - j[i];
+  }
+  L.ej[a].upload(j);
+  L.ew[a].upload(w);
+  L.elo[a] = lo;
+  L.ehi[a] = hi;
+}
+
+struct LoopRange {
+  int ixmin, ixmax, iymin, iymax, izmin, izmax;
+};
+
+// Loop limits of sorhalfpass3d (bounds <= 0 excluded, f3d_mgrid.F:2159-2171).
+static LoopRange sor_range(const Level &L, const int *lb) {
+  LoopRange r{0, L.nx, 0, L.ny, 0, L.nz};
+  if (lb[0] <= 0) r.ixmin = 1;
+  if (lb[1] <= 0) r.ixmax = L.nx - 1;
+  if (lb[2] <= 0) r.iymin = 1;
+  if (lb[3] <= 0) r.iymax = L.ny - 1;
+  if (lb[4] <= 0) r.izmin = 1;
+  if (lb[5] <= 0) r.izmax = L.nz - 1;
+  return r;
+}
+
+// Loop limits of residual3d and expand3d (bounds == 0 excluded).
+static LoopRange res_range(const Level &L, const int *lb) {
+  LoopRange r{0, L.nx, 0, L.ny, 0, L.nz};
+  if (lb[0] == 0) r.ixmin = 1;
+  if (lb[1] == 0) r.ixmax = L.nx - 1;
+  if (lb[2] == 0) r.iymin = 1;
+  if (lb[3] == 0) r.iymax = L.ny - 1;
+  if (lb[4] == 0) r.izmin = 1;
+  if (lb[5] == 0) r.izmax = L.nz - 1;
+  return r;
+}
+
+// Build one colour's subgrid records for the SOR (residual = false) or the
+// residual (residual = true) from the reference's condbndymg/condbndyres.
+static SubHost build_records(const Level &L, int level, int p, const int *lb,
+                             double mgparam, int mgform, int icndbndy,
+                             bool residual, bool mgform2init) {
+  SubHost h;
+  const std::vector<int> &pos = L.subpos_h[p];
+  const std::vector<double> &dels = L.subdels_h[p], &volt = L.subvolt_h[p];
+  int n = (int)pos.size() / 3;
+  double dxsqic, dysqic, dzsqic, dxs, dys, dzs;
+  double voltfac;
+  if (!residual) {
+    double cst = mgparam * 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+    dxsqic = L.dxsqi * cst;
+    dysqic = L.dysqi * cst;
+    dzsqic = L.dzsqi * cst;
+    dxs = dxsqic / mgparam;  // dxsqicos
+    dys = dysqic / mgparam;
+    dzs = dzsqic / mgparam;
+    voltfac = (level == 0 && mgform == 1) ? 1. : 0.;
+  } else {
+    double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+    dxs = L.dxsqi * cst;  // dxsqic of condbndyres
+    dys = L.dysqi * cst;
+    dzs = L.dzsqi * cst;
+    voltfac = (level == 0 && (mgform == 1 || mgform2init)) ? 1. : 0.;
+  }
+  for (int ic = 0; ic < n; ic++) {
+    int ix = pos[3 * ic], iy = pos[3 * ic + 1], iz = pos[3 * ic + 2];
+    if (ix == 0 && lb[0] < 1) continue;
+    if (ix == L.nx && lb[1] < 1) continue;
+    if (iy == 0 && lb[2] < 1) continue;
+    if (iy == L.ny && lb[3] < 1) continue;
+    if (iz == 0 && lb[4] < 1) continue;
+    if (iz == L.nz && lb[5] < 1) continue;
+    const double *d = &dels[6 * ic], *v = &volt[6 * ic];
+    Sym s[6];
+    double denom = 1., ppp = 1.;
+    bool active;
+    if (icndbndy == 1) {
+      bool dosub = false;
+      const double dsq[3] = {dxs, dys, dzs};
+      for (int k = 0; k < 6; k++) {
+        if (0 < d[k] && d[k] < 1. && (k < 2 || k > 3 || L.ny > 0)) {
+          s[k] = Sym{MODE_CONST, voltfac * v[k] / d[k]};
+          denom = denom + (1. - d[k]) / d[k] * dsq[k / 2];
+          ppp = std::min(ppp, d[k]);
+          dosub = true;
+        }
+      }
+      active = residual ? (ppp < 1.) : dosub;
+    } else {
+      handlesubgrid2(0, 1, s[0], s[1], dxs, d, v, voltfac, denom, ppp);
+      if (L.ny > 0) handlesubgrid2(2, 3, s[2], s[3], dys, d, v, voltfac, denom, ppp);
+      handlesubgrid2(4, 5, s[4], s[5], dzs, d, v, voltfac, denom, ppp);
+      active = ppp < 1.;
+    }
+    if (!active) continue;
+    uint32_t m = 0, z = 0;
+    for (int k = 0; k < 6; k++) m |= (uint32_t)s[k].mode << (2 * k);
+    if (residual && icndbndy == 2) {
+      for (int k = 0; k < 6; k++)
+        if (0. >= d[k] && d[k] >= -1. && (k < 2 || k > 3 || L.ny > 0)) z |= 1u << k;
+    }
+    h.pos.insert(h.pos.end(), {ix, iy, iz});
+    h.modes.push_back(m);
+    h.zero.push_back(z);
+    for (int k = 0; k < 6; k++) h.k.push_back(s[k].k);
+    h.denom.push_back(denom);
+    h.fac.push_back(mgform2init ? 1. : ppp);
+  }
+  return h;
+}
+
+template <typename T>
+CondDev<T> &cond_of(Level &L);
+template <>
+CondDev<double> &cond_of<double>(Level &L) { return L.c64; }
+template <>
+CondDev<float> &cond_of<float>(Level &L) { return L.c32; }
+
+template <typename T>
+DevBuf<T> &phi_of(Level &L);
+template <>
+DevBuf<double> &phi_of<double>(Level &L) { return L.phi64; }
+template <>
+DevBuf<float> &phi_of<float>(Level &L) { return L.phi32; }
+template <typename T>
+DevBuf<T> &rho_of(Level &L);
+template <>
+DevBuf<double> &rho_of<double>(Level &L) { return L.rho64; }
+template <>
+DevBuf<float> &rho_of<float>(Level &L) { return L.rho32; }
+template <typename T>
+DevBuf<T> &res_of(Level &L);
+template <>
+DevBuf<double> &res_of<double>(Level &L) { return L.res64; }
+template <>
+DevBuf<float> &res_of<float>(Level &L) { return L.res32; }
+
+struct Params {
+  double mgparam;
+  int mgform, down, up, lcnd, icnd;
+};
+
+// Mask of points the dense kernels leave to the conductor kernels:
+// interior points and active subgrid points of this level.
+static void build_mask(wm_mg3d *h, Level &L, int level, const Params &P) {
+  RecKey key;
+  key.mgparam = P.mgparam;  // activity does not depend on it; keep it simple
+  key.icndbndy = P.icnd;
+  key.lcndbndy = P.lcnd;
+  if (L.hasmask && L.maskkey.icndbndy == key.icndbndy && L.maskkey.lcndbndy == key.lcndbndy)
+    return;
+  std::vector<uint32_t> m((size_t)cdiv(2 * L.g.cs, 32), 0u);
+  auto setbit = [&](int ix, int iy, int iz) {
+    long long e = rb_index(L.g, ix, iy, iz);
+    m[e >> 5] |= 1u << (e & 31);
+  };
+  std::vector<int> ipos;
+  // interior positions are kept in L via int_pos upload; rebuild from host copy
+  for (size_t i = 0; i < L.subpos_h[0].size() && false; i++) (void)i;
+  bool any = false;
+  if (L.nint > 0) {
+    std::vector<int> hp(3 * L.nint);
+    WM_HIP(hipMemcpy(hp.data(), L.int_pos.p, sizeof(int) * hp.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < L.nint; i++) setbit(hp[3 * i], hp[3 * i + 1], hp[3 * i + 2]);
+    any = true;
+  }
+  if (P.lcnd) {
+    for (int p = 0; p < 2; p++) {
+      SubHost sh = build_records(L, level, p, h->bounds, P.mgparam, 1, P.icnd, false, false);
+      for (int i = 0; i < sh.n(); i++) {
+        setbit(sh.pos[3 * i], sh.pos[3 * i + 1], sh.pos[3 * i + 2]);
+        any = true;
+      }
+    }
+  }
+  if (any) L.mask.upload(m);
+  else L.mask.release();
+  L.hasmask = any;
+  L.maskkey = key;
+}
+
+template <typename T>
+static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
+                           bool residual, int mgform2init) {
+  CondDev<T> &C = cond_of<T>(L);
+  RecKey key;
+  key.mgparam = P.mgparam;
+  key.mgform = P.mgform;
+  key.icndbndy = P.icnd;
+  key.lcndbndy = P.lcnd;
+  key.mgform2init = residual ? mgform2init : 0;
+  if (!residual) {
+    if (C.sorkey == key) return;
+    for (int p = 0; p < 2; p++) {
+      SubHost sh;
+      if (P.lcnd) sh = build_records(L, level, p, h->bounds, P.mgparam, P.mgform, P.icnd, false, false);
+      C.sor[p].upload(sh);
+    }
+    C.sorkey = key;
+  } else {
+    if (C.reskey[mgform2init] == key) return;
+    for (int p = 0; p < 2; p++) {
+      SubHost sh;
+      if (P.lcnd)
+        sh = build_records(L, level, p, h->bounds, P.mgparam, P.mgform, P.icnd, true, mgform2init != 0);
+      C.res[mgform2init][p].upload(sh);
+    }
+    C.reskey[mgform2init] = key;
+  }
+}
+
+template <typename T>
+static void ensure_buffers(Level &L, bool with_res) {
+  phi_of<T>(L).alloc(2 * L.g.cs);
+  rho_of<T>(L).alloc(2 * L.g.cs);
+  if (with_res) res_of<T>(L).alloc((size_t)(L.nx + 3) * (L.ny + 3) * (L.nz + 3));
+}
+
+template <typename T, typename Idx>
+static void launch_bc(T *u, Idx I, int nx, int ny, int nz, const int *b, int lwd, int lzd,
+                      const int *skip, hipStream_t s) {
+  // x, then y, then z; skipped entirely when nothing would change.
+  int need[3];
+  for (int a = 0; a < 3; a++)
+    need[a] = (b[2 * a] != 0 || b[2 * a + 1] != 0 || lwd || lzd);
+  int dims[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    if (!need[a]) continue;
+    int na = a == 0 ? ny : nx, nb = a == 2 ? ny : nz;
+    long long n = (long long)(na + 3) * (nb + 3);
+    hipLaunchKernelGGL((k_bc_axis<T, Idx>), dim3(blocks(n)), dim3(256), 0, s, u, I, a, nx, ny,
+                       nz, b[2 * a], b[2 * a + 1], lwd, lzd, skip);
+    WM_LAUNCHED();
+    (void)dims;
+  }
+}
+
+template <typename T>
+static void pin_and_mirror(wm_mg3d *h, Level &L, int level, const Params &P, T *phi,
+                           int usevolt, const int *skip, hipStream_t s) {
+  if (L.nint > 0) {
+    hipLaunchKernelGGL((k_pin<T>), dim3(blocks(L.nint)), dim3(256), 0, s, phi, L.g, L.nint,
+                       L.int_pos.p, L.int_volt.p, usevolt, skip);
+    WM_LAUNCHED();
+  }
+  if (P.icnd == 2 && L.neumann) {
+    const int *b = h->bounds;
+    hipLaunchKernelGGL((k_condbndymgint<T>), dim3(1), dim3(64), 0, s, phi, L.g,
+                       (int)(L.subpos_h[0].size() / 3), (int)(L.subpos_h[1].size() / 3),
+                       L.subpos_d[0].p, L.subpos_d[1].p, L.subdels_d[0].p, L.subdels_d[1].p,
+                       b[0], b[1], b[2], b[3], b[4], b[5], skip);
+    WM_LAUNCHED();
+  }
+}
+
+// sorhalfpass3d (f3d_mgrid.F:2095-2291) on level data in red-black layout.
+template <typename T>
+static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
+                     const int *skip, hipStream_t s) {
+  T *phi = phi_of<T>(L).p;
+  const T *rho = rho_of<T>(L).p;
+  LoopRange r = sor_range(L, h->bounds);
+  double cst = P.mgparam * 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+  T dxsqic = (T)(L.dxsqi * cst), dysqic = (T)(L.dysqi * cst), dzsqic = (T)(L.dzsqi * cst);
+  T spm1 = (T)(1. - P.mgparam);
+  SubList<T> sub{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (P.lcnd) sub = cond_of<T>(L).sor[c].view();
+  int nbx = (int)cdiv(L.g.W, 64);
+  int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
+  if (nyr <= 0 || nzr <= 0) return;
+  int nby = (int)cdiv(nyr, 4);
+  int nbdense = nbx * nby * nzr;
+  int nbsub = (int)cdiv(sub.n, 256);
+  hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+                     L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
+                     dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
+                     nbdense, skip);
+  WM_LAUNCHED();
+  int usevolt = (level == 0 && P.mgform == 1);
+  pin_and_mirror<T>(h, L, level, P, phi, usevolt, skip, s);
+}
+
+template <typename T>
+static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
+                    hipStream_t s) {
+  int usevolt = (level == 0 && P.mgform == 1);
+  pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
+  for (int c = 0; c <= 1; c++) {
+    halfpass<T>(h, L, level, P, c, skip, s);
+    launch_bc<T>(phi_of<T>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+  }
+}
+
+// residual3d (f3d_mgrid.F:3367-3598) from red-black phi/rho into a natural
+// res array of type TR.
+template <typename T, typename TR>
+static void residual(wm_mg3d *h, Level &L, int level, const Params &P, const T *phi,
+                     const T *rho, TR *res, int mgform2init, const int *skip, hipStream_t s) {
+  LoopRange r = res_range(L, h->bounds);
+  double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+  T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
+    dzsqic = (T)(L.dzsqi * P.mgparam * cst);
+  long long n = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
+  hipLaunchKernelGGL((k_residual<T, TR>), dim3(blocks(n)), dim3(256), 0, s, phi, rho, res, L.g,
+                     r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic, dysqic,
+                     dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, skip);
+  WM_LAUNCHED();
+  if (L.hasmask) {
+    ensure_records<T>(h, L, level, P, true, mgform2init);
+    CondDev<T> &C = cond_of<T>(L);
+    T dxsqics = (T)(P.mgparam * L.dxsqi * cst), dysqics = (T)(P.mgparam * L.dysqi * cst),
+      dzsqics = (T)(P.mgparam * L.dzsqi * cst);
+    for (int p = 0; p < 2; p++) {
+      SubList<T> sub = C.res[mgform2init][p].view();
+      int nint = p == 0 ? L.nint : 0;
+      long long cnt = (long long)nint + sub.n;
+      if (cnt == 0) continue;
+      hipLaunchKernelGGL((k_residual_cond<T, TR>), dim3(blocks(cnt)), dim3(256), 0, s, phi,
+                         rho, res, L.g, nint, L.int_pos.p, sub, dxsqics, dysqics, dzsqics,
+                         (T)P.mgparam, skip);
+      WM_LAUNCHED();
+    }
+  }
+  launch_bc<TR>(res, NatIdx{NatGeom{L.nx, L.ny, L.nz}}, L.nx, L.ny, L.nz, h->bounds, 0, 0,
+                skip, s);
+}
+
+struct RhoIdx {
+  int nx, ny;
+  __device__ long long operator()(int ix, int iy, int iz) const {
+    return (long long)ix + (long long)(nx + 1) * ((long long)iy + (long long)(ny + 1) * iz);
+  }
+};
+
+template <typename TR, typename TC, typename OutIdx>
+static void restrict_to(Level &L, const TR *res, TC *out, OutIdx O, const int *skip,
+                        hipStream_t s, int nxc, int nyc, int nzc) {
+  RestrictAxis ax{L.rlo[0].p, L.rhi[0].p, L.rw[0].p, L.rcmin[0], L.rcmax[0]};
+  RestrictAxis ay{L.rlo[1].p, L.rhi[1].p, L.rw[1].p, L.rcmin[1], L.rcmax[1]};
+  RestrictAxis az{L.rlo[2].p, L.rhi[2].p, L.rw[2].p, L.rcmin[2], L.rcmax[2]};
+  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
+  if constexpr (std::is_same<TR, double>::value && std::is_same<TC, double>::value) {
+    hipLaunchKernelGGL((k_restrict<TR, TC, OutIdx>), dim3(blocks(n)), dim3(256), 0, s, res,
+                       NatGeom{L.nx, L.ny, L.nz}, out, O, nxc, nyc, nzc, ax, ay, az, L.ff, skip);
+  } else {
+    hipLaunchKernelGGL((k_restrict_t<TC, OutIdx>), dim3(blocks(n)), dim3(256), 0, s, res,
+                       NatGeom{L.nx, L.ny, L.nz}, out, O, nxc, nyc, nzc, ax, ay, az, (TC)L.ff,
+                       skip);
+  }
+  WM_LAUNCHED();
+}
+
+template <typename T, typename FI, typename CI>
+static void expand_from(Level &L, T *phi, FI F, const T *pc, CI Cc, const int *skip,
+                        hipStream_t s) {
+  ExpandAxis ax{L.ej[0].p, L.ew[0].p, L.elo[0], L.ehi[0]};
+  ExpandAxis ay{L.ej[1].p, L.ew[1].p, L.elo[1], L.ehi[1]};
+  ExpandAxis az{L.ej[2].p, L.ew[2].p, L.elo[2], L.ehi[2]};
+  long long n = (long long)std::max(0, ax.hi - ax.lo + 1) * std::max(0, ay.hi - ay.lo + 1) *
+                std::max(0, az.hi - az.lo + 1);
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_expand<T, FI, CI>), dim3(blocks(n)), dim3(256), 0, s, phi, F, pc, Cc,
+                     ax, ay, az, skip);
+  WM_LAUNCHED();
+}
+
+// vcycle (f3d_mgrid.F:1101-1591).
+template <typename T>
+static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s) {
+  Level &L = *h->lv[level];
+  for (int i = 0; i < P.down; i++) sorpass<T>(h, L, level, P, skip, s);
+  if (L.coarser) {
+    Level &Lc = *h->lv[level + 1];
+    residual<T, T>(h, L, level, P, phi_of<T>(L).p, rho_of<T>(L).p, res_of<T>(L).p, 0, skip, s);
+    restrict_to<T, T>(L, res_of<T>(L).p, rho_of<T>(Lc).p, RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny,
+                      Lc.nz);
+    hipLaunchKernelGGL((k_fill<T>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
+                       phi_of<T>(Lc).p, 2 * Lc.g.cs, (T)0., skip);
+    WM_LAUNCHED();
+    vcycle<T>(h, level + 1, P, skip, s);
+    expand_from<T>(L, phi_of<T>(L).p, RBIdx{L.g}, phi_of<T>(Lc).p, RBIdx{Lc.g}, skip, s);
+    launch_bc<T>(phi_of<T>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+  }
+  for (int i = 0; i < P.up; i++) sorpass<T>(h, L, level, P, skip, s);
+}
+
+template <typename T>
+static void prepare_levels(wm_mg3d *h, const Params &P) {
+  for (size_t l = 0; l < h->lv.size(); l++) {
+    Level &L = *h->lv[l];
+    ensure_buffers<T>(L, L.coarser);
+    build_mask(h, L, (int)l, P);
+    ensure_records<T>(h, L, (int)l, P, false, 0);
+    if (L.coarser && L.hasmask) ensure_records<T>(h, L, (int)l, P, true, 0);
+  }
+}
+
+static int nconv_blocks(const RBGeom &g) {
+  long long n = 2 * g.cs;
+  return (int)std::min<long long>(2048, std::max<long long>(1, cdiv(n, 256 * 8)));
+}
+
+// One iteration of the multigrid3dsolve loop (f3d_mgrid.F:313-434).
+static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int mgmaxiters,
+                      hipStream_t s) {
+  Level &L0 = *h->lv[0];
+  const int *skip = &h->state.p->done;
+  if (prec == WM_PREC_FP64 && P.mgform == 1) {
+    vcycle<double>(h, 0, P, skip, s);
+  } else {
+    // Residual-correction form: the error equation on the residual of the
+    // saved phi (mgform == 2), with the V-cycle in double or float.
+    Params Pinit = P;
+    Pinit.mgform = 2;
+    pin_and_mirror<double>(h, L0, 0, Pinit, h->save64.p, 1, skip, s);
+    residual<double, double>(h, L0, 0, Pinit, h->save64.p, h->rhosave64.p, L0.res64.p, 1,
+                             skip, s);
+    long long nr = (long long)(L0.nx + 1) * (L0.ny + 1) * (L0.nz + 1);
+    long long np = 2 * L0.g.cs;
+    if (prec == WM_PREC_FP64) {
+      hipLaunchKernelGGL((k_res_to_rho<double>), dim3(blocks(nr)), dim3(256), 0, s,
+                         L0.res64.p, L0.rho64.p, L0.g, skip);
+      WM_LAUNCHED();
+      hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p, np,
+                         0., skip);
+      WM_LAUNCHED();
+      vcycle<double>(h, 0, Pinit, skip, s);
+      // phi = phi + phisave (the error plus the saved phi)
+      hipLaunchKernelGGL((k_dc_add<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
+                         L0.phi64.p, h->save64.p, np, skip);
+      WM_LAUNCHED();
+    } else {
+      hipLaunchKernelGGL((k_res_to_rho<float>), dim3(blocks(nr)), dim3(256), 0, s,
+                         L0.res64.p, L0.rho32.p, L0.g, skip);
+      WM_LAUNCHED();
+      hipLaunchKernelGGL((k_fill<float>), dim3(blocks(np)), dim3(256), 0, s, L0.phi32.p, np,
+                         0.f, skip);
+      WM_LAUNCHED();
+      vcycle<float>(h, 0, Pinit, skip, s);
+      hipLaunchKernelGGL((k_dc_add<float>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
+                         L0.phi32.p, h->save64.p, np, skip);
+      WM_LAUNCHED();
+    }
+    launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, skip, s);
+  }
+  hipLaunchKernelGGL((k_converge<double>), dim3(nconv_blocks(L0.g)), dim3(256), 0, s,
+                     L0.phi64.p, h->save64.p, L0.g, h->state.p, mgtol, mgmaxiters, 1);
+  WM_LAUNCHED();
+}
+
+__global__ void k_state_reset(SolveState *st) {
+  st->errbits = 0ull;
+  st->arrived = 0u;
+  st->iters = 0;
+  st->done = 0;
+  st->err = 0.;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                   double dz, const int64_t *bounds, int64_t mgmaxlevels) {
+  WM_ENTRY({
+    WM_REQUIRE(out != nullptr, "wm_mg3d_create: out is NULL");
+    WM_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "wm_mg3d_create: need nx, ny, nz >= 2");
+    WM_REQUIRE(dx > 0 && dy > 0 && dz > 0, "wm_mg3d_create: cell sizes must be positive");
+    WM_REQUIRE((nx + 3) * (ny + 3) * (nz + 3) < (1ll << 31) * 8ll, "wm_mg3d_create: grid too large");
+    for (int i = 0; i < 6; i++)
+      WM_REQUIRE(bounds[i] >= 0 && bounds[i] <= 2,
+                 "wm_mg3d_create: bounds must be 0 (Dirichlet), 1 (Neumann) or 2 (periodic)");
+    for (int a = 0; a < 3; a++)
+      WM_REQUIRE((bounds[2 * a] == 2) == (bounds[2 * a + 1] == 2),
+                 "wm_mg3d_create: periodic bounds must be set on both sides of an axis");
+    std::unique_ptr<wm_mg3d> h(new wm_mg3d());
+    h->nx = nx;
+    h->ny = ny;
+    h->nz = nz;
+    h->dx = dx;
+    h->dy = dy;
+    h->dz = dz;
+    for (int i = 0; i < 6; i++) h->bounds[i] = (int)bounds[i];
+    // Levels: getmglevels (f3d_mgrid.F:3654-3845), capped as vcycle does.
+    int64_t lnx = nx, lny = ny, lnz = nz;
+    double ldx = dx, ldy = dy, ldz = dz;
+    for (int level = 0;; level++) {
+      std::unique_ptr<Level> L(new Level());
+      L->nx = (int)lnx;
+      L->ny = (int)lny;
+      L->nz = (int)lnz;
+      L->dx = ldx;
+      L->dy = ldy;
+      L->dz = ldz;
+      L->dxsqi = 1. / (ldx * ldx);
+      L->dysqi = 1. / (ldy * ldy);
+      L->dzsqi = 1. / (ldz * ldz);
+      L->g = make_geom((int)lnx, (int)lny, (int)lnz);
+      bool more = (lnx >= 4 && lny >= 4 && lnz >= 4 && level < mgmaxlevels && level < 100);
+      h->lv.push_back(std::move(L));
+      if (!more) break;
+      int64_t nxc, nyc, nzc;
+      double dxc, dyc, dzc;
+      next_coarse(lnx, lny, lnz, ldx, ldy, ldz, nxc, nyc, nzc, dxc, dyc, dzc);
+      Level &F = *h->lv.back();
+      F.coarser = true;
+      F.ff = (F.dxsqi + F.dysqi + F.dzsqi) /
+             (1. / (dxc * dxc) + 1. / (dyc * dyc) + 1. / (dzc * dzc));
+      restrict_axis((int)lnx, (int)nxc, h->bounds[0], h->bounds[1], F, 0);
+      restrict_axis((int)lny, (int)nyc, h->bounds[2], h->bounds[3], F, 1);
+      restrict_axis((int)lnz, (int)nzc, h->bounds[4], h->bounds[5], F, 2);
+      expand_axis((int)lnx, (int)nxc, h->bounds[0], h->bounds[1], F, 0);
+      expand_axis((int)lny, (int)nyc, h->bounds[2], h->bounds[3], F, 1);
+      expand_axis((int)lnz, (int)nzc, h->bounds[4], h->bounds[5], F, 2);
+      lnx = nxc;
+      lny = nyc;
+      lnz = nzc;
+      ldx = dxc;
+      ldy = dyc;
+      ldz = dzc;
+    }
+    h->state.alloc(1);
+    WM_HIP(hipHostMalloc((void **)&h->state_h, sizeof(SolveState), hipHostMallocDefault));
+    WM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    WM_HIP(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+    WM_HIP(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
+    *out = h.release();
+  });
+}
+
+int wm_mg3d_destroy(wm_mg3d *h) {
+  WM_ENTRY({
+    if (h) {
+      (void)hipDeviceSynchronize();
+      delete h;
+    }
+  });
+}
+
+int wm_mg3d_nlevels(const wm_mg3d *h, int64_t *n) {
+  WM_ENTRY({
+    WM_REQUIRE(h && n, "wm_mg3d_nlevels: NULL argument");
+    *n = (int64_t)h->lv.size();
+  });
+}
+
+int wm_mg3d_level_dims(const wm_mg3d *h, int64_t level, int64_t *nxyz, double *dxyz) {
+  WM_ENTRY({
+    WM_REQUIRE(h && level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_level_dims: bad level");
+    const Level &L = *h->lv[level];
+    if (nxyz) {
+      nxyz[0] = L.nx;
+      nxyz[1] = L.ny;
+      nxyz[2] = L.nz;
+    }
+    if (dxyz) {
+      dxyz[0] = L.dx;
+      dxyz[1] = L.dy;
+      dxyz[2] = L.dz;
+    }
+  });
+}
+
+int wm_mg3d_set_conductors(wm_mg3d *h, const wm_conductor_lists *c) {
+  WM_ENTRY({
+    WM_REQUIRE(h != nullptr, "wm_mg3d_set_conductors: NULL handle");
+    for (size_t l = 0; l < h->lv.size(); l++) {
+      Level &L = *h->lv[l];
+      L.nint = 0;
+      L.int_pos.release();
+      L.int_volt.release();
+      for (int p = 0; p < 2; p++) {
+        L.subpos_h[p].clear();
+        L.subdels_h[p].clear();
+        L.subvolt_h[p].clear();
+        L.subpos_d[p].release();
+        L.subdels_d[p].release();
+      }
+      L.neumann = false;
+      L.hasmask = false;
+      L.mask.release();
+      L.c64.reset();
+      L.c32.reset();
+    }
+    h->conductors_set = false;
+    if (h->gexec) {
+      (void)hipGraphExecDestroy(h->gexec);
+      h->gexec = nullptr;
+      h->gkey = wm_mg3d::GKey();
+    }
+    if (c == nullptr) return 0;
+    WM_REQUIRE(c->levels >= (int64_t)h->lv.size() || (c->int_n == 0 && c->sub_n[0] == 0 && c->sub_n[1] == 0),
+               "wm_mg3d_set_conductors: conductor data has fewer levels than the solver");
+    for (size_t l = 0; l < h->lv.size(); l++) {
+      Level &L = *h->lv[l];
+      if (c->int_n > 0) {
+        int64_t a = c->int_istart[l], b = c->int_istart[l + 1];
+        std::vector<int> pos;
+        std::vector<double> volt;
+        for (int64_t i = a; i < b; i++) {
+          int ix = (int)c->int_indx[3 * i], iy = (int)c->int_indx[3 * i + 1],
+              iz = (int)c->int_indx[3 * i + 2];
+          WM_REQUIRE(ix >= 0 && ix <= L.nx && iy >= 0 && iy <= L.ny && iz >= 0 && iz <= L.nz,
+                     "wm_mg3d_set_conductors: interior point outside the level's grid");
+          pos.insert(pos.end(), {ix, iy, iz});
+          volt.push_back(c->int_volt[i]);
+        }
+        L.nint = (int)volt.size();
+        L.int_pos.upload(pos);
+        L.int_volt.upload(volt);
+      }
+      for (int p = 0; p < 2; p++) {
+        if (c->sub_n[p] == 0) continue;
+        int64_t a = c->sub_istart[p][l], b = c->sub_istart[p][l + 1];
+        for (int64_t i = a; i < b; i++) {
+          int ix = (int)c->sub_indx[p][3 * i], iy = (int)c->sub_indx[p][3 * i + 1],
+              iz = (int)c->sub_indx[p][3 * i + 2];
+          WM_REQUIRE(ix >= 0 && ix <= L.nx && iy >= 0 && iy <= L.ny && iz >= 0 && iz <= L.nz,
+                     "wm_mg3d_set_conductors: subgrid point outside the level's grid");
+          WM_REQUIRE(((ix + iy + iz) & 1) == p,
+                     "wm_mg3d_set_conductors: subgrid point in the wrong parity list");
+          L.subpos_h[p].insert(L.subpos_h[p].end(), {ix, iy, iz});
+          for (int k = 0; k < 6; k++) {
+            double d = c->sub_dels[p][6 * i + k];
+            L.subdels_h[p].push_back(d);
+            L.subvolt_h[p].push_back(c->sub_volt[p][6 * i + k]);
+            if (0. >= d && d > -1.) L.neumann = true;
+          }
+        }
+        L.subpos_d[p].upload(L.subpos_h[p]);
+        L.subdels_d[p].upload(L.subdels_h[p]);
+      }
+    }
+    h->conductors_set = true;
+  });
+}
+
+int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t mgform,
+                  int64_t mgmaxiters, double mgtol, int64_t downpasses, int64_t uppasses,
+                  int64_t lcndbndy, int64_t icndbndy, int64_t precision, int64_t *mgiters,
+                  double *mgerror, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(h && phi && rho && mgiters && mgerror, "wm_mg3d_solve: NULL argument");
+    WM_REQUIRE(mgform == 1 || mgform == 2, "wm_mg3d_solve: mgform must be 1 or 2");
+    WM_REQUIRE(icndbndy == 1 || icndbndy == 2, "wm_mg3d_solve: icndbndy must be 1 or 2");
+    WM_REQUIRE(precision == WM_PREC_FP64 || precision == WM_PREC_FP32_DC ||
+                   precision == WM_PREC_BF16_DC,
+               "wm_mg3d_solve: unknown precision");
+    WM_REQUIRE(downpasses >= 0 && uppasses >= 0, "wm_mg3d_solve: negative pass count");
+    int prec = (int)precision;
+    if (prec == WM_PREC_BF16_DC) prec = WM_PREC_FP32_DC;
+    hipStream_t us = wm_stream(stream);
+    hipStream_t s = h->s;
+    Params P{mgparam, (int)mgform, (int)downpasses, (int)uppasses, (int)lcndbndy, (int)icndbndy};
+    Level &L0 = *h->lv[0];
+    double eps0 = wm_eps0();
+    double reps0c = mgparam / (eps0 * 2. * (L0.dxsqi + L0.dysqi + L0.dzsqi));
+    double rinv = 1. / reps0c;
+    // Buffers and conductor records.
+    prepare_levels<double>(h, P);
+    bool dc = !(prec == WM_PREC_FP64 && P.mgform == 1);
+    if (prec == WM_PREC_FP32_DC) prepare_levels<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd});
+    if (dc) {
+      L0.res64.alloc((size_t)(L0.nx + 3) * (L0.ny + 3) * (L0.nz + 3));
+      h->rhosave64.alloc(2 * L0.g.cs);
+      Params P2 = P;
+      P2.mgform = 2;
+      if (prec == WM_PREC_FP64) prepare_levels<double>(h, P2);
+      ensure_records<double>(h, L0, 0, P2, true, 1);
+    }
+    h->save64.alloc(2 * L0.g.cs);
+    // Inputs.
+    WM_HIP(hipEventRecord(h->ev_in, us));
+    WM_HIP(hipStreamWaitEvent(s, h->ev_in, 0));
+    long long nr = (long long)(h->nx + 1) * (h->ny + 1) * (h->nz + 1);
+    long long nn = (long long)(h->nx + 3) * (h->ny + 3) * (h->nz + 3);
+    long long np = 2 * L0.g.cs;
+    double *rho0 = dc ? h->rhosave64.p : L0.rho64.p;
+    hipLaunchKernelGGL(k_rho_in, dim3(blocks(nr)), dim3(256), 0, s, rho, rho0, L0.g, reps0c, rinv);
+    WM_LAUNCHED();
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L0.phi64.p,
+                       L0.g);
+    WM_LAUNCHED();
+    launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr, s);
+    hipLaunchKernelGGL((k_copy<double, double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
+                       h->save64.p, np, nullptr);
+    WM_LAUNCHED();
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, s, h->state.p);
+    WM_LAUNCHED();
+    int iters = 0;
+    double err = 2. * mgtol + 1.;
+    static const bool eager = getenv("WARPMI_EAGER") != nullptr;
+    if (mgmaxiters > 0 && eager) {
+      // Plain stream launches, one V-cycle at a time.
+      for (int k = 0; k < mgmaxiters; k++) {
+        iteration(h, prec, P, mgtol, (int)mgmaxiters, s);
+        WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
+        WM_HIP(hipStreamSynchronize(s));
+        if (h->state_h->done) break;
+      }
+      iters = h->state_h->iters;
+      err = h->state_h->err;
+      h->last_iters = iters;
+    } else if (mgmaxiters > 0) {
+      // Capture one iteration as a graph (rebuilt when the parameters change).
+      wm_mg3d::GKey key;
+      key.prec = prec;
+      key.mgform = (int)mgform;
+      key.down = (int)downpasses;
+      key.up = (int)uppasses;
+      key.lcnd = (int)lcndbndy;
+      key.icnd = (int)icndbndy;
+      key.maxit = (int)mgmaxiters;
+      key.mgparam = mgparam;
+      key.mgtol = mgtol;
+      if (!h->gexec || !(h->gkey == key)) {
+        if (h->gexec) {
+          WM_HIP(hipGraphExecDestroy(h->gexec));
+          h->gexec = nullptr;
+        }
+        hipGraph_t graph;
+        WM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          iteration(h, prec, P, mgtol, (int)mgmaxiters, s);
+        } catch (...) {
+          hipGraph_t g2;
+          (void)hipStreamEndCapture(s, &g2);
+          if (g2) (void)hipGraphDestroy(g2);
+          throw;
+        }
+        WM_HIP(hipStreamEndCapture(s, &graph));
+        WM_HIP(hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0));
+        WM_HIP(hipGraphDestroy(graph));
+        h->gkey = key;
+      }
+      int batch = std::max(1, std::min<int>(h->last_iters > 0 ? h->last_iters : 4, (int)mgmaxiters));
+      int launched = 0;
+      for (;;) {
+        for (int b = 0; b < batch && launched < mgmaxiters; b++, launched++)
+          WM_HIP(hipGraphLaunch(h->gexec, s));
+        WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
+        WM_HIP(hipStreamSynchronize(s));
+        if (h->state_h->done || launched >= mgmaxiters) break;
+        batch = 1;
+      }
+      iters = h->state_h->iters;
+      err = h->state_h->err;
+      h->last_iters = iters;
+    }
+    // Outputs: phi in natural layout, Dirichlet guards filled
+    // (applyboundaryconditions3d with lwithdirichlet, f3d_mgrid.F:441-444).
+    hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L0.phi64.p, phi,
+                       L0.g);
+    WM_LAUNCHED();
+    launch_bc<double>(phi, NatIdx{NatGeom{L0.nx, L0.ny, L0.nz}}, L0.nx, L0.ny, L0.nz, h->bounds,
+                      1, 0, nullptr, s);
+    WM_HIP(hipEventRecord(h->ev_out, s));
+    WM_HIP(hipStreamWaitEvent(us, h->ev_out, 0));
+    WM_HIP(hipStreamSynchronize(s));
+    *mgiters = iters;
+    *mgerror = err;
+  });
+}
+
+int wm_mg3d_sorhalfpass(wm_mg3d *h, int64_t level, int64_t parity, double *phi, const double *rho,
+                        double mgparam, int64_t mgform, int64_t lcndbndy, int64_t icndbndy,
+                        void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(h && phi && rho, "wm_mg3d_sorhalfpass: NULL argument");
+    WM_REQUIRE(level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_sorhalfpass: bad level");
+    WM_REQUIRE(parity == 0 || parity == 1, "wm_mg3d_sorhalfpass: parity must be 0 or 1");
+    hipStream_t s = wm_stream(stream);
+    Level &L = *h->lv[level];
+    Params P{mgparam, (int)mgform, 1, 1, (int)lcndbndy, (int)icndbndy};
+    ensure_buffers<double>(L, L.coarser);
+    build_mask(h, L, (int)level, P);
+    ensure_records<double>(h, L, (int)level, P, false, 0);
+    long long nn = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
+    long long nr = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
+    hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
+    WM_LAUNCHED();
+    halfpass<double>(h, L, (int)level, P, (int)parity, nullptr, s);
+    hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L.phi64.p, phi, L.g);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_mg3d_sorpass(wm_mg3d *h, int64_t level, double *phi, const double *rho, double mgparam,
+                    int64_t mgform, int64_t lcndbndy, int64_t icndbndy, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(h && phi && rho, "wm_mg3d_sorpass: NULL argument");
+    WM_REQUIRE(level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_sorpass: bad level");
+    hipStream_t s = wm_stream(stream);
+    Level &L = *h->lv[level];
+    Params P{mgparam, (int)mgform, 1, 1, (int)lcndbndy, (int)icndbndy};
+    ensure_buffers<double>(L, L.coarser);
+    build_mask(h, L, (int)level, P);
+    ensure_records<double>(h, L, (int)level, P, false, 0);
+    long long nn = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
+    long long nr = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
+    hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
+    WM_LAUNCHED();
+    sorpass<double>(h, L, (int)level, P, nullptr, s);
+    hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L.phi64.p, phi, L.g);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_mg3d_residual(wm_mg3d *h, int64_t level, const double *phi, const double *rho, double *res,
+                     double mgparam, int64_t mgform, int64_t mgform2init, int64_t lcndbndy,
+                     int64_t icndbndy, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(h && phi && rho && res, "wm_mg3d_residual: NULL argument");
+    WM_REQUIRE(level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_residual: bad level");
+    hipStream_t s = wm_stream(stream);
+    Level &L = *h->lv[level];
+    Params P{mgparam, (int)mgform, 1, 1, (int)lcndbndy, (int)icndbndy};
+    ensure_buffers<double>(L, L.coarser);
+    build_mask(h, L, (int)level, P);
+    long long nn = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
+    long long nr = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
+    hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
+    WM_LAUNCHED();
+    residual<double, double>(h, L, (int)level, P, L.phi64.p, L.rho64.p, res, mgform2init ? 1 : 0,
+                             nullptr, s);
+  });
+}
+
+int wm_mg3d_restrict(wm_mg3d *h, int64_t level, const double *res, double *rhocoarse,
+                     void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(h && res && rhocoarse, "wm_mg3d_restrict: NULL argument");
+    WM_REQUIRE(level >= 0 && level + 1 < (int64_t)h->lv.size(), "wm_mg3d_restrict: no coarser level");
+    Level &L = *h->lv[level];
+    Level &Lc = *h->lv[level + 1];
+    restrict_to<double, double>(L, res, rhocoarse, RhoIdx{Lc.nx, Lc.ny}, nullptr,
+                                wm_stream(stream), Lc.nx, Lc.ny, Lc.nz);
+  });
+}
+
+int wm_mg3d_expand(wm_mg3d *h, int64_t level, double *phi, const double *phicoarse, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(h && phi && phicoarse, "wm_mg3d_expand: NULL argument");
+    WM_REQUIRE(level >= 0 && level + 1 < (int64_t)h->lv.size(), "wm_mg3d_expand: no coarser level");
+    Level &L = *h->lv[level];
+    Level &Lc = *h->lv[level + 1];
+    expand_from<double>(L, phi, NatIdx{NatGeom{L.nx, L.ny, L.nz}}, phicoarse,
+                        NatIdx{NatGeom{Lc.nx, Lc.ny, Lc.nz}}, nullptr, wm_stream(stream));
+  });
+}
+
+int wm_applyboundaryconditions3d(int64_t nx, int64_t ny, int64_t nz, double *u,
+                                 const int64_t *bounds, int64_t lwithdirichlet,
+                                 int64_t lzerodirichlet, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(u && bounds, "wm_applyboundaryconditions3d: NULL argument");
+    int b[6];
+    for (int i = 0; i < 6; i++) b[i] = (int)bounds[i];
+    launch_bc<double>(u, NatIdx{NatGeom{(int)nx, (int)ny, (int)nz}}, (int)nx, (int)ny, (int)nz, b,
+                      (int)lwithdirichlet, (int)lzerodirichlet, nullptr, wm_stream(stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,658 @@
+// Device kernels of the multigrid solver (pywarp90/f3d_mgrid.F), for
+// gfx950.  Included by mg3d.hip only.
+//
+// Layout.  Each level's phi and rho live in a red-black split layout: the
+// padded (-1:nx+1,-1:ny+1,-1:nz+1) array is split by the colour
+// c = (ix+iy+iz) mod 2 into two arrays of rows of W = (nx+4)/2 elements.
+// With px = ix+1 (likewise py, pz) the element of a point is
+//   c*cs + (px>>1) + W*(py + PY*pz).
+// A half sweep of colour c then streams one array (read+write) and reads
+// the other: every cache line moved carries only useful data, so the
+// HBM traffic of a half sweep is the algorithmic 2 scalars per node.
+// Residuals stay in the reference's natural layout with one guard cell.
+//
+// Every kernel takes `skip`: when *skip != 0 the kernel returns at once.
+// The solver sets it when the convergence test is met so that V-cycles
+// enqueued ahead of the test become no-ops.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct RBGeom {
+  int nx, ny, nz;  // cells of the level
+  int W, PY, PZ;   // elements per half row, padded rows and planes
+  long long cs;    // elements per colour array
+};
+
+__host__ __device__ inline long long rb_index(const RBGeom &g, int ix, int iy,
+                                              int iz) {
+  int px = ix + 1, py = iy + 1, pz = iz + 1;
+  int c = (px + py + pz + 1) & 1;
+  return c * g.cs + (px >> 1) + (long long)g.W * (py + (long long)g.PY * pz);
+}
+
+struct NatGeom {
+  int nx, ny, nz;
+};
+
+__host__ __device__ inline long long nat_index(const NatGeom &g, int ix, int iy,
+                                               int iz) {
+  return (long long)(ix + 1) +
+         (long long)(g.nx + 3) * ((long long)(iy + 1) + (long long)(g.ny + 3) * (iz + 1));
+}
+
+// Accessors so that the same kernel body serves both layouts.
+struct RBIdx {
+  RBGeom g;
+  __device__ long long operator()(int ix, int iy, int iz) const {
+    return rb_index(g, ix, iy, iz);
+  }
+};
+struct NatIdx {
+  NatGeom g;
+  __device__ long long operator()(int ix, int iy, int iz) const {
+    return nat_index(g, ix, iy, iz);
+  }
+};
+
+#define WM_SKIP() \
+  if (skip && *skip) return
+
+// Side encodings of the subgrid-conductor stencil, one per neighbour
+// (order mx,px,my,py,mz,pz), 2 bits each: the neighbour value is used raw,
+// multiplied by k, divided by k, or replaced by the constant k.  This is
+// the set of forms handlesubgrid2 (f3d_mgrid.F:2628-2764) and condbndymg
+// icndbndy=1 (2846-2895) produce; the constants are computed on the host
+// with the reference's expressions.
+enum { SIDE_RAW = 0, SIDE_MUL = 1, SIDE_DIV = 2, SIDE_CONST = 3 };
+
+template <typename T>
+__device__ inline T side_value(uint32_t mode, T k, T raw) {
+  if (mode == SIDE_RAW) return raw;
+  if (mode == SIDE_MUL) return raw * k;
+  if (mode == SIDE_DIV) return raw / k;
+  return k;
+}
+
+// One colour's active subgrid points on one level (SoA, device).
+template <typename T>
+struct SubList {
+  int n;
+  const int *pos;        // (n,3) natural ix,iy,iz
+  const uint32_t *modes; // 2 bits per side
+  const T *k;            // (n,6)
+  const T *denom;        // (n)
+  const T *fac;          // (n) residual factor ppp (residual lists only)
+  const uint32_t *zero;  // (n) 6-bit mask of neighbours whose residual is zeroed
+};
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8
+// XCDs round robin, so block b runs on XCD b%8.  Remap so that each XCD
+// walks a contiguous range of tiles and its L2 holds the neighbouring
+// planes it needs.
+__device__ inline int xcd_remap(int b, int nb) {
+  int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// SOR half sweep (sorhalfpass3d, f3d_mgrid.F:2229-2243) fused with the
+// subgrid-conductor update of the same colour (condbndymg 2766-2980).
+// Dense part: one thread per colour-c point, 64 x 4 tiles.  Points in the
+// conductor mask are left to the sparse part, which reads the point's value
+// before this sweep (the reference's prevphi) and other-colour neighbours,
+// neither of which the dense part writes.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_sor_halfpass(
+    T *__restrict__ phi, const T *__restrict__ rho, RBGeom g, int c, int ixmin,
+    int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
+    T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
+    int nbx, int nby, int nbdense, const int *__restrict__ skip) {
+  WM_SKIP();
+  int b = blockIdx.x;
+  if (b < nbdense) {
+    int lb = xcd_remap(b, nbdense);
+    int bx = lb % nbx;
+    int t = lb / nbx;
+    int by = t % nby;
+    int bz = t / nby;
+    int j = bx * 64 + (threadIdx.x & 63);
+    int iy = iymin + by * 4 + (threadIdx.x >> 6);
+    int iz = izmin + bz;
+    if (iy > iymax || iz > izmax) return;
+    int py = iy + 1, pz = iz + 1;
+    int p0 = (c + py + pz + 1) & 1;
+    int px = p0 + 2 * j;
+    int ix = px - 1;
+    if (ix < ixmin || ix > ixmax) return;
+    long long row = (long long)g.W * (py + (long long)g.PY * pz);
+    long long e = row + j;
+    long long ce = c * g.cs + e;
+    if (mask && ((mask[ce >> 5] >> (ce & 31)) & 1u)) return;
+    const T *o = phi + (1 - c) * g.cs;
+    T pxm = o[row + ((px - 1) >> 1)];
+    T pxp = o[row + ((px + 1) >> 1)];
+    T pym = o[e - g.W];
+    T pyp = o[e + g.W];
+    long long pl = (long long)g.W * g.PY;
+    T pzm = o[e - pl];
+    T pzp = o[e + pl];
+    T own = phi[ce];
+    phi[ce] = rho[ce] + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
+              (pzm + pzp) * dzsqic + spm1 * own;
+    return;
+  }
+  int i = (b - nbdense) * 256 + threadIdx.x;
+  if (i >= sub.n) return;
+  int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
+  uint32_t m = sub.modes[i];
+  const T *k = sub.k + 6 * i;
+  T pxm = side_value<T>(m & 3u, k[0], phi[rb_index(g, ix - 1, iy, iz)]);
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], phi[rb_index(g, ix + 1, iy, iz)]);
+  T pym = side_value<T>((m >> 4) & 3u, k[2], phi[rb_index(g, ix, iy - 1, iz)]);
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], phi[rb_index(g, ix, iy + 1, iz)]);
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], phi[rb_index(g, ix, iy, iz - 1)]);
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], phi[rb_index(g, ix, iy, iz + 1)]);
+  long long ce = rb_index(g, ix, iy, iz);
+  T prev = phi[ce];
+  phi[ce] = (rho[ce] + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
+             (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev;
+}
+
+// cond_potmg (f3d_mgrid.F:2426-2472): pin interior conductor points.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pin(T *__restrict__ phi, RBGeom g, int n,
+                                             const int *__restrict__ pos,
+                                             const double *__restrict__ volt,
+                                             int usevolt,
+                                             const int *__restrict__ skip) {
+  WM_SKIP();
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  long long e = rb_index(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  phi[e] = usevolt ? (T)volt[i] : (T)0.;
+}
+
+// condbndymgint (f3d_mgrid.F:2982-3092): Neumann subgrid mirrors.  The
+// reference walks the even then the odd list in order and later writes win,
+// so this runs as one sequential thread.  It is launched only when a
+// conductor has Neumann surfaces (no configuration in use has any).
+template <typename T>
+__global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
+                                const int *__restrict__ pos0,
+                                const int *__restrict__ pos1,
+                                const double *__restrict__ dels0,
+                                const double *__restrict__ dels1, int lb0,
+                                int lb1, int lb2, int lb3, int lb4, int lb5,
+                                const int *__restrict__ skip) {
+  WM_SKIP();
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int nx = g.nx, ny = g.ny, nz = g.nz;
+  for (int p = 0; p < 2; p++) {
+    int n = p ? n1 : n0;
+    const int *pos = p ? pos1 : pos0;
+    const double *dels = p ? dels1 : dels0;
+    for (int ic = 0; ic < n; ic++) {
+      int ix = pos[3 * ic], iy = pos[3 * ic + 1], iz = pos[3 * ic + 2];
+      const double *d = dels + 6 * ic;
+      T v = phi[rb_index(g, ix, iy, iz)];
+      if (!((ix - 1 == 0 && lb0 == 0) || (ix + 1 == nx && lb1 == 0) ||
+            (ix == 0 && lb0 < 1) || (ix == nx && lb1 < 1))) {
+        if (0. >= d[0] && d[0] > -1.) phi[rb_index(g, ix - 1, iy, iz)] = v;
+        if (0. >= d[1] && d[1] > -1.) phi[rb_index(g, ix + 1, iy, iz)] = v;
+      }
+      if (!((iy - 1 == 0 && lb2 == 0) || (iy + 1 == ny && lb3 == 0) ||
+            (iy == 0 && lb2 < 1) || (iy == ny && lb3 < 1))) {
+        if (0. >= d[2] && d[2] > -1.) phi[rb_index(g, ix, iy - 1, iz)] = v;
+        if (0. >= d[3] && d[3] > -1.) phi[rb_index(g, ix, iy + 1, iz)] = v;
+      }
+      if (!((iz - 1 == 0 && lb4 == 0) || (iz + 1 == nz && lb5 == 0) ||
+            (iz == 0 && lb4 < 1) || (iz == nz && lb5 < 1))) {
+        if (0. >= d[4] && d[4] > -1.) phi[rb_index(g, ix, iy, iz - 1)] = v;
+        if (0. >= d[5] && d[5] > -1.) phi[rb_index(g, ix, iy, iz + 1)] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Residual (residual3d, f3d_mgrid.F:3367-3598): one thread per element of
+// the natural res array including its guard cells.  Points outside the loop
+// range get 0 (the reference zeroes Dirichlet faces and guards; Neumann and
+// periodic guards are filled afterwards by the boundary kernel).  Points in
+// the conductor mask are written by k_residual_cond.
+// ---------------------------------------------------------------------------
+template <typename T, typename TR>
+__global__ __launch_bounds__(256) void k_residual(
+    const T *__restrict__ phi, const T *__restrict__ rho, TR *__restrict__ res,
+    RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax,
+    T dxsqic, T dysqic, T dzsqic, T mgparam, const uint32_t *__restrict__ mask,
+    const int *__restrict__ skip) {
+  WM_SKIP();
+  int sx = g.nx + 3, sy = g.ny + 3;
+  long long n = (long long)sx * sy * (g.nz + 3);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % sx) - 1;
+  long long u = t / sx;
+  int iy = (int)(u % sy) - 1;
+  int iz = (int)(u / sy) - 1;
+  if (ix < ixmin || ix > ixmax || iy < iymin || iy > iymax || iz < izmin ||
+      iz > izmax) {
+    res[t] = (TR)0.;
+    return;
+  }
+  long long ce = rb_index(g, ix, iy, iz);
+  if (mask && ((mask[ce >> 5] >> (ce & 31)) & 1u)) return;
+  T r = rho[ce] +
+        (phi[rb_index(g, ix - 1, iy, iz)] + phi[rb_index(g, ix + 1, iy, iz)]) * dxsqic +
+        (phi[rb_index(g, ix, iy - 1, iz)] + phi[rb_index(g, ix, iy + 1, iz)]) * dysqic +
+        (phi[rb_index(g, ix, iy, iz - 1)] + phi[rb_index(g, ix, iy, iz + 1)]) * dzsqic -
+        phi[ce] * mgparam;
+  res[t] = (TR)r;
+}
+
+// cond_potmgres + condbndyres (f3d_mgrid.F:2474-2506, 3094-3365): interior
+// points get 0, active subgrid points the conductor form.  With Neumann
+// surfaces the reference zeroes some neighbours' residuals while walking
+// the even list then the odd list; launching one parity per call (even
+// first) reproduces that order.
+template <typename T, typename TR>
+__global__ __launch_bounds__(256) void k_residual_cond(
+    const T *__restrict__ phi, const T *__restrict__ rho, TR *__restrict__ res,
+    RBGeom g, int nint, const int *__restrict__ ipos, SubList<T> sub,
+    T dxsqics, T dysqics, T dzsqics, T mgparam, const int *__restrict__ skip) {
+  WM_SKIP();
+  NatGeom ng{g.nx, g.ny, g.nz};
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nint) {
+    res[nat_index(ng, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2])] = (TR)0.;
+    return;
+  }
+  i -= nint;
+  if (i >= sub.n) return;
+  int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
+  uint32_t m = sub.modes[i];
+  const T *k = sub.k + 6 * i;
+  T pxm = side_value<T>(m & 3u, k[0], phi[rb_index(g, ix - 1, iy, iz)]);
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], phi[rb_index(g, ix + 1, iy, iz)]);
+  T pym = side_value<T>((m >> 4) & 3u, k[2], phi[rb_index(g, ix, iy - 1, iz)]);
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], phi[rb_index(g, ix, iy + 1, iz)]);
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], phi[rb_index(g, ix, iy, iz - 1)]);
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], phi[rb_index(g, ix, iy, iz + 1)]);
+  long long ce = rb_index(g, ix, iy, iz);
+  T r = sub.fac[i] * (rho[ce] + (pxm + pxp) * dxsqics + (pym + pyp) * dysqics +
+                      (pzm + pzp) * dzsqics - phi[ce] * mgparam * sub.denom[i]);
+  res[nat_index(ng, ix, iy, iz)] = (TR)r;
+  uint32_t z = sub.zero[i];
+  if (z) {
+    if (z & 1u) res[nat_index(ng, ix - 1, iy, iz)] = (TR)0.;
+    if (z & 2u) res[nat_index(ng, ix + 1, iy, iz)] = (TR)0.;
+    if (z & 4u) res[nat_index(ng, ix, iy - 1, iz)] = (TR)0.;
+    if (z & 8u) res[nat_index(ng, ix, iy + 1, iz)] = (TR)0.;
+    if (z & 16u) res[nat_index(ng, ix, iy, iz - 1)] = (TR)0.;
+    if (z & 32u) res[nat_index(ng, ix, iy, iz + 1)] = (TR)0.;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// applyboundaryconditions3d (f3d_mgrid.F:3847-3955), one guard cell, done
+// as three launches (x, then y, then z) because the y and z parts read guard
+// values the earlier parts wrote.  Each thread owns one grid line normal to
+// the face, so the order of the reference's statements is kept per line.
+// ---------------------------------------------------------------------------
+template <typename T, typename Idx>
+__global__ void k_bc_axis(T *__restrict__ u, Idx I, int axis, int nx, int ny,
+                          int nz, int bm, int bp, int lwithdirichlet,
+                          int lzerodirichlet, const int *__restrict__ skip) {
+  WM_SKIP();
+  // The line is parametrised by (a, b) over the two other axes, each
+  // covering -1..n+1; ranges the reference does not touch are filtered.
+  int na, nb, n;
+  if (axis == 0) { na = ny; nb = nz; n = nx; }
+  else if (axis == 1) { na = nx; nb = nz; n = ny; }
+  else { na = nx; nb = ny; n = nz; }
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(na + 3) * (nb + 3)) return;
+  int a = (int)(t % (na + 3)) - 1;
+  int b = (int)(t / (na + 3)) - 1;
+  auto at = [&](int i) -> T & {
+    if (axis == 0) return u[I(i, a, b)];
+    if (axis == 1) return u[I(a, i, b)];
+    return u[I(a, b, i)];
+  };
+  // The extrapolation and mirror/periodic statements cover the interior
+  // range of the other axes in x (0:ny,0:nz), all of x and 0:nz in y, and
+  // everything in z; zeroing covers everything.
+  bool core;
+  if (axis == 0) core = (a >= 0 && a <= ny && b >= 0 && b <= nz);
+  else if (axis == 1) core = (b >= 0 && b <= nz);
+  else core = true;
+  if (lwithdirichlet) {
+    if (core) {
+      if (bm == 0) at(-1) = (T)2. * at(0) - at(1);
+      if (bp == 0) at(n + 1) = (T)2. * at(n) - at(n - 1);
+    }
+  } else if (lzerodirichlet) {
+    if (bm == 0) at(-1) = (T)0.;
+    if (bp == 0) at(n + 1) = (T)0.;
+  }
+  if (!core) return;
+  if (bm == 1) at(-1) = at(1);
+  if (bp == 1) at(n + 1) = at(n - 1);
+  if (bm == 2 && bp == 2) {
+    at(-1) = at(n - 1);
+    T v0 = at(0), v1 = at(1);
+    at(n) = v0;
+    at(n + 1) = v1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// restrict3d (f3d_mgrid.F:1593-1775): one thread per coarse node.  The
+// per-axis window limits and tent weights are the reference's precomputed
+// tables (wxa, ixmina, ixmaxa), built on the host.
+// ---------------------------------------------------------------------------
+struct RestrictAxis {
+  const int *lo, *hi;  // clipped window per coarse index
+  const double *w;     // (nc+1, 5) weights indexed from the unclipped start
+  int cmin, cmax;      // computed coarse range
+};
+
+template <typename TR, typename TC, typename OutIdx>
+__global__ __launch_bounds__(256) void k_restrict(
+    const TR *__restrict__ res, NatGeom fine, TC *__restrict__ out, OutIdx O,
+    int nxc, int nyc, int nzc, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
+    double ff, const int *__restrict__ skip) {
+  WM_SKIP();
+  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ixc = (int)(t % (nxc + 1));
+  long long u = t / (nxc + 1);
+  int iyc = (int)(u % (nyc + 1));
+  int izc = (int)(u / (nyc + 1));
+  long long oi = O(ixc, iyc, izc);
+  if (ixc < ax.cmin || ixc > ax.cmax || iyc < ay.cmin || iyc > ay.cmax ||
+      izc < az.cmin || izc > az.cmax) {
+    out[oi] = (TC)0.;
+    return;
+  }
+  const double *wx = ax.w + 5 * ixc, *wy = ay.w + 5 * iyc, *wz = az.w + 5 * izc;
+  int x0 = ax.lo[ixc], x1 = ax.hi[ixc], y0 = ay.lo[iyc], y1 = ay.hi[iyc];
+  int z0 = az.lo[izc], z1 = az.hi[izc];
+  double r = 0., w = 0.;
+  for (int iz = z0; iz <= z1; iz++)
+    for (int iy = y0; iy <= y1; iy++)
+      for (int ix = x0; ix <= x1; ix++) {
+        double ww = wx[ix - x0] * wy[iy - y0] * wz[iz - z0];
+        r = r + ww * (double)res[nat_index(fine, ix, iy, iz)];
+        w = w + ww;
+      }
+  if (w > 0.) {
+    r = r / w;
+    out[oi] = (TC)(r * ff);
+  } else {
+    out[oi] = (TC)0.;
+  }
+}
+
+// Single-precision variant: the tent weights and sums are taken in T.
+template <typename T, typename OutIdx>
+__global__ __launch_bounds__(256) void k_restrict_t(
+    const T *__restrict__ res, NatGeom fine, T *__restrict__ out, OutIdx O,
+    int nxc, int nyc, int nzc, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
+    T ff, const int *__restrict__ skip) {
+  WM_SKIP();
+  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ixc = (int)(t % (nxc + 1));
+  long long u = t / (nxc + 1);
+  int iyc = (int)(u % (nyc + 1));
+  int izc = (int)(u / (nyc + 1));
+  long long oi = O(ixc, iyc, izc);
+  if (ixc < ax.cmin || ixc > ax.cmax || iyc < ay.cmin || iyc > ay.cmax ||
+      izc < az.cmin || izc > az.cmax) {
+    out[oi] = (T)0.;
+    return;
+  }
+  const double *wx = ax.w + 5 * ixc, *wy = ay.w + 5 * iyc, *wz = az.w + 5 * izc;
+  int x0 = ax.lo[ixc], x1 = ax.hi[ixc], y0 = ay.lo[iyc], y1 = ay.hi[iyc];
+  int z0 = az.lo[izc], z1 = az.hi[izc];
+  T r = 0., w = 0.;
+  for (int iz = z0; iz <= z1; iz++)
+    for (int iy = y0; iy <= y1; iy++)
+      for (int ix = x0; ix <= x1; ix++) {
+        T ww = (T)wx[ix - x0] * (T)wy[iy - y0] * (T)wz[iz - z0];
+        r = r + ww * res[nat_index(fine, ix, iy, iz)];
+        w = w + ww;
+      }
+  out[oi] = w > (T)0. ? (r / w) * ff : (T)0.;
+}
+
+// ---------------------------------------------------------------------------
+// expand3d (f3d_mgrid.F:1777-1947): trilinear prolongation added to the
+// fine phi, one thread per fine point in the loop range.
+// ---------------------------------------------------------------------------
+struct ExpandAxis {
+  const int *j;     // coarse index per fine index
+  const double *w;  // weight per fine index
+  int lo, hi;       // fine loop range
+};
+
+template <typename T, typename FineIdx, typename CoarseIdx>
+__global__ __launch_bounds__(256) void k_expand(
+    T *__restrict__ phi, FineIdx F, const T *__restrict__ pc, CoarseIdx Cc,
+    ExpandAxis ax, ExpandAxis ay, ExpandAxis az, const int *__restrict__ skip) {
+  WM_SKIP();
+  int nxr = ax.hi - ax.lo + 1, nyr = ay.hi - ay.lo + 1, nzr = az.hi - az.lo + 1;
+  if (nxr <= 0 || nyr <= 0 || nzr <= 0) return;
+  long long n = (long long)nxr * nyr * nzr;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = ax.lo + (int)(t % nxr);
+  long long u = t / nxr;
+  int iy = ay.lo + (int)(u % nyr);
+  int iz = az.lo + (int)(u / nyr);
+  int jx = ax.j[ix], jy = ay.j[iy], jz = az.j[iz];
+  T wx = (T)ax.w[ix], wy = (T)ay.w[iy], wz = (T)az.w[iz];
+  T one = (T)1.;
+  long long fi = F(ix, iy, iz);
+  phi[fi] = phi[fi] +
+            (one - wx) * (one - wy) * (one - wz) * pc[Cc(jx, jy, jz)] +
+            wx * (one - wy) * (one - wz) * pc[Cc(jx + 1, jy, jz)] +
+            (one - wx) * wy * (one - wz) * pc[Cc(jx, jy + 1, jz)] +
+            wx * wy * (one - wz) * pc[Cc(jx + 1, jy + 1, jz)] +
+            (one - wx) * (one - wy) * wz * pc[Cc(jx, jy, jz + 1)] +
+            wx * (one - wy) * wz * pc[Cc(jx + 1, jy, jz + 1)] +
+            (one - wx) * wy * wz * pc[Cc(jx, jy + 1, jz + 1)] +
+            wx * wy * wz * pc[Cc(jx + 1, jy + 1, jz + 1)];
+}
+
+// ---------------------------------------------------------------------------
+// Whole-array helpers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, long long n, T v,
+                                              const int *__restrict__ skip) {
+  WM_SKIP();
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// natural -> red-black (all elements, guards included).
+template <typename T>
+__global__ __launch_bounds__(256) void k_nat_to_rb(const double *__restrict__ src,
+                                                   T *__restrict__ dst, RBGeom g) {
+  long long sx = g.nx + 3, sy = g.ny + 3;
+  long long n = sx * sy * (g.nz + 3);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % sx) - 1;
+  long long u = t / sx;
+  int iy = (int)(u % sy) - 1;
+  int iz = (int)(u / sy) - 1;
+  dst[rb_index(g, ix, iy, iz)] = (T)src[t];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rb_to_nat(const T *__restrict__ src,
+                                                   double *__restrict__ dst,
+                                                   RBGeom g) {
+  long long sx = g.nx + 3, sy = g.ny + 3;
+  long long n = sx * sy * (g.nz + 3);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % sx) - 1;
+  long long u = t / sx;
+  int iy = (int)(u % sy) - 1;
+  int iz = (int)(u / sy) - 1;
+  dst[t] = (double)src[rb_index(g, ix, iy, iz)];
+}
+
+// rho (0:nx)^3 natural -> red-black, scaled by reps0c (multigrid3dsolve
+// f3d_mgrid.F:221-224); the caller's rho is left as the reference leaves
+// it after its closing rescale (478-492): (rho*reps0c)*(1/reps0c).
+__global__ __launch_bounds__(256) void k_rho_in(double *__restrict__ rho,
+                                                double *__restrict__ dst, RBGeom g,
+                                                double reps0c, double rinv) {
+  long long sx = g.nx + 1, sy = g.ny + 1;
+  long long n = sx * sy * (g.nz + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % sx);
+  long long u = t / sx;
+  int iy = (int)(u % sy);
+  int iz = (int)(u / sy);
+  double s = rho[t] * reps0c;
+  dst[rb_index(g, ix, iy, iz)] = s;
+  rho[t] = s * rinv;
+}
+
+// rho (0:nx)^3 natural -> red-black, unscaled (per-kernel entry points).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rho_to_rb(const double *__restrict__ rho,
+                                                   T *__restrict__ dst, RBGeom g) {
+  long long sx = g.nx + 1, sy = g.ny + 1;
+  long long n = sx * sy * (g.nz + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % sx);
+  long long u = t / sx;
+  int iy = (int)(u % sy);
+  int iz = (int)(u / sy);
+  dst[rb_index(g, ix, iy, iz)] = (T)rho[t];
+}
+
+// State of the V-cycle loop on the device.
+struct SolveState {
+  unsigned long long errbits;  // max |dphi| as the bits of a double >= 0
+  unsigned int arrived;        // blocks done in the current reduction
+  int iters;                   // mgiters
+  int done;                    // the loop has ended (also the skip flag)
+  int pad;
+  double err;                  // mgerror of the last V-cycle
+};
+
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// The convergence test of multigrid3dsolve (f3d_mgrid.F:383-392, 313):
+// mgerror = max over nodes 0..n of |phisave-phi|, then phisave = phi.  The
+// last block to finish closes the reduction, counts the V-cycle and sets
+// `done` when mgerror <= mgtol or mgiters == mgmaxiters.
+template <typename T>
+__global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
+                                                  T *__restrict__ save, RBGeom g,
+                                                  SolveState *st, double mgtol,
+                                                  int mgmaxiters, int docopy) {
+  if (st->done) return;
+  __shared__ double part[4];
+  long long n = 2 * g.cs;
+  double m = 0.;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n;
+       t += (long long)gridDim.x * 256) {
+    int c = (int)(t / g.cs);
+    long long e = t - c * g.cs;
+    int j = (int)(e % g.W);
+    long long r = e / g.W;
+    int py = (int)(r % g.PY);
+    int pz = (int)(r / g.PY);
+    int px = ((c + py + pz + 1) & 1) + 2 * j;
+    int ix = px - 1, iy = py - 1, iz = pz - 1;
+    T cur = phi[t];
+    if (ix >= 0 && ix <= g.nx && iy >= 0 && iy <= g.ny && iz >= 0 && iz <= g.nz) {
+      double d = fabs((double)save[t] - (double)cur);
+      m = d > m ? d : m;
+    }
+    if (docopy) save[t] = cur;
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = part[0];
+    for (int k = 1; k < 4; k++) b = part[k] > b ? part[k] : b;
+    atomicMax(&st->errbits, (unsigned long long)__double_as_longlong(b));
+    __threadfence();
+    unsigned int prev = atomicAdd(&st->arrived, 1u);
+    if (prev == gridDim.x - 1) {
+      __threadfence();
+      double err = __longlong_as_double(
+          (long long)atomicAdd(&st->errbits, 0ull));
+      st->err = err;
+      st->iters = st->iters + 1;
+      st->errbits = 0ull;
+      st->arrived = 0u;
+      if (!(err > mgtol && st->iters < mgmaxiters)) st->done = 1;
+    }
+  }
+}
+
+// Defect-correction update: phi64 = e + phisave over the whole array
+// (mgform=2 in multigrid3dsolve, f3d_mgrid.F:364-368) with e the
+// low-precision correction.
+template <typename TL>
+__global__ __launch_bounds__(256) void k_dc_add(double *phi, const TL *e,
+                                                const double *__restrict__ save,
+                                                long long n,
+                                                const int *__restrict__ skip) {
+  WM_SKIP();
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) phi[t] = (double)e[t] + save[t];
+}
+
+// Copy of level-0 residual (natural) into the low-precision rho (red-black),
+// rho(0:n) = res(0:n) of the mgform=2 loop (f3d_mgrid.F:341).
+template <typename TL>
+__global__ __launch_bounds__(256) void k_res_to_rho(const double *__restrict__ res,
+                                                    TL *__restrict__ rho, RBGeom g,
+                                                    const int *__restrict__ skip) {
+  WM_SKIP();
+  long long sx = g.nx + 1, sy = g.ny + 1;
+  long long n = sx * sy * (g.nz + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % sx);
+  long long u = t / sx;
+  int iy = (int)(u % sy);
+  int iz = (int)(u / sy);
+  NatGeom ng{g.nx, g.ny, g.nz};
+  rho[rb_index(g, ix, iy, iz)] = (TL)res[nat_index(ng, ix, iy, iz)];
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_copy(const TA *__restrict__ a,
+                                              TB *__restrict__ b, long long n,
+                                              const int *__restrict__ skip) {
+  WM_SKIP();
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) b[t] = (TB)a[t];
+}

@@ -1,0 +1,524 @@
+// Particle kernels of the electrostatic PIC step for gfx950: CIC deposit
+// (setrho3ddirect1), field gather (sete_from_phi_linear3d), grid E
+// (getselfe3d) and the leapfrog kicks/drift (epush3d, xpush3d), from
+// pywarp90/w3d.F.  All arithmetic is FP64 in the reference's operation
+// order (the library is built with -ffp-contract=off), so integer cell
+// indices and every per-particle value are bit-identical to the reference;
+// only the order in which deposits are summed into rho differs.
+#include "wm_internal.h"
+#include "../../include/warpmi.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace {
+
+struct Mesh {
+  int nx, ny, nz;
+  double xmmin, ymmin, zmmin, zgrid;
+  double dxi, dyi, dzi;
+};
+
+// CIC index and weights, setrho3ddirect1 (w3d.F:7883-7904): int() of the
+// scaled offset (truncation), fraction by subtraction.  Returns false for a
+// particle whose cell is not inside the grid (the reference does not check).
+__device__ inline bool cic(const Mesh &m, double x, double y, double z, long long &ii,
+                           long long &jj, long long &kk, double &u1, double &v1,
+                           double &w1) {
+  double sx = (x - m.xmmin) * m.dxi;
+  double sy = (y - m.ymmin) * m.dyi;
+  double sz = (z - m.zgrid - m.zmmin) * m.dzi;
+  if (!(sx >= 0. && sx < (double)m.nx && sy >= 0. && sy < (double)m.ny && sz >= 0. &&
+        sz < (double)m.nz))
+    return false;
+  ii = (long long)sx;
+  jj = (long long)sy;
+  kk = (long long)sz;
+  u1 = sx - ii;
+  v1 = sy - jj;
+  w1 = sz - kk;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long long np,
+                                                const double *__restrict__ xp,
+                                                const double *__restrict__ yp,
+                                                const double *__restrict__ zp, Mesh m,
+                                                double g,
+                                                unsigned long long *__restrict__ nskip) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  long long ii, jj, kk;
+  double u1, v1, w1;
+  if (!cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1)) {
+    if (nskip) atomicAdd(nskip, 1ull);
+    return;
+  }
+  double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
+  double *r = rho + ii + sx * jj + sxy * kk;
+  atomicAdd(r, u0 * v0 * w0 * g);
+  atomicAdd(r + 1, u1 * v0 * w0 * g);
+  atomicAdd(r + sx, u0 * v1 * w0 * g);
+  atomicAdd(r + sx + 1, u1 * v1 * w0 * g);
+  atomicAdd(r + sxy, u0 * v0 * w1 * g);
+  atomicAdd(r + sxy + 1, u1 * v0 * w1 * g);
+  atomicAdd(r + sxy + sx, u0 * v1 * w1 * g);
+  atomicAdd(r + sxy + sx + 1, u1 * v1 * w1 * g);
+}
+
+// Cell key of each particle for the sorted deposit; out-of-grid particles
+// get the key ncell (sorted last, not deposited).
+__global__ __launch_bounds__(256) void k_keys(long long np, const double *__restrict__ xp,
+                                              const double *__restrict__ yp,
+                                              const double *__restrict__ zp, Mesh m,
+                                              unsigned int *__restrict__ key,
+                                              unsigned int *__restrict__ idx) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  long long ii, jj, kk;
+  double u1, v1, w1;
+  unsigned int ncell = (unsigned int)((long long)m.nx * m.ny * m.nz);
+  if (cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1))
+    key[ip] = (unsigned int)(ii + (long long)m.nx * (jj + (long long)m.ny * kk));
+  else
+    key[ip] = ncell;
+  idx[ip] = (unsigned int)ip;
+}
+
+// Deposit in cell order.  Lanes of a wavefront that hold particles of the
+// same cell add their eight corner weights with a segmented reduction over
+// the wave; the first lane of each run then does the eight atomic adds.
+__global__ __launch_bounds__(256) void k_setrho_sorted(
+    double *__restrict__ rho, long long np, const double *__restrict__ xp,
+    const double *__restrict__ yp, const double *__restrict__ zp, Mesh m, double g,
+    const unsigned int *__restrict__ key, const unsigned int *__restrict__ idx) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  unsigned int ncell = (unsigned int)((long long)m.nx * m.ny * m.nz);
+  unsigned int k = t < np ? key[t] : ncell;
+  double c[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  long long ii = 0, jj = 0, kk = 0;
+  if (k < ncell) {
+    long long ip = idx[t];
+    double u1, v1, w1;
+    cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1);
+    double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+    c[0] = u0 * v0 * w0 * g;
+    c[1] = u1 * v0 * w0 * g;
+    c[2] = u0 * v1 * w0 * g;
+    c[3] = u1 * v1 * w0 * g;
+    c[4] = u0 * v0 * w1 * g;
+    c[5] = u1 * v0 * w1 * g;
+    c[6] = u0 * v1 * w1 * g;
+    c[7] = u1 * v1 * w1 * g;
+  }
+  // Segmented inclusive suffix sums: lane L accumulates lanes L+1.. of the
+  // same key, so the first lane of each run holds the run's total.
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned int ko = __shfl_down(k, off, 64);
+    bool same = (lane + off < 64) && ko == k;
+    for (int q = 0; q < 8; q++) {
+      double o = __shfl_down(c[q], off, 64);
+      if (same) c[q] += o;
+    }
+    // Keys of a run are contiguous, so extending by doubling stays inside it.
+  }
+  unsigned int kp = __shfl_up(k, 1, 64);
+  bool head = (lane == 0) || kp != k;
+  if (!head || k >= ncell) return;
+  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
+  double *r = rho + ii + sx * jj + sxy * kk;
+  atomicAdd(r, c[0]);
+  atomicAdd(r + 1, c[1]);
+  atomicAdd(r + sx, c[2]);
+  atomicAdd(r + sx + 1, c[3]);
+  atomicAdd(r + sxy, c[4]);
+  atomicAdd(r + sxy + 1, c[5]);
+  atomicAdd(r + sxy + sx, c[6]);
+  atomicAdd(r + sxy + sx + 1, c[7]);
+}
+
+// sete_from_phi_linear3d (w3d.F:3963-4107): E from centred differences of
+// phi at the eight cell corners, trilinear weights, in the reference's sum
+// order.  Particles outside the grid get no field.
+__device__ inline void gather_e(const double *__restrict__ phi, const Mesh &m, double tdxi,
+                                double tdyi, double tdzi, double x, double y, double z,
+                                double &ext, double &eyt, double &ezt) {
+  long long i, j, k;
+  double u1, v1, w1;
+  if (!cic(m, x, y, z, i, j, k, u1, v1, w1)) {
+    ext = eyt = ezt = 0.;
+    return;
+  }
+  double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+  long long sx = m.nx + 3, sxy = (long long)(m.nx + 3) * (m.ny + 3);
+  // p(a,b,c) = phi(i+a, j+b, k+c), phi dimensioned (-1:nx+1)^3
+  const double *p0 = phi + (i + 1) + sx * (j + 1) + sxy * (k + 1);
+#define P(a, b, c) p0[(a) + sx * (b) + sxy * (c)]
+  ext = tdxi * (u0 * v0 * w0 * (P(-1, 0, 0) - P(1, 0, 0)) + u1 * v0 * w0 * (P(0, 0, 0) - P(2, 0, 0)) +
+                u0 * v1 * w0 * (P(-1, 1, 0) - P(1, 1, 0)) + u1 * v1 * w0 * (P(0, 1, 0) - P(2, 1, 0)) +
+                u0 * v0 * w1 * (P(-1, 0, 1) - P(1, 0, 1)) + u1 * v0 * w1 * (P(0, 0, 1) - P(2, 0, 1)) +
+                u0 * v1 * w1 * (P(-1, 1, 1) - P(1, 1, 1)) + u1 * v1 * w1 * (P(0, 1, 1) - P(2, 1, 1)));
+  eyt = tdyi * (u0 * v0 * w0 * (P(0, -1, 0) - P(0, 1, 0)) + u1 * v0 * w0 * (P(1, -1, 0) - P(1, 1, 0)) +
+                u0 * v1 * w0 * (P(0, 0, 0) - P(0, 2, 0)) + u1 * v1 * w0 * (P(1, 0, 0) - P(1, 2, 0)) +
+                u0 * v0 * w1 * (P(0, -1, 1) - P(0, 1, 1)) + u1 * v0 * w1 * (P(1, -1, 1) - P(1, 1, 1)) +
+                u0 * v1 * w1 * (P(0, 0, 1) - P(0, 2, 1)) + u1 * v1 * w1 * (P(1, 0, 1) - P(1, 2, 1)));
+  ezt = tdzi * (u0 * v0 * w0 * (P(0, 0, -1) - P(0, 0, 1)) + u1 * v0 * w0 * (P(1, 0, -1) - P(1, 0, 1)) +
+                u0 * v1 * w0 * (P(0, 1, -1) - P(0, 1, 1)) + u1 * v1 * w0 * (P(1, 1, -1) - P(1, 1, 1)) +
+                u0 * v0 * w1 * (P(0, 0, 0) - P(0, 0, 2)) + u1 * v0 * w1 * (P(1, 0, 0) - P(1, 0, 2)) +
+                u0 * v1 * w1 * (P(0, 1, 0) - P(0, 1, 2)) + u1 * v1 * w1 * (P(1, 1, 0) - P(1, 1, 2)));
+#undef P
+}
+
+__global__ __launch_bounds__(256) void k_sete(const double *__restrict__ phi, long long np,
+                                              const double *__restrict__ xp,
+                                              const double *__restrict__ yp,
+                                              const double *__restrict__ zp, Mesh m,
+                                              double tdxi, double tdyi, double tdzi,
+                                              double *__restrict__ ex, double *__restrict__ ey,
+                                              double *__restrict__ ez) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  double a, b, c;
+  gather_e(phi, m, tdxi, tdyi, tdzi, xp[ip], yp[ip], zp[ip], a, b, c);
+  ex[ip] = ex[ip] + a;
+  ey[ip] = ey[ip] + b;
+  ez[ip] = ez[ip] + c;
+}
+
+__global__ __launch_bounds__(256) void k_epush(long long np, double *__restrict__ ux,
+                                               double *__restrict__ uy, double *__restrict__ uz,
+                                               const double *__restrict__ ex,
+                                               const double *__restrict__ ey,
+                                               const double *__restrict__ ez, double cst) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  ux[ip] = ux[ip] + ex[ip] * cst;
+  uy[ip] = uy[ip] + ey[ip] * cst;
+  uz[ip] = uz[ip] + ez[ip] * cst;
+}
+
+__global__ __launch_bounds__(256) void k_xpush(long long np, double *__restrict__ x,
+                                               double *__restrict__ y, double *__restrict__ z,
+                                               const double *__restrict__ ux,
+                                               const double *__restrict__ uy,
+                                               const double *__restrict__ uz,
+                                               const double *__restrict__ gaminv, double dtp) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  double gi = gaminv[ip];
+  x[ip] = x[ip] + ux[ip] * gi * dtp;
+  y[ip] = y[ip] + uy[ip] * gi * dtp;
+  z[ip] = z[ip] + uz[ip] * gi * dtp;
+}
+
+// padvnc3d "fullv" for B = 0 (w3d.F:1816, 1956-1986): E = 0 + gather,
+// two half kicks, one drift.
+__global__ __launch_bounds__(256) void k_gather_push(
+    const double *__restrict__ phi, long long np, double *__restrict__ x, double *__restrict__ y,
+    double *__restrict__ z, double *__restrict__ ux, double *__restrict__ uy,
+    double *__restrict__ uz, const double *__restrict__ gaminv, double *__restrict__ ex,
+    double *__restrict__ ey, double *__restrict__ ez, Mesh m, double tdxi, double tdyi,
+    double tdzi, double ch, double dt) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  double xi = x[ip], yi = y[ip], zi = z[ip];
+  double a, b, c;
+  gather_e(phi, m, tdxi, tdyi, tdzi, xi, yi, zi, a, b, c);
+  double ea = 0. + a, eb = 0. + b, ec = 0. + c;
+  double vx = ux[ip], vy = uy[ip], vz = uz[ip];
+  vx = vx + ea * ch;
+  vy = vy + eb * ch;
+  vz = vz + ec * ch;
+  vx = vx + ea * ch;
+  vy = vy + eb * ch;
+  vz = vz + ec * ch;
+  double gi = gaminv[ip];
+  x[ip] = xi + vx * gi * dt;
+  y[ip] = yi + vy * gi * dt;
+  z[ip] = zi + vz * gi * dt;
+  ux[ip] = vx;
+  uy[ip] = vy;
+  uz[ip] = vz;
+  if (ex) {
+    ex[ip] = ea;
+    ey[ip] = eb;
+    ez[ip] = ec;
+  }
+}
+
+// getselfe3d (w3d.F:6023-6109), nxguarde = 0.
+__global__ __launch_bounds__(256) void k_getselfe(const double *__restrict__ phi, int nx, int ny,
+                                                  int nz, double *__restrict__ selfe,
+                                                  double tdxi, double tdyi, double tdzi,
+                                                  int lzero) {
+  long long n = (long long)(nx + 1) * (ny + 1) * (nz + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % (nx + 1));
+  long long u = t / (nx + 1);
+  int iy = (int)(u % (ny + 1));
+  int iz = (int)(u / (ny + 1));
+  long long sx = nx + 3, sxy = (long long)(nx + 3) * (ny + 3);
+  const double *p = phi + (ix + 1) + sx * (iy + 1) + sxy * (iz + 1);
+  double gx = tdxi * (p[-1] - p[1]);
+  double gy = tdyi * (p[-sx] - p[sx]);
+  double gz = tdzi * (p[-sxy] - p[sxy]);
+  double *e = selfe + 3 * t;
+  if (lzero) {
+    e[0] = gx;
+    e[1] = gy;
+    e[2] = gz;
+  } else {
+    e[0] = e[0] + gx;
+    e[1] = e[1] + gy;
+    e[2] = e[2] + gz;
+  }
+}
+
+// applyrhoboundaryconditions3d (w3d.F:2660-2750) for one axis: Neumann
+// faces double, periodic faces sum then copy.  One thread per face line.
+__global__ void k_rhobc(double *__restrict__ rho, int axis, int nx, int ny, int nz, int bm,
+                        int bp) {
+  int na = axis == 0 ? ny : nx, nb = axis == 2 ? ny : nz, n = axis == 0 ? nx : axis == 1 ? ny : nz;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)(na + 1) * (nb + 1)) return;
+  int a = (int)(t % (na + 1)), b = (int)(t / (na + 1));
+  long long sx = nx + 1, sxy = (long long)(nx + 1) * (ny + 1);
+  auto at = [&](int i) -> double & {
+    if (axis == 0) return rho[i + sx * a + sxy * b];
+    if (axis == 1) return rho[a + sx * i + sxy * b];
+    return rho[a + sx * b + sxy * i];
+  };
+  if (bm == 1) at(0) = at(0) + at(0);
+  if (bp == 1) at(n) = at(n) + at(n);
+  if (bm == 2 || bp == 2) {
+    at(0) = at(0) + at(n);
+    at(n) = at(0);
+  }
+}
+
+static int nblk(long long n) { return (int)std::max<long long>(1, cdiv(n, 256)); }
+
+static Mesh make_mesh(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                      double xmmin, double ymmin, double zmmin, double zgrid) {
+  Mesh m;
+  m.nx = (int)nx;
+  m.ny = (int)ny;
+  m.nz = (int)nz;
+  m.xmmin = xmmin;
+  m.ymmin = ymmin;
+  m.zmmin = zmmin;
+  m.zgrid = zgrid;
+  m.dxi = 1. / dx;
+  m.dyi = 1. / dy;
+  m.dzi = 1. / dz;
+  return m;
+}
+
+static void check_grid(int64_t nx, int64_t ny, int64_t nz) {
+  WM_REQUIRE(nx > 0 && ny > 0 && nz > 0, "grid must have nx, ny, nz > 0");
+  WM_REQUIRE((nx + 1) * (ny + 1) * (nz + 1) < (1ll << 32), "grid too large for 32-bit cell keys");
+}
+
+// Charge per unit volume of one macroparticle: g = wght*q, then /dx, /dy,
+// /dz in sequence (w3d.F:7782-7785).
+static double deposit_g(double q, double wght, int64_t nx, int64_t ny, int64_t nz, double dx,
+                        double dy, double dz) {
+  double g = wght * q;
+  if (nx > 0) g = g / dx;
+  if (ny > 0) g = g / dy;
+  if (nz > 0) g = g / dz;
+  return g;
+}
+
+struct SortedWork {
+  size_t keys, idx, keys2, idx2, temp, total;
+};
+
+static SortedWork sorted_layout(int64_t np, int64_t ncell) {
+  SortedWork w{};
+  size_t temp = 0;
+  int endbit = 1;
+  while ((1ll << endbit) <= ncell) endbit++;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (unsigned int *)nullptr,
+                                           (unsigned int *)nullptr, (unsigned int *)nullptr,
+                                           (unsigned int *)nullptr, (int)np, 0, endbit);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t n4 = al(sizeof(unsigned int) * (size_t)std::max<int64_t>(np, 1));
+  w.keys = 0;
+  w.idx = n4;
+  w.keys2 = 2 * n4;
+  w.idx2 = 3 * n4;
+  w.temp = 4 * n4;
+  w.total = 4 * n4 + al(temp);
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wm_setrho3d(double *rho, int64_t np, const double *xp, const double *yp, const double *zp,
+                double zgrid, double q, double wght, int64_t nx, int64_t ny, int64_t nz,
+                double dx, double dy, double dz, double xmmin, double ymmin, double zmmin,
+                int64_t *nskipped, void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    WM_REQUIRE(np >= 0, "wm_setrho3d: np < 0");
+    if (nskipped) *nskipped = 0;
+    if (np == 0) return 0;
+    WM_REQUIRE(rho && xp && yp && zp, "wm_setrho3d: NULL array");
+    hipStream_t s = wm_stream(stream);
+    Mesh m = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
+    double g = deposit_g(q, wght, nx, ny, nz, dx, dy, dz);
+    unsigned long long *cnt = nullptr;
+    if (nskipped) {
+      WM_HIP(hipMallocAsync((void **)&cnt, sizeof(unsigned long long), s));
+      WM_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
+    }
+    hipLaunchKernelGGL(k_setrho, dim3(nblk(np)), dim3(256), 0, s, rho, (long long)np, xp, yp, zp,
+                       m, g, cnt);
+    WM_LAUNCHED();
+    if (nskipped) {
+      unsigned long long h = 0;
+      WM_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+      WM_HIP(hipFreeAsync(cnt, s));
+      WM_HIP(hipStreamSynchronize(s));
+      *nskipped = (int64_t)h;
+    }
+  });
+}
+
+int64_t wm_setrho3d_sorted_worksize(int64_t np, int64_t nx, int64_t ny, int64_t nz) {
+  return (int64_t)sorted_layout(np, nx * ny * nz).total;
+}
+
+int wm_setrho3d_sorted(double *rho, int64_t np, const double *xp, const double *yp,
+                       const double *zp, double zgrid, double q, double wght, int64_t nx,
+                       int64_t ny, int64_t nz, double dx, double dy, double dz, double xmmin,
+                       double ymmin, double zmmin, void *work, void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    WM_REQUIRE(np >= 0 && np < (1ll << 31), "wm_setrho3d_sorted: np out of range");
+    if (np == 0) return 0;
+    WM_REQUIRE(rho && xp && yp && zp && work, "wm_setrho3d_sorted: NULL array");
+    hipStream_t s = wm_stream(stream);
+    Mesh m = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
+    double g = deposit_g(q, wght, nx, ny, nz, dx, dy, dz);
+    int64_t ncell = nx * ny * nz;
+    SortedWork w = sorted_layout(np, ncell);
+    char *base = (char *)work;
+    unsigned int *keys = (unsigned int *)(base + w.keys), *idx = (unsigned int *)(base + w.idx);
+    unsigned int *keys2 = (unsigned int *)(base + w.keys2), *idx2 = (unsigned int *)(base + w.idx2);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(np)), dim3(256), 0, s, (long long)np, xp, yp, zp, m, keys,
+                       idx);
+    WM_LAUNCHED();
+    size_t temp = w.total - w.temp;
+    int endbit = 1;
+    while ((1ll << endbit) <= ncell) endbit++;
+    WM_HIP(hipcub::DeviceRadixSort::SortPairs(base + w.temp, temp, keys, keys2, idx, idx2, (int)np,
+                                              0, endbit, s));
+    hipLaunchKernelGGL(k_setrho_sorted, dim3(nblk(np)), dim3(256), 0, s, rho, (long long)np, xp, yp,
+                       zp, m, g, keys2, idx2);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_applyrhoboundaryconditions3d(int64_t nx, int64_t ny, int64_t nz, double *rho,
+                                    const int64_t *bounds, void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    WM_REQUIRE(rho && bounds, "wm_applyrhoboundaryconditions3d: NULL argument");
+    hipStream_t s = wm_stream(stream);
+    for (int a = 0; a < 3; a++) {
+      int bm = (int)bounds[2 * a], bp = (int)bounds[2 * a + 1];
+      if (bm == 0 && bp == 0) continue;
+      int na = a == 0 ? (int)ny : (int)nx, nb = a == 2 ? (int)ny : (int)nz;
+      hipLaunchKernelGGL(k_rhobc, dim3(nblk((long long)(na + 1) * (nb + 1))), dim3(256), 0, s, rho,
+                         a, (int)nx, (int)ny, (int)nz, bm, bp);
+      WM_LAUNCHED();
+    }
+  });
+}
+
+int wm_sete3d(const double *phi, int64_t np, const double *xp, const double *yp, const double *zp,
+              double zgrid, double xmmin, double ymmin, double zmmin, double dx, double dy,
+              double dz, int64_t nx, int64_t ny, int64_t nz, double *ex, double *ey, double *ez,
+              void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    if (np == 0) return 0;
+    WM_REQUIRE(phi && xp && yp && zp && ex && ey && ez, "wm_sete3d: NULL array");
+    Mesh m = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
+    double tdxi = 1. / (2. * dx), tdyi = 1. / (2. * dy), tdzi = 1. / (2. * dz);
+    hipLaunchKernelGGL(k_sete, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi, (long long)np,
+                       xp, yp, zp, m, tdxi, tdyi, tdzi, ex, ey, ez);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_epush3d(int64_t np, double *uxp, double *uyp, double *uzp, const double *ex,
+               const double *ey, const double *ez, double q, double m, double ddt, void *stream) {
+  WM_ENTRY({
+    if (np == 0) return 0;
+    WM_REQUIRE(uxp && uyp && uzp && ex && ey && ez, "wm_epush3d: NULL array");
+    double cst = q * ddt / m;
+    hipLaunchKernelGGL(k_epush, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), (long long)np, uxp,
+                       uyp, uzp, ex, ey, ez, cst);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_xpush3d(int64_t np, double *xp, double *yp, double *zp, const double *uxp,
+               const double *uyp, const double *uzp, const double *gaminv, double dtp,
+               void *stream) {
+  WM_ENTRY({
+    if (np == 0) return 0;
+    WM_REQUIRE(xp && yp && zp && uxp && uyp && uzp && gaminv, "wm_xpush3d: NULL array");
+    hipLaunchKernelGGL(k_xpush, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), (long long)np, xp,
+                       yp, zp, uxp, uyp, uzp, gaminv, dtp);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, double *zp,
+                     double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                     double *ey, double *ez, double zgrid, double xmmin, double ymmin,
+                     double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
+                     int64_t nz, double q, double m, double dt, void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    if (np == 0) return 0;
+    WM_REQUIRE(phi && xp && yp && zp && uxp && uyp && uzp && gaminv, "wm_gather_push3d: NULL array");
+    WM_REQUIRE((ex && ey && ez) || (!ex && !ey && !ez), "wm_gather_push3d: give all or none of ex, ey, ez");
+    Mesh me = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
+    double tdxi = 1. / (2. * dx), tdyi = 1. / (2. * dy), tdzi = 1. / (2. * dz);
+    // fulldt_s = dt*ndts*dtscale, halfdt_s = 0.5*fulldt_s (w3d.F:1747-1748);
+    // epush3d's const = q*ddt/m (w3d.F:2765).
+    double fulldt = dt * 1 * 1.;
+    double halfdt = 0.5 * fulldt;
+    double ch = q * halfdt / m;
+    hipLaunchKernelGGL(k_gather_push, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
+                       (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi, tdyi,
+                       tdzi, ch, fulldt);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz, double *selfe, double dx,
+                  double dy, double dz, int64_t lzero, void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    WM_REQUIRE(phi && selfe, "wm_getselfe3d: NULL array");
+    double tdxi = 0.5 / dx, tdyi = 0.5 / dy, tdzi = 0.5 / dz;
+    long long n = (long long)(nx + 1) * (ny + 1) * (nz + 1);
+    hipLaunchKernelGGL(k_getselfe, dim3(nblk(n)), dim3(256), 0, wm_stream(stream), phi, (int)nx,
+                       (int)ny, (int)nz, selfe, tdxi, tdyi, tdzi, (int)lzero);
+    WM_LAUNCHED();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// Internal helpers shared by the libwarpmi translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+void wm_set_error(const std::string &msg);
+
+struct wm_error : std::runtime_error {
+  int code;
+  wm_error(const std::string &m, int c = -1) : std::runtime_error(m), code(c) {}
+};
+
+#define WM_HIP(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess)                                                  \
+      throw wm_error(std::string(#x) + ": " + hipGetErrorString(e_), -2); \
+  } while (0)
+
+#define WM_REQUIRE(cond, msg)                      \
+  do {                                             \
+    if (!(cond)) throw wm_error(std::string(msg)); \
+  } while (0)
+
+// Runs body, turning exceptions into the ABI's error codes.
+#define WM_ENTRY(...)                      \
+  try {                                    \
+    __VA_ARGS__;                           \
+    return 0;                              \
+  } catch (const wm_error &e) {            \
+    wm_set_error(e.what());                \
+    return e.code;                         \
+  } catch (const std::exception &e) {      \
+    wm_set_error(e.what());                \
+    return -3;                             \
+  }
+
+static inline hipStream_t wm_stream(void *s) { return (hipStream_t)s; }
+
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Launch check: reports the launch error of the kernel just enqueued.
+#define WM_LAUNCHED() WM_HIP(hipGetLastError())
+
+// Physical constant used by the solver: eps0 = 1/(mu0*clight*clight),
+// mu0 = 4*pi/10000000 (scripts/warp.py:198-202; pywarp90/top.v:172-175).
+static inline double wm_eps0() {
+  double pi = 3.14159265358979323;
+  double clight = 2.99792458e+8;
+  double mu0 = 4 * pi / 10000000;
+  return 1 / (mu0 * clight * clight);
+}

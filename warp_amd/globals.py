@@ -1,0 +1,66 @@
+"""Package-level inputs mirroring the Forthon packages the reference solver
+reads its defaults from (pywarp90/top.v, w3d.v, f3d.v): ``top``, ``w3d``
+and ``f3d``.  Only the variables the electrostatic PIC path uses exist.
+"""
+import math
+
+
+class _Package:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def __repr__(self):
+        return "%s(%s)" % (type(self).__name__, ", ".join(
+            "%s=%r" % kv for kv in sorted(self.__dict__.items()) if not kv[0].startswith("_")))
+
+
+# Physical constants (pywarp90/top.v:172-181; scripts/warp.py:198-202).
+pi = 3.14159265358979323
+amu = 1.660538921e-27
+clight = 2.99792458e+8
+echarge = 1.602176565e-19
+emass = 9.10938291e-31
+mu0 = 4 * pi / 10000000
+eps0 = 1 / (mu0 * clight * clight)
+largepos = 1.0e+36
+dirichlet, neumann, periodic = 0, 1, 2
+
+
+class Top(_Package):
+    pass
+
+
+class W3D(_Package):
+    pass
+
+
+class F3D(_Package):
+    pass
+
+
+# top: time step and particle data (top.v); fstype=12 once a solver is
+# registered (fieldsolver.py:181).
+top = Top(dt=0., time=0., it=0, fstype=-1, pgroup=None, zbeam=0., zgrid=0.,
+          depos="direct1", efetch=1, lresetparticlee=True, steptime=0.,
+          fstime=0., lrtime=0.)
+
+# w3d: mesh and boundary inputs (w3d.v), plus the FieldSolveAPI globals
+# (w3d.v:432-458) through which fetche receives particle positions.
+w3d = W3D(nx=2, ny=2, nz=2, xmmin=0., xmmax=1., ymmin=0., ymmax=1., zmmin=0., zmmax=1.,
+          nxguardphi=1, nyguardphi=1, nzguardphi=1, nxguardrho=0, nyguardrho=0,
+          nzguardrho=0, nxguarde=0, nyguarde=0, nzguarde=0,
+          bound0=dirichlet, boundnz=dirichlet, boundxy=dirichlet,
+          l2symtry=False, l4symtry=False,
+          # FieldSolveAPI
+          npfsapi=0, xfsapi=None, yfsapi=None, zfsapi=None, exfsapi=None,
+          eyfsapi=None, ezfsapi=None, phifsapi=None, pgroupfsapi=None,
+          ipminfsapi=1, jsfsapi=-1, ndtsfsapi=1, lzerorhofsapi=True,
+          lfinalizerhofsapi=True, js1fsapi=-1, js2fsapi=-1)
+
+# f3d: multigrid controls with their defaults (f3d.v:169-231).
+f3d = F3D(gridmode=1, mgparam=1.2, mgmaxiters=100, mgmaxlevels=101, mgtol=1.e-6,
+          mgform=1, downpasses=1, uppasses=1, mgverbose=1, mgntverbose=1,
+          lcndbndy=True, icndbndy=2, laddconductor=False, lprecalccoeffs=False,
+          mgiters=0, mgerror=0.)
+
+assert math.isclose(eps0, 8.854187817620389e-12, rel_tol=1e-15)

@@ -1,0 +1,401 @@
+"""MultiGrid3D, the drop-in for scripts/multigrid.py's registered 3-D
+multigrid field solver, running on an MI355X through libwarpmi.
+
+Interface as in the reference (multigrid.py:19-953): construct with the
+mesh and f3d multigrid inputs, ``registersolver(solver)``, install
+conductors with ``installconductor``, then per step ``loadrho()``,
+``solve()``, ``fetche()``.  ``phi``, ``rho`` and ``selfe`` are FP64 torch
+tensors on the GPU in the reference's Fortran layout: phi is
+(-1:nx+1,-1:ny+1,-1:nz+1), i.e. a C-contiguous tensor of shape
+(nz+3, ny+3, nx+3); rho is (nz+1, ny+1, nx+1).  After ``solve`` the solver
+holds ``mgiters`` and ``mgerror``; non-convergence is reported, not raised
+(f3d_mgrid.F:460-467).
+
+The hot loop never falls back to the CPU: if libwarpmi or a HIP device is
+missing, construction fails.
+"""
+import ctypes as C
+import math
+import sys
+import time
+
+import torch
+
+from . import _native as N
+from .fieldsolver import FieldSolver
+from .generateconductors import ConductorData, Grid, installconductor as _install
+from .globals import eps0, f3d, top, w3d
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+
+class MultiGrid3D(FieldSolver):
+    __f3dinputs__ = ['gridmode', 'mgparam', 'downpasses', 'uppasses', 'mgmaxiters', 'mgtol',
+                     'mgmaxlevels', 'mgform', 'mgverbose', 'mgntverbose', 'lcndbndy',
+                     'icndbndy', 'laddconductor', 'lprecalccoeffs']
+
+    def __init__(self, precision="fp64", device=None, sorted_deposit=True, **kw):
+        FieldSolver.__init__(self, kwdict=kw)
+        self.processdefaultsfrompackage(MultiGrid3D.__f3dinputs__, f3d, kw)
+        assert len(kw.keys()) == 0, "Bad keyword arguemnts %s" % list(kw.keys())
+        if self.lprecalccoeffs:
+            raise NotImplementedError("lprecalccoeffs: the coefficients are always derived on the fly")
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be one of %s" % list(PRECISIONS))
+        self.precision = precision
+        self.sorted_deposit = sorted_deposit
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        if self.device.type != "cuda" or not torch.cuda.is_available():
+            raise RuntimeError("MultiGrid3D needs a HIP device (torch.cuda is not available)")
+        N.lib()
+        self.solvergeom = "XYZgeom"
+        self.mgiters = 0
+        self.mgerror = 0.
+        self.lwithselfe = 0
+        self._phi = torch.zeros((self.nz + 3, self.ny + 3, self.nx + 3), dtype=torch.float64,
+                                device=self.device)
+        self._rho = torch.zeros((self.nz + 1, self.ny + 1, self.nx + 1), dtype=torch.float64,
+                                device=self.device)
+        self._selfe = None
+        self._work = None
+        b = (C.c_int64 * 6)(*self.bounds)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            N.call("wm_mg3d_create", C.byref(h), self.nx, self.ny, self.nz, self.dx, self.dy,
+                   self.dz, b, self.mgmaxlevels)
+        self._h = h
+        self.initializeconductors()
+        self.timers = dict(deposit=0., solve=0., fetche=0.)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                N.lib().wm_mg3d_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    # arrays (multigrid.py:317-421)
+    @property
+    def phi(self):
+        "Electrostatic potential array, including guard cells"
+        return self._phi
+
+    @phi.setter
+    def phi(self, value):
+        self._phi[...] = value
+
+    @property
+    def rho(self):
+        "Charge density array"
+        return self._rho
+
+    @rho.setter
+    def rho(self, value):
+        self._rho[...] = value
+
+    @property
+    def selfe(self):
+        "Electric field array (3, nx+1, ny+1, nz+1) in Fortran order"
+        self.calcselfe()
+        return self._selfe
+
+    def getphi(self):
+        'Returns the phi array without the guard cells'
+        return self._phi[1:-1, 1:-1, 1:-1]
+
+    def getrho(self):
+        'Returns the rho array without the guard cells'
+        return self._rho
+
+    def getselfe(self, recalculate=None, lzero=True):
+        'Returns the E field array without the guard cells'
+        self.calcselfe(lzero=lzero)
+        return self._selfe
+
+    def calcselfe(self, recalculate=None, lzero=True):
+        """getselfe3d (w3d.F:6023-6109): E = -grad phi by centred
+        differences, stored (nz+1, ny+1, nx+1, 3)."""
+        if self._selfe is None:
+            self._selfe = torch.zeros((self.nz + 1, self.ny + 1, self.nx + 1, 3),
+                                      dtype=torch.float64, device=self.device)
+        N.call("wm_getselfe3d", self._phi.data_ptr(), self.nx, self.ny, self.nz,
+               self._selfe.data_ptr(), self.dx, self.dy, self.dz, int(lzero),
+               N.stream_of(self.device))
+
+    # ------------------------------------------------------------------
+    # conductors (multigrid.py:121-250, 743-825)
+    def initializeconductors(self):
+        self.conductordatalist = []
+        self._conductors = None
+        self._installedconductors = []
+
+    def installconductor(self, conductor, xmin=None, xmax=None, ymin=None, ymax=None,
+                         zmin=None, zmax=None, dfill=None):
+        'Install the conductor into the field solver'
+        if any(v is not None for v in (xmin, xmax, ymin, ymax, zmin, zmax)):
+            raise NotImplementedError("cropping the conductor extent is not supported")
+        self.conductordatalist.append((conductor, dfill))
+
+    def uninstallconductor(self, conductor):
+        n = len(self.conductordatalist)
+        self.conductordatalist = [c for c in self.conductordatalist if c[0] is not conductor]
+        if len(self.conductordatalist) == n:
+            raise Exception('conductor was not found')
+        self.clearconductors()
+
+    def clearconductors(self):
+        "Clear out the conductor data"
+        self._conductors = None
+        self._installedconductors = []
+        N.call("wm_mg3d_set_conductors", self._h, None)
+
+    def hasconductors(self):
+        return len(self.conductordatalist) > 0
+
+    def getconductorobject(self):
+        """Installs conductors not yet installed, uploads the lists to the
+        GPU solver and returns the ConductorData."""
+        pending = [c for c in self.conductordatalist
+                   if not any(c[0] is d for d in self._installedconductors)]
+        if self._conductors is None or pending:
+            if self._conductors is None:
+                self._conductors = ConductorData(self.nx, self.ny, self.nz, self.dx, self.dy,
+                                                 self.dz)
+                pending = list(self.conductordatalist)
+                self._installedconductors = []
+            grid = Grid(self.nx, self.ny, self.nz, self.xmmin, self.xmmax, self.ymmin,
+                        self.ymmax, self.zmmin, self.zmmax)
+            for conductor, dfill in pending:
+                _install(conductor, grid, self._conductors, dfill if dfill is not None else 2.0)
+                self._installedconductors.append(conductor)
+            view = self._conductors.view()
+            N.call("wm_mg3d_set_conductors", self._h, C.addressof(view))
+        return self._conductors
+
+    # ------------------------------------------------------------------
+    # particles -> rho (fieldsolver.py:1333-1422; multigrid.py:472-510)
+    def zerosourcep(self):
+        self._rho.zero_()
+
+    def loadrho(self, lzero=None, lfinalize_rho=None, pgroups=None, **kw):
+        if not self.ldosolve and lzero:
+            return
+        if lzero is None:
+            lzero = w3d.lzerorhofsapi
+        if lfinalize_rho is None:
+            lfinalize_rho = w3d.lfinalizerhofsapi
+        t0 = time.perf_counter()
+        if lzero:
+            self.zerosourcep()
+        if pgroups is None:
+            pgroups = [top.pgroup] if top.pgroup is not None else []
+        jslist = kw.get('jslist', None)
+        for pg in pgroups:
+            for js in (jslist if jslist is not None else range(pg.ns)):
+                if pg.nps[js] == 0:
+                    continue
+                self.setsourcep(js, pg, top.zgrid)
+        self.sourcepfinalized = False
+        if lzero and lfinalize_rho:
+            self.finalizesourcep()
+        self.timers["deposit"] += time.perf_counter() - t0
+
+    def setsourcep(self, js, pgroup, zgrid):
+        sl = pgroup.species_slice(js)
+        x, y, z = pgroup.xp[sl], pgroup.yp[sl], pgroup.zp[sl]
+        q = pgroup.sq[js]
+        w = pgroup.sw[js] * pgroup.dtscale[js]
+        self.setsourcepatposition(x, y, z, zgrid, q, w)
+
+    def setsourcepatposition(self, x, y, z, zgrid, q, w):
+        """setrho3d (w3d.F:7199-7301) on the GPU."""
+        n = int(x.numel())
+        if n == 0:
+            return
+        s = N.stream_of(self.device)
+        if self.sorted_deposit:
+            need = N.lib().wm_setrho3d_sorted_worksize(n, self.nx, self.ny, self.nz)
+            if self._work is None or self._work.numel() < need:
+                self._work = torch.empty(int(need), dtype=torch.uint8, device=self.device)
+            N.call("wm_setrho3d_sorted", self._rho.data_ptr(), n, x.data_ptr(), y.data_ptr(),
+                   z.data_ptr(), zgrid, q, w, self.nx, self.ny, self.nz, self.dx, self.dy,
+                   self.dz, self.xmmin, self.ymmin, self.zmmin, self._work.data_ptr(), s)
+        else:
+            N.call("wm_setrho3d", self._rho.data_ptr(), n, x.data_ptr(), y.data_ptr(),
+                   z.data_ptr(), zgrid, q, w, self.nx, self.ny, self.nz, self.dx, self.dy,
+                   self.dz, self.xmmin, self.ymmin, self.zmmin, None, s)
+
+    def finalizesourcep(self):
+        if self.sourcepfinalized:
+            return
+        self.sourcepfinalized = True
+        self.applysourceboundaryconditions()
+
+    def applysourceboundaryconditions(self):
+        """applyrhoboundaryconditions3d (w3d.F:2660-2750)."""
+        b = (C.c_int64 * 6)(*self.bounds)
+        N.call("wm_applyrhoboundaryconditions3d", self.nx, self.ny, self.nz,
+               self._rho.data_ptr(), b, N.stream_of(self.device))
+
+    # ------------------------------------------------------------------
+    # the solve (fieldsolver.py:1575-1594; multigrid.py:846-914)
+    def solve(self, iwhich=0):
+        if not self.ldosolve:
+            return
+        self.finalizesourcep()
+        self.dosolve(iwhich)
+
+    def getmgverbose(self):
+        if self.mgntverbose < 0:
+            return 0
+        if self.mgntverbose > 1:
+            return 1 if (top.it % self.mgntverbose) == 0 else 0
+        return self.mgverbose
+
+    def dosolve(self, iwhich=0, *args):
+        if not self.l_internal_dosolve:
+            return
+        if iwhich == 1:
+            return
+        t0 = time.perf_counter()
+        if self.conductordatalist:
+            self.getconductorobject()
+        it = C.c_int64(0)
+        err = C.c_double(0.)
+        N.call("wm_mg3d_solve", self._h, self._phi.data_ptr(), self._rho.data_ptr(),
+               self.mgparam, self.mgform, self.mgmaxiters, self.mgtol, self.downpasses,
+               self.uppasses, int(self.lcndbndy), self.icndbndy,
+               PRECISIONS[self.precision], C.byref(it), C.byref(err),
+               N.stream_of(self.device))
+        self.mgiters = it.value
+        self.mgerror = err.value
+        f3d.mgiters, f3d.mgerror = self.mgiters, self.mgerror
+        mgverbose = self.getmgverbose()
+        if mgverbose >= 1 or (mgverbose >= 0 and self.mgerror > self.mgtol):
+            if self.mgerror > self.mgtol:
+                print("Multigrid: Maximum number of iterations reached", file=sys.stderr)
+            print("Multigrid: Error converged to %11.3e in %5d v-cycles"
+                  % (self.mgerror, self.mgiters), file=sys.stderr)
+        self.timers["solve"] += time.perf_counter() - t0
+
+    def getresidual(self):
+        """residual3d (f3d_mgrid.F:3367-3598) of the current phi and rho,
+        scaled by mgparam/(eps0*2*(dxsqi+dysqi+dzsqi)) as the reference's
+        multigrid.py:940-951."""
+        dxsqi, dysqi, dzsqi = 1. / self.dx ** 2, 1. / self.dy ** 2, 1. / self.dz ** 2
+        reps0c = self.mgparam / (eps0 * 2. * (dxsqi + dysqi + dzsqi))
+        rho = self._rho * reps0c
+        res = torch.zeros_like(self._phi)
+        if self.conductordatalist:
+            self.getconductorobject()
+        N.call("wm_mg3d_residual", self._h, 0, self._phi.data_ptr(), rho.data_ptr(),
+               res.data_ptr(), self.mgparam, self.mgform, 0, int(self.lcndbndy),
+               self.icndbndy, N.stream_of(self.device))
+        return res
+
+    # ------------------------------------------------------------------
+    # fields -> particles (fieldsolver.py:1434-1567; multigrid.py:521-575)
+    def fetche(self, *args, **kw):
+        'Fetches the field, uses arrays from the w3d FieldSolveAPI globals'
+        if w3d.npfsapi == 0:
+            return
+        t0 = time.perf_counter()
+        if w3d.xfsapi is not None:
+            x, y, z = w3d.xfsapi, w3d.yfsapi, w3d.zfsapi
+            ex, ey, ez = w3d.exfsapi, w3d.eyfsapi, w3d.ezfsapi
+        else:
+            pg = w3d.pgroupfsapi
+            i = w3d.ipminfsapi - 1
+            sl = slice(i, i + w3d.npfsapi)
+            x, y, z = pg.xp[sl], pg.yp[sl], pg.zp[sl]
+            ex, ey, ez = pg.ex[sl], pg.ey[sl], pg.ez[sl]
+        self.fetchfieldfrompositions(x, y, z, ex, ey, ez)
+        self.timers["fetche"] += time.perf_counter() - t0
+
+    def fetchfieldfrompositions(self, x, y, z, ex, ey, ez, *args, **kw):
+        """sete3d -> sete_from_phi_linear3d (w3d.F:3656-3777, 3963-4107):
+        adds E at the positions to ex, ey, ez."""
+        n = int(x.numel())
+        if n == 0:
+            return
+        for t in (x, y, z, ex, ey, ez):
+            if not t.is_contiguous():
+                raise ValueError("fetche arrays must be contiguous")
+        N.call("wm_sete3d", self._phi.data_ptr(), n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+               top.zgrid, self.xmmin, self.ymmin, self.zmmin, self.dx, self.dy, self.dz,
+               self.nx, self.ny, self.nz, ex.data_ptr(), ey.data_ptr(), ez.data_ptr(),
+               N.stream_of(self.device))
+
+    def fetchphi(self, *args, **kw):
+        'Fetches the potential at w3d.xfsapi.. into w3d.phifsapi'
+        if w3d.npfsapi == 0:
+            return
+        self.fetchpotentialfrompositions(w3d.xfsapi, w3d.yfsapi, w3d.zfsapi, w3d.phifsapi)
+
+    def fetchpotentialfrompositions(self, x, y, z, phi):
+        """getgrid3d (pywarp90/dtop.F:2818-) over the potential including
+        its guard cells (multigrid.py:560-575): trilinear interpolation."""
+        dx, dy, dz = self.dx, self.dy, self.dz
+        nxp, nyp, nzp = self.nx + 2, self.ny + 2, self.nz + 2
+        xmin, xmax = self.xmmin - dx, self.xmmax + dx
+        ymin, ymax = self.ymmin - dy, self.ymmax + dy
+        zmin, zmax = self.zmmin - dz + top.zgrid, self.zmmax + dz + top.zgrid
+        dxi = nxp / (xmax - xmin)
+        dyi = nyp / (ymax - ymin)
+        dzi = nzp / (zmax - zmin)
+        gx = (x - xmin) * dxi
+        gy = (y - ymin) * dyi
+        gz = (z - zmin) * dzi
+        inside = ((gx >= 0.) & (gx < nxp) & (gy >= 0.) & (gy < nyp) & (gz >= 0.) &
+                  (gz < nzp))
+        ix = gx.to(torch.int64).clamp(0, nxp - 1)
+        iy = gy.to(torch.int64).clamp(0, nyp - 1)
+        iz = gz.to(torch.int64).clamp(0, nzp - 1)
+        wx, wy, wz = gx - ix, gy - iy, gz - iz
+        p = self._phi
+        f = (p[iz, iy, ix] * (1. - wx) * (1. - wy) * (1. - wz) +
+             p[iz, iy, ix + 1] * wx * (1. - wy) * (1. - wz) +
+             p[iz, iy + 1, ix] * (1. - wx) * wy * (1. - wz) +
+             p[iz, iy + 1, ix + 1] * wx * wy * (1. - wz) +
+             p[iz + 1, iy, ix] * (1. - wx) * (1. - wy) * wz +
+             p[iz + 1, iy, ix + 1] * wx * (1. - wy) * wz +
+             p[iz + 1, iy + 1, ix] * (1. - wx) * wy * wz +
+             p[iz + 1, iy + 1, ix + 1] * wx * wy * wz)
+        f = torch.where(inside, f, phi)
+        phi.copy_(f)
+
+    # fused fullv advance used by the stepper
+    def gather_push(self, pg, js, dt):
+        """fetche + epush(dt/2) twice + xpush(dt) for species js in one pass
+        (padvnc3d fullv with B = 0, w3d.F:1816, 1956-1986)."""
+        sl = pg.species_slice(js)
+        n = sl.stop - sl.start
+        if n == 0:
+            return
+        col = lambda c: getattr(pg, c)[sl]
+        t0 = time.perf_counter()
+        N.call("wm_gather_push3d", self._phi.data_ptr(), n, col("xp").data_ptr(),
+               col("yp").data_ptr(), col("zp").data_ptr(), col("uxp").data_ptr(),
+               col("uyp").data_ptr(), col("uzp").data_ptr(), col("gaminv").data_ptr(),
+               col("ex").data_ptr(), col("ey").data_ptr(), col("ez").data_ptr(), top.zgrid,
+               self.xmmin, self.ymmin, self.zmmin, self.dx, self.dy, self.dz, self.nx,
+               self.ny, self.nz, pg.sq[js], pg.sm[js], dt, N.stream_of(self.device))
+        self.timers["fetche"] += time.perf_counter() - t0
+
+    def nlevels(self):
+        n = C.c_int64(0)
+        N.call("wm_mg3d_nlevels", self._h, C.byref(n))
+        return n.value
+
+    def level_dims(self, level):
+        nxyz = (C.c_int64 * 3)()
+        dxyz = (C.c_double * 3)()
+        N.call("wm_mg3d_level_dims", self._h, level, nxyz, dxyz)
+        return tuple(nxyz), tuple(dxyz)
+
+
+MultiGrid = MultiGrid3D
