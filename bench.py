@@ -1,0 +1,210 @@
+"""Benchmark of the 3-D electrostatic PIC step on one MI355X.
+
+Workload: 256x256x512 mesh, 32M particles of a seeded KV-like beam, the
+pipe and +10 kV annulus conductors (warp_amd/configs.py "c2").  One step is
+the reference's step3d order on the GPU: gather E + two half kicks + drift
+(padvnc3d fullv), particle boundaries, CIC deposit, multigrid solve to
+mgtol = 1e-6 V warm-started from the previous phi.  Inputs are resident in
+HBM before the timed region.
+
+Prints one JSON line (steps/s, V-cycles and ms per solve, the roofline of
+the dominant kernel, and the FP64 oracle timed on the host as the CPU
+baseline).  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--config", default="c2")
+    p.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-particles", type=int, default=4_000_000,
+                   help="particles in the CPU baseline's sample")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+              "got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from warp_amd import configs, registersolver, unregistersolver, top, _native as N
+    from warp_amd.stepper import step, generate
+
+    dev = torch.device("cuda", 0)
+    cfg = configs.CONFIGS[args.config]
+    t0 = time.time()
+    solver = configs.make_solver(cfg, precision=args.precision, device=dev, mgverbose=-1)
+    pg = configs.load_beam(cfg, device=dev)
+    top.pgroup = pg
+    top.dt = configs.DT
+    registersolver(solver)
+    log("set-up %.1f s: %d particles, mesh %s" % (time.time() - t0, pg.npmax, cfg["n"]))
+    t0 = time.time()
+    generate()
+    torch.cuda.synchronize()
+    cold_iters = solver.mgiters
+    log("first solve (cold start) %d V-cycles, %.2f s incl. conductor set-up"
+        % (cold_iters, time.time() - t0))
+
+    for _ in range(args.warmup):
+        step(1)
+    torch.cuda.synchronize()
+
+    # --- timed region
+    iters, tsolve, tdep, tpush = [], 0., 0., 0.
+    solver.timers = dict(deposit=0., solve=0., fetche=0.)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(1)
+        iters.append(solver.mgiters)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    ms_solve = 1e3 * solver.timers["solve"] / args.steps
+    log("timed %d steps: %.3f ms/step, %.3f ms/solve, V-cycles %s"
+        % (args.steps, ms_per_step, ms_solve, iters))
+
+    # --- dominant kernel: finest-level SOR half sweep, HIP events on the
+    # solver's stream around each launch (one more step, kernels launched
+    # one V-cycle at a time while timing).
+    N.call("wm_mg3d_set_timing", solver._h, 1)
+    phi_before = solver.phi.clone()
+    step(1)
+    torch.cuda.synchronize()
+    ms_tot, nl = C.c_double(0.), C.c_int64(0)
+    N.call("wm_mg3d_get_timing", solver._h, C.byref(ms_tot), C.byref(nl))
+    N.call("wm_mg3d_set_timing", solver._h, 0)
+    nx, ny, nz = cfg["n"]
+    nodes = (nx + 1) * (ny + 1) * (nz + 1)
+    s_bytes = 8
+    alg_bytes = 2 * s_bytes * nodes  # read phi (s) + rho/2 + write phi/2 per node
+    avg_ms = ms_tot.value / max(nl.value, 1)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    log("SOR half sweep: %d launches, %.4f ms avg, %.0f GB/s" % (nl.value, avg_ms, achieved))
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sor_halfpass_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # Copy of known size for scaling the FETCH_SIZE / WRITE_SIZE counters.
+    a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, cfg, solver, pg, phi_before, iters)
+
+    mgtol = solver.mgtol
+    out = {
+        "metric": "PIC steps/s at 256^2x512, 32M ptcls; V-cycles & ms per solve to mgtol=1e-6",
+        "value": 1e3 / ms_per_step,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 V-cycles in f64 defect correction",
+                  "bf16": "f32 V-cycles in f64 defect correction"}[args.precision],
+        "data": "synthetic (seeded KV beam, pipe + 10 kV annulus)",
+        "config": {"workload": "ES PIC step %dx%dx%d, %d particles, subgrid conductors"
+                               % (nx, ny, nz, pg.npmax),
+                   "mesh": [nx, ny, nz], "particles": pg.npmax, "mgtol": mgtol,
+                   "precision": args.precision, "parallelism": "1 GPU"},
+        "vcycles_per_solve": float(np.mean(iters)),
+        "vcycles_cold_solve": cold_iters,
+        "ms_per_solve": ms_solve,
+        "roofline": {"kernel": "k_sor_halfpass (finest level)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    unregistersolver(solver)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, cfg, solver, pg, phi_before, iters):
+    """The FP64 oracle (single-threaded C restatement of the reference) on
+    the host: one step of the same workload with a particle sample of
+    --cpu-particles (deposit and push times scaled to the full count) and
+    the full-mesh solve warm-started from the same phi as the GPU's last
+    solve."""
+    import numpy as np
+    from oracle import oracle as O
+    from warp_amd import configs
+    nx, ny, nz = cfg["n"]
+    x0, x1, y0, y1, z0, z1 = configs.DOMAIN
+    dx, dy, dz = (x1 - x0) / nx, (y1 - y0) / ny, (z1 - z0) / nz
+    npart = pg.npmax
+    ns = min(args.cpu_particles, npart)
+    sl = slice(0, ns)
+    x = pg.xp[sl].cpu().numpy().copy()
+    y = pg.yp[sl].cpu().numpy().copy()
+    z = pg.zp[sl].cpu().numpy().copy()
+    ux = pg.uxp[sl].cpu().numpy().copy()
+    uy = pg.uyp[sl].cpu().numpy().copy()
+    uz = pg.uzp[sl].cpu().numpy().copy()
+    gi = np.ones(ns)
+    q, m, w = pg.sq[0], pg.sm[0], pg.sw[0]
+    co = O.install_conductors(configs.conductors(cfg["conductors"], O), nx, ny, nz, x0, x1,
+                              y0, y1, z0, z1)
+    phi = phi_before.cpu().numpy().copy()
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    t = time.perf_counter()
+    ex, ey, ez = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+    O.sete3d(phi, x, y, z, ex, ey, ez, dx, dy, dz, x0, y0, z0)
+    O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
+    O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
+    O.xpush3d(x, y, z, ux, uy, uz, gi, configs.DT)
+    t_push = time.perf_counter() - t
+    t = time.perf_counter()
+    k = (x >= x0) & (x < x1) & (y >= y0) & (y < y1) & (z >= z0) & (z < z1)
+    O.setrho3d(rho, x[k], y[k], z[k], q, w, dx, dy, dz, x0, y0, z0)
+    t_dep = time.perf_counter() - t
+    rho_full = solver.rho.cpu().numpy().copy()
+    t = time.perf_counter()
+    it, err = O.multigrid3dsolve(phi, rho_full, dx, dy, dz, conductors=co)
+    t_solve = time.perf_counter() - t
+    scale = npart / ns
+    step_s = (t_push + t_dep) * scale + t_solve
+    log("CPU oracle: push %.2f s + deposit %.2f s for %d particles, solve %.2f s (%d V-cycles)"
+        % (t_push, t_dep, ns, t_solve, it))
+    return {"value": 1. / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "one step: gather+push and deposit of %d of the %d particles (times scaled "
+                      "by %.2f), full %dx%dx%d solve warm-started from the GPU's phi (%d V-cycles, "
+                      "%.2f s)" % (ns, npart, scale, nx, ny, nz, it, t_solve),
+            "ms_per_solve": 1e3 * t_solve, "vcycles": it}
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,14 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam,
                   int64_t icndbndy, int64_t precision, int64_t *mgiters,
                   double *mgerror, void *stream);
 
+/* Event timing of the finest-level FP64 SOR half sweep (the solver's
+ * dominant kernel): while on, wm_mg3d_solve launches its kernels one
+ * V-cycle at a time instead of replaying its graph, records HIP events
+ * around every finest-level half-sweep launch on its stream and adds the
+ * elapsed times up.  Setting it resets the totals. */
+int wm_mg3d_set_timing(wm_mg3d *h, int64_t on);
+int wm_mg3d_get_timing(const wm_mg3d *h, double *ms_total, int64_t *launches);
+
 /* Per-kernel entry points on one level, natural-layout device arrays of
  * that level's size.  sorhalfpass3d (f3d_mgrid.F:2095-2291), residual3d
  * (3367-3598), restrict3d (1593-1775, res has 1 guard cell, rhocoarse is

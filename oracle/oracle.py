@@ -433,6 +433,12 @@ def setrho3d(rho, xp, yp, zp, q, wght, dx, dy, dz, xmmin, ymmin, zmmin,
              zgrid=0.0):
     nz, ny, nx = rho.shape[0] - 1, rho.shape[1] - 1, rho.shape[2] - 1
     xp, yp, zp = _f64(xp), _f64(yp), _f64(zp)
+    # The restated loop has no bounds check (neither has the reference);
+    # refuse particles whose cell is outside the mesh.
+    for v, lo, d, n in ((xp, xmmin, dx, nx), (yp, ymmin, dy, ny), (zp - zgrid, zmmin, dz, nz)):
+        s = (v - lo) * (1. / d)
+        if len(s) and not (s.min() >= 0. and s.max() < n):
+            raise ValueError("setrho3d: particles outside the mesh")
     lib().oc_setrho3d(_p(rho), len(xp), _p(xp), _p(yp), _p(zp), zgrid, q, wght,
                       nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin)
 
@@ -447,6 +453,10 @@ def sete3d(phi, xp, yp, zp, ex, ey, ez, dx, dy, dz, xmmin, ymmin, zmmin,
            zgrid=0.0):
     nz, ny, nx = phi.shape[0] - 3, phi.shape[1] - 3, phi.shape[2] - 3
     xp, yp, zp = _f64(xp), _f64(yp), _f64(zp)
+    for v, lo, d, n in ((xp, xmmin, dx, nx), (yp, ymmin, dy, ny), (zp - zgrid, zmmin, dz, nz)):
+        s = (v - lo) * (1. / d)
+        if len(s) and not (s.min() >= 0. and s.max() < n):
+            raise ValueError("sete3d: particles outside the mesh")
     lib().oc_sete3d(_p(phi), len(xp), _p(xp), _p(yp), _p(zp), zgrid, xmmin,
                     ymmin, zmmin, dx, dy, dz, nx, ny, nz, _p(ex), _p(ey), _p(ez))
 

@@ -247,7 +247,16 @@ struct wm_mg3d {
   } gkey;
   int last_iters = 0;
   bool conductors_set = false;
+  // Event timing of the finest-level SOR half sweeps (eager mode only).
+  struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+    double ms = 0.;
+    long long count = 0;
+  } prof;
   ~wm_mg3d() {
+    for (hipEvent_t e : prof.ev) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (state_h) (void)hipHostFree(state_h);
     if (ev_in) (void)hipEventDestroy(ev_in);
@@ -595,11 +604,24 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   int nby = (int)cdiv(nyr, 4);
   int nbdense = nbx * nby * nzr;
   int nbsub = (int)cdiv(sub.n, 256);
+  bool timed = h->prof.on && level == 0 && std::is_same<T, double>::value;
+  if (timed) {
+    while ((int)h->prof.ev.size() < 2 * (h->prof.used + 1)) {
+      hipEvent_t e;
+      WM_HIP(hipEventCreate(&e));
+      h->prof.ev.push_back(e);
+    }
+    WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used], s));
+  }
   hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
                      L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
                      dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
                      nbdense, skip);
   WM_LAUNCHED();
+  if (timed) {
+    WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
+    h->prof.used++;
+  }
   int usevolt = (level == 0 && P.mgform == 1);
   pin_and_mirror<T>(h, L, level, P, phi, usevolt, skip, s);
 }
@@ -1016,13 +1038,22 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     WM_LAUNCHED();
     int iters = 0;
     double err = 2. * mgtol + 1.;
-    static const bool eager = getenv("WARPMI_EAGER") != nullptr;
-    if (mgmaxiters > 0 && eager) {
-      // Plain stream launches, one V-cycle at a time.
+    static const bool eager_env = getenv("WARPMI_EAGER") != nullptr;
+    if (mgmaxiters > 0 && (eager_env || h->prof.on)) {
+      // Plain stream launches, one V-cycle at a time (and the event timing
+      // of the finest-level half sweeps when it is switched on).
       for (int k = 0; k < mgmaxiters; k++) {
+        h->prof.used = 0;
         iteration(h, prec, P, mgtol, (int)mgmaxiters, s);
         WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         WM_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < h->prof.used; i++) {
+          float ms = 0.f;
+          WM_HIP(hipEventElapsedTime(&ms, h->prof.ev[2 * i], h->prof.ev[2 * i + 1]));
+          h->prof.ms += ms;
+          h->prof.count++;
+        }
+        h->prof.used = 0;
         if (h->state_h->done) break;
       }
       iters = h->state_h->iters;
@@ -1086,6 +1117,23 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     WM_HIP(hipStreamSynchronize(s));
     *mgiters = iters;
     *mgerror = err;
+  });
+}
+
+int wm_mg3d_set_timing(wm_mg3d *h, int64_t on) {
+  WM_ENTRY({
+    WM_REQUIRE(h != nullptr, "wm_mg3d_set_timing: NULL handle");
+    h->prof.on = on != 0;
+    h->prof.ms = 0.;
+    h->prof.count = 0;
+  });
+}
+
+int wm_mg3d_get_timing(const wm_mg3d *h, double *ms_total, int64_t *launches) {
+  WM_ENTRY({
+    WM_REQUIRE(h && ms_total && launches, "wm_mg3d_get_timing: NULL argument");
+    *ms_total = h->prof.ms;
+    *launches = h->prof.count;
   });
 }
 
