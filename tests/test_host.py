@@ -1,0 +1,162 @@
+"""CPU checks of the host side: the C ABI library loads and exports every
+entry point of include/warpmi.h, the product's conductor generation (host
+C++ in libwarpmi) equals the oracle's restatement bit for bit, and the
+registered-solver plumbing behaves as fieldsolver.py's."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "warpmi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from warp_amd import _native as N
+    L = N.lib()
+    syms = _header_symbols()
+    assert len(syms) > 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # the ctypes table binds exactly the declared entry points
+    assert sorted(N.SIGNATURES) == syms
+
+
+def test_last_error_is_a_string():
+    from warp_amd import _native as N
+    assert isinstance(N.lib().wm_last_error(), bytes)
+
+
+def _product_lists(geoms, n, dom):
+    from warp_amd.generateconductors import ConductorData, Grid, installconductor
+    nx, ny, nz = n
+    g = Grid(nx, ny, nz, *dom)
+    cd = ConductorData(nx, ny, nz, g.dx, g.dy, g.dz)
+    for c in geoms:
+        installconductor(c, g, cd)
+    return cd.lists()
+
+
+def _oracle_lists(geoms, n, dom):
+    co = O.install_conductors(geoms, *n, *dom)
+    d = ((dom[1] - dom[0]) / n[0], (dom[3] - dom[2]) / n[1], (dom[5] - dom[4]) / n[2])
+    O.checkconductors(co, *n, *d)
+    return co.lists()
+
+
+@pytest.mark.parametrize("kind,n", [("pipe_annulus", (32, 32, 64)), ("pipe_annulus", (64, 64, 128)),
+                                    ("quads", (32, 32, 64)), ("quads", (64, 64, 128))])
+def test_product_conductors_equal_oracle(kind, n):
+    """conductordelfromintercepts + checkconductors in libwarpmi against the
+    oracle: identical interior points, subgrid points, distances and
+    voltages on every level."""
+    import warp_amd
+    from warp_amd import configs
+    a = _product_lists(configs.conductors(kind, warp_amd), n, configs.DOMAIN)
+    b = _oracle_lists(configs.conductors(kind, O), n, configs.DOMAIN)
+    assert a["levels"] == b["levels"]
+    np.testing.assert_array_equal(a["int_istart"], b["int_istart"])
+    np.testing.assert_array_equal(a["int_indx"], b["int_indx"])
+    np.testing.assert_array_equal(a["int_volt"], b["int_volt"])
+    for p in (0, 1):
+        for k in ("indx", "dels", "volt", "istart"):
+            np.testing.assert_array_equal(a["sub%d_%s" % (p, k)], b["sub%d_%s" % (p, k)])
+    assert len(a["sub0_dels"]) > 0 and len(a["int_volt"]) > 0
+
+
+def test_product_mglevels_equal_oracle():
+    from warp_amd.generateconductors import getmglevels
+    for n, d in (((64, 64, 64), (0.04 / 64, 0.04 / 64, 0.08 / 64)),
+                 ((256, 256, 512), (0.04 / 256,) * 2 + (0.08 / 512,)),
+                 ((24, 22, 18), (2 / 24, 2 / 22, 2 / 18)), ((13, 40, 7), (0.1, 0.02, 0.3))):
+        assert getmglevels(*n, *d) == O.getmglevels(*n, *d)
+
+
+def test_registry():
+    """registersolver sets top.fstype = 12 and the *registered calls reach
+    every solver in order (fieldsolver.py:173-229)."""
+    import warp_amd as W
+    calls = []
+
+    class Dummy(W.FieldSolver):
+        def __init__(self, tag, **kw):
+            W.FieldSolver.__init__(self, kwdict=kw)
+            self.tag = tag
+
+        def loadrho(self, *a, **k): calls.append(("loadrho", self.tag))
+        def solve(self, iwhich=0): calls.append(("solve", self.tag))
+        def fetche(self, *a, **k): calls.append(("fetche", self.tag))
+        def fetchphi(self, *a, **k): calls.append(("fetchphi", self.tag))
+
+    a = Dummy("a", nx=8, ny=8, nz=8)
+    b = Dummy("b", nx=8, ny=8, nz=8, bounds=[0, 0, 2, 0, 1, 1])
+    assert b.bounds == [0, 0, 2, 2, 1, 1]   # periodic applies to both faces
+    W.registersolver(a)
+    W.registersolver(b)
+    try:
+        assert W.top.fstype == 12
+        assert W.getregisteredsolvers() == [a, b]
+        W.loadrhoregistered()
+        W.fieldsolregistered()
+        W.fetcheregistered()
+        assert calls == [("loadrho", "a"), ("loadrho", "b"), ("solve", "a"), ("solve", "b"),
+                         ("fetche", "a"), ("fetche", "b")]
+        W.top.fstype = 0
+        with pytest.raises(RuntimeError):
+            W.getregisteredsolvers()
+        W.top.fstype = 12
+    finally:
+        W.unregistersolver(a)
+        W.unregistersolver(b)
+    assert W.top.fstype == -1
+
+
+def test_bad_keyword_rejected():
+    import warp_amd as W
+    with pytest.raises(AssertionError):
+        W.MultiGrid3D(nx=8, ny=8, nz=8, mgtoll=1e-6)
+
+
+def test_solver_fails_loudly_without_gpu():
+    """No CPU fallback: without a HIP device the solver refuses to build."""
+    import torch
+    import warp_amd as W
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    with pytest.raises(RuntimeError, match="HIP device"):
+        W.MultiGrid3D(nx=8, ny=8, nz=8)
+
+
+def test_beam_weights_follow_current_rule():
+    """sw = I * zlen / (v q N) with the peak line charge (w3d.F:7142-7172):
+    Gaussian zlen = sqrt(2 pi) sigma_z; KV uniform zlen = zmax - zmin."""
+    from warp_amd import configs
+    from warp_amd.globals import echarge
+    n = 20000
+    pg = configs.load_beam(configs.CONFIGS["c0"], device="cpu", npart=n)
+    ntot = pg.nps[0]
+    assert 0.99 * n < ntot <= n
+    want = 1e-3 * np.sqrt(2 * np.pi) * 0.015 / (configs.VZ * echarge * n)
+    assert pg.sw[0] == pytest.approx(want, rel=1e-12)
+    x = pg.xp[:ntot].numpy()
+    assert abs(x.std() - 0.004) < 0.0002
+    lo = configs.DOMAIN
+    z = pg.zp[:ntot].numpy()
+    assert x.min() >= lo[0] and x.max() < lo[1] and z.min() >= lo[4] and z.max() < lo[5]
+    # seeded: the same load twice
+    pg2 = configs.load_beam(configs.CONFIGS["c0"], device="cpu", npart=n)
+    assert np.array_equal(pg2.xp.numpy(), pg.xp.numpy())
+    kv = configs.load_beam(configs.CONFIGS["c2"], device="cpu", npart=n)
+    m = kv.nps[0]
+    r2 = (kv.xp[:m].numpy() / 0.006) ** 2 + (kv.yp[:m].numpy() / 0.006) ** 2
+    assert r2.max() <= 1.0
+    assert kv.sw[0] == pytest.approx(1e-3 * 0.06 / (configs.VZ * echarge * n), rel=1e-12)

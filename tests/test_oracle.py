@@ -1,0 +1,488 @@
+"""Pins of the FP64 oracle (CPU restatement of the reference) against the
+reference's own known-answer test and against what the mathematics fixes:
+closed forms, invariants and brute force on tiny inputs.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+
+# ---------------------------------------------------------------------------
+# multigrid3dsolve: the reference's known-answer test
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("bounds", H.all_bounds())
+def test_manufactured_kat(bounds):
+    """warp_test/MutltiGrid3D_test.py: 24x22x18 on [-1,1]^3 for every
+    boundary combination, mgtol 1e-12; the discrete solution is recovered
+    to 1e-10 in fewer than mgmaxiters V-cycles."""
+    nx, ny, nz = 24, 22, 18
+    phie, rho, phi, dx, dy, dz = H.manufactured(nx, ny, nz, bounds)
+    rho0 = rho.copy()
+    it, err = O.multigrid3dsolve(phi, rho, dx, dy, dz, bounds=bounds, mgtol=1e-12)
+    assert it < 100 and err <= 1e-12
+    # rho is scaled and restored in place (f3d_mgrid.F:224-230, 487-494)
+    np.testing.assert_allclose(rho, rho0, rtol=1e-15, atol=0)
+    t, p = phie.copy(), phi.copy()
+    if 0 not in bounds:
+        p -= p[1:-1, 1:-1, 1:-1].mean()
+        t -= t[1:-1, 1:-1, 1:-1].mean()
+    H.clearcorners(t)
+    H.clearcorners(p)
+    assert np.abs(p - t).max() < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# sorhalfpass3d: Gauss-Seidel / SOR by hand on a tiny box
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mgparam", [1.0, 1.37])
+def test_sor_by_hand(mgparam):
+    """Red-black SOR written out point by point: points with
+    (ix+iy+iz) % 2 == parity move to (1-w) phi + w * (Gauss-Seidel value
+    of the 7-point equation -lap(phi) = rho/eps0).  The oracle takes rho
+    pre-scaled by w/(2 eps0 sum(1/d^2)), as multigrid3dsolve hands it."""
+    rng = np.random.default_rng(1)
+    nx, ny, nz = 4, 5, 6
+    dx, dy, dz = 0.3, 0.4, 0.5
+    phi = rng.standard_normal((nz + 3, ny + 3, nx + 3))
+    rho = rng.standard_normal((nz + 1, ny + 1, nx + 1))
+    e0 = O.eps0()
+    wx, wy, wz = 1 / dx ** 2, 1 / dy ** 2, 1 / dz ** 2
+    scaled = rho * mgparam / (2 * e0 * (wx + wy + wz))
+    for parity in (0, 1):
+        ref = phi.copy()
+        for iz in range(1, nz):
+            for iy in range(1, ny):
+                for ix in range(1, nx):
+                    if (ix + iy + iz) % 2 != parity:
+                        continue
+                    P = lambda i, j, k: ref[k + 1, j + 1, i + 1]
+                    gs = (rho[iz, iy, ix] / e0 + wx * (P(ix - 1, iy, iz) + P(ix + 1, iy, iz)) +
+                          wy * (P(ix, iy - 1, iz) + P(ix, iy + 1, iz)) +
+                          wz * (P(ix, iy, iz - 1) + P(ix, iy, iz + 1))) / (2 * (wx + wy + wz))
+                    ref[iz + 1, iy + 1, ix + 1] = (1 - mgparam) * P(ix, iy, iz) + mgparam * gs
+        O.sorhalfpass3d(parity, phi, scaled, dx, dy, dz, (0,) * 6, mgparam=mgparam)
+        np.testing.assert_allclose(phi, ref, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# residual3d
+# ---------------------------------------------------------------------------
+def test_residual_zero_at_discrete_solution():
+    """The residual of the manufactured discrete solution vanishes to
+    rounding; the residual of phi = 0 is the scaled source."""
+    nx, ny, nz = 12, 10, 8
+    bounds = (0,) * 6
+    phie, rho, _, dx, dy, dz = H.manufactured(nx, ny, nz, bounds)
+    wx, wy, wz = 1 / dx ** 2, 1 / dy ** 2, 1 / dz ** 2
+    mgparam = 1.2
+    scaled = rho * mgparam / (2 * O.eps0() * (wx + wy + wz))
+    r = O.residual3d(phie, scaled, dx, dy, dz, bounds, mgparam=mgparam)
+    assert np.abs(r).max() < 1e-12 * np.abs(scaled).max()
+    r0 = O.residual3d(np.zeros_like(phie), scaled, dx, dy, dz, bounds, mgparam=mgparam)
+    np.testing.assert_allclose(r0[2:-2, 2:-2, 2:-2], scaled[1:-1, 1:-1, 1:-1], rtol=1e-15)
+    # Dirichlet faces carry no residual
+    assert not r0[1, :, :].any() and not r0[:, :, 1].any()
+
+
+# ---------------------------------------------------------------------------
+# restrict3d / expand3d
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,nc", [((16, 16, 16), (8, 8, 8)), ((16, 16, 32), (8, 8, 32))])
+def test_restrict_constant_and_linear(n, nc):
+    """Full weighting keeps a constant and reproduces a linear field at the
+    coarse nodes, times ff; Dirichlet coarse faces are zero."""
+    nx, ny, nz = n
+    ncx, ncy, ncz = nc
+    ff = 4.0
+    for bounds in ((1,) * 6, (0,) * 6):
+        res = np.full((nz + 3, ny + 3, nx + 3), 2.5)
+        rc = O.restrict3d(res, ncx, ncy, ncz, ff, bounds)
+        inner = rc if bounds[0] == 1 else rc[1:-1, 1:-1, 1:-1]
+        np.testing.assert_allclose(inner, 2.5 * ff, rtol=1e-14)
+        if bounds[0] == 0:
+            assert not rc[0].any() and not rc[:, :, 0].any() and not rc[:, :, -1].any()
+    # linear field, fine node i at x = i; coarse node I at x = I * nx/ncx
+    k = np.arange(-1, nz + 2)[:, None, None]
+    j = np.arange(-1, ny + 2)[None, :, None]
+    i = np.arange(-1, nx + 2)[None, None, :]
+    res = 0.3 * i - 0.7 * j + 1.1 * k + 5.
+    rc = O.restrict3d(res, ncx, ncy, ncz, ff, (0,) * 6)
+    K = np.arange(ncz + 1)[:, None, None] * (nz / ncz)
+    J = np.arange(ncy + 1)[None, :, None] * (ny / ncy)
+    I = np.arange(ncx + 1)[None, None, :] * (nx / ncx)
+    want = ff * (0.3 * I - 0.7 * J + 1.1 * K + 5.)
+    np.testing.assert_allclose(rc[1:-1, 1:-1, 1:-1], want[1:-1, 1:-1, 1:-1], rtol=1e-13,
+                               atol=1e-13 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("n,nc", [((16, 16, 16), (8, 8, 8)), ((16, 16, 32), (8, 8, 32))])
+def test_expand_linear_exact(n, nc):
+    """Trilinear prolongation adds a linear coarse field exactly at every
+    fine node off the Dirichlet faces."""
+    nx, ny, nz = n
+    ncx, ncy, ncz = nc
+    K = np.arange(-1, ncz + 2)[:, None, None] * (nz / ncz)
+    J = np.arange(-1, ncy + 2)[None, :, None] * (ny / ncy)
+    I = np.arange(-1, ncx + 2)[None, None, :] * (nx / ncx)
+    pc = 0.3 * I - 0.7 * J + 1.1 * K + 5.
+    phi = np.full((nz + 3, ny + 3, nx + 3), 1.0)
+    O.expand3d(phi, pc, (0,) * 6)
+    k = np.arange(-1, nz + 2)[:, None, None]
+    j = np.arange(-1, ny + 2)[None, :, None]
+    i = np.arange(-1, nx + 2)[None, None, :]
+    want = 1.0 + 0.3 * i - 0.7 * j + 1.1 * k + 5.
+    np.testing.assert_allclose(phi[2:-2, 2:-2, 2:-2], want[2:-2, 2:-2, 2:-2], rtol=1e-13,
+                               atol=1e-13 * np.abs(want).max())
+    # Dirichlet faces untouched
+    assert (phi[1] == 1.0).all() and (phi[:, :, -2] == 1.0).all()
+
+
+def test_mglevels_of_the_configs():
+    """getmglevels / getnextcoarselevel3d: z is not coarsened while
+    dz > 4/3 min(d); coarsening stops below 4 cells."""
+    lv = O.getmglevels(64, 64, 64, 0.04 / 64, 0.04 / 64, 0.08 / 64)
+    assert [(l["nx"], l["ny"], l["nz"]) for l in lv] == [
+        (64, 64, 64), (32, 32, 64), (16, 16, 32), (8, 8, 16), (4, 4, 8), (2, 2, 4)]
+    assert len(O.getmglevels(128, 128, 256, 0.04 / 128, 0.04 / 128, 0.08 / 256)) == 7
+    lv = O.getmglevels(256, 256, 512, 0.04 / 256, 0.04 / 256, 0.08 / 512)
+    assert len(lv) == 8 and (lv[-1]["nx"], lv[-1]["nz"]) == (2, 4)
+    # odd sizes round up to even
+    assert O.getnextcoarselevel3d(13, 13, 13, 1., 1., 1.)[:3] == (8, 8, 8)
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions
+# ---------------------------------------------------------------------------
+def test_applyboundaryconditions_rules():
+    """Neumann mirrors u(-1) = u(1), periodic wraps u(-1) = u(n-1),
+    Dirichlet with lwithdirichlet extrapolates u(-1) = 2u(0) - u(1)."""
+    rng = np.random.default_rng(2)
+    nx, ny, nz = 6, 7, 8
+    u = rng.standard_normal((nz + 3, ny + 3, nx + 3))
+    v = u.copy()
+    O.applyboundaryconditions3d(v, (1, 0, 2, 2, 0, 0), lwithdirichlet=True)
+    c = slice(1, -1)
+    np.testing.assert_array_equal(v[c, c, 0], v[c, c, 2])                     # x- Neumann
+    np.testing.assert_allclose(v[c, c, -1], 2 * v[c, c, -2] - v[c, c, -3])    # x+ Dirichlet
+    np.testing.assert_array_equal(v[c, 0, c], v[c, -3, c])                     # y periodic
+    np.testing.assert_array_equal(v[c, -1, c], v[c, 2, c])
+    np.testing.assert_allclose(v[0, c, c], 2 * v[1, c, c] - v[2, c, c])       # z- Dirichlet
+    w = u.copy()
+    O.applyboundaryconditions3d(w, (0,) * 6)
+    np.testing.assert_array_equal(w, u)   # Dirichlet without lwithdirichlet: untouched
+
+
+def test_rho_boundary_conditions():
+    """applyrhoboundaryconditions3d (w3d.F:2660-2750, no guard cells):
+    a Neumann face plane doubles, periodic faces sum into both planes."""
+    rng = np.random.default_rng(4)
+    rho = rng.standard_normal((6, 5, 4))
+    r = rho.copy()
+    O.applyrhoboundaryconditions3d(r, (1, 0, 2, 2, 0, 1))
+    want = rho.copy()
+    want[:, :, 0] *= 2
+    s = want[:, 0, :] + want[:, -1, :]
+    want[:, 0, :] = s
+    want[:, -1, :] = s
+    want[-1, :, :] *= 2
+    np.testing.assert_allclose(r, want, rtol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# conductors: intercepts and subgrid distances against geometry
+# ---------------------------------------------------------------------------
+def _node(L, lev, p, idx):
+    ix, iy, iz = idx
+    return L["x0"] + ix * L["dx"][lev], L["y0"] + iy * L["dy"][lev], L["z0"] + iz * L["dz"][lev]
+
+
+def _check_lists(lists, levels, g0, inside, on_surface):
+    """Every subgrid distance < 1 lands on the surface (on_surface), every
+    interior point is inside (inside), and on level 0 every node outside
+    next to an inside node is a subgrid point (brute force)."""
+    x0, y0, z0, dx, dy, dz = g0
+    dirs = ((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1))
+    nsub = 0
+    for p in (0, 1):
+        ist = lists["sub%d_istart" % p]
+        for lev in range(len(levels)):
+            d = (dx * levels[lev]["lx"], dy * levels[lev]["ly"], dz * levels[lev]["lz"])
+            for k in range(ist[lev], ist[lev + 1]):
+                ix, iy, iz = lists["sub%d_indx" % p][k]
+                pos = np.array([x0 + ix * d[0], y0 + iy * d[1], z0 + iz * d[2]])
+                assert (ix + iy + iz) % 2 == p
+                for m, (ax, sgn) in enumerate(dirs):
+                    dl = lists["sub%d_dels" % p][k, m]
+                    if dl < 1.0:
+                        q = pos.copy()
+                        q[ax] += sgn * dl * d[ax]
+                        assert on_surface(q, d), (lev, (ix, iy, iz), m, dl)
+                        nsub += 1
+    assert nsub > 0
+    ii = lists["int_istart"]
+    for lev in range(len(levels)):
+        d = (dx * levels[lev]["lx"], dy * levels[lev]["ly"], dz * levels[lev]["lz"])
+        for k in range(ii[lev], ii[lev + 1]):
+            ix, iy, iz = lists["int_indx"][k]
+            assert inside(np.array([x0 + ix * d[0], y0 + iy * d[1], z0 + iz * d[2]]))
+
+
+def _brute_adjacent(lists, n, g0, inside):
+    nx, ny, nz = n
+    x0, y0, z0, dx, dy, dz = g0
+    sub = {tuple(v) for v in lists["int_indx"][:lists["int_istart"][1]]}  # pinned
+    for p in (0, 1):
+        e = lists["sub%d_istart" % p][1]
+        sub |= {tuple(v) for v in lists["sub%d_indx" % p][:e]}
+    ins = np.zeros((nz + 1, ny + 1, nx + 1), bool)
+    for iz in range(nz + 1):
+        for iy in range(ny + 1):
+            for ix in range(nx + 1):
+                ins[iz, iy, ix] = inside(np.array([x0 + ix * dx, y0 + iy * dy, z0 + iz * dz]))
+    for iz in range(1, nz):
+        for iy in range(1, ny):
+            for ix in range(1, nx):
+                if ins[iz, iy, ix]:
+                    continue
+                nb = (ins[iz, iy, ix - 1] or ins[iz, iy, ix + 1] or ins[iz, iy - 1, ix] or
+                      ins[iz, iy + 1, ix] or ins[iz - 1, iy, ix] or ins[iz + 1, iy, ix])
+                if nb:
+                    assert (ix, iy, iz) in sub, (ix, iy, iz)
+
+
+def _install(geoms, n, dom):
+    co = O.install_conductors(geoms, *n, *dom)
+    d = ((dom[1] - dom[0]) / n[0], (dom[3] - dom[2]) / n[1], (dom[5] - dom[4]) / n[2])
+    O.checkconductors(co, *n, *d)
+    levels = O.getmglevels(*n, *d)
+    return co.lists(), levels, (dom[0], dom[2], dom[4]) + d
+
+
+def test_zcylinder_intercepts_on_circle():
+    """ZCylinder (f3d_conductors.F:2496-): x-line intercepts at
+    xc +- sqrt(r^2 - (y-yc)^2), z intercepts on the end caps."""
+    n, dom = (16, 16, 32), H.DOMAIN
+    r, L, xc, yc, zc = 0.0061, 0.031, 0.0011, -0.0017, 0.0403
+    lists, levels, g0 = _install([O.ZCylinder(r, L, 7., xcent=xc, ycent=yc, zcent=zc)], n, dom)
+
+    def inside(q):
+        return (q[0] - xc) ** 2 + (q[1] - yc) ** 2 < r * r and abs(q[2] - zc) < L / 2
+
+    def on_surface(q, d):
+        rr = np.hypot(q[0] - xc, q[1] - yc)
+        on_side = abs(rr - r) < 1e-12 and abs(q[2] - zc) <= L / 2 + 1e-12
+        on_cap = abs(abs(q[2] - zc) - L / 2) < 1e-12 and rr <= r + 1e-12
+        return on_side or on_cap
+    _check_lists(lists, levels, g0, inside, on_surface)
+    _brute_adjacent(lists, n, g0, inside)
+    assert (lists["int_volt"] == 7.).all()
+
+
+def test_zcylinderout_and_annulus_intercepts():
+    """ZCylinderOut (f3d_conductors.F:3158-3256) and the ZSrfrv annulus:
+    intercepts on r = rout, r = rin, rmax and the planes z = zmin, zmax."""
+    n, dom = (16, 16, 32), H.DOMAIN
+    lists, levels, g0 = _install(H.pipe_and_annulus(O), n, dom)
+
+    def inside(q, e=1e-12):
+        # e > 0 counts nodes on the surface as inside (they are pinned)
+        rr = np.hypot(q[0], q[1])
+        if rr > H.PIPE_R - e:
+            return True
+        return H.ANN_RIN - e < rr < H.ANN_ROUT + e and H.ANN_Z0 - e < q[2] < H.ANN_Z1 + e
+
+    def on_surface(q, d):
+        rr = np.hypot(q[0], q[1])
+        if abs(rr - H.PIPE_R) < 1e-12:
+            return True
+        if abs(rr - H.ANN_RIN) < 1e-12 and H.ANN_Z0 - 1e-12 <= q[2] <= H.ANN_Z1 + 1e-12:
+            return True
+        return (min(abs(q[2] - H.ANN_Z0), abs(q[2] - H.ANN_Z1)) < 1e-12 and
+                H.ANN_RIN - 1e-12 <= rr <= H.ANN_ROUT + 1e-12)
+    _check_lists(lists, levels, g0, inside, on_surface)
+    # a node one cell from a surface node needs no subgrid stencil
+    _brute_adjacent(lists, n, g0, lambda q: inside(q, -1e-12))
+
+
+# ---------------------------------------------------------------------------
+# conductors in the solve: closed forms
+# ---------------------------------------------------------------------------
+def _coax_error(n):
+    nx = ny = n
+    nz = 4
+    dom = (-0.02, 0.02, -0.02, 0.02, 0.0, 0.01)
+    a, b, V, xc, yc = 0.005, 0.018, 1000., 0.0003, -0.0002
+    g = [O.ZCylinder(a, 0.1, V, xcent=xc, ycent=yc, zcent=0.005, condid=1),
+         O.ZCylinderOut(b, 0.1, 0., xcent=xc, ycent=yc, zcent=0.005, condid=2)]
+    co = O.install_conductors(g, nx, ny, nz, *dom)
+    dx, dy, dz = 0.04 / nx, 0.04 / ny, 0.01 / nz
+    phi = np.zeros((nz + 3, ny + 3, nx + 3))
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    it, err = O.multigrid3dsolve(phi, rho, dx, dy, dz, bounds=(0, 0, 0, 0, 1, 1),
+                                 conductors=co, mgtol=1e-9, mgmaxiters=500)
+    assert err <= 1e-9
+    x = dom[0] + dx * np.arange(nx + 1)
+    y = dom[2] + dy * np.arange(ny + 1)
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    r = np.hypot(X - xc, Y - yc)
+    exact = V * np.log(r / b) / np.log(a / b)
+    m = (r > a) & (r < b)
+    # uniform in z (Neumann ends)
+    assert np.abs(phi[1:-1, 1:-1, 1:-1] - phi[3, 1:-1, 1:-1][None]).max() < 1e-8
+    return np.abs(phi[3, 1:-1, 1:-1] - exact)[m].max()
+
+
+def test_coax_second_order():
+    """Shortley-Weller subgrid boundaries: the coaxial potential
+    V ln(r/b)/ln(a/b) converges at second order under refinement."""
+    e32, e64, e128 = _coax_error(32), _coax_error(64), _coax_error(128)
+    assert e32 < 2e-3 * 1000.
+    assert e32 / e64 > 3.5 and e64 / e128 > 3.5
+
+
+@pytest.mark.parametrize("nz,z0", [(32, 0.05123), (16, 0.0313)])
+def test_plane_at_fractional_offset_is_exactly_linear(nz, z0):
+    """A conducting slab z >= z0 at V over phi(z=0) = 0 with Neumann x, y:
+    the subgrid stencil reproduces phi = V z / z0 to round-off."""
+    nx = ny = 4
+    dom = (-0.02, 0.02, -0.02, 0.02, 0.0, 0.08)
+    V = 100.
+    co = O.install_conductors([O.ZCylinder(1.0, 0.3, V, zcent=z0 + 0.15, condid=1)],
+                              nx, ny, nz, *dom)
+    dz = 0.08 / nz
+    phi = np.zeros((nz + 3, ny + 3, nx + 3))
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    it, err = O.multigrid3dsolve(phi, rho, 0.04 / nx, 0.04 / ny, dz, bounds=(1, 1, 1, 1, 0, 0),
+                                 conductors=co, mgtol=1e-11, mgmaxiters=2000)
+    z = dz * np.arange(nz + 1)
+    k = z < z0
+    want = (V * z[k] / z0)[:, None, None]
+    assert np.abs(phi[1:-1, 1:-1, 1:-1][k] - want).max() < 1e-10
+
+
+def test_reflection_symmetry():
+    """Pipe + annulus with a source symmetric under x -> -x gives a
+    potential symmetric to the solver tolerance."""
+    nx, ny, nz = 32, 32, 64
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    dx, dy, dz = 0.04 / nx, 0.04 / ny, 0.08 / nz
+    rng = np.random.default_rng(9)
+    x = rng.normal(0, 0.004, 5000)
+    x = np.concatenate([x, -x])
+    y = np.tile(rng.normal(0, 0.004, 5000), 2)
+    z = np.tile(0.04 + rng.normal(0, 0.01, 5000), 2)
+    k = (np.abs(x) < 0.019) & (np.abs(y) < 0.019) & (z > 0.001) & (z < 0.079)
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    O.setrho3d(rho, x[k], y[k], z[k], 1.6e-19, 1e6, dx, dy, dz, -0.02, -0.02, 0.)
+    np.testing.assert_allclose(rho, rho[:, :, ::-1], rtol=1e-12, atol=1e-12 * rho.max())
+    phi = np.zeros((nz + 3, ny + 3, nx + 3))
+    it, err = O.multigrid3dsolve(phi, rho, dx, dy, dz, conductors=co, mgtol=1e-8,
+                                 mgmaxiters=300)
+    assert err <= 1e-8
+    assert np.abs(phi - phi[:, :, ::-1]).max() < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# particles
+# ---------------------------------------------------------------------------
+def _cic_brute(rho, x, y, z, q, w, dx, dy, dz, x0, y0, z0):
+    for xp, yp, zp in zip(x, y, z):
+        fx, fy, fz = (xp - x0) / dx, (yp - y0) / dy, (zp - z0) / dz
+        i, j, k = int(np.floor(fx)), int(np.floor(fy)), int(np.floor(fz))
+        u, v, t = fx - i, fy - j, fz - k
+        for a, wa in ((0, 1 - u), (1, u)):
+            for b, wb in ((0, 1 - v), (1, v)):
+                for c, wc in ((0, 1 - t), (1, t)):
+                    rho[k + c, j + b, i + a] += q * w * wa * wb * wc / (dx * dy * dz)
+
+
+def test_deposit_brute_force_and_charge():
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 4, 5, 6
+    dx, dy, dz = 0.1, 0.2, 0.3
+    x0, y0, z0 = -0.2, 0.1, 1.0
+    npart = 200
+    x = x0 + rng.uniform(0, nx * dx, npart) * 0.999
+    y = y0 + rng.uniform(0, ny * dy, npart) * 0.999
+    z = z0 + rng.uniform(0, nz * dz, npart) * 0.999
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    O.setrho3d(rho, x, y, z, 2.0, 3.0, dx, dy, dz, x0, y0, z0)
+    ref = np.zeros_like(rho)
+    _cic_brute(ref, x, y, z, 2.0, 3.0, dx, dy, dz, x0, y0, z0)
+    np.testing.assert_allclose(rho, ref, rtol=1e-12, atol=1e-12 * ref.max())
+    assert abs(rho.sum() * dx * dy * dz - 2.0 * 3.0 * npart) < 1e-10 * npart
+    # a particle on a node puts all its charge there (dyadic mesh: exact)
+    dx, dy, dz, x0, y0, z0 = 0.125, 0.25, 0.5, -0.5, 0.25, 1.0
+    r1 = np.zeros_like(rho)
+    O.setrho3d(r1, [x0 + 2 * dx], [y0 + 3 * dy], [z0 + 1 * dz], 1.0, 1.0, dx, dy, dz, x0, y0, z0)
+    assert r1[1, 3, 2] == pytest.approx(1. / (dx * dy * dz), rel=1e-12)
+    assert np.count_nonzero(r1) == 1
+
+
+def test_gather_linear_field_exact():
+    """sete_from_phi_linear3d: a linear phi gives E = -grad phi at any
+    position; a particle on a node gets the centred difference there."""
+    rng = np.random.default_rng(6)
+    nx, ny, nz = 6, 7, 8
+    dx, dy, dz = 0.1, 0.2, 0.15
+    x0, y0, z0 = 0.3, -0.1, 0.0
+    K = np.arange(-1, nz + 2)[:, None, None] * dz + z0
+    J = np.arange(-1, ny + 2)[None, :, None] * dy + y0
+    I = np.arange(-1, nx + 2)[None, None, :] * dx + x0
+    phi = 2.0 * I - 3.0 * J + 0.5 * K + 1.0
+    n = 100
+    x = x0 + rng.uniform(0, nx * dx * 0.999, n)
+    y = y0 + rng.uniform(0, ny * dy * 0.999, n)
+    z = z0 + rng.uniform(0, nz * dz * 0.999, n)
+    ex, ey, ez = np.zeros(n), np.zeros(n), np.zeros(n)
+    O.sete3d(phi, x, y, z, ex, ey, ez, dx, dy, dz, x0, y0, z0)
+    np.testing.assert_allclose(ex, -2.0, rtol=1e-12)
+    np.testing.assert_allclose(ey, 3.0, rtol=1e-12)
+    np.testing.assert_allclose(ez, -0.5, rtol=1e-12)
+    # accumulates into E (w3d.F:4103-4105)
+    O.sete3d(phi, x, y, z, ex, ey, ez, dx, dy, dz, x0, y0, z0)
+    np.testing.assert_allclose(ex, -4.0, rtol=1e-12)
+    p2 = rng.standard_normal(phi.shape)
+    e = [np.zeros(1) for _ in range(3)]
+    O.sete3d(p2, [x0 + 2 * dx], [y0 + 3 * dy], [z0 + 4 * dz], *e, dx, dy, dz, x0, y0, z0)
+    P = lambda i, j, k: p2[k + 1, j + 1, i + 1]
+    assert e[0][0] == pytest.approx((P(1, 3, 4) - P(3, 3, 4)) / (2 * dx), rel=1e-12)
+    assert e[1][0] == pytest.approx((P(2, 2, 4) - P(2, 4, 4)) / (2 * dy), rel=1e-12)
+    assert e[2][0] == pytest.approx((P(2, 3, 3) - P(2, 3, 5)) / (2 * dz), rel=1e-12)
+
+
+def test_leapfrog_constant_field_exact():
+    """fullv ordering (half kick, half kick, drift) in a constant field:
+    after k steps x = x0 + k u0 dt + a dt^2 k (k+1)/2, u = u0 + k a dt."""
+    q, m, dt = 1.6e-19, 6.5e-26, 1e-9
+    E = np.array([1.0e4, -2.0e3, 5.0e2])
+    a = q * E / m
+    x = np.array([0.001]); y = np.array([-0.002]); z = np.array([0.03])
+    ux = np.array([10.]); uy = np.array([-20.]); uz = np.array([1e6])
+    X0 = np.array([x[0], y[0], z[0]])
+    U0 = np.array([ux[0], uy[0], uz[0]])
+    g = np.ones(1)
+    kmax = 50
+    for _ in range(kmax):
+        ex, ey, ez = (np.full(1, E[0]), np.full(1, E[1]), np.full(1, E[2]))
+        O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * dt)
+        O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * dt)
+        O.xpush3d(x, y, z, ux, uy, uz, g, dt)
+    want_u = U0 + kmax * a * dt
+    want_x = X0 + kmax * U0 * dt + a * dt * dt * kmax * (kmax + 1) / 2
+    np.testing.assert_allclose([ux[0], uy[0], uz[0]], want_u, rtol=1e-12)
+    np.testing.assert_allclose([x[0], y[0], z[0]], want_x, rtol=1e-12, atol=1e-15)
+
+
+def test_getselfe_linear():
+    nx, ny, nz = 5, 6, 7
+    dx, dy, dz = 0.2, 0.1, 0.3
+    K = np.arange(-1, nz + 2)[:, None, None] * dz
+    J = np.arange(-1, ny + 2)[None, :, None] * dy
+    I = np.arange(-1, nx + 2)[None, None, :] * dx
+    phi = -1.5 * I + 2.5 * J + 4.0 * K
+    se = O.getselfe3d(phi, dx, dy, dz)
+    np.testing.assert_allclose(se[..., 0], 1.5, rtol=1e-12)
+    np.testing.assert_allclose(se[..., 1], -2.5, rtol=1e-12)
+    np.testing.assert_allclose(se[..., 2], -4.0, rtol=1e-12)

@@ -44,11 +44,13 @@ class MultiGrid3D(FieldSolver):
             raise ValueError("precision must be one of %s" % list(PRECISIONS))
         self.precision = precision
         self.sorted_deposit = sorted_deposit
+        if not torch.cuda.is_available():
+            raise RuntimeError("MultiGrid3D needs a HIP device (torch.cuda is not available)")
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = torch.device(device)
-        if self.device.type != "cuda" or not torch.cuda.is_available():
-            raise RuntimeError("MultiGrid3D needs a HIP device (torch.cuda is not available)")
+        if self.device.type != "cuda":
+            raise RuntimeError("MultiGrid3D needs a HIP device, got %s" % self.device)
         N.lib()
         self.solvergeom = "XYZgeom"
         self.mgiters = 0
