@@ -181,14 +181,41 @@ int wm_xpush3d(int64_t np, double *xp, double *yp, double *zp,
 
 /* The "fullv" advance of padvnc3d for B = 0 (w3d.F:1816, 1956-1986):
  * E zeroed and fetched (sete3d), epush3d(dt/2) twice, xpush3d(dt), in one
- * pass.  ex, ey, ez may be NULL; when given they receive E. */
+ * pass.  ex, ey, ez may be NULL; when given they receive E.  nlost (device
+ * memory, may be NULL) is incremented by the number of particles whose new
+ * position fails particleboundaries3d's absorbing test
+ * x in [xmmin, xmmax), y in [ymmin, ymmax), z in [zmmin+zgrid, zmmax+zgrid)
+ * (w3d.F:11292-). */
 int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp,
                      double *zp, double *uxp, double *uyp, double *uzp,
                      const double *gaminv, double *ex, double *ey,
                      double *ez, double zgrid, double xmmin, double ymmin,
                      double zmmin, double dx, double dy, double dz,
                      int64_t nx, int64_t ny, int64_t nz, double q, double m,
-                     double dt, void *stream);
+                     double dt, double xmmax, double ymmax, double zmmax,
+                     int64_t *nlost, void *stream);
+
+/* particlesortxyzwithcopy (top.F:5095-5138) with particlesortbyindex
+ * (top.F:5140-5203): the permutation that orders np particles by the block
+ * index int(|x-xmmin|/dx) + (nx+1)*(iy + (ny+1)*iz), stable, so equal
+ * blocks keep their order as in the reference's counting sort.  perm[i]
+ * (device, np entries) is the old index of the particle that goes to
+ * position i.  work must hold wm_particle_sort_worksize() bytes.  Indices
+ * outside [0, n] are clamped (the reference requires in-grid particles). */
+int64_t wm_particle_sort_worksize(int64_t np, int64_t nx, int64_t ny,
+                                  int64_t nz);
+int wm_particle_sort(int64_t np, const double *xp, const double *yp,
+                     const double *zp, double xmmin, double ymmin,
+                     double zmmin, double dx, double dy, double dz,
+                     int64_t nx, int64_t ny, int64_t nz, uint32_t *perm,
+                     void *work, void *stream);
+
+/* particlecopysorteddata (top.F:5205-5222) for up to 16 columns at once:
+ * dst[c][i] = src[c][perm[i]].  src and dst are host arrays of ncol device
+ * pointers; dst columns must not alias src columns. */
+int wm_permute_columns(int64_t np, int64_t ncol, const double *const *src,
+                       double *const *dst, const uint32_t *perm,
+                       void *stream);
 
 /* getselfe3d (w3d.F:6023-6109), nxguarde = 0. */
 int wm_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,

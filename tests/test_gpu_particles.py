@@ -28,12 +28,26 @@ def _t(a, device):
     return torch.from_numpy(np.ascontiguousarray(a)).to(device)
 
 
-@pytest.mark.parametrize("sorted_", [False, True])
-def test_deposit_matches_oracle(device, sorted_):
+def _cell_order(x, y, z):
+    """Stable order by CIC cell (x fastest), as after particlesortxyzwithcopy."""
+    ix = ((x - LO[0]) * (1. / DX)).astype(np.int64)
+    iy = ((y - LO[1]) * (1. / DY)).astype(np.int64)
+    iz = ((z - LO[2]) * (1. / DZ)).astype(np.int64)
+    return np.argsort(ix + (NX + 1) * (iy + (NY + 1) * iz), kind="stable")
+
+
+@pytest.mark.parametrize("order", ["random", "sorted_call", "cell_ordered"])
+def test_deposit_matches_oracle(device, order):
     """setrho3d: the GPU sums in another order, so rho agrees to rounding
-    (relative 1e-13 of the peak) and the total charge is conserved."""
+    (relative 1e-13 of the peak) and the total charge is conserved.
+    Random order (every lane its own run), a sort inside the call, and
+    particles already in cell order (long runs reduced across the wave)."""
     from warp_amd import _native as N
     x, y, z = _beam(200000, 1)
+    if order == "cell_ordered":
+        p = _cell_order(x, y, z)
+        x, y, z = x[p], y[p], z[p]
+    sorted_ = order == "sorted_call"
     q, w = 1.602176565e-19, 3.7e4
     rho = torch.zeros((NZ + 1, NY + 1, NX + 1), dtype=torch.float64, device=device)
     args = [rho.data_ptr(), len(x), _t(x, device).data_ptr(), _t(y, device).data_ptr(),
@@ -70,6 +84,64 @@ def test_deposit_skips_out_of_grid(device):
            N.stream_of(device))
     assert ns.value == 3
     assert abs(rho.sum().item() - 1.0 / (DX * DY * DZ)) < 1e-6
+
+
+def test_deposit_runs_with_outside_particles(device):
+    """Cell-ordered particles interleaved with out-of-grid ones: runs are
+    cut at every skipped particle and the rest still sums exactly."""
+    from warp_amd import _native as N
+    x, y, z = _beam(20000, 11)
+    p = _cell_order(x, y, z)
+    x, y, z = x[p], y[p], z[p]
+    xo = x.copy()
+    xo[::7] = 0.5            # every 7th particle outside the mesh
+    keep = [_t(v, device) for v in (xo, y, z)]
+    rho = torch.zeros((NZ + 1, NY + 1, NX + 1), dtype=torch.float64, device=device)
+    ns = C.c_int64(-1)
+    N.call("wm_setrho3d", rho.data_ptr(), len(x), keep[0].data_ptr(), keep[1].data_ptr(),
+           keep[2].data_ptr(), 0.0, 1.0, 2.0, NX, NY, NZ, DX, DY, DZ, *LO, C.byref(ns),
+           N.stream_of(device))
+    inside = np.ones(len(x), bool)
+    inside[::7] = False
+    assert ns.value == (~inside).sum()
+    ro = np.zeros((NZ + 1, NY + 1, NX + 1))
+    O.setrho3d(ro, x[inside], y[inside], z[inside], 1.0, 2.0, DX, DY, DZ, *LO)
+    assert np.abs(rho.cpu().numpy() - ro).max() <= 1e-13 * np.abs(ro).max()
+
+
+def test_particle_sort_matches_counting_sort(device):
+    """particlesortxyzwithcopy: the permutation equals a stable sort by the
+    reference's block index int(|x-xmmin|/dx) + (nx+1)*(iy + (ny+1)*iz),
+    and every column is permuted with it."""
+    from warp_amd import _native as N
+    from warp_amd.particles import ParticleGroup, particlesortxyzwithcopy
+    x, y, z = _beam(300000, 12)
+    n = len(x)
+    rng = np.random.default_rng(13)
+    u = [rng.normal(0, 1e3, n) for _ in range(3)]
+    key = ((np.abs(x - LO[0]) / DX).astype(np.int64) +
+           (NX + 1) * ((np.abs(y - LO[1]) / DY).astype(np.int64) +
+                       (NY + 1) * (np.abs(z - LO[2]) / DZ).astype(np.int64)))
+    want = np.argsort(key, kind="stable")
+    perm = torch.empty(n, dtype=torch.int32, device=device)
+    work = torch.empty(N.lib().wm_particle_sort_worksize(n, NX, NY, NZ), dtype=torch.uint8,
+                       device=device)
+    g = [_t(v, device) for v in (x, y, z)]
+    N.call("wm_particle_sort", n, *[a.data_ptr() for a in g], *LO, DX, DY, DZ, NX, NY, NZ,
+           perm.data_ptr(), work.data_ptr(), N.stream_of(device))
+    np.testing.assert_array_equal(perm.cpu().numpy(), want)
+    # two species: each sorted within its own slice
+    pg = ParticleGroup(device)
+    h = n // 3
+    for a, b in ((0, h), (h, n)):
+        js = pg.addspecies(1.0, 1.0, 1.0)
+        pg.addparticles(js, x[a:b], y[a:b], z[a:b], u[0][a:b], u[1][a:b], u[2][a:b])
+    particlesortxyzwithcopy(pg, DX, DY, DZ, *LO, NX, NY, NZ)
+    for a, b in ((0, h), (h, n)):
+        p = a + np.argsort(key[a:b], kind="stable")
+        np.testing.assert_array_equal(pg.xp[a:b].cpu().numpy(), x[p])
+        np.testing.assert_array_equal(pg.zp[a:b].cpu().numpy(), z[p])
+        np.testing.assert_array_equal(pg.uyp[a:b].cpu().numpy(), u[1][p])
 
 
 def _phi(seed):
@@ -130,9 +202,10 @@ def test_gather_push_matches_sequence(device):
     g = [_t(a, device) for a in (x, y, z)] + [_t(a, device) for a in u]
     gi = torch.ones(n, dtype=torch.float64, device=device)
     E = [torch.zeros(n, dtype=torch.float64, device=device) for _ in range(3)]
+    nlost = torch.zeros(1, dtype=torch.int64, device=device)
     N.call("wm_gather_push3d", _t(phi, device).data_ptr(), n, *[a.data_ptr() for a in g],
            gi.data_ptr(), *[a.data_ptr() for a in E], 0.0, *LO, DX, DY, DZ, NX, NY, NZ, q, m,
-           dt, N.stream_of(device))
+           dt, 0.02, 0.02, 0.08, nlost.data_ptr(), N.stream_of(device))
     xo, uo = [x.copy(), y.copy(), z.copy()], [a.copy() for a in u]
     eo = [np.zeros(n) for _ in range(3)]
     O.sete3d(phi, *xo, *eo, DX, DY, DZ, *LO)
@@ -141,6 +214,9 @@ def test_gather_push_matches_sequence(device):
     O.xpush3d(*xo, *uo, np.ones(n), dt)
     for a, b in zip(g + E, xo + uo + eo):
         np.testing.assert_array_equal(a.cpu().numpy(), b)
+    X, Y, Z = xo
+    out = ~((X >= LO[0]) & (X < 0.02) & (Y >= LO[1]) & (Y < 0.02) & (Z >= 0.) & (Z < 0.08))
+    assert int(nlost.item()) == int(out.sum())
 
 
 def test_getselfe_bitwise(device):

@@ -69,7 +69,12 @@ SIGNATURES = {
     "wm_epush3d": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _d, _d, _d, _P]),
     "wm_xpush3d": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _P, _d, _P]),
     "wm_gather_push3d": (C.c_int, [_P, _i64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _d,
-                                   _d, _d, _d, _d, _d, _d, _i64, _i64, _i64, _d, _d, _d, _P]),
+                                   _d, _d, _d, _d, _d, _d, _i64, _i64, _i64, _d, _d, _d,
+                                   _d, _d, _d, _P, _P]),
+    "wm_particle_sort_worksize": (_i64, [_i64, _i64, _i64, _i64]),
+    "wm_particle_sort": (C.c_int, [_i64, _P, _P, _P, _d, _d, _d, _d, _d, _d, _i64, _i64, _i64,
+                                   _P, _P, _P]),
+    "wm_permute_columns": (C.c_int, [_i64, _i64, _P, _P, _P, _P]),
     "wm_getselfe3d": (C.c_int, [_P, _i64, _i64, _i64, _P, _d, _d, _d, _i64, _P]),
     "wm_zcylinder_intercepts": (C.c_int, [_d, _d, _d, _d, _d, _P, _d, _i64]),
     "wm_zsrfrv_intercepts": (C.c_int, [_i64, _P, _P, _d, _d, _d, _P]),

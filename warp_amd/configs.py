@@ -11,7 +11,8 @@ Choices the configuration text leaves open, fixed here:
   * the annulus electrode (ZAnnulus, r_in = 1.2 cm, +10 kV, z in [3, 3.5]
     cm) has r_out = 1.8 cm, touching the pipe;
   * the KV-like beam of the 256^2 x 512 problem is uniform in z over
-    [1, 7] cm and carries 1 mA;
+    [0.5, 6.5] cm (its head reaches z = 8 cm after 15 steps of 1 mm) and
+    carries 1 mA;
   * the quadrupole channel's rods are 1.5 cm long, one focusing and one
     defocusing rod set per 4 cm period.
 """
@@ -72,7 +73,7 @@ def load_beam(cfg, device="cuda", npart=None):
                                   charge=echarge, mass=MASS, current=cfg["current"],
                                   device=device)
     else:
-        pg, js, n = load_kv_ellipse(npart, 0.006, 0.006, 0.01, 0.07, VZ, VSPREAD, DOMAIN, seed,
+        pg, js, n = load_kv_ellipse(npart, 0.006, 0.006, 0.005, 0.065, VZ, VSPREAD, DOMAIN, seed,
                                     charge=echarge, mass=MASS, current=cfg["current"],
                                     device=device)
     return pg

@@ -211,6 +211,9 @@ struct Level {
   DevBuf<double> int_volt;
   DevBuf<uint32_t> mask;
   bool hasmask = false;
+  // an active subgrid record sits on an interior (pinned) point, so the
+  // pin after each half sweep is not a no-op
+  bool sub_on_interior = false;
   std::vector<int> subpos_h[2];
   std::vector<double> subdels_h[2], subvolt_h[2];
   DevBuf<int> subpos_d[2];
@@ -497,10 +500,13 @@ static void build_mask(wm_mg3d *h, Level &L, int level, const Params &P) {
     for (int i = 0; i < L.nint; i++) setbit(hp[3 * i], hp[3 * i + 1], hp[3 * i + 2]);
     any = true;
   }
+  L.sub_on_interior = false;
   if (P.lcnd) {
     for (int p = 0; p < 2; p++) {
       SubHost sh = build_records(L, level, p, h->bounds, P.mgparam, 1, P.icnd, false, false);
       for (int i = 0; i < sh.n(); i++) {
+        long long e = rb_index(L.g, sh.pos[3 * i], sh.pos[3 * i + 1], sh.pos[3 * i + 2]);
+        if ((m[e >> 5] >> (e & 31)) & 1u) L.sub_on_interior = true;
         setbit(sh.pos[3 * i], sh.pos[3 * i + 1], sh.pos[3 * i + 2]);
         any = true;
       }
@@ -589,7 +595,7 @@ static void pin_and_mirror(wm_mg3d *h, Level &L, int level, const Params &P, T *
 // sorhalfpass3d (f3d_mgrid.F:2095-2291) on level data in red-black layout.
 template <typename T>
 static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
-                     const int *skip, hipStream_t s) {
+                     const int *skip, hipStream_t s, bool inpass = true) {
   T *phi = phi_of<T>(L).p;
   const T *rho = rho_of<T>(L).p;
   LoopRange r = sor_range(L, h->bounds);
@@ -622,8 +628,17 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
     h->prof.used++;
   }
-  int usevolt = (level == 0 && P.mgform == 1);
-  pin_and_mirror<T>(h, L, level, P, phi, usevolt, skip, s);
+  // cond_potmg after the sweep (f3d_mgrid.F:2276-2285).  The dense sweep
+  // skips every interior point and the pass starts with a pin, so the
+  // pin is a no-op unless a subgrid record or a periodic wrap can write an
+  // interior point, or Neumann subgrid ghosts must be mirrored.  A half
+  // sweep called alone (inpass false) always pins.
+  const int *b = h->bounds;
+  bool periodic = b[0] == 2 || b[2] == 2 || b[4] == 2;
+  if (!inpass || L.sub_on_interior || periodic || (P.icnd == 2 && L.neumann)) {
+    int usevolt = (level == 0 && P.mgform == 1);
+    pin_and_mirror<T>(h, L, level, P, phi, usevolt, skip, s);
+  }
 }
 
 template <typename T>
@@ -1155,7 +1170,7 @@ int wm_mg3d_sorhalfpass(wm_mg3d *h, int64_t level, int64_t parity, double *phi, 
     hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
     hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
     WM_LAUNCHED();
-    halfpass<double>(h, L, (int)level, P, (int)parity, nullptr, s);
+    halfpass<double>(h, L, (int)level, P, (int)parity, nullptr, s, false);
     hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L.phi64.p, phi, L.g);
     WM_LAUNCHED();
   });

@@ -41,6 +41,60 @@ __device__ inline bool cic(const Mesh &m, double x, double y, double z, long lon
   return true;
 }
 
+// Adds the eight corner charges c[] of each lane's particle to rho.  Lanes
+// that are consecutive in the wave and share a cell form a run; each run's
+// charges are summed over the wave (segmented suffix sums with the run
+// number as segment id, so any particle order is handled) and the run's
+// first lane does the eight atomic adds.  With particles ordered by cell
+// most of the atomics disappear; in random order every lane is its own run
+// and the reduction is skipped.
+__device__ inline void deposit_runs(double *__restrict__ rho, const Mesh &m, bool ok,
+                                    long long ii, long long jj, long long kk, double c[8]) {
+  int lane = threadIdx.x & 63;
+  long long key = ok ? ii + (long long)(m.nx + 1) * (jj + (long long)(m.ny + 1) * kk) : -1;
+  long long kprev = __shfl_up(key, 1, 64);
+  bool head = ok && (lane == 0 || kprev != key);
+  unsigned long long heads = __ballot(head);
+  unsigned long long oks = __ballot(ok);
+  if (heads != oks) {
+    int seg = __popcll(heads & ((2ull << lane) - 1ull));
+    if (!ok) seg = -1 - lane;
+    for (int off = 1; off < 64; off <<= 1) {
+      int so = __shfl_down(seg, off, 64);
+      bool same = (lane + off < 64) && so == seg;
+      for (int q = 0; q < 8; q++) {
+        double o = __shfl_down(c[q], off, 64);
+        if (same) c[q] += o;
+      }
+    }
+  }
+  if (!head) return;
+  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
+  double *r = rho + ii + sx * jj + sxy * kk;
+  atomicAdd(r, c[0]);
+  atomicAdd(r + 1, c[1]);
+  atomicAdd(r + sx, c[2]);
+  atomicAdd(r + sx + 1, c[3]);
+  atomicAdd(r + sxy, c[4]);
+  atomicAdd(r + sxy + 1, c[5]);
+  atomicAdd(r + sxy + sx, c[6]);
+  atomicAdd(r + sxy + sx + 1, c[7]);
+}
+
+// Corner charges of setrho3ddirect1 (w3d.F:7906-7916): u0*v0*w0*g, ...
+__device__ inline void corner_charges(double u1, double v1, double w1, double g, double c[8]) {
+  double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+  c[0] = u0 * v0 * w0 * g;
+  c[1] = u1 * v0 * w0 * g;
+  c[2] = u0 * v1 * w0 * g;
+  c[3] = u1 * v1 * w0 * g;
+  c[4] = u0 * v0 * w1 * g;
+  c[5] = u1 * v0 * w1 * g;
+  c[6] = u0 * v1 * w1 * g;
+  c[7] = u1 * v1 * w1 * g;
+}
+
+// Deposit in the particles' own order.
 __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long long np,
                                                 const double *__restrict__ xp,
                                                 const double *__restrict__ yp,
@@ -48,28 +102,19 @@ __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long l
                                                 double g,
                                                 unsigned long long *__restrict__ nskip) {
   long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (ip >= np) return;
-  long long ii, jj, kk;
-  double u1, v1, w1;
-  if (!cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1)) {
-    if (nskip) atomicAdd(nskip, 1ull);
-    return;
+  long long ii = 0, jj = 0, kk = 0;
+  double u1, v1, w1, c[8];
+  bool ok = false;
+  if (ip < np) {
+    ok = cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1);
+    if (!ok && nskip) atomicAdd(nskip, 1ull);
   }
-  double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
-  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
-  double *r = rho + ii + sx * jj + sxy * kk;
-  atomicAdd(r, u0 * v0 * w0 * g);
-  atomicAdd(r + 1, u1 * v0 * w0 * g);
-  atomicAdd(r + sx, u0 * v1 * w0 * g);
-  atomicAdd(r + sx + 1, u1 * v1 * w0 * g);
-  atomicAdd(r + sxy, u0 * v0 * w1 * g);
-  atomicAdd(r + sxy + 1, u1 * v0 * w1 * g);
-  atomicAdd(r + sxy + sx, u0 * v1 * w1 * g);
-  atomicAdd(r + sxy + sx + 1, u1 * v1 * w1 * g);
+  if (ok) corner_charges(u1, v1, w1, g, c);
+  deposit_runs(rho, m, ok, ii, jj, kk, c);
 }
 
-// Cell key of each particle for the sorted deposit; out-of-grid particles
-// get the key ncell (sorted last, not deposited).
+// Cell key of each particle (x fastest); out-of-grid particles get the key
+// ncell and sort last.
 __global__ __launch_bounds__(256) void k_keys(long long np, const double *__restrict__ xp,
                                               const double *__restrict__ yp,
                                               const double *__restrict__ zp, Mesh m,
@@ -87,57 +132,58 @@ __global__ __launch_bounds__(256) void k_keys(long long np, const double *__rest
   idx[ip] = (unsigned int)ip;
 }
 
-// Deposit in cell order.  Lanes of a wavefront that hold particles of the
-// same cell add their eight corner weights with a segmented reduction over
-// the wave; the first lane of each run then does the eight atomic adds.
+// Block index of particlesortxyzwithcopy (top.F:5114-5120):
+// int(abs(x-xmmin)/dx) + (nx+1)*(iy + (ny+1)*iz).  The reference requires
+// particles inside the grid; here each index is clamped to [0, n] so that
+// any input gives a valid key.
+__global__ __launch_bounds__(256) void k_sortkeys(long long np, const double *__restrict__ xp,
+                                                  const double *__restrict__ yp,
+                                                  const double *__restrict__ zp, double xmmin,
+                                                  double ymmin, double zmmin, double dx,
+                                                  double dy, double dz, int nx, int ny, int nz,
+                                                  unsigned int *__restrict__ key,
+                                                  unsigned int *__restrict__ idx) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  double fx = fabs(xp[ip] - xmmin) / dx, fy = fabs(yp[ip] - ymmin) / dy,
+         fz = fabs(zp[ip] - zmmin) / dz;
+  long long ix = fx < (double)nx ? (long long)fx : nx;
+  long long iy = fy < (double)ny ? (long long)fy : ny;
+  long long iz = fz < (double)nz ? (long long)fz : nz;
+  key[ip] = (unsigned int)(ix + (long long)(nx + 1) * (iy + (long long)(ny + 1) * iz));
+  idx[ip] = (unsigned int)ip;
+}
+
+// Deposit through a permutation that orders the particles by cell.
 __global__ __launch_bounds__(256) void k_setrho_sorted(
     double *__restrict__ rho, long long np, const double *__restrict__ xp,
     const double *__restrict__ yp, const double *__restrict__ zp, Mesh m, double g,
-    const unsigned int *__restrict__ key, const unsigned int *__restrict__ idx) {
+    const unsigned int *__restrict__ idx) {
   long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  int lane = threadIdx.x & 63;
-  unsigned int ncell = (unsigned int)((long long)m.nx * m.ny * m.nz);
-  unsigned int k = t < np ? key[t] : ncell;
-  double c[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   long long ii = 0, jj = 0, kk = 0;
-  if (k < ncell) {
+  double u1, v1, w1, c[8];
+  bool ok = false;
+  if (t < np) {
     long long ip = idx[t];
-    double u1, v1, w1;
-    cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1);
-    double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
-    c[0] = u0 * v0 * w0 * g;
-    c[1] = u1 * v0 * w0 * g;
-    c[2] = u0 * v1 * w0 * g;
-    c[3] = u1 * v1 * w0 * g;
-    c[4] = u0 * v0 * w1 * g;
-    c[5] = u1 * v0 * w1 * g;
-    c[6] = u0 * v1 * w1 * g;
-    c[7] = u1 * v1 * w1 * g;
+    ok = cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1);
   }
-  // Segmented inclusive suffix sums: lane L accumulates lanes L+1.. of the
-  // same key, so the first lane of each run holds the run's total.
-  for (int off = 1; off < 64; off <<= 1) {
-    unsigned int ko = __shfl_down(k, off, 64);
-    bool same = (lane + off < 64) && ko == k;
-    for (int q = 0; q < 8; q++) {
-      double o = __shfl_down(c[q], off, 64);
-      if (same) c[q] += o;
-    }
-    // Keys of a run are contiguous, so extending by doubling stays inside it.
-  }
-  unsigned int kp = __shfl_up(k, 1, 64);
-  bool head = (lane == 0) || kp != k;
-  if (!head || k >= ncell) return;
-  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
-  double *r = rho + ii + sx * jj + sxy * kk;
-  atomicAdd(r, c[0]);
-  atomicAdd(r + 1, c[1]);
-  atomicAdd(r + sx, c[2]);
-  atomicAdd(r + sx + 1, c[3]);
-  atomicAdd(r + sxy, c[4]);
-  atomicAdd(r + sxy + 1, c[5]);
-  atomicAdd(r + sxy + sx, c[6]);
-  atomicAdd(r + sxy + sx + 1, c[7]);
+  if (ok) corner_charges(u1, v1, w1, g, c);
+  deposit_runs(rho, m, ok, ii, jj, kk, c);
+}
+
+// dst[c][i] = src[c][perm[i]] for up to 16 columns.
+struct Columns {
+  const double *src[16];
+  double *dst[16];
+  int n;
+};
+
+__global__ __launch_bounds__(256) void k_permute(long long np, Columns cols,
+                                                 const unsigned int *__restrict__ perm) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= np) return;
+  long long p = perm[i];
+  for (int c = 0; c < cols.n; c++) cols.dst[c][i] = cols.src[c][p];
 }
 
 // sete_from_phi_linear3d (w3d.F:3963-4107): E from centred differences of
@@ -215,37 +261,50 @@ __global__ __launch_bounds__(256) void k_xpush(long long np, double *__restrict_
 }
 
 // padvnc3d "fullv" for B = 0 (w3d.F:1816, 1956-1986): E = 0 + gather,
-// two half kicks, one drift.
+// two half kicks, one drift.  Particles whose new position leaves
+// [xmmin,xmmax) x [ymmin,ymmax) x [zmmin+zgrid,zmmax+zgrid) (the absorbing
+// test of particleboundaries3d) are counted into *nlost.
 __global__ __launch_bounds__(256) void k_gather_push(
     const double *__restrict__ phi, long long np, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ z, double *__restrict__ ux, double *__restrict__ uy,
     double *__restrict__ uz, const double *__restrict__ gaminv, double *__restrict__ ex,
     double *__restrict__ ey, double *__restrict__ ez, Mesh m, double tdxi, double tdyi,
-    double tdzi, double ch, double dt) {
+    double tdzi, double ch, double dt, double xmmax, double ymmax, double zmmax,
+    unsigned long long *__restrict__ nlost) {
   long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (ip >= np) return;
-  double xi = x[ip], yi = y[ip], zi = z[ip];
-  double a, b, c;
-  gather_e(phi, m, tdxi, tdyi, tdzi, xi, yi, zi, a, b, c);
-  double ea = 0. + a, eb = 0. + b, ec = 0. + c;
-  double vx = ux[ip], vy = uy[ip], vz = uz[ip];
-  vx = vx + ea * ch;
-  vy = vy + eb * ch;
-  vz = vz + ec * ch;
-  vx = vx + ea * ch;
-  vy = vy + eb * ch;
-  vz = vz + ec * ch;
-  double gi = gaminv[ip];
-  x[ip] = xi + vx * gi * dt;
-  y[ip] = yi + vy * gi * dt;
-  z[ip] = zi + vz * gi * dt;
-  ux[ip] = vx;
-  uy[ip] = vy;
-  uz[ip] = vz;
-  if (ex) {
-    ex[ip] = ea;
-    ey[ip] = eb;
-    ez[ip] = ec;
+  bool lost = false;
+  if (ip < np) {
+    double xi = x[ip], yi = y[ip], zi = z[ip];
+    double a, b, c;
+    gather_e(phi, m, tdxi, tdyi, tdzi, xi, yi, zi, a, b, c);
+    double ea = 0. + a, eb = 0. + b, ec = 0. + c;
+    double vx = ux[ip], vy = uy[ip], vz = uz[ip];
+    vx = vx + ea * ch;
+    vy = vy + eb * ch;
+    vz = vz + ec * ch;
+    vx = vx + ea * ch;
+    vy = vy + eb * ch;
+    vz = vz + ec * ch;
+    double gi = gaminv[ip];
+    double xn = xi + vx * gi * dt, yn = yi + vy * gi * dt, zn = zi + vz * gi * dt;
+    x[ip] = xn;
+    y[ip] = yn;
+    z[ip] = zn;
+    ux[ip] = vx;
+    uy[ip] = vy;
+    uz[ip] = vz;
+    if (ex) {
+      ex[ip] = ea;
+      ey[ip] = eb;
+      ez[ip] = ec;
+    }
+    lost = !(xn >= m.xmmin && xn < xmmax && yn >= m.ymmin && yn < ymmax &&
+             zn >= m.zmmin + m.zgrid && zn < zmmax + m.zgrid);
+  }
+  if (nlost) {
+    unsigned long long b = __ballot(lost);
+    if (b && (threadIdx.x & 63) == (__ffsll((long long)b) - 1))
+      atomicAdd(nlost, (unsigned long long)__popcll(b));
   }
 }
 
@@ -422,7 +481,7 @@ int wm_setrho3d_sorted(double *rho, int64_t np, const double *xp, const double *
     WM_HIP(hipcub::DeviceRadixSort::SortPairs(base + w.temp, temp, keys, keys2, idx, idx2, (int)np,
                                               0, endbit, s));
     hipLaunchKernelGGL(k_setrho_sorted, dim3(nblk(np)), dim3(256), 0, s, rho, (long long)np, xp, yp,
-                       zp, m, g, keys2, idx2);
+                       zp, m, g, idx2);
     WM_LAUNCHED();
   });
 }
@@ -488,7 +547,8 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
                      double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
                      double *ey, double *ez, double zgrid, double xmmin, double ymmin,
                      double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
-                     int64_t nz, double q, double m, double dt, void *stream) {
+                     int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
+                     double zmmax, int64_t *nlost, void *stream) {
   WM_ENTRY({
     check_grid(nx, ny, nz);
     if (np == 0) return 0;
@@ -503,7 +563,58 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
     double ch = q * halfdt / m;
     hipLaunchKernelGGL(k_gather_push, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
                        (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi, tdyi,
-                       tdzi, ch, fulldt);
+                       tdzi, ch, fulldt, xmmax, ymmax, zmmax, (unsigned long long *)nlost);
+    WM_LAUNCHED();
+  });
+}
+
+int64_t wm_particle_sort_worksize(int64_t np, int64_t nx, int64_t ny, int64_t nz) {
+  return (int64_t)sorted_layout(np, (nx + 1) * (ny + 1) * (nz + 1)).total;
+}
+
+int wm_particle_sort(int64_t np, const double *xp, const double *yp, const double *zp,
+                     double xmmin, double ymmin, double zmmin, double dx, double dy, double dz,
+                     int64_t nx, int64_t ny, int64_t nz, uint32_t *perm, void *work,
+                     void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    WM_REQUIRE(np >= 0 && np < (1ll << 31), "wm_particle_sort: np out of range");
+    if (np == 0) return 0;
+    WM_REQUIRE(xp && yp && zp && perm && work, "wm_particle_sort: NULL array");
+    hipStream_t s = wm_stream(stream);
+    int64_t nkey = (nx + 1) * (ny + 1) * (nz + 1);
+    SortedWork w = sorted_layout(np, nkey);
+    char *base = (char *)work;
+    unsigned int *keys = (unsigned int *)(base + w.keys), *idx = (unsigned int *)(base + w.idx);
+    unsigned int *keys2 = (unsigned int *)(base + w.keys2);
+    hipLaunchKernelGGL(k_sortkeys, dim3(nblk(np)), dim3(256), 0, s, (long long)np, xp, yp, zp,
+                       xmmin, ymmin, zmmin, dx, dy, dz, (int)nx, (int)ny, (int)nz, keys, idx);
+    WM_LAUNCHED();
+    size_t temp = w.total - w.temp;
+    int endbit = 1;
+    while ((1ll << endbit) <= nkey) endbit++;
+    // LSD radix sort is stable: equal keys keep their order, as in the
+    // reference's counting sort (top.F:5156-5174).
+    WM_HIP(hipcub::DeviceRadixSort::SortPairs(base + w.temp, temp, keys, keys2, idx,
+                                              (unsigned int *)perm, (int)np, 0, endbit, s));
+  });
+}
+
+int wm_permute_columns(int64_t np, int64_t ncol, const double *const *src, double *const *dst,
+                       const uint32_t *perm, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(ncol >= 0 && ncol <= 16, "wm_permute_columns: at most 16 columns");
+    if (np == 0 || ncol == 0) return 0;
+    WM_REQUIRE(src && dst && perm, "wm_permute_columns: NULL argument");
+    Columns c{};
+    c.n = (int)ncol;
+    for (int i = 0; i < ncol; i++) {
+      WM_REQUIRE(src[i] && dst[i] && src[i] != dst[i], "wm_permute_columns: bad column pointer");
+      c.src[i] = src[i];
+      c.dst[i] = dst[i];
+    }
+    hipLaunchKernelGGL(k_permute, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), (long long)np, c,
+                       (const unsigned int *)perm);
     WM_LAUNCHED();
   });
 }

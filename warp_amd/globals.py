@@ -42,7 +42,9 @@ class F3D(_Package):
 # registered (fieldsolver.py:181).
 top = Top(dt=0., time=0., it=0, fstype=-1, pgroup=None, zbeam=0., zgrid=0.,
           depos="direct1", efetch=1, lresetparticlee=True, steptime=0.,
-          fstime=0., lrtime=0.)
+          fstime=0., lrtime=0.,
+          # steps between particlesortxyzwithcopy calls in step(); 0 = never
+          particlesortinterval=0)
 
 # w3d: mesh and boundary inputs (w3d.v), plus the FieldSolveAPI globals
 # (w3d.v:432-458) through which fetche receives particle positions.

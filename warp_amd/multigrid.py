@@ -34,7 +34,7 @@ class MultiGrid3D(FieldSolver):
                      'mgmaxlevels', 'mgform', 'mgverbose', 'mgntverbose', 'lcndbndy',
                      'icndbndy', 'laddconductor', 'lprecalccoeffs']
 
-    def __init__(self, precision="fp64", device=None, sorted_deposit=True, **kw):
+    def __init__(self, precision="fp64", device=None, sorted_deposit=False, **kw):
         FieldSolver.__init__(self, kwdict=kw)
         self.processdefaultsfrompackage(MultiGrid3D.__f3dinputs__, f3d, kw)
         assert len(kw.keys()) == 0, "Bad keyword arguemnts %s" % list(kw.keys())
@@ -62,6 +62,7 @@ class MultiGrid3D(FieldSolver):
                                 device=self.device)
         self._selfe = None
         self._work = None
+        self._nlost = None
         b = (C.c_int64 * 6)(*self.bounds)
         h = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -380,13 +381,32 @@ class MultiGrid3D(FieldSolver):
             return
         col = lambda c: getattr(pg, c)[sl]
         t0 = time.perf_counter()
+        if self._nlost is None:
+            self._nlost = torch.zeros(1, dtype=torch.int64, device=self.device)
         N.call("wm_gather_push3d", self._phi.data_ptr(), n, col("xp").data_ptr(),
                col("yp").data_ptr(), col("zp").data_ptr(), col("uxp").data_ptr(),
                col("uyp").data_ptr(), col("uzp").data_ptr(), col("gaminv").data_ptr(),
                col("ex").data_ptr(), col("ey").data_ptr(), col("ez").data_ptr(), top.zgrid,
                self.xmmin, self.ymmin, self.zmmin, self.dx, self.dy, self.dz, self.nx,
-               self.ny, self.nz, pg.sq[js], pg.sm[js], dt, N.stream_of(self.device))
+               self.ny, self.nz, pg.sq[js], pg.sm[js], dt, self.xmmax, self.ymmax, self.zmmax,
+               self._nlost.data_ptr(), N.stream_of(self.device))
         self.timers["fetche"] += time.perf_counter() - t0
+
+    def take_lost_count(self):
+        """Particles counted outside the mesh by gather_push since the last
+        call (reads one integer from the device, then resets it)."""
+        if self._nlost is None:
+            return None
+        n = int(self._nlost.item())
+        if n:
+            self._nlost.zero_()
+        return n
+
+    def sortparticles(self, pgroup):
+        """particlesortxyzwithcopy on this solver's mesh (top.F:5095)."""
+        from .particles import particlesortxyzwithcopy
+        particlesortxyzwithcopy(pgroup, self.dx, self.dy, self.dz, self.xmmin, self.ymmin,
+                                self.zmmin + top.zgrid, self.nx, self.ny, self.nz)
 
     def nlevels(self):
         n = C.c_int64(0)

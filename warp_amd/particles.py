@@ -86,6 +86,43 @@ class ParticleGroup:
         self.ins, self.nps = new_ins, new_nps
 
 
+def particlesortxyzwithcopy(pgroup, dx, dy, dz, xmmin, ymmin, zmmin, nx, ny, nz):
+    """Orders each species' particles by mesh block (top.F:5095-5203):
+    block int(|x-xmmin|/dx) + (nx+1)*(iy + (ny+1)*iz), stable within a
+    block.  Every column is permuted (particlecopysorteddata).  On the GPU:
+    cell keys + radix sort + one gather pass over the columns."""
+    import ctypes as C
+    from . import _native as N
+    if pgroup.npmax == 0:
+        return
+    if pgroup.device.type != "cuda":
+        raise RuntimeError("particlesortxyzwithcopy runs on the HIP device")
+    s = N.stream_of(pgroup.device)
+    new = {c: torch.empty_like(getattr(pgroup, c)) for c in pgroup.COLUMNS}
+    k = len(pgroup.COLUMNS)
+    covered = 0
+    for js in range(pgroup.ns):
+        sl = pgroup.species_slice(js)
+        n = sl.stop - sl.start
+        if n == 0:
+            continue
+        covered += n
+        need = int(N.lib().wm_particle_sort_worksize(n, nx, ny, nz))
+        work = torch.empty(need, dtype=torch.uint8, device=pgroup.device)
+        perm = torch.empty(n, dtype=torch.int32, device=pgroup.device)
+        N.call("wm_particle_sort", n, pgroup.xp[sl].data_ptr(), pgroup.yp[sl].data_ptr(),
+               pgroup.zp[sl].data_ptr(), xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz,
+               perm.data_ptr(), work.data_ptr(), s)
+        src = (C.c_void_p * k)(*[getattr(pgroup, c)[sl].data_ptr() for c in pgroup.COLUMNS])
+        dst = (C.c_void_p * k)(*[new[c][sl].data_ptr() for c in pgroup.COLUMNS])
+        N.call("wm_permute_columns", n, k, src, dst, perm.data_ptr(), s)
+        del work, perm
+    if covered != pgroup.npmax:
+        raise RuntimeError("particle arrays hold particles outside every species")
+    for c in pgroup.COLUMNS:
+        setattr(pgroup, c, new[c])
+
+
 def gaussian_weight(current, sigmaz, vz, charge, npart):
     """Macroparticle weight for a beam of peak current I whose line charge is
     a Gaussian of rms length sigmaz: sw = I*sqrt(2 pi)*sigmaz/(vz*q*N)

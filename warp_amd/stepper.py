@@ -18,9 +18,12 @@ def generate():
     FS.fieldsolregistered()
 
 
-def particleboundaries3d(pg, solver):
+def particleboundaries3d(pg, solver, nlost=None):
     """Absorbing particle boundaries (particleboundaries3d, w3d.F:11292-):
-    particles leaving the mesh are removed."""
+    particles leaving the mesh are removed.  nlost, when known (counted by
+    the fused advance), skips the scan if no particle left."""
+    if nlost == 0:
+        return
     x, y, z = pg.xp, pg.yp, pg.zp
     keep = ((x >= solver.xmmin) & (x < solver.xmmax) & (y >= solver.ymmin) &
             (y < solver.ymmax) & (z >= solver.zmmin + top.zgrid) &
@@ -67,7 +70,13 @@ def step(n=1, pgroup=None, boundaries=True):
         t0 = time.perf_counter()
         padvnc3d_fullv(pg, top.dt)
         if boundaries and solvers:
-            particleboundaries3d(pg, solvers[0])
+            nlost = None
+            if len(solvers) == 1 and hasattr(solvers[0], "take_lost_count"):
+                nlost = solvers[0].take_lost_count()
+            particleboundaries3d(pg, solvers[0], nlost)
+        if (top.particlesortinterval > 0 and solvers and hasattr(solvers[0], "sortparticles")
+                and top.it % top.particlesortinterval == 0):
+            solvers[0].sortparticles(pg)
         FS.loadrhoregistered()
         FS.fieldsolregistered()
         top.it += 1
