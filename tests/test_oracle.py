@@ -12,14 +12,17 @@ from tests import helpers as H
 # multigrid3dsolve: the reference's known-answer test
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("bounds", H.all_bounds())
-def test_manufactured_kat(bounds):
+@pytest.mark.parametrize("mgform", [1, 2])
+def test_manufactured_kat(bounds, mgform):
     """warp_test/MutltiGrid3D_test.py: 24x22x18 on [-1,1]^3 for every
     boundary combination, mgtol 1e-12; the discrete solution is recovered
-    to 1e-10 in fewer than mgmaxiters V-cycles."""
+    to 1e-10 in fewer than mgmaxiters V-cycles.  mgform 2 (the residual-
+    correction form, f3d_mgrid.F:336-372) must reach the same solution."""
     nx, ny, nz = 24, 22, 18
     phie, rho, phi, dx, dy, dz = H.manufactured(nx, ny, nz, bounds)
     rho0 = rho.copy()
-    it, err = O.multigrid3dsolve(phi, rho, dx, dy, dz, bounds=bounds, mgtol=1e-12)
+    it, err = O.multigrid3dsolve(phi, rho, dx, dy, dz, bounds=bounds, mgtol=1e-12,
+                                 mgform=mgform)
     assert it < 100 and err <= 1e-12
     # rho is scaled and restored in place (f3d_mgrid.F:224-230, 487-494)
     np.testing.assert_allclose(rho, rho0, rtol=1e-15, atol=0)
@@ -341,8 +344,9 @@ def test_coax_second_order():
     assert e32 / e64 > 3.5 and e64 / e128 > 3.5
 
 
+@pytest.mark.parametrize("icndbndy", [1, 2])
 @pytest.mark.parametrize("nz,z0", [(32, 0.05123), (16, 0.0313)])
-def test_plane_at_fractional_offset_is_exactly_linear(nz, z0):
+def test_plane_at_fractional_offset_is_exactly_linear(nz, z0, icndbndy):
     """A conducting slab z >= z0 at V over phi(z=0) = 0 with Neumann x, y:
     the subgrid stencil reproduces phi = V z / z0 to round-off."""
     nx = ny = 4
@@ -354,7 +358,8 @@ def test_plane_at_fractional_offset_is_exactly_linear(nz, z0):
     phi = np.zeros((nz + 3, ny + 3, nx + 3))
     rho = np.zeros((nz + 1, ny + 1, nx + 1))
     it, err = O.multigrid3dsolve(phi, rho, 0.04 / nx, 0.04 / ny, dz, bounds=(1, 1, 1, 1, 0, 0),
-                                 conductors=co, mgtol=1e-11, mgmaxiters=2000)
+                                 conductors=co, mgtol=1e-11, mgmaxiters=2000,
+                                 icndbndy=icndbndy)
     z = dz * np.arange(nz + 1)
     k = z < z0
     want = (V * z[k] / z0)[:, None, None]
