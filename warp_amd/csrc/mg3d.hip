@@ -202,6 +202,10 @@ struct Level {
   DevBuf<int> rlo[3], rhi[3];
   DevBuf<double> rw[3];
   int rcmin[3], rcmax[3];
+  // widest window and the fine span of the computed range, per axis
+  int rwidth[3] = {0, 0, 0}, rflo[3] = {0, 0, 0}, rfhi[3] = {-1, -1, -1};
+  // widest fine span of a tile of 32 (x) or 8 (y) consecutive coarse nodes
+  int rspan32[3] = {0, 0, 0}, rspan8[3] = {0, 0, 0};
   DevBuf<int> ej[3];
   DevBuf<double> ew[3];
   int elo[3], ehi[3];
@@ -326,6 +330,14 @@ static void restrict_axis(int n, int nc, int lbm, int lbp, Level &L, int a) {
   L.rw[a].upload(w);
   L.rcmin[a] = cmin;
   L.rcmax[a] = cmax;
+  L.rwidth[a] = 0;
+  for (int ic = cmin; ic <= cmax; ic++) L.rwidth[a] = std::max(L.rwidth[a], hi[ic] - lo[ic] + 1);
+  for (int c0 = cmin; c0 <= cmax; c0 += 32)
+    L.rspan32[a] = std::max(L.rspan32[a], hi[std::min(c0 + 31, cmax)] - lo[c0] + 1);
+  for (int c0 = cmin; c0 <= cmax; c0 += 8)
+    L.rspan8[a] = std::max(L.rspan8[a], hi[std::min(c0 + 7, cmax)] - lo[c0] + 1);
+  L.rflo[a] = cmax >= cmin ? lo[cmin] : 0;
+  L.rfhi[a] = cmax >= cmin ? hi[cmax] : -1;
 }
 
 // expand3d's per-axis tables (f3d_mgrid.F:1833-1844).
@@ -604,12 +616,19 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   T spm1 = (T)(1. - P.mgparam);
   SubList<T> sub{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (P.lcnd) sub = cond_of<T>(L).sor[c].view();
-  int nbx = (int)cdiv(L.g.W, 64);
   int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
   if (nyr <= 0 || nzr <= 0) return;
   int nby = (int)cdiv(nyr, 4);
-  int nbdense = nbx * nby * nzr;
   int nbsub = (int)cdiv(sub.n, 256);
+  static const bool tiled = getenv("WARPMI_SOR_TILED") != nullptr;
+  // z-marching sweep: colour elements j0..j1 cover ix in [ixmin, ixmax]
+  // for both row parities; z chunks sized for about 2048 dense blocks.
+  int j0 = (r.ixmin + 1) / 2, j1 = (r.ixmax + 1) / 2;
+  int nbx = tiled ? (int)cdiv(L.g.W, 64) : (int)cdiv(j1 - j0 + 1, 64);
+  long long cols = (long long)nbx * nby;
+  int zc = (int)std::min<long long>(64, std::max<long long>(2, cdiv((long long)nzr * cols, 2048)));
+  int nbz = (int)cdiv(nzr, zc);
+  int nbdense = tiled ? nbx * nby * nzr : nbx * nby * nbz;
   bool timed = h->prof.on && level == 0 && std::is_same<T, double>::value;
   if (timed) {
     while ((int)h->prof.ev.size() < 2 * (h->prof.used + 1)) {
@@ -619,10 +638,16 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
     }
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used], s));
   }
-  hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
-                     L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
-                     dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
-                     nbdense, skip);
+  if (tiled)
+    hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+                       L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
+                       dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
+                       nbdense, skip);
+  else
+    hipLaunchKernelGGL((k_sor_zmarch<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+                       L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
+                       dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0, nbx, nby,
+                       zc, nbdense, skip);
   WM_LAUNCHED();
   if (timed) {
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
@@ -725,6 +750,74 @@ static void expand_from(Level &L, T *phi, FI F, const T *pc, CI Cc, const int *s
   WM_LAUNCHED();
 }
 
+// residual3d + restrict3d in one pass over the fine level (k_resrestrict)
+// when every boundary is Dirichlet, no subgrid surface is Neumann (whose
+// residual zeroes neighbours) and every restriction window is at most 3
+// points wide and inside residual3d's loop range.  The conductor points'
+// residuals are written first into the natural residual array, from which
+// the fused kernel takes them.  Returns false when the separate kernels
+// must run.
+template <typename T>
+static bool resrestrict(wm_mg3d *h, Level &L, int level, const Params &P, Level &Lc,
+                        const int *skip, hipStream_t s) {
+  static const bool off = getenv("WARPMI_NO_RESRESTRICT") != nullptr;
+  const int *b = h->bounds;
+  for (int a = 0; a < 6; a++)
+    if (b[a] != 0) return false;
+  if (off || L.neumann) return false;
+  LoopRange r = res_range(L, b);
+  int rmin[3] = {r.ixmin, r.iymin, r.izmin}, rmax[3] = {r.ixmax, r.iymax, r.izmax};
+  if (L.rspan32[0] > 2 * 32 + 1 || L.rspan8[1] > 2 * 8 + 1) return false;
+  for (int a = 0; a < 3; a++) {
+    if (L.rwidth[a] > 3) return false;
+    if (L.rcmax[a] >= L.rcmin[a] && (L.rflo[a] < rmin[a] || L.rfhi[a] > rmax[a])) return false;
+  }
+  RestrictAxis ax{L.rlo[0].p, L.rhi[0].p, L.rw[0].p, L.rcmin[0], L.rcmax[0]};
+  RestrictAxis ay{L.rlo[1].p, L.rhi[1].p, L.rw[1].p, L.rcmin[1], L.rcmax[1]};
+  RestrictAxis az{L.rlo[2].p, L.rhi[2].p, L.rw[2].p, L.rcmin[2], L.rcmax[2]};
+  T *res = res_of<T>(L).p;
+  const T *phi = phi_of<T>(L).p;
+  const T *rho = rho_of<T>(L).p;
+  double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+  if (L.hasmask) {
+    ensure_records<T>(h, L, level, P, true, 0);
+    CondDev<T> &C = cond_of<T>(L);
+    T dxsqics = (T)(P.mgparam * L.dxsqi * cst), dysqics = (T)(P.mgparam * L.dysqi * cst),
+      dzsqics = (T)(P.mgparam * L.dzsqi * cst);
+    for (int p = 0; p < 2; p++) {
+      SubList<T> sub = C.res[0][p].view();
+      int nint = p == 0 ? L.nint : 0;
+      long long cnt = (long long)nint + sub.n;
+      if (cnt == 0) continue;
+      hipLaunchKernelGGL((k_residual_cond<T, T>), dim3(blocks(cnt)), dim3(256), 0, s, phi, rho,
+                         res, L.g, nint, L.int_pos.p, sub, dxsqics, dysqics, dzsqics,
+                         (T)P.mgparam, skip);
+      WM_LAUNCHED();
+    }
+  }
+  long long nco = (long long)(Lc.nx + 1) * (Lc.ny + 1) * (Lc.nz + 1);
+  hipLaunchKernelGGL((k_zero_faces<T, RBIdx>), dim3(blocks(nco)), dim3(256), 0, s,
+                     rho_of<T>(Lc).p, RBIdx{Lc.g}, Lc.nx, Lc.ny, Lc.nz, ax, ay, az, skip);
+  WM_LAUNCHED();
+  int ncx = L.rcmax[0] - L.rcmin[0] + 1, ncy = L.rcmax[1] - L.rcmin[1] + 1,
+      ncz = L.rcmax[2] - L.rcmin[2] + 1;
+  if (ncx <= 0 || ncy <= 0 || ncz <= 0) return true;
+  int nbx = (int)cdiv(ncx, 32), nby = (int)cdiv(ncy, 8);
+  long long tiles = (long long)nbx * nby;
+  int zc = (int)std::min<long long>(32, std::max<long long>(1, cdiv((long long)ncz * tiles, 2048)));
+  int nbz = (int)cdiv(ncz, zc);
+  int nb = nbx * nby * nbz;
+  T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
+    dzsqic = (T)(L.dzsqi * P.mgparam * cst);
+  constexpr bool tablew = std::is_same<T, double>::value;
+  hipLaunchKernelGGL((k_resrestrict<T, tablew, RBIdx>), dim3(nb), dim3(256), 0, s, phi, rho,
+                     (const T *)res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
+                     dxsqic, dysqic, dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr,
+                     rho_of<T>(Lc).p, RBIdx{Lc.g}, ax, ay, az, (T)L.ff, nbx, nby, zc, nb, skip);
+  WM_LAUNCHED();
+  return true;
+}
+
 // vcycle (f3d_mgrid.F:1101-1591).
 template <typename T>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s) {
@@ -732,9 +825,12 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   for (int i = 0; i < P.down; i++) sorpass<T>(h, L, level, P, skip, s);
   if (L.coarser) {
     Level &Lc = *h->lv[level + 1];
-    residual<T, T>(h, L, level, P, phi_of<T>(L).p, rho_of<T>(L).p, res_of<T>(L).p, 0, skip, s);
-    restrict_to<T, T>(L, res_of<T>(L).p, rho_of<T>(Lc).p, RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny,
-                      Lc.nz);
+    if (!resrestrict<T>(h, L, level, P, Lc, skip, s)) {
+      residual<T, T>(h, L, level, P, phi_of<T>(L).p, rho_of<T>(L).p, res_of<T>(L).p, 0, skip,
+                     s);
+      restrict_to<T, T>(L, res_of<T>(L).p, rho_of<T>(Lc).p, RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny,
+                        Lc.nz);
+    }
     hipLaunchKernelGGL((k_fill<T>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
                        phi_of<T>(Lc).p, 2 * Lc.g.cs, (T)0., skip);
     WM_LAUNCHED();

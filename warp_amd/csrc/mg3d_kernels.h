@@ -160,6 +160,107 @@ __global__ __launch_bounds__(256) void k_sor_halfpass(
              (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev;
 }
 
+// ---------------------------------------------------------------------------
+// SOR half sweep, z-marching form.  Same arithmetic and the same sparse
+// tail as k_sor_halfpass; the dense part gives each thread one column
+// (colour element j of row iy) and walks it through ZC planes, so the
+// other-colour values below, at and above the point stay in registers and
+// each plane of the other colour is fetched from HBM once.  A wavefront is
+// one row segment of 64 elements: the x-neighbour on the far side comes
+// from the next lane (a shuffle), the y-neighbours from the rows beside
+// it, which the neighbouring waves of the block load in the same step.
+// Blocks: (row segment jseg, 4 rows, z chunk), remapped per XCD.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_sor_zmarch(
+    T *__restrict__ phi, const T *__restrict__ rho, RBGeom g, int c, int ixmin,
+    int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
+    T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
+    int j0, int nbx, int nby, int zc, int nbdense, const int *__restrict__ skip) {
+  WM_SKIP();
+  int b = blockIdx.x;
+  if (b < nbdense) {
+    int lb = xcd_remap(b, nbdense);
+    int bx = lb % nbx;
+    int t = lb / nbx;
+    int by = t % nby;
+    int bz = t / nby;
+    int lane = threadIdx.x & 63;
+    int j = j0 + bx * 64 + lane;
+    int iy = iymin + by * 4 + (threadIdx.x >> 6);
+    if (iy > iymax) return;  // whole wave: no shuffles are skipped mid-wave
+    int za = izmin + bz * zc;
+    int zb = min(za + zc - 1, izmax);
+    int py = iy + 1;
+    long long pl = (long long)g.W * g.PY;
+    const T *o = phi + (1 - c) * g.cs;
+    T *me = phi + c * g.cs;
+    const T *rh = rho + c * g.cs;
+    // Loads use jc, clamped into the padded row, so they stay in bounds for
+    // lanes past the end of the row; such lanes never store.
+    int jc = min(j, g.W - 1);
+    long long rowc = (long long)g.W * py + jc;
+    long long meoff = (long long)c * g.cs + rowc;  // mask index within a plane
+    // Loads of one plane (padded index pz), issued a plane ahead of use.
+    auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw) {
+      long long base = pl * pz;
+      ozp = o[rowc + base + pl];
+      oym = o[rowc + base - g.W];
+      oyp = o[rowc + base + g.W];
+      own = me[rowc + base];
+      r = rh[rowc + base];
+      int p0 = (c + py + pz + 1) & 1;
+      int jn = p0 ? j + 1 : j - 1;
+      bool e = p0 ? (lane == 63) : (lane == 0);
+      edge = e ? o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)] : (T)0.;
+      mw = mask ? mask[(meoff + base) >> 5] : 0u;
+    };
+    T oz_m = o[rowc + pl * za];        // padded plane za: iz = za-1
+    T oz_0 = o[rowc + pl * (za + 1)];  // iz = za
+    T ozp, oym, oyp, own, r, eg;
+    uint32_t mw;
+    load(za + 1, ozp, oym, oyp, own, r, eg, mw);
+    for (int iz = za; iz <= zb; iz++) {
+      int pz = iz + 1;
+      T nozp = 0, noym = 0, noyp = 0, nown = 0, nr = 0, neg = 0;
+      uint32_t nmw = 0;
+      if (iz < zb) load(pz + 1, nozp, noym, noyp, nown, nr, neg, nmw);
+      int p0 = (c + py + pz + 1) & 1;
+      // other-colour x-neighbours: (j-1, j) when p0 == 0, (j, j+1) when 1
+      T sh = p0 ? __shfl_down(oz_0, 1, 64) : __shfl_up(oz_0, 1, 64);
+      if (p0 ? (lane == 63) : (lane == 0)) sh = eg;
+      int ix = p0 + 2 * j - 1;
+      long long ce = meoff + pl * pz;
+      bool active = j < g.W && ix >= ixmin && ix <= ixmax && !((mw >> (ce & 31)) & 1u);
+      if (active) {
+        T pxm = p0 ? oz_0 : sh;
+        T pxp = p0 ? sh : oz_0;
+        me[rowc + pl * pz] = r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic +
+                             (oz_m + ozp) * dzsqic + spm1 * own;
+      }
+      oz_m = oz_0;
+      oz_0 = ozp;
+      ozp = nozp; oym = noym; oyp = noyp; own = nown; r = nr; eg = neg; mw = nmw;
+    }
+    return;
+  }
+  int i = (b - nbdense) * 256 + threadIdx.x;
+  if (i >= sub.n) return;
+  int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
+  uint32_t m = sub.modes[i];
+  const T *k = sub.k + 6 * i;
+  T pxm = side_value<T>(m & 3u, k[0], phi[rb_index(g, ix - 1, iy, iz)]);
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], phi[rb_index(g, ix + 1, iy, iz)]);
+  T pym = side_value<T>((m >> 4) & 3u, k[2], phi[rb_index(g, ix, iy - 1, iz)]);
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], phi[rb_index(g, ix, iy + 1, iz)]);
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], phi[rb_index(g, ix, iy, iz - 1)]);
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], phi[rb_index(g, ix, iy, iz + 1)]);
+  long long ce = rb_index(g, ix, iy, iz);
+  T prev = phi[ce];
+  phi[ce] = (rho[ce] + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
+             (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev;
+}
+
 // cond_potmg (f3d_mgrid.F:2426-2472): pin interior conductor points.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pin(T *__restrict__ phi, RBGeom g, int n,
@@ -430,6 +531,150 @@ __global__ __launch_bounds__(256) void k_restrict_t(
         w = w + ww;
       }
   out[oi] = w > (T)0. ? (r / w) * ff : (T)0.;
+}
+
+// ---------------------------------------------------------------------------
+// residual3d fused with restrict3d, for all-Dirichlet boundaries and windows
+// at most 3 fine points wide (2:1 or no coarsening per axis).  A block owns
+// RT_X x RT_Y coarse nodes of the computed range and walks a chunk of coarse
+// planes; the fine residuals of the window planes are computed into an LDS
+// ring of 4 fine planes, each fine plane once per block, and every coarse
+// node then sums its window from LDS in restrict3d's order
+// (z, y, x; f3d_mgrid.F:1707-1737).  Fine points outside residual3d's loop
+// range or in the conductor mask take the value already in the natural
+// residual array `res`, where the face-zeroing kernel and
+// cond_potmgres/condbndyres (k_residual_cond) have put it, exactly as the
+// separate kernels leave it.  Coarse nodes outside the computed range are
+// written elsewhere (k_zero_faces).
+// ---------------------------------------------------------------------------
+#define RT_X 32
+#define RT_Y 8
+#define RT_FX (2 * RT_X + 1)
+#define RT_FY (2 * RT_Y + 1)
+
+template <typename T, bool TABLEW>
+__device__ inline void restrict_sum(const T *lds, int fxl, int fyl, const int *zslot,
+                                    int x0, int x1, int y0, int y1, int z0, int z1,
+                                    const double *wx, const double *wy, const double *wz,
+                                    T ff, T &out) {
+  if (TABLEW) {
+    double r = 0., w = 0.;
+    for (int iz = z0; iz <= z1; iz++) {
+      const T *pl = lds + zslot[iz - z0] * (RT_FX * RT_FY);
+      for (int iy = y0; iy <= y1; iy++)
+        for (int ix = x0; ix <= x1; ix++) {
+          double ww = wx[ix - x0] * wy[iy - y0] * wz[iz - z0];
+          r = r + ww * (double)pl[(iy - fyl) * RT_FX + (ix - fxl)];
+          w = w + ww;
+        }
+    }
+    if (w > 0.) {
+      r = r / w;
+      out = (T)(r * (double)ff);
+    } else {
+      out = (T)0.;
+    }
+  } else {
+    T r = 0., w = 0.;
+    for (int iz = z0; iz <= z1; iz++) {
+      const T *pl = lds + zslot[iz - z0] * (RT_FX * RT_FY);
+      for (int iy = y0; iy <= y1; iy++)
+        for (int ix = x0; ix <= x1; ix++) {
+          T ww = (T)wx[ix - x0] * (T)wy[iy - y0] * (T)wz[iz - z0];
+          r = r + ww * pl[(iy - fyl) * RT_FX + (ix - fxl)];
+          w = w + ww;
+        }
+    }
+    out = w > (T)0. ? (r / w) * ff : (T)0.;
+  }
+}
+
+template <typename T, bool TABLEW, typename OutIdx>
+__global__ __launch_bounds__(256) void k_resrestrict(
+    const T *__restrict__ phi, const T *__restrict__ rho, const T *__restrict__ res,
+    RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax,
+    T dxsqic, T dysqic, T dzsqic, T mgparam, const uint32_t *__restrict__ mask,
+    T *__restrict__ out, OutIdx O, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
+    T ff, int nbx, int nby, int zc, int nblocks, const int *__restrict__ skip) {
+  WM_SKIP();
+  __shared__ T lds[4 * RT_FX * RT_FY];
+  int lb = xcd_remap(blockIdx.x, nblocks);
+  int bx = lb % nbx;
+  int t = lb / nbx;
+  int by = t % nby;
+  int bz = t / nby;
+  // coarse tile inside the computed range
+  int cx0 = ax.cmin + bx * RT_X, cx1 = min(cx0 + RT_X - 1, ax.cmax);
+  int cy0 = ay.cmin + by * RT_Y, cy1 = min(cy0 + RT_Y - 1, ay.cmax);
+  int cz0 = az.cmin + bz * zc, cz1 = min(cz0 + zc - 1, az.cmax);
+  int fxl = ax.lo[cx0], fxh = ax.hi[cx1];
+  int fyl = ay.lo[cy0], fyh = ay.hi[cy1];
+  int fw = fxh - fxl + 1, fh = fyh - fyl + 1;
+  int tx = threadIdx.x & (RT_X - 1), ty = threadIdx.x / RT_X;
+  int cx = cx0 + tx, cy = cy0 + ty;
+  bool mine = cx <= cx1 && cy <= cy1;
+  int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+  const double *wx = ax.w, *wy = ay.w;
+  if (mine) {
+    x0 = ax.lo[cx]; x1 = ax.hi[cx]; wx = ax.w + 5 * cx;
+    y0 = ay.lo[cy]; y1 = ay.hi[cy]; wy = ay.w + 5 * cy;
+  }
+  int sx = g.nx + 3, sxy = (g.nx + 3) * (g.ny + 3);
+  int done = az.lo[cz0] - 1;  // last fine plane in the ring
+  for (int cz = cz0; cz <= cz1; cz++) {
+    int z0 = az.lo[cz], z1 = az.hi[cz];
+    for (int fz = max(done + 1, z0); fz <= z1; fz++) {
+      T *pl = lds + (fz & 3) * (RT_FX * RT_FY);
+      bool zin = fz >= izmin && fz <= izmax;
+      for (int e = threadIdx.x; e < fw * fh; e += 256) {
+        int fx = fxl + e % fw, fy = fyl + e / fw;
+        T v;
+        long long ce = rb_index(g, fx, fy, fz);
+        bool dense = zin && fx >= ixmin && fx <= ixmax && fy >= iymin && fy <= iymax &&
+                     !(mask && ((mask[ce >> 5] >> (ce & 31)) & 1u));
+        if (dense) {
+          v = rho[ce] +
+              (phi[rb_index(g, fx - 1, fy, fz)] + phi[rb_index(g, fx + 1, fy, fz)]) * dxsqic +
+              (phi[rb_index(g, fx, fy - 1, fz)] + phi[rb_index(g, fx, fy + 1, fz)]) * dysqic +
+              (phi[rb_index(g, fx, fy, fz - 1)] + phi[rb_index(g, fx, fy, fz + 1)]) * dzsqic -
+              phi[ce] * mgparam;
+        } else {
+          v = res[(fx + 1) + (long long)sx * (fy + 1) + (long long)sxy * (fz + 1)];
+        }
+        pl[(fy - fyl) * RT_FX + (fx - fxl)] = v;
+      }
+      done = fz;
+    }
+    __syncthreads();
+    if (mine) {
+      int zslot[3] = {z0 & 3, (z0 + 1) & 3, (z0 + 2) & 3};
+      T o;
+      restrict_sum<T, TABLEW>(lds, fxl, fyl, zslot, x0, x1, y0, y1, z0, z1, wx, wy,
+                              az.w + 5 * cz, ff, o);
+      out[O(cx, cy, cz)] = o;
+    }
+    __syncthreads();
+  }
+}
+
+// Zero the coarse nodes restrict3d leaves outside its computed range
+// (Dirichlet faces, f3d_mgrid.F:1704-1751).
+template <typename TC, typename OutIdx>
+__global__ __launch_bounds__(256) void k_zero_faces(TC *__restrict__ out, OutIdx O, int nxc,
+                                                    int nyc, int nzc, RestrictAxis ax,
+                                                    RestrictAxis ay, RestrictAxis az,
+                                                    const int *__restrict__ skip) {
+  WM_SKIP();
+  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ixc = (int)(t % (nxc + 1));
+  long long u = t / (nxc + 1);
+  int iyc = (int)(u % (nyc + 1));
+  int izc = (int)(u / (nyc + 1));
+  if (ixc < ax.cmin || ixc > ax.cmax || iyc < ay.cmin || iyc > ay.cmax || izc < az.cmin ||
+      izc > az.cmax)
+    out[O(ixc, iyc, izc)] = (TC)0.;
 }
 
 // ---------------------------------------------------------------------------
