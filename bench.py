@@ -142,7 +142,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 V-cycles in f64 defect correction",
-                  "bf16": "f32 V-cycles in f64 defect correction"}[args.precision],
+                  "bf16": "bf16 finest level, f32 coarse levels, f64 defect correction"
+                  }[args.precision],
         "data": "synthetic (seeded KV beam, pipe + 10 kV annulus)",
         "config": {"workload": "ES PIC step %dx%dx%d, %d particles, subgrid conductors"
                                % (nx, ny, nz, npart0),

@@ -195,6 +195,17 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp,
                      double dt, double xmmax, double ymmax, double zmmax,
                      int64_t *nlost, void *stream);
 
+/* getzmmnt (top.F:521-1360), itask 2: the whole-beam moment sums
+ * zmmnts0(1:33) of np particles into mm (device, 33 doubles, overwritten;
+ * mm[k-1] = zmmnts0(k)), summed per block then over blocks in a fixed
+ * order.  usegaminv selects the lrelativ branch (velocities u*gaminv).
+ * work must hold wm_getzmmnt_worksize(np) bytes. */
+int64_t wm_getzmmnt_worksize(int64_t np);
+int wm_getzmmnt(int64_t np, const double *xp, const double *yp,
+                const double *zp, const double *uxp, const double *uyp,
+                const double *uzp, const double *gaminv, int64_t usegaminv,
+                double *mm, void *work, void *stream);
+
 /* particlesortxyzwithcopy (top.F:5095-5138) with particlesortbyindex
  * (top.F:5140-5203): the permutation that orders np particles by the block
  * index int(|x-xmmin|/dx) + (nx+1)*(iy + (ny+1)*iz), stable, so equal

@@ -184,3 +184,79 @@ void oc_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
         }
       }
 }
+
+/* getzmmnt, top.F:521-1360, itask == 2, the serial (non-J90) sums of the
+ * whole-beam moments zmmnts0(1:33) (top.F:1059-1136).  usegaminv selects
+ * the branch for relativistic velocities without momentum moments
+ * (lrelativ and not l_momentum_moments).  mm[k-1] holds zmmnts0(k). */
+void oc_getzmmnt(int64_t np, const double *xp, const double *yp, const double *zp,
+                 const double *uxp, const double *uyp, const double *uzp,
+                 const double *gaminv, int64_t usegaminv, double *mm) {
+  const double smallpos = 1.0e-36;
+  int64_t ip;
+  for (ip = 0; ip < np; ip++) {
+    double x = xp[ip], y = yp[ip], z = zp[ip], ux = uxp[ip], uy = uyp[ip], uz = uzp[ip];
+    double vzi = 1. / (uz + smallpos);
+    mm[0] = mm[0] + 1.;
+    mm[1] = mm[1] + x;
+    mm[2] = mm[2] + y;
+    mm[3] = mm[3] + z;
+    if (!usegaminv) {
+      mm[6] = mm[6] + ux;
+      mm[7] = mm[7] + uy;
+      mm[8] = mm[8] + uz;
+      mm[9] = mm[9] + x * y;
+      mm[11] = mm[11] + x * x;
+      mm[12] = mm[12] + y * y;
+      mm[13] = mm[13] + z * z;
+      mm[16] = mm[16] + ux * ux;
+      mm[17] = mm[17] + uy * uy;
+      mm[18] = mm[18] + uz * uz;
+      mm[21] = mm[21] + z * uz;
+      mm[24] = mm[24] + x * uz;
+      mm[25] = mm[25] + y * uz;
+      mm[26] = mm[26] + ux * uz;
+      mm[27] = mm[27] + uy * uz;
+    } else {
+      double g = gaminv[ip];
+      mm[6] = mm[6] + ux * g;
+      mm[7] = mm[7] + uy * g;
+      mm[8] = mm[8] + uz * g;
+      mm[9] = mm[9] + x * y;
+      mm[11] = mm[11] + x * x;
+      mm[12] = mm[12] + y * y;
+      mm[13] = mm[13] + z * z;
+      mm[16] = mm[16] + (g * ux) * (g * ux);
+      mm[17] = mm[17] + (g * uy) * (g * uy);
+      mm[18] = mm[18] + (g * uz) * (g * uz);
+      mm[21] = mm[21] + z * g * uz;
+      mm[24] = mm[24] + x * uz * g;
+      mm[25] = mm[25] + y * uz * g;
+      mm[26] = mm[26] + ux * uz * (g * g);
+      mm[27] = mm[27] + uy * uz * (g * g);
+    }
+    mm[4] = mm[4] + ux * vzi;
+    mm[5] = mm[5] + uy * vzi;
+    mm[10] = mm[10] + ux * uy * (vzi * vzi);
+    mm[14] = mm[14] + (ux * vzi) * (ux * vzi);
+    mm[15] = mm[15] + (uy * vzi) * (uy * vzi);
+    mm[19] = mm[19] + x * ux * vzi;
+    mm[20] = mm[20] + y * uy * vzi;
+    mm[22] = mm[22] + x * uy * vzi;
+    mm[23] = mm[23] + y * ux * vzi;
+    if (!usegaminv) {
+      mm[28] = mm[28] + x * ux;
+      mm[29] = mm[29] + y * uy;
+      mm[30] = mm[30] + x * uy;
+      mm[31] = mm[31] + y * ux;
+      mm[32] = mm[32] + ux * uy;
+    } else {
+      double g = gaminv[ip];
+      mm[28] = mm[28] + x * ux * g;
+      mm[29] = mm[29] + y * uy * g;
+      mm[30] = mm[30] + x * uy * g;
+      mm[31] = mm[31] + y * ux * g;
+      mm[32] = mm[32] + ux * uy * (g * g);
+    }
+  }
+}

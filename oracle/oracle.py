@@ -107,6 +107,7 @@ def _declare(L):
     L.oc_epush3d.argtypes = [_i64, P, P, P, P, P, P, _d, _d, _d]
     L.oc_xpush3d.argtypes = [_i64, P, P, P, P, P, P, P, _d]
     L.oc_getselfe3d.argtypes = [P, _i64, _i64, _i64, P, _d, _d, _d, _i64]
+    L.oc_getzmmnt.argtypes = [_i64, P, P, P, P, P, P, P, _i64, P]
     L.oc_conductordelfromintercepts.restype = _i64
     L.oc_conductordelfromintercepts.argtypes = [P, P, _d, _d, _i64, P]
     L.oc_zcylinder_intercepts.argtypes = [_d, _d, _d, _d, _d, P, _d, _i64]
@@ -477,6 +478,14 @@ def getselfe3d(phi, dx, dy, dz, selfe=None, lzero=True):
         selfe = np.zeros((nz + 1, ny + 1, nx + 1, 3))
     lib().oc_getselfe3d(_p(phi), nx, ny, nz, _p(selfe), dx, dy, dz, int(lzero))
     return selfe
+
+
+def getzmmnt(xp, yp, zp, uxp, uyp, uzp, gaminv, usegaminv=False):
+    """The 33 whole-beam moment sums zmmnts0 of getzmmnt (top.F:1059-1136)."""
+    mm = np.zeros(33)
+    a = [_f64(v) for v in (xp, yp, zp, uxp, uyp, uzp, gaminv)]
+    lib().oc_getzmmnt(len(a[0]), *[_p(v) for v in a], int(usegaminv), _p(mm))
+    return mm
 
 
 def sorhalfpass3d(parity, phi, rho, dx, dy, dz, bounds, mgparam=1.2, mgform=1,

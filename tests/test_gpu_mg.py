@@ -182,12 +182,13 @@ def test_conductor_solve_mgform2_bitwise(device):
     np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
 
 
-def test_conductor_solve_fp32_defect_correction(device):
-    """FP32 V-cycles in an FP64 defect-correction loop still reach mgtol and
-    agree with the FP64 oracle to the tolerance the convergence test
-    implies."""
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_conductor_solve_defect_correction(device, precision):
+    """FP32 V-cycles (or BF16 on the finest level and FP32 below) in an
+    FP64 defect-correction loop still reach mgtol and agree with the FP64
+    oracle to the tolerance the convergence test implies."""
     nx, ny, nz = 64, 64, 128
-    s, rho = _c1_like(device, nx, ny, nz, precision="fp32")
+    s, rho = _c1_like(device, nx, ny, nz, precision=precision)
     s.rho = torch.from_numpy(rho).to(device)
     s.solve()
     assert s.mgerror <= s.mgtol
@@ -209,3 +210,29 @@ def test_nonconvergence_reported(device):
     s.rho = torch.from_numpy(rho).to(device)
     s.solve()
     assert s.mgiters == 1 and s.mgerror > s.mgtol
+
+
+def test_quadrupole_channel_solve_bitwise(device):
+    """The quadrupole channel's rods (ZCylinder, +-15 kV) inside the pipe:
+    full FP64 solve, same phi bit for bit and the same V-cycle count as the
+    oracle (32x32x64, Gaussian beam source)."""
+    import warp_amd
+    from warp_amd import configs
+    nx, ny, nz = 32, 32, 64
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    s = _solver(nx, ny, nz, lo, hi, (0,) * 6)
+    for g in configs.conductors("quads", warp_amd):
+        s.installconductor(g)
+    rng = np.random.default_rng(17)
+    x, y = rng.normal(0, 0.003, 20000), rng.normal(0, 0.003, 20000)
+    z = 0.04 + rng.normal(0, 0.015, 20000)
+    k = (np.abs(x) < 0.0199) & (np.abs(y) < 0.0199) & (z > 0) & (z < 0.0799)
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    O.setrho3d(rho, x[k], y[k], z[k], 1.602176565e-19, 5e5, s.dx, s.dy, s.dz, *lo)
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    co = O.install_conductors(configs.conductors("quads", O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    it, err = O.multigrid3dsolve(po, rho.copy(), s.dx, s.dy, s.dz, conductors=co)
+    assert s.mgiters == it and s.mgerror == err
+    np.testing.assert_array_equal(s.phi.cpu().numpy(), po)

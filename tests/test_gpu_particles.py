@@ -238,3 +238,34 @@ def test_rho_boundaries_bitwise(device, bounds):
     ro = r.copy()
     O.applyrhoboundaryconditions3d(ro, bounds)
     np.testing.assert_array_equal(rg.cpu().numpy(), ro)
+
+
+def test_getzmmnt_matches_oracle(device):
+    """Whole-beam moment sums on the GPU against the serial oracle (sums in
+    another order: 1e-12 of each sum's scale), and the finalized
+    emittances to 1e-9 (vz moments, which cancel ~1e6-fold, to 1e-5)."""
+    from warp_amd import diagnostics as D
+    from warp_amd.particles import ParticleGroup
+    rng = np.random.default_rng(31)
+    n = 300000
+    x, y = rng.normal(1e-3, 4e-3, n), rng.normal(-2e-3, 3e-3, n)
+    z = rng.normal(0.04, 0.01, n)
+    ux, uy = 0.3e6 * x + rng.normal(0, 1e3, n), rng.normal(0, 1e3, n)
+    uz = 1e6 + rng.normal(0, 1e3, n)
+    pg = ParticleGroup(device)
+    js = pg.addspecies(1.0, 1.0, 1.0)
+    pg.addparticles(js, x[:n // 2], y[:n // 2], z[:n // 2], ux[:n // 2], uy[:n // 2], uz[:n // 2])
+    js = pg.addspecies(1.0, 1.0, 1.0)
+    pg.addparticles(js, x[n // 2:], y[n // 2:], z[n // 2:], ux[n // 2:], uy[n // 2:], uz[n // 2:])
+    got = D.zmmnts0(pg)
+    want = O.getzmmnt(x, y, z, ux, uy, uz, np.ones(n))
+    scale = O.getzmmnt(np.abs(x), np.abs(y), np.abs(z), np.abs(ux), np.abs(uy), np.abs(uz),
+                       np.ones(n))
+    for k in range(33):
+        assert abs(got[k] - want[k]) <= 1e-12 * scale[k], k
+    a, b = D.finalize(got), D.finalize(want)
+    for k in ("xrms", "yrms", "zrms", "xprms", "epsx", "epsy", "epsnx", "epsny", "epsr",
+              "epsg", "epsh"):
+        assert a[k] == pytest.approx(b[k], rel=1e-9), k
+    for k in ("vzrms", "epsnz"):
+        assert a[k] == pytest.approx(b[k], rel=1e-5), k

@@ -491,3 +491,53 @@ def test_getselfe_linear():
     np.testing.assert_allclose(se[..., 0], 1.5, rtol=1e-12)
     np.testing.assert_allclose(se[..., 1], -2.5, rtol=1e-12)
     np.testing.assert_allclose(se[..., 2], -4.0, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# beam moments (getzmmnt) and emittances
+# ---------------------------------------------------------------------------
+def _moment_table(x, y, z, ux, uy, uz):
+    """zmmnts0(k) named as top.F:1366-1399 reads them back (k = 1..33)."""
+    xp, yp = ux / uz, uy / uz
+    return {1: len(x), 2: x.sum(), 3: y.sum(), 4: z.sum(), 5: xp.sum(), 6: yp.sum(),
+            7: ux.sum(), 8: uy.sum(), 9: uz.sum(), 10: (x * y).sum(), 11: (xp * yp).sum(),
+            12: (x * x).sum(), 13: (y * y).sum(), 14: (z * z).sum(), 15: (xp * xp).sum(),
+            16: (yp * yp).sum(), 17: (ux * ux).sum(), 18: (uy * uy).sum(), 19: (uz * uz).sum(),
+            20: (x * xp).sum(), 21: (y * yp).sum(), 22: (z * uz).sum(), 23: (x * yp).sum(),
+            24: (y * xp).sum(), 25: (x * uz).sum(), 26: (y * uz).sum(), 27: (ux * uz).sum(),
+            28: (uy * uz).sum(), 29: (x * ux).sum(), 30: (y * uy).sum(), 31: (x * uy).sum(),
+            32: (y * ux).sum(), 33: (ux * uy).sum()}
+
+
+def test_getzmmnt_sums_named():
+    """Each zmmnts0(k) is the sum the finalization names it by."""
+    rng = np.random.default_rng(21)
+    n = 5000
+    x, y, z = rng.normal(1e-3, 4e-3, n), rng.normal(-2e-3, 3e-3, n), rng.normal(0.04, 0.01, n)
+    ux, uy, uz = rng.normal(10, 1e3, n), rng.normal(-5, 1e3, n), 1e6 + rng.normal(0, 1e3, n)
+    mm = O.getzmmnt(x, y, z, ux, uy, uz, np.ones(n))
+    want = _moment_table(x, y, z, ux, uy, uz)
+    for k, v in want.items():
+        assert mm[k - 1] == pytest.approx(v, rel=1e-11, abs=1e-300), k
+    # the gaminv branch with gaminv = 1 gives the same sums
+    np.testing.assert_allclose(O.getzmmnt(x, y, z, ux, uy, uz, np.ones(n), True), mm, rtol=1e-15)
+
+
+def test_emittance_closed_forms():
+    """An ellipse x = a cos t, x' = b sin t (t uniform over a period) has
+    epsx = 2ab; a beam with x' = k x has zero emittance."""
+    from warp_amd.diagnostics import finalize
+    n = 4096
+    t = 2 * np.pi * (np.arange(n) + 0.5) / n
+    a, b, uz = 3e-3, 2e-3, 1e6
+    x, ux = a * np.cos(t), b * uz * np.sin(t)
+    y, uy = 2 * a * np.sin(t), 0.5 * b * uz * np.cos(t)
+    z, uzv = np.full(n, 0.04), np.full(n, uz)
+    d = finalize(O.getzmmnt(x, y, z, ux, uy, uzv, np.ones(n)))
+    assert d["epsx"] == pytest.approx(2 * a * b, rel=1e-12)
+    assert d["epsy"] == pytest.approx(2 * (2 * a) * (0.5 * b), rel=1e-12)
+    assert d["xrms"] == pytest.approx(a / np.sqrt(2), rel=1e-12)
+    d2 = finalize(O.getzmmnt(x, y, z, 0.7 * uz * x, uy, uzv, np.ones(n)))
+    assert d2["epsx"] < 1e-6 * d["epsx"]   # sqrt of a rounding-level difference
+    # normalized: epsnx = 4 sqrt(<x^2><vx^2> - <x vx>^2) / c * 1e6 = epsx * uz / c * 1e6
+    assert d["epsnx"] == pytest.approx(d["epsx"] * uz / 2.99792458e8 * 1e6, rel=1e-12)

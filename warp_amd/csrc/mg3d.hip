@@ -196,6 +196,7 @@ struct Level {
   // buffers (allocated on demand)
   DevBuf<double> phi64, rho64, res64;
   DevBuf<float> phi32, rho32, res32;
+  DevBuf<bf16> phi16, rho16;  // finest level of the BF16 mode
   // transfer to the next coarser level
   bool coarser = false;
   double ff = 0.;
@@ -470,12 +471,27 @@ template <>
 DevBuf<double> &phi_of<double>(Level &L) { return L.phi64; }
 template <>
 DevBuf<float> &phi_of<float>(Level &L) { return L.phi32; }
+template <>
+DevBuf<bf16> &phi_of<bf16>(Level &L) { return L.phi16; }
 template <typename T>
 DevBuf<T> &rho_of(Level &L);
 template <>
 DevBuf<double> &rho_of<double>(Level &L) { return L.rho64; }
 template <>
 DevBuf<float> &rho_of<float>(Level &L) { return L.rho32; }
+template <>
+DevBuf<bf16> &rho_of<bf16>(Level &L) { return L.rho16; }
+
+// Storage type of the next coarser level: BF16 is used on the finest level
+// only, the levels below it are FP32.
+template <typename S>
+struct Coarse {
+  typedef S type;
+};
+template <>
+struct Coarse<bf16> {
+  typedef float type;
+};
 template <typename T>
 DevBuf<T> &res_of(Level &L);
 template <>
@@ -605,11 +621,12 @@ static void pin_and_mirror(wm_mg3d *h, Level &L, int level, const Params &P, T *
 }
 
 // sorhalfpass3d (f3d_mgrid.F:2095-2291) on level data in red-black layout.
-template <typename T>
+template <typename S>
 static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
                      const int *skip, hipStream_t s, bool inpass = true) {
-  T *phi = phi_of<T>(L).p;
-  const T *rho = rho_of<T>(L).p;
+  typedef typename St<S>::T T;
+  S *phi = phi_of<S>(L).p;
+  const S *rho = rho_of<S>(L).p;
   LoopRange r = sor_range(L, h->bounds);
   double cst = P.mgparam * 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
   T dxsqic = (T)(L.dxsqi * cst), dysqic = (T)(L.dysqi * cst), dzsqic = (T)(L.dzsqi * cst);
@@ -620,7 +637,8 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   if (nyr <= 0 || nzr <= 0) return;
   int nby = (int)cdiv(nyr, 4);
   int nbsub = (int)cdiv(sub.n, 256);
-  static const bool tiled = getenv("WARPMI_SOR_TILED") != nullptr;
+  static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
+  const bool tiled = tiled_env && !std::is_same<S, bf16>::value;
   // z-marching sweep: colour elements j0..j1 cover ix in [ixmin, ixmax]
   // for both row parities; z chunks sized for about 2048 dense blocks.
   int j0 = (r.ixmin + 1) / 2, j1 = (r.ixmax + 1) / 2;
@@ -638,13 +656,15 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
     }
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used], s));
   }
-  if (tiled)
-    hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
-                       L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
-                       dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
-                       nbdense, skip);
-  else
-    hipLaunchKernelGGL((k_sor_zmarch<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+  if constexpr (!std::is_same<S, bf16>::value) {
+    if (tiled)
+      hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+                         L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
+                         dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
+                         nbdense, skip);
+  }
+  if (!tiled)
+    hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
                        L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
                        dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0, nbx, nby,
                        zc, nbdense, skip);
@@ -662,7 +682,7 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   bool periodic = b[0] == 2 || b[2] == 2 || b[4] == 2;
   if (!inpass || L.sub_on_interior || periodic || (P.icnd == 2 && L.neumann)) {
     int usevolt = (level == 0 && P.mgform == 1);
-    pin_and_mirror<T>(h, L, level, P, phi, usevolt, skip, s);
+    pin_and_mirror<S>(h, L, level, P, phi, usevolt, skip, s);
   }
 }
 
@@ -679,15 +699,16 @@ static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int 
 
 // residual3d (f3d_mgrid.F:3367-3598) from red-black phi/rho into a natural
 // res array of type TR.
-template <typename T, typename TR>
-static void residual(wm_mg3d *h, Level &L, int level, const Params &P, const T *phi,
-                     const T *rho, TR *res, int mgform2init, const int *skip, hipStream_t s) {
+template <typename S, typename TR>
+static void residual(wm_mg3d *h, Level &L, int level, const Params &P, const S *phi,
+                     const S *rho, TR *res, int mgform2init, const int *skip, hipStream_t s) {
+  typedef typename St<S>::T T;
   LoopRange r = res_range(L, h->bounds);
   double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
   T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
     dzsqic = (T)(L.dzsqi * P.mgparam * cst);
   long long n = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
-  hipLaunchKernelGGL((k_residual<T, TR>), dim3(blocks(n)), dim3(256), 0, s, phi, rho, res, L.g,
+  hipLaunchKernelGGL((k_residual<S, TR>), dim3(blocks(n)), dim3(256), 0, s, phi, rho, res, L.g,
                      r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic, dysqic,
                      dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, skip);
   WM_LAUNCHED();
@@ -701,7 +722,7 @@ static void residual(wm_mg3d *h, Level &L, int level, const Params &P, const T *
       int nint = p == 0 ? L.nint : 0;
       long long cnt = (long long)nint + sub.n;
       if (cnt == 0) continue;
-      hipLaunchKernelGGL((k_residual_cond<T, TR>), dim3(blocks(cnt)), dim3(256), 0, s, phi,
+      hipLaunchKernelGGL((k_residual_cond<S, TR>), dim3(blocks(cnt)), dim3(256), 0, s, phi,
                          rho, res, L.g, nint, L.int_pos.p, sub, dxsqics, dysqics, dzsqics,
                          (T)P.mgparam, skip);
       WM_LAUNCHED();
@@ -757,9 +778,11 @@ static void expand_from(Level &L, T *phi, FI F, const T *pc, CI Cc, const int *s
 // residuals are written first into the natural residual array, from which
 // the fused kernel takes them.  Returns false when the separate kernels
 // must run.
-template <typename T>
+template <typename S>
 static bool resrestrict(wm_mg3d *h, Level &L, int level, const Params &P, Level &Lc,
                         const int *skip, hipStream_t s) {
+  typedef typename St<S>::T T;
+  typedef typename Coarse<S>::type SC;
   static const bool off = getenv("WARPMI_NO_RESRESTRICT") != nullptr;
   const int *b = h->bounds;
   for (int a = 0; a < 6; a++)
@@ -776,8 +799,8 @@ static bool resrestrict(wm_mg3d *h, Level &L, int level, const Params &P, Level 
   RestrictAxis ay{L.rlo[1].p, L.rhi[1].p, L.rw[1].p, L.rcmin[1], L.rcmax[1]};
   RestrictAxis az{L.rlo[2].p, L.rhi[2].p, L.rw[2].p, L.rcmin[2], L.rcmax[2]};
   T *res = res_of<T>(L).p;
-  const T *phi = phi_of<T>(L).p;
-  const T *rho = rho_of<T>(L).p;
+  const S *phi = phi_of<S>(L).p;
+  const S *rho = rho_of<S>(L).p;
   double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
   if (L.hasmask) {
     ensure_records<T>(h, L, level, P, true, 0);
@@ -789,15 +812,15 @@ static bool resrestrict(wm_mg3d *h, Level &L, int level, const Params &P, Level 
       int nint = p == 0 ? L.nint : 0;
       long long cnt = (long long)nint + sub.n;
       if (cnt == 0) continue;
-      hipLaunchKernelGGL((k_residual_cond<T, T>), dim3(blocks(cnt)), dim3(256), 0, s, phi, rho,
+      hipLaunchKernelGGL((k_residual_cond<S, T>), dim3(blocks(cnt)), dim3(256), 0, s, phi, rho,
                          res, L.g, nint, L.int_pos.p, sub, dxsqics, dysqics, dzsqics,
                          (T)P.mgparam, skip);
       WM_LAUNCHED();
     }
   }
   long long nco = (long long)(Lc.nx + 1) * (Lc.ny + 1) * (Lc.nz + 1);
-  hipLaunchKernelGGL((k_zero_faces<T, RBIdx>), dim3(blocks(nco)), dim3(256), 0, s,
-                     rho_of<T>(Lc).p, RBIdx{Lc.g}, Lc.nx, Lc.ny, Lc.nz, ax, ay, az, skip);
+  hipLaunchKernelGGL((k_zero_faces<SC, RBIdx>), dim3(blocks(nco)), dim3(256), 0, s,
+                     rho_of<SC>(Lc).p, RBIdx{Lc.g}, Lc.nx, Lc.ny, Lc.nz, ax, ay, az, skip);
   WM_LAUNCHED();
   int ncx = L.rcmax[0] - L.rcmin[0] + 1, ncy = L.rcmax[1] - L.rcmin[1] + 1,
       ncz = L.rcmax[2] - L.rcmin[2] + 1;
@@ -810,35 +833,53 @@ static bool resrestrict(wm_mg3d *h, Level &L, int level, const Params &P, Level 
   T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
     dzsqic = (T)(L.dzsqi * P.mgparam * cst);
   constexpr bool tablew = std::is_same<T, double>::value;
-  hipLaunchKernelGGL((k_resrestrict<T, tablew, RBIdx>), dim3(nb), dim3(256), 0, s, phi, rho,
+  hipLaunchKernelGGL((k_resrestrict<S, T, SC, tablew, RBIdx>), dim3(nb), dim3(256), 0, s, phi,
+                     rho,
                      (const T *)res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
                      dxsqic, dysqic, dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr,
-                     rho_of<T>(Lc).p, RBIdx{Lc.g}, ax, ay, az, (T)L.ff, nbx, nby, zc, nb, skip);
+                     rho_of<SC>(Lc).p, RBIdx{Lc.g}, ax, ay, az, (T)L.ff, nbx, nby, zc, nb, skip);
   WM_LAUNCHED();
   return true;
 }
 
-// vcycle (f3d_mgrid.F:1101-1591).
-template <typename T>
+template <typename S>
+static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
+  typedef typename Coarse<S>::type SC;
+  ExpandAxis ax{L.ej[0].p, L.ew[0].p, L.elo[0], L.ehi[0]};
+  ExpandAxis ay{L.ej[1].p, L.ew[1].p, L.elo[1], L.ehi[1]};
+  ExpandAxis az{L.ej[2].p, L.ew[2].p, L.elo[2], L.ehi[2]};
+  long long rows = 2ll * std::max(0, ay.hi - ay.lo + 1) * std::max(0, az.hi - az.lo + 1);
+  if (rows == 0 || ax.hi < ax.lo) return;
+  int nb = (int)std::min<long long>(8192, cdiv(rows, 4));
+  hipLaunchKernelGGL((k_expand_rb<S, SC>), dim3(nb), dim3(256), 0, s, phi_of<S>(L).p, L.g,
+                     (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, skip);
+  WM_LAUNCHED();
+}
+
+// vcycle (f3d_mgrid.F:1101-1591) on level data of storage type S (the
+// coarser levels use Coarse<S>).
+template <typename S>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s) {
+  typedef typename St<S>::T T;
+  typedef typename Coarse<S>::type SC;
   Level &L = *h->lv[level];
-  for (int i = 0; i < P.down; i++) sorpass<T>(h, L, level, P, skip, s);
+  for (int i = 0; i < P.down; i++) sorpass<S>(h, L, level, P, skip, s);
   if (L.coarser) {
     Level &Lc = *h->lv[level + 1];
-    if (!resrestrict<T>(h, L, level, P, Lc, skip, s)) {
-      residual<T, T>(h, L, level, P, phi_of<T>(L).p, rho_of<T>(L).p, res_of<T>(L).p, 0, skip,
+    if (!resrestrict<S>(h, L, level, P, Lc, skip, s)) {
+      residual<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
                      s);
-      restrict_to<T, T>(L, res_of<T>(L).p, rho_of<T>(Lc).p, RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny,
-                        Lc.nz);
+      restrict_to<T, SC>(L, res_of<T>(L).p, rho_of<SC>(Lc).p, RBIdx{Lc.g}, skip, s, Lc.nx,
+                         Lc.ny, Lc.nz);
     }
-    hipLaunchKernelGGL((k_fill<T>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
-                       phi_of<T>(Lc).p, 2 * Lc.g.cs, (T)0., skip);
+    hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
+                       phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
     WM_LAUNCHED();
-    vcycle<T>(h, level + 1, P, skip, s);
-    expand_from<T>(L, phi_of<T>(L).p, RBIdx{L.g}, phi_of<T>(Lc).p, RBIdx{Lc.g}, skip, s);
-    launch_bc<T>(phi_of<T>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+    vcycle<SC>(h, level + 1, P, skip, s);
+    expand_rb<S>(L, Lc, skip, s);
+    launch_bc<S>(phi_of<S>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
   }
-  for (int i = 0; i < P.up; i++) sorpass<T>(h, L, level, P, skip, s);
+  for (int i = 0; i < P.up; i++) sorpass<S>(h, L, level, P, skip, s);
 }
 
 template <typename T>
@@ -857,6 +898,32 @@ static int nconv_blocks(const RBGeom &g) {
   return (int)std::min<long long>(2048, std::max<long long>(1, cdiv(n, 256 * 8)));
 }
 
+// The low-precision half of a defect-correction iteration: the FP64
+// residual becomes the source of the error equation in storage type S, the
+// V-cycle runs from a zero error, and the error is added to the FP64 phi.
+template <typename S>
+static void dc_lowp(wm_mg3d *h, const Params &Pinit, long long nr, long long np, bool fused,
+                    double mgtol, int mgmaxiters, const int *skip, hipStream_t s) {
+  Level &L0 = *h->lv[0];
+  hipLaunchKernelGGL((k_res_to_rho<S>), dim3(blocks(nr)), dim3(256), 0, s, L0.res64.p,
+                     rho_of<S>(L0).p, L0.g, skip);
+  WM_LAUNCHED();
+  S zero;
+  std::memset(&zero, 0, sizeof(S));
+  hipLaunchKernelGGL((k_fill<S>), dim3(blocks(np)), dim3(256), 0, s, phi_of<S>(L0).p, np, zero,
+                     skip);
+  WM_LAUNCHED();
+  vcycle<S>(h, 0, Pinit, skip, s);
+  if (fused) {
+    hipLaunchKernelGGL((k_dc_update<S>), dim3(nconv_blocks(L0.g)), dim3(256), 0, s, L0.phi64.p,
+                       (const S *)phi_of<S>(L0).p, L0.g, h->state.p, mgtol, mgmaxiters);
+  } else {
+    hipLaunchKernelGGL((k_dc_add<S>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
+                       (const S *)phi_of<S>(L0).p, h->save64.p, np, skip);
+  }
+  WM_LAUNCHED();
+}
+
 // One iteration of the multigrid3dsolve loop (f3d_mgrid.F:313-434).
 static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int mgmaxiters,
                       hipStream_t s) {
@@ -867,11 +934,16 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
   } else {
     // Residual-correction form: the error equation on the residual of the
     // saved phi (mgform == 2), with the V-cycle in double or float.
+    // Without periodic boundaries the update is done in place on phi64
+    // and fused with the convergence test (k_dc_update); with them the
+    // boundary conditions must come between the update and the test.
+    const int *b = h->bounds;
+    bool fused = prec != WM_PREC_FP64 && !(b[0] == 2 || b[2] == 2 || b[4] == 2);
+    double *base = fused ? L0.phi64.p : h->save64.p;
     Params Pinit = P;
     Pinit.mgform = 2;
-    pin_and_mirror<double>(h, L0, 0, Pinit, h->save64.p, 1, skip, s);
-    residual<double, double>(h, L0, 0, Pinit, h->save64.p, h->rhosave64.p, L0.res64.p, 1,
-                             skip, s);
+    pin_and_mirror<double>(h, L0, 0, Pinit, base, 1, skip, s);
+    residual<double, double>(h, L0, 0, Pinit, base, h->rhosave64.p, L0.res64.p, 1, skip, s);
     long long nr = (long long)(L0.nx + 1) * (L0.ny + 1) * (L0.nz + 1);
     long long np = 2 * L0.g.cs;
     if (prec == WM_PREC_FP64) {
@@ -886,19 +958,13 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
       hipLaunchKernelGGL((k_dc_add<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
                          L0.phi64.p, h->save64.p, np, skip);
       WM_LAUNCHED();
+    } else if (prec == WM_PREC_FP32_DC) {
+      dc_lowp<float>(h, Pinit, nr, np, fused, mgtol, mgmaxiters, skip, s);
     } else {
-      hipLaunchKernelGGL((k_res_to_rho<float>), dim3(blocks(nr)), dim3(256), 0, s,
-                         L0.res64.p, L0.rho32.p, L0.g, skip);
-      WM_LAUNCHED();
-      hipLaunchKernelGGL((k_fill<float>), dim3(blocks(np)), dim3(256), 0, s, L0.phi32.p, np,
-                         0.f, skip);
-      WM_LAUNCHED();
-      vcycle<float>(h, 0, Pinit, skip, s);
-      hipLaunchKernelGGL((k_dc_add<float>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
-                         L0.phi32.p, h->save64.p, np, skip);
-      WM_LAUNCHED();
+      dc_lowp<bf16>(h, Pinit, nr, np, fused, mgtol, mgmaxiters, skip, s);
     }
     launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, skip, s);
+    if (fused) return;
   }
   hipLaunchKernelGGL((k_converge<double>), dim3(nconv_blocks(L0.g)), dim3(256), 0, s,
                      L0.phi64.p, h->save64.p, L0.g, h->state.p, mgtol, mgmaxiters, 1);
@@ -1108,7 +1174,6 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
                "wm_mg3d_solve: unknown precision");
     WM_REQUIRE(downpasses >= 0 && uppasses >= 0, "wm_mg3d_solve: negative pass count");
     int prec = (int)precision;
-    if (prec == WM_PREC_BF16_DC) prec = WM_PREC_FP32_DC;
     hipStream_t us = wm_stream(stream);
     hipStream_t s = h->s;
     Params P{mgparam, (int)mgform, (int)downpasses, (int)uppasses, (int)lcndbndy, (int)icndbndy};
@@ -1119,7 +1184,12 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     // Buffers and conductor records.
     prepare_levels<double>(h, P);
     bool dc = !(prec == WM_PREC_FP64 && P.mgform == 1);
-    if (prec == WM_PREC_FP32_DC) prepare_levels<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd});
+    if (prec != WM_PREC_FP64)
+      prepare_levels<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd});
+    if (prec == WM_PREC_BF16_DC) {
+      L0.phi16.alloc(2 * L0.g.cs);
+      L0.rho16.alloc(2 * L0.g.cs);
+    }
     if (dc) {
       L0.res64.alloc((size_t)(L0.nx + 3) * (L0.ny + 3) * (L0.nz + 3));
       h->rhosave64.alloc(2 * L0.g.cs);

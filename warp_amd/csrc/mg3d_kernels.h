@@ -58,6 +58,31 @@ struct NatIdx {
 #define WM_SKIP() \
   if (skip && *skip) return
 
+// Storage types.  Level arrays hold S; arithmetic is done in St<S>::T.
+// For double and float both are the same type and the conversions are the
+// identity.  bf16 is the upper half of an IEEE single, kept as raw bits and
+// rounded to nearest even on store.
+struct bf16 {
+  uint16_t u;
+};
+template <typename S>
+struct St {
+  typedef S T;
+  __host__ __device__ static S ld(S v) { return v; }
+  __host__ __device__ static S st(S v) { return v; }
+};
+template <>
+struct St<bf16> {
+  typedef float T;
+  __device__ static float ld(bf16 v) { return __uint_as_float((uint32_t)v.u << 16); }
+  __device__ static bf16 st(float f) {
+    uint32_t x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return bf16{(uint16_t)((x >> 16) | 0x40u)};
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return bf16{(uint16_t)(x >> 16)};
+  }
+};
+
 // Side encodings of the subgrid-conductor stencil, one per neighbour
 // (order mx,px,my,py,mz,pz), 2 bits each: the neighbour value is used raw,
 // multiplied by k, divided by k, or replaced by the constant k.  This is
@@ -171,9 +196,9 @@ __global__ __launch_bounds__(256) void k_sor_halfpass(
 // it, which the neighbouring waves of the block load in the same step.
 // Blocks: (row segment jseg, 4 rows, z chunk), remapped per XCD.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename S, typename T = typename St<S>::T>
 __global__ __launch_bounds__(256) void k_sor_zmarch(
-    T *__restrict__ phi, const T *__restrict__ rho, RBGeom g, int c, int ixmin,
+    S *__restrict__ phi, const S *__restrict__ rho, RBGeom g, int c, int ixmin,
     int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
     T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
     int j0, int nbx, int nby, int zc, int nbdense, const int *__restrict__ skip) {
@@ -193,9 +218,9 @@ __global__ __launch_bounds__(256) void k_sor_zmarch(
     int zb = min(za + zc - 1, izmax);
     int py = iy + 1;
     long long pl = (long long)g.W * g.PY;
-    const T *o = phi + (1 - c) * g.cs;
-    T *me = phi + c * g.cs;
-    const T *rh = rho + c * g.cs;
+    const S *o = phi + (1 - c) * g.cs;
+    S *me = phi + c * g.cs;
+    const S *rh = rho + c * g.cs;
     // Loads use jc, clamped into the padded row, so they stay in bounds for
     // lanes past the end of the row; such lanes never store.
     int jc = min(j, g.W - 1);
@@ -204,19 +229,19 @@ __global__ __launch_bounds__(256) void k_sor_zmarch(
     // Loads of one plane (padded index pz), issued a plane ahead of use.
     auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw) {
       long long base = pl * pz;
-      ozp = o[rowc + base + pl];
-      oym = o[rowc + base - g.W];
-      oyp = o[rowc + base + g.W];
-      own = me[rowc + base];
-      r = rh[rowc + base];
+      ozp = St<S>::ld(o[rowc + base + pl]);
+      oym = St<S>::ld(o[rowc + base - g.W]);
+      oyp = St<S>::ld(o[rowc + base + g.W]);
+      own = St<S>::ld(me[rowc + base]);
+      r = St<S>::ld(rh[rowc + base]);
       int p0 = (c + py + pz + 1) & 1;
       int jn = p0 ? j + 1 : j - 1;
       bool e = p0 ? (lane == 63) : (lane == 0);
-      edge = e ? o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)] : (T)0.;
+      edge = e ? St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]) : (T)0.;
       mw = mask ? mask[(meoff + base) >> 5] : 0u;
     };
-    T oz_m = o[rowc + pl * za];        // padded plane za: iz = za-1
-    T oz_0 = o[rowc + pl * (za + 1)];  // iz = za
+    T oz_m = St<S>::ld(o[rowc + pl * za]);        // padded plane za: iz = za-1
+    T oz_0 = St<S>::ld(o[rowc + pl * (za + 1)]);  // iz = za
     T ozp, oym, oyp, own, r, eg;
     uint32_t mw;
     load(za + 1, ozp, oym, oyp, own, r, eg, mw);
@@ -235,8 +260,8 @@ __global__ __launch_bounds__(256) void k_sor_zmarch(
       if (active) {
         T pxm = p0 ? oz_0 : sh;
         T pxp = p0 ? sh : oz_0;
-        me[rowc + pl * pz] = r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic +
-                             (oz_m + ozp) * dzsqic + spm1 * own;
+        me[rowc + pl * pz] = St<S>::st(r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic +
+                                       (oz_m + ozp) * dzsqic + spm1 * own);
       }
       oz_m = oz_0;
       oz_0 = ozp;
@@ -249,21 +274,21 @@ __global__ __launch_bounds__(256) void k_sor_zmarch(
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
   const T *k = sub.k + 6 * i;
-  T pxm = side_value<T>(m & 3u, k[0], phi[rb_index(g, ix - 1, iy, iz)]);
-  T pxp = side_value<T>((m >> 2) & 3u, k[1], phi[rb_index(g, ix + 1, iy, iz)]);
-  T pym = side_value<T>((m >> 4) & 3u, k[2], phi[rb_index(g, ix, iy - 1, iz)]);
-  T pyp = side_value<T>((m >> 6) & 3u, k[3], phi[rb_index(g, ix, iy + 1, iz)]);
-  T pzm = side_value<T>((m >> 8) & 3u, k[4], phi[rb_index(g, ix, iy, iz - 1)]);
-  T pzp = side_value<T>((m >> 10) & 3u, k[5], phi[rb_index(g, ix, iy, iz + 1)]);
+  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(phi[rb_index(g, ix - 1, iy, iz)]));
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(phi[rb_index(g, ix + 1, iy, iz)]));
+  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(phi[rb_index(g, ix, iy - 1, iz)]));
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(phi[rb_index(g, ix, iy + 1, iz)]));
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(phi[rb_index(g, ix, iy, iz - 1)]));
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(phi[rb_index(g, ix, iy, iz + 1)]));
   long long ce = rb_index(g, ix, iy, iz);
-  T prev = phi[ce];
-  phi[ce] = (rho[ce] + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
-             (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev;
+  T prev = St<S>::ld(phi[ce]);
+  phi[ce] = St<S>::st((St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
+                       (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev);
 }
 
 // cond_potmg (f3d_mgrid.F:2426-2472): pin interior conductor points.
-template <typename T>
-__global__ __launch_bounds__(256) void k_pin(T *__restrict__ phi, RBGeom g, int n,
+template <typename S>
+__global__ __launch_bounds__(256) void k_pin(S *__restrict__ phi, RBGeom g, int n,
                                              const int *__restrict__ pos,
                                              const double *__restrict__ volt,
                                              int usevolt,
@@ -272,7 +297,8 @@ __global__ __launch_bounds__(256) void k_pin(T *__restrict__ phi, RBGeom g, int 
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   long long e = rb_index(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-  phi[e] = usevolt ? (T)volt[i] : (T)0.;
+  typedef typename St<S>::T T;
+  phi[e] = St<S>::st(usevolt ? (T)volt[i] : (T)0.);
 }
 
 // condbndymgint (f3d_mgrid.F:2982-3092): Neumann subgrid mirrors.  The
@@ -324,9 +350,9 @@ __global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
 // periodic guards are filled afterwards by the boundary kernel).  Points in
 // the conductor mask are written by k_residual_cond.
 // ---------------------------------------------------------------------------
-template <typename T, typename TR>
+template <typename S, typename TR, typename T = typename St<S>::T>
 __global__ __launch_bounds__(256) void k_residual(
-    const T *__restrict__ phi, const T *__restrict__ rho, TR *__restrict__ res,
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res,
     RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax,
     T dxsqic, T dysqic, T dzsqic, T mgparam, const uint32_t *__restrict__ mask,
     const int *__restrict__ skip) {
@@ -346,11 +372,13 @@ __global__ __launch_bounds__(256) void k_residual(
   }
   long long ce = rb_index(g, ix, iy, iz);
   if (mask && ((mask[ce >> 5] >> (ce & 31)) & 1u)) return;
-  T r = rho[ce] +
-        (phi[rb_index(g, ix - 1, iy, iz)] + phi[rb_index(g, ix + 1, iy, iz)]) * dxsqic +
-        (phi[rb_index(g, ix, iy - 1, iz)] + phi[rb_index(g, ix, iy + 1, iz)]) * dysqic +
-        (phi[rb_index(g, ix, iy, iz - 1)] + phi[rb_index(g, ix, iy, iz + 1)]) * dzsqic -
-        phi[ce] * mgparam;
+#define LD(a) St<S>::ld(a)
+  T r = LD(rho[ce]) +
+        (LD(phi[rb_index(g, ix - 1, iy, iz)]) + LD(phi[rb_index(g, ix + 1, iy, iz)])) * dxsqic +
+        (LD(phi[rb_index(g, ix, iy - 1, iz)]) + LD(phi[rb_index(g, ix, iy + 1, iz)])) * dysqic +
+        (LD(phi[rb_index(g, ix, iy, iz - 1)]) + LD(phi[rb_index(g, ix, iy, iz + 1)])) * dzsqic -
+        LD(phi[ce]) * mgparam;
+#undef LD
   res[t] = (TR)r;
 }
 
@@ -359,9 +387,9 @@ __global__ __launch_bounds__(256) void k_residual(
 // surfaces the reference zeroes some neighbours' residuals while walking
 // the even list then the odd list; launching one parity per call (even
 // first) reproduces that order.
-template <typename T, typename TR>
+template <typename S, typename TR, typename T = typename St<S>::T>
 __global__ __launch_bounds__(256) void k_residual_cond(
-    const T *__restrict__ phi, const T *__restrict__ rho, TR *__restrict__ res,
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res,
     RBGeom g, int nint, const int *__restrict__ ipos, SubList<T> sub,
     T dxsqics, T dysqics, T dzsqics, T mgparam, const int *__restrict__ skip) {
   WM_SKIP();
@@ -376,15 +404,15 @@ __global__ __launch_bounds__(256) void k_residual_cond(
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
   const T *k = sub.k + 6 * i;
-  T pxm = side_value<T>(m & 3u, k[0], phi[rb_index(g, ix - 1, iy, iz)]);
-  T pxp = side_value<T>((m >> 2) & 3u, k[1], phi[rb_index(g, ix + 1, iy, iz)]);
-  T pym = side_value<T>((m >> 4) & 3u, k[2], phi[rb_index(g, ix, iy - 1, iz)]);
-  T pyp = side_value<T>((m >> 6) & 3u, k[3], phi[rb_index(g, ix, iy + 1, iz)]);
-  T pzm = side_value<T>((m >> 8) & 3u, k[4], phi[rb_index(g, ix, iy, iz - 1)]);
-  T pzp = side_value<T>((m >> 10) & 3u, k[5], phi[rb_index(g, ix, iy, iz + 1)]);
+  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(phi[rb_index(g, ix - 1, iy, iz)]));
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(phi[rb_index(g, ix + 1, iy, iz)]));
+  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(phi[rb_index(g, ix, iy - 1, iz)]));
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(phi[rb_index(g, ix, iy + 1, iz)]));
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(phi[rb_index(g, ix, iy, iz - 1)]));
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(phi[rb_index(g, ix, iy, iz + 1)]));
   long long ce = rb_index(g, ix, iy, iz);
-  T r = sub.fac[i] * (rho[ce] + (pxm + pxp) * dxsqics + (pym + pyp) * dysqics +
-                      (pzm + pzp) * dzsqics - phi[ce] * mgparam * sub.denom[i]);
+  T r = sub.fac[i] * (St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqics + (pym + pyp) * dysqics +
+                      (pzm + pzp) * dzsqics - St<S>::ld(phi[ce]) * mgparam * sub.denom[i]);
   res[nat_index(ng, ix, iy, iz)] = (TR)r;
   uint32_t z = sub.zero[i];
   if (z) {
@@ -432,12 +460,13 @@ __global__ void k_bc_axis(T *__restrict__ u, Idx I, int axis, int nx, int ny,
   else core = true;
   if (lwithdirichlet) {
     if (core) {
-      if (bm == 0) at(-1) = (T)2. * at(0) - at(1);
-      if (bp == 0) at(n + 1) = (T)2. * at(n) - at(n - 1);
+      typedef typename St<T>::T C;
+      if (bm == 0) at(-1) = St<T>::st((C)2. * St<T>::ld(at(0)) - St<T>::ld(at(1)));
+      if (bp == 0) at(n + 1) = St<T>::st((C)2. * St<T>::ld(at(n)) - St<T>::ld(at(n - 1)));
     }
   } else if (lzerodirichlet) {
-    if (bm == 0) at(-1) = (T)0.;
-    if (bp == 0) at(n + 1) = (T)0.;
+    if (bm == 0) at(-1) = St<T>::st((typename St<T>::T)0.);
+    if (bp == 0) at(n + 1) = St<T>::st((typename St<T>::T)0.);
   }
   if (!core) return;
   if (bm == 1) at(-1) = at(1);
@@ -589,13 +618,15 @@ __device__ inline void restrict_sum(const T *lds, int fxl, int fyl, const int *z
   }
 }
 
-template <typename T, bool TABLEW, typename OutIdx>
+template <typename S, typename TR, typename TC, bool TABLEW, typename OutIdx>
 __global__ __launch_bounds__(256) void k_resrestrict(
-    const T *__restrict__ phi, const T *__restrict__ rho, const T *__restrict__ res,
+    const S *__restrict__ phi, const S *__restrict__ rho, const TR *__restrict__ res,
     RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax,
-    T dxsqic, T dysqic, T dzsqic, T mgparam, const uint32_t *__restrict__ mask,
-    T *__restrict__ out, OutIdx O, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
-    T ff, int nbx, int nby, int zc, int nblocks, const int *__restrict__ skip) {
+    typename St<S>::T dxsqic, typename St<S>::T dysqic, typename St<S>::T dzsqic,
+    typename St<S>::T mgparam, const uint32_t *__restrict__ mask, TC *__restrict__ out,
+    OutIdx O, RestrictAxis ax, RestrictAxis ay, RestrictAxis az, typename St<S>::T ff,
+    int nbx, int nby, int zc, int nblocks, const int *__restrict__ skip) {
+  typedef typename St<S>::T T;
   WM_SKIP();
   __shared__ T lds[4 * RT_FX * RT_FY];
   int lb = xcd_remap(blockIdx.x, nblocks);
@@ -633,13 +664,15 @@ __global__ __launch_bounds__(256) void k_resrestrict(
         bool dense = zin && fx >= ixmin && fx <= ixmax && fy >= iymin && fy <= iymax &&
                      !(mask && ((mask[ce >> 5] >> (ce & 31)) & 1u));
         if (dense) {
-          v = rho[ce] +
-              (phi[rb_index(g, fx - 1, fy, fz)] + phi[rb_index(g, fx + 1, fy, fz)]) * dxsqic +
-              (phi[rb_index(g, fx, fy - 1, fz)] + phi[rb_index(g, fx, fy + 1, fz)]) * dysqic +
-              (phi[rb_index(g, fx, fy, fz - 1)] + phi[rb_index(g, fx, fy, fz + 1)]) * dzsqic -
-              phi[ce] * mgparam;
+#define LD(a) St<S>::ld(a)
+          v = LD(rho[ce]) +
+              (LD(phi[rb_index(g, fx - 1, fy, fz)]) + LD(phi[rb_index(g, fx + 1, fy, fz)])) * dxsqic +
+              (LD(phi[rb_index(g, fx, fy - 1, fz)]) + LD(phi[rb_index(g, fx, fy + 1, fz)])) * dysqic +
+              (LD(phi[rb_index(g, fx, fy, fz - 1)]) + LD(phi[rb_index(g, fx, fy, fz + 1)])) * dzsqic -
+              LD(phi[ce]) * mgparam;
+#undef LD
         } else {
-          v = res[(fx + 1) + (long long)sx * (fy + 1) + (long long)sxy * (fz + 1)];
+          v = (T)res[(fx + 1) + (long long)sx * (fy + 1) + (long long)sxy * (fz + 1)];
         }
         pl[(fy - fyl) * RT_FX + (fx - fxl)] = v;
       }
@@ -651,7 +684,7 @@ __global__ __launch_bounds__(256) void k_resrestrict(
       T o;
       restrict_sum<T, TABLEW>(lds, fxl, fyl, zslot, x0, x1, y0, y1, z0, z1, wx, wy,
                               az.w + 5 * cz, ff, o);
-      out[O(cx, cy, cz)] = o;
+      out[O(cx, cy, cz)] = St<TC>::st((typename St<TC>::T)o);
     }
     __syncthreads();
   }
@@ -714,6 +747,52 @@ __global__ __launch_bounds__(256) void k_expand(
             wx * (one - wy) * wz * pc[Cc(jx + 1, jy, jz + 1)] +
             (one - wx) * wy * wz * pc[Cc(jx, jy + 1, jz + 1)] +
             wx * wy * wz * pc[Cc(jx + 1, jy + 1, jz + 1)];
+}
+
+// expand3d on red-black fine and coarse arrays: one wavefront per fine row
+// of one colour, lanes over the row's elements, so every fine load and
+// store is contiguous; same sum as k_expand.
+template <typename S, typename SC>
+__global__ __launch_bounds__(256) void k_expand_rb(
+    S *__restrict__ phi, RBGeom g, const SC *__restrict__ pc, RBGeom gc, ExpandAxis ax,
+    ExpandAxis ay, ExpandAxis az, const int *__restrict__ skip) {
+  typedef typename St<S>::T T;
+  WM_SKIP();
+  int nyr = ay.hi - ay.lo + 1, nzr = az.hi - az.lo + 1;
+  if (nyr <= 0 || nzr <= 0 || ax.hi < ax.lo) return;
+  int lane = threadIdx.x & 63;
+  int nrows = 2 * nyr * nzr;
+  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += gridDim.x * 4) {
+    int c = row / (nyr * nzr);
+    int rr = row - c * (nyr * nzr);
+    int iz = az.lo + rr / nyr;
+    int iy = ay.lo + rr % nyr;
+    int py = iy + 1, pz = iz + 1;
+    int p0 = (c + py + pz + 1) & 1;
+    int jy = ay.j[iy], jz = az.j[iz];
+    T wy = (T)ay.w[iy], wz = (T)az.w[iz];
+    T one = (T)1.;
+    long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
+    // j range of this row: ix = p0 + 2j - 1 in [ax.lo, ax.hi]
+    int jlo = (ax.lo + 2 - p0) / 2, jhi = (ax.hi + 1 - p0) / 2;
+    for (int j = jlo + lane; j <= jhi; j += 64) {
+      int ix = p0 + 2 * j - 1;
+      int jx = ax.j[ix];
+      T wx = (T)ax.w[ix];
+      long long fi = base + j;
+#define PC(a, b, c) ((T)St<SC>::ld(pc[rb_index(gc, a, b, c)]))
+      phi[fi] = St<S>::st(St<S>::ld(phi[fi]) +
+                (one - wx) * (one - wy) * (one - wz) * PC(jx, jy, jz) +
+                wx * (one - wy) * (one - wz) * PC(jx + 1, jy, jz) +
+                (one - wx) * wy * (one - wz) * PC(jx, jy + 1, jz) +
+                wx * wy * (one - wz) * PC(jx + 1, jy + 1, jz) +
+                (one - wx) * (one - wy) * wz * PC(jx, jy, jz + 1) +
+                wx * (one - wy) * wz * PC(jx + 1, jy, jz + 1) +
+                (one - wx) * wy * wz * PC(jx, jy + 1, jz + 1) +
+                wx * wy * wz * PC(jx + 1, jy + 1, jz + 1));
+#undef PC
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -813,32 +892,11 @@ __device__ inline double wave_max(double v) {
 // mgerror = max over nodes 0..n of |phisave-phi|, then phisave = phi.  The
 // last block to finish closes the reduction, counts the V-cycle and sets
 // `done` when mgerror <= mgtol or mgiters == mgmaxiters.
-template <typename T>
-__global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
-                                                  T *__restrict__ save, RBGeom g,
-                                                  SolveState *st, double mgtol,
-                                                  int mgmaxiters, int docopy) {
-  if (st->done) return;
-  __shared__ double part[4];
-  long long n = 2 * g.cs;
-  double m = 0.;
-  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n;
-       t += (long long)gridDim.x * 256) {
-    int c = (int)(t / g.cs);
-    long long e = t - c * g.cs;
-    int j = (int)(e % g.W);
-    long long r = e / g.W;
-    int py = (int)(r % g.PY);
-    int pz = (int)(r / g.PY);
-    int px = ((c + py + pz + 1) & 1) + 2 * j;
-    int ix = px - 1, iy = py - 1, iz = pz - 1;
-    T cur = phi[t];
-    if (ix >= 0 && ix <= g.nx && iy >= 0 && iy <= g.ny && iz >= 0 && iz <= g.nz) {
-      double d = fabs((double)save[t] - (double)cur);
-      m = d > m ? d : m;
-    }
-    if (docopy) save[t] = cur;
-  }
+// Block max -> global max with a last-block reduction that updates the
+// solve state: mgerror, mgiters, and done when the loop of
+// multigrid3dsolve would stop (f3d_mgrid.F:383-434).
+__device__ inline void converge_finish(double m, double *part, SolveState *st, double mgtol,
+                                       int mgmaxiters) {
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -861,6 +919,72 @@ __global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
+                                                  T *__restrict__ save, RBGeom g,
+                                                  SolveState *st, double mgtol,
+                                                  int mgmaxiters, int docopy) {
+  if (st->done) return;
+  __shared__ double part[4];
+  // one wavefront per padded row of one colour; lanes walk the row
+  int lane = threadIdx.x & 63;
+  int nrows = 2 * g.PY * g.PZ;
+  double m = 0.;
+  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += gridDim.x * 4) {
+    int c = row / (g.PY * g.PZ);
+    int rr = row - c * (g.PY * g.PZ);
+    int pz = rr / g.PY, py = rr - pz * g.PY;
+    int p0 = (c + py + pz + 1) & 1;
+    bool rowin = py >= 1 && py <= g.ny + 1 && pz >= 1 && pz <= g.nz + 1;
+    long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
+    for (int j = lane; j < g.W; j += 64) {
+      T cur = phi[base + j];
+      int ix = p0 + 2 * j - 1;
+      if (rowin && ix >= 0 && ix <= g.nx) {
+        double d = fabs((double)save[base + j] - (double)cur);
+        m = d > m ? d : m;
+      }
+      if (docopy) save[base + j] = cur;
+    }
+  }
+  converge_finish(m, part, st, mgtol, mgmaxiters);
+}
+
+// Defect-correction update fused with the convergence test: phi += e over
+// every element (guards included) and max |phi_new - phi_old| over the
+// nodes 0..n, reduced as in k_converge.  phi is both the saved phi of the
+// iteration and the result, so no copy is kept.
+template <typename TL>
+__global__ __launch_bounds__(256) void k_dc_update(double *__restrict__ phi,
+                                                   const TL *__restrict__ e, RBGeom g,
+                                                   SolveState *st, double mgtol,
+                                                   int mgmaxiters) {
+  if (st->done) return;
+  __shared__ double part[4];
+  int lane = threadIdx.x & 63;
+  int nrows = 2 * g.PY * g.PZ;
+  double m = 0.;
+  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += gridDim.x * 4) {
+    int c = row / (g.PY * g.PZ);
+    int rr = row - c * (g.PY * g.PZ);
+    int pz = rr / g.PY, py = rr - pz * g.PY;
+    int p0 = (c + py + pz + 1) & 1;
+    bool rowin = py >= 1 && py <= g.ny + 1 && pz >= 1 && pz <= g.nz + 1;
+    long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
+    for (int j = lane; j < g.W; j += 64) {
+      double old = phi[base + j];
+      double nw = (double)St<TL>::ld(e[base + j]) + old;
+      int ix = p0 + 2 * j - 1;
+      if (rowin && ix >= 0 && ix <= g.nx) {
+        double d = fabs(old - nw);
+        m = d > m ? d : m;
+      }
+      phi[base + j] = nw;
+    }
+  }
+  converge_finish(m, part, st, mgtol, mgmaxiters);
+}
+
 // Defect-correction update: phi64 = e + phisave over the whole array
 // (mgform=2 in multigrid3dsolve, f3d_mgrid.F:364-368) with e the
 // low-precision correction.
@@ -871,7 +995,7 @@ __global__ __launch_bounds__(256) void k_dc_add(double *phi, const TL *e,
                                                 const int *__restrict__ skip) {
   WM_SKIP();
   long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t < n) phi[t] = (double)e[t] + save[t];
+  if (t < n) phi[t] = (double)St<TL>::ld(e[t]) + save[t];
 }
 
 // Copy of level-0 residual (natural) into the low-precision rho (red-black),
@@ -890,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_res_to_rho(const double *__restrict__ r
   int iy = (int)(u % sy);
   int iz = (int)(u / sy);
   NatGeom ng{g.nx, g.ny, g.nz};
-  rho[rb_index(g, ix, iy, iz)] = (TL)res[nat_index(ng, ix, iy, iz)];
+  rho[rb_index(g, ix, iy, iz)] = St<TL>::st((typename St<TL>::T)res[nat_index(ng, ix, iy, iz)]);
 }
 
 template <typename TA, typename TB>

@@ -359,6 +359,86 @@ __global__ void k_rhobc(double *__restrict__ rho, int axis, int nx, int ny, int 
   }
 }
 
+// getzmmnt's whole-beam moment sums zmmnts0(1:33) (top.F:1059-1136), in
+// two deterministic passes: per-block partial sums (grid-stride loop per
+// thread, then a fixed-order reduction over the block), then one block
+// that adds the partials in block order.
+#define NZMMNT 33
+__device__ inline void zmmnt_add(double *mm, double x, double y, double z, double ux, double uy,
+                                 double uz, double g, bool usegaminv) {
+  const double smallpos = 1.0e-36;
+  double vzi = 1. / (uz + smallpos);
+  mm[0] += 1.;
+  mm[1] += x;
+  mm[2] += y;
+  mm[3] += z;
+  if (!usegaminv) {
+    mm[6] += ux; mm[7] += uy; mm[8] += uz;
+    mm[16] += ux * ux; mm[17] += uy * uy; mm[18] += uz * uz;
+    mm[21] += z * uz; mm[24] += x * uz; mm[25] += y * uz;
+    mm[26] += ux * uz; mm[27] += uy * uz;
+    mm[28] += x * ux; mm[29] += y * uy; mm[30] += x * uy; mm[31] += y * ux;
+    mm[32] += ux * uy;
+  } else {
+    mm[6] += ux * g; mm[7] += uy * g; mm[8] += uz * g;
+    mm[16] += (g * ux) * (g * ux); mm[17] += (g * uy) * (g * uy); mm[18] += (g * uz) * (g * uz);
+    mm[21] += z * g * uz; mm[24] += x * uz * g; mm[25] += y * uz * g;
+    mm[26] += ux * uz * (g * g); mm[27] += uy * uz * (g * g);
+    mm[28] += x * ux * g; mm[29] += y * uy * g; mm[30] += x * uy * g; mm[31] += y * ux * g;
+    mm[32] += ux * uy * (g * g);
+  }
+  mm[9] += x * y;
+  mm[11] += x * x;
+  mm[12] += y * y;
+  mm[13] += z * z;
+  mm[4] += ux * vzi;
+  mm[5] += uy * vzi;
+  mm[10] += ux * uy * (vzi * vzi);
+  mm[14] += (ux * vzi) * (ux * vzi);
+  mm[15] += (uy * vzi) * (uy * vzi);
+  mm[19] += x * ux * vzi;
+  mm[20] += y * uy * vzi;
+  mm[22] += x * uy * vzi;
+  mm[23] += y * ux * vzi;
+}
+
+__global__ __launch_bounds__(256) void k_zmmnt_partial(
+    long long np, const double *__restrict__ xp, const double *__restrict__ yp,
+    const double *__restrict__ zp, const double *__restrict__ uxp,
+    const double *__restrict__ uyp, const double *__restrict__ uzp,
+    const double *__restrict__ gaminv, int usegaminv, double *__restrict__ part) {
+  __shared__ double red[4][NZMMNT];
+  double mm[NZMMNT];
+  for (int k = 0; k < NZMMNT; k++) mm[k] = 0.;
+  for (long long ip = (long long)blockIdx.x * 256 + threadIdx.x; ip < np;
+       ip += (long long)gridDim.x * 256)
+    zmmnt_add(mm, xp[ip], yp[ip], zp[ip], uxp[ip], uyp[ip], uzp[ip],
+              usegaminv ? gaminv[ip] : 1., usegaminv != 0);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < NZMMNT; k++) {
+    double v = mm[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[w][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NZMMNT) {
+    int k = threadIdx.x;
+    part[(long long)blockIdx.x * NZMMNT + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
+}
+
+__global__ void k_zmmnt_final(const double *__restrict__ part, int nb, double *__restrict__ mm) {
+  int k = threadIdx.x;
+  if (k >= NZMMNT) return;
+  double s = 0.;
+  for (int b = 0; b < nb; b++) s += part[(long long)b * NZMMNT + k];
+  mm[k] = s;
+}
+
+static int zmmnt_blocks(long long np) {
+  return (int)std::max<long long>(1, std::min<long long>(1024, (np + 255) / 256));
+}
+
 static int nblk(long long n) { return (int)std::max<long long>(1, cdiv(n, 256)); }
 
 static Mesh make_mesh(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
@@ -564,6 +644,29 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
     hipLaunchKernelGGL(k_gather_push, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
                        (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi, tdyi,
                        tdzi, ch, fulldt, xmmax, ymmax, zmmax, (unsigned long long *)nlost);
+    WM_LAUNCHED();
+  });
+}
+
+int64_t wm_getzmmnt_worksize(int64_t np) {
+  return (int64_t)sizeof(double) * NZMMNT * zmmnt_blocks(np);
+}
+
+int wm_getzmmnt(int64_t np, const double *xp, const double *yp, const double *zp,
+                const double *uxp, const double *uyp, const double *uzp, const double *gaminv,
+                int64_t usegaminv, double *mm, void *work, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(np >= 0, "wm_getzmmnt: np < 0");
+    WM_REQUIRE(mm && work, "wm_getzmmnt: NULL output or work");
+    WM_REQUIRE(np == 0 || (xp && yp && zp && uxp && uyp && uzp && (gaminv || !usegaminv)),
+               "wm_getzmmnt: NULL array");
+    hipStream_t s = wm_stream(stream);
+    int nb = zmmnt_blocks(np);
+    double *part = (double *)work;
+    hipLaunchKernelGGL(k_zmmnt_partial, dim3(nb), dim3(256), 0, s, (long long)np, xp, yp, zp, uxp,
+                       uyp, uzp, gaminv, (int)usegaminv, part);
+    WM_LAUNCHED();
+    hipLaunchKernelGGL(k_zmmnt_final, dim3(1), dim3(64), 0, s, (const double *)part, nb, mm);
     WM_LAUNCHED();
   });
 }
