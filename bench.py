@@ -107,7 +107,7 @@ def main():
     N.call("wm_mg3d_set_timing", solver._h, 0)
     nx, ny, nz = cfg["n"]
     nodes = (nx + 1) * (ny + 1) * (nz + 1)
-    s_bytes = 8
+    s_bytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.precision]  # finest-level storage
     alg_bytes = 2 * s_bytes * nodes  # read phi (s) + rho/2 + write phi/2 per node
     avg_ms = ms_tot.value / max(nl.value, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
@@ -153,7 +153,7 @@ def main():
         "vcycles_per_solve": float(np.mean(iters)),
         "vcycles_cold_solve": cold_iters,
         "ms_per_solve": ms_solve,
-        "roofline": {"kernel": "k_sor_halfpass (finest level)", "bound": "hbm",
+        "roofline": {"kernel": "k_sor_zmarch (finest-level SOR half sweep)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},

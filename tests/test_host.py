@@ -160,3 +160,32 @@ def test_beam_weights_follow_current_rule():
     r2 = (kv.xp[:m].numpy() / 0.006) ** 2 + (kv.yp[:m].numpy() / 0.006) ** 2
     assert r2.max() <= 1.0
     assert kv.sw[0] == pytest.approx(1e-3 * 0.06 / (configs.VZ * echarge * n), rel=1e-12)
+
+
+def test_find_mgparam_search_logic():
+    """find_mgparam's search on a stand-in solver whose V-cycle count is a
+    convex function of mgparam (minimum at 1.6): the search stops at a
+    mgparam with the minimal count, and restores mgtol."""
+    import torch
+    from warp_amd.find_mgparam import find_mgparam
+
+    class Stub:
+        def __init__(self):
+            self.mgparam, self.downpasses, self.uppasses = 1.2, 1, 1
+            self.mgiters, self.mgmaxiters, self.mgtol, self.mgerror = 0, 100, 1e-6, 0.
+            self.nx = self.ny = self.nz = 4
+            self.bounds = [0] * 6
+            self.phi = torch.zeros((7, 7, 7), dtype=torch.float64)
+            self.calls = 0
+
+        def solve(self, iwhich=0):
+            self.calls += 1
+            self.mgiters = 5 + int(round(400 * (self.mgparam - 1.6) ** 2)) + 2 * (self.downpasses - 1)
+            time.sleep(1e-3 * self.mgiters * (self.downpasses + self.uppasses))
+
+    import time
+    s = Stub()
+    find_mgparam(s)
+    assert abs(s.mgparam - 1.6) < 0.05
+    assert 5 + int(round(400 * (s.mgparam - 1.6) ** 2)) == 5
+    assert s.downpasses == 1 and s.uppasses == 1 and s.mgtol == 1e-6

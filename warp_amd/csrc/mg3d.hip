@@ -580,7 +580,7 @@ template <typename T>
 static void ensure_buffers(Level &L, bool with_res) {
   phi_of<T>(L).alloc(2 * L.g.cs);
   rho_of<T>(L).alloc(2 * L.g.cs);
-  if (with_res) res_of<T>(L).alloc((size_t)(L.nx + 3) * (L.ny + 3) * (L.nz + 3));
+  if (with_res) res_of<T>(L).alloc(std::max<size_t>(2 * L.g.cs, (size_t)(L.nx + 3) * (L.ny + 3) * (L.nz + 3)));
 }
 
 template <typename T, typename Idx>
@@ -644,10 +644,12 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   int j0 = (r.ixmin + 1) / 2, j1 = (r.ixmax + 1) / 2;
   int nbx = tiled ? (int)cdiv(L.g.W, 64) : (int)cdiv(j1 - j0 + 1, 64);
   long long cols = (long long)nbx * nby;
-  int zc = (int)std::min<long long>(64, std::max<long long>(2, cdiv((long long)nzr * cols, 2048)));
+  static const int zc_env = getenv("WARPMI_SOR_ZC") ? atoi(getenv("WARPMI_SOR_ZC")) : 0;
+  int zc = zc_env > 0 ? zc_env
+                      : (int)std::min<long long>(64, std::max<long long>(2, cdiv((long long)nzr * cols, 2048)));
   int nbz = (int)cdiv(nzr, zc);
   int nbdense = tiled ? nbx * nby * nzr : nbx * nby * nbz;
-  bool timed = h->prof.on && level == 0 && std::is_same<T, double>::value;
+  bool timed = h->prof.on && level == 0;
   if (timed) {
     while ((int)h->prof.ev.size() < 2 * (h->prof.used + 1)) {
       hipEvent_t e;
@@ -697,21 +699,39 @@ static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int 
   }
 }
 
-// residual3d (f3d_mgrid.F:3367-3598) from red-black phi/rho into a natural
-// res array of type TR.
+// residual3d (f3d_mgrid.F:3367-3598) from red-black phi/rho into the
+// red-black array res (storage TR, same geometry): the points outside the
+// loop range are zeroed (k_res_shell), the dense points computed in one
+// z-marching launch for both colours, then cond_potmgres/condbndyres
+// (k_residual_cond, even list then odd list) and the boundary conditions.
 template <typename S, typename TR>
-static void residual(wm_mg3d *h, Level &L, int level, const Params &P, const S *phi,
-                     const S *rho, TR *res, int mgform2init, const int *skip, hipStream_t s) {
+static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const S *phi,
+                        const S *rho, TR *res, int mgform2init, const int *skip, hipStream_t s) {
   typedef typename St<S>::T T;
   LoopRange r = res_range(L, h->bounds);
   double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
   T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
     dzsqic = (T)(L.dzsqi * P.mgparam * cst);
-  long long n = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
-  hipLaunchKernelGGL((k_residual<S, TR>), dim3(blocks(n)), dim3(256), 0, s, phi, rho, res, L.g,
-                     r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic, dysqic,
-                     dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, skip);
+  long long nsh = 4ll * ((long long)(L.ny + 3) * (L.nz + 3) + (long long)(L.nx + 3) * (L.nz + 3) +
+                         (long long)(L.nx + 3) * (L.ny + 3));
+  hipLaunchKernelGGL((k_res_shell<TR>), dim3(blocks(nsh)), dim3(256), 0, s, res, L.g, r.ixmin,
+                     r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, skip);
   WM_LAUNCHED();
+  int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
+  if (nyr > 0 && nzr > 0 && r.ixmax >= r.ixmin) {
+    int nby = (int)cdiv(nyr, 4);
+    int j0 = (r.ixmin + 1) / 2, j1 = (r.ixmax + 1) / 2;
+    int nbx = (int)cdiv(j1 - j0 + 1, 64);
+    long long cols = (long long)nbx * nby;
+    int zc = (int)std::min<long long>(64, std::max<long long>(2, cdiv((long long)nzr * cols, 2048)));
+    int nbz = (int)cdiv(nzr, zc);
+    int nbdense = nbx * nby * nbz;
+    hipLaunchKernelGGL((k_residual_zmarch<S, TR>), dim3(2 * nbdense), dim3(256), 0, s, phi, rho,
+                       res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
+                       dysqic, dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, j0, nbx,
+                       nby, zc, nbdense, skip);
+    WM_LAUNCHED();
+  }
   if (L.hasmask) {
     ensure_records<T>(h, L, level, P, true, mgform2init);
     CondDev<T> &C = cond_of<T>(L);
@@ -722,14 +742,13 @@ static void residual(wm_mg3d *h, Level &L, int level, const Params &P, const S *
       int nint = p == 0 ? L.nint : 0;
       long long cnt = (long long)nint + sub.n;
       if (cnt == 0) continue;
-      hipLaunchKernelGGL((k_residual_cond<S, TR>), dim3(blocks(cnt)), dim3(256), 0, s, phi,
-                         rho, res, L.g, nint, L.int_pos.p, sub, dxsqics, dysqics, dzsqics,
-                         (T)P.mgparam, skip);
+      hipLaunchKernelGGL((k_residual_cond<S, TR, T, RBIdx>), dim3(blocks(cnt)), dim3(256), 0, s,
+                         phi, rho, res, RBIdx{L.g}, L.g, nint, L.int_pos.p, sub, dxsqics,
+                         dysqics, dzsqics, (T)P.mgparam, skip);
       WM_LAUNCHED();
     }
   }
-  launch_bc<TR>(res, NatIdx{NatGeom{L.nx, L.ny, L.nz}}, L.nx, L.ny, L.nz, h->bounds, 0, 0,
-                skip, s);
+  launch_bc<TR>(res, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
 }
 
 struct RhoIdx {
@@ -739,20 +758,19 @@ struct RhoIdx {
   }
 };
 
-template <typename TR, typename TC, typename OutIdx>
-static void restrict_to(Level &L, const TR *res, TC *out, OutIdx O, const int *skip,
+template <typename TR, typename TC, typename OutIdx, typename FineIdx>
+static void restrict_to(Level &L, const TR *res, FineIdx F, TC *out, OutIdx O, const int *skip,
                         hipStream_t s, int nxc, int nyc, int nzc) {
   RestrictAxis ax{L.rlo[0].p, L.rhi[0].p, L.rw[0].p, L.rcmin[0], L.rcmax[0]};
   RestrictAxis ay{L.rlo[1].p, L.rhi[1].p, L.rw[1].p, L.rcmin[1], L.rcmax[1]};
   RestrictAxis az{L.rlo[2].p, L.rhi[2].p, L.rw[2].p, L.rcmin[2], L.rcmax[2]};
   long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
   if constexpr (std::is_same<TR, double>::value && std::is_same<TC, double>::value) {
-    hipLaunchKernelGGL((k_restrict<TR, TC, OutIdx>), dim3(blocks(n)), dim3(256), 0, s, res,
-                       NatGeom{L.nx, L.ny, L.nz}, out, O, nxc, nyc, nzc, ax, ay, az, L.ff, skip);
+    hipLaunchKernelGGL((k_restrict<TR, TC, OutIdx, FineIdx>), dim3(blocks(n)), dim3(256), 0, s,
+                       res, F, out, O, nxc, nyc, nzc, ax, ay, az, L.ff, skip);
   } else {
-    hipLaunchKernelGGL((k_restrict_t<TC, OutIdx>), dim3(blocks(n)), dim3(256), 0, s, res,
-                       NatGeom{L.nx, L.ny, L.nz}, out, O, nxc, nyc, nzc, ax, ay, az, (TC)L.ff,
-                       skip);
+    hipLaunchKernelGGL((k_restrict_t<TC, OutIdx, FineIdx>), dim3(blocks(n)), dim3(256), 0, s,
+                       res, F, out, O, nxc, nyc, nzc, ax, ay, az, (TC)L.ff, skip);
   }
   WM_LAUNCHED();
 }
@@ -771,76 +789,6 @@ static void expand_from(Level &L, T *phi, FI F, const T *pc, CI Cc, const int *s
   WM_LAUNCHED();
 }
 
-// residual3d + restrict3d in one pass over the fine level (k_resrestrict)
-// when every boundary is Dirichlet, no subgrid surface is Neumann (whose
-// residual zeroes neighbours) and every restriction window is at most 3
-// points wide and inside residual3d's loop range.  The conductor points'
-// residuals are written first into the natural residual array, from which
-// the fused kernel takes them.  Returns false when the separate kernels
-// must run.
-template <typename S>
-static bool resrestrict(wm_mg3d *h, Level &L, int level, const Params &P, Level &Lc,
-                        const int *skip, hipStream_t s) {
-  typedef typename St<S>::T T;
-  typedef typename Coarse<S>::type SC;
-  static const bool off = getenv("WARPMI_NO_RESRESTRICT") != nullptr;
-  const int *b = h->bounds;
-  for (int a = 0; a < 6; a++)
-    if (b[a] != 0) return false;
-  if (off || L.neumann) return false;
-  LoopRange r = res_range(L, b);
-  int rmin[3] = {r.ixmin, r.iymin, r.izmin}, rmax[3] = {r.ixmax, r.iymax, r.izmax};
-  if (L.rspan32[0] > 2 * 32 + 1 || L.rspan8[1] > 2 * 8 + 1) return false;
-  for (int a = 0; a < 3; a++) {
-    if (L.rwidth[a] > 3) return false;
-    if (L.rcmax[a] >= L.rcmin[a] && (L.rflo[a] < rmin[a] || L.rfhi[a] > rmax[a])) return false;
-  }
-  RestrictAxis ax{L.rlo[0].p, L.rhi[0].p, L.rw[0].p, L.rcmin[0], L.rcmax[0]};
-  RestrictAxis ay{L.rlo[1].p, L.rhi[1].p, L.rw[1].p, L.rcmin[1], L.rcmax[1]};
-  RestrictAxis az{L.rlo[2].p, L.rhi[2].p, L.rw[2].p, L.rcmin[2], L.rcmax[2]};
-  T *res = res_of<T>(L).p;
-  const S *phi = phi_of<S>(L).p;
-  const S *rho = rho_of<S>(L).p;
-  double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
-  if (L.hasmask) {
-    ensure_records<T>(h, L, level, P, true, 0);
-    CondDev<T> &C = cond_of<T>(L);
-    T dxsqics = (T)(P.mgparam * L.dxsqi * cst), dysqics = (T)(P.mgparam * L.dysqi * cst),
-      dzsqics = (T)(P.mgparam * L.dzsqi * cst);
-    for (int p = 0; p < 2; p++) {
-      SubList<T> sub = C.res[0][p].view();
-      int nint = p == 0 ? L.nint : 0;
-      long long cnt = (long long)nint + sub.n;
-      if (cnt == 0) continue;
-      hipLaunchKernelGGL((k_residual_cond<S, T>), dim3(blocks(cnt)), dim3(256), 0, s, phi, rho,
-                         res, L.g, nint, L.int_pos.p, sub, dxsqics, dysqics, dzsqics,
-                         (T)P.mgparam, skip);
-      WM_LAUNCHED();
-    }
-  }
-  long long nco = (long long)(Lc.nx + 1) * (Lc.ny + 1) * (Lc.nz + 1);
-  hipLaunchKernelGGL((k_zero_faces<SC, RBIdx>), dim3(blocks(nco)), dim3(256), 0, s,
-                     rho_of<SC>(Lc).p, RBIdx{Lc.g}, Lc.nx, Lc.ny, Lc.nz, ax, ay, az, skip);
-  WM_LAUNCHED();
-  int ncx = L.rcmax[0] - L.rcmin[0] + 1, ncy = L.rcmax[1] - L.rcmin[1] + 1,
-      ncz = L.rcmax[2] - L.rcmin[2] + 1;
-  if (ncx <= 0 || ncy <= 0 || ncz <= 0) return true;
-  int nbx = (int)cdiv(ncx, 32), nby = (int)cdiv(ncy, 8);
-  long long tiles = (long long)nbx * nby;
-  int zc = (int)std::min<long long>(32, std::max<long long>(1, cdiv((long long)ncz * tiles, 2048)));
-  int nbz = (int)cdiv(ncz, zc);
-  int nb = nbx * nby * nbz;
-  T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
-    dzsqic = (T)(L.dzsqi * P.mgparam * cst);
-  constexpr bool tablew = std::is_same<T, double>::value;
-  hipLaunchKernelGGL((k_resrestrict<S, T, SC, tablew, RBIdx>), dim3(nb), dim3(256), 0, s, phi,
-                     rho,
-                     (const T *)res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
-                     dxsqic, dysqic, dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr,
-                     rho_of<SC>(Lc).p, RBIdx{Lc.g}, ax, ay, az, (T)L.ff, nbx, nby, zc, nb, skip);
-  WM_LAUNCHED();
-  return true;
-}
 
 template <typename S>
 static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
@@ -866,12 +814,10 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   for (int i = 0; i < P.down; i++) sorpass<S>(h, L, level, P, skip, s);
   if (L.coarser) {
     Level &Lc = *h->lv[level + 1];
-    if (!resrestrict<S>(h, L, level, P, Lc, skip, s)) {
-      residual<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
-                     s);
-      restrict_to<T, SC>(L, res_of<T>(L).p, rho_of<SC>(Lc).p, RBIdx{Lc.g}, skip, s, Lc.nx,
-                         Lc.ny, Lc.nz);
-    }
+    residual_rb<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
+                      s);
+    restrict_to<T, SC>(L, (const T *)res_of<T>(L).p, RBIdx{L.g}, rho_of<SC>(Lc).p, RBIdx{Lc.g},
+                       skip, s, Lc.nx, Lc.ny, Lc.nz);
     hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
                        phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
     WM_LAUNCHED();
@@ -902,12 +848,10 @@ static int nconv_blocks(const RBGeom &g) {
 // residual becomes the source of the error equation in storage type S, the
 // V-cycle runs from a zero error, and the error is added to the FP64 phi.
 template <typename S>
-static void dc_lowp(wm_mg3d *h, const Params &Pinit, long long nr, long long np, bool fused,
-                    double mgtol, int mgmaxiters, const int *skip, hipStream_t s) {
+static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long long np,
+                    bool fused, double mgtol, int mgmaxiters, const int *skip, hipStream_t s) {
   Level &L0 = *h->lv[0];
-  hipLaunchKernelGGL((k_res_to_rho<S>), dim3(blocks(nr)), dim3(256), 0, s, L0.res64.p,
-                     rho_of<S>(L0).p, L0.g, skip);
-  WM_LAUNCHED();
+  residual_rb<double, S>(h, L0, 0, Pinit, base, h->rhosave64.p, rho_of<S>(L0).p, 1, skip, s);
   S zero;
   std::memset(&zero, 0, sizeof(S));
   hipLaunchKernelGGL((k_fill<S>), dim3(blocks(np)), dim3(256), 0, s, phi_of<S>(L0).p, np, zero,
@@ -943,13 +887,9 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
     Params Pinit = P;
     Pinit.mgform = 2;
     pin_and_mirror<double>(h, L0, 0, Pinit, base, 1, skip, s);
-    residual<double, double>(h, L0, 0, Pinit, base, h->rhosave64.p, L0.res64.p, 1, skip, s);
-    long long nr = (long long)(L0.nx + 1) * (L0.ny + 1) * (L0.nz + 1);
     long long np = 2 * L0.g.cs;
     if (prec == WM_PREC_FP64) {
-      hipLaunchKernelGGL((k_res_to_rho<double>), dim3(blocks(nr)), dim3(256), 0, s,
-                         L0.res64.p, L0.rho64.p, L0.g, skip);
-      WM_LAUNCHED();
+      residual_rb<double, double>(h, L0, 0, Pinit, base, h->rhosave64.p, L0.rho64.p, 1, skip, s);
       hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p, np,
                          0., skip);
       WM_LAUNCHED();
@@ -959,9 +899,9 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
                          L0.phi64.p, h->save64.p, np, skip);
       WM_LAUNCHED();
     } else if (prec == WM_PREC_FP32_DC) {
-      dc_lowp<float>(h, Pinit, nr, np, fused, mgtol, mgmaxiters, skip, s);
+      dc_lowp<float>(h, Pinit, base, np, fused, mgtol, mgmaxiters, skip, s);
     } else {
-      dc_lowp<bf16>(h, Pinit, nr, np, fused, mgtol, mgmaxiters, skip, s);
+      dc_lowp<bf16>(h, Pinit, base, np, fused, mgtol, mgmaxiters, skip, s);
     }
     launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, skip, s);
     if (fused) return;
@@ -1380,8 +1320,12 @@ int wm_mg3d_residual(wm_mg3d *h, int64_t level, const double *phi, const double 
     hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
     hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
     WM_LAUNCHED();
-    residual<double, double>(h, L, (int)level, P, L.phi64.p, L.rho64.p, res, mgform2init ? 1 : 0,
-                             nullptr, s);
+    L.res64.alloc(std::max<size_t>(2 * L.g.cs, (size_t)nn));
+    residual_rb<double, double>(h, L, (int)level, P, L.phi64.p, L.rho64.p, L.res64.p,
+                                mgform2init ? 1 : 0, nullptr, s);
+    hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L.res64.p, res,
+                       L.g);
+    WM_LAUNCHED();
   });
 }
 
@@ -1392,8 +1336,9 @@ int wm_mg3d_restrict(wm_mg3d *h, int64_t level, const double *res, double *rhoco
     WM_REQUIRE(level >= 0 && level + 1 < (int64_t)h->lv.size(), "wm_mg3d_restrict: no coarser level");
     Level &L = *h->lv[level];
     Level &Lc = *h->lv[level + 1];
-    restrict_to<double, double>(L, res, rhocoarse, RhoIdx{Lc.nx, Lc.ny}, nullptr,
-                                wm_stream(stream), Lc.nx, Lc.ny, Lc.nz);
+    restrict_to<double, double>(L, res, NatIdx{NatGeom{L.nx, L.ny, L.nz}}, rhocoarse,
+                                RhoIdx{Lc.nx, Lc.ny}, nullptr, wm_stream(stream), Lc.nx, Lc.ny,
+                                Lc.nz);
   });
 }
 

@@ -343,43 +343,119 @@ __global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
   }
 }
 
-// ---------------------------------------------------------------------------
-// Residual (residual3d, f3d_mgrid.F:3367-3598): one thread per element of
-// the natural res array including its guard cells.  Points outside the loop
-// range get 0 (the reference zeroes Dirichlet faces and guards; Neumann and
-// periodic guards are filled afterwards by the boundary kernel).  Points in
-// the conductor mask are written by k_residual_cond.
-// ---------------------------------------------------------------------------
+
+// residual3d's dense part in z-marching form (the structure of
+// k_sor_zmarch): blocks [0, nbdense) do colour 0, [nbdense, 2 nbdense)
+// colour 1; each unmasked point of the loop range gets
+// rho + sum(neighbours * d2c) - phi * mgparam (f3d_mgrid.F:3542-3546) in
+// the red-black array res (same geometry as phi).  Masked points are left
+// to k_residual_cond, points outside the range to k_res_shell.
 template <typename S, typename TR, typename T = typename St<S>::T>
-__global__ __launch_bounds__(256) void k_residual(
-    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res,
-    RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax,
-    T dxsqic, T dysqic, T dzsqic, T mgparam, const uint32_t *__restrict__ mask,
-    const int *__restrict__ skip) {
+__global__ __launch_bounds__(256) void k_residual_zmarch(
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RBGeom g,
+    int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
+    T dzsqic, T mgparam, const uint32_t *__restrict__ mask, int j0, int nbx, int nby, int zc,
+    int nbdense, const int *__restrict__ skip) {
   WM_SKIP();
-  int sx = g.nx + 3, sy = g.ny + 3;
-  long long n = (long long)sx * sy * (g.nz + 3);
+  int c = blockIdx.x >= nbdense;
+  int lb = xcd_remap(blockIdx.x - c * nbdense, nbdense);
+  int bx = lb % nbx;
+  int t = lb / nbx;
+  int by = t % nby;
+  int bz = t / nby;
+  int lane = threadIdx.x & 63;
+  int j = j0 + bx * 64 + lane;
+  int iy = iymin + by * 4 + (threadIdx.x >> 6);
+  if (iy > iymax) return;  // whole wave
+  int za = izmin + bz * zc;
+  int zb = min(za + zc - 1, izmax);
+  int py = iy + 1;
+  long long pl = (long long)g.W * g.PY;
+  const S *o = phi + (1 - c) * g.cs;
+  const S *me = phi + c * g.cs;
+  const S *rh = rho + c * g.cs;
+  TR *out = res + c * g.cs;
+  int jc = min(j, g.W - 1);
+  long long rowc = (long long)g.W * py + jc;
+  long long meoff = (long long)c * g.cs + rowc;
+  auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw) {
+    long long base = pl * pz;
+    ozp = St<S>::ld(o[rowc + base + pl]);
+    oym = St<S>::ld(o[rowc + base - g.W]);
+    oyp = St<S>::ld(o[rowc + base + g.W]);
+    own = St<S>::ld(me[rowc + base]);
+    r = St<S>::ld(rh[rowc + base]);
+    int p0 = (c + py + pz + 1) & 1;
+    int jn = p0 ? j + 1 : j - 1;
+    bool e = p0 ? (lane == 63) : (lane == 0);
+    edge = e ? St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]) : (T)0.;
+    mw = mask ? mask[(meoff + base) >> 5] : 0u;
+  };
+  T oz_m = St<S>::ld(o[rowc + pl * za]);
+  T oz_0 = St<S>::ld(o[rowc + pl * (za + 1)]);
+  T ozp, oym, oyp, own, r, eg;
+  uint32_t mw;
+  load(za + 1, ozp, oym, oyp, own, r, eg, mw);
+  for (int iz = za; iz <= zb; iz++) {
+    int pz = iz + 1;
+    T nozp = 0, noym = 0, noyp = 0, nown = 0, nr = 0, neg = 0;
+    uint32_t nmw = 0;
+    if (iz < zb) load(pz + 1, nozp, noym, noyp, nown, nr, neg, nmw);
+    int p0 = (c + py + pz + 1) & 1;
+    T sh = p0 ? __shfl_down(oz_0, 1, 64) : __shfl_up(oz_0, 1, 64);
+    if (p0 ? (lane == 63) : (lane == 0)) sh = eg;
+    int ix = p0 + 2 * j - 1;
+    long long ce = meoff + pl * pz;
+    bool active = j < g.W && ix >= ixmin && ix <= ixmax && !((mw >> (ce & 31)) & 1u);
+    if (active) {
+      T pxm = p0 ? oz_0 : sh;
+      T pxp = p0 ? sh : oz_0;
+      T v = r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic + (oz_m + ozp) * dzsqic -
+            own * mgparam;
+      out[rowc + pl * pz] = St<TR>::st((typename St<TR>::T)v);
+    }
+    oz_m = oz_0;
+    oz_0 = ozp;
+    ozp = nozp; oym = noym; oyp = noyp; own = nown; r = nr; eg = neg; mw = nmw;
+  }
+}
+
+// Zero the points of a red-black residual array outside residual3d's loop
+// range (guard cells and Dirichlet faces).  Threads cover the three pairs
+// of two-point-thick shells; interior points are not touched.
+template <typename TR>
+__global__ __launch_bounds__(256) void k_res_shell(TR *__restrict__ res, RBGeom g, int ixmin,
+                                                   int ixmax, int iymin, int iymax, int izmin,
+                                                   int izmax, const int *__restrict__ skip) {
+  WM_SKIP();
+  int nx3 = g.nx + 3, ny3 = g.ny + 3, nz3 = g.nz + 3;
+  long long nxs = 4ll * ny3 * nz3, nys = 4ll * nx3 * nz3, nzs = 4ll * nx3 * ny3;
   long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= n) return;
-  int ix = (int)(t % sx) - 1;
-  long long u = t / sx;
-  int iy = (int)(u % sy) - 1;
-  int iz = (int)(u / sy) - 1;
-  if (ix < ixmin || ix > ixmax || iy < iymin || iy > iymax || iz < izmin ||
-      iz > izmax) {
-    res[t] = (TR)0.;
+  int ix, iy, iz;
+  auto pick = [](int k, int n) { return k < 2 ? k - 1 : n - 2 + k; };  // -1, 0, n, n+1
+  if (t < nxs) {
+    int k = (int)(t % 4);
+    long long u = t / 4;
+    ix = pick(k, g.nx);
+    iy = (int)(u % ny3) - 1;
+    iz = (int)(u / ny3) - 1;
+  } else if ((t -= nxs) < nys) {
+    int k = (int)(t % 4);
+    long long u = t / 4;
+    iy = pick(k, g.ny);
+    ix = (int)(u % nx3) - 1;
+    iz = (int)(u / nx3) - 1;
+  } else if ((t -= nys) < nzs) {
+    int k = (int)(t % 4);
+    long long u = t / 4;
+    iz = pick(k, g.nz);
+    ix = (int)(u % nx3) - 1;
+    iy = (int)(u / nx3) - 1;
+  } else {
     return;
   }
-  long long ce = rb_index(g, ix, iy, iz);
-  if (mask && ((mask[ce >> 5] >> (ce & 31)) & 1u)) return;
-#define LD(a) St<S>::ld(a)
-  T r = LD(rho[ce]) +
-        (LD(phi[rb_index(g, ix - 1, iy, iz)]) + LD(phi[rb_index(g, ix + 1, iy, iz)])) * dxsqic +
-        (LD(phi[rb_index(g, ix, iy - 1, iz)]) + LD(phi[rb_index(g, ix, iy + 1, iz)])) * dysqic +
-        (LD(phi[rb_index(g, ix, iy, iz - 1)]) + LD(phi[rb_index(g, ix, iy, iz + 1)])) * dzsqic -
-        LD(phi[ce]) * mgparam;
-#undef LD
-  res[t] = (TR)r;
+  if (ix < ixmin || ix > ixmax || iy < iymin || iy > iymax || iz < izmin || iz > izmax)
+    res[rb_index(g, ix, iy, iz)] = St<TR>::st((typename St<TR>::T)0.);
 }
 
 // cond_potmgres + condbndyres (f3d_mgrid.F:2474-2506, 3094-3365): interior
@@ -387,16 +463,15 @@ __global__ __launch_bounds__(256) void k_residual(
 // surfaces the reference zeroes some neighbours' residuals while walking
 // the even list then the odd list; launching one parity per call (even
 // first) reproduces that order.
-template <typename S, typename TR, typename T = typename St<S>::T>
+template <typename S, typename TR, typename T = typename St<S>::T, typename RIdx = NatIdx>
 __global__ __launch_bounds__(256) void k_residual_cond(
-    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res,
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RIdx RI,
     RBGeom g, int nint, const int *__restrict__ ipos, SubList<T> sub,
     T dxsqics, T dysqics, T dzsqics, T mgparam, const int *__restrict__ skip) {
   WM_SKIP();
-  NatGeom ng{g.nx, g.ny, g.nz};
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i < nint) {
-    res[nat_index(ng, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2])] = (TR)0.;
+    res[RI(ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2])] = St<TR>::st((typename St<TR>::T)0.);
     return;
   }
   i -= nint;
@@ -413,15 +488,15 @@ __global__ __launch_bounds__(256) void k_residual_cond(
   long long ce = rb_index(g, ix, iy, iz);
   T r = sub.fac[i] * (St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqics + (pym + pyp) * dysqics +
                       (pzm + pzp) * dzsqics - St<S>::ld(phi[ce]) * mgparam * sub.denom[i]);
-  res[nat_index(ng, ix, iy, iz)] = (TR)r;
+  res[RI(ix, iy, iz)] = St<TR>::st((typename St<TR>::T)r);
   uint32_t z = sub.zero[i];
   if (z) {
-    if (z & 1u) res[nat_index(ng, ix - 1, iy, iz)] = (TR)0.;
-    if (z & 2u) res[nat_index(ng, ix + 1, iy, iz)] = (TR)0.;
-    if (z & 4u) res[nat_index(ng, ix, iy - 1, iz)] = (TR)0.;
-    if (z & 8u) res[nat_index(ng, ix, iy + 1, iz)] = (TR)0.;
-    if (z & 16u) res[nat_index(ng, ix, iy, iz - 1)] = (TR)0.;
-    if (z & 32u) res[nat_index(ng, ix, iy, iz + 1)] = (TR)0.;
+    if (z & 1u) res[RI(ix - 1, iy, iz)] = St<TR>::st((typename St<TR>::T)0.);
+    if (z & 2u) res[RI(ix + 1, iy, iz)] = St<TR>::st((typename St<TR>::T)0.);
+    if (z & 4u) res[RI(ix, iy - 1, iz)] = St<TR>::st((typename St<TR>::T)0.);
+    if (z & 8u) res[RI(ix, iy + 1, iz)] = St<TR>::st((typename St<TR>::T)0.);
+    if (z & 16u) res[RI(ix, iy, iz - 1)] = St<TR>::st((typename St<TR>::T)0.);
+    if (z & 32u) res[RI(ix, iy, iz + 1)] = St<TR>::st((typename St<TR>::T)0.);
   }
 }
 
@@ -490,19 +565,19 @@ struct RestrictAxis {
   int cmin, cmax;      // computed coarse range
 };
 
-template <typename TR, typename TC, typename OutIdx>
+template <typename TR, typename TC, typename OutIdx, typename FineIdx = NatIdx>
 __global__ __launch_bounds__(256) void k_restrict(
-    const TR *__restrict__ res, NatGeom fine, TC *__restrict__ out, OutIdx O,
+    const TR *__restrict__ res, FineIdx F, TC *__restrict__ out, OutIdx O,
     int nxc, int nyc, int nzc, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
     double ff, const int *__restrict__ skip) {
   WM_SKIP();
-  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
-  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  int n = (nxc + 1) * (nyc + 1) * (nzc + 1);
+  int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  int ixc = (int)(t % (nxc + 1));
-  long long u = t / (nxc + 1);
-  int iyc = (int)(u % (nyc + 1));
-  int izc = (int)(u / (nyc + 1));
+  int ixc = t % (nxc + 1);
+  int u = t / (nxc + 1);
+  int iyc = u % (nyc + 1);
+  int izc = u / (nyc + 1);
   long long oi = O(ixc, iyc, izc);
   if (ixc < ax.cmin || ixc > ax.cmax || iyc < ay.cmin || iyc > ay.cmax ||
       izc < az.cmin || izc > az.cmax) {
@@ -513,13 +588,26 @@ __global__ __launch_bounds__(256) void k_restrict(
   int x0 = ax.lo[ixc], x1 = ax.hi[ixc], y0 = ay.lo[iyc], y1 = ay.hi[iyc];
   int z0 = az.lo[izc], z1 = az.hi[izc];
   double r = 0., w = 0.;
-  for (int iz = z0; iz <= z1; iz++)
-    for (int iy = y0; iy <= y1; iy++)
-      for (int ix = x0; ix <= x1; ix++) {
-        double ww = wx[ix - x0] * wy[iy - y0] * wz[iz - z0];
-        r = r + ww * (double)res[nat_index(fine, ix, iy, iz)];
-        w = w + ww;
-      }
+  if (x1 - x0 == 2 && y1 - y0 == 2 && z1 - z0 == 2) {
+    // 2:1 in every axis: all 27 loads issued before the (same-order) sums
+    double v[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) v[q] = (double)res[F(x0 + q % 3, y0 + (q / 3) % 3, z0 + q / 9)];
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+      double ww = wx[q % 3] * wy[(q / 3) % 3] * wz[q / 9];
+      r = r + ww * v[q];
+      w = w + ww;
+    }
+  } else {
+    for (int iz = z0; iz <= z1; iz++)
+      for (int iy = y0; iy <= y1; iy++)
+        for (int ix = x0; ix <= x1; ix++) {
+          double ww = wx[ix - x0] * wy[iy - y0] * wz[iz - z0];
+          r = r + ww * (double)res[F(ix, iy, iz)];
+          w = w + ww;
+        }
+  }
   if (w > 0.) {
     r = r / w;
     out[oi] = (TC)(r * ff);
@@ -529,19 +617,19 @@ __global__ __launch_bounds__(256) void k_restrict(
 }
 
 // Single-precision variant: the tent weights and sums are taken in T.
-template <typename T, typename OutIdx>
+template <typename T, typename OutIdx, typename FineIdx = NatIdx>
 __global__ __launch_bounds__(256) void k_restrict_t(
-    const T *__restrict__ res, NatGeom fine, T *__restrict__ out, OutIdx O,
+    const T *__restrict__ res, FineIdx F, T *__restrict__ out, OutIdx O,
     int nxc, int nyc, int nzc, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
     T ff, const int *__restrict__ skip) {
   WM_SKIP();
-  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
-  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  int n = (nxc + 1) * (nyc + 1) * (nzc + 1);
+  int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  int ixc = (int)(t % (nxc + 1));
-  long long u = t / (nxc + 1);
-  int iyc = (int)(u % (nyc + 1));
-  int izc = (int)(u / (nyc + 1));
+  int ixc = t % (nxc + 1);
+  int u = t / (nxc + 1);
+  int iyc = u % (nyc + 1);
+  int izc = u / (nyc + 1);
   long long oi = O(ixc, iyc, izc);
   if (ixc < ax.cmin || ixc > ax.cmax || iyc < ay.cmin || iyc > ay.cmax ||
       izc < az.cmin || izc > az.cmax) {
@@ -552,162 +640,26 @@ __global__ __launch_bounds__(256) void k_restrict_t(
   int x0 = ax.lo[ixc], x1 = ax.hi[ixc], y0 = ay.lo[iyc], y1 = ay.hi[iyc];
   int z0 = az.lo[izc], z1 = az.hi[izc];
   T r = 0., w = 0.;
-  for (int iz = z0; iz <= z1; iz++)
-    for (int iy = y0; iy <= y1; iy++)
-      for (int ix = x0; ix <= x1; ix++) {
-        T ww = (T)wx[ix - x0] * (T)wy[iy - y0] * (T)wz[iz - z0];
-        r = r + ww * res[nat_index(fine, ix, iy, iz)];
-        w = w + ww;
-      }
-  out[oi] = w > (T)0. ? (r / w) * ff : (T)0.;
-}
-
-// ---------------------------------------------------------------------------
-// residual3d fused with restrict3d, for all-Dirichlet boundaries and windows
-// at most 3 fine points wide (2:1 or no coarsening per axis).  A block owns
-// RT_X x RT_Y coarse nodes of the computed range and walks a chunk of coarse
-// planes; the fine residuals of the window planes are computed into an LDS
-// ring of 4 fine planes, each fine plane once per block, and every coarse
-// node then sums its window from LDS in restrict3d's order
-// (z, y, x; f3d_mgrid.F:1707-1737).  Fine points outside residual3d's loop
-// range or in the conductor mask take the value already in the natural
-// residual array `res`, where the face-zeroing kernel and
-// cond_potmgres/condbndyres (k_residual_cond) have put it, exactly as the
-// separate kernels leave it.  Coarse nodes outside the computed range are
-// written elsewhere (k_zero_faces).
-// ---------------------------------------------------------------------------
-#define RT_X 32
-#define RT_Y 8
-#define RT_FX (2 * RT_X + 1)
-#define RT_FY (2 * RT_Y + 1)
-
-template <typename T, bool TABLEW>
-__device__ inline void restrict_sum(const T *lds, int fxl, int fyl, const int *zslot,
-                                    int x0, int x1, int y0, int y1, int z0, int z1,
-                                    const double *wx, const double *wy, const double *wz,
-                                    T ff, T &out) {
-  if (TABLEW) {
-    double r = 0., w = 0.;
-    for (int iz = z0; iz <= z1; iz++) {
-      const T *pl = lds + zslot[iz - z0] * (RT_FX * RT_FY);
-      for (int iy = y0; iy <= y1; iy++)
-        for (int ix = x0; ix <= x1; ix++) {
-          double ww = wx[ix - x0] * wy[iy - y0] * wz[iz - z0];
-          r = r + ww * (double)pl[(iy - fyl) * RT_FX + (ix - fxl)];
-          w = w + ww;
-        }
-    }
-    if (w > 0.) {
-      r = r / w;
-      out = (T)(r * (double)ff);
-    } else {
-      out = (T)0.;
+  if (x1 - x0 == 2 && y1 - y0 == 2 && z1 - z0 == 2) {
+    T v[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) v[q] = res[F(x0 + q % 3, y0 + (q / 3) % 3, z0 + q / 9)];
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+      T ww = (T)wx[q % 3] * (T)wy[(q / 3) % 3] * (T)wz[q / 9];
+      r = r + ww * v[q];
+      w = w + ww;
     }
   } else {
-    T r = 0., w = 0.;
-    for (int iz = z0; iz <= z1; iz++) {
-      const T *pl = lds + zslot[iz - z0] * (RT_FX * RT_FY);
+    for (int iz = z0; iz <= z1; iz++)
       for (int iy = y0; iy <= y1; iy++)
         for (int ix = x0; ix <= x1; ix++) {
           T ww = (T)wx[ix - x0] * (T)wy[iy - y0] * (T)wz[iz - z0];
-          r = r + ww * pl[(iy - fyl) * RT_FX + (ix - fxl)];
+          r = r + ww * res[F(ix, iy, iz)];
           w = w + ww;
         }
-    }
-    out = w > (T)0. ? (r / w) * ff : (T)0.;
   }
-}
-
-template <typename S, typename TR, typename TC, bool TABLEW, typename OutIdx>
-__global__ __launch_bounds__(256) void k_resrestrict(
-    const S *__restrict__ phi, const S *__restrict__ rho, const TR *__restrict__ res,
-    RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax,
-    typename St<S>::T dxsqic, typename St<S>::T dysqic, typename St<S>::T dzsqic,
-    typename St<S>::T mgparam, const uint32_t *__restrict__ mask, TC *__restrict__ out,
-    OutIdx O, RestrictAxis ax, RestrictAxis ay, RestrictAxis az, typename St<S>::T ff,
-    int nbx, int nby, int zc, int nblocks, const int *__restrict__ skip) {
-  typedef typename St<S>::T T;
-  WM_SKIP();
-  __shared__ T lds[4 * RT_FX * RT_FY];
-  int lb = xcd_remap(blockIdx.x, nblocks);
-  int bx = lb % nbx;
-  int t = lb / nbx;
-  int by = t % nby;
-  int bz = t / nby;
-  // coarse tile inside the computed range
-  int cx0 = ax.cmin + bx * RT_X, cx1 = min(cx0 + RT_X - 1, ax.cmax);
-  int cy0 = ay.cmin + by * RT_Y, cy1 = min(cy0 + RT_Y - 1, ay.cmax);
-  int cz0 = az.cmin + bz * zc, cz1 = min(cz0 + zc - 1, az.cmax);
-  int fxl = ax.lo[cx0], fxh = ax.hi[cx1];
-  int fyl = ay.lo[cy0], fyh = ay.hi[cy1];
-  int fw = fxh - fxl + 1, fh = fyh - fyl + 1;
-  int tx = threadIdx.x & (RT_X - 1), ty = threadIdx.x / RT_X;
-  int cx = cx0 + tx, cy = cy0 + ty;
-  bool mine = cx <= cx1 && cy <= cy1;
-  int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
-  const double *wx = ax.w, *wy = ay.w;
-  if (mine) {
-    x0 = ax.lo[cx]; x1 = ax.hi[cx]; wx = ax.w + 5 * cx;
-    y0 = ay.lo[cy]; y1 = ay.hi[cy]; wy = ay.w + 5 * cy;
-  }
-  int sx = g.nx + 3, sxy = (g.nx + 3) * (g.ny + 3);
-  int done = az.lo[cz0] - 1;  // last fine plane in the ring
-  for (int cz = cz0; cz <= cz1; cz++) {
-    int z0 = az.lo[cz], z1 = az.hi[cz];
-    for (int fz = max(done + 1, z0); fz <= z1; fz++) {
-      T *pl = lds + (fz & 3) * (RT_FX * RT_FY);
-      bool zin = fz >= izmin && fz <= izmax;
-      for (int e = threadIdx.x; e < fw * fh; e += 256) {
-        int fx = fxl + e % fw, fy = fyl + e / fw;
-        T v;
-        long long ce = rb_index(g, fx, fy, fz);
-        bool dense = zin && fx >= ixmin && fx <= ixmax && fy >= iymin && fy <= iymax &&
-                     !(mask && ((mask[ce >> 5] >> (ce & 31)) & 1u));
-        if (dense) {
-#define LD(a) St<S>::ld(a)
-          v = LD(rho[ce]) +
-              (LD(phi[rb_index(g, fx - 1, fy, fz)]) + LD(phi[rb_index(g, fx + 1, fy, fz)])) * dxsqic +
-              (LD(phi[rb_index(g, fx, fy - 1, fz)]) + LD(phi[rb_index(g, fx, fy + 1, fz)])) * dysqic +
-              (LD(phi[rb_index(g, fx, fy, fz - 1)]) + LD(phi[rb_index(g, fx, fy, fz + 1)])) * dzsqic -
-              LD(phi[ce]) * mgparam;
-#undef LD
-        } else {
-          v = (T)res[(fx + 1) + (long long)sx * (fy + 1) + (long long)sxy * (fz + 1)];
-        }
-        pl[(fy - fyl) * RT_FX + (fx - fxl)] = v;
-      }
-      done = fz;
-    }
-    __syncthreads();
-    if (mine) {
-      int zslot[3] = {z0 & 3, (z0 + 1) & 3, (z0 + 2) & 3};
-      T o;
-      restrict_sum<T, TABLEW>(lds, fxl, fyl, zslot, x0, x1, y0, y1, z0, z1, wx, wy,
-                              az.w + 5 * cz, ff, o);
-      out[O(cx, cy, cz)] = St<TC>::st((typename St<TC>::T)o);
-    }
-    __syncthreads();
-  }
-}
-
-// Zero the coarse nodes restrict3d leaves outside its computed range
-// (Dirichlet faces, f3d_mgrid.F:1704-1751).
-template <typename TC, typename OutIdx>
-__global__ __launch_bounds__(256) void k_zero_faces(TC *__restrict__ out, OutIdx O, int nxc,
-                                                    int nyc, int nzc, RestrictAxis ax,
-                                                    RestrictAxis ay, RestrictAxis az,
-                                                    const int *__restrict__ skip) {
-  WM_SKIP();
-  long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
-  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= n) return;
-  int ixc = (int)(t % (nxc + 1));
-  long long u = t / (nxc + 1);
-  int iyc = (int)(u % (nyc + 1));
-  int izc = (int)(u / (nyc + 1));
-  if (ixc < ax.cmin || ixc > ax.cmax || iyc < ay.cmin || iyc > ay.cmax || izc < az.cmin ||
-      izc > az.cmax)
-    out[O(ixc, iyc, izc)] = (TC)0.;
+  out[oi] = w > (T)0. ? (r / w) * ff : (T)0.;
 }
 
 // ---------------------------------------------------------------------------
@@ -773,6 +725,11 @@ __global__ __launch_bounds__(256) void k_expand_rb(
     T wy = (T)ay.w[iy], wz = (T)az.w[iz];
     T one = (T)1.;
     long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
+    // coarse row offsets of (jy+dy, jz+dz); colour of (jx, jy+dy, jz+dz) is
+    // (jx + jy + dy + jz + dz) & 1
+    long long r00 = (long long)gc.W * ((jy + 1) + (long long)gc.PY * (jz + 1));
+    long long r10 = r00 + gc.W, r01 = r00 + (long long)gc.W * gc.PY, r11 = r01 + gc.W;
+    int syz = jy + jz;
     // j range of this row: ix = p0 + 2j - 1 in [ax.lo, ax.hi]
     int jlo = (ax.lo + 2 - p0) / 2, jhi = (ax.hi + 1 - p0) / 2;
     for (int j = jlo + lane; j <= jhi; j += 64) {
@@ -780,17 +737,21 @@ __global__ __launch_bounds__(256) void k_expand_rb(
       int jx = ax.j[ix];
       T wx = (T)ax.w[ix];
       long long fi = base + j;
-#define PC(a, b, c) ((T)St<SC>::ld(pc[rb_index(gc, a, b, c)]))
-      phi[fi] = St<S>::st(St<S>::ld(phi[fi]) +
-                (one - wx) * (one - wy) * (one - wz) * PC(jx, jy, jz) +
-                wx * (one - wy) * (one - wz) * PC(jx + 1, jy, jz) +
-                (one - wx) * wy * (one - wz) * PC(jx, jy + 1, jz) +
-                wx * wy * (one - wz) * PC(jx + 1, jy + 1, jz) +
-                (one - wx) * (one - wy) * wz * PC(jx, jy, jz + 1) +
-                wx * (one - wy) * wz * PC(jx + 1, jy, jz + 1) +
-                (one - wx) * wy * wz * PC(jx, jy + 1, jz + 1) +
-                wx * wy * wz * PC(jx + 1, jy + 1, jz + 1));
+      long long e0 = (jx + 1) >> 1, e1 = (jx + 2) >> 1;
+      long long ca = (long long)((jx + syz) & 1) * gc.cs, cb = gc.cs - ca;  // parity 0 / 1
+#define PC(x, off, flip) ((T)St<SC>::ld(pc[((flip) ? cb : ca) + (x) + (off)]))
+      T c000 = PC(e0, r00, 0), c100 = PC(e1, r00, 1), c010 = PC(e0, r10, 1), c110 = PC(e1, r10, 0);
+      T c001 = PC(e0, r01, 1), c101 = PC(e1, r01, 0), c011 = PC(e0, r11, 0), c111 = PC(e1, r11, 1);
 #undef PC
+      phi[fi] = St<S>::st(St<S>::ld(phi[fi]) +
+                          (one - wx) * (one - wy) * (one - wz) * c000 +
+                          wx * (one - wy) * (one - wz) * c100 +
+                          (one - wx) * wy * (one - wz) * c010 +
+                          wx * wy * (one - wz) * c110 +
+                          (one - wx) * (one - wy) * wz * c001 +
+                          wx * (one - wy) * wz * c101 +
+                          (one - wx) * wy * wz * c011 +
+                          wx * wy * wz * c111);
     }
   }
 }
@@ -998,24 +959,6 @@ __global__ __launch_bounds__(256) void k_dc_add(double *phi, const TL *e,
   if (t < n) phi[t] = (double)St<TL>::ld(e[t]) + save[t];
 }
 
-// Copy of level-0 residual (natural) into the low-precision rho (red-black),
-// rho(0:n) = res(0:n) of the mgform=2 loop (f3d_mgrid.F:341).
-template <typename TL>
-__global__ __launch_bounds__(256) void k_res_to_rho(const double *__restrict__ res,
-                                                    TL *__restrict__ rho, RBGeom g,
-                                                    const int *__restrict__ skip) {
-  WM_SKIP();
-  long long sx = g.nx + 1, sy = g.ny + 1;
-  long long n = sx * sy * (g.nz + 1);
-  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= n) return;
-  int ix = (int)(t % sx);
-  long long u = t / sx;
-  int iy = (int)(u % sy);
-  int iz = (int)(u / sy);
-  NatGeom ng{g.nx, g.ny, g.nz};
-  rho[rb_index(g, ix, iy, iz)] = St<TL>::st((typename St<TL>::T)res[nat_index(ng, ix, iy, iz)]);
-}
 
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void k_copy(const TA *__restrict__ a,

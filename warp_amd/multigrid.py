@@ -252,6 +252,12 @@ class MultiGrid3D(FieldSolver):
         self.finalizesourcep()
         self.dosolve(iwhich)
 
+    def find_mgparam(self, lsavephi=False, resetpasses=1):
+        """Optimise mgparam and the pass counts (multigrid.py:827-831,
+        scripts/find_mgparam.py)."""
+        from .find_mgparam import find_mgparam
+        find_mgparam(self, lsavephi=lsavephi, resetpasses=resetpasses)
+
     def getmgverbose(self):
         if self.mgntverbose < 0:
             return 0
