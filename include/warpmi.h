@@ -234,6 +234,65 @@ int wm_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
                   int64_t lzero, void *stream);
 
 /* ---------------------------------------------------------------------
+ * 3-D U-Net warm start (no reference counterpart: the learned initial
+ * guess of the multigrid solve).  Activations are channels-last bf16
+ * (uint16 bit patterns), channels multiples of 16; weights are fp32
+ * masters w[co][ci][27] packed by wm_unet_pack into the forward layout
+ * wp and the mirrored/transposed layout wt of the backward-over-input
+ * convolution.
+ * ------------------------------------------------------------------- */
+
+/* 3x3x3 convolution, zero padding 1, bf16 MFMA with fp32 accumulation:
+ * y = leaky_relu(conv(x) + bias, slope) (slope 1: identity; bias may be
+ * NULL).  Output to y (bf16) and/or yf (fp32), either may be NULL.
+ * D*H*W must be a multiple of 16, cin and cout multiples of 16,
+ * cout <= 128. */
+int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf,
+                  int64_t cout, int64_t D, int64_t H, int64_t W,
+                  const uint16_t *wp, const float *bias, double slope,
+                  void *stream);
+/* dw[co][ci][tap] += sum_v x[v+tap][ci] g[v][co]; db[co] += sum_v g[v][co]
+ * (db may be NULL). */
+int wm_unet_conv3_wgrad(const uint16_t *x, int64_t cin, const uint16_t *g,
+                        int64_t cout, int64_t D, int64_t H, int64_t W,
+                        float *dw, float *db, void *stream);
+/* g *= leaky_relu'(y) elementwise (n values). */
+int wm_unet_act_bwd(uint16_t *g, const uint16_t *y, double slope, int64_t n,
+                    void *stream);
+int wm_unet_pack(const float *w, int64_t cin, int64_t cout, uint16_t *wp,
+                 uint16_t *wt, void *stream);
+/* 2x2x2 average pooling of x (D,H,W,c) into y; backward: x is the pooled
+ * gradient, y the full-size gradient. */
+int wm_unet_pool(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t c,
+                 uint16_t *y, int64_t backward, void *stream);
+/* Nearest x2 upsampling of x (D,H,W,cx) into channels [0,cx) of y (cy
+ * channels); backward sums the eight children into x-sized y. */
+int wm_unet_up(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx,
+               uint16_t *y, int64_t cy, int64_t backward, void *stream);
+int wm_unet_chcopy(const uint16_t *src, int64_t sc, int64_t soff,
+                   uint16_t *dst, int64_t dc, int64_t doff, int64_t c,
+                   int64_t nvox, void *stream);
+/* a += b, n bf16 values. */
+int wm_unet_add(uint16_t *a, const uint16_t *b, int64_t n, void *stream);
+/* Adam step (bias-corrected) on n fp32 weights. */
+int wm_unet_adam(float *w, const float *g, float *m1, float *m2, int64_t n,
+                 double lr, double b1, double b2, double eps, int64_t step,
+                 void *stream);
+/* Network input (rho*srho, mask, phi*sphi, zeros) over nodes 0..n-1 of
+ * FP64 natural rho (0:n)^3 and phi (-1:n+1)^3; mask is one byte per node
+ * (may be NULL). */
+int wm_unet_input(const double *rho, const double *phi, const uint8_t *mask,
+                  int64_t nx, int64_t ny, int64_t nz, double srho, double sphi,
+                  uint16_t *x, void *stream);
+/* phi(0..n-1) += out[.][0]/sphi where mask is 0. */
+int wm_unet_apply(const float *out, const uint8_t *mask, int64_t nx, int64_t ny,
+                  int64_t nz, double sphi, double *phi, void *stream);
+/* Mean-square loss of channel 0 of out (16-channel fp32) against tgt:
+ * *loss += mean, g (16-channel bf16) = its gradient. */
+int wm_unet_mse(const float *out, const float *tgt, int64_t n, uint16_t *g,
+                float *loss, void *stream);
+
+/* ---------------------------------------------------------------------
  * Conductor set-up on the host (pywarp90/f3d_conductors.F)
  * ------------------------------------------------------------------- */
 

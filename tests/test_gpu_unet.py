@@ -1,0 +1,216 @@
+"""The U-Net warm start's kernels against torch's CPU convolution and
+autograd in FP64 (on the same bf16-rounded operands), and the trained
+network's effect on the multigrid solve."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(a):
+    """Round an FP32/FP64 tensor to bf16 and back (CPU, FP64)."""
+    return a.to(torch.bfloat16).to(torch.float64)
+
+
+def _to_dev_bits(a_cl, device):
+    """channels-last bf16 bits (int16) on the device from a CPU tensor."""
+    return a_cl.contiguous().to(torch.bfloat16).view(torch.int16).reshape(-1).to(device)
+
+
+def _from_dev_bits(t, shape):
+    return t.cpu().view(torch.bfloat16).to(torch.float64).reshape(shape)
+
+
+def _ref_conv(x, w, b, slope):
+    """x (C,D,H,W), w (co, ci, 27) -> leaky_relu(conv3d(x, w, pad 1) + b)."""
+    co, ci = w.shape[:2]
+    y = F.conv3d(x[None], w.reshape(co, ci, 3, 3, 3), b, padding=1)[0]
+    return torch.where(y > 0, y, y * slope)
+
+
+@pytest.mark.parametrize("cin,cout,dims", [(16, 16, (4, 6, 8)), (32, 48, (4, 4, 4)),
+                                           (112, 48, (2, 4, 8))])
+def test_conv3_forward(device, cin, cout, dims):
+    from warp_amd import _native as N
+    D, H, W = dims
+    g = torch.Generator().manual_seed(cin * 7 + cout)
+    x = _bf16(torch.randn(cin, D, H, W, generator=g, dtype=torch.float64))
+    w = _bf16(torch.randn(cout, cin, 27, generator=g, dtype=torch.float64) * 0.1)
+    b = torch.randn(cout, generator=g, dtype=torch.float64).to(torch.float32)
+    wd = w.to(torch.float32).to(device)
+    wp = torch.empty(27 * cin * cout, dtype=torch.int16, device=device)
+    wt = torch.empty_like(wp)
+    s = N.stream_of(device)
+    N.call("wm_unet_pack", wd.data_ptr(), cin, cout, wp.data_ptr(), wt.data_ptr(), s)
+    xd = _to_dev_bits(x.permute(1, 2, 3, 0), device)
+    yf = torch.empty(D * H * W * cout, dtype=torch.float32, device=device)
+    bd = b.to(device)
+    N.call("wm_unet_conv3", xd.data_ptr(), cin, None, yf.data_ptr(), cout, D, H, W, wp.data_ptr(),
+           bd.data_ptr(), 0.1, s)
+    ref = _ref_conv(x, w, b.to(torch.float64), 0.1).permute(1, 2, 3, 0)
+    got = yf.cpu().to(torch.float64).reshape(D, H, W, cout)
+    assert (got - ref).abs().max() <= 2e-5 * ref.abs().max() + 1e-6
+
+
+def test_conv3_backward(device):
+    """dX (the mirrored convolution), dW and db against torch autograd."""
+    from warp_amd import _native as N
+    cin, cout, (D, H, W) = 16, 32, (4, 4, 8)
+    g = torch.Generator().manual_seed(3)
+    x = _bf16(torch.randn(cin, D, H, W, generator=g, dtype=torch.float64)).requires_grad_()
+    w = _bf16(torch.randn(cout, cin, 27, generator=g, dtype=torch.float64) * 0.1).requires_grad_()
+    b = torch.zeros(cout, dtype=torch.float64, requires_grad=True)
+    gy = _bf16(torch.randn(cout, D, H, W, generator=g, dtype=torch.float64))
+    y = F.conv3d(x[None], w.reshape(cout, cin, 3, 3, 3), b, padding=1)[0]
+    (y * gy).sum().backward()
+    s = N.stream_of(device)
+    wd = w.detach().to(torch.float32).to(device)
+    wp = torch.empty(27 * cin * cout, dtype=torch.int16, device=device)
+    wt = torch.empty_like(wp)
+    N.call("wm_unet_pack", wd.data_ptr(), cin, cout, wp.data_ptr(), wt.data_ptr(), s)
+    xd = _to_dev_bits(x.detach().permute(1, 2, 3, 0), device)
+    gd = _to_dev_bits(gy.permute(1, 2, 3, 0), device)
+    gx = torch.empty(D * H * W * cin, dtype=torch.int16, device=device)
+    N.call("wm_unet_conv3", gd.data_ptr(), cout, gx.data_ptr(), None, cin, D, H, W, wt.data_ptr(),
+           None, 1.0, s)
+    dw = torch.zeros(cout * cin * 27, dtype=torch.float32, device=device)
+    db = torch.zeros(cout, dtype=torch.float32, device=device)
+    N.call("wm_unet_conv3_wgrad", xd.data_ptr(), cin, gd.data_ptr(), cout, D, H, W, dw.data_ptr(),
+           db.data_ptr(), s)
+    gxr = x.grad.permute(1, 2, 3, 0)
+    got = _from_dev_bits(gx, (D, H, W, cin))
+    assert (got - gxr).abs().max() <= 1e-2 * gxr.abs().max()   # dX is stored in bf16
+    dwg = dw.cpu().to(torch.float64).reshape(cout, cin, 27)
+    assert (dwg - w.grad).abs().max() <= 1e-5 * w.grad.abs().max()
+    assert (db.cpu().to(torch.float64) - b.grad).abs().max() <= 1e-5 * b.grad.abs().max()
+
+
+class _RefUNet(torch.nn.Module):
+    """The same network in torch (CPU, FP64) with the device weights."""
+
+    def __init__(self, net):
+        super().__init__()
+        self.layers = [torch.nn.Parameter(_bf16(c.w.detach().cpu().to(torch.float64)))
+                       for c in net.layers()]
+        self.bias = [torch.nn.Parameter(c.b.detach().cpu().to(torch.float64)) for c in net.layers()]
+        self.acts = [c.act for c in net.layers()]
+        self.slope = net.slope
+
+    def conv(self, i, x):
+        w = self.layers[i]
+        co, ci = w.shape[:2]
+        y = F.conv3d(x[None], w.reshape(co, ci, 3, 3, 3), self.bias[i], padding=1)[0]
+        return torch.where(y > 0, y, y * self.slope) if self.acts[i] else y
+
+    def forward(self, x):
+        skips, h, i = [], x, 0
+        for lv in range(4):
+            if lv > 0:
+                h = F.avg_pool3d(h[None], 2)[0]
+            h = self.conv(i + 1, self.conv(i, h))
+            i += 2
+            skips.append(h)
+        for lv in range(3):
+            up = F.interpolate(h[None], scale_factor=2, mode="nearest")[0]
+            h = torch.cat([up, skips[2 - lv]], 0)
+            h = self.conv(i + 1, self.conv(i, h))
+            i += 2
+        return self.conv(i, h)
+
+
+def test_unet_forward_backward_against_torch(device):
+    """Whole network: output and every layer's weight gradient agree with
+    torch autograd in FP64 on the same bf16 weights (bf16 activations on
+    the GPU: a few percent; gradient directions to cosine > 0.99)."""
+    from warp_amd import _native as N
+    from warp_amd.unet import UNet3D
+    D, H, W = 8, 16, 16
+    net = UNet3D(device, seed=4)
+    g = torch.Generator().manual_seed(5)
+    x = _bf16(torch.randn(16, D, H, W, generator=g, dtype=torch.float64))
+    x[3:] = 0
+    ref = _RefUNet(net)
+    yr = ref(x)
+    tgt = torch.randn(D, H, W, generator=g, dtype=torch.float64)
+    loss = ((yr[0] - tgt) ** 2).mean()
+    loss.backward()
+    xd = _to_dev_bits(x.permute(1, 2, 3, 0), device)
+    out = net.forward(xd, (D, H, W), keep=True)
+    o = out.cpu().to(torch.float64).reshape(D, H, W, 16)[..., 0]
+    assert (o - yr[0].detach()).abs().max() <= 0.05 * yr[0].abs().max()
+    gout = torch.empty(D * H * W * 16, dtype=torch.int16, device=device)
+    lossd = torch.zeros(1, dtype=torch.float32, device=device)
+    tg = tgt.reshape(-1).to(torch.float32).to(device)
+    N.call("wm_unet_mse", out.data_ptr(), tg.data_ptr(), D * H * W, gout.data_ptr(),
+           lossd.data_ptr(), N.stream_of(device))
+    assert float(lossd.item()) == pytest.approx(float(loss), rel=0.05)
+    net.zero_grad()
+    net.backward(gout)
+    for c, pr in zip(net.layers(), ref.layers):
+        a = c.gw.cpu().to(torch.float64).reshape(-1)
+        b = pr.grad.reshape(-1)
+        cos = float((a @ b) / (a.norm() * b.norm() + 1e-300))
+        assert cos > 0.99, cos
+
+
+def test_unet_training_overfits_one_sample(device):
+    """Adam on one fixed sample drives the loss down by 10x."""
+    from warp_amd import _native as N
+    from warp_amd.unet import UNet3D
+    D, H, W = 8, 16, 16
+    net = UNet3D(device, seed=6)
+    g = torch.Generator().manual_seed(7)
+    x = torch.randn(D, H, W, 16, generator=g)
+    x[..., 3:] = 0
+    xd = _to_dev_bits(x, device)
+    tgt = torch.sin(torch.linspace(0, 3, D * H * W)).to(torch.float32).to(device)
+    gout = torch.empty(D * H * W * 16, dtype=torch.int16, device=device)
+    loss = torch.zeros(1, dtype=torch.float32, device=device)
+    hist = []
+    for _ in range(60):
+        net.zero_grad()
+        out = net.forward(xd, (D, H, W), keep=True)
+        loss.zero_()
+        N.call("wm_unet_mse", out.data_ptr(), tgt.data_ptr(), D * H * W, gout.data_ptr(),
+               loss.data_ptr(), N.stream_of(device))
+        net.backward(gout)
+        net.adam_step(3e-3)
+        hist.append(float(loss.item()))
+    assert hist[-1] < 0.1 * hist[0], hist[::10]
+
+
+def test_warm_start_gain(device):
+    """Train on pairs the multigrid generates (32x32x64, pipe + annulus,
+    random beams moving 1 mm per step), then solve held-out steps from the
+    previous phi (plain multigrid) and from the network's guess.  Every
+    solve reaches mgtol either way; the network's guess is closer to the
+    solution than the previous phi, and the V-cycles saved are reported."""
+    import warp_amd
+    from tests import helpers as H
+    from warp_amd.unet_train import WarmStart, generate_pairs, vcycles_from
+    s = warp_amd.MultiGrid3D(nx=32, ny=32, nz=64, xmmin=-0.02, xmmax=0.02, ymmin=-0.02,
+                             ymmax=0.02, zmmin=0., zmmax=0.08, bounds=[0] * 6, mgverbose=-1,
+                             mgmaxiters=400)
+    for c in H.pipe_and_annulus(warp_amd):
+        s.installconductor(c)
+    train = generate_pairs(s, 24, seed=11)
+    test = generate_pairs(s, 6, seed=12)
+    ws = WarmStart(s, seed=1)
+    ws.fit_scales(train)
+    hist = ws.train(train, 300, lr=2e-3)
+    assert np.mean(hist[-30:]) < 0.5 * np.mean(hist[:30])
+    plain, learned, e_prev, e_net = [], [], [], []
+    for rho, p0, p1 in test:
+        guess = ws.predict(rho, p0)
+        e_prev.append(float((p0 - p1).abs().max()))
+        e_net.append(float((guess - p1).abs().max()))
+        it0, err0 = vcycles_from(s, rho, p0)
+        it1, err1 = vcycles_from(s, rho, guess)
+        assert err0 <= 1e-6 and err1 <= 1e-6
+        plain.append(it0)
+        learned.append(it1)
+    print("\nwarm start: V-cycles from previous phi %s, from the network %s; max error of the "
+          "guess %.3g -> %.3g V" % (plain, learned, np.mean(e_prev), np.mean(e_net)))
+    assert np.mean(e_net) < np.mean(e_prev)

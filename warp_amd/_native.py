@@ -73,6 +73,18 @@ SIGNATURES = {
                                    _d, _d, _d, _P, _P]),
     "wm_getzmmnt_worksize": (_i64, [_i64]),
     "wm_getzmmnt": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _P, _i64, _P, _P, _P]),
+    "wm_unet_conv3": (C.c_int, [_P, _i64, _P, _P, _i64, _i64, _i64, _i64, _P, _P, _d, _P]),
+    "wm_unet_conv3_wgrad": (C.c_int, [_P, _i64, _P, _i64, _i64, _i64, _i64, _P, _P, _P]),
+    "wm_unet_act_bwd": (C.c_int, [_P, _P, _d, _i64, _P]),
+    "wm_unet_pack": (C.c_int, [_P, _i64, _i64, _P, _P, _P]),
+    "wm_unet_pool": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _P]),
+    "wm_unet_up": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _i64, _P]),
+    "wm_unet_chcopy": (C.c_int, [_P, _i64, _i64, _P, _i64, _i64, _i64, _i64, _P]),
+    "wm_unet_add": (C.c_int, [_P, _P, _i64, _P]),
+    "wm_unet_adam": (C.c_int, [_P, _P, _P, _P, _i64, _d, _d, _d, _d, _i64, _P]),
+    "wm_unet_input": (C.c_int, [_P, _P, _P, _i64, _i64, _i64, _d, _d, _P, _P]),
+    "wm_unet_apply": (C.c_int, [_P, _P, _i64, _i64, _i64, _d, _P, _P]),
+    "wm_unet_mse": (C.c_int, [_P, _P, _i64, _P, _P, _P]),
     "wm_particle_sort_worksize": (_i64, [_i64, _i64, _i64, _i64]),
     "wm_particle_sort": (C.c_int, [_i64, _P, _P, _P, _d, _d, _d, _d, _d, _d, _i64, _i64, _i64,
                                    _P, _P, _P]),

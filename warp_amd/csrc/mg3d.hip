@@ -798,7 +798,8 @@ static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
   ExpandAxis az{L.ej[2].p, L.ew[2].p, L.elo[2], L.ehi[2]};
   long long rows = 2ll * std::max(0, ay.hi - ay.lo + 1) * std::max(0, az.hi - az.lo + 1);
   if (rows == 0 || ax.hi < ax.lo) return;
-  int nb = (int)std::min<long long>(8192, cdiv(rows, 4));
+  WM_REQUIRE(rows * L.g.W < (1ll << 31), "expand3d: level too large for 32-bit indexing");
+  int nb = (int)cdiv(rows * L.g.W, 256);
   hipLaunchKernelGGL((k_expand_rb<S, SC>), dim3(nb), dim3(256), 0, s, phi_of<S>(L).p, L.g,
                      (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, skip);
   WM_LAUNCHED();

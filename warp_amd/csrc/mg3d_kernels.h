@@ -701,9 +701,9 @@ __global__ __launch_bounds__(256) void k_expand(
             wx * wy * wz * pc[Cc(jx + 1, jy + 1, jz + 1)];
 }
 
-// expand3d on red-black fine and coarse arrays: one wavefront per fine row
-// of one colour, lanes over the row's elements, so every fine load and
-// store is contiguous; same sum as k_expand.
+// expand3d on red-black fine and coarse arrays, one thread per fine
+// element in the loop range (consecutive threads walk a colour row, so the
+// fine loads and stores are contiguous); same sum as k_expand.
 template <typename S, typename SC>
 __global__ __launch_bounds__(256) void k_expand_rb(
     S *__restrict__ phi, RBGeom g, const SC *__restrict__ pc, RBGeom gc, ExpandAxis ax,
@@ -712,48 +712,45 @@ __global__ __launch_bounds__(256) void k_expand_rb(
   WM_SKIP();
   int nyr = ay.hi - ay.lo + 1, nzr = az.hi - az.lo + 1;
   if (nyr <= 0 || nzr <= 0 || ax.hi < ax.lo) return;
-  int lane = threadIdx.x & 63;
-  int nrows = 2 * nyr * nzr;
-  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += gridDim.x * 4) {
-    int c = row / (nyr * nzr);
-    int rr = row - c * (nyr * nzr);
-    int iz = az.lo + rr / nyr;
-    int iy = ay.lo + rr % nyr;
-    int py = iy + 1, pz = iz + 1;
-    int p0 = (c + py + pz + 1) & 1;
-    int jy = ay.j[iy], jz = az.j[iz];
-    T wy = (T)ay.w[iy], wz = (T)az.w[iz];
-    T one = (T)1.;
-    long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
-    // coarse row offsets of (jy+dy, jz+dz); colour of (jx, jy+dy, jz+dz) is
-    // (jx + jy + dy + jz + dz) & 1
-    long long r00 = (long long)gc.W * ((jy + 1) + (long long)gc.PY * (jz + 1));
-    long long r10 = r00 + gc.W, r01 = r00 + (long long)gc.W * gc.PY, r11 = r01 + gc.W;
-    int syz = jy + jz;
-    // j range of this row: ix = p0 + 2j - 1 in [ax.lo, ax.hi]
-    int jlo = (ax.lo + 2 - p0) / 2, jhi = (ax.hi + 1 - p0) / 2;
-    for (int j = jlo + lane; j <= jhi; j += 64) {
-      int ix = p0 + 2 * j - 1;
-      int jx = ax.j[ix];
-      T wx = (T)ax.w[ix];
-      long long fi = base + j;
-      long long e0 = (jx + 1) >> 1, e1 = (jx + 2) >> 1;
-      long long ca = (long long)((jx + syz) & 1) * gc.cs, cb = gc.cs - ca;  // parity 0 / 1
+  // one thread per element (j, row) of the rows in range, both colours;
+  // coarse index and weight from expand3d's tables (f3d_mgrid.F:1833-1844)
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int j = t % g.W;
+  int row = t / g.W;
+  if (row >= 2 * nyr * nzr) return;
+  int c = row / (nyr * nzr);
+  int rr = row - c * (nyr * nzr);
+  int iz = az.lo + rr / nyr;
+  int iy = ay.lo + rr % nyr;
+  int py = iy + 1, pz = iz + 1;
+  int p0 = (c + py + pz + 1) & 1;
+  int ix = p0 + 2 * j - 1;
+  if (ix < ax.lo || ix > ax.hi) return;
+  int jy = ay.j[iy], jz = az.j[iz];
+  T wy = (T)ay.w[iy], wz = (T)az.w[iz];
+  int jx = ax.j[ix];
+  T wx = (T)ax.w[ix];
+  T one = (T)1.;
+  long long fi = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz) + j;
+  // coarse row offsets of (jy+dy, jz+dz); colour of (jx+dx, jy+dy, jz+dz)
+  // is (jx + jy + jz + dx + dy + dz) & 1
+  long long r00 = (long long)gc.W * ((jy + 1) + (long long)gc.PY * (jz + 1));
+  long long r10 = r00 + gc.W, r01 = r00 + (long long)gc.W * gc.PY, r11 = r01 + gc.W;
+  long long e0 = (jx + 1) >> 1, e1 = (jx + 2) >> 1;
+  long long ca = (long long)((jx + jy + jz) & 1) * gc.cs, cb = gc.cs - ca;
 #define PC(x, off, flip) ((T)St<SC>::ld(pc[((flip) ? cb : ca) + (x) + (off)]))
-      T c000 = PC(e0, r00, 0), c100 = PC(e1, r00, 1), c010 = PC(e0, r10, 1), c110 = PC(e1, r10, 0);
-      T c001 = PC(e0, r01, 1), c101 = PC(e1, r01, 0), c011 = PC(e0, r11, 0), c111 = PC(e1, r11, 1);
+  T c000 = PC(e0, r00, 0), c100 = PC(e1, r00, 1), c010 = PC(e0, r10, 1), c110 = PC(e1, r10, 0);
+  T c001 = PC(e0, r01, 1), c101 = PC(e1, r01, 0), c011 = PC(e0, r11, 0), c111 = PC(e1, r11, 1);
 #undef PC
-      phi[fi] = St<S>::st(St<S>::ld(phi[fi]) +
-                          (one - wx) * (one - wy) * (one - wz) * c000 +
-                          wx * (one - wy) * (one - wz) * c100 +
-                          (one - wx) * wy * (one - wz) * c010 +
-                          wx * wy * (one - wz) * c110 +
-                          (one - wx) * (one - wy) * wz * c001 +
-                          wx * (one - wy) * wz * c101 +
-                          (one - wx) * wy * wz * c011 +
-                          wx * wy * wz * c111);
-    }
-  }
+  phi[fi] = St<S>::st(St<S>::ld(phi[fi]) +
+                      (one - wx) * (one - wy) * (one - wz) * c000 +
+                      wx * (one - wy) * (one - wz) * c100 +
+                      (one - wx) * wy * (one - wz) * c010 +
+                      wx * wy * (one - wz) * c110 +
+                      (one - wx) * (one - wy) * wz * c001 +
+                      wx * (one - wy) * wz * c101 +
+                      (one - wx) * wy * wz * c011 +
+                      wx * wy * wz * c111);
 }
 
 // ---------------------------------------------------------------------------

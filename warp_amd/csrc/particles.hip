@@ -94,13 +94,20 @@ __device__ inline void corner_charges(double u1, double v1, double w1, double g,
   c[7] = u1 * v1 * w1 * g;
 }
 
-// Deposit in the particles' own order.
+// Deposit in the particles' own order.  A block first finds the box of
+// mesh nodes its 256 particles touch; when it fits in LDS (particles
+// ordered by cell give boxes of a few dozen nodes) the charges are summed
+// there and the box is added to rho with one atomic per touched node,
+// otherwise each wave adds its runs directly (deposit_runs).
+#define DEP_BOX 2048
 __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long long np,
                                                 const double *__restrict__ xp,
                                                 const double *__restrict__ yp,
                                                 const double *__restrict__ zp, Mesh m,
                                                 double g,
                                                 unsigned long long *__restrict__ nskip) {
+  __shared__ double box[DEP_BOX];
+  __shared__ int lim[4][6];
   long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
   long long ii = 0, jj = 0, kk = 0;
   double u1, v1, w1, c[8];
@@ -110,7 +117,56 @@ __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long l
     if (!ok && nskip) atomicAdd(nskip, 1ull);
   }
   if (ok) corner_charges(u1, v1, w1, g, c);
-  deposit_runs(rho, m, ok, ii, jj, kk, c);
+  // box of the block's cells
+  int big = 1 << 30;
+  int a0 = ok ? (int)ii : big, a1 = ok ? (int)ii : -big;
+  int b0 = ok ? (int)jj : big, b1 = ok ? (int)jj : -big;
+  int c0 = ok ? (int)kk : big, c1 = ok ? (int)kk : -big;
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 = min(a0, __shfl_xor(a0, off, 64)); a1 = max(a1, __shfl_xor(a1, off, 64));
+    b0 = min(b0, __shfl_xor(b0, off, 64)); b1 = max(b1, __shfl_xor(b1, off, 64));
+    c0 = min(c0, __shfl_xor(c0, off, 64)); c1 = max(c1, __shfl_xor(c1, off, 64));
+  }
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    lim[w][0] = a0; lim[w][1] = a1; lim[w][2] = b0; lim[w][3] = b1; lim[w][4] = c0; lim[w][5] = c1;
+  }
+  __syncthreads();
+  a0 = min(min(lim[0][0], lim[1][0]), min(lim[2][0], lim[3][0]));
+  a1 = max(max(lim[0][1], lim[1][1]), max(lim[2][1], lim[3][1]));
+  b0 = min(min(lim[0][2], lim[1][2]), min(lim[2][2], lim[3][2]));
+  b1 = max(max(lim[0][3], lim[1][3]), max(lim[2][3], lim[3][3]));
+  c0 = min(min(lim[0][4], lim[1][4]), min(lim[2][4], lim[3][4]));
+  c1 = max(max(lim[0][5], lim[1][5]), max(lim[2][5], lim[3][5]));
+  if (a1 < a0) return;  // no particle of the block is in the grid
+  long long nbx = a1 - a0 + 2, nby = b1 - b0 + 2, nbz = c1 - c0 + 2;
+  if (nbx * nby * nbz > DEP_BOX) {
+    deposit_runs(rho, m, ok, ii, jj, kk, c);
+    return;
+  }
+  int bx = (int)nbx, bxy = (int)(nbx * nby), nb = (int)(nbx * nby * nbz);
+  for (int e = threadIdx.x; e < nb; e += 256) box[e] = 0.;
+  __syncthreads();
+  if (ok) {
+    double *bp = box + (ii - a0) + bx * (jj - b0) + bxy * (kk - c0);
+    atomicAdd(bp, c[0]);
+    atomicAdd(bp + 1, c[1]);
+    atomicAdd(bp + bx, c[2]);
+    atomicAdd(bp + bx + 1, c[3]);
+    atomicAdd(bp + bxy, c[4]);
+    atomicAdd(bp + bxy + 1, c[5]);
+    atomicAdd(bp + bxy + bx, c[6]);
+    atomicAdd(bp + bxy + bx + 1, c[7]);
+  }
+  __syncthreads();
+  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
+  for (int e = threadIdx.x; e < nb; e += 256) {
+    double v = box[e];
+    if (v != 0.) {
+      int ex = e % bx, ey = (e / bx) % (int)nby, ez = e / bxy;
+      atomicAdd(rho + (a0 + ex) + sx * (b0 + ey) + sxy * (c0 + ez), v);
+    }
+  }
 }
 
 // Cell key of each particle (x fastest); out-of-grid particles get the key

@@ -1,0 +1,538 @@
+// 3-D U-Net kernels for the learned multigrid warm start, gfx950.
+//
+// Activations are channels-last bf16, x[(z*H + y)*W + x][C], C a multiple of
+// 16.  A 3x3x3 convolution (zero padding 1) is an implicit GEMM:
+// M = voxels, N = output channels, K = 27 taps x input channels, computed
+// with v_mfma_f32_16x16x16_bf16 (bf16 operands, fp32 accumulation).  Weights
+// are packed per 16-channel input block so that every MFMA operand a lane
+// needs is 8 contiguous bytes:
+//   Wp[((tap*(Cin/16) + cb)*Cout + co)*16 + ci%16].
+// The backward pass over the input is the same convolution with the taps
+// mirrored and the channel roles exchanged (a second packing of the same
+// weights); the backward pass over the weights accumulates
+// dW[co][ci][tap] = sum_v x[v+tap][ci] dy[v][co] with MFMA over voxels, in
+// the layout of the fp32 master weights.
+#include "wm_internal.h"
+#include "../../include/warpmi.h"
+
+#include <algorithm>
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ inline uint16_t f2bf(float f) {
+  uint32_t x = __float_as_uint(f);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+
+__device__ inline s4 load4(const uint16_t *p) {
+  // 8-byte aligned: channel offsets are multiples of 4
+  uint2 v = *(const uint2 *)p;
+  s4 r;
+  r.x = (short)(v.x & 0xffffu);
+  r.y = (short)(v.x >> 16);
+  r.z = (short)(v.y & 0xffffu);
+  r.w = (short)(v.y >> 16);
+  return r;
+}
+
+struct Dims {
+  int D, H, W;
+};
+
+// Forward convolution.  One wavefront computes 16 consecutive voxels (in
+// x-fastest order; the voxel count is a multiple of 16) for all output
+// channels (NT = Cout/16 <= 8 accumulators); a block is 4 wavefronts.  out = act(conv + bias), act the
+// leaky ReLU of slope `slope` (1 = identity); bias may be NULL.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, int cin,
+                                               uint16_t *__restrict__ y, float *__restrict__ yf,
+                                               Dims d, const uint16_t *__restrict__ wp,
+                                               const float *__restrict__ bias, float slope,
+                                               long long nvox) {
+  int lane = threadIdx.x & 63;
+  long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (v0 >= nvox) return;
+  int cout = NT * 16;
+  int m = lane & 15, kg = lane >> 4;
+  // this lane's A-operand voxel
+  long long vm = v0 + m;
+  int xw = (int)(vm % d.W);
+  long long r = vm / d.W;
+  int yy = (int)(r % d.H);
+  int zz = (int)(r / d.H);
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  int ncb = cin >> 4;
+  for (int tap = 0; tap < 27; tap++) {
+    int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    int sz = zz + dz, sy = yy + dy, sx = xw + dx;
+    bool in = sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 && sx < d.W;
+    const uint16_t *src = x + (((long long)sz * d.H + sy) * d.W + sx) * cin + 4 * kg;
+    const uint16_t *wt = wp + (long long)tap * ncb * cout * 16 + m * 16 + 4 * kg;
+    for (int cb = 0; cb < ncb; cb++) {
+      s4 a = in ? load4(src + cb * 16) : s4{0, 0, 0, 0};
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        s4 b = load4(wt + ((long long)cb * cout + t * 16) * 16);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // lane holds D[4*kg + i][m]: voxel v0 + 4*kg + i, channel t*16 + m
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    int co = t * 16 + m;
+    float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      float v = acc[t][i] + bv;
+      v = v > 0.f ? v : v * slope;
+      long long vi = v0 + 4 * kg + i;
+      if (y) y[vi * cout + co] = f2bf(v);
+      if (yf) yf[vi * cout + co] = v;
+    }
+  }
+}
+
+// Weight gradient in the master layout: dw[(co*cin + ci)*27 + tap] +=
+// sum_v x[v+tap][ci] g[v][co].
+// One wavefront per (tap, 16 output channels, 16 input channels, chunk of
+// voxels); MFMA with M = co, N = ci, K = voxels.
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict__ x, int cin,
+                                                     const uint16_t *__restrict__ g, int cout,
+                                                     Dims d, float *__restrict__ dw,
+                                                     long long nvox, int chunk, int nchunk) {
+  int lane = threadIdx.x & 63;
+  long long wv = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  int nct = cout >> 4, nci = cin >> 4;
+  long long combos = 27ll * nct * nci;
+  if (wv >= combos * nchunk) return;
+  int ch = (int)(wv / combos);
+  int cmb = (int)(wv % combos);
+  int tap = cmb / (nct * nci);
+  int ct = (cmb / nci) % nct;
+  int cit = cmb % nci;
+  int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  int m = lane & 15, kg = lane >> 4;
+  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+  long long vbeg = (long long)ch * chunk, vend = std::min<long long>(vbeg + chunk, nvox);
+  for (long long vb = vbeg; vb < vend; vb += 16) {
+    s4 a, b;
+    short av[4], bvv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      long long v = vb + 4 * kg + i;
+      uint16_t ga = 0, xb = 0;
+      if (v < vend) {
+        ga = g[v * cout + ct * 16 + m];
+        int xw = (int)(v % d.W);
+        long long r = v / d.W;
+        int yy = (int)(r % d.H), zz = (int)(r / d.H);
+        int sz = zz + dz, sy = yy + dy, sx = xw + dx;
+        if (sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 && sx < d.W)
+          xb = x[(((long long)sz * d.H + sy) * d.W + sx) * cin + cit * 16 + m];
+      }
+      av[i] = (short)ga;
+      bvv[i] = (short)xb;
+    }
+    a = s4{av[0], av[1], av[2], av[3]};
+    b = s4{bvv[0], bvv[1], bvv[2], bvv[3]};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  // lane holds D[4*kg + i][m] = dW for co = ct*16 + 4*kg + i, ci = cit*16 + m
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int co = ct * 16 + 4 * kg + i, ci = cit * 16 + m;
+    atomicAdd(dw + ((long long)co * cin + ci) * 27 + tap, acc[i]);
+  }
+}
+
+// Bias gradient: db[co] += sum_v g[v][co].
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ g, int cout,
+                                                   long long nvox, float *__restrict__ db) {
+  __shared__ float part[256];
+  int co = threadIdx.x % cout;
+  int lanes = 256 / cout * cout;
+  float s = 0.f;
+  if ((int)threadIdx.x < lanes) {
+    for (long long v = (long long)blockIdx.x * (256 / cout) + threadIdx.x / cout; v < nvox;
+         v += (long long)gridDim.x * (256 / cout))
+      s += bf2f(g[v * cout + co]);
+  }
+  part[threadIdx.x] = (int)threadIdx.x < lanes ? s : 0.f;
+  __syncthreads();
+  if ((int)threadIdx.x < cout) {
+    float t = 0.f;
+    for (int k = threadIdx.x; k < lanes; k += cout) t += part[k];
+    atomicAdd(db + co, t);
+  }
+}
+
+// g = g * act'(y): the leaky ReLU keeps the sign, so y > 0 <=> pre > 0.
+__global__ __launch_bounds__(256) void k_act_bwd(uint16_t *__restrict__ g,
+                                                 const uint16_t *__restrict__ y, float slope,
+                                                 long long n) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  if (!(bf2f(y[t]) > 0.f)) g[t] = f2bf(bf2f(g[t]) * slope);
+}
+
+// Pack fp32 master weights w[co][ci][tap] into the forward layout, and the
+// mirrored/transposed layout of the backward-over-input convolution.
+__global__ __launch_bounds__(256) void k_pack(const float *__restrict__ w, int cin, int cout,
+                                              uint16_t *__restrict__ wp,
+                                              uint16_t *__restrict__ wt) {
+  long long n = 27ll * cin * cout;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int tap = (int)(t % 27);
+  long long u = t / 27;
+  int ci = (int)(u % cin), co = (int)(u / cin);
+  uint16_t b = f2bf(w[t]);
+  wp[(((long long)tap * (cin / 16) + ci / 16) * cout + co) * 16 + ci % 16] = b;
+  // backward over the input: input channels co, output channels ci, tap mirrored
+  int mt = 26 - tap;
+  wt[(((long long)mt * (cout / 16) + co / 16) * cin + ci) * 16 + co % 16] = b;
+}
+
+// Average pooling 2x2x2 (forward) and its adjoint.
+__global__ __launch_bounds__(256) void k_pool(const uint16_t *__restrict__ x, Dims d, int c,
+                                              uint16_t *__restrict__ y) {
+  int D2 = d.D / 2, H2 = d.H / 2, W2 = d.W / 2;
+  long long n = (long long)D2 * H2 * W2 * c;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ch = (int)(t % c);
+  long long v = t / c;
+  int xw = (int)(v % W2);
+  long long r = v / W2;
+  int yy = (int)(r % H2), zz = (int)(r / H2);
+  float s = 0.f;
+  for (int k = 0; k < 8; k++) {
+    long long sv = ((long long)(2 * zz + (k >> 2)) * d.H + 2 * yy + ((k >> 1) & 1)) * d.W + 2 * xw + (k & 1);
+    s += bf2f(x[sv * c + ch]);
+  }
+  y[t] = f2bf(0.125f * s);
+}
+
+__global__ __launch_bounds__(256) void k_pool_bwd(const uint16_t *__restrict__ gy, Dims d,
+                                                  int c, uint16_t *__restrict__ gx) {
+  long long n = (long long)d.D * d.H * d.W * c;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ch = (int)(t % c);
+  long long v = t / c;
+  int xw = (int)(v % d.W);
+  long long r = v / d.W;
+  int yy = (int)(r % d.H), zz = (int)(r / d.H);
+  long long cv = ((long long)(zz / 2) * (d.H / 2) + yy / 2) * (d.W / 2) + xw / 2;
+  gx[t] = f2bf(0.125f * bf2f(gy[cv * c + ch]));
+}
+
+// Nearest upsampling x2 of x (dims d, channels cx) into channels [0, cx) of
+// y (dims 2d, cy channels); the adjoint sums the eight children.
+__global__ __launch_bounds__(256) void k_up(const uint16_t *__restrict__ x, Dims d, int cx,
+                                            uint16_t *__restrict__ y, int cy) {
+  long long n = 8ll * d.D * d.H * d.W * cx;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ch = (int)(t % cx);
+  long long v = t / cx;
+  int W2 = 2 * d.W, H2 = 2 * d.H;
+  int xw = (int)(v % W2);
+  long long r = v / W2;
+  int yy = (int)(r % H2), zz = (int)(r / H2);
+  long long sv = ((long long)(zz / 2) * d.H + yy / 2) * d.W + xw / 2;
+  y[v * cy + ch] = x[sv * cx + ch];
+}
+
+__global__ __launch_bounds__(256) void k_up_bwd(const uint16_t *__restrict__ gy, Dims d,
+                                                int cx, int cy, uint16_t *__restrict__ gx) {
+  long long n = (long long)d.D * d.H * d.W * cx;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ch = (int)(t % cx);
+  long long v = t / cx;
+  int xw = (int)(v % d.W);
+  long long r = v / d.W;
+  int yy = (int)(r % d.H), zz = (int)(r / d.H);
+  float s = 0.f;
+  for (int k = 0; k < 8; k++) {
+    long long cv = ((long long)(2 * zz + (k >> 2)) * (2 * d.H) + 2 * yy + ((k >> 1) & 1)) * (2 * d.W) +
+                   2 * xw + (k & 1);
+    s += bf2f(gy[cv * cy + ch]);
+  }
+  gx[t] = f2bf(s);
+}
+
+// Channel copy between channels-last tensors: dst[v][doff + c] = src[v][soff + c].
+__global__ __launch_bounds__(256) void k_chcopy(const uint16_t *__restrict__ src, int sc, int soff,
+                                                uint16_t *__restrict__ dst, int dc, int doff,
+                                                int c, long long nvox) {
+  long long n = nvox * c;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ch = (int)(t % c);
+  long long v = t / c;
+  dst[v * dc + doff + ch] = src[v * sc + soff + ch];
+}
+
+// a += b elementwise (bf16).
+__global__ __launch_bounds__(256) void k_add(uint16_t *__restrict__ a,
+                                             const uint16_t *__restrict__ b, long long n) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) a[t] = f2bf(bf2f(a[t]) + bf2f(b[t]));
+}
+
+// Adam on fp32 master weights (Kingma & Ba, bias-corrected step).
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, const float *__restrict__ gw,
+                                              float *__restrict__ m1, float *__restrict__ m2,
+                                              long long n, float lr, float b1, float b2,
+                                              float eps, float c1, float c2) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  float g = gw[t];
+  float a = b1 * m1[t] + (1.f - b1) * g;
+  float b = b2 * m2[t] + (1.f - b2) * g * g;
+  m1[t] = a;
+  m2[t] = b;
+  w[t] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// Network input: channels (rho*srho, mask, phi_prev*sphi, 0...) of 16, from
+// FP64 natural arrays phi (nz+3, ny+3, nx+3) and rho (nz+1, ny+1, nx+1) over
+// the nodes 0..n-1 of each axis, and a 0/1 byte mask of the same nodes.
+__global__ __launch_bounds__(256) void k_unet_input(const double *__restrict__ rho,
+                                                    const double *__restrict__ phi,
+                                                    const uint8_t *__restrict__ mask, int nx,
+                                                    int ny, int nz, double srho, double sphi,
+                                                    uint16_t *__restrict__ x) {
+  long long n = (long long)nx * ny * nz;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  int ix = (int)(t % nx);
+  long long r = t / nx;
+  int iy = (int)(r % ny), iz = (int)(r / ny);
+  double rv = rho[ix + (long long)(nx + 1) * (iy + (long long)(ny + 1) * iz)];
+  double pv = phi[(ix + 1) + (long long)(nx + 3) * ((iy + 1) + (long long)(ny + 3) * (iz + 1))];
+  uint16_t *o = x + t * 16;
+  o[0] = f2bf((float)(rv * srho));
+  o[1] = f2bf(mask ? (float)mask[t] : 0.f);
+  o[2] = f2bf((float)(pv * sphi));
+  for (int k = 3; k < 16; k++) o[k] = 0;
+}
+
+// phi(0..n-1) = phi_prev + out/sphi at the nodes that are not pinned
+// (mask 0); out is the fp32 network output (channel 0 of 16).
+__global__ __launch_bounds__(256) void k_unet_apply(const float *__restrict__ out,
+                                                    const uint8_t *__restrict__ mask, int nx,
+                                                    int ny, int nz, double sphi,
+                                                    double *__restrict__ phi) {
+  long long n = (long long)nx * ny * nz;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  if (mask && mask[t]) return;
+  int ix = (int)(t % nx);
+  long long r = t / nx;
+  int iy = (int)(r % ny), iz = (int)(r / ny);
+  long long p = (ix + 1) + (long long)(nx + 3) * ((iy + 1) + (long long)(ny + 3) * (iz + 1));
+  phi[p] = phi[p] + (double)out[t * 16] / sphi;
+}
+
+// Mean-square loss on channel 0 and its gradient (bf16, 16 channels):
+// loss += sum (out - tgt)^2 / n, g = 2 (out - tgt) / n.
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ out,
+                                             const float *__restrict__ tgt, long long n,
+                                             uint16_t *__restrict__ g, float *__restrict__ loss) {
+  __shared__ float part[256];
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  float e = 0.f;
+  if (t < n) {
+    float d = out[t * 16] - tgt[t];
+    e = d * d / (float)n;
+    g[t * 16] = f2bf(2.f * d / (float)n);
+    for (int k = 1; k < 16; k++) g[t * 16 + k] = 0;
+  }
+  part[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+inline int nb(long long n) { return (int)std::max<long long>(1, (n + 255) / 256); }
+
+}  // namespace
+
+extern "C" {
+
+int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_t cout,
+                  int64_t D, int64_t H, int64_t W, const uint16_t *wp, const float *bias,
+                  double slope, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(x && wp && (y || yf), "wm_unet_conv3: NULL argument");
+    WM_REQUIRE(cin % 16 == 0 && cin > 0 && cout % 16 == 0 && cout >= 16 && cout <= 128,
+               "wm_unet_conv3: channels must be multiples of 16 (cout <= 128)");
+    WM_REQUIRE(D > 0 && H > 0 && W > 0 && (D * H * W) % 16 == 0,
+               "wm_unet_conv3: the voxel count must be a multiple of 16");
+    long long nvox = D * H * W;
+    Dims d{(int)D, (int)H, (int)W};
+    int blocks = (int)((nvox / 16 + 3) / 4);
+    hipStream_t s = wm_stream(stream);
+    float sl = (float)slope;
+    switch (cout / 16) {
+      case 1: hipLaunchKernelGGL(k_conv3<1>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 2: hipLaunchKernelGGL(k_conv3<2>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 3: hipLaunchKernelGGL(k_conv3<3>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 4: hipLaunchKernelGGL(k_conv3<4>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 5: hipLaunchKernelGGL(k_conv3<5>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 6: hipLaunchKernelGGL(k_conv3<6>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 7: hipLaunchKernelGGL(k_conv3<7>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 8: hipLaunchKernelGGL(k_conv3<8>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      default: throw wm_error("wm_unet_conv3: cout > 128");
+    }
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_conv3_wgrad(const uint16_t *x, int64_t cin, const uint16_t *g, int64_t cout,
+                        int64_t D, int64_t H, int64_t W, float *dw, float *db, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(x && g && dw, "wm_unet_conv3_wgrad: NULL argument");
+    WM_REQUIRE(cin % 16 == 0 && cout % 16 == 0 && cin > 0 && cout > 0,
+               "wm_unet_conv3_wgrad: channels must be multiples of 16");
+    long long nvox = D * H * W;
+    Dims d{(int)D, (int)H, (int)W};
+    int chunk = 4096;
+    int nchunk = (int)((nvox + chunk - 1) / chunk);
+    long long waves = 27ll * (cout / 16) * (cin / 16) * nchunk;
+    hipStream_t s = wm_stream(stream);
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, x,
+                       (int)cin, g, (int)cout, d, dw, nvox, chunk, nchunk);
+    WM_LAUNCHED();
+    if (db) {
+      WM_REQUIRE(256 % cout == 0 || cout <= 256, "wm_unet_conv3_wgrad: bad cout");
+      hipLaunchKernelGGL(k_bias_grad, dim3(1024), dim3(256), 0, s, g, (int)cout, nvox, db);
+      WM_LAUNCHED();
+    }
+  });
+}
+
+int wm_unet_act_bwd(uint16_t *g, const uint16_t *y, double slope, int64_t n, void *stream) {
+  WM_ENTRY({
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_act_bwd, dim3(nb(n)), dim3(256), 0, wm_stream(stream), g, y, (float)slope,
+                       (long long)n);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_pack(const float *w, int64_t cin, int64_t cout, uint16_t *wp, uint16_t *wt,
+                 void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(cin % 16 == 0 && cout % 16 == 0, "wm_unet_pack: channels must be multiples of 16");
+    long long n = 27ll * cin * cout;
+    hipLaunchKernelGGL(k_pack, dim3(nb(n)), dim3(256), 0, wm_stream(stream), w, (int)cin, (int)cout,
+                       wp, wt);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_pool(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t c, uint16_t *y,
+                 int64_t backward, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "wm_unet_pool: odd dimension");
+    Dims d{(int)D, (int)H, (int)W};
+    hipStream_t s = wm_stream(stream);
+    if (!backward) {
+      hipLaunchKernelGGL(k_pool, dim3(nb(D * H * W * c / 8)), dim3(256), 0, s, x, d, (int)c, y);
+    } else {
+      // x is the gradient at the pooled size, y the gradient at the full size
+      hipLaunchKernelGGL(k_pool_bwd, dim3(nb(D * H * W * c)), dim3(256), 0, s, x, d, (int)c, y);
+    }
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_up(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx, uint16_t *y,
+               int64_t cy, int64_t backward, void *stream) {
+  WM_ENTRY({
+    Dims d{(int)D, (int)H, (int)W};
+    hipStream_t s = wm_stream(stream);
+    if (!backward) {
+      hipLaunchKernelGGL(k_up, dim3(nb(8 * D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx, y,
+                         (int)cy);
+    } else {
+      // x: gradient at the upsampled size (cy channels, first cx used);
+      // y: gradient at the coarse size (cx channels)
+      hipLaunchKernelGGL(k_up_bwd, dim3(nb(D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx,
+                         (int)cy, y);
+    }
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_chcopy(const uint16_t *src, int64_t sc, int64_t soff, uint16_t *dst, int64_t dc,
+                   int64_t doff, int64_t c, int64_t nvox, void *stream) {
+  WM_ENTRY({
+    hipLaunchKernelGGL(k_chcopy, dim3(nb(nvox * c)), dim3(256), 0, wm_stream(stream), src, (int)sc,
+                       (int)soff, dst, (int)dc, (int)doff, (int)c, (long long)nvox);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_add(uint16_t *a, const uint16_t *b, int64_t n, void *stream) {
+  WM_ENTRY({
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_add, dim3(nb(n)), dim3(256), 0, wm_stream(stream), a, b, (long long)n);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_adam(float *w, const float *g, float *m1, float *m2, int64_t n, double lr,
+                 double b1, double b2, double eps, int64_t step, void *stream) {
+  WM_ENTRY({
+    float c1 = 1.f - powf((float)b1, (float)step), c2 = 1.f - powf((float)b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nb(n)), dim3(256), 0, wm_stream(stream), w, g, m1, m2,
+                       (long long)n, (float)lr, (float)b1, (float)b2, (float)eps, c1, c2);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_input(const double *rho, const double *phi, const uint8_t *mask, int64_t nx,
+                  int64_t ny, int64_t nz, double srho, double sphi, uint16_t *x, void *stream) {
+  WM_ENTRY({
+    hipLaunchKernelGGL(k_unet_input, dim3(nb(nx * ny * nz)), dim3(256), 0, wm_stream(stream), rho,
+                       phi, mask, (int)nx, (int)ny, (int)nz, srho, sphi, x);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_apply(const float *out, const uint8_t *mask, int64_t nx, int64_t ny, int64_t nz,
+                  double sphi, double *phi, void *stream) {
+  WM_ENTRY({
+    hipLaunchKernelGGL(k_unet_apply, dim3(nb(nx * ny * nz)), dim3(256), 0, wm_stream(stream), out,
+                       mask, (int)nx, (int)ny, (int)nz, sphi, phi);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_mse(const float *out, const float *tgt, int64_t n, uint16_t *g, float *loss,
+                void *stream) {
+  WM_ENTRY({
+    hipLaunchKernelGGL(k_mse, dim3(nb(n)), dim3(256), 0, wm_stream(stream), out, tgt,
+                       (long long)n, g, loss);
+    WM_LAUNCHED();
+  });
+}
+
+}  // extern "C"
