@@ -245,8 +245,7 @@ int wm_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
 /* 3x3x3 convolution, zero padding 1, bf16 MFMA with fp32 accumulation:
  * y = leaky_relu(conv(x) + bias, slope) (slope 1: identity; bias may be
  * NULL).  Output to y (bf16) and/or yf (fp32), either may be NULL.
- * D*H*W must be a multiple of 16, cin and cout multiples of 16,
- * cout <= 128. */
+ * cin and cout multiples of 16, cout <= 128. */
 int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf,
                   int64_t cout, int64_t D, int64_t H, int64_t W,
                   const uint16_t *wp, const float *bias, double slope,

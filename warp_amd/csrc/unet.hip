@@ -46,7 +46,7 @@ struct Dims {
 };
 
 // Forward convolution.  One wavefront computes 16 consecutive voxels (in
-// x-fastest order; the voxel count is a multiple of 16) for all output
+// x-fastest order; lanes past the last voxel load zeros and store nothing) for all output
 // channels (NT = Cout/16 <= 8 accumulators); a block is 4 wavefronts.  out = act(conv + bias), act the
 // leaky ReLU of slope `slope` (1 = identity); bias may be NULL.
 template <int NT>
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
   for (int tap = 0; tap < 27; tap++) {
     int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
     int sz = zz + dz, sy = yy + dy, sx = xw + dx;
-    bool in = sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 && sx < d.W;
+    bool in = vm < nvox && sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 && sx < d.W;
     const uint16_t *src = x + (((long long)sz * d.H + sy) * d.W + sx) * cin + 4 * kg;
     const uint16_t *wt = wp + (long long)tap * ncb * cout * 16 + m * 16 + 4 * kg;
     for (int cb = 0; cb < ncb; cb++) {
@@ -95,6 +95,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
       float v = acc[t][i] + bv;
       v = v > 0.f ? v : v * slope;
       long long vi = v0 + 4 * kg + i;
+      if (vi >= nvox) continue;
       if (y) y[vi * cout + co] = f2bf(v);
       if (yf) yf[vi * cout + co] = v;
     }
@@ -382,11 +383,10 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
     WM_REQUIRE(x && wp && (y || yf), "wm_unet_conv3: NULL argument");
     WM_REQUIRE(cin % 16 == 0 && cin > 0 && cout % 16 == 0 && cout >= 16 && cout <= 128,
                "wm_unet_conv3: channels must be multiples of 16 (cout <= 128)");
-    WM_REQUIRE(D > 0 && H > 0 && W > 0 && (D * H * W) % 16 == 0,
-               "wm_unet_conv3: the voxel count must be a multiple of 16");
+    WM_REQUIRE(D > 0 && H > 0 && W > 0, "wm_unet_conv3: empty grid");
     long long nvox = D * H * W;
     Dims d{(int)D, (int)H, (int)W};
-    int blocks = (int)((nvox / 16 + 3) / 4);
+    int blocks = (int)(((nvox + 15) / 16 + 3) / 4);
     hipStream_t s = wm_stream(stream);
     float sl = (float)slope;
     switch (cout / 16) {

@@ -53,8 +53,8 @@ class _Conv:
 
 
 class UNet3D:
-    """The warm-start network on a D x H x W node grid (each divisible by 8,
-    and D*H*W/512 by 16 so the coarsest level fills whole wavefronts)."""
+    """The warm-start network on a D x H x W node grid (each divisible
+    by 8)."""
 
     def __init__(self, device, seed=0, channels=CHANNELS, slope=SLOPE):
         self.device = torch.device(device)
