@@ -635,8 +635,13 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   if (P.lcnd) sub = cond_of<T>(L).sor[c].view();
   int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
   if (nyr <= 0 || nzr <= 0) return;
-  int nby = (int)cdiv(nyr, 4);
-  int nbsub = (int)cdiv(sub.n, 256);
+  // rows per z-marching block (wavefronts of 64 lanes, one row each)
+  static const int rows_env = getenv("WARPMI_SOR_ROWS") ? atoi(getenv("WARPMI_SOR_ROWS")) : 0;
+  static const bool tiled_env0 = getenv("WARPMI_SOR_TILED") != nullptr;
+  int rows = rows_env > 0 && rows_env <= 16 && !tiled_env0 ? rows_env : 4;
+  int nthr = 64 * rows;
+  int nby = (int)cdiv(nyr, rows);
+  int nbsub = (int)cdiv(sub.n, nthr);
   static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
   const bool tiled = tiled_env && !std::is_same<S, bf16>::value;
   // z-marching sweep: colour elements j0..j1 cover ix in [ixmin, ixmax]
@@ -660,13 +665,14 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   }
   if constexpr (!std::is_same<S, bf16>::value) {
     if (tiled)
-      hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+      hipLaunchKernelGGL((k_sor_halfpass<T>), dim3(nbdense + (int)cdiv(sub.n, 256)), dim3(256), 0,
+                         s, phi, rho,
                          L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
                          dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
                          nbdense, skip);
   }
   if (!tiled)
-    hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbdense + nbsub), dim3(256), 0, s, phi, rho,
+    hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbdense + nbsub), dim3(nthr), 0, s, phi, rho,
                        L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
                        dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0, nbx, nby,
                        zc, nbdense, skip);

@@ -194,10 +194,10 @@ __global__ __launch_bounds__(256) void k_sor_halfpass(
 // one row segment of 64 elements: the x-neighbour on the far side comes
 // from the next lane (a shuffle), the y-neighbours from the rows beside
 // it, which the neighbouring waves of the block load in the same step.
-// Blocks: (row segment jseg, 4 rows, z chunk), remapped per XCD.
+// Blocks: (row segment jseg, blockDim/64 rows, z chunk), remapped per XCD.
 // ---------------------------------------------------------------------------
 template <typename S, typename T = typename St<S>::T>
-__global__ __launch_bounds__(256) void k_sor_zmarch(
+__global__ __launch_bounds__(1024) void k_sor_zmarch(
     S *__restrict__ phi, const S *__restrict__ rho, RBGeom g, int c, int ixmin,
     int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
     T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void k_sor_zmarch(
     int bz = t / nby;
     int lane = threadIdx.x & 63;
     int j = j0 + bx * 64 + lane;
-    int iy = iymin + by * 4 + (threadIdx.x >> 6);
+    int iy = iymin + by * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (iy > iymax) return;  // whole wave: no shuffles are skipped mid-wave
     int za = izmin + bz * zc;
     int zb = min(za + zc - 1, izmax);
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_sor_zmarch(
     }
     return;
   }
-  int i = (b - nbdense) * 256 + threadIdx.x;
+  int i = (b - nbdense) * (int)blockDim.x + threadIdx.x;
   if (i >= sub.n) return;
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];

@@ -67,6 +67,10 @@ class UNet3D:
         self.dec = [[mk(c3 + c2, c2), mk(c2, c2)], [mk(c2 + c1, c1), mk(c1, c1)],
                     [mk(c1 + c0, c0), mk(c0, c0)]]
         self.out = mk(c0, 16, act=False)  # channel 0 is the output
+        # the last layer starts at zero: the untrained network predicts no
+        # change, i.e. the previous phi, which is the plain method
+        self.out.w.zero_()
+        self.out.pack()
         self.channels = channels
         self.step_count = 0
 

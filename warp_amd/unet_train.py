@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .globals import echarge
+from .globals import eps0
 from .unet import UNet3D, conductor_mask, make_input
 
 
@@ -118,6 +118,28 @@ class WarmStart:
                self.solver.ny, self.solver.nz, self.sdphi, phi.data_ptr(),
                N.stream_of(self.solver.device))
         return phi
+
+
+def residual_max(solver, rho, phi):
+    """max |residual| of phi for the source rho on the finest level
+    (residual3d, as the V-cycle computes it; conductors included)."""
+    res = torch.zeros_like(phi)
+    reps0c = solver.mgparam / (eps0 * 2. * (1. / solver.dx ** 2 + 1. / solver.dy ** 2 +
+                                             1. / solver.dz ** 2))
+    scaled = rho * reps0c
+    N.call("wm_mg3d_residual", solver._h, 0, phi.data_ptr(), scaled.contiguous().data_ptr(),
+           res.data_ptr(), solver.mgparam, solver.mgform, 0, int(solver.lcndbndy),
+           solver.icndbndy, N.stream_of(solver.device))
+    return float(res.abs().max())
+
+
+def choose_guess(solver, ws, rho, phi_prev):
+    """The network's guess unless its residual is larger than that of the
+    previous phi (then the plain method's guess: a fallback)."""
+    guess = ws.predict(rho, phi_prev)
+    if residual_max(solver, rho, guess) < residual_max(solver, rho, phi_prev):
+        return guess, False
+    return phi_prev, True
 
 
 def vcycles_from(solver, rho, phi0, mgtol=1e-6):
