@@ -57,6 +57,21 @@ struct Sym {
   double k = 0.;
 };
 
+// Row kernels are instantiated for KW = ceil(W / 64) groups of 64 elements
+// per lane pass (1-4, else 8 with a loop over the row).
+static int kw_of(int W) {
+  int k = (W + 63) / 64;
+  return k <= 4 ? k : 8;
+}
+#define WM_KW_DISPATCH(W, LAUNCH)                 \
+  switch (kw_of(W)) {                             \
+    case 1: { constexpr int KW = 1; LAUNCH; } break; \
+    case 2: { constexpr int KW = 2; LAUNCH; } break; \
+    case 3: { constexpr int KW = 3; LAUNCH; } break; \
+    case 4: { constexpr int KW = 4; LAUNCH; } break; \
+    default: { constexpr int KW = 8; LAUNCH; } break; \
+  }
+
 static Sym sym_mul(Sym a, double k) {
   if (a.mode == MODE_RAW) return Sym{MODE_MUL, k};
   if (a.mode == MODE_CONST) return Sym{MODE_CONST, a.k * k};
@@ -239,6 +254,7 @@ struct wm_mg3d {
   // level-0 extras
   DevBuf<double> save64, rhosave64;
   DevBuf<SolveState> state;
+  DevBuf<double> partials;  // per-block maxima of the convergence test
   SolveState *state_h = nullptr;  // pinned
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -804,10 +820,23 @@ static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
   ExpandAxis az{L.ej[2].p, L.ew[2].p, L.elo[2], L.ehi[2]};
   long long rows = 2ll * std::max(0, ay.hi - ay.lo + 1) * std::max(0, az.hi - az.lo + 1);
   if (rows == 0 || ax.hi < ax.lo) return;
-  WM_REQUIRE(rows * L.g.W < (1ll << 31), "expand3d: level too large for 32-bit indexing");
-  int nb = (int)cdiv(rows * L.g.W, 256);
-  hipLaunchKernelGGL((k_expand_rb<S, SC>), dim3(nb), dim3(256), 0, s, phi_of<S>(L).p, L.g,
-                     (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, skip);
+  WM_REQUIRE(2 * L.g.cs < (1ll << 31) && 2 * Lc.g.cs < (1ll << 31),
+             "expand3d: level too large for 32-bit indexing");
+  // elements j = jlo..jlo+nj-1 hold the fine points ix = 2j-1, 2j in range
+  int jlo = (ax.lo + 1) >> 1, nj = ((ax.hi + 1) >> 1) - jlo + 1;
+  WM_REQUIRE(jlo + nj <= L.g.W, "expand3d: loop range outside the padded rows");
+  int nyr = ay.hi - ay.lo + 1, nzr = az.hi - az.lo + 1;
+  int bpc = (int)cdiv((long long)nyr * nj, 256);
+  // 16 planes per thread, 4 when that leaves fewer than 1024 blocks
+  if ((long long)bpc * cdiv(nzr, 16) >= 1024) {
+    hipLaunchKernelGGL((k_expand_rb<S, SC, 16>), dim3(bpc * (int)cdiv(nzr, 16)), dim3(256), 0, s,
+                       phi_of<S>(L).p, L.g, (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, jlo,
+                       nj, bpc, skip);
+  } else {
+    hipLaunchKernelGGL((k_expand_rb<S, SC, 4>), dim3(bpc * (int)cdiv(nzr, 4)), dim3(256), 0, s,
+                       phi_of<S>(L).p, L.g, (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, jlo,
+                       nj, bpc, skip);
+  }
   WM_LAUNCHED();
 }
 
@@ -847,8 +876,8 @@ static void prepare_levels(wm_mg3d *h, const Params &P) {
 }
 
 static int nconv_blocks(const RBGeom &g) {
-  long long n = 2 * g.cs;
-  return (int)std::min<long long>(2048, std::max<long long>(1, cdiv(n, 256 * 8)));
+  long long rows = 2ll * g.PY * g.PZ;
+  return (int)std::min<long long>(std::min(1024, WM_CONV_BLOCKS), std::max<long long>(1, cdiv(rows, 8)));
 }
 
 // The low-precision half of a defect-correction iteration: the FP64
@@ -866,8 +895,13 @@ static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long lo
   WM_LAUNCHED();
   vcycle<S>(h, 0, Pinit, skip, s);
   if (fused) {
-    hipLaunchKernelGGL((k_dc_update<S>), dim3(nconv_blocks(L0.g)), dim3(256), 0, s, L0.phi64.p,
-                       (const S *)phi_of<S>(L0).p, L0.g, h->state.p, mgtol, mgmaxiters);
+    WM_KW_DISPATCH(L0.g.W, hipLaunchKernelGGL((k_dc_update<S, KW>), dim3(nconv_blocks(L0.g)),
+                                              dim3(256), 0, s, L0.phi64.p,
+                                              (const S *)phi_of<S>(L0).p, L0.g, h->state.p,
+                                              h->partials.p));
+    WM_LAUNCHED();
+    hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(256), 0, s, h->partials.p,
+                       nconv_blocks(L0.g), h->state.p, mgtol, mgmaxiters);
   } else {
     hipLaunchKernelGGL((k_dc_add<S>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
                        (const S *)phi_of<S>(L0).p, h->save64.p, np, skip);
@@ -913,14 +947,16 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
     launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, skip, s);
     if (fused) return;
   }
-  hipLaunchKernelGGL((k_converge<double>), dim3(nconv_blocks(L0.g)), dim3(256), 0, s,
-                     L0.phi64.p, h->save64.p, L0.g, h->state.p, mgtol, mgmaxiters, 1);
+  WM_KW_DISPATCH(L0.g.W, hipLaunchKernelGGL((k_converge<double, KW>), dim3(nconv_blocks(L0.g)),
+                                            dim3(256), 0, s, L0.phi64.p, h->save64.p, L0.g,
+                                            h->state.p, h->partials.p, 1));
+  WM_LAUNCHED();
+  hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(256), 0, s, h->partials.p,
+                     nconv_blocks(L0.g), h->state.p, mgtol, mgmaxiters);
   WM_LAUNCHED();
 }
 
 __global__ void k_state_reset(SolveState *st) {
-  st->errbits = 0ull;
-  st->arrived = 0u;
   st->iters = 0;
   st->done = 0;
   st->err = 0.;
@@ -939,7 +975,8 @@ int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz, double dx,
     WM_REQUIRE(out != nullptr, "wm_mg3d_create: out is NULL");
     WM_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "wm_mg3d_create: need nx, ny, nz >= 2");
     WM_REQUIRE(dx > 0 && dy > 0 && dz > 0, "wm_mg3d_create: cell sizes must be positive");
-    WM_REQUIRE((nx + 3) * (ny + 3) * (nz + 3) < (1ll << 31) * 8ll, "wm_mg3d_create: grid too large");
+    WM_REQUIRE((nx + 4) * (ny + 3) * (nz + 3) < (1ll << 31),
+               "wm_mg3d_create: grid too large (the red-black arrays are indexed with 32 bits)");
     for (int i = 0; i < 6; i++)
       WM_REQUIRE(bounds[i] >= 0 && bounds[i] <= 2,
                  "wm_mg3d_create: bounds must be 0 (Dirichlet), 1 (Neumann) or 2 (periodic)");
@@ -993,6 +1030,7 @@ int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz, double dx,
       ldz = dzc;
     }
     h->state.alloc(1);
+    h->partials.alloc(WM_CONV_BLOCKS);
     WM_HIP(hipHostMalloc((void **)&h->state_h, sizeof(SolveState), hipHostMallocDefault));
     WM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     WM_HIP(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));

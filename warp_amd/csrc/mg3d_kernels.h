@@ -701,56 +701,111 @@ __global__ __launch_bounds__(256) void k_expand(
             wx * wy * wz * pc[Cc(jx + 1, jy + 1, jz + 1)];
 }
 
-// expand3d on red-black fine and coarse arrays, one thread per fine
-// element in the loop range (consecutive threads walk a colour row, so the
-// fine loads and stores are contiguous); same sum as k_expand.
-template <typename S, typename SC>
+// A value that is the same on every lane of the wavefront, kept in a scalar
+// register.
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Element of point (x, y, z) (each in -1..n+1) of a red-black array.
+__device__ inline int rb_el(const RBGeom &g, int x, int y, int z) {
+  return ((x + y + z) & 1) * (int)g.cs + ((x + 1) >> 1) + g.W * ((y + 1) + g.PY * (z + 1));
+}
+
+// expand3d on red-black fine and coarse arrays.  Each thread owns element
+// j of one fine row iy in both colour arrays, i.e. the fine points
+// ix = 2j-1 and ix = 2j, and walks them through ZC fine planes; lanes run
+// over the flattened (iy, j) range so that every lane has work.  The eight
+// coarse values a fine point needs (x in {jx, jx+1}, rows jy, jy+1, planes
+// jz, jz+1) stay in registers while consecutive fine planes share a coarse
+// plane; the fine values of the next plane are loaded a plane ahead.  Same
+// sum, term for term, as k_expand.
+template <typename S, typename SC, int ZC>
 __global__ __launch_bounds__(256) void k_expand_rb(
     S *__restrict__ phi, RBGeom g, const SC *__restrict__ pc, RBGeom gc, ExpandAxis ax,
-    ExpandAxis ay, ExpandAxis az, const int *__restrict__ skip) {
+    ExpandAxis ay, ExpandAxis az, int jlo, int nj, int bpc, const int *__restrict__ skip) {
   typedef typename St<S>::T T;
   WM_SKIP();
-  int nyr = ay.hi - ay.lo + 1, nzr = az.hi - az.lo + 1;
-  if (nyr <= 0 || nzr <= 0 || ax.hi < ax.lo) return;
-  // one thread per element (j, row) of the rows in range, both colours;
-  // coarse index and weight from expand3d's tables (f3d_mgrid.F:1833-1844)
-  int t = blockIdx.x * 256 + threadIdx.x;
-  int j = t % g.W;
-  int row = t / g.W;
-  if (row >= 2 * nyr * nzr) return;
-  int c = row / (nyr * nzr);
-  int rr = row - c * (nyr * nzr);
-  int iz = az.lo + rr / nyr;
-  int iy = ay.lo + rr % nyr;
-  int py = iy + 1, pz = iz + 1;
-  int p0 = (c + py + pz + 1) & 1;
-  int ix = p0 + 2 * j - 1;
-  if (ix < ax.lo || ix > ax.hi) return;
-  int jy = ay.j[iy], jz = az.j[iz];
-  T wy = (T)ay.w[iy], wz = (T)az.w[iz];
-  int jx = ax.j[ix];
-  T wx = (T)ax.w[ix];
-  T one = (T)1.;
-  long long fi = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz) + j;
-  // coarse row offsets of (jy+dy, jz+dz); colour of (jx+dx, jy+dy, jz+dz)
-  // is (jx + jy + jz + dx + dy + dz) & 1
-  long long r00 = (long long)gc.W * ((jy + 1) + (long long)gc.PY * (jz + 1));
-  long long r10 = r00 + gc.W, r01 = r00 + (long long)gc.W * gc.PY, r11 = r01 + gc.W;
-  long long e0 = (jx + 1) >> 1, e1 = (jx + 2) >> 1;
-  long long ca = (long long)((jx + jy + jz) & 1) * gc.cs, cb = gc.cs - ca;
-#define PC(x, off, flip) ((T)St<SC>::ld(pc[((flip) ? cb : ca) + (x) + (off)]))
-  T c000 = PC(e0, r00, 0), c100 = PC(e1, r00, 1), c010 = PC(e0, r10, 1), c110 = PC(e1, r10, 0);
-  T c001 = PC(e0, r01, 1), c101 = PC(e1, r01, 0), c011 = PC(e0, r11, 0), c111 = PC(e1, r11, 1);
-#undef PC
-  phi[fi] = St<S>::st(St<S>::ld(phi[fi]) +
-                      (one - wx) * (one - wy) * (one - wz) * c000 +
-                      wx * (one - wy) * (one - wz) * c100 +
-                      (one - wx) * wy * (one - wz) * c010 +
-                      wx * wy * (one - wz) * c110 +
-                      (one - wx) * (one - wy) * wz * c001 +
-                      wx * (one - wy) * wz * c101 +
-                      (one - wx) * wy * wz * c011 +
-                      wx * wy * wz * c111);
+  const int nyr = ay.hi - ay.lo + 1;
+  const int bz = blockIdx.x / bpc;
+  const int t = (blockIdx.x - bz * bpc) * 256 + threadIdx.x;
+  if (t >= nyr * nj) return;
+  const int r = t / nj;
+  const int iy = ay.lo + r;
+  const int j = jlo + (t - r * nj);
+  const int iz0 = az.lo + bz * ZC;
+  const int nzp = min(ZC, az.hi - iz0 + 1);
+  // coarse index and weights from expand3d's tables (f3d_mgrid.F:1833-1844)
+  const int ixA = 2 * j - 1, ixB = 2 * j;
+  const bool okA = ixA >= ax.lo && ixA <= ax.hi, okB = ixB >= ax.lo && ixB <= ax.hi;
+  const int qA = ixA < ax.lo ? ax.lo : (ixA > ax.hi ? ax.hi : ixA);
+  const int qB = ixB < ax.lo ? ax.lo : (ixB > ax.hi ? ax.hi : ixB);
+  const int xA = ax.j[qA], xB = ax.j[qB];
+  const T wxA = (T)ax.w[qA], wxB = (T)ax.w[qB];
+  const int jy = ay.j[iy];
+  const T wy = (T)ay.w[iy];
+  const T one = (T)1.;
+  const int cs = (int)g.cs;
+  const int plane = g.W * g.PY;
+  const int rowoff = g.W * (iy + 1) + j;
+  // coarse plane jzc (a..) and jzc+1 (b..); index 0-1: x = xA, xA+1 and
+  // 2-3: x = xB, xB+1; first digit: row jy or jy+1
+  T a00, a01, a02, a03, a10, a11, a12, a13;
+  T b00, b01, b02, b03, b10, b11, b12, b13;
+#define LDC(x, y, z) ((T)St<SC>::ld(pc[rb_el(gc, (x), (y), (z))]))
+  int jzc = -1000;
+  // the point ix = 2j-1 of plane iz has colour (iy + iz + 1) & 1
+  int cA = (iy + iz0 + 1) & 1;
+  T fA = St<S>::ld(phi[cA * cs + rowoff + plane * (iz0 + 1)]);
+  T fB = St<S>::ld(phi[(cA ^ 1) * cs + rowoff + plane * (iz0 + 1)]);
+  for (int s = 0; s < nzp; s++) {
+    const int iz = iz0 + s;
+    const int jz = az.j[iz];
+    const T wz = (T)az.w[iz];
+    if (jz != jzc) {
+      if (jz == jzc + 1) {
+        a00 = b00; a01 = b01; a02 = b02; a03 = b03;
+        a10 = b10; a11 = b11; a12 = b12; a13 = b13;
+      } else {
+        a00 = LDC(xA, jy, jz); a01 = LDC(xA + 1, jy, jz);
+        a02 = LDC(xB, jy, jz); a03 = LDC(xB + 1, jy, jz);
+        a10 = LDC(xA, jy + 1, jz); a11 = LDC(xA + 1, jy + 1, jz);
+        a12 = LDC(xB, jy + 1, jz); a13 = LDC(xB + 1, jy + 1, jz);
+      }
+      b00 = LDC(xA, jy, jz + 1); b01 = LDC(xA + 1, jy, jz + 1);
+      b02 = LDC(xB, jy, jz + 1); b03 = LDC(xB + 1, jy, jz + 1);
+      b10 = LDC(xA, jy + 1, jz + 1); b11 = LDC(xA + 1, jy + 1, jz + 1);
+      b12 = LDC(xB, jy + 1, jz + 1); b13 = LDC(xB + 1, jy + 1, jz + 1);
+      jzc = jz;
+    }
+    const int fb = rowoff + plane * (iz + 1);
+    const int eA = cA * cs + fb, eB = (cA ^ 1) * cs + fb;
+    const T gA = fA, gB = fB;
+    if (s + 1 < nzp) {
+      fA = St<S>::ld(phi[(cA ^ 1) * cs + fb + plane]);
+      fB = St<S>::ld(phi[cA * cs + fb + plane]);
+    }
+    T w = wxA;
+    const T vA = gA + (one - w) * (one - wy) * (one - wz) * a00 +
+                 w * (one - wy) * (one - wz) * a01 +
+                 (one - w) * wy * (one - wz) * a10 +
+                 w * wy * (one - wz) * a11 +
+                 (one - w) * (one - wy) * wz * b00 +
+                 w * (one - wy) * wz * b01 +
+                 (one - w) * wy * wz * b10 +
+                 w * wy * wz * b11;
+    w = wxB;
+    const T vB = gB + (one - w) * (one - wy) * (one - wz) * a02 +
+                 w * (one - wy) * (one - wz) * a03 +
+                 (one - w) * wy * (one - wz) * a12 +
+                 w * wy * (one - wz) * a13 +
+                 (one - w) * (one - wy) * wz * b02 +
+                 w * (one - wy) * wz * b03 +
+                 (one - w) * wy * wz * b12 +
+                 w * wy * wz * b13;
+    if (okA) phi[eA] = St<S>::st(vA);
+    if (okB) phi[eB] = St<S>::st(vB);
+    cA ^= 1;
+  }
+#undef LDC
 }
 
 // ---------------------------------------------------------------------------
@@ -830,13 +885,13 @@ __global__ __launch_bounds__(256) void k_rho_to_rb(const double *__restrict__ rh
 
 // State of the V-cycle loop on the device.
 struct SolveState {
-  unsigned long long errbits;  // max |dphi| as the bits of a double >= 0
-  unsigned int arrived;        // blocks done in the current reduction
-  int iters;                   // mgiters
-  int done;                    // the loop has ended (also the skip flag)
-  int pad;
-  double err;                  // mgerror of the last V-cycle
+  int iters;   // mgiters
+  int done;    // the loop has ended (also the skip flag)
+  double err;  // mgerror of the last V-cycle
 };
+
+// Most blocks of a convergence-test launch (size of the partials buffer).
+#define WM_CONV_BLOCKS 2048
 
 __device__ inline double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -847,100 +902,161 @@ __device__ inline double wave_max(double v) {
 }
 
 // The convergence test of multigrid3dsolve (f3d_mgrid.F:383-392, 313):
-// mgerror = max over nodes 0..n of |phisave-phi|, then phisave = phi.  The
-// last block to finish closes the reduction, counts the V-cycle and sets
-// `done` when mgerror <= mgtol or mgiters == mgmaxiters.
-// Block max -> global max with a last-block reduction that updates the
-// solve state: mgerror, mgiters, and done when the loop of
-// multigrid3dsolve would stop (f3d_mgrid.F:383-434).
-__device__ inline void converge_finish(double m, double *part, SolveState *st, double mgtol,
-                                       int mgmaxiters) {
+// mgerror = max over nodes 0..n of |phisave-phi|.  Two stages: each block
+// of the test writes its max to partials[blockIdx.x] (plain stores, no
+// atomics or fences), and k_converge_close, the next launch, takes the
+// max of the partials, counts the V-cycle and sets `done` when the loop of
+// multigrid3dsolve would stop (mgerror <= mgtol or mgiters == mgmaxiters,
+// f3d_mgrid.F:383-434).
+__device__ inline void block_max_to(double m, double *part, double *partials) {
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double b = part[0];
     for (int k = 1; k < 4; k++) b = part[k] > b ? part[k] : b;
-    atomicMax(&st->errbits, (unsigned long long)__double_as_longlong(b));
-    __threadfence();
-    unsigned int prev = atomicAdd(&st->arrived, 1u);
-    if (prev == gridDim.x - 1) {
-      __threadfence();
-      double err = __longlong_as_double(
-          (long long)atomicAdd(&st->errbits, 0ull));
-      st->err = err;
-      st->iters = st->iters + 1;
-      st->errbits = 0ull;
-      st->arrived = 0u;
-      if (!(err > mgtol && st->iters < mgmaxiters)) st->done = 1;
-    }
+    partials[blockIdx.x] = b;
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
-                                                  T *__restrict__ save, RBGeom g,
-                                                  SolveState *st, double mgtol,
-                                                  int mgmaxiters, int docopy) {
+__global__ __launch_bounds__(256) void k_converge_close(const double *__restrict__ partials,
+                                                        int n, SolveState *st, double mgtol,
+                                                        int mgmaxiters) {
   if (st->done) return;
   __shared__ double part[4];
-  // one wavefront per padded row of one colour; lanes walk the row
-  int lane = threadIdx.x & 63;
-  int nrows = 2 * g.PY * g.PZ;
   double m = 0.;
-  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += gridDim.x * 4) {
-    int c = row / (g.PY * g.PZ);
-    int rr = row - c * (g.PY * g.PZ);
-    int pz = rr / g.PY, py = rr - pz * g.PY;
-    int p0 = (c + py + pz + 1) & 1;
-    bool rowin = py >= 1 && py <= g.ny + 1 && pz >= 1 && pz <= g.nz + 1;
-    long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
-    for (int j = lane; j < g.W; j += 64) {
-      T cur = phi[base + j];
-      int ix = p0 + 2 * j - 1;
-      if (rowin && ix >= 0 && ix <= g.nx) {
-        double d = fabs((double)save[base + j] - (double)cur);
-        m = d > m ? d : m;
+  for (int i = threadIdx.x; i < n; i += 256) m = partials[i] > m ? partials[i] : m;
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double err = part[0];
+    for (int k = 1; k < 4; k++) err = part[k] > err ? part[k] : err;
+    st->err = err;
+    st->iters = st->iters + 1;
+    if (!(err > mgtol && st->iters < mgmaxiters)) st->done = 1;
+  }
+}
+
+// Row geometry shared by the row-walking reductions: padded row `row` of
+// colour c starts at element base; its nodes 0..n are the elements whose
+// ix = p0 + 2j - 1 lies in [0, nx] when the row itself is in range.
+struct RowInfo {
+  int base, p0;
+  bool in;
+};
+
+__device__ inline RowInfo row_info(const RBGeom &g, int row) {
+  int plane = g.PY * g.PZ;
+  int c = row / plane;
+  int rr = row - c * plane;
+  int pz = rr / g.PY, py = rr - pz * g.PY;
+  RowInfo r;
+  r.p0 = (c + py + pz + 1) & 1;
+  r.in = py >= 1 && py <= g.ny + 1 && pz >= 1 && pz <= g.nz + 1;
+  r.base = c * (int)g.cs + g.W * (py + g.PY * pz);
+  return r;
+}
+
+// Convergence test.  One wavefront per padded row of one colour, two rows
+// in flight per wavefront, KW 64-element groups per row loaded before use.
+template <typename T, int KW>
+__global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
+                                                  T *__restrict__ save, RBGeom g,
+                                                  const SolveState *st,
+                                                  double *__restrict__ partials, int docopy) {
+  if (st->done) return;
+  __shared__ double part[4];
+  const int lane = threadIdx.x & 63;
+  const int nrows = 2 * g.PY * g.PZ;
+  const int stride = gridDim.x * 4;
+  double m = 0.;
+  for (int row = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6)); row < nrows;
+       row += 2 * stride) {
+    RowInfo ri[2];
+    bool live[2];
+    T cur[2][KW], sv[2][KW];
+    for (int j0 = 0; j0 < g.W; j0 += 64 * KW) {
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        live[r] = row + r * stride < nrows;
+        ri[r] = row_info(g, live[r] ? row + r * stride : row);
+#pragma unroll
+        for (int k = 0; k < KW; k++) {
+          int j = j0 + lane + 64 * k;
+          int e = ri[r].base + (j < g.W ? j : g.W - 1);
+          cur[r][k] = phi[e];
+          sv[r][k] = save[e];
+        }
       }
-      if (docopy) save[base + j] = cur;
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+#pragma unroll
+        for (int k = 0; k < KW; k++) {
+          int j = j0 + lane + 64 * k;
+          int ix = ri[r].p0 + 2 * j - 1;
+          if (live[r] && ri[r].in && ix >= 0 && ix <= g.nx) {
+            double d = fabs((double)sv[r][k] - (double)cur[r][k]);
+            m = d > m ? d : m;
+          }
+          if (docopy && live[r] && j < g.W) save[ri[r].base + j] = cur[r][k];
+        }
+      }
     }
   }
-  converge_finish(m, part, st, mgtol, mgmaxiters);
+  block_max_to(m, part, partials);
 }
 
 // Defect-correction update fused with the convergence test: phi += e over
 // every element (guards included) and max |phi_new - phi_old| over the
 // nodes 0..n, reduced as in k_converge.  phi is both the saved phi of the
 // iteration and the result, so no copy is kept.
-template <typename TL>
+template <typename TL, int KW>
 __global__ __launch_bounds__(256) void k_dc_update(double *__restrict__ phi,
                                                    const TL *__restrict__ e, RBGeom g,
-                                                   SolveState *st, double mgtol,
-                                                   int mgmaxiters) {
+                                                   const SolveState *st,
+                                                   double *__restrict__ partials) {
   if (st->done) return;
   __shared__ double part[4];
-  int lane = threadIdx.x & 63;
-  int nrows = 2 * g.PY * g.PZ;
+  const int lane = threadIdx.x & 63;
+  const int nrows = 2 * g.PY * g.PZ;
+  const int stride = gridDim.x * 4;
   double m = 0.;
-  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += gridDim.x * 4) {
-    int c = row / (g.PY * g.PZ);
-    int rr = row - c * (g.PY * g.PZ);
-    int pz = rr / g.PY, py = rr - pz * g.PY;
-    int p0 = (c + py + pz + 1) & 1;
-    bool rowin = py >= 1 && py <= g.ny + 1 && pz >= 1 && pz <= g.nz + 1;
-    long long base = (long long)c * g.cs + (long long)g.W * (py + (long long)g.PY * pz);
-    for (int j = lane; j < g.W; j += 64) {
-      double old = phi[base + j];
-      double nw = (double)St<TL>::ld(e[base + j]) + old;
-      int ix = p0 + 2 * j - 1;
-      if (rowin && ix >= 0 && ix <= g.nx) {
-        double d = fabs(old - nw);
-        m = d > m ? d : m;
+  for (int row = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6)); row < nrows;
+       row += 2 * stride) {
+    RowInfo ri[2];
+    bool live[2];
+    double old[2][KW], nw[2][KW];
+    for (int j0 = 0; j0 < g.W; j0 += 64 * KW) {
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        live[r] = row + r * stride < nrows;
+        ri[r] = row_info(g, live[r] ? row + r * stride : row);
+#pragma unroll
+        for (int k = 0; k < KW; k++) {
+          int j = j0 + lane + 64 * k;
+          int q = ri[r].base + (j < g.W ? j : g.W - 1);
+          old[r][k] = phi[q];
+          nw[r][k] = (double)St<TL>::ld(e[q]);
+        }
       }
-      phi[base + j] = nw;
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+#pragma unroll
+        for (int k = 0; k < KW; k++) {
+          int j = j0 + lane + 64 * k;
+          int ix = ri[r].p0 + 2 * j - 1;
+          double v = nw[r][k] + old[r][k];
+          if (live[r] && ri[r].in && ix >= 0 && ix <= g.nx) {
+            double d = fabs(old[r][k] - v);
+            m = d > m ? d : m;
+          }
+          if (live[r] && j < g.W) phi[ri[r].base + j] = v;
+        }
+      }
     }
   }
-  converge_finish(m, part, st, mgtol, mgmaxiters);
+  block_max_to(m, part, partials);
 }
 
 // Defect-correction update: phi64 = e + phisave over the whole array
