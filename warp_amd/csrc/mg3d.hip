@@ -163,8 +163,11 @@ struct SubDev {
   DevBuf<uint32_t> modes, zero;
   DevBuf<T> k, denom, fac;
   int n = 0;
+  bool anyzero = false;  // some record zeroes a neighbour's residual
   void upload(const SubHost &h) {
     n = h.n();
+    anyzero = false;
+    for (uint32_t z : h.zero) anyzero = anyzero || z != 0;
     pos.upload(h.pos);
     modes.upload(h.modes);
     zero.upload(h.zero);
@@ -225,6 +228,7 @@ struct Level {
   DevBuf<int> ej[3];
   DevBuf<double> ew[3];
   int elo[3], ehi[3];
+  bool rstd[3] = {false, false, false};  // restriction windows are the plain 2:1 ones
   // conductors
   int nint = 0;
   DevBuf<int> int_pos;
@@ -342,6 +346,13 @@ static void restrict_axis(int n, int nc, int lbm, int lbp, Level &L, int a) {
       w[5 * ic + (i - imin)] = 1. - std::fabs(ic - (i + 0.) * di);
     }
   }
+  // the plain 2:1 windows [2ic-1, 2ic+1] with weights (1/2, 1, 1/2) over
+  // [1, nc-1] (Dirichlet ends)
+  bool std21 = n == 2 * nc && lbm == 0 && lbp == 0 && nc >= 2;
+  for (int ic = cmin; ic <= cmax && std21; ic++)
+    std21 = lo[ic] == 2 * ic - 1 && hi[ic] == 2 * ic + 1 && w[5 * ic] == 0.5 &&
+            w[5 * ic + 1] == 1. && w[5 * ic + 2] == 0.5;
+  L.rstd[a] = std21;
   L.rlo[a].upload(lo);
   L.rhi[a].upload(hi);
   L.rw[a].upload(w);
@@ -759,9 +770,20 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
     CondDev<T> &C = cond_of<T>(L);
     T dxsqics = (T)(P.mgparam * L.dxsqi * cst), dysqics = (T)(P.mgparam * L.dysqi * cst),
       dzsqics = (T)(P.mgparam * L.dzsqi * cst);
+    // interior points get 0 before any record is evaluated (cond_potmgres
+    // then condbndyres): in one launch unless a record sits on an interior
+    // point
+    bool split = L.sub_on_interior && L.nint > 0;
+    if (split) {
+      SubList<T> none{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+      hipLaunchKernelGGL((k_residual_cond<S, TR, T, RBIdx>), dim3(blocks(L.nint)), dim3(256), 0,
+                         s, phi, rho, res, RBIdx{L.g}, L.g, L.nint, L.int_pos.p, none, dxsqics,
+                         dysqics, dzsqics, (T)P.mgparam, skip);
+      WM_LAUNCHED();
+    }
     for (int p = 0; p < 2; p++) {
       SubList<T> sub = C.res[mgform2init][p].view();
-      int nint = p == 0 ? L.nint : 0;
+      int nint = p == 0 && !split ? L.nint : 0;
       long long cnt = (long long)nint + sub.n;
       if (cnt == 0) continue;
       hipLaunchKernelGGL((k_residual_cond<S, TR, T, RBIdx>), dim3(blocks(cnt)), dim3(256), 0, s,
@@ -840,6 +862,96 @@ static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
   WM_LAUNCHED();
 }
 
+// residual3d + restrict3d as one kernel (k_resrestrict) where that is
+// exact: six Dirichlet boundaries, plain 2:1 restriction windows on every
+// axis, and no conductor record that zeroes a neighbour's residual.  The
+// conductor points' residuals are written first by k_residual_cond.
+static bool fused_restrict_ok(wm_mg3d *h, const Level &L, bool anyzero) {
+  static const bool off = getenv("WARPMI_NO_FUSED_RESTRICT") != nullptr;
+  if (off || anyzero) return false;
+  for (int i = 0; i < 6; i++)
+    if (h->bounds[i] != 0) return false;
+  return L.rstd[0] && L.rstd[1] && L.rstd[2];
+}
+
+template <typename S, typename TR, typename TC>
+static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params &P,
+                        const int *skip, hipStream_t s) {
+  typedef typename St<S>::T T;
+  typedef typename RestrictArith<TR, TC>::type RA;
+  const S *phi = phi_of<S>(L).p;
+  const S *rho = rho_of<S>(L).p;
+  TR *res = res_of<TR>(L).p;
+  double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+  T dxsqic = (T)(L.dxsqi * P.mgparam * cst), dysqic = (T)(L.dysqi * P.mgparam * cst),
+    dzsqic = (T)(L.dzsqi * P.mgparam * cst);
+  if (L.hasmask) {
+    // the interior points' zeros are written once per solve
+    // (zero_interior_res); here the records, even list then odd
+    CondDev<T> &C = cond_of<T>(L);
+    for (int p = 0; p < 2; p++) {
+      SubList<T> sub = C.res[0][p].view();
+      if (sub.n == 0) continue;
+      hipLaunchKernelGGL((k_residual_cond<S, TR, T, RBIdx>), dim3(blocks(sub.n)), dim3(256), 0, s,
+                         phi, rho, res, RBIdx{L.g}, L.g, 0, L.int_pos.p, sub, dxsqic, dysqic,
+                         dzsqic, (T)P.mgparam, skip);
+      WM_LAUNCHED();
+    }
+  }
+  const int ncx = Lc.nx - 1, ncy = Lc.ny - 1, ncz = Lc.nz - 1;
+  // tiles: up to 127 coarse columns and `maxpairs` fine pairs (one per
+  // thread) per block, sizes balanced over the range; z chunks of up to 16
+  // coarse planes, fewer when that leaves under 1024 blocks
+  static const int mp_env = getenv("WARPMI_RESTRICT_PAIRS") ? atoi(getenv("WARPMI_RESTRICT_PAIRS")) : 0;
+  const int maxpairs = mp_env >= 64 && mp_env <= 1024 ? mp_env : 640;
+  int ntx = (int)cdiv(ncx, 127);
+  int cxt = (int)cdiv(ncx, ntx);
+  int cytmax = std::max(1, (maxpairs / (cxt + 1) - 1) / 2);
+  int nty = (int)cdiv(ncy, cytmax);
+  int cyt = (int)cdiv(ncy, nty);
+  long long cols = (long long)ntx * nty;
+  int zc = (int)std::max<long long>(1, std::min<long long>(16, cols * ncz / 1024));
+  static const int zc_env = getenv("WARPMI_RESTRICT_ZC") ? atoi(getenv("WARPMI_RESTRICT_ZC")) : 0;
+  if (zc_env > 0) zc = zc_env;
+  int ntz = (int)cdiv(ncz, zc);
+  long long nb = cols * ntz;
+  int npairs = (cxt + 1) * (2 * cyt + 1);
+  int threads = (int)(64 * cdiv(npairs, 64));
+  size_t shm = 2 * sizeof(RA) * 2 * (cxt + 1) * (2 * cyt + 1);
+  WM_REQUIRE(threads <= 1024 && nb < (1ll << 31) && shm <= 65536,
+             "restrict3d: tile does not fit the fused kernel");
+  hipLaunchKernelGGL((k_resrestrict<S, TR, TC>), dim3((int)nb), dim3(threads), shm, s, phi, rho,
+                     (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic, dzsqic,
+                     (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, cxt, cyt, zc, ntx, nty, (int)nb,
+                     L.ff, skip);
+  WM_LAUNCHED();
+}
+
+// Zeros of the interior conductor points in the residual arrays of the
+// levels that use k_resrestrict (cond_potmgres, f3d_mgrid.F:2474-2506).  On
+// those levels nothing else writes these points of res, so once per solve
+// is enough; the records written after them in every V-cycle still win
+// where one sits on an interior point, as in the reference.
+static Params P2of(Params P) {
+  P.mgform = 2;
+  return P;
+}
+
+template <typename T>
+static void zero_interior_res(wm_mg3d *h, const Params &P, hipStream_t s) {
+  for (size_t l = 0; l + 1 < h->lv.size(); l++) {
+    Level &L = *h->lv[l];
+    if (!L.coarser || !L.hasmask || L.nint == 0) continue;
+    CondDev<T> &C = cond_of<T>(L);
+    if (!fused_restrict_ok(h, L, C.res[0][0].anyzero || C.res[0][1].anyzero)) continue;
+    SubList<T> none{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL((k_residual_cond<T, T, T, RBIdx>), dim3(blocks(L.nint)), dim3(256), 0, s,
+                       (const T *)nullptr, (const T *)nullptr, res_of<T>(L).p, RBIdx{L.g}, L.g,
+                       L.nint, L.int_pos.p, none, (T)0., (T)0., (T)0., (T)P.mgparam, nullptr);
+    WM_LAUNCHED();
+  }
+}
+
 // vcycle (f3d_mgrid.F:1101-1591) on level data of storage type S (the
 // coarser levels use Coarse<S>).
 template <typename S>
@@ -850,10 +962,19 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   for (int i = 0; i < P.down; i++) sorpass<S>(h, L, level, P, skip, s);
   if (L.coarser) {
     Level &Lc = *h->lv[level + 1];
-    residual_rb<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
-                      s);
-    restrict_to<T, SC>(L, (const T *)res_of<T>(L).p, RBIdx{L.g}, rho_of<SC>(Lc).p, RBIdx{Lc.g},
-                       skip, s, Lc.nx, Lc.ny, Lc.nz);
+    bool anyzero = false;
+    if (L.hasmask) {
+      ensure_records<T>(h, L, level, P, true, 0);
+      anyzero = cond_of<T>(L).res[0][0].anyzero || cond_of<T>(L).res[0][1].anyzero;
+    }
+    if (fused_restrict_ok(h, L, anyzero)) {
+      resrestrict<S, T, SC>(h, L, Lc, level, P, skip, s);
+    } else {
+      residual_rb<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
+                        s);
+      restrict_to<T, SC>(L, (const T *)res_of<T>(L).p, RBIdx{L.g}, rho_of<SC>(Lc).p,
+                         RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny, Lc.nz);
+    }
     hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
                        phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
     WM_LAUNCHED();
@@ -1202,6 +1323,10 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     WM_LAUNCHED();
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, s, h->state.p);
     WM_LAUNCHED();
+    if (prec == WM_PREC_FP64)
+      zero_interior_res<double>(h, dc ? P2of(P) : P, s);
+    else
+      zero_interior_res<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd}, s);
     int iters = 0;
     double err = 2. * mgtol + 1.;
     static const bool eager_env = getenv("WARPMI_EAGER") != nullptr;

@@ -42,6 +42,11 @@ __host__ __device__ inline long long nat_index(const NatGeom &g, int ix, int iy,
 }
 
 // Accessors so that the same kernel body serves both layouts.
+// Element of point (x, y, z) (each in -1..n+1) of a red-black array.
+__device__ inline int rb_el(const RBGeom &g, int x, int y, int z) {
+  return ((x + y + z) & 1) * (int)g.cs + ((x + 1) >> 1) + g.W * ((y + 1) + g.PY * (z + 1));
+}
+
 struct RBIdx {
   RBGeom g;
   __device__ long long operator()(int ix, int iy, int iz) const {
@@ -662,6 +667,176 @@ __global__ __launch_bounds__(256) void k_restrict_t(
   out[oi] = w > (T)0. ? (r / w) * ff : (T)0.;
 }
 
+// Arithmetic of restrict3d: double for double residual and coarse source,
+// else the coarse type (as k_restrict_t).
+template <typename TR, typename TC> struct RestrictArith { typedef TC type; };
+template <> struct RestrictArith<double, double> { typedef double type; };
+
+// ---------------------------------------------------------------------------
+// residual3d + restrict3d in one pass, for a level whose six boundaries are
+// Dirichlet and whose restriction windows are the plain 2:1 windows
+// [2j-1, 2j+1] over the coarse range [1, nc-1] in every axis.  Then every
+// fine point a window reads lies in residual3d's loop range [1, n-1], the
+// tent weights are (1/2, 1, 1/2) in each axis and the weight sum of every
+// window is exactly 8 (all partial sums are multiples of 1/8), so the fine
+// residual is computed where it is used and never stored.  Conductor points
+// (mask) take their residual from res, where k_residual_cond has written
+// it.
+//
+// One block per tile of coarse points (cxt x cyt) over a chunk of zc coarse
+// planes.  For each fine plane, phase 1 computes the residuals of the
+// tile's fine window into LDS: a thread owns a pair of fine points
+// (ix = 2j-1, 2j: element j of both colour arrays) of one fine row, walks
+// it through the planes with its z-neighbours in registers, and issues
+// every load one plane ahead.  Phase 2 adds the plane's 3x3 terms to each
+// coarse point's running sum in restrict3d's order (planes 2jz-1, 2jz,
+// 2jz+1; within a plane rows, then columns), so the sums and the result are
+// those of k_residual_zmarch + k_restrict, bit for bit.
+// ---------------------------------------------------------------------------
+template <typename S, typename TR, typename TC>
+__global__ __launch_bounds__(1024) void k_resrestrict(
+    const S *__restrict__ phi, const S *__restrict__ rho, const TR *__restrict__ res,
+    TC *__restrict__ out, RBGeom g, RBGeom gc, typename St<S>::T dxsqic,
+    typename St<S>::T dysqic, typename St<S>::T dzsqic, typename St<S>::T mgparam,
+    const uint32_t *__restrict__ mask, int cxt, int cyt, int zc, int ntx, int nty, int nblocks,
+    double ff, const int *__restrict__ skip) {
+  typedef typename St<S>::T T;
+  typedef typename RestrictArith<TR, TC>::type RA;
+  extern __shared__ __align__(16) unsigned char wm_smem[];
+  RA *lres = reinterpret_cast<RA *>(wm_smem);
+  WM_SKIP();
+  const int ncx = gc.nx - 1, ncy = gc.ny - 1, ncz = gc.nz - 1;  // coarse range [1, nc-1]
+  const int lb = xcd_remap(blockIdx.x, nblocks);
+  const int tx = lb % ntx, tr = lb / ntx, ty = tr % nty, tz = tr / nty;
+  const int cx0 = 1 + tx * cxt, cy0 = 1 + ty * cyt, cz0 = 1 + tz * zc;
+  const int mx = min(cxt, ncx - cx0 + 1), my = min(cyt, ncy - cy0 + 1);
+  const int cz1 = min(cz0 + zc - 1, ncz);
+  const int npx = mx + 1, nfy = 2 * my + 1;  // pairs j = cx0..cx0+mx, rows 2cy0-1..2cy1+1
+  const int nfx = 2 * (cxt + 1);             // LDS row stride
+  const int lsz = nfx * (2 * cyt + 1);       // LDS values per buffer
+  const int cs = (int)g.cs, W = g.W, pl = g.W * g.PY;
+  const int izf = 2 * cz0 - 1, izl = 2 * cz1 + 1;  // fine planes of the chunk
+
+  // phase 1: the owned pair column
+  const int pc = threadIdx.x;
+  const bool own = pc < npx * nfy;
+  const int pcc = own ? pc : 0;
+  const int pcy = pcc / npx, pcx = pcc - pcy * npx;
+  const int y = 2 * cy0 - 1 + pcy;
+  const int rowoff = W * (y + 1) + cx0 + pcx;
+  const int lidx = pcy * nfx + 2 * pcx;
+  // loads of plane iz: the centre pair of plane iz+1 into (ca, cb), the rest
+  // of plane iz into the named values
+  auto elA = [&](int iz) { return ((y + iz + 1) & 1) * cs + rowoff + pl * (iz + 1); };
+  auto elB = [&](int iz) { return (((y + iz + 1) & 1) ^ 1) * cs + rowoff + pl * (iz + 1); };
+  auto load = [&](int iz, T &ca, T &cb, T &ya_m, T &ya_p, T &yb_m, T &yb_p, T &x_a, T &x_b,
+                  T &r_a, T &r_b) {
+    const int eA = elA(iz), eB = elB(iz);
+    ca = St<S>::ld(phi[eB + pl]);
+    cb = St<S>::ld(phi[eA + pl]);
+    ya_m = St<S>::ld(phi[eB - W]);
+    ya_p = St<S>::ld(phi[eB + W]);
+    yb_m = St<S>::ld(phi[eA - W]);
+    yb_p = St<S>::ld(phi[eA + W]);
+    x_a = St<S>::ld(phi[eB - 1]);
+    x_b = St<S>::ld(phi[eA + 1]);
+    r_a = St<S>::ld(rho[eA]);
+    r_b = St<S>::ld(rho[eB]);
+  };
+  // conductor points: mask words two planes ahead, their residuals (from
+  // res, only where the bit is set) one plane ahead
+  auto mbit = [&](uint32_t w, int e) { return (w >> (e & 31)) & 1u; };
+  auto loadmask = [&](int iz, uint32_t &m_a, uint32_t &m_b) {
+    m_a = mask ? mask[elA(iz) >> 5] : 0u;
+    m_b = mask ? mask[elB(iz) >> 5] : 0u;
+  };
+  auto loadres = [&](int iz, uint32_t m_a, uint32_t m_b, RA &v_a, RA &v_b) {
+    const int eA = elA(iz), eB = elB(iz);
+    v_a = mbit(m_a, eA) ? (RA)St<TR>::ld(res[eA]) : (RA)0.;
+    v_b = mbit(m_b, eB) ? (RA)St<TR>::ld(res[eB]) : (RA)0.;
+  };
+  T am = St<S>::ld(phi[elB(izf) - pl]);
+  T bm = St<S>::ld(phi[elA(izf) - pl]);
+  T a0 = St<S>::ld(phi[elA(izf)]);
+  T b0 = St<S>::ld(phi[elB(izf)]);
+  T ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb;
+  uint32_t wa, wb, wa1 = 0, wb1 = 0;
+  RA sa, sb;
+  load(izf, ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb);
+  loadmask(izf, wa, wb);
+  if (izf < izl) loadmask(izf + 1, wa1, wb1);
+  loadres(izf, wa, wb, sa, sb);
+  // phase 2: the owned coarse point and its running sum
+  const int q = threadIdx.x;
+  const bool cown = q < mx * my;
+  const int qq = cown ? q : 0;
+  const int cyl = qq / mx, cxl = qq - cyl * mx;
+  const int lq = 2 * cyl * nfx + 2 * cxl;
+  RA rs = (RA)0.;
+  const RA half = (RA)0.5, one = (RA)1.;
+  for (int iz = izf; iz <= izl; iz++) {
+    RA *buf = lres + (iz & 1) * lsz;
+    T nap = 0, nbp = 0, nyam = 0, nyap = 0, nybm = 0, nybp = 0, nxa = 0, nxb = 0, nra = 0, nrb = 0;
+    uint32_t wa2 = 0, wb2 = 0;
+    RA nsa = (RA)0., nsb = (RA)0.;
+    if (iz < izl) {
+      load(iz + 1, nap, nbp, nyam, nyap, nybm, nybp, nxa, nxb, nra, nrb);
+      loadres(iz + 1, wa1, wb1, nsa, nsb);
+      if (iz + 1 < izl) loadmask(iz + 2, wa2, wb2);
+    }
+    if (own) {
+      RA vA, vB;
+      if (mbit(wa, elA(iz))) {
+        vA = sa;
+      } else {
+        T v = ra + (xa + b0) * dxsqic + (yam + yap) * dysqic + (am + ap) * dzsqic - a0 * mgparam;
+        vA = (RA)(TR)v;
+      }
+      if (mbit(wb, elB(iz))) {
+        vB = sb;
+      } else {
+        T v = rb + (a0 + xb) * dxsqic + (ybm + ybp) * dysqic + (bm + bp) * dzsqic - b0 * mgparam;
+        vB = (RA)(TR)v;
+      }
+      buf[lidx] = vA;
+      buf[lidx + 1] = vB;
+    }
+    // march: planes iz, iz+1 become iz-1, iz
+    am = a0; bm = b0; a0 = ap; b0 = bp; ap = nap; bp = nbp;
+    yam = nyam; yap = nyap; ybm = nybm; ybp = nybp;
+    xa = nxa; xb = nxb; ra = nra; rb = nrb;
+    wa = wa1; wb = wb1; wa1 = wa2; wb1 = wb2; sa = nsa; sb = nsb;
+    __syncthreads();
+    if (cown) {
+      // phase 2: the plane's terms; weight of window position d (0..2) is
+      // 1/2, 1, 1/2 and ww = wx * wy * wz as restrict3d forms it
+      RA v[9];
+#pragma unroll
+      for (int d = 0; d < 9; d++) v[d] = buf[lq + (d / 3) * nfx + d % 3];
+      auto add = [&](RA wz) {
+#pragma unroll
+        for (int d = 0; d < 9; d++) {
+          RA ww = (d % 3 == 1 ? one : half) * (d / 3 == 1 ? one : half) * wz;
+          rs = rs + ww * v[d];
+        }
+      };
+      if (iz & 1) {
+        const int jz = (iz + 1) >> 1;  // iz = 2(jz-1)+1 = 2jz-1
+        if (jz - 1 >= cz0) {          // last plane of coarse plane jz-1
+          add(half);
+          out[rb_el(gc, cx0 + cxl, cy0 + cyl, jz - 1)] = (TC)((rs / (RA)8.) * (RA)ff);
+        }
+        if (jz <= cz1) {  // first plane of coarse plane jz
+          rs = (RA)0.;
+          add(half);
+        }
+      } else {
+        add(one);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // expand3d (f3d_mgrid.F:1777-1947): trilinear prolongation added to the
 // fine phi, one thread per fine point in the loop range.
@@ -704,11 +879,6 @@ __global__ __launch_bounds__(256) void k_expand(
 // A value that is the same on every lane of the wavefront, kept in a scalar
 // register.
 __device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Element of point (x, y, z) (each in -1..n+1) of a red-black array.
-__device__ inline int rb_el(const RBGeom &g, int x, int y, int z) {
-  return ((x + y + z) & 1) * (int)g.cs + ((x + 1) >> 1) + g.W * ((y + 1) + g.PY * (z + 1));
-}
 
 // expand3d on red-black fine and coarse arrays.  Each thread owns element
 // j of one fine row iy in both colour arrays, i.e. the fine points
