@@ -482,7 +482,24 @@ static SubHost build_records(const Level &L, int level, int p, const int *lb,
     h.denom.push_back(denom);
     h.fac.push_back(mgform2init ? 1. : ppp);
   }
-  return h;
+  // records in red-black element order, so that neighbouring threads of the
+  // record kernels touch neighbouring memory
+  int nr = h.n();
+  std::vector<std::pair<long long, int>> ord(nr);
+  for (int i = 0; i < nr; i++)
+    ord[i] = {rb_index(L.g, h.pos[3 * i], h.pos[3 * i + 1], h.pos[3 * i + 2]), i};
+  std::stable_sort(ord.begin(), ord.end());
+  SubHost o;
+  for (auto &q : ord) {
+    int i = q.second;
+    o.pos.insert(o.pos.end(), {h.pos[3 * i], h.pos[3 * i + 1], h.pos[3 * i + 2]});
+    o.modes.push_back(h.modes[i]);
+    o.zero.push_back(h.zero[i]);
+    for (int k = 0; k < 6; k++) o.k.push_back(h.k[6 * i + k]);
+    o.denom.push_back(h.denom[i]);
+    o.fac.push_back(h.fac[i]);
+  }
+  return o;
 }
 
 template <typename T>
@@ -668,7 +685,7 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   int rows = rows_env > 0 && rows_env <= 16 && !tiled_env0 ? rows_env : 4;
   int nthr = 64 * rows;
   int nby = (int)cdiv(nyr, rows);
-  int nbsub = (int)cdiv(sub.n, nthr);
+  int nbsub = (int)(8 * cdiv(cdiv(sub.n, nthr), 8));
   static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
   const bool tiled = tiled_env && !std::is_same<S, bf16>::value;
   // z-marching sweep: colour elements j0..j1 cover ix in [ixmin, ixmax]
@@ -702,7 +719,7 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
     hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbdense + nbsub), dim3(nthr), 0, s, phi, rho,
                        L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
                        dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0, nbx, nby,
-                       zc, nbdense, skip);
+                       zc, nbdense, nbsub, skip);
   WM_LAUNCHED();
   if (timed) {
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
@@ -887,14 +904,13 @@ static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params
     dzsqic = (T)(L.dzsqi * P.mgparam * cst);
   if (L.hasmask) {
     // the interior points' zeros are written once per solve
-    // (zero_interior_res); here the records, even list then odd
+    // (zero_interior_res); here the records of both colours
     CondDev<T> &C = cond_of<T>(L);
-    for (int p = 0; p < 2; p++) {
-      SubList<T> sub = C.res[0][p].view();
-      if (sub.n == 0) continue;
-      hipLaunchKernelGGL((k_residual_cond<S, TR, T, RBIdx>), dim3(blocks(sub.n)), dim3(256), 0, s,
-                         phi, rho, res, RBIdx{L.g}, L.g, 0, L.int_pos.p, sub, dxsqic, dysqic,
-                         dzsqic, (T)P.mgparam, skip);
+    SubList<T> s0 = C.res[0][0].view(), s1 = C.res[0][1].view();
+    if (s0.n + s1.n > 0) {
+      hipLaunchKernelGGL((k_residual_rec2<S, TR, T, RBIdx>), dim3(blocks(s0.n + s1.n)), dim3(256),
+                         0, s, phi, rho, res, RBIdx{L.g}, L.g, s0, s1, dxsqic, dysqic, dzsqic,
+                         (T)P.mgparam, skip);
       WM_LAUNCHED();
     }
   }

@@ -206,10 +206,13 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     S *__restrict__ phi, const S *__restrict__ rho, RBGeom g, int c, int ixmin,
     int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
     T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
-    int j0, int nbx, int nby, int zc, int nbdense, const int *__restrict__ skip) {
+    int j0, int nbx, int nby, int zc, int nbdense, int nbsub, const int *__restrict__ skip) {
   WM_SKIP();
-  int b = blockIdx.x;
-  if (b < nbdense) {
+  // blocks [0, nbsub): the subgrid records, scheduled first so that they
+  // overlap the dense sweep; nbsub is a multiple of 8, so the dense block
+  // b runs on XCD b % 8 as xcd_remap assumes
+  int b = (int)blockIdx.x - nbsub;
+  if (b >= 0) {
     int lb = xcd_remap(b, nbdense);
     int bx = lb % nbx;
     int t = lb / nbx;
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     }
     return;
   }
-  int i = (b - nbdense) * (int)blockDim.x + threadIdx.x;
+  int i = (int)blockIdx.x * (int)blockDim.x + threadIdx.x;
   if (i >= sub.n) return;
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
@@ -468,19 +471,13 @@ __global__ __launch_bounds__(256) void k_res_shell(TR *__restrict__ res, RBGeom 
 // surfaces the reference zeroes some neighbours' residuals while walking
 // the even list then the odd list; launching one parity per call (even
 // first) reproduces that order.
-template <typename S, typename TR, typename T = typename St<S>::T, typename RIdx = NatIdx>
-__global__ __launch_bounds__(256) void k_residual_cond(
-    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RIdx RI,
-    RBGeom g, int nint, const int *__restrict__ ipos, SubList<T> sub,
-    T dxsqics, T dysqics, T dzsqics, T mgparam, const int *__restrict__ skip) {
-  WM_SKIP();
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < nint) {
-    res[RI(ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2])] = St<TR>::st((typename St<TR>::T)0.);
-    return;
-  }
-  i -= nint;
-  if (i >= sub.n) return;
+// One conductor record's residual (condbndyres) and the neighbour zeros
+// of its Neumann sides.
+template <typename S, typename TR, typename T, typename RIdx>
+__device__ inline void residual_record(const S *__restrict__ phi, const S *__restrict__ rho,
+                                       TR *__restrict__ res, RIdx RI, const RBGeom &g,
+                                       const SubList<T> &sub, int i, T dxsqics, T dysqics,
+                                       T dzsqics, T mgparam) {
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
   const T *k = sub.k + 6 * i;
@@ -503,6 +500,40 @@ __global__ __launch_bounds__(256) void k_residual_cond(
     if (z & 16u) res[RI(ix, iy, iz - 1)] = St<TR>::st((typename St<TR>::T)0.);
     if (z & 32u) res[RI(ix, iy, iz + 1)] = St<TR>::st((typename St<TR>::T)0.);
   }
+}
+
+template <typename S, typename TR, typename T = typename St<S>::T, typename RIdx = NatIdx>
+__global__ __launch_bounds__(256) void k_residual_cond(
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RIdx RI,
+    RBGeom g, int nint, const int *__restrict__ ipos, SubList<T> sub,
+    T dxsqics, T dysqics, T dzsqics, T mgparam, const int *__restrict__ skip) {
+  WM_SKIP();
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nint) {
+    res[RI(ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2])] = St<TR>::st((typename St<TR>::T)0.);
+    return;
+  }
+  i -= nint;
+  if (i >= sub.n) return;
+  residual_record<S, TR, T, RIdx>(phi, rho, res, RI, g, sub, i, dxsqics, dysqics, dzsqics,
+                                  mgparam);
+}
+
+// Both colours' records in one launch, for record lists that zero no
+// neighbour's residual (then the order of the lists does not matter).
+template <typename S, typename TR, typename T = typename St<S>::T, typename RIdx = NatIdx>
+__global__ __launch_bounds__(256) void k_residual_rec2(
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RIdx RI,
+    RBGeom g, SubList<T> sub0, SubList<T> sub1, T dxsqics, T dysqics, T dzsqics, T mgparam,
+    const int *__restrict__ skip) {
+  WM_SKIP();
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < sub0.n)
+    residual_record<S, TR, T, RIdx>(phi, rho, res, RI, g, sub0, i, dxsqics, dysqics, dzsqics,
+                                    mgparam);
+  else if (i - sub0.n < sub1.n)
+    residual_record<S, TR, T, RIdx>(phi, rho, res, RI, g, sub1, i - sub0.n, dxsqics, dysqics,
+                                    dzsqics, mgparam);
 }
 
 // ---------------------------------------------------------------------------
