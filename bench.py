@@ -76,24 +76,27 @@ def main():
     log("first solve (cold start) %d V-cycles, %.2f s incl. conductor set-up"
         % (cold_iters, time.time() - t0))
 
-    for _ in range(args.warmup):
-        step(1)
+    if args.warmup > 0:
+        step(args.warmup)
     torch.cuda.synchronize()
 
-    # --- timed region
-    iters, tsolve, tdep, tpush = [], 0., 0., 0.
+    # --- timed region: one step(K) call, as a run script makes it; the
+    # reference's cadence of special steps (synchv half kick + moments every
+    # top.nhist = 5 steps and on the last step of the call) is included
     solver.timers = dict(deposit=0., solve=0., fetche=0.)
+    nspecial0 = top.nspecial
+    nsolve0 = len(solver.mgiters_history)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(1)
-        iters.append(solver.mgiters)
+    step(args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    iters = solver.mgiters_history[nsolve0:]
+    nspecial = top.nspecial - nspecial0
     ms_per_step = 1e3 * elapsed / args.steps
     ms_solve = 1e3 * solver.timers["solve"] / args.steps
-    log("timed %d steps: %.3f ms/step, %.3f ms/solve, V-cycles %s"
-        % (args.steps, ms_per_step, ms_solve, iters))
+    log("timed %d steps (%d special): %.3f ms/step, %.3f ms/solve, V-cycles %s"
+        % (args.steps, nspecial, ms_per_step, ms_solve, iters))
 
     # --- dominant kernel: finest-level SOR half sweep, HIP events on the
     # solver's stream around each launch (one more step, kernels launched
@@ -127,7 +130,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, cfg, solver, pg, phi_before, iters)
+        cpu = cpu_baseline(args, cfg, solver, pg, phi_before, iters, nspecial / args.steps)
 
     mgtol = solver.mgtol
     out = {
@@ -163,12 +166,13 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(args, cfg, solver, pg, phi_before, iters):
+def cpu_baseline(args, cfg, solver, pg, phi_before, iters, fspecial):
     """The FP64 oracle (single-threaded C restatement of the reference) on
     the host: one step of the same workload with a particle sample of
-    --cpu-particles (deposit and push times scaled to the full count) and
-    the full-mesh solve warm-started from the same phi as the GPU's last
-    solve."""
+    --cpu-particles (deposit and push times scaled to the full count), the
+    full-mesh solve warm-started from the same phi as the GPU's last solve,
+    and the synchv half kick + beam moments of special steps weighted by
+    their share fspecial of the timed steps."""
     import numpy as np
     from oracle import oracle as O
     from warp_amd import configs
@@ -202,18 +206,26 @@ def cpu_baseline(args, cfg, solver, pg, phi_before, iters):
     k = (x >= x0) & (x < x1) & (y >= y0) & (y < y1) & (z >= z0) & (z < z1)
     O.setrho3d(rho, x[k], y[k], z[k], q, w, dx, dy, dz, x0, y0, z0)
     t_dep = time.perf_counter() - t
+    xs, ys, zs, uxs, uys, uzs = x[k], y[k], z[k], ux[k], uy[k], uz[k]
+    t = time.perf_counter()
+    es = [np.zeros(len(xs)) for _ in range(3)]
+    O.sete3d(phi, xs, ys, zs, *es, dx, dy, dz, x0, y0, z0)
+    O.epush3d(uxs, uys, uzs, *es, q, m, 0.5 * configs.DT)
+    O.getzmmnt(xs, ys, zs, uxs, uys, uzs, np.ones(len(xs)))
+    t_synch = time.perf_counter() - t
     rho_full = solver.rho.cpu().numpy().copy()
     t = time.perf_counter()
     it, err = O.multigrid3dsolve(phi, rho_full, dx, dy, dz, conductors=co)
     t_solve = time.perf_counter() - t
     scale = npart / ns
-    step_s = (t_push + t_dep) * scale + t_solve
-    log("CPU oracle: push %.2f s + deposit %.2f s for %d particles, solve %.2f s (%d V-cycles)"
-        % (t_push, t_dep, ns, t_solve, it))
+    step_s = (t_push + t_dep + fspecial * t_synch) * scale + t_solve
+    log("CPU oracle: push %.2f s + deposit %.2f s + synchv/moments %.2f s for %d particles, "
+        "solve %.2f s (%d V-cycles)" % (t_push, t_dep, t_synch, ns, t_solve, it))
     return {"value": 1. / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
             "sample": "one step: gather+push and deposit of every %d-th particle (%d of %d, times scaled "
-                      "by %.2f), full %dx%dx%d solve warm-started from the GPU's phi (%d V-cycles, "
-                      "%.2f s)" % (stride, ns, npart, scale, nx, ny, nz, it, t_solve),
+                      "by %.2f), synchv kick + moments weighted %.2f, full %dx%dx%d solve "
+                      "warm-started from the GPU's phi (%d V-cycles, %.2f s)"
+                      % (stride, ns, npart, scale, fspecial, nx, ny, nz, it, t_solve),
             "ms_per_solve": 1e3 * t_solve, "vcycles": it}
 
 

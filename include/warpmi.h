@@ -195,6 +195,24 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp,
                      double dt, double xmmax, double ymmax, double zmmax,
                      int64_t *nlost, void *stream);
 
+/* padvnc3d(center) for B = 0, no lattice elements (w3d.F:1575-2330), one
+ * pass per particle: E zeroed and fetched (sete3d) into ex, ey, ez (may be
+ * NULL), then by center
+ *   WM_CENTER_FULLV  (w3d.F:1848-1986): epush3d(dt/2) twice, xpush3d(dt);
+ *   WM_CENTER_HALFV  (w3d.F:2063-2182): epush3d(dt/2), xpush3d(dt);
+ *   WM_CENTER_SYNCHV (w3d.F:2186-2311): epush3d(dt/2), no drift.
+ * nlost as in wm_gather_push3d (untouched by synchv, which moves no
+ * particle).  wm_gather_push3d is center WM_CENTER_FULLV. */
+#define WM_CENTER_FULLV 0
+#define WM_CENTER_HALFV 1
+#define WM_CENTER_SYNCHV 2
+int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *zp,
+                double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                double *ey, double *ez, double zgrid, double xmmin, double ymmin,
+                double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
+                int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
+                double zmmax, int64_t *nlost, int64_t center, void *stream);
+
 /* getzmmnt (top.F:521-1360), itask 2: the whole-beam moment sums
  * zmmnts0(1:33) of np particles into mm (device, 33 doubles, overwritten;
  * mm[k-1] = zmmnts0(k)), summed per block then over blocks in a fixed

@@ -1,0 +1,100 @@
+"""The step driver end to end (generate, then step(7) with the reference's
+special-step cadence) on the GPU against the same sequence run with the
+FP64 oracle: deposit, multigrid solve, E gather, halfv/fullv/synchv pushes,
+absorbing boundaries and the beam moments.  The deposit sums in another
+order on the GPU (atomics), so the states agree to round-off, not bit for
+bit: positions and velocities to 1e-10 relative, the moments to 1e-9."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 32, 32, 64
+DOM = (-0.02, 0.02, -0.02, 0.02, 0.0, 0.08)
+
+
+def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist):
+    dx, dy, dz = 0.04 / NX, 0.04 / NY, 0.08 / NZ
+    lo = (DOM[0], DOM[2], DOM[4])
+    phi = np.zeros((NZ + 3, NY + 3, NX + 3))
+
+    def solve(x, y, z):
+        rho = np.zeros((NZ + 1, NY + 1, NX + 1))
+        O.setrho3d(rho, x, y, z, q, w, dx, dy, dz, *lo)
+        O.multigrid3dsolve(phi, rho, dx, dy, dz)
+
+    def efield(x, y, z):
+        e = [np.zeros(len(x)) for _ in range(3)]
+        O.sete3d(phi, x, y, z, *e, dx, dy, dz, *lo)
+        return e
+
+    solve(x, y, z)
+    special, centers = True, []
+    for it in range(1, nsteps + 1):
+        e = efield(x, y, z)
+        center = "halfv" if special else "fullv"
+        centers.append(center)
+        for _ in range(1 if special else 2):
+            O.epush3d(*u, *e, q, m, 0.5 * (dt * 1 * 1.))
+        O.xpush3d(x, y, z, *u, np.ones(len(x)), dt * 1 * 1.)
+        k = (x >= DOM[0]) & (x < DOM[1]) & (y >= DOM[2]) & (y < DOM[3]) & (z >= DOM[4]) & \
+            (z < DOM[5])
+        x, y, z, u = x[k], y[k], z[k], [a[k] for a in u]
+        solve(x, y, z)
+        special = it % nhist == 0 or it == nsteps
+        if special:
+            centers.append("synchv")
+            e = efield(x, y, z)
+            O.epush3d(*u, *e, q, m, 0.5 * (dt * 1 * 1.))
+    return x, y, z, u, centers
+
+
+def test_step_driver_matches_oracle(device):
+    import warp_amd
+    from warp_amd import diagnostics, stepper
+    from warp_amd.globals import top
+    from warp_amd.particles import load_gaussian
+    pg, js, n = load_gaussian(20000, (0.004, 0.004, 0.015), 0.04, 1.0e6, 1.0e-3, DOM, 31,
+                              device=device)
+    x0 = [getattr(pg, c).cpu().numpy().copy() for c in ("xp", "yp", "zp")]
+    u0 = [getattr(pg, c).cpu().numpy().copy() for c in ("uxp", "uyp", "uzp")]
+    s = warp_amd.MultiGrid3D(nx=NX, ny=NY, nz=NZ, xmmin=DOM[0], xmmax=DOM[1], ymmin=DOM[2],
+                             ymmax=DOM[3], zmmin=DOM[4], zmmax=DOM[5], bounds=[0] * 6,
+                             mgverbose=-1)
+    saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time, nhist=top.nhist,
+                 lspecial=top.lspecial, particlesortinterval=top.particlesortinterval)
+    calls = []
+    orig = s.gather_push
+
+    def spy(pg_, js_, dt_, center="fullv"):
+        calls.append(center)
+        return orig(pg_, js_, dt_, center)
+
+    s.gather_push = spy
+    try:
+        top.pgroup, top.dt, top.it, top.time, top.nhist = pg, 1.0e-9, 0, 0., 5
+        top.particlesortinterval = 0
+        warp_amd.registersolver(s)
+        stepper.generate()
+        stepper.step(7)
+        xo, yo, zo, uo, centers = _oracle_run(*[a.copy() for a in x0], [a.copy() for a in u0],
+                                              pg.sq[0], pg.sm[0], pg.sw[0], 1.0e-9, 7, 5)
+        assert calls == centers
+        assert centers == ["halfv", "fullv", "fullv", "fullv", "fullv", "synchv", "halfv",
+                           "fullv", "synchv"]
+        got = [getattr(pg, c).cpu().numpy() for c in ("xp", "yp", "zp", "uxp", "uyp", "uzp")]
+        assert len(got[0]) == len(xo)
+        for a, b in zip(got, [xo, yo, zo] + uo):
+            scale = np.abs(b).max()
+            assert np.abs(a - b).max() <= 1e-10 * scale
+        mm = diagnostics.finalize(list(O.getzmmnt(xo, yo, zo, *uo, np.ones(len(xo)))))
+        for key in ("xrms", "yrms", "zrms", "epsx", "epsy", "vzbar"):
+            assert top.moments[key] == pytest.approx(mm[key], rel=1e-9), key
+        assert top.it == 7 and top.lspecial
+    finally:
+        warp_amd.unregistersolver(s)
+        for k, v in saved.items():
+            setattr(top, k, v)

@@ -189,3 +189,36 @@ def test_find_mgparam_search_logic():
     assert abs(s.mgparam - 1.6) < 0.05
     assert 5 + int(round(400 * (s.mgparam - 1.6) ** 2)) == 5
     assert s.downpasses == 1 and s.uppasses == 1 and s.mgtol == 1e-6
+
+
+def test_thisstep_control_arrays():
+    """dtop.F:764-789: (start, end, increment) then single steps."""
+    from warp_amd.stepper import thisstep
+    c = [10, 30, 5] + [0] * 47
+    assert [i for i in range(1, 40) if thisstep(i, c)] == [10, 15, 20, 25, 30]
+    assert thisstep(0, c)  # the zero entries after the triple match step 0
+    c = [0, 0, 0, 7, 12] + [0] * 45
+    assert [i for i in range(1, 20) if thisstep(i, c)] == [7, 12]
+    assert not any(thisstep(i, [0] * 50) for i in range(1, 100))
+
+
+def test_special_step_cadence():
+    """step3d's lspecial (w3d.F:854-880) with the defaults nhist = 5,
+    itdump = 500: every 5th step, every 500th, the last step of a call,
+    and all steps when allspecl."""
+    from warp_amd.globals import top
+    from warp_amd.stepper import special_step
+    saved = (top.it, top.time, top.allspecl)
+    try:
+        top.time = 0.
+        got = []
+        for it in range(1, 13):
+            top.it = it
+            got.append(special_step(llast=(it == 12))[0])
+        assert [i for i, sp in zip(range(1, 13), got) if sp] == [5, 10, 12]
+        top.it = 500
+        assert special_step(False) == (True, True)
+        top.it, top.allspecl = 3, True
+        assert special_step(False)[0]
+    finally:
+        top.it, top.time, top.allspecl = saved

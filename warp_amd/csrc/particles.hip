@@ -320,6 +320,10 @@ __global__ __launch_bounds__(256) void k_xpush(long long np, double *__restrict_
 // two half kicks, one drift.  Particles whose new position leaves
 // [xmmin,xmmax) x [ymmin,ymmax) x [zmmin+zgrid,zmmax+zgrid) (the absorbing
 // test of particleboundaries3d) are counted into *nlost.
+// CENTER selects padvnc3d's centring (w3d.F:1848-2330, B = 0, no lattice):
+// 0 "fullv": two half kicks with E(x), then the drift; 1 "halfv": one half
+// kick, then the drift; 2 "synchv": one half kick, no drift.
+template <int CENTER>
 __global__ __launch_bounds__(256) void k_gather_push(
     const double *__restrict__ phi, long long np, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ z, double *__restrict__ ux, double *__restrict__ uy,
@@ -338,14 +342,11 @@ __global__ __launch_bounds__(256) void k_gather_push(
     vx = vx + ea * ch;
     vy = vy + eb * ch;
     vz = vz + ec * ch;
-    vx = vx + ea * ch;
-    vy = vy + eb * ch;
-    vz = vz + ec * ch;
-    double gi = gaminv[ip];
-    double xn = xi + vx * gi * dt, yn = yi + vy * gi * dt, zn = zi + vz * gi * dt;
-    x[ip] = xn;
-    y[ip] = yn;
-    z[ip] = zn;
+    if (CENTER == 0) {
+      vx = vx + ea * ch;
+      vy = vy + eb * ch;
+      vz = vz + ec * ch;
+    }
     ux[ip] = vx;
     uy[ip] = vy;
     uz[ip] = vz;
@@ -354,8 +355,15 @@ __global__ __launch_bounds__(256) void k_gather_push(
       ey[ip] = eb;
       ez[ip] = ec;
     }
-    lost = !(xn >= m.xmmin && xn < xmmax && yn >= m.ymmin && yn < ymmax &&
-             zn >= m.zmmin + m.zgrid && zn < zmmax + m.zgrid);
+    if (CENTER != 2) {
+      double gi = gaminv[ip];
+      double xn = xi + vx * gi * dt, yn = yi + vy * gi * dt, zn = zi + vz * gi * dt;
+      x[ip] = xn;
+      y[ip] = yn;
+      z[ip] = zn;
+      lost = !(xn >= m.xmmin && xn < xmmax && yn >= m.ymmin && yn < ymmax &&
+               zn >= m.zmmin + m.zgrid && zn < zmmax + m.zgrid);
+    }
   }
   if (nlost) {
     unsigned long long b = __ballot(lost);
@@ -685,11 +693,25 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
                      double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
                      int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
                      double zmmax, int64_t *nlost, void *stream) {
+  return wm_padvnc3d(phi, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, zgrid, xmmin, ymmin,
+                     zmmin, dx, dy, dz, nx, ny, nz, q, m, dt, xmmax, ymmax, zmmax, nlost,
+                     WM_CENTER_FULLV, stream);
+}
+
+int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *zp,
+                double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                double *ey, double *ez, double zgrid, double xmmin, double ymmin, double zmmin,
+                double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz, double q,
+                double m, double dt, double xmmax, double ymmax, double zmmax, int64_t *nlost,
+                int64_t center, void *stream) {
   WM_ENTRY({
     check_grid(nx, ny, nz);
+    WM_REQUIRE(center == WM_CENTER_FULLV || center == WM_CENTER_HALFV ||
+                   center == WM_CENTER_SYNCHV,
+               "wm_padvnc3d: center must be WM_CENTER_FULLV, _HALFV or _SYNCHV");
     if (np == 0) return 0;
-    WM_REQUIRE(phi && xp && yp && zp && uxp && uyp && uzp && gaminv, "wm_gather_push3d: NULL array");
-    WM_REQUIRE((ex && ey && ez) || (!ex && !ey && !ez), "wm_gather_push3d: give all or none of ex, ey, ez");
+    WM_REQUIRE(phi && xp && yp && zp && uxp && uyp && uzp && gaminv, "wm_padvnc3d: NULL array");
+    WM_REQUIRE((ex && ey && ez) || (!ex && !ey && !ez), "wm_padvnc3d: give all or none of ex, ey, ez");
     Mesh me = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
     double tdxi = 1. / (2. * dx), tdyi = 1. / (2. * dy), tdzi = 1. / (2. * dz);
     // fulldt_s = dt*ndts*dtscale, halfdt_s = 0.5*fulldt_s (w3d.F:1747-1748);
@@ -697,9 +719,19 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
     double fulldt = dt * 1 * 1.;
     double halfdt = 0.5 * fulldt;
     double ch = q * halfdt / m;
-    hipLaunchKernelGGL(k_gather_push, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
-                       (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi, tdyi,
-                       tdzi, ch, fulldt, xmmax, ymmax, zmmax, (unsigned long long *)nlost);
+    unsigned long long *nl = (unsigned long long *)nlost;
+    if (center == WM_CENTER_FULLV)
+      hipLaunchKernelGGL(k_gather_push<0>, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
+                         (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi,
+                         tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+    else if (center == WM_CENTER_HALFV)
+      hipLaunchKernelGGL(k_gather_push<1>, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
+                         (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi,
+                         tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+    else
+      hipLaunchKernelGGL(k_gather_push<2>, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
+                         (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi,
+                         tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
     WM_LAUNCHED();
   });
 }

@@ -44,7 +44,14 @@ top = Top(dt=0., time=0., it=0, fstype=-1, pgroup=None, zbeam=0., zgrid=0.,
           depos="direct1", efetch=1, lresetparticlee=True, steptime=0.,
           fstime=0., lrtime=0.,
           # steps between particlesortxyzwithcopy calls in step(); 0 = never
-          particlesortinterval=0)
+          particlesortinterval=0,
+          # step controls of w3dexe/step3d (top.v): "special" steps end with
+          # the synchv half kick and the beam moments
+          nt=20000000, tstop=1.0e36, zstop=1.0e36, itmomnts=[0] * 50, nhist=5, itdump=500,
+          allspecl=False, lspecial=False, nspecial=0,
+          # beam moments of the last special step, and the moment history
+          # (every nhist steps): dicts keyed by top's variable names
+          moments={}, hist=[])
 
 # w3d: mesh and boundary inputs (w3d.v), plus the FieldSolveAPI globals
 # (w3d.v:432-458) through which fetche receives particle positions.

@@ -54,6 +54,7 @@ class MultiGrid3D(FieldSolver):
         N.lib()
         self.solvergeom = "XYZgeom"
         self.mgiters = 0
+        self.mgiters_history = []  # mgiters of every solve, in order
         self.mgerror = 0.
         self.lwithselfe = 0
         self._phi = torch.zeros((self.nz + 3, self.ny + 3, self.nx + 3), dtype=torch.float64,
@@ -281,6 +282,7 @@ class MultiGrid3D(FieldSolver):
                PRECISIONS[self.precision], C.byref(it), C.byref(err),
                N.stream_of(self.device))
         self.mgiters = it.value
+        self.mgiters_history.append(self.mgiters)
         self.mgerror = err.value
         f3d.mgiters, f3d.mgerror = self.mgiters, self.mgerror
         mgverbose = self.getmgverbose()
@@ -378,9 +380,12 @@ class MultiGrid3D(FieldSolver):
         phi.copy_(f)
 
     # fused fullv advance used by the stepper
-    def gather_push(self, pg, js, dt):
-        """fetche + epush(dt/2) twice + xpush(dt) for species js in one pass
-        (padvnc3d fullv with B = 0, w3d.F:1816, 1956-1986)."""
+    _CENTERS = {"fullv": 0, "halfv": 1, "synchv": 2}
+
+    def gather_push(self, pg, js, dt, center="fullv"):
+        """fetche and padvnc3d's kicks and drift for species js in one pass
+        (B = 0, w3d.F:1575-2330): "fullv" epush(dt/2) twice + xpush(dt),
+        "halfv" epush(dt/2) + xpush(dt), "synchv" epush(dt/2)."""
         sl = pg.species_slice(js)
         n = sl.stop - sl.start
         if n == 0:
@@ -389,13 +394,13 @@ class MultiGrid3D(FieldSolver):
         t0 = time.perf_counter()
         if self._nlost is None:
             self._nlost = torch.zeros(1, dtype=torch.int64, device=self.device)
-        N.call("wm_gather_push3d", self._phi.data_ptr(), n, col("xp").data_ptr(),
+        N.call("wm_padvnc3d", self._phi.data_ptr(), n, col("xp").data_ptr(),
                col("yp").data_ptr(), col("zp").data_ptr(), col("uxp").data_ptr(),
                col("uyp").data_ptr(), col("uzp").data_ptr(), col("gaminv").data_ptr(),
                col("ex").data_ptr(), col("ey").data_ptr(), col("ez").data_ptr(), top.zgrid,
                self.xmmin, self.ymmin, self.zmmin, self.dx, self.dy, self.dz, self.nx,
                self.ny, self.nz, pg.sq[js], pg.sm[js], dt, self.xmmax, self.ymmax, self.zmmax,
-               self._nlost.data_ptr(), N.stream_of(self.device))
+               self._nlost.data_ptr(), self._CENTERS[center], N.stream_of(self.device))
         self.timers["fetche"] += time.perf_counter() - t0
 
     def take_lost_count(self):

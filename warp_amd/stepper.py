@@ -1,21 +1,54 @@
 """Time-step driver restating the reference's electrostatic step3d
-(pywarp90/w3d.F:676-1029) for registered solvers: padvnc3d "fullv"
-(fetch E, two half kicks, drift), particle boundaries, charge deposition
-(loadrho) and the field solve, then time bookkeeping.  ``generate`` does
-the initial deposit and solve (w3dgen -> step3d("w3dgen")).
+(pywarp90/w3d.F:511-1029) for registered solvers.  Each step: padvnc3d
+"halfv" when the previous step was special (velocities at the integer
+time level) else "fullv" (fetch E, half kicks, drift), particle boundaries,
+charge deposition (loadrho) and the field solve; a special step then ends
+with padvnc3d "synchv" (fetch E at the new positions, half kick) and the
+beam moments (getzmmnt).  ``generate`` does the initial deposit and solve
+and the moments of the loaded beam (w3dgen -> step3d("w3dgen"), "gen").
 """
 import time
 
 import torch
 
+from . import diagnostics
 from . import fieldsolver as FS
 from .globals import top, w3d
 
+MACHEPS = 1.0e-14  # pywarp90/top.h
+
+
+def thisstep(it, icontrol):
+    """dtop.F:764-789: icontrol = (start, end, increment, extra steps...)."""
+    istart, iend, iinc = icontrol[0], icontrol[1], icontrol[2]
+    if iinc != 0 and istart <= it <= iend and (it - istart) % iinc == 0:
+        return True
+    return any(it == i for i in icontrol[3:])
+
+
+def special_step(llast):
+    """The lspecial flag of step3d (w3d.F:854-880) for the current top.it,
+    top.time: end of run, moments, history, dump or last step of this call
+    (the plot controls, always off here, are not modelled)."""
+    lfinishd = (top.it >= top.nt) or (top.time >= top.tstop * (1. - MACHEPS)) or \
+        (top.zbeam >= top.zstop)
+    lmoments = thisstep(top.it, top.itmomnts)
+    lhist = top.nhist != 0 and top.it % top.nhist == 0
+    ldump = top.it % top.itdump == 0
+    return bool(lfinishd or lmoments or lhist or ldump or llast or top.it == 0 or top.allspecl), lhist
+
 
 def generate():
-    """Initial loadrho and field solve (scripts/ctl.py:65-75)."""
+    """Initial loadrho and field solve (scripts/ctl.py:65-75), then the
+    "gen" phase: E fetched at the loaded positions, velocities unchanged,
+    moments of the loaded beam; the first step is then a halfv step."""
     FS.loadrhoregistered()
     FS.fieldsolregistered()
+    pg = top.pgroup
+    if pg is not None and pg.npmax > 0:
+        padvnc3d("gen", pg, top.dt)
+        top.moments = diagnostics.getzmmnt(pg)
+    top.lspecial = True
 
 
 def particleboundaries3d(pg, solver, nlost=None):
@@ -32,12 +65,14 @@ def particleboundaries3d(pg, solver, nlost=None):
         pg.compact(keep)
 
 
-def padvnc3d_fullv(pg, dt):
-    """padvnc3d("fullv") for B = 0 (w3d.F:1780-1986)."""
+def padvnc3d(center, pg, dt):
+    """padvnc3d(center) for B = 0 (w3d.F:1575-2330): "fullv" (two half
+    kicks, drift), "halfv" (one half kick, drift), "synchv" (one half
+    kick), "gen" (E fetched only)."""
     solvers = FS.getregisteredsolvers()
-    if len(solvers) == 1 and hasattr(solvers[0], "gather_push"):
+    if center != "gen" and len(solvers) == 1 and hasattr(solvers[0], "gather_push"):
         for js in range(pg.ns):
-            solvers[0].gather_push(pg, js, dt)
+            solvers[0].gather_push(pg, js, dt, center)
         return
     halfdt = 0.5 * (dt * 1 * 1.)
     from . import _native as N
@@ -52,23 +87,33 @@ def padvnc3d_fullv(pg, dt):
         w3d.xfsapi = None
         FS.fetcheregistered()
         w3d.npfsapi = 0
+        if center == "gen":
+            continue
         dev = pg.xp.device
         s = N.stream_of(dev)
         col = lambda c: getattr(pg, c)[sl].data_ptr()
-        for _ in range(2):
+        for _ in range(2 if center == "fullv" else 1):
             N.call("wm_epush3d", n, col("uxp"), col("uyp"), col("uzp"), col("ex"), col("ey"),
                    col("ez"), pg.sq[js], pg.sm[js], halfdt, s)
-        N.call("wm_xpush3d", n, col("xp"), col("yp"), col("zp"), col("uxp"), col("uyp"),
-               col("uzp"), col("gaminv"), dt * 1 * 1., s)
+        if center != "synchv":
+            N.call("wm_xpush3d", n, col("xp"), col("yp"), col("zp"), col("uxp"), col("uyp"),
+                   col("uzp"), col("gaminv"), dt * 1 * 1., s)
+
+
+def padvnc3d_fullv(pg, dt):
+    """padvnc3d("fullv")."""
+    padvnc3d("fullv", pg, dt)
 
 
 def step(n=1, pgroup=None, boundaries=True):
-    """Advance n time steps (scripts/ctl.py:78-104 -> w3dexe)."""
+    """Advance n time steps (scripts/ctl.py:78-104 -> w3dexe -> step3d)."""
     pg = pgroup if pgroup is not None else top.pgroup
     solvers = FS.getregisteredsolvers()
-    for _ in range(n):
+    for k in range(n):
         t0 = time.perf_counter()
-        padvnc3d_fullv(pg, top.dt)
+        top.it += 1
+        padvnc3d("halfv" if top.lspecial else "fullv", pg, top.dt)
+        top.time += top.dt
         if boundaries and solvers:
             nlost = None
             if len(solvers) == 1 and hasattr(solvers[0], "take_lost_count"):
@@ -79,6 +124,11 @@ def step(n=1, pgroup=None, boundaries=True):
             solvers[0].sortparticles(pg)
         FS.loadrhoregistered()
         FS.fieldsolregistered()
-        top.it += 1
-        top.time += top.dt
+        top.lspecial, lhist = special_step(llast=(k == n - 1))
+        if top.lspecial:
+            top.nspecial += 1
+            padvnc3d("synchv", pg, top.dt)
+            top.moments = diagnostics.getzmmnt(pg)
+            if lhist:
+                top.hist.append(dict(it=top.it, time=top.time, **top.moments))
         top.steptime += time.perf_counter() - t0
