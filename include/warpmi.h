@@ -213,6 +213,25 @@ int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *z
                 int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
                 double zmmax, int64_t *nlost, int64_t center, void *stream);
 
+/* wm_padvnc3d with efetch = 3: E interpolated from the precomputed grid
+ * field selfe (as wm_getselfe3d leaves it; sete3d -> sete_from_e_linear3d,
+ * w3d.F:3702-3715, 4362-4460) instead of the centred differences of phi. */
+int wm_padvnc3d_selfe(const double *selfe, int64_t np, double *xp, double *yp, double *zp,
+                      double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                      double *ey, double *ez, double zgrid, double xmmin, double ymmin,
+                      double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
+                      int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
+                      double zmmax, int64_t *nlost, int64_t center, void *stream);
+
+/* sete3d with efetch = 3 (w3d.F:3702-3715 -> sete_from_e_linear3d
+ * 4362-4460): adds to ex, ey, ez the trilinear interpolation of selfe
+ * (3, 0:nx, 0:ny, 0:nz), component fastest, nxguarde = 0.  Particles whose
+ * scaled position lies outside [0, n] in any axis get nothing. */
+int wm_sete3d_from_e(const double *selfe, int64_t np, const double *xp, const double *yp,
+                     const double *zp, double zgrid, double xmmin, double ymmin, double zmmin,
+                     double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz,
+                     double *ex, double *ey, double *ez, void *stream);
+
 /* getzmmnt (top.F:521-1360), itask 2: the whole-beam moment sums
  * zmmnts0(1:33) of np particles into mm (device, 33 doubles, overwritten;
  * mm[k-1] = zmmnts0(k)), summed per block then over blocks in a fixed

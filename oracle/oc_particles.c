@@ -185,6 +185,44 @@ void oc_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
       }
 }
 
+/* sete_from_e_linear3d, w3d.F:4362-4460 (no symmetry), nxguarde = 0:
+ * E at each particle by trilinear interpolation of the precomputed grid
+ * field selfe(3, 0:nx, 0:ny, 0:nz), added to ex, ey, ez.  Particles with
+ * a scaled position outside [0, n] are skipped (w3d.F:4425-4427).  For a
+ * particle exactly on the upper face (i = n) the reference reads selfe at
+ * n+1 with a zero weight; here that term reads selfe at n (same zero
+ * contribution for finite fields). */
+void oc_sete_from_e3d(const double *selfe, int64_t np, const double *xp,
+                      const double *yp, const double *zp, double zgrid,
+                      double xmmin, double ymmin, double zmmin, double dx,
+                      double dy, double dz, int64_t nx, int64_t ny, int64_t nz,
+                      double *ex, double *ey, double *ez) {
+  double dxi = 1. / dx, dyi = 1. / dy, dzi = 1. / dz;
+  int64_t ip;
+  for (ip = 0; ip < np; ip++) {
+    double xi = (xp[ip] - xmmin) * dxi;
+    double yj = (yp[ip] - ymmin) * dyi;
+    double zk = (zp[ip] - zgrid - zmmin) * dzi;
+    if (!(xi >= 0. && xi <= nx && yj >= 0. && yj <= ny && zk >= 0. && zk <= nz)) continue;
+    int64_t i = (int64_t)xi, j = (int64_t)yj, k = (int64_t)zk;
+    int64_t i1 = i < nx ? i + 1 : i, j1 = j < ny ? j + 1 : j, k1 = k < nz ? k + 1 : k;
+    double u1 = xi - i, v1 = yj - j, w1 = zk - k;
+    double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+    double out[3];
+    int c;
+#define E(a, b, d) selfe[3 * IDXR(a, b, d) + c]
+    for (c = 0; c < 3; c++)
+      out[c] = u0 * v0 * w0 * E(i, j, k) + u1 * v0 * w0 * E(i1, j, k) +
+               u0 * v1 * w0 * E(i, j1, k) + u1 * v1 * w0 * E(i1, j1, k) +
+               u0 * v0 * w1 * E(i, j, k1) + u1 * v0 * w1 * E(i1, j, k1) +
+               u0 * v1 * w1 * E(i, j1, k1) + u1 * v1 * w1 * E(i1, j1, k1);
+#undef E
+    ex[ip] = ex[ip] + out[0];
+    ey[ip] = ey[ip] + out[1];
+    ez[ip] = ez[ip] + out[2];
+  }
+}
+
 /* getzmmnt, top.F:521-1360, itask == 2, the serial (non-J90) sums of the
  * whole-beam moments zmmnts0(1:33) (top.F:1059-1136).  usegaminv selects
  * the branch for relativistic velocities without momentum moments

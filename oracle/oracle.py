@@ -107,6 +107,8 @@ def _declare(L):
     L.oc_epush3d.argtypes = [_i64, P, P, P, P, P, P, _d, _d, _d]
     L.oc_xpush3d.argtypes = [_i64, P, P, P, P, P, P, P, _d]
     L.oc_getselfe3d.argtypes = [P, _i64, _i64, _i64, P, _d, _d, _d, _i64]
+    L.oc_sete_from_e3d.argtypes = [P, _i64, P, P, P, _d, _d, _d, _d, _d, _d, _d, _i64,
+                                   _i64, _i64, P, P, P]
     L.oc_getzmmnt.argtypes = [_i64, P, P, P, P, P, P, P, _i64, P]
     L.oc_conductordelfromintercepts.restype = _i64
     L.oc_conductordelfromintercepts.argtypes = [P, P, _d, _d, _i64, P]
@@ -470,6 +472,17 @@ def epush3d(uxp, uyp, uzp, ex, ey, ez, q, m, ddt):
 def xpush3d(xp, yp, zp, uxp, uyp, uzp, gaminv, dtp):
     lib().oc_xpush3d(len(xp), _p(xp), _p(yp), _p(zp), _p(uxp), _p(uyp),
                      _p(uzp), _p(gaminv), dtp)
+
+
+def sete_from_e3d(selfe, xp, yp, zp, ex, ey, ez, dx, dy, dz, xmmin, ymmin, zmmin,
+                  zgrid=0.0):
+    """sete3d with efetch=3: trilinear interpolation of the grid field selfe
+    (nz+1, ny+1, nx+1, 3), added to ex, ey, ez (w3d.F:4362-4460)."""
+    nz, ny, nx = selfe.shape[0] - 1, selfe.shape[1] - 1, selfe.shape[2] - 1
+    xp, yp, zp = _f64(xp), _f64(yp), _f64(zp)
+    lib().oc_sete_from_e3d(_p(np.ascontiguousarray(selfe)), len(xp), _p(xp), _p(yp), _p(zp),
+                           zgrid, xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, _p(ex), _p(ey),
+                           _p(ez))
 
 
 def getselfe3d(phi, dx, dy, dz, selfe=None, lzero=True):

@@ -141,6 +141,11 @@ void oc_xpush3d(int64_t np, double *xp, double *yp, double *zp,
 void oc_getzmmnt(int64_t np, const double *xp, const double *yp, const double *zp,
                  const double *uxp, const double *uyp, const double *uzp,
                  const double *gaminv, int64_t usegaminv, double *mm);
+void oc_sete_from_e3d(const double *selfe, int64_t np, const double *xp,
+                      const double *yp, const double *zp, double zgrid,
+                      double xmmin, double ymmin, double zmmin, double dx,
+                      double dy, double dz, int64_t nx, int64_t ny, int64_t nz,
+                      double *ex, double *ey, double *ez);
 void oc_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
                    double *selfe, double dx, double dy, double dz,
                    int64_t lzero);

@@ -309,3 +309,52 @@ def test_padvnc3d_rejects_bad_center(device):
         N.call("wm_padvnc3d", None, 0, None, None, None, None, None, None, None, None, None,
                None, 0.0, *LO, DX, DY, DZ, NX, NY, NZ, 1., 1., 1e-9, 0.02, 0.02, 0.08, None, 7,
                N.stream_of(device))
+
+
+def test_sete_from_e_bitwise(device):
+    """efetch=3 gather (sete_from_e_linear3d) on the GPU equals the oracle
+    bit for bit, including particles on the upper faces and outside."""
+    from warp_amd import _native as N
+    phi = _phi(12)
+    se = O.getselfe3d(phi, DX, DY, DZ)
+    x, y, z = _beam(50000, 13)
+    x = np.concatenate([x, [LO[0] + NX * DX, -0.03, 0.0]])
+    y = np.concatenate([y, [0.0, 0.0, LO[1] + NY * DY]])
+    z = np.concatenate([z, [0.04, 0.04, NZ * DZ]])
+    n = len(x)
+    e0 = [np.full(n, 0.25) for _ in range(3)]
+    eo = [a.copy() for a in e0]
+    O.sete_from_e3d(se, x, y, z, *eo, DX, DY, DZ, *LO)
+    eg = [_t(a, device) for a in e0]
+    sd, xd, yd, zd = _t(se, device), _t(x, device), _t(y, device), _t(z, device)
+    N.call("wm_sete3d_from_e", sd.data_ptr(), n, xd.data_ptr(), yd.data_ptr(), zd.data_ptr(),
+           0.0, *LO, DX, DY, DZ, NX, NY, NZ, *[a.data_ptr() for a in eg], N.stream_of(device))
+    for a, b in zip(eg, eo):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)
+
+
+def test_padvnc3d_selfe_bitwise(device):
+    """The fused fullv advance with efetch=3 equals E=0, sete_from_e,
+    epush3d(dt/2) twice, xpush3d(dt) of the oracle."""
+    from warp_amd import _native as N
+    x, y, z = _beam(60000, 14)
+    n = len(x)
+    rng = np.random.default_rng(15)
+    u = [rng.normal(0, 1e3, n), rng.normal(0, 1e3, n), 1e6 + rng.normal(0, 1e3, n)]
+    se = O.getselfe3d(_phi(16) * 100., DX, DY, DZ)
+    q, m, dt = 1.602176565e-19, 39 * 1.660538921e-27, 1e-9
+    g = [_t(a, device) for a in (x, y, z)] + [_t(a, device) for a in u]
+    gi = torch.ones(n, dtype=torch.float64, device=device)
+    E = [torch.zeros(n, dtype=torch.float64, device=device) for _ in range(3)]
+    sd = _t(se, device)
+    N.call("wm_padvnc3d_selfe", sd.data_ptr(), n, *[a.data_ptr() for a in g],
+           gi.data_ptr(), *[a.data_ptr() for a in E], 0.0, *LO, DX, DY, DZ, NX, NY, NZ, q, m,
+           dt, 0.02, 0.02, 0.08, None, 0, N.stream_of(device))
+    xo, uo = [x.copy(), y.copy(), z.copy()], [a.copy() for a in u]
+    eo = [np.zeros(n) for _ in range(3)]
+    O.sete_from_e3d(se, *xo, *eo, DX, DY, DZ, *LO)
+    O.epush3d(*uo, *eo, q, m, 0.5 * (dt * 1 * 1.))
+    O.epush3d(*uo, *eo, q, m, 0.5 * (dt * 1 * 1.))
+    O.xpush3d(*xo, *uo, np.ones(n), dt)
+    for a, b in zip(g + E, xo + uo + eo):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)

@@ -16,7 +16,7 @@ NX, NY, NZ = 32, 32, 64
 DOM = (-0.02, 0.02, -0.02, 0.02, 0.0, 0.08)
 
 
-def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist):
+def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist, efetch=1):
     dx, dy, dz = 0.04 / NX, 0.04 / NY, 0.08 / NZ
     lo = (DOM[0], DOM[2], DOM[4])
     phi = np.zeros((NZ + 3, NY + 3, NX + 3))
@@ -28,7 +28,10 @@ def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist):
 
     def efield(x, y, z):
         e = [np.zeros(len(x)) for _ in range(3)]
-        O.sete3d(phi, x, y, z, *e, dx, dy, dz, *lo)
+        if efetch == 3:
+            O.sete_from_e3d(O.getselfe3d(phi, dx, dy, dz), x, y, z, *e, dx, dy, dz, *lo)
+        else:
+            O.sete3d(phi, x, y, z, *e, dx, dy, dz, *lo)
         return e
 
     solve(x, y, z)
@@ -52,7 +55,8 @@ def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist):
     return x, y, z, u, centers
 
 
-def test_step_driver_matches_oracle(device):
+@pytest.mark.parametrize("efetch", [1, 3])
+def test_step_driver_matches_oracle(device, efetch):
     import warp_amd
     from warp_amd import diagnostics, stepper
     from warp_amd.globals import top
@@ -65,7 +69,8 @@ def test_step_driver_matches_oracle(device):
                              ymmax=DOM[3], zmmin=DOM[4], zmmax=DOM[5], bounds=[0] * 6,
                              mgverbose=-1)
     saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time, nhist=top.nhist,
-                 lspecial=top.lspecial, particlesortinterval=top.particlesortinterval)
+                 lspecial=top.lspecial, particlesortinterval=top.particlesortinterval,
+                 efetch=top.efetch)
     calls = []
     orig = s.gather_push
 
@@ -77,11 +82,13 @@ def test_step_driver_matches_oracle(device):
     try:
         top.pgroup, top.dt, top.it, top.time, top.nhist = pg, 1.0e-9, 0, 0., 5
         top.particlesortinterval = 0
+        top.efetch = efetch
         warp_amd.registersolver(s)
         stepper.generate()
         stepper.step(7)
         xo, yo, zo, uo, centers = _oracle_run(*[a.copy() for a in x0], [a.copy() for a in u0],
-                                              pg.sq[0], pg.sm[0], pg.sw[0], 1.0e-9, 7, 5)
+                                              pg.sq[0], pg.sm[0], pg.sw[0], 1.0e-9, 7, 5,
+                                              efetch)
         assert calls == centers
         assert centers == ["halfv", "fullv", "fullv", "fullv", "fullv", "synchv", "halfv",
                            "fullv", "synchv"]

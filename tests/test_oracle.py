@@ -493,6 +493,57 @@ def test_getselfe_linear():
     np.testing.assert_allclose(se[..., 2], -4.0, rtol=1e-12)
 
 
+def test_sete_from_e_linear_field_exact():
+    """Trilinear interpolation reproduces a field linear in x, y, z, and a
+    particle on a node gets that node's value (sete_from_e_linear3d)."""
+    nx, ny, nz, dx, dy, dz = 8, 6, 10, 0.1, 0.2, 0.3
+    lo = (-0.4, -0.6, 0.1)
+    Z, Y, X = np.meshgrid(lo[2] + np.arange(nz + 1) * dz, lo[1] + np.arange(ny + 1) * dy,
+                          lo[0] + np.arange(nx + 1) * dx, indexing="ij")
+    coef = np.array([[1., 2., -3., 0.5], [0.3, -1., 4., 2.], [-2., 0.1, 0.2, -0.7]])
+    se = np.stack([c[0] + c[1] * X + c[2] * Y + c[3] * Z for c in coef], -1)
+    rng = np.random.default_rng(3)
+    n = 2000
+    x = lo[0] + rng.uniform(0, nx * dx, n)
+    y = lo[1] + rng.uniform(0, ny * dy, n)
+    z = lo[2] + rng.uniform(0, nz * dz, n)
+    e = [np.zeros(n) for _ in range(3)]
+    O.sete_from_e3d(se, x, y, z, *e, dx, dy, dz, *lo)
+    for c, ec in zip(coef, e):
+        np.testing.assert_allclose(ec, c[0] + c[1] * x + c[2] * y + c[3] * z, atol=1e-12)
+    # nodes on a dyadic mesh (scaled positions exact), including the upper
+    # faces, where the weight-zero neighbour lies beyond the mesh
+    bx, by, bz = 0.125, 0.25, 0.5
+    ii = np.array([0, 3, nx, nx]); jj = np.array([0, 2, ny, 0]); kk = np.array([0, 5, nz, nz])
+    xn, yn, zn = ii * bx, jj * by, kk * bz
+    e = [np.zeros(4) for _ in range(3)]
+    O.sete_from_e3d(se, xn, yn, zn, *e, bx, by, bz, 0., 0., 0.)
+    for c in range(3):
+        assert np.array_equal(e[c], se[kk, jj, ii, c])
+    # accumulates, and skips particles outside [0, n]
+    e = [np.ones(2) for _ in range(3)]
+    O.sete_from_e3d(se, np.array([-0.01, xn[1]]), yn[:2], zn[:2], *e, bx, by, bz, 0., 0., 0.)
+    assert e[0][0] == 1. and e[0][1] == 1. + se[kk[1], jj[1], ii[1], 0]
+
+
+def test_efetch3_of_linear_phi_is_constant():
+    """getselfe3d then sete_from_e_linear3d of a linear phi gives -grad phi
+    at any position, as the phi gather (efetch 1) does."""
+    nx, ny, nz, dx, dy, dz = 5, 6, 7, 0.2, 0.1, 0.3
+    K = np.arange(-1, nz + 2)[:, None, None] * dz
+    J = np.arange(-1, ny + 2)[None, :, None] * dy
+    I = np.arange(-1, nx + 2)[None, None, :] * dx
+    phi = -1.5 * I + 2.5 * J + 4.0 * K
+    se = O.getselfe3d(phi, dx, dy, dz)
+    rng = np.random.default_rng(4)
+    x, y, z = rng.uniform(0, nx * dx, 300), rng.uniform(0, ny * dy, 300), rng.uniform(0, nz * dz, 300)
+    e = [np.zeros(300) for _ in range(3)]
+    O.sete_from_e3d(se, x, y, z, *e, dx, dy, dz, 0., 0., 0.)
+    np.testing.assert_allclose(e[0], 1.5, rtol=1e-12)
+    np.testing.assert_allclose(e[1], -2.5, rtol=1e-12)
+    np.testing.assert_allclose(e[2], -4.0, rtol=1e-12)
+
+
 # ---------------------------------------------------------------------------
 # beam moments (getzmmnt) and emittances
 # ---------------------------------------------------------------------------

@@ -74,6 +74,11 @@ SIGNATURES = {
     "wm_padvnc3d": (C.c_int, [_P, _i64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _d, _d, _d,
                               _d, _d, _d, _d, _i64, _i64, _i64, _d, _d, _d, _d, _d, _d, _P,
                               _i64, _P]),
+    "wm_padvnc3d_selfe": (C.c_int, [_P, _i64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _d, _d,
+                                    _d, _d, _d, _d, _d, _i64, _i64, _i64, _d, _d, _d, _d, _d,
+                                    _d, _P, _i64, _P]),
+    "wm_sete3d_from_e": (C.c_int, [_P, _i64, _P, _P, _P, _d, _d, _d, _d, _d, _d, _d, _i64,
+                                   _i64, _i64, _P, _P, _P, _P]),
     "wm_getzmmnt_worksize": (_i64, [_i64]),
     "wm_getzmmnt": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _P, _i64, _P, _P, _P]),
     "wm_unet_conv3": (C.c_int, [_P, _i64, _P, _P, _i64, _i64, _i64, _i64, _P, _P, _d, _P]),

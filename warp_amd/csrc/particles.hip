@@ -274,6 +274,59 @@ __device__ inline void gather_e(const double *__restrict__ phi, const Mesh &m, d
 #undef P
 }
 
+// sete_from_e_linear3d (w3d.F:4362-4460): E at (x, y, z) by trilinear
+// interpolation of the grid field selfe(3, 0:nx, 0:ny, 0:nz) (component
+// fastest, nxguarde = 0).  A scaled position outside [0, n] gets no field
+// (the reference skips the particle).  On the upper face (i = n) the term
+// of weight zero reads selfe at n instead of n+1.
+__device__ inline void gather_e_grid(const double *__restrict__ selfe, const Mesh &m, double x,
+                                     double y, double z, double &ext, double &eyt,
+                                     double &ezt) {
+  double xi = (x - m.xmmin) * m.dxi;
+  double yj = (y - m.ymmin) * m.dyi;
+  double zk = (z - m.zgrid - m.zmmin) * m.dzi;
+  // written as a negation so that a NaN position is rejected too
+  if (!(xi >= 0. && xi <= (double)m.nx && yj >= 0. && yj <= (double)m.ny && zk >= 0. &&
+        zk <= (double)m.nz)) {
+    ext = eyt = ezt = 0.;
+    return;
+  }
+  long long i = (long long)xi, j = (long long)yj, k = (long long)zk;
+  long long i1 = i < m.nx ? i + 1 : i, j1 = j < m.ny ? j + 1 : j, k1 = k < m.nz ? k + 1 : k;
+  double u1 = xi - i, v1 = yj - j, w1 = zk - k;
+  double u0 = 1. - u1, v0 = 1. - v1, w0 = 1. - w1;
+  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
+  const double *e000 = selfe + 3 * (i + sx * j + sxy * k), *e100 = selfe + 3 * (i1 + sx * j + sxy * k);
+  const double *e010 = selfe + 3 * (i + sx * j1 + sxy * k), *e110 = selfe + 3 * (i1 + sx * j1 + sxy * k);
+  const double *e001 = selfe + 3 * (i + sx * j + sxy * k1), *e101 = selfe + 3 * (i1 + sx * j + sxy * k1);
+  const double *e011 = selfe + 3 * (i + sx * j1 + sxy * k1), *e111 = selfe + 3 * (i1 + sx * j1 + sxy * k1);
+  double out[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    out[c] = u0 * v0 * w0 * e000[c] + u1 * v0 * w0 * e100[c] + u0 * v1 * w0 * e010[c] +
+             u1 * v1 * w0 * e110[c] + u0 * v0 * w1 * e001[c] + u1 * v0 * w1 * e101[c] +
+             u0 * v1 * w1 * e011[c] + u1 * v1 * w1 * e111[c];
+  ext = out[0];
+  eyt = out[1];
+  ezt = out[2];
+}
+
+__global__ __launch_bounds__(256) void k_sete_from_e(const double *__restrict__ selfe,
+                                                     long long np, const double *__restrict__ xp,
+                                                     const double *__restrict__ yp,
+                                                     const double *__restrict__ zp, Mesh m,
+                                                     double *__restrict__ ex,
+                                                     double *__restrict__ ey,
+                                                     double *__restrict__ ez) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  double a, b, c;
+  gather_e_grid(selfe, m, xp[ip], yp[ip], zp[ip], a, b, c);
+  ex[ip] = ex[ip] + a;
+  ey[ip] = ey[ip] + b;
+  ez[ip] = ez[ip] + c;
+}
+
 __global__ __launch_bounds__(256) void k_sete(const double *__restrict__ phi, long long np,
                                               const double *__restrict__ xp,
                                               const double *__restrict__ yp,
@@ -322,8 +375,10 @@ __global__ __launch_bounds__(256) void k_xpush(long long np, double *__restrict_
 // test of particleboundaries3d) are counted into *nlost.
 // CENTER selects padvnc3d's centring (w3d.F:1848-2330, B = 0, no lattice):
 // 0 "fullv": two half kicks with E(x), then the drift; 1 "halfv": one half
-// kick, then the drift; 2 "synchv": one half kick, no drift.
-template <int CENTER>
+// kick, then the drift; 2 "synchv": one half kick, no drift.  EGRID: E
+// from the precomputed grid field (efetch = 3, `phi` is then selfe) instead
+// of the centred differences of phi (efetch = 1).
+template <int CENTER, bool EGRID = false>
 __global__ __launch_bounds__(256) void k_gather_push(
     const double *__restrict__ phi, long long np, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ z, double *__restrict__ ux, double *__restrict__ uy,
@@ -336,7 +391,10 @@ __global__ __launch_bounds__(256) void k_gather_push(
   if (ip < np) {
     double xi = x[ip], yi = y[ip], zi = z[ip];
     double a, b, c;
-    gather_e(phi, m, tdxi, tdyi, tdzi, xi, yi, zi, a, b, c);
+    if (EGRID)
+      gather_e_grid(phi, m, xi, yi, zi, a, b, c);
+    else
+      gather_e(phi, m, tdxi, tdyi, tdzi, xi, yi, zi, a, b, c);
     double ea = 0. + a, eb = 0. + b, ec = 0. + c;
     double vx = ux[ip], vy = uy[ip], vz = uz[ip];
     vx = vx + ea * ch;
@@ -698,12 +756,16 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
                      WM_CENTER_FULLV, stream);
 }
 
-int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *zp,
-                double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
-                double *ey, double *ez, double zgrid, double xmmin, double ymmin, double zmmin,
-                double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz, double q,
-                double m, double dt, double xmmax, double ymmax, double zmmax, int64_t *nlost,
-                int64_t center, void *stream) {
+}  // extern "C"
+
+template <bool EGRID>
+static int padvnc3d_launch(const double *phi, int64_t np, double *xp, double *yp, double *zp,
+                           double *uxp, double *uyp, double *uzp, const double *gaminv,
+                           double *ex, double *ey, double *ez, double zgrid, double xmmin,
+                           double ymmin, double zmmin, double dx, double dy, double dz,
+                           int64_t nx, int64_t ny, int64_t nz, double q, double m, double dt,
+                           double xmmax, double ymmax, double zmmax, int64_t *nlost,
+                           int64_t center, void *stream) {
   WM_ENTRY({
     check_grid(nx, ny, nz);
     WM_REQUIRE(center == WM_CENTER_FULLV || center == WM_CENTER_HALFV ||
@@ -721,17 +783,56 @@ int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *z
     double ch = q * halfdt / m;
     unsigned long long *nl = (unsigned long long *)nlost;
     if (center == WM_CENTER_FULLV)
-      hipLaunchKernelGGL(k_gather_push<0>, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
-                         (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi,
-                         tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+      hipLaunchKernelGGL((k_gather_push<0, EGRID>), dim3(nblk(np)), dim3(256), 0,
+                         wm_stream(stream), phi, (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv,
+                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
     else if (center == WM_CENTER_HALFV)
-      hipLaunchKernelGGL(k_gather_push<1>, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
-                         (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi,
-                         tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+      hipLaunchKernelGGL((k_gather_push<1, EGRID>), dim3(nblk(np)), dim3(256), 0,
+                         wm_stream(stream), phi, (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv,
+                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
     else
-      hipLaunchKernelGGL(k_gather_push<2>, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), phi,
-                         (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, me, tdxi,
-                         tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+      hipLaunchKernelGGL((k_gather_push<2, EGRID>), dim3(nblk(np)), dim3(256), 0,
+                         wm_stream(stream), phi, (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv,
+                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+    WM_LAUNCHED();
+  });
+}
+
+extern "C" {
+
+int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *zp,
+                double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                double *ey, double *ez, double zgrid, double xmmin, double ymmin, double zmmin,
+                double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz, double q,
+                double m, double dt, double xmmax, double ymmax, double zmmax, int64_t *nlost,
+                int64_t center, void *stream) {
+  return padvnc3d_launch<false>(phi, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, zgrid,
+                                xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, q, m, dt, xmmax,
+                                ymmax, zmmax, nlost, center, stream);
+}
+
+int wm_padvnc3d_selfe(const double *selfe, int64_t np, double *xp, double *yp, double *zp,
+                      double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                      double *ey, double *ez, double zgrid, double xmmin, double ymmin,
+                      double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
+                      int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
+                      double zmmax, int64_t *nlost, int64_t center, void *stream) {
+  return padvnc3d_launch<true>(selfe, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, zgrid,
+                               xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, q, m, dt, xmmax,
+                               ymmax, zmmax, nlost, center, stream);
+}
+
+int wm_sete3d_from_e(const double *selfe, int64_t np, const double *xp, const double *yp,
+                     const double *zp, double zgrid, double xmmin, double ymmin, double zmmin,
+                     double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz,
+                     double *ex, double *ey, double *ez, void *stream) {
+  WM_ENTRY({
+    check_grid(nx, ny, nz);
+    if (np == 0) return 0;
+    WM_REQUIRE(selfe && xp && yp && zp && ex && ey && ez, "wm_sete3d_from_e: NULL array");
+    Mesh m = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
+    hipLaunchKernelGGL(k_sete_from_e, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), selfe,
+                       (long long)np, xp, yp, zp, m, ex, ey, ez);
     WM_LAUNCHED();
   });
 }

@@ -120,6 +120,15 @@ class MultiGrid3D(FieldSolver):
         self.calcselfe(lzero=lzero)
         return self._selfe
 
+    def _gridfield(self):
+        """selfe of the current phi (recomputed after each solve) for the
+        efetch=3 gathers."""
+        key = len(self.mgiters_history)
+        if self._selfe is None or getattr(self, "_selfe_key", None) != key:
+            self.calcselfe(lzero=True)
+            self._selfe_key = key
+        return self._selfe
+
     def calcselfe(self, recalculate=None, lzero=True):
         """getselfe3d (w3d.F:6023-6109): E = -grad phi by centred
         differences, stored (nz+1, ny+1, nx+1, 3)."""
@@ -285,6 +294,10 @@ class MultiGrid3D(FieldSolver):
         self.mgiters_history.append(self.mgiters)
         self.mgerror = err.value
         f3d.mgiters, f3d.mgerror = self.mgiters, self.mgerror
+        if top.efetch == 3:
+            # the grid field the efetch=3 gather interpolates, computed once
+            # per solve (setfieldpforparticles -> calcselfep, multigrid.py:617-629)
+            self._gridfield()
         mgverbose = self.getmgverbose()
         if mgverbose >= 1 or (mgverbose >= 0 and self.mgerror > self.mgtol):
             if self.mgerror > self.mgtol:
@@ -328,14 +341,25 @@ class MultiGrid3D(FieldSolver):
         self.timers["fetche"] += time.perf_counter() - t0
 
     def fetchfieldfrompositions(self, x, y, z, ex, ey, ez, *args, **kw):
-        """sete3d -> sete_from_phi_linear3d (w3d.F:3656-3777, 3963-4107):
-        adds E at the positions to ex, ey, ez."""
+        """sete3d (w3d.F:3656-3777): adds E at the positions to ex, ey, ez,
+        from the centred differences of phi (sete_from_phi_linear3d,
+        3963-4107) or, with top.efetch == 3, by interpolation of the grid
+        field selfe (sete_from_e_linear3d, 4362-4460)."""
         n = int(x.numel())
         if n == 0:
             return
         for t in (x, y, z, ex, ey, ez):
             if not t.is_contiguous():
                 raise ValueError("fetche arrays must be contiguous")
+        if top.efetch not in (1, 3):
+            raise NotImplementedError("efetch=%s: only 1 (from phi) and 3 (from the grid E)"
+                                      % top.efetch)
+        if top.efetch == 3:
+            N.call("wm_sete3d_from_e", self._gridfield().data_ptr(), n, x.data_ptr(), y.data_ptr(),
+                   z.data_ptr(), top.zgrid, self.xmmin, self.ymmin, self.zmmin, self.dx,
+                   self.dy, self.dz, self.nx, self.ny, self.nz, ex.data_ptr(), ey.data_ptr(),
+                   ez.data_ptr(), N.stream_of(self.device))
+            return
         N.call("wm_sete3d", self._phi.data_ptr(), n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
                top.zgrid, self.xmmin, self.ymmin, self.zmmin, self.dx, self.dy, self.dz,
                self.nx, self.ny, self.nz, ex.data_ptr(), ey.data_ptr(), ez.data_ptr(),
@@ -385,7 +409,8 @@ class MultiGrid3D(FieldSolver):
     def gather_push(self, pg, js, dt, center="fullv"):
         """fetche and padvnc3d's kicks and drift for species js in one pass
         (B = 0, w3d.F:1575-2330): "fullv" epush(dt/2) twice + xpush(dt),
-        "halfv" epush(dt/2) + xpush(dt), "synchv" epush(dt/2)."""
+        "halfv" epush(dt/2) + xpush(dt), "synchv" epush(dt/2).  E as
+        fetchfieldfrompositions takes it (top.efetch 1 or 3)."""
         sl = pg.species_slice(js)
         n = sl.stop - sl.start
         if n == 0:
@@ -394,7 +419,14 @@ class MultiGrid3D(FieldSolver):
         t0 = time.perf_counter()
         if self._nlost is None:
             self._nlost = torch.zeros(1, dtype=torch.int64, device=self.device)
-        N.call("wm_padvnc3d", self._phi.data_ptr(), n, col("xp").data_ptr(),
+        if top.efetch not in (1, 3):
+            raise NotImplementedError("efetch=%s: only 1 (from phi) and 3 (from the grid E)"
+                                      % top.efetch)
+        if top.efetch == 3:
+            fn, grid = "wm_padvnc3d_selfe", self._gridfield()
+        else:
+            fn, grid = "wm_padvnc3d", self._phi
+        N.call(fn, grid.data_ptr(), n, col("xp").data_ptr(),
                col("yp").data_ptr(), col("zp").data_ptr(), col("uxp").data_ptr(),
                col("uyp").data_ptr(), col("uzp").data_ptr(), col("gaminv").data_ptr(),
                col("ex").data_ptr(), col("ey").data_ptr(), col("ez").data_ptr(), top.zgrid,
