@@ -549,12 +549,18 @@ __global__ __launch_bounds__(256) void k_zmmnt_partial(
   }
 }
 
-__global__ void k_zmmnt_final(const double *__restrict__ part, int nb, double *__restrict__ mm) {
-  int k = threadIdx.x;
-  if (k >= NZMMNT) return;
-  double s = 0.;
-  for (int b = 0; b < nb; b++) s += part[(long long)b * NZMMNT + k];
-  mm[k] = s;
+// Sum of the block partials, one wavefront per moment: lane l adds blocks
+// l, l+64, ... in order, then the 64 lane sums are combined by a fixed
+// butterfly, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_zmmnt_final(const double *__restrict__ part, int nb,
+                                                     double *__restrict__ mm) {
+  const int lane = threadIdx.x & 63;
+  for (int k = threadIdx.x >> 6; k < NZMMNT; k += 4) {
+    double s = 0.;
+    for (int b = lane; b < nb; b += 64) s += part[(long long)b * NZMMNT + k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) mm[k] = s;
+  }
 }
 
 static int zmmnt_blocks(long long np) {
@@ -855,7 +861,7 @@ int wm_getzmmnt(int64_t np, const double *xp, const double *yp, const double *zp
     hipLaunchKernelGGL(k_zmmnt_partial, dim3(nb), dim3(256), 0, s, (long long)np, xp, yp, zp, uxp,
                        uyp, uzp, gaminv, (int)usegaminv, part);
     WM_LAUNCHED();
-    hipLaunchKernelGGL(k_zmmnt_final, dim3(1), dim3(64), 0, s, (const double *)part, nb, mm);
+    hipLaunchKernelGGL(k_zmmnt_final, dim3(1), dim3(256), 0, s, (const double *)part, nb, mm);
     WM_LAUNCHED();
   });
 }
