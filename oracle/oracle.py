@@ -2,7 +2,7 @@
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 import this module.  It wraps liboracle.so (oc_mgrid.c, oc_particles.c,
-oc_conductors.c), a loop-for-loop C restatement of the reference's Fortran
+oc_conductors.c, oc_correctede.c), a loop-for-loop C restatement of the reference's Fortran
 (pywarp90/f3d_mgrid.F, w3d.F, f3d_conductors.F), and restates the Python
 geometry bookkeeping of scripts/generateconductors.py (Grid.getmeshnew,
 installconductor) that feeds it.
@@ -110,6 +110,11 @@ def _declare(L):
     L.oc_sete_from_e3d.argtypes = [P, _i64, P, P, P, _d, _d, _d, _d, _d, _d, _d, _i64,
                                    _i64, _i64, P, P, P]
     L.oc_getzmmnt.argtypes = [_i64, P, P, P, P, P, P, P, _i64, P]
+    L.oc_setup_icgrid.argtypes = [_i64, _i64, _i64, _i64, P, _i64, P, _i64, P, P]
+    L.oc_efield_subgrid.argtypes = [P, _i64, _i64, _i64, _d, _d, _d, P, _i64, P, P, P, P, P]
+    L.oc_fixefield.argtypes = [P, _i64, _i64, _i64, _i64, P]
+    L.oc_sete3d_conductor.argtypes = [P, P, P, P, P, P, P, _i64, P, P, P, _d, _d, _d, _d, _d,
+                                      _d, _d, _i64, _i64, _i64, _i64, P, P, P]
     L.oc_conductordelfromintercepts.restype = _i64
     L.oc_conductordelfromintercepts.argtypes = [P, P, _d, _d, _i64, P]
     L.oc_zcylinder_intercepts.argtypes = [_d, _d, _d, _d, _d, P, _d, _i64]
@@ -553,3 +558,79 @@ def applyboundaryconditions3d(u, bounds, lwithdirichlet=False,
     b = np.array(bounds, np.int64)
     lib().oc_applyboundaryconditions3d(nx, ny, nz, 1, 1, 1, _p(u), _p(b),
                                        int(lwithdirichlet), int(lzerodirichlet))
+
+
+# ---------------------------------------------------------------------------
+# Conductor-corrected E field (f3d.lcorrectede)
+# ---------------------------------------------------------------------------
+OC_INTERIOR = 2147483647
+
+
+class CondField:
+    """setupconductorfielddata (f3d_mgrid.F:4581-4633) for the finest level:
+    the node classification icgrid (nz+1, ny+1, nx+1) and the per-record
+    E arrays of both parity lists, from a lists() dict of Conductors."""
+
+    def __init__(self, lists, nx, ny, nz):
+        self.nx, self.ny, self.nz = nx, ny, nz
+
+        def lev0(key, start):
+            a = np.ascontiguousarray(lists[key], dtype=np.int64 if "indx" in key else np.float64)
+            st = lists[start]
+            return a[st[0]:st[1]] if len(st) > 1 else a[:0]
+        self.int_indx = lev0("int_indx", "int_istart").reshape(-1, 3).copy()
+        self.sub = []
+        for p in range(2):
+            self.sub.append(dict(indx=lev0("sub%d_indx" % p, "sub%d_istart" % p).reshape(-1, 3).copy(),
+                                 dels=lev0("sub%d_dels" % p, "sub%d_istart" % p).reshape(-1, 6).copy(),
+                                 volt=lev0("sub%d_volt" % p, "sub%d_istart" % p).reshape(-1, 6).copy()))
+        # the reference skips the subgrid E when either whole list (all
+        # levels) is empty (f3d_mgrid.F:4659-4661)
+        self.skip = len(lists["sub0_indx"]) == 0 or len(lists["sub1_indx"]) == 0
+        self.icgrid = np.zeros((nz + 1, ny + 1, nx + 1), np.int32)
+        e, o = self.sub
+        lib().oc_setup_icgrid(nx, ny, nz, len(self.int_indx), _p(self.int_indx), len(e["indx"]),
+                              _p(e["indx"]), len(o["indx"]), _p(o["indx"]), _p(self.icgrid))
+        for d in self.sub:
+            n = len(d["indx"])
+            d["efield0"] = np.zeros((n, 3))
+            d["efieldd"] = np.zeros((n, 6))
+
+    def efield(self, phi, dx, dy, dz):
+        """getefieldatconductorsubgrid (f3d_mgrid.F:4635-4735) for the
+        potential phi (nz+3, ny+3, nx+3)."""
+        if self.skip:
+            return
+        assert phi.shape == (self.nz + 3, self.ny + 3, self.nx + 3) and phi.flags.c_contiguous
+        for d in self.sub:
+            lib().oc_efield_subgrid(_p(phi), self.nx, self.ny, self.nz, dx, dy, dz,
+                                    _p(self.icgrid), len(d["indx"]), _p(d["indx"]),
+                                    _p(d["dels"]), _p(d["volt"]), _p(d["efield0"]),
+                                    _p(d["efieldd"]))
+
+    def fixefield(self, selfe):
+        """fixefieldatconductorpoints (f3d_mgrid.F:4737-4772) in place on
+        selfe (nz+1, ny+1, nx+1, 3)."""
+        if len(self.int_indx) == 0:
+            return
+        assert selfe.flags.c_contiguous
+        lib().oc_fixefield(_p(selfe), self.nx, self.ny, self.nz, len(self.int_indx),
+                           _p(self.int_indx))
+
+
+def sete3d_conductor(cf, phi, selfe, xp, yp, zp, ex, ey, ez, dx, dy, dz, xmmin, ymmin, zmmin,
+                     efetch=1, zgrid=0.0):
+    """sete3dwithconductor (f3d_mgrid.F:4774-5010, 5164-5233) for efetch 1
+    (from phi) or 3 (from selfe): E added to ex, ey, ez."""
+    if efetch not in (1, 3):
+        raise ValueError("efetch must be 1 or 3")
+    xp, yp, zp = _f64(xp), _f64(yp), _f64(zp)
+    e, o = cf.sub
+    z = np.zeros(1)
+    sp = _p(np.ascontiguousarray(selfe)) if selfe is not None else _p(z)
+    if efetch == 3 and selfe is None:
+        raise ValueError("efetch 3 needs selfe")
+    lib().oc_sete3d_conductor(_p(cf.icgrid), _p(e["efield0"]), _p(e["efieldd"]),
+                              _p(o["efield0"]), _p(o["efieldd"]), _p(phi), sp, len(xp), _p(xp),
+                              _p(yp), _p(zp), zgrid, xmmin, ymmin, zmmin, dx, dy, dz, cf.nx,
+                              cf.ny, cf.nz, efetch, _p(ex), _p(ey), _p(ez))

@@ -146,6 +146,21 @@ void oc_sete_from_e3d(const double *selfe, int64_t np, const double *xp,
                       double xmmin, double ymmin, double zmmin, double dx,
                       double dy, double dz, int64_t nx, int64_t ny, int64_t nz,
                       double *ex, double *ey, double *ez);
+void oc_setup_icgrid(int64_t nx, int64_t ny, int64_t nz, int64_t nint,
+                     const int64_t *int_indx, int64_t ne, const int64_t *e_indx,
+                     int64_t no, const int64_t *o_indx, int32_t *icgrid);
+void oc_efield_subgrid(const double *phi, int64_t nx, int64_t ny, int64_t nz, double dx,
+                       double dy, double dz, const int32_t *icgrid, int64_t n,
+                       const int64_t *indx, const double *dels, const double *volt,
+                       double *efield0, double *efieldd);
+void oc_fixefield(double *selfe, int64_t nx, int64_t ny, int64_t nz, int64_t nint,
+                  const int64_t *int_indx);
+void oc_sete3d_conductor(const int32_t *icgrid, const double *e0_even, const double *ed_even,
+                         const double *e0_odd, const double *ed_odd, const double *phi,
+                         const double *selfe, int64_t np, const double *xp, const double *yp,
+                         const double *zp, double zgrid, double xmmin, double ymmin,
+                         double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
+                         int64_t nz, int64_t efetch, double *ex, double *ey, double *ez);
 void oc_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
                    double *selfe, double dx, double dy, double dz,
                    int64_t lzero);

@@ -88,3 +88,86 @@ def pipe_and_annulus(mod):
     return [mod.ZCylinderOut(PIPE_R, PIPE_LEN, 0., zcent=0.04, condid=1),
             mod.ZAnnulus(ANN_RIN, ANN_ROUT, ANN_Z1 - ANN_Z0, ANN_V,
                          zcent=0.5 * (ANN_Z0 + ANN_Z1), condid=2)]
+
+
+def slab_lists(n3, axis, side, ic, frac, V, dfill=2.0):
+    """Conductor lists (level 0 only) of a slab at voltage V normal to
+    `axis` on a grid of n3 = (nx, ny, nz) cells.  side = -1: the slab fills
+    s <= (ic - frac) ds, covering the nodes of index < ic; side = +1: it
+    fills s >= (ic + frac) ds, covering the nodes of index > ic.  The nodes
+    of index ic are subgrid points whose neighbour across the surface is
+    covered (dels = frac there, dfill in the five open directions)."""
+    nx, ny, nz = n3
+    nodes = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                                 indexing="ij"), -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    s = nodes[:, axis]
+    cov = s < ic if side < 0 else s > ic
+    out = dict(levels=1, int_indx=np.ascontiguousarray(nodes[cov]).astype(np.int64),
+               int_volt=np.full(int(cov.sum()), float(V)),
+               int_istart=np.array([0, int(cov.sum())], np.int64))
+    k = 2 * axis + (0 if side < 0 else 1)
+    for p in range(2):
+        pts = nodes[(s == ic) & (nodes.sum(1) % 2 == p)].astype(np.int64)
+        dels = np.full((len(pts), 6), dfill)
+        dels[:, k] = frac
+        volt = np.zeros((len(pts), 6))
+        volt[:, k] = V
+        out["sub%d_indx" % p] = np.ascontiguousarray(pts)
+        out["sub%d_dels" % p] = dels
+        out["sub%d_volt" % p] = volt
+        out["sub%d_istart" % p] = np.array([0, len(pts)], np.int64)
+    return out
+
+
+def slab_surface(axis, side, ic, frac, d3):
+    """Position of the slab's surface along its axis."""
+    return (ic + side * frac) * d3[axis]
+
+
+def slab_f(t, coef):
+    g, h, q = coef
+    return g + h * t + q * t * t
+
+
+def slab_phi(n3, d3, axis, side, ic, frac, V, coef):
+    """phi (nz+3, ny+3, nx+3) of the slab: V on the covered nodes, and
+    V + (s - s_c) f(t) elsewhere, with s along the axis, s_c the surface,
+    t the next axis' coordinate and f(t) = g + h t + q t^2."""
+    nx, ny, nz = n3
+    c = [np.arange(-1, n + 2) for n in n3]
+    I = np.meshgrid(*c, indexing="ij")          # (nx+3, ny+3, nz+3)
+    si = I[axis]
+    s = si * d3[axis]
+    ta = (axis + 1) % 3
+    t = I[ta] * d3[ta]
+    sc = slab_surface(axis, side, ic, frac, d3)
+    cov = si < ic if side < 0 else si > ic
+    phi = np.where(cov, float(V), V + (s - sc) * slab_f(t, coef))
+    return np.ascontiguousarray(phi.transpose(2, 1, 0))
+
+
+def lists_struct(d):
+    """A WmConductorLists over the arrays of a lists() dict (kept alive on
+    the returned struct)."""
+    import ctypes as C
+    from warp_amd import _native as N
+    v = N.WmConductorLists()
+    keep = []
+
+    def P(a, t):
+        a = np.ascontiguousarray(a, dtype=np.int64 if t is C.c_int64 else np.float64).reshape(-1)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(t))
+    v.levels = int(d["levels"])
+    v.int_n = len(d["int_indx"])
+    v.int_indx = P(d["int_indx"], C.c_int64)
+    v.int_volt = P(d["int_volt"], C.c_double)
+    v.int_istart = P(d["int_istart"], C.c_int64)
+    for p in range(2):
+        v.sub_n[p] = len(d["sub%d_indx" % p])
+        v.sub_indx[p] = P(d["sub%d_indx" % p], C.c_int64)
+        v.sub_dels[p] = P(d["sub%d_dels" % p], C.c_double)
+        v.sub_volt[p] = P(d["sub%d_volt" % p], C.c_double)
+        v.sub_istart[p] = P(d["sub%d_istart" % p], C.c_int64)
+    v._keep = keep
+    return v

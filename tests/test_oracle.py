@@ -592,3 +592,186 @@ def test_emittance_closed_forms():
     assert d2["epsx"] < 1e-6 * d["epsx"]   # sqrt of a rounding-level difference
     # normalized: epsnx = 4 sqrt(<x^2><vx^2> - <x vx>^2) / c * 1e6 = epsx * uz / c * 1e6
     assert d["epsnx"] == pytest.approx(d["epsx"] * uz / 2.99792458e8 * 1e6, rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# conductor-corrected E (lcorrectede)
+# ---------------------------------------------------------------------------
+SLAB_N3, SLAB_D3 = (9, 7, 8), (0.1, 0.2, 0.15)
+SLAB_COEF = (-7.5, 4.0, 9.0)
+
+
+def slab_case(axis, side, frac, efetch, ic=4, V=3.0, n=3000, seed=0):
+    """Corrected E of random particles near a slab (oracle); returns
+    (CondField, phi, selfe, positions, E) for the checks."""
+    n3, d3 = SLAB_N3, SLAB_D3
+    d = H.slab_lists(n3, axis, side, ic, frac, V)
+    phi = H.slab_phi(n3, d3, axis, side, ic, frac, V, SLAB_COEF)
+    cf = O.CondField(d, *n3)
+    cf.efield(phi, *d3)
+    selfe = None
+    if efetch == 3:
+        selfe = O.getselfe3d(phi, *d3)
+        cf.fixefield(selfe)
+    rng = np.random.default_rng(seed + 10 * axis + (side > 0))
+    pos = [rng.uniform(0, n3[a] * d3[a], n) for a in range(3)]
+    e = [np.zeros(n) for _ in range(3)]
+    O.sete3d_conductor(cf, phi, selfe, *pos, *e, *d3, 0., 0., 0., efetch=efetch)
+    return cf, phi, selfe, pos, e
+
+
+def slab_expected(axis, side, frac, pos, ic=4):
+    """The field the corrected gather must give near the slab of
+    slab_case: with phi = V + (s - s_c) f(t) on the vacuum nodes, the
+    centred differences there and the one-sided subgrid E (and its
+    extrapolation into the conductor) are the exact nodal field
+    E_s = -f(t_j), E_t = -(s - s_c) f'(t_j), E_r = 0, so the gather is the
+    interpolation of the exact nodal field: linear in t for E_s; for E_t,
+    bilinear in (s, t) in vacuum cells and, in the cell cut by the
+    surface, the vacuum face's value -(s_ic - s_c) f'(t).  Cells whose
+    nodes are all covered get E = 0."""
+    n3, d3 = SLAB_N3, SLAB_D3
+    ta, ra = (axis + 1) % 3, (axis + 2) % 3
+    g, h, q = SLAB_COEF
+    s, t = pos[axis], pos[ta]
+    si = np.floor(s / d3[axis]).astype(int)
+    tj = np.floor(t / d3[ta]).astype(int)
+    v = t / d3[ta] - tj
+    t0, t1 = tj * d3[ta], (tj + 1) * d3[ta]
+    sc = H.slab_surface(axis, side, ic, frac, d3)
+    fp = lambda tt: h + 2 * q * tt
+    es = -((1 - v) * H.slab_f(t0, SLAB_COEF) + v * H.slab_f(t1, SLAB_COEF))
+    et = -(s - sc) * ((1 - v) * fp(t0) + v * fp(t1))
+    if side < 0:
+        cut, inside = si == ic - 1, si < ic - 1
+    else:
+        cut, inside = si == ic, si > ic
+    et = np.where(cut, -(ic * d3[axis] - sc) * ((1 - v) * fp(t0) + v * fp(t1)), et)
+    es = np.where(inside, 0., es)
+    et = np.where(inside, 0., et)
+    out = [None] * 3
+    out[axis], out[ta], out[ra] = es, et, np.zeros_like(es)
+    return out, cut, inside
+
+
+@pytest.mark.parametrize("efetch", [1, 3])
+@pytest.mark.parametrize("frac", [0.4, 0.85, 1.0])
+@pytest.mark.parametrize("axis,side", [(0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1)])
+def test_correctede_slab_exact(axis, side, frac, efetch):
+    """A slab at V normal to each axis, on either side, with
+    phi = V + (s - s_c) f(t) beyond it: the one-sided subgrid E (efield0)
+    and its extrapolation into the conductor (efieldd) are the exact field
+    at the subgrid nodes, and every particle gets the interpolation of the
+    exact nodal field, including in the cell cut by the surface, where
+    centred differences of phi are wrong; cells with only covered nodes get
+    E = 0."""
+    cf, phi, selfe, pos, e = slab_case(axis, side, frac, efetch)
+    want, cut, inside = slab_expected(axis, side, frac, pos)
+    ta = (axis + 1) % 3
+    k = 2 * axis + (0 if side < 0 else 1)
+    for d in cf.sub:
+        tn = d["indx"][:, ta] * SLAB_D3[ta]
+        np.testing.assert_allclose(d["efield0"][:, axis], -H.slab_f(tn, SLAB_COEF), rtol=1e-12)
+        np.testing.assert_allclose(d["efieldd"][:, k], -H.slab_f(tn, SLAB_COEF), rtol=1e-12)
+        others = [c for c in range(6) if c != k]
+        assert np.all(d["efieldd"][:, others] == 0.)   # only one side is covered
+    assert cut.sum() > 100 and inside.sum() > 100
+    for c in range(3):
+        scale = np.abs(want[c]).max() + 1e-300
+        np.testing.assert_allclose(e[c], want[c], rtol=0, atol=1e-12 * scale)
+        assert np.all(e[c][inside] == 0.)
+
+
+def test_correctede_fixes_what_plain_gather_gets_wrong():
+    """The same slab through the plain gather: the cell cut by the surface
+    gets the wrong field (the check above is not vacuous)."""
+    axis, side, frac, ic = 0, -1, 0.4, 4
+    phi = H.slab_phi(SLAB_N3, SLAB_D3, axis, side, ic, frac, 3.0, SLAB_COEF)
+    _, _, _, pos, _ = slab_case(axis, side, frac, 1)
+    want, cut, _ = slab_expected(axis, side, frac, pos)
+    e = [np.zeros(int(cut.sum())) for _ in range(3)]
+    inmesh = [p[cut] for p in pos]
+    O.sete3d(phi, *inmesh, *e, *SLAB_D3, 0., 0., 0.)
+    assert np.abs(e[0] - want[0][cut]).max() > 0.1 * np.abs(want[0]).max()
+
+
+def test_correctede_vacuum_is_the_plain_gather():
+    """With no conductors every cell takes the vacuum path: efetch 3 is
+    bit-for-bit sete_from_e_linear3d; efetch 1 differs from
+    sete_from_phi_linear3d only by where 1/(2 dx) multiplies (per term in
+    sete3dwithconductor, once per sum in sete3d)."""
+    nx, ny, nz, dx, dy, dz = 6, 5, 7, 0.1, 0.12, 0.09
+    rng = np.random.default_rng(8)
+    phi = rng.standard_normal((nz + 3, ny + 3, nx + 3))
+    empty = dict(levels=1, int_indx=np.zeros((0, 3), np.int64), int_volt=np.zeros(0),
+                 int_istart=np.zeros(2, np.int64))
+    for p in range(2):
+        empty["sub%d_indx" % p] = np.zeros((0, 3), np.int64)
+        empty["sub%d_dels" % p] = np.zeros((0, 6))
+        empty["sub%d_volt" % p] = np.zeros((0, 6))
+        empty["sub%d_istart" % p] = np.zeros(2, np.int64)
+    cf = O.CondField(empty, nx, ny, nz)
+    assert not cf.icgrid.any()
+    n = 500
+    x = rng.uniform(0, (nx - 1e-9) * dx, n)
+    y = rng.uniform(0, (ny - 1e-9) * dy, n)
+    z = rng.uniform(0, (nz - 1e-9) * dz, n)
+    se = O.getselfe3d(phi, dx, dy, dz)
+    a = [np.zeros(n) for _ in range(3)]
+    b = [np.zeros(n) for _ in range(3)]
+    O.sete3d_conductor(cf, phi, se, x, y, z, *a, dx, dy, dz, 0., 0., 0., efetch=3)
+    O.sete_from_e3d(se, x, y, z, *b, dx, dy, dz, 0., 0., 0.)
+    for c in range(3):
+        assert np.array_equal(a[c], b[c])
+    a = [np.zeros(n) for _ in range(3)]
+    b = [np.zeros(n) for _ in range(3)]
+    O.sete3d_conductor(cf, phi, None, x, y, z, *a, dx, dy, dz, 0., 0., 0., efetch=1)
+    O.sete3d(phi, x, y, z, *b, dx, dy, dz, 0., 0., 0.)
+    for c in range(3):
+        np.testing.assert_allclose(a[c], b[c], rtol=1e-13, atol=1e-13 * np.abs(b[c]).max())
+
+
+def test_icgrid_rules():
+    """setupconductorfielddata: interior points marked above every record
+    index, even records +(i+1), odd -(i+1), later lists overwrite."""
+    d = H.slab_lists((6, 3, 3), 0, -1, 2, 0.5, 1.0)
+    # put the first even record's node into the interior list too: the even
+    # list, written later, wins
+    d["int_indx"] = np.vstack([d["int_indx"], d["sub0_indx"][:1]])
+    d["int_volt"] = np.append(d["int_volt"], 1.0)
+    d["int_istart"] = np.array([0, len(d["int_volt"])], np.int64)
+    cf = O.CondField(d, 6, 3, 3)
+    ic = cf.icgrid
+    assert np.all(ic[:, :, :2] == O.OC_INTERIOR)
+    for p, sgn in ((0, 1), (1, -1)):
+        for i, (x, y, z) in enumerate(d["sub%d_indx" % p]):
+            assert ic[z, y, x] == sgn * (i + 1)
+    assert np.all(ic[:, :, 3:] == 0)
+
+
+def test_correctede_empty_parity_list_skips_subgrid_field():
+    """getefieldatconductorsubgrid returns at once when either parity list
+    is empty (f3d_mgrid.F:4659-4661): the other list's E stays zero."""
+    d = H.slab_lists((6, 0, 0), 0, -1, 2, 0.5, 1.0)   # a single record, even
+    assert len(d["sub0_indx"]) == 1 and len(d["sub1_indx"]) == 0
+    cf = O.CondField(d, 6, 0, 0)
+    cf.efield(H.slab_phi((6, 0, 0), (0.1,) * 3, 0, -1, 2, 0.5, 1.0, (2., 0., 0.)), 0.1, 0.1, 0.1)
+    assert not cf.sub[0]["efield0"].any()
+
+
+def test_fixefield_mesh_aligned_face():
+    """A conductor face on a mesh plane (no subgrid points): the centred
+    grid E at the face node is half the vacuum field; doubling it at the
+    interior points restores -g there and leaves zero deeper inside."""
+    nx, ny, nz, dx, V, g, ic = 8, 3, 3, 0.1, 2.0, 5.0, 3
+    d = H.slab_lists((nx, ny, nz), 0, -1, ic + 1, 1.0, V)     # covered nodes ix <= ic
+    x = (np.arange(-1, nx + 2) - ic) * dx
+    phi = np.ascontiguousarray(np.broadcast_to(np.where(x <= 0, V, V + g * x),
+                                               (nz + 3, ny + 3, nx + 3)))
+    se = O.getselfe3d(phi, 0.1, 0.1, 0.1)
+    np.testing.assert_allclose(se[:, :, ic, 0], -g / 2, rtol=1e-12)
+    cf = O.CondField(d, nx, ny, nz)
+    cf.fixefield(se)
+    np.testing.assert_allclose(se[:, :, ic, 0], -g, rtol=1e-12)
+    assert np.all(se[:, :, :ic, 0] == 0.)
+    np.testing.assert_allclose(se[:, :, ic + 1:nx, 0], -g, rtol=1e-12)
