@@ -232,6 +232,65 @@ int wm_sete3d_from_e(const double *selfe, int64_t np, const double *xp, const do
                      double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz,
                      double *ex, double *ey, double *ez, void *stream);
 
+/* Conductor-corrected E field (f3d.lcorrectede, pywarp90/f3d.v:152) on the
+ * finest level: a handle holding the conductor field data of
+ * ConductorType (icgrid, evensubgrid/oddsubgrid efield0 and efieldd,
+ * pywarp90/f3d.v:105-164).
+ *
+ * setupconductorfielddata (f3d_mgrid.F:4581-4633): classifies the nodes
+ * (0:nx, 0:ny, 0:nz) from the level-0 part of the lists (interior nodes,
+ * then the even, then the odd subgrid points; a later list overwrites an
+ * earlier one).  Host work plus uploads; call again when the conductors
+ * change. */
+typedef struct wm_condfield wm_condfield;
+int wm_condfield_create(wm_condfield **out, int64_t nx, int64_t ny, int64_t nz,
+                        const wm_conductor_lists *c);
+int wm_condfield_destroy(wm_condfield *cf);
+
+/* getefieldatconductorsubgrid (f3d_mgrid.F:4635-4735): the one-sided E at
+ * every level-0 subgrid point from phi (-1:nx+1)^3 (dx, dy, dz of the
+ * finest level), and its extrapolation into the conductor where the
+ * neighbour is an interior node.  Does nothing when either parity list is
+ * empty, as the reference. */
+int wm_condfield_efield(wm_condfield *cf, const double *phi, double dx, double dy,
+                        double dz, void *stream);
+
+/* fixefieldatconductorpoints (f3d_mgrid.F:4737-4772): doubles selfe
+ * (3, 0:nx, 0:ny, 0:nz) at the level-0 interior nodes (once per listing). */
+int wm_condfield_fixefield(const wm_condfield *cf, double *selfe, void *stream);
+
+/* The handle's arrays, for inspection: nsub[2] gets the record counts of
+ * the even and odd lists; each non-NULL destination (device or host
+ * memory) gets a copy of icgrid (int32, (nx+1)(ny+1)(nz+1), x fastest: 0
+ * vacuum, 2147483647 interior, +(r+1) even record r, -(r+1) odd record
+ * r), efield0 (3 per record) and efieldd (6 per record) of the even and
+ * odd lists.  Returns after the copies are done. */
+int wm_condfield_read(const wm_condfield *cf, int64_t *nsub, int32_t *icgrid,
+                      double *e0_even, double *ed_even, double *e0_odd, double *ed_odd,
+                      void *stream);
+
+/* sete3dwithconductor (f3d_mgrid.F:4774-5010 with GETEEFROME 5164-5233),
+ * efetch 1 (from phi) or 3 (from selfe, as wm_getselfe3d and
+ * wm_condfield_fixefield leave it), no symmetries: adds E to ex, ey, ez.
+ * Cells with a conductor node take the weighted one-sided fields;
+ * particles outside [0, n] on some axis get nothing. */
+int wm_sete3d_conductor(const wm_condfield *cf, const double *phi, const double *selfe,
+                        int64_t efetch, int64_t np, const double *xp, const double *yp,
+                        const double *zp, double zgrid, double xmmin, double ymmin,
+                        double zmmin, double dx, double dy, double dz, double *ex,
+                        double *ey, double *ez, void *stream);
+
+/* wm_padvnc3d with E from sete3dwithconductor (efetch 1 or 3) instead of
+ * sete3d: the advance padvnc3d makes under f3d.lcorrectede
+ * (w3d.F:1575-2330 -> fetche3d -> multigrid.py:528-546). */
+int wm_padvnc3d_conductor(const wm_condfield *cf, const double *phi, const double *selfe,
+                          int64_t efetch, int64_t np, double *xp, double *yp, double *zp,
+                          double *uxp, double *uyp, double *uzp, const double *gaminv,
+                          double *ex, double *ey, double *ez, double zgrid, double xmmin,
+                          double ymmin, double zmmin, double dx, double dy, double dz,
+                          double q, double m, double dt, double xmmax, double ymmax,
+                          double zmmax, int64_t *nlost, int64_t center, void *stream);
+
 /* getzmmnt (top.F:521-1360), itask 2: the whole-beam moment sums
  * zmmnts0(1:33) of np particles into mm (device, 33 doubles, overwritten;
  * mm[k-1] = zmmnts0(k)), summed per block then over blocks in a fixed

@@ -1,7 +1,9 @@
 """The step driver end to end (generate, then step(7) with the reference's
 special-step cadence) on the GPU against the same sequence run with the
 FP64 oracle: deposit, multigrid solve, E gather, halfv/fullv/synchv pushes,
-absorbing boundaries and the beam moments.  The deposit sums in another
+absorbing boundaries and the beam moments; in free space, and with the
+pipe and annulus installed and the conductor-corrected gather
+(f3d.lcorrectede).  The deposit sums in another
 order on the GPU (atomics), so the states agree to round-off, not bit for
 bit: positions and velocities to 1e-10 relative, the moments to 1e-9."""
 import numpy as np
@@ -9,6 +11,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
+from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
@@ -16,19 +19,31 @@ NX, NY, NZ = 32, 32, 64
 DOM = (-0.02, 0.02, -0.02, 0.02, 0.0, 0.08)
 
 
-def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist, efetch=1):
+def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist, efetch=1, lcorrectede=False):
     dx, dy, dz = 0.04 / NX, 0.04 / NY, 0.08 / NZ
     lo = (DOM[0], DOM[2], DOM[4])
     phi = np.zeros((NZ + 3, NY + 3, NX + 3))
+    co = cf = None
+    if lcorrectede:
+        co = O.install_conductors(H.pipe_and_annulus(O), NX, NY, NZ, *DOM)
+        O.checkconductors(co, NX, NY, NZ, dx, dy, dz)
+        cf = O.CondField(co.lists(), NX, NY, NZ)
 
     def solve(x, y, z):
         rho = np.zeros((NZ + 1, NY + 1, NX + 1))
         O.setrho3d(rho, x, y, z, q, w, dx, dy, dz, *lo)
-        O.multigrid3dsolve(phi, rho, dx, dy, dz)
+        O.multigrid3dsolve(phi, rho, dx, dy, dz, conductors=co)
 
     def efield(x, y, z):
         e = [np.zeros(len(x)) for _ in range(3)]
-        if efetch == 3:
+        if lcorrectede:
+            cf.efield(phi, dx, dy, dz)
+            se = None
+            if efetch == 3:
+                se = O.getselfe3d(phi, dx, dy, dz)
+                cf.fixefield(se)
+            O.sete3d_conductor(cf, phi, se, x, y, z, *e, dx, dy, dz, *lo, efetch=efetch)
+        elif efetch == 3:
             O.sete_from_e3d(O.getselfe3d(phi, dx, dy, dz), x, y, z, *e, dx, dy, dz, *lo)
         else:
             O.sete3d(phi, x, y, z, *e, dx, dy, dz, *lo)
@@ -55,11 +70,11 @@ def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist, efetch=1):
     return x, y, z, u, centers
 
 
-@pytest.mark.parametrize("efetch", [1, 3])
-def test_step_driver_matches_oracle(device, efetch):
+@pytest.mark.parametrize("efetch,lcorrectede", [(1, False), (3, False), (1, True), (3, True)])
+def test_step_driver_matches_oracle(device, efetch, lcorrectede):
     import warp_amd
     from warp_amd import diagnostics, stepper
-    from warp_amd.globals import top
+    from warp_amd.globals import f3d, top
     from warp_amd.particles import load_gaussian
     pg, js, n = load_gaussian(20000, (0.004, 0.004, 0.015), 0.04, 1.0e6, 1.0e-3, DOM, 31,
                               device=device)
@@ -68,6 +83,10 @@ def test_step_driver_matches_oracle(device, efetch):
     s = warp_amd.MultiGrid3D(nx=NX, ny=NY, nz=NZ, xmmin=DOM[0], xmmax=DOM[1], ymmin=DOM[2],
                              ymmax=DOM[3], zmmin=DOM[4], zmmax=DOM[5], bounds=[0] * 6,
                              mgverbose=-1)
+    if lcorrectede:
+        for c in H.pipe_and_annulus(warp_amd):
+            s.installconductor(c)
+    saved_lc = f3d.lcorrectede
     saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time, nhist=top.nhist,
                  lspecial=top.lspecial, particlesortinterval=top.particlesortinterval,
                  efetch=top.efetch)
@@ -83,12 +102,13 @@ def test_step_driver_matches_oracle(device, efetch):
         top.pgroup, top.dt, top.it, top.time, top.nhist = pg, 1.0e-9, 0, 0., 5
         top.particlesortinterval = 0
         top.efetch = efetch
+        f3d.lcorrectede = lcorrectede
         warp_amd.registersolver(s)
         stepper.generate()
         stepper.step(7)
         xo, yo, zo, uo, centers = _oracle_run(*[a.copy() for a in x0], [a.copy() for a in u0],
                                               pg.sq[0], pg.sm[0], pg.sw[0], 1.0e-9, 7, 5,
-                                              efetch)
+                                              efetch, lcorrectede)
         assert calls == centers
         assert centers == ["halfv", "fullv", "fullv", "fullv", "fullv", "synchv", "halfv",
                            "fullv", "synchv"]
@@ -102,6 +122,7 @@ def test_step_driver_matches_oracle(device, efetch):
             assert top.moments[key] == pytest.approx(mm[key], rel=1e-9), key
         assert top.it == 7 and top.lspecial
     finally:
+        f3d.lcorrectede = saved_lc
         warp_amd.unregistersolver(s)
         for k, v in saved.items():
             setattr(top, k, v)

@@ -79,6 +79,16 @@ SIGNATURES = {
                                     _d, _P, _i64, _P]),
     "wm_sete3d_from_e": (C.c_int, [_P, _i64, _P, _P, _P, _d, _d, _d, _d, _d, _d, _d, _i64,
                                    _i64, _i64, _P, _P, _P, _P]),
+    "wm_condfield_create": (C.c_int, [_P, _i64, _i64, _i64, _P]),
+    "wm_condfield_destroy": (C.c_int, [_P]),
+    "wm_condfield_efield": (C.c_int, [_P, _P, _d, _d, _d, _P]),
+    "wm_condfield_fixefield": (C.c_int, [_P, _P, _P]),
+    "wm_condfield_read": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "wm_sete3d_conductor": (C.c_int, [_P, _P, _P, _i64, _i64, _P, _P, _P, _d, _d, _d, _d, _d,
+                                      _d, _d, _P, _P, _P, _P]),
+    "wm_padvnc3d_conductor": (C.c_int, [_P, _P, _P, _i64, _i64, _P, _P, _P, _P, _P, _P, _P,
+                                        _P, _P, _P, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d,
+                                        _d, _d, _d, _P, _i64, _P]),
     "wm_getzmmnt_worksize": (_i64, [_i64]),
     "wm_getzmmnt": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _P, _i64, _P, _P, _P]),
     "wm_unet_conv3": (C.c_int, [_P, _i64, _P, _P, _i64, _i64, _i64, _i64, _P, _P, _d, _P]),

@@ -23,33 +23,6 @@
 
 namespace {
 
-template <typename T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void alloc(size_t count) {
-    if (count == n && p) return;
-    release();
-    if (count == 0) return;
-    WM_HIP(hipMalloc((void **)&p, sizeof(T) * count));
-    WM_HIP(hipMemset(p, 0, sizeof(T) * count));
-    n = count;
-  }
-  void upload(const std::vector<T> &v) {
-    alloc(v.size());
-    if (!v.empty()) WM_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-  }
-};
-
 enum { MODE_RAW = SIDE_RAW, MODE_MUL = SIDE_MUL, MODE_DIV = SIDE_DIV, MODE_CONST = SIDE_CONST };
 
 struct Sym {
@@ -563,8 +536,6 @@ static void build_mask(wm_mg3d *h, Level &L, int level, const Params &P) {
     m[e >> 5] |= 1u << (e & 31);
   };
   std::vector<int> ipos;
-  // interior positions are kept in L via int_pos upload; rebuild from host copy
-  for (size_t i = 0; i < L.subpos_h[0].size() && false; i++) (void)i;
   bool any = false;
   if (L.nint > 0) {
     std::vector<int> hp(3 * L.nint);

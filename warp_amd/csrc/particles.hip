@@ -6,6 +6,7 @@
 // indices and every per-particle value are bit-identical to the reference;
 // only the order in which deposits are summed into rho differs.
 #include "wm_internal.h"
+#include "condfield.h"
 #include "../../include/warpmi.h"
 
 #include <hipcub/hipcub.hpp>
@@ -377,24 +378,31 @@ __global__ __launch_bounds__(256) void k_xpush(long long np, double *__restrict_
 // 0 "fullv": two half kicks with E(x), then the drift; 1 "halfv": one half
 // kick, then the drift; 2 "synchv": one half kick, no drift.  EGRID: E
 // from the precomputed grid field (efetch = 3, `phi` is then selfe) instead
-// of the centred differences of phi (efetch = 1).
-template <int CENTER, bool EGRID = false>
+// of the centred differences of phi (efetch = 1); FIELD_COND: E of
+// sete3dwithconductor (f3d.lcorrectede) from the conductor field data in
+// cg, efetch 1 or 3.
+enum { FIELD_PHI = 0, FIELD_GRID = 1, FIELD_COND = 2 };
+
+template <int CENTER, int FIELD = FIELD_PHI>
 __global__ __launch_bounds__(256) void k_gather_push(
     const double *__restrict__ phi, long long np, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ z, double *__restrict__ ux, double *__restrict__ uy,
     double *__restrict__ uz, const double *__restrict__ gaminv, double *__restrict__ ex,
     double *__restrict__ ey, double *__restrict__ ez, Mesh m, double tdxi, double tdyi,
     double tdzi, double ch, double dt, double xmmax, double ymmax, double zmmax,
-    unsigned long long *__restrict__ nlost) {
+    unsigned long long *__restrict__ nlost, CondGather cg) {
   long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
   bool lost = false;
   if (ip < np) {
     double xi = x[ip], yi = y[ip], zi = z[ip];
     double a, b, c;
-    if (EGRID)
+    if (FIELD == FIELD_GRID) {
       gather_e_grid(phi, m, xi, yi, zi, a, b, c);
-    else
+    } else if (FIELD == FIELD_COND) {
+      if (!gather_e_cond(cg, xi, yi, zi, a, b, c)) a = b = c = 0.;
+    } else {
       gather_e(phi, m, tdxi, tdyi, tdzi, xi, yi, zi, a, b, c);
+    }
     double ea = 0. + a, eb = 0. + b, ec = 0. + c;
     double vx = ux[ip], vy = uy[ip], vz = uz[ip];
     vx = vx + ea * ch;
@@ -764,21 +772,22 @@ int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp, doub
 
 }  // extern "C"
 
-template <bool EGRID>
+template <int FIELD>
 static int padvnc3d_launch(const double *phi, int64_t np, double *xp, double *yp, double *zp,
                            double *uxp, double *uyp, double *uzp, const double *gaminv,
                            double *ex, double *ey, double *ez, double zgrid, double xmmin,
                            double ymmin, double zmmin, double dx, double dy, double dz,
                            int64_t nx, int64_t ny, int64_t nz, double q, double m, double dt,
                            double xmmax, double ymmax, double zmmax, int64_t *nlost,
-                           int64_t center, void *stream) {
+                           int64_t center, const CondGather &cg, void *stream) {
   WM_ENTRY({
     check_grid(nx, ny, nz);
     WM_REQUIRE(center == WM_CENTER_FULLV || center == WM_CENTER_HALFV ||
                    center == WM_CENTER_SYNCHV,
                "wm_padvnc3d: center must be WM_CENTER_FULLV, _HALFV or _SYNCHV");
     if (np == 0) return 0;
-    WM_REQUIRE(phi && xp && yp && zp && uxp && uyp && uzp && gaminv, "wm_padvnc3d: NULL array");
+    WM_REQUIRE((phi || FIELD == FIELD_COND) && xp && yp && zp && uxp && uyp && uzp && gaminv,
+               "wm_padvnc3d: NULL array");
     WM_REQUIRE((ex && ey && ez) || (!ex && !ey && !ez), "wm_padvnc3d: give all or none of ex, ey, ez");
     Mesh me = make_mesh(nx, ny, nz, dx, dy, dz, xmmin, ymmin, zmmin, zgrid);
     double tdxi = 1. / (2. * dx), tdyi = 1. / (2. * dy), tdzi = 1. / (2. * dz);
@@ -789,17 +798,17 @@ static int padvnc3d_launch(const double *phi, int64_t np, double *xp, double *yp
     double ch = q * halfdt / m;
     unsigned long long *nl = (unsigned long long *)nlost;
     if (center == WM_CENTER_FULLV)
-      hipLaunchKernelGGL((k_gather_push<0, EGRID>), dim3(nblk(np)), dim3(256), 0,
+      hipLaunchKernelGGL((k_gather_push<0, FIELD>), dim3(nblk(np)), dim3(256), 0,
                          wm_stream(stream), phi, (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv,
-                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl, cg);
     else if (center == WM_CENTER_HALFV)
-      hipLaunchKernelGGL((k_gather_push<1, EGRID>), dim3(nblk(np)), dim3(256), 0,
+      hipLaunchKernelGGL((k_gather_push<1, FIELD>), dim3(nblk(np)), dim3(256), 0,
                          wm_stream(stream), phi, (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv,
-                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl, cg);
     else
-      hipLaunchKernelGGL((k_gather_push<2, EGRID>), dim3(nblk(np)), dim3(256), 0,
+      hipLaunchKernelGGL((k_gather_push<2, FIELD>), dim3(nblk(np)), dim3(256), 0,
                          wm_stream(stream), phi, (long long)np, xp, yp, zp, uxp, uyp, uzp, gaminv,
-                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl);
+                         ex, ey, ez, me, tdxi, tdyi, tdzi, ch, fulldt, xmmax, ymmax, zmmax, nl, cg);
     WM_LAUNCHED();
   });
 }
@@ -812,9 +821,9 @@ int wm_padvnc3d(const double *phi, int64_t np, double *xp, double *yp, double *z
                 double dx, double dy, double dz, int64_t nx, int64_t ny, int64_t nz, double q,
                 double m, double dt, double xmmax, double ymmax, double zmmax, int64_t *nlost,
                 int64_t center, void *stream) {
-  return padvnc3d_launch<false>(phi, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, zgrid,
-                                xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, q, m, dt, xmmax,
-                                ymmax, zmmax, nlost, center, stream);
+  return padvnc3d_launch<FIELD_PHI>(phi, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez,
+                                    zgrid, xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, q, m, dt,
+                                    xmmax, ymmax, zmmax, nlost, center, CondGather{}, stream);
 }
 
 int wm_padvnc3d_selfe(const double *selfe, int64_t np, double *xp, double *yp, double *zp,
@@ -823,9 +832,29 @@ int wm_padvnc3d_selfe(const double *selfe, int64_t np, double *xp, double *yp, d
                       double zmmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
                       int64_t nz, double q, double m, double dt, double xmmax, double ymmax,
                       double zmmax, int64_t *nlost, int64_t center, void *stream) {
-  return padvnc3d_launch<true>(selfe, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez, zgrid,
-                               xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, q, m, dt, xmmax,
-                               ymmax, zmmax, nlost, center, stream);
+  return padvnc3d_launch<FIELD_GRID>(selfe, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez,
+                                     zgrid, xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, q, m, dt,
+                                     xmmax, ymmax, zmmax, nlost, center, CondGather{}, stream);
+}
+
+int wm_padvnc3d_conductor(const wm_condfield *cf, const double *phi, const double *selfe,
+                          int64_t efetch, int64_t np, double *xp, double *yp, double *zp,
+                          double *uxp, double *uyp, double *uzp, const double *gaminv, double *ex,
+                          double *ey, double *ez, double zgrid, double xmmin, double ymmin,
+                          double zmmin, double dx, double dy, double dz, double q, double m,
+                          double dt, double xmmax, double ymmax, double zmmax, int64_t *nlost,
+                          int64_t center, void *stream) {
+  CondGather cg;
+  int rc = [&]() -> int {
+    WM_ENTRY({
+      cg = wm_condfield_gather(cf, phi, selfe, efetch, zgrid, xmmin, ymmin, zmmin, dx, dy, dz);
+    });
+  }();
+  if (rc != 0) return rc;
+  return padvnc3d_launch<FIELD_COND>(phi, np, xp, yp, zp, uxp, uyp, uzp, gaminv, ex, ey, ez,
+                                     zgrid, xmmin, ymmin, zmmin, dx, dy, dz, cg.v.nx, cg.v.ny,
+                                     cg.v.nz, q, m, dt, xmmax, ymmax, zmmax, nlost, center, cg,
+                                     stream);
 }
 
 int wm_sete3d_from_e(const double *selfe, int64_t np, const double *xp, const double *yp,

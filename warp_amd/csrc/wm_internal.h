@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 void wm_set_error(const std::string &msg);
 
@@ -41,6 +42,34 @@ struct wm_error : std::runtime_error {
 static inline hipStream_t wm_stream(void *s) { return (hipStream_t)s; }
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Owning device array (hipMalloc'ed, zero-filled on allocation).
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    WM_HIP(hipMalloc((void **)&p, sizeof(T) * count));
+    WM_HIP(hipMemset(p, 0, sizeof(T) * count));
+    n = count;
+  }
+  void upload(const std::vector<T> &v) {
+    alloc(v.size());
+    if (!v.empty()) WM_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  }
+};
 
 // Launch check: reports the launch error of the kernel just enqueued.
 #define WM_LAUNCHED() WM_HIP(hipGetLastError())

@@ -70,6 +70,8 @@ w3d = W3D(nx=2, ny=2, nz=2, xmmin=0., xmmax=1., ymmin=0., ymmax=1., zmmin=0., zm
 f3d = F3D(gridmode=1, mgparam=1.2, mgmaxiters=100, mgmaxlevels=101, mgtol=1.e-6,
           mgform=1, downpasses=1, uppasses=1, mgverbose=1, mgntverbose=1,
           lcndbndy=True, icndbndy=2, laddconductor=False, lprecalccoeffs=False,
-          mgiters=0, mgerror=0.)
+          mgiters=0, mgerror=0.,
+          # E near conductors from one-sided differences (f3d.v:152-153)
+          lcorrectede=False)
 
 assert math.isclose(eps0, 8.854187817620389e-12, rel_tol=1e-15)

@@ -64,6 +64,9 @@ class MultiGrid3D(FieldSolver):
         self._selfe = None
         self._work = None
         self._nlost = None
+        self._cf = None            # conductor field data for f3d.lcorrectede
+        self._cf_key = None
+        self._cond_gen = 0         # counts uploads of conductor lists
         b = (C.c_int64 * 6)(*self.bounds)
         h = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -75,6 +78,7 @@ class MultiGrid3D(FieldSolver):
 
     def __del__(self):
         try:
+            self._drop_condfield()
             if getattr(self, "_h", None):
                 N.lib().wm_mg3d_destroy(self._h)
                 self._h = None
@@ -123,11 +127,56 @@ class MultiGrid3D(FieldSolver):
     def _gridfield(self):
         """selfe of the current phi (recomputed after each solve) for the
         efetch=3 gathers."""
-        key = len(self.mgiters_history)
+        key = (len(self.mgiters_history), bool(f3d.lcorrectede))
         if self._selfe is None or getattr(self, "_selfe_key", None) != key:
             self.calcselfe(lzero=True)
+            if f3d.lcorrectede:
+                # E at the conductor nodes (multigrid.py:638-646)
+                N.call("wm_condfield_fixefield", self._condfield(), self._selfe.data_ptr(),
+                       N.stream_of(self.device))
             self._selfe_key = key
         return self._selfe
+
+    # ------------------------------------------------------------------
+    # conductor-corrected E (f3d.lcorrectede; multigrid.py:528-546, 602-646)
+    def _drop_condfield(self):
+        cf = getattr(self, "_cf", None)
+        if cf:
+            N.lib().wm_condfield_destroy(cf)
+        self._cf = None
+        self._cf_key = None
+
+    def _condfield(self):
+        """setupconductorfielddata (f3d_mgrid.F:4581-4633): the node
+        classification of the installed conductors on this grid, kept
+        until the conductors change."""
+        if self.conductordatalist:
+            obj = self.getconductorobject()
+        else:
+            obj = None
+        key = self._cond_gen
+        if self._cf is None or self._cf_key != key:
+            self._drop_condfield()
+            view = obj.view() if obj is not None else None
+            h = C.c_void_p()
+            with torch.cuda.device(self.device):
+                N.call("wm_condfield_create", C.byref(h), self.nx, self.ny, self.nz,
+                       C.addressof(view) if view is not None else None)
+            self._cf = h
+            self._cf_key = key
+            self._cf_phi_key = None
+        return self._cf
+
+    def _condfield_efield(self):
+        """getefieldatconductorsubgrid (f3d_mgrid.F:4635-4735) of the
+        current phi, once per solve."""
+        cf = self._condfield()
+        key = len(self.mgiters_history)
+        if getattr(self, "_cf_phi_key", None) != key:
+            N.call("wm_condfield_efield", cf, self._phi.data_ptr(), self.dx, self.dy, self.dz,
+                   N.stream_of(self.device))
+            self._cf_phi_key = key
+        return cf
 
     def calcselfe(self, recalculate=None, lzero=True):
         """getselfe3d (w3d.F:6023-6109): E = -grad phi by centred
@@ -145,6 +194,7 @@ class MultiGrid3D(FieldSolver):
         self.conductordatalist = []
         self._conductors = None
         self._installedconductors = []
+        self._cond_gen = getattr(self, "_cond_gen", 0) + 1
 
     def installconductor(self, conductor, xmin=None, xmax=None, ymin=None, ymax=None,
                          zmin=None, zmax=None, dfill=None):
@@ -164,6 +214,8 @@ class MultiGrid3D(FieldSolver):
         "Clear out the conductor data"
         self._conductors = None
         self._installedconductors = []
+        self._drop_condfield()
+        self._cond_gen += 1
         N.call("wm_mg3d_set_conductors", self._h, None)
 
     def hasconductors(self):
@@ -187,6 +239,7 @@ class MultiGrid3D(FieldSolver):
                 self._installedconductors.append(conductor)
             view = self._conductors.view()
             N.call("wm_mg3d_set_conductors", self._h, C.addressof(view))
+            self._cond_gen += 1
         return self._conductors
 
     # ------------------------------------------------------------------
@@ -294,6 +347,8 @@ class MultiGrid3D(FieldSolver):
         self.mgiters_history.append(self.mgiters)
         self.mgerror = err.value
         f3d.mgiters, f3d.mgerror = self.mgiters, self.mgerror
+        if f3d.lcorrectede:
+            self._condfield_efield()
         if top.efetch == 3:
             # the grid field the efetch=3 gather interpolates, computed once
             # per solve (setfieldpforparticles -> calcselfep, multigrid.py:617-629)
@@ -344,7 +399,8 @@ class MultiGrid3D(FieldSolver):
         """sete3d (w3d.F:3656-3777): adds E at the positions to ex, ey, ez,
         from the centred differences of phi (sete_from_phi_linear3d,
         3963-4107) or, with top.efetch == 3, by interpolation of the grid
-        field selfe (sete_from_e_linear3d, 4362-4460)."""
+        field selfe (sete_from_e_linear3d, 4362-4460); with f3d.lcorrectede
+        by sete3dwithconductor (f3d_mgrid.F:4774-5010), as multigrid.py:528-546."""
         n = int(x.numel())
         if n == 0:
             return
@@ -354,6 +410,14 @@ class MultiGrid3D(FieldSolver):
         if top.efetch not in (1, 3):
             raise NotImplementedError("efetch=%s: only 1 (from phi) and 3 (from the grid E)"
                                       % top.efetch)
+        if f3d.lcorrectede:
+            cf = self._condfield_efield()
+            selfe = self._gridfield().data_ptr() if top.efetch == 3 else None
+            N.call("wm_sete3d_conductor", cf, self._phi.data_ptr(), selfe, top.efetch, n,
+                   x.data_ptr(), y.data_ptr(), z.data_ptr(), top.zgrid, self.xmmin, self.ymmin,
+                   self.zmmin, self.dx, self.dy, self.dz, ex.data_ptr(), ey.data_ptr(),
+                   ez.data_ptr(), N.stream_of(self.device))
+            return
         if top.efetch == 3:
             N.call("wm_sete3d_from_e", self._gridfield().data_ptr(), n, x.data_ptr(), y.data_ptr(),
                    z.data_ptr(), top.zgrid, self.xmmin, self.ymmin, self.zmmin, self.dx,
@@ -410,7 +474,7 @@ class MultiGrid3D(FieldSolver):
         """fetche and padvnc3d's kicks and drift for species js in one pass
         (B = 0, w3d.F:1575-2330): "fullv" epush(dt/2) twice + xpush(dt),
         "halfv" epush(dt/2) + xpush(dt), "synchv" epush(dt/2).  E as
-        fetchfieldfrompositions takes it (top.efetch 1 or 3)."""
+        fetchfieldfrompositions takes it (top.efetch 1 or 3, f3d.lcorrectede)."""
         sl = pg.species_slice(js)
         n = sl.stop - sl.start
         if n == 0:
@@ -422,6 +486,19 @@ class MultiGrid3D(FieldSolver):
         if top.efetch not in (1, 3):
             raise NotImplementedError("efetch=%s: only 1 (from phi) and 3 (from the grid E)"
                                       % top.efetch)
+        if f3d.lcorrectede:
+            cf = self._condfield_efield()
+            selfe = self._gridfield().data_ptr() if top.efetch == 3 else None
+            N.call("wm_padvnc3d_conductor", cf, self._phi.data_ptr(), selfe, top.efetch, n,
+                   col("xp").data_ptr(), col("yp").data_ptr(), col("zp").data_ptr(),
+                   col("uxp").data_ptr(), col("uyp").data_ptr(), col("uzp").data_ptr(),
+                   col("gaminv").data_ptr(), col("ex").data_ptr(), col("ey").data_ptr(),
+                   col("ez").data_ptr(), top.zgrid, self.xmmin, self.ymmin, self.zmmin, self.dx,
+                   self.dy, self.dz, pg.sq[js], pg.sm[js], dt, self.xmmax, self.ymmax,
+                   self.zmmax, self._nlost.data_ptr(), self._CENTERS[center],
+                   N.stream_of(self.device))
+            self.timers["fetche"] += time.perf_counter() - t0
+            return
         if top.efetch == 3:
             fn, grid = "wm_padvnc3d_selfe", self._gridfield()
         else:
