@@ -116,7 +116,7 @@ def main():
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     log("SOR half sweep: %d launches, %.4f ms avg, %.0f GB/s" % (nl.value, avg_ms, achieved))
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "sor_zmarch_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "sor_zmarch_256x256x512_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")

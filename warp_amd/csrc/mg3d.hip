@@ -248,6 +248,8 @@ struct wm_mg3d {
   } gkey;
   int last_iters = 0;
   bool conductors_set = false;
+  int ncu = 256;                 // compute units of the device
+  size_t lds_per_cu = 160 * 1024;
   // Event timing of the finest-level SOR half sweeps (eager mode only).
   struct Prof {
     bool on = false;
@@ -862,6 +864,51 @@ static bool fused_restrict_ok(wm_mg3d *h, const Level &L, bool anyzero) {
   return L.rstd[0] && L.rstd[1] && L.rstd[2];
 }
 
+// Resident blocks per CU of k_resrestrict<S, TR, TC, MASK> by block size
+// (index: wavefronts per block, 1..16; LDS not counted).  Filled by the
+// first call, which wm_mg3d_create makes outside any stream capture.
+template <typename S, typename TR, typename TC, bool MASK>
+static const int *resrestrict_occupancy() {
+  static int tab[17] = {-1};
+  if (tab[0] < 0) {
+    for (int k = 1; k <= 16; k++) {
+      int n = 0;
+      WM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, reinterpret_cast<const void *>(&k_resrestrict<S, TR, TC, MASK>), 64 * k, 0));
+      tab[k] = std::max(n, 1);
+    }
+    tab[0] = 0;
+  }
+  return tab;
+}
+
+static void prepare_occupancy() {
+  resrestrict_occupancy<double, double, double, false>();
+  resrestrict_occupancy<double, double, double, true>();
+  resrestrict_occupancy<float, float, float, false>();
+  resrestrict_occupancy<float, float, float, true>();
+  resrestrict_occupancy<bf16, float, float, false>();
+  resrestrict_occupancy<bf16, float, float, true>();
+}
+
+// z chunk of a z-marching launch over `cols` columns and `ncz` coarse
+// planes: each block walks 2 zc + 1 fine planes, and `slots` blocks run at
+// once, so the time goes as ceil(blocks / slots) * (2 zc + 1).  Takes the
+// cheapest zc up to 32, the larger one on a tie (fewer halo planes).
+static int zchunk_for_rounds(long long cols, int ncz, long long slots) {
+  int best = 1;
+  long long bestcost = -1;
+  for (int zc = 1; zc <= std::min(32, ncz); zc++) {
+    long long nb = cols * cdiv(ncz, zc);
+    long long cost = cdiv(nb, slots) * (2 * zc + 1);
+    if (bestcost < 0 || cost <= bestcost) {
+      bestcost = cost;
+      best = zc;
+    }
+  }
+  return best;
+}
+
 template <typename S, typename TR, typename TC>
 static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params &P,
                         const int *skip, hipStream_t s) {
@@ -887,8 +934,8 @@ static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params
   }
   const int ncx = Lc.nx - 1, ncy = Lc.ny - 1, ncz = Lc.nz - 1;
   // tiles: up to 127 coarse columns and `maxpairs` fine pairs (one per
-  // thread) per block, sizes balanced over the range; z chunks of up to 16
-  // coarse planes, fewer when that leaves under 1024 blocks
+  // thread) per block, sizes balanced over the range; the z chunk from the
+  // kernel's occupancy (zchunk_for_rounds)
   static const int mp_env = getenv("WARPMI_RESTRICT_PAIRS") ? atoi(getenv("WARPMI_RESTRICT_PAIRS")) : 0;
   const int maxpairs = mp_env >= 64 && mp_env <= 1024 ? mp_env : 640;
   int ntx = (int)cdiv(ncx, 127);
@@ -897,20 +944,29 @@ static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params
   int nty = (int)cdiv(ncy, cytmax);
   int cyt = (int)cdiv(ncy, nty);
   long long cols = (long long)ntx * nty;
-  int zc = (int)std::max<long long>(1, std::min<long long>(16, cols * ncz / 1024));
+  int npairs = (cxt + 1) * (2 * cyt + 1);
+  int threads = (int)(64 * cdiv(npairs, 64));
+  size_t shm = 2 * sizeof(RA) * 2 * (cxt + 1) * (2 * cyt + 1);
+  WM_REQUIRE(threads <= 1024 && shm <= 65536, "restrict3d: tile does not fit the fused kernel");
+  const int *occ = L.hasmask ? resrestrict_occupancy<S, TR, TC, true>()
+                             : resrestrict_occupancy<S, TR, TC, false>();
+  long long per_cu = std::min<long long>(occ[threads / 64], (long long)(h->lds_per_cu / shm));
+  int zc = zchunk_for_rounds(cols, ncz, std::max<long long>(1, per_cu) * h->ncu);
   static const int zc_env = getenv("WARPMI_RESTRICT_ZC") ? atoi(getenv("WARPMI_RESTRICT_ZC")) : 0;
   if (zc_env > 0) zc = zc_env;
   int ntz = (int)cdiv(ncz, zc);
   long long nb = cols * ntz;
-  int npairs = (cxt + 1) * (2 * cyt + 1);
-  int threads = (int)(64 * cdiv(npairs, 64));
-  size_t shm = 2 * sizeof(RA) * 2 * (cxt + 1) * (2 * cyt + 1);
-  WM_REQUIRE(threads <= 1024 && nb < (1ll << 31) && shm <= 65536,
-             "restrict3d: tile does not fit the fused kernel");
-  hipLaunchKernelGGL((k_resrestrict<S, TR, TC>), dim3((int)nb), dim3(threads), shm, s, phi, rho,
-                     (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic, dzsqic,
-                     (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, cxt, cyt, zc, ntx, nty, (int)nb,
-                     L.ff, skip);
+  WM_REQUIRE(nb < (1ll << 31), "restrict3d: too many blocks");
+  if (L.hasmask)
+    hipLaunchKernelGGL((k_resrestrict<S, TR, TC, true>), dim3((int)nb), dim3(threads), shm, s, phi,
+                       rho, (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic, dzsqic,
+                       (T)P.mgparam, (const uint32_t *)L.mask.p, cxt, cyt, zc, ntx, nty, (int)nb,
+                       L.ff, skip);
+  else
+    hipLaunchKernelGGL((k_resrestrict<S, TR, TC, false>), dim3((int)nb), dim3(threads), shm, s,
+                       phi, rho, (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic,
+                       dzsqic, (T)P.mgparam, (const uint32_t *)nullptr, cxt, cyt, zc, ntx, nty,
+                       (int)nb, L.ff, skip);
   WM_LAUNCHED();
 }
 
@@ -1141,6 +1197,13 @@ int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz, double dx,
     h->partials.alloc(WM_CONV_BLOCKS);
     WM_HIP(hipHostMalloc((void **)&h->state_h, sizeof(SolveState), hipHostMallocDefault));
     WM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    int dev = 0, ncu = 0, lds = 0;
+    WM_HIP(hipGetDevice(&dev));
+    WM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    WM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    if (ncu > 0) h->ncu = ncu;
+    if (lds > 0) h->lds_per_cu = (size_t)lds;
+    prepare_occupancy();
     WM_HIP(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
     WM_HIP(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
     *out = h.release();

@@ -724,7 +724,12 @@ template <> struct RestrictArith<double, double> { typedef double type; };
 // 2jz+1; within a plane rows, then columns), so the sums and the result are
 // those of k_residual_zmarch + k_restrict, bit for bit.
 // ---------------------------------------------------------------------------
-template <typename S, typename TR, typename TC>
+//
+// Every plane issues the same loads unconditionally (the last plane's
+// prefetch re-reads a valid plane, unmasked lanes read a fixed element of
+// res), so the compiler's wait before the LDS store covers only the
+// previous plane's loads and the prefetch stays in flight across it.
+template <typename S, typename TR, typename TC, bool MASK>
 __global__ __launch_bounds__(1024) void k_resrestrict(
     const S *__restrict__ phi, const S *__restrict__ rho, const TR *__restrict__ res,
     TC *__restrict__ out, RBGeom g, RBGeom gc, typename St<S>::T dxsqic,
@@ -778,13 +783,24 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
   // res, only where the bit is set) one plane ahead
   auto mbit = [&](uint32_t w, int e) { return (w >> (e & 31)) & 1u; };
   auto loadmask = [&](int iz, uint32_t &m_a, uint32_t &m_b) {
-    m_a = mask ? mask[elA(iz) >> 5] : 0u;
-    m_b = mask ? mask[elB(iz) >> 5] : 0u;
+    if (MASK) {
+      m_a = mask[elA(iz) >> 5];
+      m_b = mask[elB(iz) >> 5];
+    } else {
+      m_a = m_b = 0u;
+    }
   };
   auto loadres = [&](int iz, uint32_t m_a, uint32_t m_b, RA &v_a, RA &v_b) {
+    if (!MASK) {
+      v_a = v_b = (RA)0.;
+      return;
+    }
     const int eA = elA(iz), eB = elB(iz);
-    v_a = mbit(m_a, eA) ? (RA)St<TR>::ld(res[eA]) : (RA)0.;
-    v_b = mbit(m_b, eB) ? (RA)St<TR>::ld(res[eB]) : (RA)0.;
+    const bool ba = mbit(m_a, eA), bb = mbit(m_b, eB);
+    const RA ta = (RA)St<TR>::ld(res[ba ? eA : 0]);
+    const RA tb = (RA)St<TR>::ld(res[bb ? eB : 0]);
+    v_a = ba ? ta : (RA)0.;
+    v_b = bb ? tb : (RA)0.;
   };
   T am = St<S>::ld(phi[elB(izf) - pl]);
   T bm = St<S>::ld(phi[elA(izf) - pl]);
@@ -795,7 +811,7 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
   RA sa, sb;
   load(izf, ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb);
   loadmask(izf, wa, wb);
-  if (izf < izl) loadmask(izf + 1, wa1, wb1);
+  loadmask(min(izf + 1, izl), wa1, wb1);
   loadres(izf, wa, wb, sa, sb);
   // phase 2: the owned coarse point and its running sum
   const int q = threadIdx.x;
@@ -807,14 +823,13 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
   const RA half = (RA)0.5, one = (RA)1.;
   for (int iz = izf; iz <= izl; iz++) {
     RA *buf = lres + (iz & 1) * lsz;
-    T nap = 0, nbp = 0, nyam = 0, nyap = 0, nybm = 0, nybp = 0, nxa = 0, nxb = 0, nra = 0, nrb = 0;
-    uint32_t wa2 = 0, wb2 = 0;
-    RA nsa = (RA)0., nsb = (RA)0.;
-    if (iz < izl) {
-      load(iz + 1, nap, nbp, nyam, nyap, nybm, nybp, nxa, nxb, nra, nrb);
-      loadres(iz + 1, wa1, wb1, nsa, nsb);
-      if (iz + 1 < izl) loadmask(iz + 2, wa2, wb2);
-    }
+    T nap, nbp, nyam, nyap, nybm, nybp, nxa, nxb, nra, nrb;
+    uint32_t wa2, wb2;
+    RA nsa, nsb;
+    // next plane (the last plane re-reads itself; those values are unused)
+    load(min(iz + 1, izl), nap, nbp, nyam, nyap, nybm, nybp, nxa, nxb, nra, nrb);
+    loadres(min(iz + 1, izl), wa1, wb1, nsa, nsb);
+    loadmask(min(iz + 2, izl), wa2, wb2);
     if (own) {
       RA vA, vB;
       if (mbit(wa, elA(iz))) {
