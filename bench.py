@@ -166,13 +166,26 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the process's CPU set, capped by
+    OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
 def cpu_baseline(args, cfg, solver, pg, phi_before, iters, fspecial):
-    """The FP64 oracle (single-threaded C restatement of the reference) on
-    the host: one step of the same workload with a particle sample of
-    --cpu-particles (deposit and push times scaled to the full count), the
-    full-mesh solve warm-started from the same phi as the GPU's last solve,
-    and the synchv half kick + beam moments of special steps weighted by
-    their share fspecial of the timed steps."""
+    """The FP64 oracle (the C restatement of the reference; its independent
+    loops run under OpenMP with unchanged results, the deposit stays
+    serial) on the host: one step of the same workload with a particle
+    sample of --cpu-particles (deposit and push times scaled to the full
+    count), the full-mesh solve warm-started from the same phi as the GPU's
+    last solve, and the synchv half kick + beam moments of special steps
+    weighted by their share fspecial of the timed steps.  Timed on all the
+    process's threads (the value) and on one thread."""
+    import ctypes as C
     import numpy as np
     from oracle import oracle as O
     from warp_amd import configs
@@ -183,50 +196,62 @@ def cpu_baseline(args, cfg, solver, pg, phi_before, iters, fspecial):
     stride = max(1, -(-npart // args.cpu_particles))
     sl = slice(0, npart, stride)  # every stride-th particle: the whole beam
     ns = len(range(npart)[sl])
-    x = pg.xp[sl].cpu().numpy().copy()
-    y = pg.yp[sl].cpu().numpy().copy()
-    z = pg.zp[sl].cpu().numpy().copy()
-    ux = pg.uxp[sl].cpu().numpy().copy()
-    uy = pg.uyp[sl].cpu().numpy().copy()
-    uz = pg.uzp[sl].cpu().numpy().copy()
-    gi = np.ones(ns)
+    cols0 = [getattr(pg, c)[sl].cpu().numpy().copy() for c in ("xp", "yp", "zp", "uxp", "uyp",
+                                                                "uzp")]
     q, m, w = pg.sq[0], pg.sm[0], pg.sw[0]
     co = O.install_conductors(configs.conductors(cfg["conductors"], O), nx, ny, nz, x0, x1,
                               y0, y1, z0, z1)
-    phi = phi_before.cpu().numpy().copy()
-    rho = np.zeros((nz + 1, ny + 1, nx + 1))
-    t = time.perf_counter()
-    ex, ey, ez = np.zeros(ns), np.zeros(ns), np.zeros(ns)
-    O.sete3d(phi, x, y, z, ex, ey, ez, dx, dy, dz, x0, y0, z0)
-    O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
-    O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
-    O.xpush3d(x, y, z, ux, uy, uz, gi, configs.DT)
-    t_push = time.perf_counter() - t
-    t = time.perf_counter()
-    k = (x >= x0) & (x < x1) & (y >= y0) & (y < y1) & (z >= z0) & (z < z1)
-    O.setrho3d(rho, x[k], y[k], z[k], q, w, dx, dy, dz, x0, y0, z0)
-    t_dep = time.perf_counter() - t
-    xs, ys, zs, uxs, uys, uzs = x[k], y[k], z[k], ux[k], uy[k], uz[k]
-    t = time.perf_counter()
-    es = [np.zeros(len(xs)) for _ in range(3)]
-    O.sete3d(phi, xs, ys, zs, *es, dx, dy, dz, x0, y0, z0)
-    O.epush3d(uxs, uys, uzs, *es, q, m, 0.5 * configs.DT)
-    O.getzmmnt(xs, ys, zs, uxs, uys, uzs, np.ones(len(xs)))
-    t_synch = time.perf_counter() - t
-    rho_full = solver.rho.cpu().numpy().copy()
-    t = time.perf_counter()
-    it, err = O.multigrid3dsolve(phi, rho_full, dx, dy, dz, conductors=co)
-    t_solve = time.perf_counter() - t
+    phi0 = phi_before.cpu().numpy().copy()
+    rho0 = solver.rho.cpu().numpy().copy()
+    lib = O.lib()
+    lib.oc_set_threads.argtypes = [C.c_int64]
+
+    def one_step(threads):
+        lib.oc_set_threads(threads)
+        x, y, z, ux, uy, uz = [a.copy() for a in cols0]
+        gi = np.ones(ns)
+        phi = phi0.copy()
+        rho = np.zeros((nz + 1, ny + 1, nx + 1))
+        t = time.perf_counter()
+        ex, ey, ez = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+        O.sete3d(phi, x, y, z, ex, ey, ez, dx, dy, dz, x0, y0, z0)
+        O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
+        O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
+        O.xpush3d(x, y, z, ux, uy, uz, gi, configs.DT)
+        t_push = time.perf_counter() - t
+        t = time.perf_counter()
+        k = (x >= x0) & (x < x1) & (y >= y0) & (y < y1) & (z >= z0) & (z < z1)
+        O.setrho3d(rho, x[k], y[k], z[k], q, w, dx, dy, dz, x0, y0, z0)
+        t_dep = time.perf_counter() - t
+        xs, ys, zs, uxs, uys, uzs = x[k], y[k], z[k], ux[k], uy[k], uz[k]
+        t = time.perf_counter()
+        es = [np.zeros(len(xs)) for _ in range(3)]
+        O.sete3d(phi, xs, ys, zs, *es, dx, dy, dz, x0, y0, z0)
+        O.epush3d(uxs, uys, uzs, *es, q, m, 0.5 * configs.DT)
+        O.getzmmnt(xs, ys, zs, uxs, uys, uzs, np.ones(len(xs)))
+        t_synch = time.perf_counter() - t
+        rho_full = rho0.copy()
+        t = time.perf_counter()
+        it, err = O.multigrid3dsolve(phi, rho_full, dx, dy, dz, conductors=co)
+        t_solve = time.perf_counter() - t
+        step_s = (t_push + t_dep + fspecial * t_synch) * (npart / ns) + t_solve
+        log("CPU oracle, %d thread(s): push %.2f s + deposit %.2f s + synchv/moments %.2f s for "
+            "%d particles, solve %.2f s (%d V-cycles)" % (threads, t_push, t_dep, t_synch, ns,
+                                                         t_solve, it))
+        return step_s, t_solve, it
+
+    nthr = cpu_threads()
+    step_n, solve_n, it = one_step(nthr)
+    step_1, solve_1, _ = one_step(1)
     scale = npart / ns
-    step_s = (t_push + t_dep + fspecial * t_synch) * scale + t_solve
-    log("CPU oracle: push %.2f s + deposit %.2f s + synchv/moments %.2f s for %d particles, "
-        "solve %.2f s (%d V-cycles)" % (t_push, t_dep, t_synch, ns, t_solve, it))
-    return {"value": 1. / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
+    return {"value": 1. / step_n, "unit": "steps/s", "cores": nthr, "kind": "port",
             "sample": "one step: gather+push and deposit of every %d-th particle (%d of %d, times scaled "
-                      "by %.2f), synchv kick + moments weighted %.2f, full %dx%dx%d solve "
-                      "warm-started from the GPU's phi (%d V-cycles, %.2f s)"
-                      % (stride, ns, npart, scale, fspecial, nx, ny, nz, it, t_solve),
-            "ms_per_solve": 1e3 * t_solve, "vcycles": it}
+                      "by %.2f; the deposit serial), synchv kick + moments weighted %.2f, full "
+                      "%dx%dx%d solve warm-started from the GPU's phi (%d V-cycles, %.2f s on %d "
+                      "threads)" % (stride, ns, npart, scale, fspecial, nx, ny, nz, it, solve_n,
+                                    nthr),
+            "ms_per_solve": 1e3 * solve_n, "vcycles": it,
+            "value_1thread": 1. / step_1, "ms_per_solve_1thread": 1e3 * solve_1}
 
 
 if __name__ == "__main__":

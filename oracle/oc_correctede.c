@@ -113,6 +113,7 @@ void oc_sete3d_conductor(const int32_t *icgrid, const double *e0_even, const dou
 #define ICG(a, b, c) (((a) > nx || (b) > ny || (c) > nz) ? 0 : icgrid[IDXR(a, b, c)])
 #define PHI(a, b, c) phi[IDXP(clampi(a, -1, nx + 1), clampi(b, -1, ny + 1), clampi(c, -1, nz + 1))]
 #define SE(id, a, b, c) selfe[3 * IDXR(clampi(a, 0, nx), clampi(b, 0, ny), clampi(c, 0, nz)) + (id)]
+  OC_PARFOR()
   for (ip = 0; ip < np; ip++) {
     double xi = (xp[ip] - xmmin) * dxi, yj = (yp[ip] - ymmin) * dyi;
     double zk = (zp[ip] - zgrid - zmmin) * dzi;

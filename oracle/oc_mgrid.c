@@ -28,6 +28,26 @@ static double dmin(double a, double b) { return a < b ? a : b; }
 
 /* top.eps0 = 1/(mu0*clight*clight), mu0 = 4*pi/10000000 (scripts/warp.py:198-202,
  * pi and clight from pywarp90/top.v:172-175). */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void oc_set_threads(int64_t n) {
+#ifdef _OPENMP
+  omp_set_num_threads((int)(n > 0 ? n : 1));
+#else
+  (void)n;
+#endif
+}
+
+int64_t oc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 double oc_eps0(void) {
   double pi = 3.14159265358979323;
   double clight = 2.99792458e+8;
@@ -625,6 +645,7 @@ void oc_sorhalfpass3d(int64_t parity, int64_t mglevel,
       c->sub_prevphi[p][ic] = PHI(q[0], q[1], q[2]);
     }
   }
+  OC_PARFOR(private(iy, ix))
   for (iz = izmin; iz <= izmax; iz++)
     for (iy = iymin; iy <= iymax; iy++) {
       int64_t ix1 = ixmin + (ixmin + iy + iz + parity) % 2;
@@ -685,6 +706,7 @@ void oc_residual3d(int64_t nx, int64_t ny, int64_t nz,
     for (iz = -g; iz <= 0; iz++)
       for (iy = -g; iy <= ny + g; iy++)
         for (ix = -g; ix <= nx + g; ix++) RES(ix, iy, iz) = 0.;
+  OC_PARFOR(private(iy, ix))
   for (iz = izmin; iz <= izmax; iz++) {
     if (lb[2] == 0)
       for (iy = -g; iy <= 0; iy++)
@@ -772,6 +794,7 @@ void oc_restrict3d(int64_t nx, int64_t ny, int64_t nz, const double *res,
   if (lbc[4] == 0)
     for (iyc = 0; iyc <= nyc; iyc++)
       for (ixc = 0; ixc <= nxc; ixc++) RC(ixc, iyc, 0) = 0.;
+  OC_PARFOR(private(iyc, ixc, iz, iy, ix))
   for (izc = izcmin; izc <= izcmax; izc++) {
     int64_t izmin = izmina[izc], izmax = izmaxa[izc];
     const double *wz = wza + (nw + 1) * izc;
@@ -824,6 +847,7 @@ void oc_expand3d(int64_t nx, int64_t ny, int64_t nz, double *phi,
   if (lb[5] == 0) izmax = nz - 1;
 #define PC(i, j, k) phicoarse[IDX(i, j, k, nxc, nyc, 1)]
   int64_t ix, iy, iz;
+  OC_PARFOR(private(iy, ix))
   for (iz = izmin; iz <= izmax; iz++) {
     int64_t jz = (iz * nzc) / nz;
     double wz = 1. * (iz * nzc) / nz - jz;
@@ -864,6 +888,7 @@ static void vcycle(int64_t mglevel, int64_t nx, int64_t ny, int64_t nz,
     double *res = (double *)malloc(sizeof(double) * nres);
     /* Fortran leaves res uninitialised; every element read later is
      * written by residual3d, so the fill value is immaterial. */
+    OC_PARFOR()
     for (int64_t k = 0; k < nres; k++) res[k] = 0.;
     oc_residual3d(nx, ny, nz, dxsqi, dysqi, dzsqi, phi, rho, res, mglevel, bounds,
                   mgparam, mgform, 0, lcndbndy, icndbndy, c);
@@ -906,6 +931,7 @@ int64_t oc_multigrid3dsolve(int64_t nx, int64_t ny, int64_t nz,
   int64_t nrho = (nx + 1) * (ny + 1) * (nz + 1);
   int64_t nphi = (nx + 3) * (ny + 3) * (nz + 3);
   int64_t k, ix, iy, iz;
+  OC_PARFOR()
   for (k = 0; k < nrho; k++) rho[k] = rho[k] * reps0c;
   oc_applyboundaryconditions3d(nx, ny, nz, 1, 1, 1, phi, bounds, 0, 0);
   double *rhosave = NULL, *res = NULL;
@@ -915,11 +941,14 @@ int64_t oc_multigrid3dsolve(int64_t nx, int64_t ny, int64_t nz,
     memcpy(rhosave, rho, sizeof(double) * nrho);
   }
   double *phisave = (double *)malloc(sizeof(double) * nphi);
+  double *planeerr = (double *)malloc(sizeof(double) * (nz + 1));
+  int64_t *planenan = (int64_t *)malloc(sizeof(int64_t) * (nz + 1));
   *mgiters = 0;
   *mgerror = 2. * mgtol + 1.;
   while (*mgerror > mgtol && *mgiters < mgmaxiters) {
     *mgiters = *mgiters + 1;
-    memcpy(phisave, phi, sizeof(double) * nphi);
+    OC_PARFOR()
+    for (k = 0; k < nphi; k++) phisave[k] = phi[k];
     if (mgform == 2) {
       cond_potmg(c, nx, ny, phisave, 0, mgform, 1);
       condbndymgint(c, nx, ny, nz, phisave, bounds, 0, icndbndy);
@@ -936,13 +965,26 @@ int64_t oc_multigrid3dsolve(int64_t nx, int64_t ny, int64_t nz,
       for (k = 0; k < nphi; k++) phi[k] = phi[k] + phisave[k];
       oc_applyboundaryconditions3d(nx, ny, nz, 1, 1, 1, phi, bounds, 0, 0);
     }
+    /* err = err > d ? err : d over the nodes in order.  Each plane is
+     * folded from 0 in parallel; a NaN restarts the fold, so a plane that
+     * holds one gives its own fold, any other plane its maximum with the
+     * running value: the serial loop's result, NaNs included. */
     double err = 0.;
-    for (iz = 0; iz <= nz; iz++)
+    OC_PARFOR(private(iy, ix))
+    for (iz = 0; iz <= nz; iz++) {
+      double pe = 0.;
+      int64_t nan = 0;
       for (iy = 0; iy <= ny; iy++)
         for (ix = 0; ix <= nx; ix++) {
           double d = fabs(phisave[IDX(ix, iy, iz, nx, ny, 1)] - PHI(ix, iy, iz));
-          err = err > d ? err : d;
+          if (d != d) nan = 1;
+          pe = pe > d ? pe : d;
         }
+      planeerr[iz] = pe;
+      planenan[iz] = nan;
+    }
+    for (iz = 0; iz <= nz; iz++)
+      err = planenan[iz] ? planeerr[iz] : (err > planeerr[iz] ? err : planeerr[iz]);
     *mgerror = err;
   }
   oc_applyboundaryconditions3d(nx, ny, nz, 1, 1, 1, phi, bounds, 1, 0);
@@ -952,7 +994,10 @@ int64_t oc_multigrid3dsolve(int64_t nx, int64_t ny, int64_t nz,
     free(res);
   }
   free(phisave);
+  free(planeerr);
+  free(planenan);
   reps0c = 1. / reps0c;
+  OC_PARFOR()
   for (k = 0; k < nrho; k++) rho[k] = rho[k] * reps0c;
   return 0;
 }

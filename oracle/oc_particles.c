@@ -100,6 +100,7 @@ void oc_sete3d(const double *phi, int64_t np, const double *xp,
   double dxi = 1. / dx, dyi = 1. / dy, dzi = 1. / dz;
   int64_t ip;
 #define P(a, b, c) phi[IDXP(i + (a), j + (b), k + (c))]
+  OC_PARFOR()
   for (ip = 0; ip < np; ip++) {
     int64_t i = (int64_t)((xp[ip] - xmmin) * dxi);
     int64_t j = (int64_t)((yp[ip] - ymmin) * dyi);
@@ -145,6 +146,7 @@ void oc_epush3d(int64_t np, double *uxp, double *uyp, double *uzp,
                 double q, double m, double ddt) {
   double cst = q * ddt / m;
   int64_t ip;
+  OC_PARFOR()
   for (ip = 0; ip < np; ip++) {
     uxp[ip] = uxp[ip] + ex[ip] * cst;
     uyp[ip] = uyp[ip] + ey[ip] * cst;
@@ -157,6 +159,7 @@ void oc_xpush3d(int64_t np, double *xp, double *yp, double *zp,
                 const double *uxp, const double *uyp, const double *uzp,
                 const double *gaminv, double dtp) {
   int64_t ip;
+  OC_PARFOR()
   for (ip = 0; ip < np; ip++) {
     xp[ip] = xp[ip] + uxp[ip] * gaminv[ip] * dtp;
     yp[ip] = yp[ip] + uyp[ip] * gaminv[ip] * dtp;
@@ -170,6 +173,7 @@ void oc_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
                    int64_t lzero) {
   double tdxi = 0.5 / dx, tdyi = 0.5 / dy, tdzi = 0.5 / dz;
   int64_t ix, iy, iz;
+  OC_PARFOR(private(iy, ix))
   for (iz = 0; iz <= nz; iz++)
     for (iy = 0; iy <= ny; iy++)
       for (ix = 0; ix <= nx; ix++) {
@@ -199,6 +203,7 @@ void oc_sete_from_e3d(const double *selfe, int64_t np, const double *xp,
                       double *ex, double *ey, double *ez) {
   double dxi = 1. / dx, dyi = 1. / dy, dzi = 1. / dz;
   int64_t ip;
+  OC_PARFOR()
   for (ip = 0; ip < np; ip++) {
     double xi = (xp[ip] - xmmin) * dxi;
     double yj = (yp[ip] - ymmin) * dyi;

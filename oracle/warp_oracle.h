@@ -15,6 +15,18 @@
 #define WARP_ORACLE_H
 #include <stdint.h>
 
+/* Loops whose iterations write disjoint elements and read nothing another
+ * iteration writes run in parallel under OpenMP; the results are the same
+ * bits as the serial loop. */
+#define OC_PRAGMA(x) _Pragma(#x)
+#ifdef _OPENMP
+#define OC_PARFOR(...) OC_PRAGMA(omp parallel for schedule(static) __VA_ARGS__)
+#else
+#define OC_PARFOR(...)
+#endif
+void oc_set_threads(int64_t n);
+int64_t oc_get_threads(void);
+
 #define OC_MAXLEVELS 101
 
 /* Conductor data, the same model as ConductorType / ConductorInteriorType /

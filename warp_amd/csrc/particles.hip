@@ -95,12 +95,17 @@ __device__ inline void corner_charges(double u1, double v1, double w1, double g,
   c[7] = u1 * v1 * w1 * g;
 }
 
-// Deposit in the particles' own order.  A block first finds the box of
-// mesh nodes its 256 particles touch; when it fits in LDS (particles
-// ordered by cell give boxes of a few dozen nodes) the charges are summed
+// Deposit in the particles' own order.  A block takes 256*DEP_K
+// consecutive particles, DEP_K per thread, assigned so that in one wave
+// instruction the lanes are 4*DEP_K particles apart: with particles ordered
+// by cell that is about one cell apart, so the lanes' LDS atomics hit
+// different nodes instead of piling onto the few nodes of one cell.  The
+// block first finds the box of mesh nodes its particles touch; when it fits
+// in LDS (a few hundred nodes for ordered particles) the charges are summed
 // there and the box is added to rho with one atomic per touched node,
-// otherwise each wave adds its runs directly (deposit_runs).
+// otherwise every particle adds its eight charges to rho directly.
 #define DEP_BOX 2048
+#define DEP_K 4
 __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long long np,
                                                 const double *__restrict__ xp,
                                                 const double *__restrict__ yp,
@@ -109,27 +114,36 @@ __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long l
                                                 unsigned long long *__restrict__ nskip) {
   __shared__ double box[DEP_BOX];
   __shared__ int lim[4][6];
-  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
-  long long ii = 0, jj = 0, kk = 0;
-  double u1, v1, w1, c[8];
-  bool ok = false;
-  if (ip < np) {
-    ok = cic(m, xp[ip], yp[ip], zp[ip], ii, jj, kk, u1, v1, w1);
-    if (!ok && nskip) atomicAdd(nskip, 1ull);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const long long base = (long long)blockIdx.x * 256 * DEP_K + (long long)(l * 4 + w) * DEP_K;
+  long long ii[DEP_K], jj[DEP_K], kk[DEP_K];
+  double u1[DEP_K], v1[DEP_K], w1[DEP_K];
+  bool ok[DEP_K];
+  const int big = 1 << 30;
+  int a0 = big, a1 = -big, b0 = big, b1 = -big, c0 = big, c1 = -big;
+#pragma unroll
+  for (int k = 0; k < DEP_K; k++) {
+    long long ip = base + k;
+    ok[k] = false;
+    if (ip < np) {
+      ok[k] = cic(m, xp[ip], yp[ip], zp[ip], ii[k], jj[k], kk[k], u1[k], v1[k], w1[k]);
+      if (!ok[k] && nskip) atomicAdd(nskip, 1ull);
+    }
+    if (ok[k]) {
+      a0 = min(a0, (int)ii[k]);
+      a1 = max(a1, (int)ii[k]);
+      b0 = min(b0, (int)jj[k]);
+      b1 = max(b1, (int)jj[k]);
+      c0 = min(c0, (int)kk[k]);
+      c1 = max(c1, (int)kk[k]);
+    }
   }
-  if (ok) corner_charges(u1, v1, w1, g, c);
-  // box of the block's cells
-  int big = 1 << 30;
-  int a0 = ok ? (int)ii : big, a1 = ok ? (int)ii : -big;
-  int b0 = ok ? (int)jj : big, b1 = ok ? (int)jj : -big;
-  int c0 = ok ? (int)kk : big, c1 = ok ? (int)kk : -big;
   for (int off = 32; off > 0; off >>= 1) {
     a0 = min(a0, __shfl_xor(a0, off, 64)); a1 = max(a1, __shfl_xor(a1, off, 64));
     b0 = min(b0, __shfl_xor(b0, off, 64)); b1 = max(b1, __shfl_xor(b1, off, 64));
     c0 = min(c0, __shfl_xor(c0, off, 64)); c1 = max(c1, __shfl_xor(c1, off, 64));
   }
-  int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
+  if (l == 0) {
     lim[w][0] = a0; lim[w][1] = a1; lim[w][2] = b0; lim[w][3] = b1; lim[w][4] = c0; lim[w][5] = c1;
   }
   __syncthreads();
@@ -140,16 +154,35 @@ __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long l
   c0 = min(min(lim[0][4], lim[1][4]), min(lim[2][4], lim[3][4]));
   c1 = max(max(lim[0][5], lim[1][5]), max(lim[2][5], lim[3][5]));
   if (a1 < a0) return;  // no particle of the block is in the grid
-  long long nbx = a1 - a0 + 2, nby = b1 - b0 + 2, nbz = c1 - c0 + 2;
+  const long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
+  const long long nbx = a1 - a0 + 2, nby = b1 - b0 + 2, nbz = c1 - c0 + 2;
   if (nbx * nby * nbz > DEP_BOX) {
-    deposit_runs(rho, m, ok, ii, jj, kk, c);
+#pragma unroll
+    for (int k = 0; k < DEP_K; k++) {
+      if (!ok[k]) continue;
+      double c[8];
+      corner_charges(u1[k], v1[k], w1[k], g, c);
+      double *r = rho + ii[k] + sx * jj[k] + sxy * kk[k];
+      atomicAdd(r, c[0]);
+      atomicAdd(r + 1, c[1]);
+      atomicAdd(r + sx, c[2]);
+      atomicAdd(r + sx + 1, c[3]);
+      atomicAdd(r + sxy, c[4]);
+      atomicAdd(r + sxy + 1, c[5]);
+      atomicAdd(r + sxy + sx, c[6]);
+      atomicAdd(r + sxy + sx + 1, c[7]);
+    }
     return;
   }
-  int bx = (int)nbx, bxy = (int)(nbx * nby), nb = (int)(nbx * nby * nbz);
+  const int bx = (int)nbx, bxy = (int)(nbx * nby), nb = (int)(nbx * nby * nbz);
   for (int e = threadIdx.x; e < nb; e += 256) box[e] = 0.;
   __syncthreads();
-  if (ok) {
-    double *bp = box + (ii - a0) + bx * (jj - b0) + bxy * (kk - c0);
+#pragma unroll
+  for (int k = 0; k < DEP_K; k++) {
+    if (!ok[k]) continue;
+    double c[8];
+    corner_charges(u1[k], v1[k], w1[k], g, c);
+    double *bp = box + (ii[k] - a0) + bx * (jj[k] - b0) + bxy * (kk[k] - c0);
     atomicAdd(bp, c[0]);
     atomicAdd(bp + 1, c[1]);
     atomicAdd(bp + bx, c[2]);
@@ -160,7 +193,6 @@ __global__ __launch_bounds__(256) void k_setrho(double *__restrict__ rho, long l
     atomicAdd(bp + bxy + bx + 1, c[7]);
   }
   __syncthreads();
-  long long sx = m.nx + 1, sxy = (long long)(m.nx + 1) * (m.ny + 1);
   for (int e = threadIdx.x; e < nb; e += 256) {
     double v = box[e];
     if (v != 0.) {
@@ -654,8 +686,8 @@ int wm_setrho3d(double *rho, int64_t np, const double *xp, const double *yp, con
       WM_HIP(hipMallocAsync((void **)&cnt, sizeof(unsigned long long), s));
       WM_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
     }
-    hipLaunchKernelGGL(k_setrho, dim3(nblk(np)), dim3(256), 0, s, rho, (long long)np, xp, yp, zp,
-                       m, g, cnt);
+    hipLaunchKernelGGL(k_setrho, dim3((unsigned)cdiv(np, 256 * DEP_K)), dim3(256), 0, s, rho,
+                       (long long)np, xp, yp, zp, m, g, cnt);
     WM_LAUNCHED();
     if (nskipped) {
       unsigned long long h = 0;
