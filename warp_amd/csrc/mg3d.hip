@@ -713,9 +713,9 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
 
 template <typename T>
 static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
-                    hipStream_t s) {
+                    hipStream_t s, bool pin = true) {
   int usevolt = (level == 0 && P.mgform == 1);
-  pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
+  if (pin) pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
   for (int c = 0; c <= 1; c++) {
     halfpass<T>(h, L, level, P, c, skip, s);
     launch_bc<T>(phi_of<T>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
@@ -997,12 +997,18 @@ static void zero_interior_res(wm_mg3d *h, const Params &P, hipStream_t s) {
 
 // vcycle (f3d_mgrid.F:1101-1591) on level data of storage type S (the
 // coarser levels use Coarse<S>).
+// zeroed: the caller has just filled this level's phi with zeros.  The pin
+// at the start of the first pass then writes zeros over zeros (interior
+// points are pinned to 0 unless on level 0 with mgform 1, which is never
+// zero-filled) and the Neumann mirrors copy zeros, so it is left out.
 template <typename S>
-static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s) {
+static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
+                   bool zeroed = false) {
   typedef typename St<S>::T T;
   typedef typename Coarse<S>::type SC;
   Level &L = *h->lv[level];
-  for (int i = 0; i < P.down; i++) sorpass<S>(h, L, level, P, skip, s);
+  const bool novolt = !(level == 0 && P.mgform == 1);
+  for (int i = 0; i < P.down; i++) sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0));
   if (L.coarser) {
     Level &Lc = *h->lv[level + 1];
     bool anyzero = false;
@@ -1021,7 +1027,7 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
     hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
                        phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
     WM_LAUNCHED();
-    vcycle<SC>(h, level + 1, P, skip, s);
+    vcycle<SC>(h, level + 1, P, skip, s, true);
     expand_rb<S>(L, Lc, skip, s);
     launch_bc<S>(phi_of<S>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
   }
@@ -1057,7 +1063,7 @@ static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long lo
   hipLaunchKernelGGL((k_fill<S>), dim3(blocks(np)), dim3(256), 0, s, phi_of<S>(L0).p, np, zero,
                      skip);
   WM_LAUNCHED();
-  vcycle<S>(h, 0, Pinit, skip, s);
+  vcycle<S>(h, 0, Pinit, skip, s, true);
   if (fused) {
     WM_KW_DISPATCH(L0.g.W, hipLaunchKernelGGL((k_dc_update<S, KW>), dim3(nconv_blocks(L0.g)),
                                               dim3(256), 0, s, L0.phi64.p,
@@ -1098,7 +1104,7 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
       hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p, np,
                          0., skip);
       WM_LAUNCHED();
-      vcycle<double>(h, 0, Pinit, skip, s);
+      vcycle<double>(h, 0, Pinit, skip, s, true);
       // phi = phi + phisave (the error plus the saved phi)
       hipLaunchKernelGGL((k_dc_add<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
                          L0.phi64.p, h->save64.p, np, skip);
