@@ -32,6 +32,9 @@ def parse():
     p.add_argument("--config", default="c2")
     p.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--guess", default="previous", choices=["previous", "linear"],
+                   help="initial guess of each solve: the previous phi (the reference's) or "
+                        "its linear extrapolation in time")
     p.add_argument("--sort-interval", type=int, default=25,
                    help="steps between particle sorts by cell (0: only at set-up)")
     p.add_argument("--cpu-particles", type=int, default=4_000_000,
@@ -57,7 +60,8 @@ def main():
     dev = torch.device("cuda", 0)
     cfg = configs.CONFIGS[args.config]
     t0 = time.time()
-    solver = configs.make_solver(cfg, precision=args.precision, device=dev, mgverbose=-1)
+    solver = configs.make_solver(cfg, precision=args.precision, device=dev, mgverbose=-1,
+                                 guess=args.guess)
     pg = configs.load_beam(cfg, device=dev)
     top.pgroup = pg
     top.dt = configs.DT
@@ -152,7 +156,8 @@ def main():
                                % (nx, ny, nz, npart0),
                    "mesh": [nx, ny, nz], "particles": npart0, "particles_at_end": pg.npmax,
                    "mgtol": mgtol, "particle_sort_interval": args.sort_interval,
-                   "precision": args.precision, "parallelism": "1 GPU"},
+                   "precision": args.precision, "initial_guess": args.guess,
+                   "parallelism": "1 GPU"},
         "vcycles_per_solve": float(np.mean(iters)),
         "vcycles_cold_solve": cold_iters,
         "ms_per_solve": ms_solve,

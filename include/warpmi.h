@@ -90,6 +90,12 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam,
                   int64_t icndbndy, int64_t precision, int64_t *mgiters,
                   double *mgerror, void *stream);
 
+/* Initial guess of the next solve by linear extrapolation in time:
+ * phi <- 2 phi - prev and prev <- the old phi, over n values.  No
+ * counterpart in the reference, whose solve starts from the previous phi
+ * (fieldsolver.py:1653-1658); the solve still stops on mgtol. */
+int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream);
+
 /* Event timing of the finest-level FP64 SOR half sweep (the solver's
  * dominant kernel): while on, wm_mg3d_solve launches its kernels one
  * V-cycle at a time instead of replaying its graph, records HIP events

@@ -126,3 +126,55 @@ def test_step_driver_matches_oracle(device, efetch, lcorrectede):
         warp_amd.unregistersolver(s)
         for k, v in saved.items():
             setattr(top, k, v)
+
+
+def test_phi_extrapolate_kernel(device):
+    """wm_phi_extrapolate: phi <- 2 phi - prev, prev <- old phi, bitwise."""
+    from warp_amd import _native as N
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(100003), rng.standard_normal(100003)
+    pa, pb = torch.from_numpy(a.copy()).to(device), torch.from_numpy(b.copy()).to(device)
+    N.call("wm_phi_extrapolate", pa.data_ptr(), pb.data_ptr(), pa.numel(), N.stream_of(device))
+    assert np.array_equal(pa.cpu().numpy(), 2. * a - b)
+    assert np.array_equal(pb.cpu().numpy(), a)
+
+
+def test_linear_guess_reaches_mgtol(device):
+    """The step driver with the initial guess extrapolated from the two
+    previous solves: every solve still stops on mgtol, the particles end
+    where the reference's guess (the previous phi) takes them up to the
+    solves' tolerance, and the extrapolated guess needs no more V-cycles."""
+    import warp_amd
+    from warp_amd import stepper
+    from warp_amd.globals import top
+    from warp_amd.particles import load_gaussian
+    out = {}
+    for guess in ("previous", "linear"):
+        pg, js, n = load_gaussian(20000, (0.004, 0.004, 0.015), 0.04, 1.0e6, 1.0e-3, DOM, 31,
+                                  device=device)
+        s = warp_amd.MultiGrid3D(nx=NX, ny=NY, nz=NZ, xmmin=DOM[0], xmmax=DOM[1], ymmin=DOM[2],
+                                 ymmax=DOM[3], zmmin=DOM[4], zmmax=DOM[5], bounds=[0] * 6,
+                                 mgverbose=-1, guess=guess)
+        for c in H.pipe_and_annulus(warp_amd):
+            s.installconductor(c)
+        saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time,
+                     particlesortinterval=top.particlesortinterval)
+        try:
+            top.pgroup, top.dt, top.it, top.time = pg, 1.0e-9, 0, 0.
+            top.particlesortinterval = 0
+            warp_amd.registersolver(s)
+            stepper.generate()
+            stepper.step(8)
+            errs = s.mgerror
+            out[guess] = ([getattr(pg, c).cpu().numpy() for c in ("xp", "zp", "uzp")],
+                          list(s.mgiters_history), errs)
+        finally:
+            warp_amd.unregistersolver(s)
+            for k, v in saved.items():
+                setattr(top, k, v)
+    (a, ha, ea), (b, hb, eb) = out["previous"], out["linear"]
+    print("\nV-cycles per solve: previous phi %s, extrapolated %s" % (ha, hb))
+    assert ea <= 1e-6 and eb <= 1e-6
+    assert sum(hb[3:]) <= sum(ha[3:])
+    for u, v in zip(a, b):
+        assert np.abs(u - v).max() <= 1e-9 * np.abs(u).max()

@@ -126,6 +126,12 @@ def test_bad_keyword_rejected():
         W.MultiGrid3D(nx=8, ny=8, nz=8, mgtoll=1e-6)
 
 
+def test_unknown_initial_guess_rejected():
+    import warp_amd as W
+    with pytest.raises(ValueError, match="guess"):
+        W.MultiGrid3D(nx=8, ny=8, nz=8, guess="quadratic")
+
+
 def test_solver_fails_loudly_without_gpu():
     """No CPU fallback: without a HIP device the solver refuses to build."""
     import torch

@@ -50,6 +50,7 @@ SIGNATURES = {
     "wm_mg3d_set_conductors": (C.c_int, [_P, _P]),
     "wm_mg3d_solve": (C.c_int, [_P, _P, _P, _d, _i64, _i64, _d, _i64, _i64, _i64,
                                 _i64, _i64, _P, _P, _P]),
+    "wm_phi_extrapolate": (C.c_int, [_P, _P, _i64, _P]),
     "wm_mg3d_set_timing": (C.c_int, [_P, _i64]),
     "wm_mg3d_get_timing": (C.c_int, [_P, _P, _P]),
     "wm_mg3d_sorhalfpass": (C.c_int, [_P, _i64, _i64, _P, _P, _d, _i64, _i64, _i64, _P]),

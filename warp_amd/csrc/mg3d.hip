@@ -1467,6 +1467,17 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
   });
 }
 
+int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(n >= 0, "wm_phi_extrapolate: n < 0");
+    if (n == 0) return 0;
+    WM_REQUIRE(phi && prev, "wm_phi_extrapolate: NULL array");
+    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)cdiv(n, 256)), dim3(256), 0,
+                       wm_stream(stream), phi, prev, (long long)n);
+    WM_LAUNCHED();
+  });
+}
+
 int wm_mg3d_set_timing(wm_mg3d *h, int64_t on) {
   WM_ENTRY({
     WM_REQUIRE(h != nullptr, "wm_mg3d_set_timing: NULL handle");

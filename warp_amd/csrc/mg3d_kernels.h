@@ -1289,6 +1289,17 @@ __global__ __launch_bounds__(256) void k_dc_add(double *phi, const TL *e,
 }
 
 
+// Initial guess by linear extrapolation in time: phi <- 2 phi - prev and
+// prev <- the old phi, in one pass.
+__global__ __launch_bounds__(256) void k_extrapolate(double *__restrict__ phi,
+                                                     double *__restrict__ prev, long long n) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  double a = phi[t];
+  phi[t] = 2. * a - prev[t];
+  prev[t] = a;
+}
+
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void k_copy(const TA *__restrict__ a,
                                               TB *__restrict__ b, long long n,

@@ -34,7 +34,8 @@ class MultiGrid3D(FieldSolver):
                      'mgmaxlevels', 'mgform', 'mgverbose', 'mgntverbose', 'lcndbndy',
                      'icndbndy', 'laddconductor', 'lprecalccoeffs']
 
-    def __init__(self, precision="fp64", device=None, sorted_deposit=False, **kw):
+    def __init__(self, precision="fp64", device=None, sorted_deposit=False, guess="previous",
+                 **kw):
         FieldSolver.__init__(self, kwdict=kw)
         self.processdefaultsfrompackage(MultiGrid3D.__f3dinputs__, f3d, kw)
         assert len(kw.keys()) == 0, "Bad keyword arguemnts %s" % list(kw.keys())
@@ -44,6 +45,12 @@ class MultiGrid3D(FieldSolver):
             raise ValueError("precision must be one of %s" % list(PRECISIONS))
         self.precision = precision
         self.sorted_deposit = sorted_deposit
+        if guess not in ("previous", "linear"):
+            raise ValueError("guess must be 'previous' (the reference's) or 'linear'")
+        # initial guess of each solve: the previous phi (the reference), or
+        # its linear extrapolation from the two previous solves
+        self.guess = guess
+        self._phiprev = None
         if not torch.cuda.is_available():
             raise RuntimeError("MultiGrid3D needs a HIP device (torch.cuda is not available)")
         if device is None:
@@ -336,6 +343,8 @@ class MultiGrid3D(FieldSolver):
         t0 = time.perf_counter()
         if self.conductordatalist:
             self.getconductorobject()
+        if self.guess == "linear":
+            self._extrapolate_guess()
         it = C.c_int64(0)
         err = C.c_double(0.)
         N.call("wm_mg3d_solve", self._h, self._phi.data_ptr(), self._rho.data_ptr(),
@@ -360,6 +369,18 @@ class MultiGrid3D(FieldSolver):
             print("Multigrid: Error converged to %11.3e in %5d v-cycles"
                   % (self.mgerror, self.mgiters), file=sys.stderr)
         self.timers["solve"] += time.perf_counter() - t0
+
+    def _extrapolate_guess(self):
+        """phi <- 2 phi_(n-1) - phi_(n-2) before a solve (the previous phi
+        alone while fewer than two solves have run); keeps phi_(n-1) for the
+        next one."""
+        if not self.mgiters_history:
+            return  # phi is not the result of a solve yet
+        if self._phiprev is None:
+            self._phiprev = self._phi.clone()
+            return
+        N.call("wm_phi_extrapolate", self._phi.data_ptr(), self._phiprev.data_ptr(),
+               self._phi.numel(), N.stream_of(self.device))
 
     def getresidual(self):
         """residual3d (f3d_mgrid.F:3367-3598) of the current phi and rho,
