@@ -87,6 +87,36 @@ def test_conv3_backward(device):
     assert (db.cpu().to(torch.float64) - b.grad).abs().max() <= 1e-5 * b.grad.abs().max()
 
 
+@pytest.mark.parametrize("cin,cout,dims", [(16, 16, (8, 16, 16)), (16, 64, (4, 6, 8)),
+                                           (112, 48, (2, 4, 8)), (64, 64, (4, 8, 8)),
+                                           (80, 32, (3, 5, 7)), (48, 16, (1, 2, 2)),
+                                           (32, 48, (16, 8, 8))])
+def test_conv3_weight_gradient_shapes(device, cin, cout, dims):
+    """dW and db of every channel combination the network uses (and odd
+    grid sizes: partial voxel steps, rows shorter than a step) against torch
+    autograd in FP64 on the same bf16 operands."""
+    from warp_amd import _native as N
+    D, H, W = dims
+    g = torch.Generator().manual_seed(cin + 3 * cout + D)
+    x = _bf16(torch.randn(cin, D, H, W, generator=g, dtype=torch.float64)).requires_grad_()
+    w = torch.zeros(cout, cin, 27, dtype=torch.float64, requires_grad=True)
+    b = torch.zeros(cout, dtype=torch.float64, requires_grad=True)
+    gy = _bf16(torch.randn(cout, D, H, W, generator=g, dtype=torch.float64))
+    y = F.conv3d(x[None], w.reshape(cout, cin, 3, 3, 3), b, padding=1)[0]
+    (y * gy).sum().backward()
+    s = N.stream_of(device)
+    xd = _to_dev_bits(x.detach().permute(1, 2, 3, 0), device)
+    gd = _to_dev_bits(gy.permute(1, 2, 3, 0), device)
+    dw = torch.full((cout * cin * 27,), 0.5, dtype=torch.float32, device=device)
+    db = torch.zeros(cout, dtype=torch.float32, device=device)
+    N.call("wm_unet_conv3_wgrad", xd.data_ptr(), cin, gd.data_ptr(), cout, D, H, W, dw.data_ptr(),
+           db.data_ptr(), s)
+    dwg = dw.cpu().to(torch.float64).reshape(cout, cin, 27) - 0.5   # accumulates onto dw
+    scale = w.grad.abs().max()
+    assert (dwg - w.grad).abs().max() <= 1e-5 * scale + 1e-6
+    assert (db.cpu().to(torch.float64) - b.grad).abs().max() <= 1e-5 * b.grad.abs().max()
+
+
 class _RefUNet(torch.nn.Module):
     """The same network in torch (CPU, FP64) with the device weights."""
 
@@ -160,7 +190,9 @@ def test_unet_forward_backward_against_torch(device):
 
 
 def test_unet_training_overfits_one_sample(device):
-    """Adam on one fixed sample drives the loss down by 10x."""
+    """Adam on one fixed sample drives the loss down by 10x (at a learning
+    rate inside Adam's stable range for this problem: at 3e-3 the loss
+    oscillates and the outcome depends on the last bits of the gradients)."""
     from warp_amd import _native as N
     from warp_amd.unet import UNet3D
     D, H, W = 8, 16, 16
@@ -173,14 +205,14 @@ def test_unet_training_overfits_one_sample(device):
     gout = torch.empty(D * H * W * 16, dtype=torch.int16, device=device)
     loss = torch.zeros(1, dtype=torch.float32, device=device)
     hist = []
-    for _ in range(60):
+    for _ in range(100):
         net.zero_grad()
         out = net.forward(xd, (D, H, W), keep=True)
         loss.zero_()
         N.call("wm_unet_mse", out.data_ptr(), tgt.data_ptr(), D * H * W, gout.data_ptr(),
                loss.data_ptr(), N.stream_of(device))
         net.backward(gout)
-        net.adam_step(3e-3)
+        net.adam_step(1e-3)
         hist.append(float(loss.item()))
     assert hist[-1] < 0.1 * hist[0], hist[::10]
 

@@ -155,6 +155,145 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict_
   }
 }
 
+// Weight gradient, tiled: dw[(co*cin + ci)*27 + tap] += sum_v x[v+tap][ci]
+// g[v][co].  A block owns one 16-channel input tile `cit` and the voxels
+// [vb, ve) of its chunk; its four wavefronts split the 27 taps (wave w takes
+// taps w, w+4, ...) and keep every output tile of their taps in registers
+// (NT = cout/16 tiles per tap).  Per step of 16 voxels the block stages in
+// LDS, transposed so that each MFMA operand is one 8-byte read, the g tile
+// gT[co][16 voxels] and the 27 tap-shifted x tiles xT[tap][16 ci][16 voxels]
+// (zero outside the grid); the next step's global loads are issued before
+// the current step's MFMAs, and the LDS is double-buffered so one barrier
+// per step suffices.  The block's sums go to `part` (no atomics); a second
+// kernel adds the chunks in order.
+template <int NT>
+__global__ __launch_bounds__(256) void k_wgrad_tiles(const uint16_t *__restrict__ x, int cin,
+                                                     const uint16_t *__restrict__ g, Dims d,
+                                                     long long nvox, long long chunk,
+                                                     float *__restrict__ part) {
+  constexpr int COUT = NT * 16;
+  __shared__ __align__(16) uint16_t gT[2][COUT * 16];
+  __shared__ __align__(16) uint16_t xT[2][27 * 16 * 16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  const int nci = cin >> 4;
+  const int cit = blockIdx.x % nci;
+  const long long ch = blockIdx.x / nci;
+  const long long vb = ch * chunk, ve = min(vb + chunk, nvox);
+  const int nstep = (int)((ve - vb + 15) / 16);
+  // staging roles: g: thread t < 4*COUT loads 4 channels (8 bytes) of voxel
+  // t / (COUT/4); x: thread t handles voxel i = t & 15 and the (tap, quarter)
+  // pairs e = (t >> 4) + 16 k, k = 0..6 (108 pairs)
+  const int gi = t / (COUT / 4), gc = (t % (COUT / 4)) * 4;
+  const bool gload = t < 4 * COUT;
+  const int xi = t & 15;
+  uint2 gr = make_uint2(0u, 0u);
+  uint2 xr[7];
+  auto fetch = [&](int st) {
+    long long v0 = vb + 16ll * st;
+    long long vg = v0 + gi;
+    gr = make_uint2(0u, 0u);
+    if (gload && vg < ve) gr = *(const uint2 *)(g + vg * COUT + gc);
+    long long vx = v0 + xi;
+    int xw = 0, yy = 0, zz = 0;
+    bool vin = vx < ve;
+    if (vin) {
+      xw = (int)(vx % d.W);
+      long long r = vx / d.W;
+      yy = (int)(r % d.H);
+      zz = (int)(r / d.H);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      int e = (t >> 4) + 16 * k;  // 0..111
+      xr[k] = make_uint2(0u, 0u);
+      if (e < 108 && vin) {
+        int tap = e >> 2, q = e & 3;
+        int sz = zz + tap / 9 - 1, sy = yy + (tap / 3) % 3 - 1, sx = xw + tap % 3 - 1;
+        if (sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 && sx < d.W)
+          xr[k] = *(const uint2 *)(x + (((long long)sz * d.H + sy) * d.W + sx) * cin +
+                                   cit * 16 + q * 4);
+      }
+    }
+  };
+  auto stage = [&](int b) {
+    if (gload) {
+      uint16_t *p = gT[b];
+      p[(gc + 0) * 16 + gi] = (uint16_t)(gr.x & 0xffffu);
+      p[(gc + 1) * 16 + gi] = (uint16_t)(gr.x >> 16);
+      p[(gc + 2) * 16 + gi] = (uint16_t)(gr.y & 0xffffu);
+      p[(gc + 3) * 16 + gi] = (uint16_t)(gr.y >> 16);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      int e = (t >> 4) + 16 * k;
+      if (e < 108) {
+        int tap = e >> 2, q = e & 3;
+        uint16_t *p = xT[b] + tap * 256 + (q * 4) * 16 + xi;
+        p[0] = (uint16_t)(xr[k].x & 0xffffu);
+        p[16] = (uint16_t)(xr[k].x >> 16);
+        p[32] = (uint16_t)(xr[k].y & 0xffffu);
+        p[48] = (uint16_t)(xr[k].y >> 16);
+      }
+    }
+  };
+  f4 acc[7][NT];
+#pragma unroll
+  for (int a = 0; a < 7; a++)
+#pragma unroll
+    for (int c = 0; c < NT; c++) acc[a][c] = f4{0.f, 0.f, 0.f, 0.f};
+  if (nstep > 0) fetch(0);
+  for (int st = 0; st < nstep; st++) {
+    const int b = st & 1;
+    stage(b);
+    if (st + 1 < nstep) fetch(st + 1);
+    __syncthreads();
+    s4 A[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) A[c] = load4(gT[b] + (c * 16 + m) * 16 + 4 * kg);
+#pragma unroll
+    for (int a = 0; a < 7; a++) {
+      int tap = w + 4 * a;
+      if (tap < 27) {
+        s4 B = load4(xT[b] + tap * 256 + m * 16 + 4 * kg);
+#pragma unroll
+        for (int c = 0; c < NT; c++)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[c], B, acc[a][c], 0, 0, 0);
+      }
+    }
+  }
+  // lane holds D[4*kg + i][m]: co = c*16 + 4*kg + i, ci = cit*16 + m
+  float *pb = part + (long long)blockIdx.x * 27 * COUT * 16;
+#pragma unroll
+  for (int a = 0; a < 7; a++) {
+    int tap = w + 4 * a;
+    if (tap >= 27) continue;
+#pragma unroll
+    for (int c = 0; c < NT; c++)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        pb[((long long)tap * COUT + c * 16 + 4 * kg + i) * 16 + m] = acc[a][c][i];
+  }
+}
+
+// dw[(co*cin + ci)*27 + tap] += sum over the chunks, in chunk order, of the
+// partial sums of k_wgrad_tiles.
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ part, int cin,
+                                                      int cout, long long nchunk,
+                                                      float *__restrict__ dw) {
+  long long o = (long long)blockIdx.x * 256 + threadIdx.x;
+  long long n = 27ll * cout * cin;
+  if (o >= n) return;
+  int tap = (int)(o % 27);
+  long long r = o / 27;
+  int ci = (int)(r % cin), co = (int)(r / cin);
+  int nci = cin >> 4, cit = ci >> 4, cl = ci & 15;
+  float s = 0.f;
+  for (long long ch = 0; ch < nchunk; ch++)
+    s += part[((ch * nci + cit) * 27 + tap) * (long long)cout * 16 + (long long)co * 16 + cl];
+  dw[o] = dw[o] + s;
+}
+
 // Bias gradient: db[co] += sum_v g[v][co].
 __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ g, int cout,
                                                    long long nvox, float *__restrict__ db) {
@@ -412,13 +551,37 @@ int wm_unet_conv3_wgrad(const uint16_t *x, int64_t cin, const uint16_t *g, int64
                "wm_unet_conv3_wgrad: channels must be multiples of 16");
     long long nvox = D * H * W;
     Dims d{(int)D, (int)H, (int)W};
-    int chunk = 4096;
-    int nchunk = (int)((nvox + chunk - 1) / chunk);
-    long long waves = 27ll * (cout / 16) * (cin / 16) * nchunk;
     hipStream_t s = wm_stream(stream);
-    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, x,
-                       (int)cin, g, (int)cout, d, dw, nvox, chunk, nchunk);
-    WM_LAUNCHED();
+    if (cout <= 64) {
+      // about 512 blocks, chunks a multiple of 16 voxels
+      long long nci = cin / 16;
+      long long nchunk = std::max<long long>(1, std::min<long long>(512 / nci, (nvox + 255) / 256));
+      long long chunk = 16 * ((nvox + 16 * nchunk - 1) / (16 * nchunk));
+      nchunk = (nvox + chunk - 1) / chunk;
+      float *part = nullptr;
+      size_t bytes = sizeof(float) * (size_t)(nchunk * nci * 27 * cout * 16);
+      WM_HIP(hipMallocAsync((void **)&part, bytes, s));
+      unsigned nbk = (unsigned)(nchunk * nci);
+      switch (cout / 16) {
+        case 1: hipLaunchKernelGGL(k_wgrad_tiles<1>, dim3(nbk), dim3(256), 0, s, x, (int)cin, g, d, nvox, chunk, part); break;
+        case 2: hipLaunchKernelGGL(k_wgrad_tiles<2>, dim3(nbk), dim3(256), 0, s, x, (int)cin, g, d, nvox, chunk, part); break;
+        case 3: hipLaunchKernelGGL(k_wgrad_tiles<3>, dim3(nbk), dim3(256), 0, s, x, (int)cin, g, d, nvox, chunk, part); break;
+        default: hipLaunchKernelGGL(k_wgrad_tiles<4>, dim3(nbk), dim3(256), 0, s, x, (int)cin, g, d, nvox, chunk, part); break;
+      }
+      WM_LAUNCHED();
+      long long n = 27ll * cout * cin;
+      hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
+                         (int)cin, (int)cout, nchunk, dw);
+      WM_LAUNCHED();
+      WM_HIP(hipFreeAsync(part, s));
+    } else {
+      int chunk = 4096;
+      int nchunk = (int)((nvox + chunk - 1) / chunk);
+      long long waves = 27ll * (cout / 16) * (cin / 16) * nchunk;
+      hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, x,
+                         (int)cin, g, (int)cout, d, dw, nvox, chunk, nchunk);
+      WM_LAUNCHED();
+    }
     if (db) {
       WM_REQUIRE(256 % cout == 0 || cout <= 256, "wm_unet_conv3_wgrad: bad cout");
       hipLaunchKernelGGL(k_bias_grad, dim3(1024), dim3(256), 0, s, g, (int)cout, nvox, db);
