@@ -45,61 +45,82 @@ struct Dims {
   int D, H, W;
 };
 
-// Forward convolution.  One wavefront computes 16 consecutive voxels (in
-// x-fastest order; lanes past the last voxel load zeros and store nothing) for all output
-// channels (NT = Cout/16 <= 8 accumulators); a block is 4 wavefronts.  out = act(conv + bias), act the
-// leaky ReLU of slope `slope` (1 = identity); bias may be NULL.
-template <int NT>
+// Forward convolution.  One wavefront computes VT tiles of 16 consecutive
+// voxels (in x-fastest order; lanes past the last voxel load zeros and
+// store nothing) for all output channels (NT = Cout/16 <= 8 accumulators
+// per tile); each weight operand it loads feeds the MFMAs of all VT tiles.
+// A block is 4 wavefronts.  out = act(conv + bias), act the leaky ReLU of
+// slope `slope` (1 = identity); bias may be NULL.
+template <int NT, int VT>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, int cin,
                                                uint16_t *__restrict__ y, float *__restrict__ yf,
                                                Dims d, const uint16_t *__restrict__ wp,
                                                const float *__restrict__ bias, float slope,
                                                long long nvox) {
   int lane = threadIdx.x & 63;
-  long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 * VT;
   if (v0 >= nvox) return;
   int cout = NT * 16;
   int m = lane & 15, kg = lane >> 4;
-  // this lane's A-operand voxel
-  long long vm = v0 + m;
-  int xw = (int)(vm % d.W);
-  long long r = vm / d.W;
-  int yy = (int)(r % d.H);
-  int zz = (int)(r / d.H);
-  f4 acc[NT];
+  // this lane's A-operand voxel of each tile
+  long long vm[VT];
+  int xw[VT], yy[VT], zz[VT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < VT; j++) {
+    vm[j] = v0 + 16 * j + m;
+    xw[j] = (int)(vm[j] % d.W);
+    long long r = vm[j] / d.W;
+    yy[j] = (int)(r % d.H);
+    zz[j] = (int)(r / d.H);
+  }
+  f4 acc[VT][NT];
+#pragma unroll
+  for (int j = 0; j < VT; j++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[j][t] = f4{0.f, 0.f, 0.f, 0.f};
   int ncb = cin >> 4;
   for (int tap = 0; tap < 27; tap++) {
     int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    int sz = zz + dz, sy = yy + dy, sx = xw + dx;
-    bool in = vm < nvox && sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 && sx < d.W;
-    const uint16_t *src = x + (((long long)sz * d.H + sy) * d.W + sx) * cin + 4 * kg;
+    const uint16_t *src[VT];
+    bool in[VT];
+#pragma unroll
+    for (int j = 0; j < VT; j++) {
+      int sz = zz[j] + dz, sy = yy[j] + dy, sx = xw[j] + dx;
+      in[j] = vm[j] < nvox && sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 &&
+              sx < d.W;
+      src[j] = x + (in[j] ? (((long long)sz * d.H + sy) * d.W + sx) * cin : 0) + 4 * kg;
+    }
     const uint16_t *wt = wp + (long long)tap * ncb * cout * 16 + m * 16 + 4 * kg;
     for (int cb = 0; cb < ncb; cb++) {
-      s4 a = in ? load4(src + cb * 16) : s4{0, 0, 0, 0};
+      s4 a[VT];
+#pragma unroll
+      for (int j = 0; j < VT; j++) a[j] = in[j] ? load4(src[j] + cb * 16) : s4{0, 0, 0, 0};
 #pragma unroll
       for (int t = 0; t < NT; t++) {
         s4 b = load4(wt + ((long long)cb * cout + t * 16) * 16);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < VT; j++)
+          acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j], b, acc[j][t], 0, 0, 0);
       }
     }
   }
-  // lane holds D[4*kg + i][m]: voxel v0 + 4*kg + i, channel t*16 + m
+  // lane holds D[4*kg + i][m]: voxel v0 + 16*j + 4*kg + i, channel t*16 + m
 #pragma unroll
-  for (int t = 0; t < NT; t++) {
-    int co = t * 16 + m;
-    float bv = bias ? bias[co] : 0.f;
+  for (int j = 0; j < VT; j++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      float v = acc[t][i] + bv;
-      v = v > 0.f ? v : v * slope;
-      long long vi = v0 + 4 * kg + i;
-      if (vi >= nvox) continue;
-      if (y) y[vi * cout + co] = f2bf(v);
-      if (yf) yf[vi * cout + co] = v;
+    for (int t = 0; t < NT; t++) {
+      int co = t * 16 + m;
+      float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float v = acc[j][t][i] + bv;
+        v = v > 0.f ? v : v * slope;
+        long long vi = v0 + 16 * j + 4 * kg + i;
+        if (vi >= nvox) continue;
+        if (y) y[vi * cout + co] = f2bf(v);
+        if (yf) yf[vi * cout + co] = v;
+      }
     }
-  }
 }
 
 // Weight gradient in the master layout: dw[(co*cin + ci)*27 + tap] +=
@@ -525,18 +546,20 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
     WM_REQUIRE(D > 0 && H > 0 && W > 0, "wm_unet_conv3: empty grid");
     long long nvox = D * H * W;
     Dims d{(int)D, (int)H, (int)W};
-    int blocks = (int)(((nvox + 15) / 16 + 3) / 4);
+    // 4 voxel tiles per wavefront up to 64 output channels, 2 above
+    const int vt = cout <= 64 ? 4 : 2;
+    int blocks = (int)((((nvox + 15) / 16 + vt - 1) / vt + 3) / 4);
     hipStream_t s = wm_stream(stream);
     float sl = (float)slope;
     switch (cout / 16) {
-      case 1: hipLaunchKernelGGL(k_conv3<1>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 2: hipLaunchKernelGGL(k_conv3<2>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 3: hipLaunchKernelGGL(k_conv3<3>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 4: hipLaunchKernelGGL(k_conv3<4>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 5: hipLaunchKernelGGL(k_conv3<5>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 6: hipLaunchKernelGGL(k_conv3<6>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 7: hipLaunchKernelGGL(k_conv3<7>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 8: hipLaunchKernelGGL(k_conv3<8>, dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 1: hipLaunchKernelGGL((k_conv3<1, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 2: hipLaunchKernelGGL((k_conv3<2, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 3: hipLaunchKernelGGL((k_conv3<3, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 4: hipLaunchKernelGGL((k_conv3<4, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 5: hipLaunchKernelGGL((k_conv3<5, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 6: hipLaunchKernelGGL((k_conv3<6, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 7: hipLaunchKernelGGL((k_conv3<7, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 8: hipLaunchKernelGGL((k_conv3<8, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
       default: throw wm_error("wm_unet_conv3: cout > 128");
     }
     WM_LAUNCHED();
