@@ -308,6 +308,32 @@ int wm_getzmmnt(int64_t np, const double *xp, const double *yp,
                 const double *uzp, const double *gaminv, int64_t usegaminv,
                 double *mm, void *work, void *stream);
 
+/* gtlchg3dfromrho (dw3d.F:229-320; called by MultiGrid3D.gtlchg,
+ * multigrid.py:433-449) for one processor: planes[iz] (device, nz+1
+ * doubles) receives the charge summed over z plane iz of rho ((nx+1)(ny+1)
+ * (nz+1) doubles, no guard cells) -- the rows iy >= 1 doubled under
+ * l2symtry, the quadrant weights 1/2/4 under l4symtry (l2symtry wins when
+ * both are set, as in the reference) -- and linechg[0..nzzarr] (device) the
+ * line charge [C/m] interpolated onto the Z_arrays mesh zzmin + iz*dzz of
+ * the beam frame (zbeam).  Each plane is summed in the reference's order;
+ * the results are the reference's bits. */
+int wm_gtlchg3d(const double *rho, int64_t nx, int64_t ny, int64_t nz,
+                double dx, double dy, double dz, double zgrid, double zmmin,
+                double zzmin, double zbeam, double dzz, int64_t nzzarr,
+                int64_t l2symtry, int64_t l4symtry, double *planes,
+                double *linechg, void *stream);
+
+/* getese3dfromrhophi (dw3d.F:708-768; called by MultiGrid3D.getese,
+ * multigrid.py:451-467): ese[0] (device) = dx*dy*dz/2 * sum of rho*phi
+ * over the mesh nodes, rho without guard cells, phi with one guard cell,
+ * with the reference's symmetry factors (l4symtry wins when both are set)
+ * and summation order (columns over z, then over y, then over x).  work
+ * must hold wm_getese3d_worksize(nx, ny) bytes. */
+int64_t wm_getese3d_worksize(int64_t nx, int64_t ny);
+int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny,
+                int64_t nz, double dx, double dy, double dz, int64_t l4symtry,
+                int64_t l2symtry, void *work, double *ese, void *stream);
+
 /* particlesortxyzwithcopy (top.F:5095-5138) with particlesortbyindex
  * (top.F:5140-5203): the permutation that orders np particles by the block
  * index int(|x-xmmin|/dx) + (nx+1)*(iy + (ny+1)*iz), stable, so equal

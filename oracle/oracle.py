@@ -115,6 +115,12 @@ def _declare(L):
     L.oc_fixefield.argtypes = [P, _i64, _i64, _i64, _i64, P]
     L.oc_sete3d_conductor.argtypes = [P, P, P, P, P, P, P, _i64, P, P, P, _d, _d, _d, _d, _d,
                                       _d, _d, _i64, _i64, _i64, _i64, P, P, P]
+    L.oc_plane_sums.argtypes = [P, _i64, _i64, _i64, _i64, _i64, P]
+    L.oc_linechg_from_planes.argtypes = [P, _i64, _d, _d, _d, _d, _d, _d, _d, _d, _i64, P]
+    L.oc_gtlchg3d.argtypes = [P, _i64, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _d, _i64,
+                              _i64, _i64, P, P]
+    L.oc_getese3d.restype = _d
+    L.oc_getese3d.argtypes = [P, P, _i64, _i64, _i64, _d, _d, _d, _i64, _i64]
     L.oc_conductordelfromintercepts.restype = _i64
     L.oc_conductordelfromintercepts.argtypes = [P, P, _d, _d, _i64, P]
     L.oc_zcylinder_intercepts.argtypes = [_d, _d, _d, _d, _d, P, _d, _i64]
@@ -504,6 +510,45 @@ def getzmmnt(xp, yp, zp, uxp, uyp, uzp, gaminv, usegaminv=False):
     a = [_f64(v) for v in (xp, yp, zp, uxp, uyp, uzp, gaminv)]
     lib().oc_getzmmnt(len(a[0]), *[_p(v) for v in a], int(usegaminv), _p(mm))
     return mm
+
+
+def plane_sums(rho, l2symtry=False, l4symtry=False):
+    """The charge of each z plane of rho (nz+1, ny+1, nx+1), gtlchg3dfromrho's
+    scrtch (dw3d.F:254-279)."""
+    nz, ny, nx = (d - 1 for d in rho.shape)
+    out = np.zeros(nz + 1)
+    lib().oc_plane_sums(_p(_f64(rho)), nx, ny, nz, int(l2symtry), int(l4symtry), _p(out))
+    return out
+
+
+def linechg_from_planes(planes, dx, dy, dz, zgrid, zmmin, zzmin, zbeam, dzz, nzzarr):
+    """dw3d.F:282-316: plane sums -> linechg(0:nzzarr)."""
+    planes = _f64(planes)
+    out = np.zeros(nzzarr + 1)
+    lib().oc_linechg_from_planes(_p(planes), len(planes) - 1, dx, dy, dz, zgrid, zmmin,
+                                 zzmin, zbeam, dzz, nzzarr, _p(out))
+    return out
+
+
+def gtlchg3d(rho, dx, dy, dz, zgrid, zmmin, zzmin, zbeam, dzz, nzzarr, l2symtry=False,
+             l4symtry=False):
+    """gtlchg3dfromrho (dw3d.F:229-320): returns (linechg, plane sums)."""
+    rho = _f64(rho)
+    nz, ny, nx = (d - 1 for d in rho.shape)
+    scr = np.zeros(nz + 1)
+    lc = np.zeros(nzzarr + 1)
+    lib().oc_gtlchg3d(_p(rho), nx, ny, nz, dx, dy, dz, zgrid, zmmin, zzmin, zbeam, dzz,
+                      nzzarr, int(l2symtry), int(l4symtry), _p(scr), _p(lc))
+    return lc, scr
+
+
+def getese3d(rho, phi, dx, dy, dz, l4symtry=False, l2symtry=False):
+    """getese3dfromrhophi (dw3d.F:708-768): electrostatic energy of rho
+    (nz+1, ny+1, nx+1) in phi (nz+3, ny+3, nx+3)."""
+    nz, ny, nx = (d - 1 for d in rho.shape)
+    assert phi.shape == (nz + 3, ny + 3, nx + 3)
+    return float(lib().oc_getese3d(_p(_f64(rho)), _p(_f64(phi)), nx, ny, nz, dx, dy, dz,
+                                   int(l4symtry), int(l2symtry)))
 
 
 def sorhalfpass3d(parity, phi, rho, dx, dy, dz, bounds, mgparam=1.2, mgform=1,

@@ -14,6 +14,8 @@
 #ifndef WARP_ORACLE_H
 #define WARP_ORACLE_H
 #include <stdint.h>
+#include <stdlib.h>
+#include <math.h>
 
 /* Loops whose iterations write disjoint elements and read nothing another
  * iteration writes run in parallel under OpenMP; the results are the same
@@ -190,6 +192,19 @@ void oc_zsrfrv_intercepts(int64_t nn, const double *rsrf, const double *zsrf,
                           double xcent, double ycent, double zcent,
                           oc_intercepts *icpt, int64_t count_only,
                           int64_t *counts);
+
+/* oc_fielddiag.c: line charge and electrostatic energy (dw3d.F). */
+void oc_plane_sums(const double *rho, int64_t nx, int64_t ny, int64_t nz,
+                   int64_t l2symtry, int64_t l4symtry, double *scrtch);
+void oc_linechg_from_planes(const double *scrtch, int64_t nz, double dx, double dy,
+                            double dz, double zgrid, double zmmin, double zzmin,
+                            double zbeam, double dzz, int64_t nzzarr, double *linechg);
+void oc_gtlchg3d(const double *rho, int64_t nx, int64_t ny, int64_t nz, double dx,
+                 double dy, double dz, double zgrid, double zmmin, double zzmin,
+                 double zbeam, double dzz, int64_t nzzarr, int64_t l2symtry,
+                 int64_t l4symtry, double *scrtch, double *linechg);
+double oc_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny, int64_t nz,
+                   double dx, double dy, double dz, int64_t l4symtry, int64_t l2symtry);
 
 #ifdef __cplusplus
 }

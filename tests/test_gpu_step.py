@@ -89,7 +89,8 @@ def test_step_driver_matches_oracle(device, efetch, lcorrectede):
     saved_lc = f3d.lcorrectede
     saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time, nhist=top.nhist,
                  lspecial=top.lspecial, particlesortinterval=top.particlesortinterval,
-                 efetch=top.efetch)
+                 efetch=top.efetch, nzzarr=top.nzzarr, zzmin=top.zzmin, zzmax=top.zzmax,
+                 dzz=top.dzz, linechg=top.linechg, ese=top.ese)
     calls = []
     orig = s.gather_push
 
@@ -121,6 +122,14 @@ def test_step_driver_matches_oracle(device, efetch, lcorrectede):
         for key in ("xrms", "yrms", "zrms", "epsx", "epsy", "vzbar"):
             assert top.moments[key] == pytest.approx(mm[key], rel=1e-9), key
         assert top.it == 7 and top.lspecial
+        # field diagnostics of the last (special) step, on the solver's rho
+        # and phi (w3d.F:997-1003)
+        rho, phi = s.rho.cpu().numpy(), s.phi.cpu().numpy()
+        lco, _ = O.gtlchg3d(rho, s.dx, s.dy, s.dz, top.zgrid, s.zmmin, top.zzmin, top.zbeam,
+                            top.dzz, top.nzzarr)
+        assert np.array_equal(top.linechg.cpu().numpy(), lco)
+        assert top.ese == O.getese3d(rho, phi, s.dx, s.dy, s.dz)
+        assert top.hist[-1]["it"] == 5 and top.hist[-1]["ese"] > 0.
     finally:
         f3d.lcorrectede = saved_lc
         warp_amd.unregistersolver(s)

@@ -775,3 +775,112 @@ def test_fixefield_mesh_aligned_face():
     np.testing.assert_allclose(se[:, :, ic, 0], -g, rtol=1e-12)
     assert np.all(se[:, :, :ic, 0] == 0.)
     np.testing.assert_allclose(se[:, :, ic + 1:nx, 0], -g, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Field diagnostics: line charge (gtlchg3dfromrho) and electrostatic energy
+# (getese3dfromrhophi)
+# ---------------------------------------------------------------------------
+def _with_guards(interior, fill=1.0e30):
+    """phi with one guard cell around interior, the guards holding a value
+    that would dominate any sum they entered."""
+    return np.pad(interior, 1, constant_values=fill)
+
+
+def test_getese_constant_fields_closed_form():
+    """rho = 1/4 and phi = 2 on every node: ese = dx*dy*dz/2 * 1/2 * (number
+    of nodes), exact in binary; the 1e30 guard cells of phi must not enter."""
+    nx, ny, nz = 6, 4, 5
+    dx, dy, dz = 0.5, 0.25, 0.125
+    rho = np.full((nz + 1, ny + 1, nx + 1), 0.25)
+    phi = _with_guards(np.full((nz + 1, ny + 1, nx + 1), 2.0))
+    e = O.getese3d(rho, phi, dx, dy, dz)
+    assert e == 0.5 * dx * dy * dz * 0.5 * (nx + 1) * (ny + 1) * (nz + 1)
+
+
+def test_getese_is_half_the_integral_of_rho_phi():
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 9, 7, 12
+    dx, dy, dz = 0.1, 0.2, 0.05
+    rho = rng.standard_normal((nz + 1, ny + 1, nx + 1))
+    inner = rng.standard_normal((nz + 1, ny + 1, nx + 1))
+    e = O.getese3d(rho, _with_guards(inner), dx, dy, dz)
+    assert e == pytest.approx(0.5 * dx * dy * dz * float(np.sum(rho * inner)), rel=1e-13)
+
+
+def _mirror(a, axes):
+    """The full mesh of a field stored only for index >= 0 along axes
+    (numpy axes of a (z, y, x) array): reflected through index 0."""
+    for ax in axes:
+        a = np.concatenate([np.flip(a, axis=ax).take(range(a.shape[ax] - 1), axis=ax), a],
+                           axis=ax)
+    return a
+
+
+@pytest.mark.parametrize("sym", ["l2symtry", "l4symtry"])
+def test_symmetry_factors_equal_the_reflected_full_mesh(sym):
+    """l4symtry stores the quadrant x, y >= 0: each plane's charge and the
+    energy equal those of the full mesh made by reflecting it through x = 0
+    and y = 0.  l2symtry: the plane charge reflects through y = 0
+    (dw3d.F:262-265) and the energy through x = 0 (the reference doubles the
+    columns ix >= 1, dw3d.F:752-756)."""
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 5, 4, 6
+    dx, dy, dz = 0.1, 0.2, 0.05
+    rho = rng.standard_normal((nz + 1, ny + 1, nx + 1))
+    inner = rng.standard_normal((nz + 1, ny + 1, nx + 1))
+    kw = {sym: True}
+    if sym == "l4symtry":
+        ax_lc, ax_e = (1, 2), (1, 2)
+    else:
+        ax_lc, ax_e = (1,), (2,)
+    full_planes = O.plane_sums(_mirror(rho, ax_lc))
+    np.testing.assert_allclose(O.plane_sums(rho, **kw), full_planes, rtol=1e-13, atol=0)
+    ef = O.getese3d(_mirror(rho, ax_e), _with_guards(_mirror(inner, ax_e)), dx, dy, dz)
+    assert O.getese3d(rho, _with_guards(inner), dx, dy, dz, **kw) == pytest.approx(ef, rel=1e-13)
+
+
+def test_line_charge_integrates_to_the_deposited_charge():
+    """CIC charge of random particles (setrho3d) summed plane by plane and
+    interpolated onto a Z_arrays mesh aligned with the grid:
+    sum(linechg)*dz is the total charge q*w*np, and each plane sum is the
+    plane's charge density times 1/(dx*dy) of its line charge."""
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 9, 7, 12
+    dx, dy, dz = 0.1, 0.2, 0.05
+    x0, y0, z0 = -0.45, -0.7, 0.1
+    n = 5000
+    x = x0 + rng.uniform(0, nx * dx, n)
+    y = y0 + rng.uniform(0, ny * dy, n)
+    z = z0 + rng.uniform(0, nz * dz, n)
+    q, w = 1.6e-19, 3.0e5
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    O.setrho3d(rho, x, y, z, q, w, dx, dy, dz, x0, y0, z0)
+    lc, planes = O.gtlchg3d(rho, dx, dy, dz, 0., z0, z0, 0., dz, nz)
+    assert lc.sum() * dz == pytest.approx(q * w * n, rel=1e-12)
+    np.testing.assert_allclose(planes, rho.sum(axis=(1, 2)), rtol=1e-13, atol=0)
+    # iz*dzz/dz is iz up to rounding, so the weights are 1 and 0 to ~1e-16
+    np.testing.assert_allclose(lc, planes * dx * dy, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("zbeam,nzzarr,dzz_over_dz,zgrid", [
+    (0.013, 12, 1.0, 0.0), (-0.021, 27, 0.6, 0.0), (0.004, 7, 2.3, 0.031),
+    (0.9, 12, 1.0, 0.0)])
+def test_line_charge_interpolation_is_linear_in_the_beam_frame(zbeam, nzzarr, dzz_over_dz,
+                                                               zgrid):
+    """Point iz of the Z_arrays mesh sits at zzmin + iz*dzz in the beam
+    frame, zbeam + zzmin + iz*dzz in the lab and (that - zgrid - zmmin)/dz
+    in grid units; its line charge is the plane charges times dx*dy,
+    linearly interpolated there: zero below the first plane and tapering to
+    zero over the cell above the last (dw3d.F:305-313)."""
+    rng = np.random.default_rng(17)
+    nz, dx, dy, dz = 12, 0.1, 0.2, 0.05
+    zmmin, zzmin = 0.1, 0.08
+    planes = rng.uniform(0.5, 2.0, nz + 1)
+    dzz = dzz_over_dz * dz
+    lc = O.linechg_from_planes(planes, dx, dy, dz, zgrid, zmmin, zzmin, zbeam, dzz, nzzarr)
+    zg = (zbeam + zzmin + np.arange(nzzarr + 1) * dzz - zgrid - zmmin) / dz
+    want = np.interp(zg, np.arange(nz + 2), np.append(planes * dx * dy, 0.), left=0., right=0.)
+    np.testing.assert_allclose(lc, want, rtol=0, atol=1e-14 * (planes * dx * dy).max())
+    if zbeam == 0.9:
+        assert not lc.any()

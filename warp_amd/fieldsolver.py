@@ -72,6 +72,33 @@ def fetchphiregistered():
     _each("fetchphi")
 
 
+# The diagnostics step3d calls through the registered solvers
+# (fieldsolver.py:234-257; w3d.F:987-1002): rhodia every step, the others
+# on special steps.
+def rhodiaregistered():
+    _each("rhodia")
+
+
+def gtlchgregistered():
+    _each("gtlchg")
+
+
+def srhoaxregistered():
+    _each("srhoax")
+
+
+def geteseregistered():
+    _each("getese")
+
+
+def sphiaxregistered():
+    _each("sphiax")
+
+
+def sezaxregistered():
+    _each("sezax")
+
+
 def fieldsolve(iwhich=0):
     """scripts/fieldsolver.py:49-89 for a registered solver."""
     for f in getregisteredsolvers():
@@ -160,3 +187,23 @@ class FieldSolver:
     def installconductor(self, conductor, xmin=None, xmax=None, ymin=None, ymax=None,
                          zmin=None, zmax=None, dfill=None):
         raise NotImplementedError
+
+    # Diagnostics a solver may provide (fieldsolver.py:1053-1064): none by
+    # default.
+    def rhodia(self):
+        pass
+
+    def gtlchg(self):
+        pass
+
+    def srhoax(self):
+        pass
+
+    def getese(self):
+        pass
+
+    def sphiax(self):
+        pass
+
+    def sezax(self):
+        pass

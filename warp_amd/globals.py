@@ -51,7 +51,12 @@ top = Top(dt=0., time=0., it=0, fstype=-1, pgroup=None, zbeam=0., zgrid=0.,
           allspecl=False, lspecial=False, nspecial=0,
           # beam moments of the last special step, and the moment history
           # (every nhist steps): dicts keyed by top's variable names
-          moments={}, hist=[])
+          moments={}, hist=[],
+          # Z_arrays mesh of the line charge (top.v:1615-1629; set from the
+          # field mesh by w3d.F:337-341 when left zero) and the field
+          # diagnostics of special steps: line charge [C/m] and
+          # electrostatic energy [J] (top.v:1898)
+          nzzarr=0, zzmin=0., zzmax=0., dzz=0., linechg=None, ese=0.)
 
 # w3d: mesh and boundary inputs (w3d.v), plus the FieldSolveAPI globals
 # (w3d.v:432-458) through which fetche receives particle positions.
@@ -60,6 +65,8 @@ w3d = W3D(nx=2, ny=2, nz=2, xmmin=0., xmmax=1., ymmin=0., ymmax=1., zmmin=0., zm
           nzguardrho=0, nxguarde=0, nyguarde=0, nzguarde=0,
           bound0=dirichlet, boundnz=dirichlet, boundxy=dirichlet,
           l2symtry=False, l4symtry=False,
+          # field diagnostics switches (w3d.v:70-74)
+          lgetese3d=True, lgtlchg3d=True,
           # FieldSolveAPI
           npfsapi=0, xfsapi=None, yfsapi=None, zfsapi=None, exfsapi=None,
           eyfsapi=None, ezfsapi=None, phifsapi=None, pgroupfsapi=None,

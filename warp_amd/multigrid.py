@@ -74,6 +74,8 @@ class MultiGrid3D(FieldSolver):
         self._cf = None            # conductor field data for f3d.lcorrectede
         self._cf_key = None
         self._cond_gen = 0         # counts uploads of conductor lists
+        self._planes = None        # plane charges of gtlchg
+        self._esework = None       # column sums of getese, then ese itself
         b = (C.c_int64 * 6)(*self.bounds)
         h = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -381,6 +383,47 @@ class MultiGrid3D(FieldSolver):
             return
         N.call("wm_phi_extrapolate", self._phi.data_ptr(), self._phiprev.data_ptr(),
                self._phi.numel(), N.stream_of(self.device))
+
+    # ------------------------------------------------------------------
+    # field diagnostics of special steps (multigrid.py:433-467)
+    def gtlchg(self):
+        """Line charge of rho into top.linechg (gtlchg3dfromrho,
+        dw3d.F:229-320): a float64 tensor on the device with the
+        top.nzzarr+1 points of the Z_arrays mesh.  The Z_arrays mesh
+        defaults to the field mesh (w3d.F:337-341)."""
+        if not w3d.lgtlchg3d:
+            return
+        if top.nzzarr == 0:
+            top.nzzarr = self.nz
+        if top.zzmin == 0.:
+            top.zzmin = self.zmmin
+        if top.zzmax == 0.:
+            top.zzmax = self.zmmax
+        top.dzz = (top.zzmax - top.zzmin) / top.nzzarr
+        lc = top.linechg
+        if not (isinstance(lc, torch.Tensor) and lc.device == self.device
+                and lc.numel() == top.nzzarr + 1):
+            lc = top.linechg = torch.zeros(top.nzzarr + 1, dtype=torch.float64,
+                                           device=self.device)
+        if self._planes is None or self._planes.numel() != self.nz + 1:
+            self._planes = torch.empty(self.nz + 1, dtype=torch.float64, device=self.device)
+        N.call("wm_gtlchg3d", self._rho.data_ptr(), self.nx, self.ny, self.nz, self.dx, self.dy,
+               self.dz, top.zgrid, self.zmminlocal, top.zzmin, top.zbeam, top.dzz, top.nzzarr,
+               int(self.l2symtry), int(self.l4symtry), self._planes.data_ptr(), lc.data_ptr(),
+               N.stream_of(self.device))
+
+    def getese(self):
+        """Electrostatic energy sum(rho*phi)*dx*dy*dz/2 into top.ese
+        (getese3dfromrhophi, dw3d.F:708-768)."""
+        need = int(N.lib().wm_getese3d_worksize(self.nx, self.ny))
+        ncol = need // 8
+        if self._esework is None or self._esework.numel() < ncol + 1:
+            self._esework = torch.empty(ncol + 1, dtype=torch.float64, device=self.device)
+        out = self._esework[ncol:ncol + 1]
+        N.call("wm_getese3d", self._rho.data_ptr(), self._phi.data_ptr(), self.nx, self.ny,
+               self.nz, self.dx, self.dy, self.dz, int(self.l4symtry), int(self.l2symtry),
+               self._esework.data_ptr(), out.data_ptr(), N.stream_of(self.device))
+        top.ese = float(out.item())
 
     def getresidual(self):
         """residual3d (f3d_mgrid.F:3367-3598) of the current phi and rho,

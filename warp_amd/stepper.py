@@ -4,8 +4,10 @@
 time level) else "fullv" (fetch E, half kicks, drift), particle boundaries,
 charge deposition (loadrho) and the field solve; a special step then ends
 with padvnc3d "synchv" (fetch E at the new positions, half kick) and the
-beam moments (getzmmnt).  ``generate`` does the initial deposit and solve
-and the moments of the loaded beam (w3dgen -> step3d("w3dgen"), "gen").
+beam moments (getzmmnt) and the field diagnostics (line charge top.linechg,
+electrostatic energy top.ese).  ``generate`` does the initial deposit and
+solve and the diagnostics of the loaded beam (w3dgen -> step3d("w3dgen"),
+"gen").
 """
 import time
 
@@ -48,7 +50,23 @@ def generate():
     if pg is not None and pg.npmax > 0:
         padvnc3d("gen", pg, top.dt)
         top.moments = diagnostics.getzmmnt(pg)
+    field_diagnostics(special=True)
     top.lspecial = True
+
+
+def field_diagnostics(special):
+    """The field diagnostics at the end of step3d (w3d.F:987-1002): rhodia
+    on every step; line charge, on-axis rho, electrostatic energy, on-axis
+    phi and Ez on special steps."""
+    if not FS.getregisteredsolvers():
+        return
+    FS.rhodiaregistered()
+    if special:
+        FS.gtlchgregistered()
+        FS.srhoaxregistered()
+        FS.geteseregistered()
+        FS.sphiaxregistered()
+        FS.sezaxregistered()
 
 
 def particleboundaries3d(pg, solver, nlost=None):
@@ -129,6 +147,7 @@ def step(n=1, pgroup=None, boundaries=True):
             top.nspecial += 1
             padvnc3d("synchv", pg, top.dt)
             top.moments = diagnostics.getzmmnt(pg)
-            if lhist:
-                top.hist.append(dict(it=top.it, time=top.time, **top.moments))
+        field_diagnostics(top.lspecial)
+        if top.lspecial and lhist:
+            top.hist.append(dict(it=top.it, time=top.time, ese=top.ese, **top.moments))
         top.steptime += time.perf_counter() - t0
