@@ -207,6 +207,7 @@ struct Level {
   DevBuf<int> int_pos;
   DevBuf<double> int_volt;
   DevBuf<uint32_t> mask;
+  DevBuf<uint32_t> imask;  // interior conductor points only (zeroed by k_expand_rb)
   bool hasmask = false;
   // an active subgrid record sits on an interior (pinned) point, so the
   // pin after each half sweep is not a no-op
@@ -237,6 +238,12 @@ struct wm_mg3d {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // captured V-cycle graphs
   hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec_rest = nullptr;
+  void drop_graphs() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gexec_rest) (void)hipGraphExecDestroy(gexec_rest);
+    gexec = gexec_rest = nullptr;
+  }
   struct GKey {
     int prec = -1, mgform = -1, down = -1, up = -1, lcnd = -1, icnd = -1, maxit = -1;
     double mgparam = -1., mgtol = -1.;
@@ -260,7 +267,7 @@ struct wm_mg3d {
   } prof;
   ~wm_mg3d() {
     for (hipEvent_t e : prof.ev) (void)hipEventDestroy(e);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    drop_graphs();
     if (state_h) (void)hipHostFree(state_h);
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
@@ -544,6 +551,9 @@ static void build_mask(wm_mg3d *h, Level &L, int level, const Params &P) {
     WM_HIP(hipMemcpy(hp.data(), L.int_pos.p, sizeof(int) * hp.size(), hipMemcpyDeviceToHost));
     for (int i = 0; i < L.nint; i++) setbit(hp[3 * i], hp[3 * i + 1], hp[3 * i + 2]);
     any = true;
+    L.imask.upload(m);
+  } else {
+    L.imask.release();
   }
   L.sub_on_interior = false;
   if (P.lcnd) {
@@ -824,14 +834,33 @@ static void expand_from(Level &L, T *phi, FI F, const T *pc, CI Cc, const int *s
 }
 
 
+// On a level without voltages (an error level: coarser than the finest, or
+// the finest in the residual-correction form) the pin that starts the
+// first pass after the expansion writes zero at the interior conductor
+// points.  When nothing comes between the two -- six Dirichlet boundaries
+// (launch_bc does nothing) and no Neumann subgrid mirror -- the expansion
+// writes that zero itself and the pin is left out: the values are those of
+// expand-then-pin.
+static bool pin_fusable(const wm_mg3d *h, const Level &L, int level, const Params &P) {
+  const int *b = h->bounds;
+  for (int i = 0; i < 6; i++)
+    if (b[i] != 0) return false;
+  const bool usevolt = (level == 0 && P.mgform == 1);
+  return !usevolt && L.nint > 0 && L.imask.p != nullptr && !(P.icnd == 2 && L.neumann);
+}
+
 template <typename S>
-static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
+static void expand_rb(wm_mg3d *h, Level &L, Level &Lc, int level, const Params &P,
+                      const int *skip, hipStream_t s, bool fusepin = false) {
   typedef typename Coarse<S>::type SC;
   ExpandAxis ax{L.ej[0].p, L.ew[0].p, L.elo[0], L.ehi[0]};
   ExpandAxis ay{L.ej[1].p, L.ew[1].p, L.elo[1], L.ehi[1]};
   ExpandAxis az{L.ej[2].p, L.ew[2].p, L.elo[2], L.ehi[2]};
   long long rows = 2ll * std::max(0, ay.hi - ay.lo + 1) * std::max(0, az.hi - az.lo + 1);
-  if (rows == 0 || ax.hi < ax.lo) return;
+  if (rows == 0 || ax.hi < ax.lo) {
+    if (fusepin) pin_and_mirror<S>(h, L, level, P, phi_of<S>(L).p, 0, skip, s);
+    return;
+  }
   WM_REQUIRE(2 * L.g.cs < (1ll << 31) && 2 * Lc.g.cs < (1ll << 31),
              "expand3d: level too large for 32-bit indexing");
   // elements j = jlo..jlo+nj-1 hold the fine points ix = 2j-1, 2j in range
@@ -840,14 +869,15 @@ static void expand_rb(Level &L, Level &Lc, const int *skip, hipStream_t s) {
   int nyr = ay.hi - ay.lo + 1, nzr = az.hi - az.lo + 1;
   int bpc = (int)cdiv((long long)nyr * nj, 256);
   // 16 planes per thread, 4 when that leaves fewer than 1024 blocks
+  const uint32_t *imask = fusepin ? L.imask.p : nullptr;
   if ((long long)bpc * cdiv(nzr, 16) >= 1024) {
     hipLaunchKernelGGL((k_expand_rb<S, SC, 16>), dim3(bpc * (int)cdiv(nzr, 16)), dim3(256), 0, s,
                        phi_of<S>(L).p, L.g, (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, jlo,
-                       nj, bpc, skip);
+                       nj, bpc, imask, skip);
   } else {
     hipLaunchKernelGGL((k_expand_rb<S, SC, 4>), dim3(bpc * (int)cdiv(nzr, 4)), dim3(256), 0, s,
                        phi_of<S>(L).p, L.g, (const SC *)phi_of<SC>(Lc).p, Lc.g, ax, ay, az, jlo,
-                       nj, bpc, skip);
+                       nj, bpc, imask, skip);
   }
   WM_LAUNCHED();
 }
@@ -911,7 +941,7 @@ static int zchunk_for_rounds(long long cols, int ncz, long long slots) {
 
 template <typename S, typename TR, typename TC>
 static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params &P,
-                        const int *skip, hipStream_t s) {
+                        const int *skip, hipStream_t s, TC *zfill, long long nzfill) {
   typedef typename St<S>::T T;
   typedef typename RestrictArith<TR, TC>::type RA;
   const S *phi = phi_of<S>(L).p;
@@ -956,17 +986,19 @@ static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params
   if (zc_env > 0) zc = zc_env;
   int ntz = (int)cdiv(ncz, zc);
   long long nb = cols * ntz;
-  WM_REQUIRE(nb < (1ll << 31), "restrict3d: too many blocks");
+  // extra blocks zeroing zfill, about 8 elements per thread
+  long long nbz = nzfill > 0 ? std::max<long long>(1, cdiv(nzfill, 8ll * threads)) : 0;
+  WM_REQUIRE(nb + nbz < (1ll << 31), "restrict3d: too many blocks");
   if (L.hasmask)
-    hipLaunchKernelGGL((k_resrestrict<S, TR, TC, true>), dim3((int)nb), dim3(threads), shm, s, phi,
-                       rho, (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic, dzsqic,
-                       (T)P.mgparam, (const uint32_t *)L.mask.p, cxt, cyt, zc, ntx, nty, (int)nb,
-                       L.ff, skip);
+    hipLaunchKernelGGL((k_resrestrict<S, TR, TC, true>), dim3((int)(nb + nbz)), dim3(threads), shm,
+                       s, phi, rho, (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic,
+                       dzsqic, (T)P.mgparam, (const uint32_t *)L.mask.p, cxt, cyt, zc, ntx, nty,
+                       (int)nb, L.ff, zfill, nzfill, skip);
   else
-    hipLaunchKernelGGL((k_resrestrict<S, TR, TC, false>), dim3((int)nb), dim3(threads), shm, s,
-                       phi, rho, (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic, dysqic,
-                       dzsqic, (T)P.mgparam, (const uint32_t *)nullptr, cxt, cyt, zc, ntx, nty,
-                       (int)nb, L.ff, skip);
+    hipLaunchKernelGGL((k_resrestrict<S, TR, TC, false>), dim3((int)(nb + nbz)), dim3(threads),
+                       shm, s, phi, rho, (const TR *)res, rho_of<TC>(Lc).p, L.g, Lc.g, dxsqic,
+                       dysqic, dzsqic, (T)P.mgparam, (const uint32_t *)nullptr, cxt, cyt, zc, ntx,
+                       nty, (int)nb, L.ff, zfill, nzfill, skip);
   WM_LAUNCHED();
 }
 
@@ -1003,12 +1035,17 @@ static void zero_interior_res(wm_mg3d *h, const Params &P, hipStream_t s) {
 // zero-filled) and the Neumann mirrors copy zeros, so it is left out.
 template <typename S>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
-                   bool zeroed = false) {
+                   bool zeroed = false, bool repin = true) {
   typedef typename St<S>::T T;
   typedef typename Coarse<S>::type SC;
   Level &L = *h->lv[level];
   const bool novolt = !(level == 0 && P.mgform == 1);
-  for (int i = 0; i < P.down; i++) sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0));
+  bool fusepin = false;
+  // The first pass pins unless phi starts at zero without voltages, or
+  // (repin false) the caller knows the pinned points still hold their
+  // values from the previous V-cycle.
+  for (int i = 0; i < P.down; i++)
+    sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0) && (repin || i > 0));
   if (L.coarser) {
     Level &Lc = *h->lv[level + 1];
     bool anyzero = false;
@@ -1016,22 +1053,26 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
       ensure_records<T>(h, L, level, P, true, 0);
       anyzero = cond_of<T>(L).res[0][0].anyzero || cond_of<T>(L).res[0][1].anyzero;
     }
+    // the coarse phi starts at zero: zeroed by the fused kernel's extra
+    // blocks, or by k_fill after the separate residual and restriction
     if (fused_restrict_ok(h, L, anyzero)) {
-      resrestrict<S, T, SC>(h, L, Lc, level, P, skip, s);
+      resrestrict<S, T, SC>(h, L, Lc, level, P, skip, s, phi_of<SC>(Lc).p, 2 * Lc.g.cs);
     } else {
       residual_rb<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
                         s);
       restrict_to<T, SC>(L, (const T *)res_of<T>(L).p, RBIdx{L.g}, rho_of<SC>(Lc).p,
                          RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny, Lc.nz);
+      hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
+                         phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
+      WM_LAUNCHED();
     }
-    hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
-                       phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
-    WM_LAUNCHED();
     vcycle<SC>(h, level + 1, P, skip, s, true);
-    expand_rb<S>(L, Lc, skip, s);
+    fusepin = P.up > 0 && pin_fusable(h, L, level, P);
+    expand_rb<S>(h, L, Lc, level, P, skip, s, fusepin);
     launch_bc<S>(phi_of<S>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
   }
-  for (int i = 0; i < P.up; i++) sorpass<S>(h, L, level, P, skip, s);
+  // the first up pass's pin already ran inside the expansion when fusepin
+  for (int i = 0; i < P.up; i++) sorpass<S>(h, L, level, P, skip, s, !(fusepin && i == 0));
 }
 
 template <typename T>
@@ -1080,12 +1121,27 @@ static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long lo
 }
 
 // One iteration of the multigrid3dsolve loop (f3d_mgrid.F:313-434).
+// Whether a V-cycle of the plain FP64 solve may skip the pin that starts
+// its first pass on the finest level: the previous V-cycle ended with
+// passes that began with a pin (after the expansion), and under these
+// conditions nothing since wrote a pinned point -- the dense sweeps skip
+// interior points, no subgrid record writes one, no periodic wrap copies
+// over one, no Neumann ghost needs mirroring (the same conditions under
+// which halfpass leaves out the pin after a sweep); the convergence test
+// only reads phi.
+static bool repin_skippable(const wm_mg3d *h, const Params &P) {
+  const Level &L0 = *h->lv[0];
+  const int *b = h->bounds;
+  bool periodic = b[0] == 2 || b[2] == 2 || b[4] == 2;
+  return !(L0.sub_on_interior || periodic || (P.icnd == 2 && L0.neumann));
+}
+
 static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int mgmaxiters,
-                      hipStream_t s) {
+                      hipStream_t s, bool repin = true) {
   Level &L0 = *h->lv[0];
   const int *skip = &h->state.p->done;
   if (prec == WM_PREC_FP64 && P.mgform == 1) {
-    vcycle<double>(h, 0, P, skip, s);
+    vcycle<double>(h, 0, P, skip, s, false, repin);
   } else {
     // Residual-correction form: the error equation on the residual of the
     // saved phi (mgform == 2), with the V-cycle in double or float.
@@ -1267,15 +1323,13 @@ int wm_mg3d_set_conductors(wm_mg3d *h, const wm_conductor_lists *c) {
       L.neumann = false;
       L.hasmask = false;
       L.mask.release();
+      L.imask.release();
       L.c64.reset();
       L.c32.reset();
     }
     h->conductors_set = false;
-    if (h->gexec) {
-      (void)hipGraphExecDestroy(h->gexec);
-      h->gexec = nullptr;
-      h->gkey = wm_mg3d::GKey();
-    }
+    h->drop_graphs();
+    h->gkey = wm_mg3d::GKey();
     if (c == nullptr) return 0;
     WM_REQUIRE(c->levels >= (int64_t)h->lv.size() || (c->int_n == 0 && c->sub_n[0] == 0 && c->sub_n[1] == 0),
                "wm_mg3d_set_conductors: conductor data has fewer levels than the solver");
@@ -1389,9 +1443,10 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     if (mgmaxiters > 0 && (eager_env || h->prof.on)) {
       // Plain stream launches, one V-cycle at a time (and the event timing
       // of the finest-level half sweeps when it is switched on).
+      const bool noskip = !repin_skippable(h, P);
       for (int k = 0; k < mgmaxiters; k++) {
         h->prof.used = 0;
-        iteration(h, prec, P, mgtol, (int)mgmaxiters, s);
+        iteration(h, prec, P, mgtol, (int)mgmaxiters, s, k == 0 || noskip);
         WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         WM_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < h->prof.used; i++) {
@@ -1419,30 +1474,36 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       key.mgparam = mgparam;
       key.mgtol = mgtol;
       if (!h->gexec || !(h->gkey == key)) {
-        if (h->gexec) {
-          WM_HIP(hipGraphExecDestroy(h->gexec));
-          h->gexec = nullptr;
-        }
-        hipGraph_t graph;
-        WM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-          iteration(h, prec, P, mgtol, (int)mgmaxiters, s);
-        } catch (...) {
-          hipGraph_t g2;
-          (void)hipStreamEndCapture(s, &g2);
-          if (g2) (void)hipGraphDestroy(g2);
-          throw;
-        }
-        WM_HIP(hipStreamEndCapture(s, &graph));
-        WM_HIP(hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0));
-        WM_HIP(hipGraphDestroy(graph));
+        h->drop_graphs();
+        // gexec: one V-cycle and the convergence test; gexec_rest: the
+        // same without the first pin, for the V-cycles after the first
+        // when repin_skippable holds
+        auto capture = [&](hipGraphExec_t *out, bool repin) {
+          hipGraph_t graph;
+          WM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          try {
+            iteration(h, prec, P, mgtol, (int)mgmaxiters, s, repin);
+          } catch (...) {
+            hipGraph_t g2;
+            (void)hipStreamEndCapture(s, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            throw;
+          }
+          WM_HIP(hipStreamEndCapture(s, &graph));
+          hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          WM_HIP(e);
+        };
+        capture(&h->gexec, true);
+        if (prec == WM_PREC_FP64 && P.mgform == 1 && repin_skippable(h, P))
+          capture(&h->gexec_rest, false);
         h->gkey = key;
       }
       int batch = std::max(1, std::min<int>(h->last_iters > 0 ? h->last_iters : 4, (int)mgmaxiters));
       int launched = 0;
       for (;;) {
         for (int b = 0; b < batch && launched < mgmaxiters; b++, launched++)
-          WM_HIP(hipGraphLaunch(h->gexec, s));
+          WM_HIP(hipGraphLaunch(launched > 0 && h->gexec_rest ? h->gexec_rest : h->gexec, s));
         WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         WM_HIP(hipStreamSynchronize(s));
         if (h->state_h->done || launched >= mgmaxiters) break;

@@ -722,7 +722,8 @@ template <> struct RestrictArith<double, double> { typedef double type; };
 // every load one plane ahead.  Phase 2 adds the plane's 3x3 terms to each
 // coarse point's running sum in restrict3d's order (planes 2jz-1, 2jz,
 // 2jz+1; within a plane rows, then columns), so the sums and the result are
-// those of k_residual_zmarch + k_restrict, bit for bit.
+// those of k_residual_zmarch + k_restrict, bit for bit.  Blocks past the
+// nblocks tiles zero the coarse phi (one launch fewer per level).
 // ---------------------------------------------------------------------------
 //
 // Every plane issues the same loads unconditionally (the last plane's
@@ -735,12 +736,21 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
     TC *__restrict__ out, RBGeom g, RBGeom gc, typename St<S>::T dxsqic,
     typename St<S>::T dysqic, typename St<S>::T dzsqic, typename St<S>::T mgparam,
     const uint32_t *__restrict__ mask, int cxt, int cyt, int zc, int ntx, int nty, int nblocks,
-    double ff, const int *__restrict__ skip) {
+    double ff, TC *__restrict__ zfill, long long nzfill, const int *__restrict__ skip) {
   typedef typename St<S>::T T;
   typedef typename RestrictArith<TR, TC>::type RA;
   extern __shared__ __align__(16) unsigned char wm_smem[];
   RA *lres = reinterpret_cast<RA *>(wm_smem);
   WM_SKIP();
+  if ((int)blockIdx.x >= nblocks) {
+    // the blocks past the tiles zero zfill[0, nzfill): the coarse level's
+    // phi, its initial error, which nothing in this launch reads
+    const long long stride = (long long)(gridDim.x - nblocks) * blockDim.x;
+    for (long long t = (long long)(blockIdx.x - nblocks) * blockDim.x + threadIdx.x; t < nzfill;
+         t += stride)
+      zfill[t] = (TC)0.;
+    return;
+  }
   const int ncx = gc.nx - 1, ncy = gc.ny - 1, ncz = gc.nz - 1;  // coarse range [1, nc-1]
   const int lb = xcd_remap(blockIdx.x, nblocks);
   const int tx = lb % ntx, tr = lb / ntx, ty = tr % nty, tz = tr / nty;
@@ -937,7 +947,8 @@ __device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlan
 template <typename S, typename SC, int ZC>
 __global__ __launch_bounds__(256) void k_expand_rb(
     S *__restrict__ phi, RBGeom g, const SC *__restrict__ pc, RBGeom gc, ExpandAxis ax,
-    ExpandAxis ay, ExpandAxis az, int jlo, int nj, int bpc, const int *__restrict__ skip) {
+    ExpandAxis ay, ExpandAxis az, int jlo, int nj, int bpc, const uint32_t *__restrict__ imask,
+    const int *__restrict__ skip) {
   typedef typename St<S>::T T;
   WM_SKIP();
   const int nyr = ay.hi - ay.lo + 1;
@@ -1017,8 +1028,12 @@ __global__ __launch_bounds__(256) void k_expand_rb(
                  w * (one - wy) * wz * b03 +
                  (one - w) * wy * wz * b12 +
                  w * wy * wz * b13;
-    if (okA) phi[eA] = St<S>::st(vA);
-    if (okB) phi[eB] = St<S>::st(vB);
+    // points set in imask take the value the pin without voltages
+    // (cond_potmg on an error level) gives them: zero
+    const bool pA = imask && ((imask[eA >> 5] >> (eA & 31)) & 1u);
+    const bool pB = imask && ((imask[eB >> 5] >> (eB & 31)) & 1u);
+    if (okA) phi[eA] = St<S>::st(pA ? (T)0. : vA);
+    if (okB) phi[eB] = St<S>::st(pB ? (T)0. : vB);
     cA ^= 1;
   }
 #undef LDC
