@@ -304,7 +304,11 @@ static RBGeom make_geom(int nx, int ny, int nz) {
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
+  // elements per half row: (nx+4)/2 are used; rows of 64 elements or more
+  // are padded to a multiple of 16 (128 bytes in FP64) so that, with the
+  // arrays' 15-element lead, element 1 of every row starts a cache line
   g.W = (nx + 4) / 2;
+  if (g.W >= 64) g.W = (g.W + 15) / 16 * 16;
   g.PY = ny + 3;
   g.PZ = nz + 3;
   g.cs = (long long)g.W * g.PY * g.PZ;
@@ -605,9 +609,9 @@ static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
 
 template <typename T>
 static void ensure_buffers(Level &L, bool with_res) {
-  phi_of<T>(L).alloc(2 * L.g.cs);
-  rho_of<T>(L).alloc(2 * L.g.cs);
-  if (with_res) res_of<T>(L).alloc(std::max<size_t>(2 * L.g.cs, (size_t)(L.nx + 3) * (L.ny + 3) * (L.nz + 3)));
+  phi_of<T>(L).alloc_rb(2 * L.g.cs);
+  rho_of<T>(L).alloc_rb(2 * L.g.cs);
+  if (with_res) res_of<T>(L).alloc_rb(std::max<size_t>(2 * L.g.cs, (size_t)(L.nx + 3) * (L.ny + 3) * (L.nz + 3)));
 }
 
 template <typename T, typename Idx>
@@ -1106,7 +1110,7 @@ static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long lo
   WM_LAUNCHED();
   vcycle<S>(h, 0, Pinit, skip, s, true);
   if (fused) {
-    WM_KW_DISPATCH(L0.g.W, hipLaunchKernelGGL((k_dc_update<S, KW>), dim3(nconv_blocks(L0.g)),
+    WM_KW_DISPATCH((L0.nx + 4) / 2, hipLaunchKernelGGL((k_dc_update<S, KW>), dim3(nconv_blocks(L0.g)),
                                               dim3(256), 0, s, L0.phi64.p,
                                               (const S *)phi_of<S>(L0).p, L0.g, h->state.p,
                                               h->partials.p));
@@ -1173,7 +1177,7 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
     launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, skip, s);
     if (fused) return;
   }
-  WM_KW_DISPATCH(L0.g.W, hipLaunchKernelGGL((k_converge<double, KW>), dim3(nconv_blocks(L0.g)),
+  WM_KW_DISPATCH((L0.nx + 4) / 2, hipLaunchKernelGGL((k_converge<double, KW>), dim3(nconv_blocks(L0.g)),
                                             dim3(256), 0, s, L0.phi64.p, h->save64.p, L0.g,
                                             h->state.p, h->partials.p, 1));
   WM_LAUNCHED();
@@ -1201,7 +1205,8 @@ int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz, double dx,
     WM_REQUIRE(out != nullptr, "wm_mg3d_create: out is NULL");
     WM_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "wm_mg3d_create: need nx, ny, nz >= 2");
     WM_REQUIRE(dx > 0 && dy > 0 && dz > 0, "wm_mg3d_create: cell sizes must be positive");
-    WM_REQUIRE((nx + 4) * (ny + 3) * (nz + 3) < (1ll << 31),
+    WM_REQUIRE(nx < (1 << 30) && ny < (1 << 30) && nz < (1 << 30) &&
+                   2 * make_geom((int)nx, (int)ny, (int)nz).cs + 64 < (1ll << 31),
                "wm_mg3d_create: grid too large (the red-black arrays are indexed with 32 bits)");
     for (int i = 0; i < 6; i++)
       WM_REQUIRE(bounds[i] >= 0 && bounds[i] <= 2,
@@ -1403,18 +1408,18 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     if (prec != WM_PREC_FP64)
       prepare_levels<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd});
     if (prec == WM_PREC_BF16_DC) {
-      L0.phi16.alloc(2 * L0.g.cs);
-      L0.rho16.alloc(2 * L0.g.cs);
+      L0.phi16.alloc_rb(2 * L0.g.cs);
+      L0.rho16.alloc_rb(2 * L0.g.cs);
     }
     if (dc) {
-      L0.res64.alloc((size_t)(L0.nx + 3) * (L0.ny + 3) * (L0.nz + 3));
-      h->rhosave64.alloc(2 * L0.g.cs);
+      L0.res64.alloc_rb(std::max<size_t>(2 * L0.g.cs, (size_t)(L0.nx + 3) * (L0.ny + 3) * (L0.nz + 3)));
+      h->rhosave64.alloc_rb(2 * L0.g.cs);
       Params P2 = P;
       P2.mgform = 2;
       if (prec == WM_PREC_FP64) prepare_levels<double>(h, P2);
       ensure_records<double>(h, L0, 0, P2, true, 1);
     }
-    h->save64.alloc(2 * L0.g.cs);
+    h->save64.alloc_rb(2 * L0.g.cs);
     // Inputs.
     WM_HIP(hipEventRecord(h->ev_in, us));
     WM_HIP(hipStreamWaitEvent(s, h->ev_in, 0));
@@ -1618,7 +1623,7 @@ int wm_mg3d_residual(wm_mg3d *h, int64_t level, const double *phi, const double 
     hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
     hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
     WM_LAUNCHED();
-    L.res64.alloc(std::max<size_t>(2 * L.g.cs, (size_t)nn));
+    L.res64.alloc_rb(std::max<size_t>(2 * L.g.cs, (size_t)nn));
     residual_rb<double, double>(h, L, (int)level, P, L.phi64.p, L.rho64.p, L.res64.p,
                                 mgform2init ? 1 : 0, nullptr, s);
     hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L.res64.p, res,

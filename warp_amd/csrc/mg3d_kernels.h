@@ -1201,13 +1201,14 @@ __global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
   const int lane = threadIdx.x & 63;
   const int nrows = 2 * g.PY * g.PZ;
   const int stride = gridDim.x * 4;
+  const int Wu = (g.nx + 4) >> 1;  // used elements per row (rows may be padded)
   double m = 0.;
   for (int row = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6)); row < nrows;
        row += 2 * stride) {
     RowInfo ri[2];
     bool live[2];
     T cur[2][KW], sv[2][KW];
-    for (int j0 = 0; j0 < g.W; j0 += 64 * KW) {
+    for (int j0 = 0; j0 < Wu; j0 += 64 * KW) {
 #pragma unroll
       for (int r = 0; r < 2; r++) {
         live[r] = row + r * stride < nrows;
@@ -1215,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
 #pragma unroll
         for (int k = 0; k < KW; k++) {
           int j = j0 + lane + 64 * k;
-          int e = ri[r].base + (j < g.W ? j : g.W - 1);
+          int e = ri[r].base + (j < Wu ? j : Wu - 1);
           cur[r][k] = phi[e];
           sv[r][k] = save[e];
         }
@@ -1230,7 +1231,7 @@ __global__ __launch_bounds__(256) void k_converge(const T *__restrict__ phi,
             double d = fabs((double)sv[r][k] - (double)cur[r][k]);
             m = d > m ? d : m;
           }
-          if (docopy && live[r] && j < g.W) save[ri[r].base + j] = cur[r][k];
+          if (docopy && live[r] && j < Wu) save[ri[r].base + j] = cur[r][k];
         }
       }
     }
@@ -1252,13 +1253,14 @@ __global__ __launch_bounds__(256) void k_dc_update(double *__restrict__ phi,
   const int lane = threadIdx.x & 63;
   const int nrows = 2 * g.PY * g.PZ;
   const int stride = gridDim.x * 4;
+  const int Wu = (g.nx + 4) >> 1;  // used elements per row (rows may be padded)
   double m = 0.;
   for (int row = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6)); row < nrows;
        row += 2 * stride) {
     RowInfo ri[2];
     bool live[2];
     double old[2][KW], nw[2][KW];
-    for (int j0 = 0; j0 < g.W; j0 += 64 * KW) {
+    for (int j0 = 0; j0 < Wu; j0 += 64 * KW) {
 #pragma unroll
       for (int r = 0; r < 2; r++) {
         live[r] = row + r * stride < nrows;
@@ -1266,7 +1268,7 @@ __global__ __launch_bounds__(256) void k_dc_update(double *__restrict__ phi,
 #pragma unroll
         for (int k = 0; k < KW; k++) {
           int j = j0 + lane + 64 * k;
-          int q = ri[r].base + (j < g.W ? j : g.W - 1);
+          int q = ri[r].base + (j < Wu ? j : Wu - 1);
           old[r][k] = phi[q];
           nw[r][k] = (double)St<TL>::ld(e[q]);
         }
@@ -1282,7 +1284,7 @@ __global__ __launch_bounds__(256) void k_dc_update(double *__restrict__ phi,
             double d = fabs(old[r][k] - v);
             m = d > m ? d : m;
           }
-          if (live[r] && j < g.W) phi[ri[r].base + j] = v;
+          if (live[r] && j < Wu) phi[ri[r].base + j] = v;
         }
       }
     }

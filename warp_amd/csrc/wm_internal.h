@@ -43,28 +43,37 @@ static inline hipStream_t wm_stream(void *s) { return (hipStream_t)s; }
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Owning device array (hipMalloc'ed, zero-filled on allocation).
+// Owning device array (hipMalloc'ed, zero-filled on allocation).  With a
+// lead of k elements, p points k elements past the start of the allocation
+// (the red-black arrays use it to put element 1 of every row on a 128-byte
+// boundary).
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
-  size_t n = 0;
+  T *raw = nullptr;
+  size_t n = 0, lead = 0;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
+    if (raw) (void)hipFree(raw);
+    p = raw = nullptr;
+    n = lead = 0;
   }
-  void alloc(size_t count) {
-    if (count == n && p) return;
+  void alloc(size_t count, size_t lead_ = 0) {
+    if (count == n && p && lead_ == lead) return;
     release();
     if (count == 0) return;
-    WM_HIP(hipMalloc((void **)&p, sizeof(T) * count));
-    WM_HIP(hipMemset(p, 0, sizeof(T) * count));
+    WM_HIP(hipMalloc((void **)&raw, sizeof(T) * (count + lead_)));
+    WM_HIP(hipMemset(raw, 0, sizeof(T) * (count + lead_)));
+    p = raw + lead_;
     n = count;
+    lead = lead_;
   }
+  // a red-black array: element 1 of each row 128-byte aligned when the row
+  // length is a multiple of 128 bytes
+  void alloc_rb(size_t count) { alloc(count, 128 / sizeof(T) - 1); }
   void upload(const std::vector<T> &v) {
     alloc(v.size());
     if (!v.empty()) WM_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
