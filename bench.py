@@ -9,7 +9,7 @@ HBM before the timed region.
 
 Prints one JSON line (steps/s, V-cycles and ms per solve, the roofline of
 the dominant kernel, and the FP64 oracle timed on the host as the CPU
-baseline).  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+baseline).  ``--gpus N`` with N > 1 exits: the path runs on one GPU.
 """
 import argparse
 import ctypes as C
@@ -49,7 +49,7 @@ def log(*a):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+        print("bench.py runs on one GPU only (the solver has no multi-GPU path); "
               "got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np
