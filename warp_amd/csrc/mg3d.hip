@@ -231,6 +231,7 @@ struct wm_mg3d {
   std::vector<std::unique_ptr<Level>> lv;
   // level-0 extras
   DevBuf<double> save64, rhosave64;
+  DevBuf<double> convparts;  // per-wavefront maxima of the fused convergence test
   DevBuf<SolveState> state;
   DevBuf<double> partials;  // per-block maxima of the convergence test
   SolveState *state_h = nullptr;  // pinned
@@ -654,7 +655,9 @@ static void pin_and_mirror(wm_mg3d *h, Level &L, int level, const Params &P, T *
 // sorhalfpass3d (f3d_mgrid.F:2095-2291) on level data in red-black layout.
 template <typename S>
 static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
-                     const int *skip, hipStream_t s, bool inpass = true) {
+                     const int *skip, hipStream_t s, bool inpass = true,
+                     double *cpart = nullptr, double *cpart_end = nullptr,
+                     int *nslots = nullptr) {
   typedef typename St<S>::T T;
   S *phi = phi_of<S>(L).p;
   const S *rho = rho_of<S>(L).p;
@@ -685,7 +688,9 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
                       : (int)std::min<long long>(64, std::max<long long>(2, cdiv((long long)nzr * cols, 2048)));
   int nbz = (int)cdiv(nzr, zc);
   int nbdense = tiled ? nbx * nby * nzr : nbx * nby * nbz;
-  bool timed = h->prof.on && level == 0;
+  // the event timing covers the plain finest-level sweeps (the sweeps that
+  // carry the convergence test move more bytes per point)
+  bool timed = h->prof.on && level == 0 && cpart == nullptr;
   if (timed) {
     while ((int)h->prof.ev.size() < 2 * (h->prof.used + 1)) {
       hipEvent_t e;
@@ -702,11 +707,23 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
                          dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, nbx, nby,
                          nbdense, skip);
   }
-  if (!tiled)
+  if (cpart) {
+    // the sweep with the convergence test folded in (FP64 only)
+    long long ns = (long long)(nbdense + nbsub) * (nthr / 64);
+    const bool fits = !tiled && std::is_same<S, double>::value && ns <= cpart_end - cpart;
+    WM_REQUIRE(fits, "sorhalfpass3d: the fused convergence test does not fit this sweep");
+    if constexpr (std::is_same<S, double>::value)
+      hipLaunchKernelGGL((k_sor_zmarch<S, T, true>), dim3(nbdense + nbsub), dim3(nthr), 0, s,
+                         phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
+                         dxsqic, dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0,
+                         nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip);
+    *nslots = (int)ns;
+  } else if (!tiled) {
     hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbdense + nbsub), dim3(nthr), 0, s, phi, rho,
                        L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
                        dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0, nbx, nby,
-                       zc, nbdense, nbsub, skip);
+                       zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip);
+  }
   WM_LAUNCHED();
   if (timed) {
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
@@ -727,13 +744,19 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
 
 template <typename T>
 static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
-                    hipStream_t s, bool pin = true) {
+                    hipStream_t s, bool pin = true, double *cpart = nullptr,
+                    double *cpart_end = nullptr, int *nslots = nullptr) {
   int usevolt = (level == 0 && P.mgform == 1);
   if (pin) pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
+  int used = 0;
   for (int c = 0; c <= 1; c++) {
-    halfpass<T>(h, L, level, P, c, skip, s);
+    int ns = 0;
+    halfpass<T>(h, L, level, P, c, skip, s, true, cpart ? cpart + used : nullptr, cpart_end,
+                &ns);
+    used += ns;
     launch_bc<T>(phi_of<T>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
   }
+  if (nslots) *nslots = used;
 }
 
 // residual3d (f3d_mgrid.F:3367-3598) from red-black phi/rho into the
@@ -1039,7 +1062,8 @@ static void zero_interior_res(wm_mg3d *h, const Params &P, hipStream_t s) {
 // zero-filled) and the Neumann mirrors copy zeros, so it is left out.
 template <typename S>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
-                   bool zeroed = false, bool repin = true) {
+                   bool zeroed = false, bool repin = true, double *cpart = nullptr,
+                   double *cpart_end = nullptr, int *nslots = nullptr) {
   typedef typename St<S>::T T;
   typedef typename Coarse<S>::type SC;
   Level &L = *h->lv[level];
@@ -1075,8 +1099,11 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
     expand_rb<S>(h, L, Lc, level, P, skip, s, fusepin);
     launch_bc<S>(phi_of<S>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
   }
-  // the first up pass's pin already ran inside the expansion when fusepin
-  for (int i = 0; i < P.up; i++) sorpass<S>(h, L, level, P, skip, s, !(fusepin && i == 0));
+  // the first up pass's pin already ran inside the expansion when fusepin;
+  // the last up pass carries the convergence test when cpart is given
+  for (int i = 0; i < P.up; i++)
+    sorpass<S>(h, L, level, P, skip, s, !(fusepin && i == 0), i == P.up - 1 ? cpart : nullptr,
+               cpart_end, nslots);
 }
 
 template <typename T>
@@ -1115,7 +1142,7 @@ static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long lo
                                               (const S *)phi_of<S>(L0).p, L0.g, h->state.p,
                                               h->partials.p));
     WM_LAUNCHED();
-    hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(256), 0, s, h->partials.p,
+    hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(1024), 0, s, h->partials.p,
                        nconv_blocks(L0.g), h->state.p, mgtol, mgmaxiters);
   } else {
     hipLaunchKernelGGL((k_dc_add<S>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
@@ -1145,7 +1172,22 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
   Level &L0 = *h->lv[0];
   const int *skip = &h->state.p->done;
   if (prec == WM_PREC_FP64 && P.mgform == 1) {
-    vcycle<double>(h, 0, P, skip, s, false, repin);
+    // Where the leading pin is left out (repin false) nothing but the last
+    // up pass's sweeps changes a point between its save at the end of the
+    // previous V-cycle and the test: those sweeps compare every point they
+    // write with save and copy it there, and k_converge is not needed.  The
+    // first V-cycle of a solve (pin, full copy) keeps k_converge.
+    static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
+    const bool fuseconv = !repin && P.up >= 1 && h->convparts.p && !tiled_env;
+    int nslots = 0;
+    vcycle<double>(h, 0, P, skip, s, false, repin, fuseconv ? h->convparts.p : nullptr,
+                   h->convparts.p + h->convparts.n, &nslots);
+    if (fuseconv) {
+      hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(1024), 0, s, h->convparts.p, nslots,
+                         h->state.p, mgtol, mgmaxiters);
+      WM_LAUNCHED();
+      return;
+    }
   } else {
     // Residual-correction form: the error equation on the residual of the
     // saved phi (mgform == 2), with the V-cycle in double or float.
@@ -1181,7 +1223,7 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
                                             dim3(256), 0, s, L0.phi64.p, h->save64.p, L0.g,
                                             h->state.p, h->partials.p, 1));
   WM_LAUNCHED();
-  hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(256), 0, s, h->partials.p,
+  hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(1024), 0, s, h->partials.p,
                      nconv_blocks(L0.g), h->state.p, mgtol, mgmaxiters);
   WM_LAUNCHED();
 }
@@ -1442,6 +1484,19 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       zero_interior_res<double>(h, dc ? P2of(P) : P, s);
     else
       zero_interior_res<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd}, s);
+    if (prec == WM_PREC_FP64 && P.mgform == 1 && P.up >= 1 && repin_skippable(h, P)) {
+      // slots of the two fused half sweeps: at most one per wavefront of
+      // rows x z chunks of the dense part plus one per 64 records, with room
+      const RBGeom &g0 = L0.g;
+      long long nrec = 0;
+      if (P.lcnd) nrec = cond_of<double>(L0).sor[0].n + cond_of<double>(L0).sor[1].n;
+      long long cap = 2 * ((long long)(g0.W / 64 + 1) * (g0.PY + 16) * g0.PZ) + nrec / 64 + 4096;
+      if ((long long)h->convparts.n < cap) {
+        h->drop_graphs();  // captured graphs hold the old buffer's address
+        h->gkey = wm_mg3d::GKey();
+        h->convparts.alloc((size_t)cap);
+      }
+    }
     int iters = 0;
     double err = 2. * mgtol + 1.;
     static const bool eager_env = getenv("WARPMI_EAGER") != nullptr;

@@ -116,6 +116,16 @@ struct SubList {
   const uint32_t *zero;  // (n) 6-bit mask of neighbours whose residual is zeroed
 };
 
+// Largest of the wavefront's values (lane 0 holds it); NaN values are
+// passed over, as by the convergence test's comparisons.
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 // XCD-aware block order: hardware hands consecutive workgroups to the 8
 // XCDs round robin, so block b runs on XCD b%8.  Remap so that each XCD
 // walks a contiguous range of tiles and its L2 holds the neighbouring
@@ -201,13 +211,48 @@ __global__ __launch_bounds__(256) void k_sor_halfpass(
 // it, which the neighbouring waves of the block load in the same step.
 // Blocks: (row segment jseg, blockDim/64 rows, z chunk), remapped per XCD.
 // ---------------------------------------------------------------------------
-template <typename S, typename T = typename St<S>::T>
+// One subgrid record of the sweep (condbndymg's update, as the record part
+// of k_sor_zmarch below) that also compares the new value with save, copies
+// it there and returns |save - new|.
+template <typename S, typename T>
+__device__ inline double sor_record_conv(S *__restrict__ phi, const S *__restrict__ rho,
+                                         const RBGeom &g, const SubList<T> &sub, int i,
+                                         T dxsqic, T dysqic, T dzsqic, T spm1,
+                                         S *__restrict__ save) {
+  int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
+  uint32_t m = sub.modes[i];
+  const T *k = sub.k + 6 * i;
+  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(phi[rb_index(g, ix - 1, iy, iz)]));
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(phi[rb_index(g, ix + 1, iy, iz)]));
+  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(phi[rb_index(g, ix, iy - 1, iz)]));
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(phi[rb_index(g, ix, iy + 1, iz)]));
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(phi[rb_index(g, ix, iy, iz - 1)]));
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(phi[rb_index(g, ix, iy, iz + 1)]));
+  long long ce = rb_index(g, ix, iy, iz);
+  T prev = St<S>::ld(phi[ce]);
+  const S nv = St<S>::st((St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
+                          (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev);
+  phi[ce] = nv;
+  const double d = fabs((double)St<S>::ld(save[ce]) - (double)St<S>::ld(nv));
+  save[ce] = nv;
+  return d > 0. ? d : 0.;
+}
+
+//
+// CONV (the last half sweeps of a V-cycle, FP64): each point written is also
+// compared with save, its value before the V-cycle, and copied there; every
+// wavefront writes the largest |save - new| of its points to cpart[slot],
+// slot = block * (wavefronts per block) + wavefront, 0 when it has none.
+template <typename S, typename T = typename St<S>::T, bool CONV = false>
 __global__ __launch_bounds__(1024) void k_sor_zmarch(
     S *__restrict__ phi, const S *__restrict__ rho, RBGeom g, int c, int ixmin,
     int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
     T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
-    int j0, int nbx, int nby, int zc, int nbdense, int nbsub, const int *__restrict__ skip) {
+    int j0, int nbx, int nby, int zc, int nbdense, int nbsub, S *__restrict__ save,
+    double *__restrict__ cpart, const int *__restrict__ skip) {
   WM_SKIP();
+  const int slot = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+  double cmax = 0.;
   // blocks [0, nbsub): the subgrid records, scheduled first so that they
   // overlap the dense sweep; nbsub is a multiple of 8, so the dense block
   // b runs on XCD b % 8 as xcd_remap assumes
@@ -221,7 +266,10 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     int lane = threadIdx.x & 63;
     int j = j0 + bx * 64 + lane;
     int iy = iymin + by * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (iy > iymax) return;  // whole wave: no shuffles are skipped mid-wave
+    if (iy > iymax) {  // whole wave: no shuffles are skipped mid-wave
+      if (CONV && lane == 0) cpart[slot] = 0.;
+      return;
+    }
     int za = izmin + bz * zc;
     int zb = min(za + zc - 1, izmax);
     int py = iy + 1;
@@ -235,8 +283,10 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     long long rowc = (long long)g.W * py + jc;
     long long meoff = (long long)c * g.cs + rowc;  // mask index within a plane
     // Loads of one plane (padded index pz), issued a plane ahead of use.
-    auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw) {
+    auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw,
+                    T &sv) {
       long long base = pl * pz;
+      if (CONV) sv = St<S>::ld(save[c * g.cs + rowc + base]);
       ozp = St<S>::ld(o[rowc + base + pl]);
       oym = St<S>::ld(o[rowc + base - g.W]);
       oyp = St<S>::ld(o[rowc + base + g.W]);
@@ -250,14 +300,14 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     };
     T oz_m = St<S>::ld(o[rowc + pl * za]);        // padded plane za: iz = za-1
     T oz_0 = St<S>::ld(o[rowc + pl * (za + 1)]);  // iz = za
-    T ozp, oym, oyp, own, r, eg;
+    T ozp, oym, oyp, own, r, eg, sv = 0;
     uint32_t mw;
-    load(za + 1, ozp, oym, oyp, own, r, eg, mw);
+    load(za + 1, ozp, oym, oyp, own, r, eg, mw, sv);
     for (int iz = za; iz <= zb; iz++) {
       int pz = iz + 1;
-      T nozp = 0, noym = 0, noyp = 0, nown = 0, nr = 0, neg = 0;
+      T nozp = 0, noym = 0, noyp = 0, nown = 0, nr = 0, neg = 0, nsv = 0;
       uint32_t nmw = 0;
-      if (iz < zb) load(pz + 1, nozp, noym, noyp, nown, nr, neg, nmw);
+      if (iz < zb) load(pz + 1, nozp, noym, noyp, nown, nr, neg, nmw, nsv);
       int p0 = (c + py + pz + 1) & 1;
       // other-colour x-neighbours: (j-1, j) when p0 == 0, (j, j+1) when 1
       T sh = p0 ? __shfl_down(oz_0, 1, 64) : __shfl_up(oz_0, 1, 64);
@@ -268,16 +318,34 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
       if (active) {
         T pxm = p0 ? oz_0 : sh;
         T pxp = p0 ? sh : oz_0;
-        me[rowc + pl * pz] = St<S>::st(r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic +
-                                       (oz_m + ozp) * dzsqic + spm1 * own);
+        const S nv = St<S>::st(r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic +
+                               (oz_m + ozp) * dzsqic + spm1 * own);
+        me[rowc + pl * pz] = nv;
+        if (CONV) {
+          const double d = fabs((double)sv - (double)St<S>::ld(nv));
+          cmax = d > cmax ? d : cmax;
+          save[c * g.cs + rowc + pl * pz] = nv;
+        }
       }
       oz_m = oz_0;
       oz_0 = ozp;
-      ozp = nozp; oym = noym; oyp = noyp; own = nown; r = nr; eg = neg; mw = nmw;
+      ozp = nozp; oym = noym; oyp = noyp; own = nown; r = nr; eg = neg; mw = nmw; sv = nsv;
+    }
+    if (CONV) {
+      cmax = wave_max(cmax);
+      if (lane == 0) cpart[slot] = cmax;
     }
     return;
   }
   int i = (int)blockIdx.x * (int)blockDim.x + threadIdx.x;
+  if (CONV) {
+    // all lanes reach the reduction; lanes past the list contribute 0
+    if (i < sub.n) cmax = sor_record_conv<S, T>(phi, rho, g, sub, i, dxsqic, dysqic, dzsqic,
+                                                spm1, save);
+    cmax = wave_max(cmax);
+    if ((threadIdx.x & 63) == 0) cpart[slot] = cmax;
+    return;
+  }
   if (i >= sub.n) return;
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
@@ -1124,14 +1192,6 @@ struct SolveState {
 // Most blocks of a convergence-test launch (size of the partials buffer).
 #define WM_CONV_BLOCKS 2048
 
-__device__ inline double wave_max(double v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_down(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-
 // The convergence test of multigrid3dsolve (f3d_mgrid.F:383-392, 313):
 // mgerror = max over nodes 0..n of |phisave-phi|.  Two stages: each block
 // of the test writes its max to partials[blockIdx.x] (plain stores, no
@@ -1150,19 +1210,29 @@ __device__ inline void block_max_to(double m, double *part, double *partials) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_converge_close(const double *__restrict__ partials,
-                                                        int n, SolveState *st, double mgtol,
-                                                        int mgmaxiters) {
+// One block of 1024 threads; each thread keeps four loads in flight.
+__global__ __launch_bounds__(1024) void k_converge_close(const double *__restrict__ partials,
+                                                         int n, SolveState *st, double mgtol,
+                                                         int mgmaxiters) {
   if (st->done) return;
-  __shared__ double part[4];
+  __shared__ double part[16];
   double m = 0.;
-  for (int i = threadIdx.x; i < n; i += 256) m = partials[i] > m ? partials[i] : m;
+  int i = threadIdx.x;
+  for (; i + 3 * 1024 < n; i += 4 * 1024) {
+    const double a = partials[i], b = partials[i + 1024], c = partials[i + 2048],
+                 d = partials[i + 3072];
+    m = a > m ? a : m;
+    m = b > m ? b : m;
+    m = c > m ? c : m;
+    m = d > m ? d : m;
+  }
+  for (; i < n; i += 1024) m = partials[i] > m ? partials[i] : m;
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double err = part[0];
-    for (int k = 1; k < 4; k++) err = part[k] > err ? part[k] : err;
+    for (int k = 1; k < 16; k++) err = part[k] > err ? part[k] : err;
     st->err = err;
     st->iters = st->iters + 1;
     if (!(err > mgtol && st->iters < mgmaxiters)) st->done = 1;
