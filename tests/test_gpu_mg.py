@@ -170,6 +170,31 @@ def test_conductor_solve_fp64_bitwise(device):
     np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
 
 
+@pytest.mark.parametrize("down,up", [(2, 1), (1, 2), (2, 2), (0, 1), (1, 0)])
+def test_conductor_solve_pass_counts_bitwise(device, down, up):
+    """downpasses/uppasses other than 1 with the pipe and annulus, cold and
+    warm solves: same phi, mgiters and mgerror bit for bit (the convergence
+    test rides on the last up pass's sweeps after the first V-cycle, and is
+    a separate launch when there is no up pass)."""
+    nx, ny, nz = 32, 32, 64
+    s, rho = _c1_like(device, nx, ny, nz)
+    s.downpasses, s.uppasses = down, up
+    s.rho = torch.from_numpy(rho).to(device)
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    ro = rho.copy()
+    for k in range(2):
+        if k == 1:
+            # a changed source for the warm solve
+            ro *= 1.25
+            s.rho = torch.from_numpy(ro).to(device)
+        s.solve()
+        it, err = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co,
+                                     downpasses=down, uppasses=up)
+        assert s.mgiters == it and s.mgerror == err, (k, s.mgiters, it, s.mgerror, err)
+        np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+
+
 def test_conductor_solve_mgform2_bitwise(device):
     nx, ny, nz = 32, 32, 64
     s, rho = _c1_like(device, nx, ny, nz, mgform=2)

@@ -1154,17 +1154,18 @@ static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long lo
 // One iteration of the multigrid3dsolve loop (f3d_mgrid.F:313-434).
 // Whether a V-cycle of the plain FP64 solve may skip the pin that starts
 // its first pass on the finest level: the previous V-cycle ended with
-// passes that began with a pin (after the expansion), and under these
-// conditions nothing since wrote a pinned point -- the dense sweeps skip
-// interior points, no subgrid record writes one, no periodic wrap copies
-// over one, no Neumann ghost needs mirroring (the same conditions under
-// which halfpass leaves out the pin after a sweep); the convergence test
-// only reads phi.
+// passes that began with a pin (after the expansion; so there must be an
+// up pass -- without one the expansion's changes to pinned points stand
+// until the next V-cycle's leading pin), and under these conditions nothing
+// since wrote a pinned point -- the dense sweeps skip interior points, no
+// subgrid record writes one, no periodic wrap copies over one, no Neumann
+// ghost needs mirroring (the same conditions under which halfpass leaves
+// out the pin after a sweep); the convergence test only reads phi.
 static bool repin_skippable(const wm_mg3d *h, const Params &P) {
   const Level &L0 = *h->lv[0];
   const int *b = h->bounds;
   bool periodic = b[0] == 2 || b[2] == 2 || b[4] == 2;
-  return !(L0.sub_on_interior || periodic || (P.icnd == 2 && L0.neumann));
+  return P.up >= 1 && !(L0.sub_on_interior || periodic || (P.icnd == 2 && L0.neumann));
 }
 
 static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int mgmaxiters,
