@@ -1474,13 +1474,19 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     double *rho0 = dc ? h->rhosave64.p : L0.rho64.p;
     hipLaunchKernelGGL(k_rho_in, dim3(blocks(nr)), dim3(256), 0, s, rho, rho0, L0.g, reps0c, rinv);
     WM_LAUNCHED();
+    // the saved phi: written by the conversion itself when the boundary
+    // conditions change nothing (six Dirichlet bounds), else copied after them
+    bool alldir = true;
+    for (int i = 0; i < 6; i++) alldir = alldir && h->bounds[i] == 0;
     hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L0.phi64.p,
-                       L0.g);
+                       L0.g, alldir ? h->save64.p : nullptr);
     WM_LAUNCHED();
     launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr, s);
-    hipLaunchKernelGGL((k_copy<double, double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
-                       h->save64.p, np, nullptr);
-    WM_LAUNCHED();
+    if (!alldir) {
+      hipLaunchKernelGGL((k_copy<double, double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
+                         h->save64.p, np, nullptr);
+      WM_LAUNCHED();
+    }
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, s, h->state.p);
     WM_LAUNCHED();
     if (prec == WM_PREC_FP64)

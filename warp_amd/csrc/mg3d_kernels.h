@@ -1118,10 +1118,12 @@ __global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, long long n, T 
   if (t < n) a[t] = v;
 }
 
-// natural -> red-black (all elements, guards included).
+// natural -> red-black (all elements, guards included); dst2, when given,
+// receives the same values (the solve's saved phi).
 template <typename T>
 __global__ __launch_bounds__(256) void k_nat_to_rb(const double *__restrict__ src,
-                                                   T *__restrict__ dst, RBGeom g) {
+                                                   T *__restrict__ dst, RBGeom g,
+                                                   T *__restrict__ dst2 = nullptr) {
   long long sx = g.nx + 3, sy = g.ny + 3;
   long long n = sx * sy * (g.nz + 3);
   long long t = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -1130,7 +1132,10 @@ __global__ __launch_bounds__(256) void k_nat_to_rb(const double *__restrict__ sr
   long long u = t / sx;
   int iy = (int)(u % sy) - 1;
   int iz = (int)(u / sy) - 1;
-  dst[rb_index(g, ix, iy, iz)] = (T)src[t];
+  const long long e = rb_index(g, ix, iy, iz);
+  const T v = (T)src[t];
+  dst[e] = v;
+  if (dst2) dst2[e] = v;
 }
 
 template <typename T>
