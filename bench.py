@@ -121,7 +121,8 @@ def main():
     log("SOR half sweep: %d launches, %.4f ms avg, %.0f GB/s" % (nl.value, avg_ms, achieved))
     traffic = None
     tf = os.path.join(ROOT, "profiles", "sor_zmarch_256x256x512_traffic.json")
-    if os.path.exists(tf):
+    # the counters were read for the FP64 sweep of this configuration only
+    if args.precision == "fp64" and args.config == "c2" and os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
