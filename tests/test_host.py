@@ -120,6 +120,47 @@ def test_registry():
     assert W.top.fstype == -1
 
 
+def test_diagnostic_hooks_and_step_order():
+    """The diagnostics step3d calls through the registered solvers
+    (fieldsolver.py:234-257): rhodia on every step, gtlchg, srhoax, getese,
+    sphiax, sezax in that order on special steps (w3d.F:987-1002); a solver
+    that defines none of them inherits the reference's no-ops
+    (fieldsolver.py:1053-1064)."""
+    import warp_amd as W
+    from warp_amd import fieldsolver as FS
+    from warp_amd import stepper
+    calls = []
+
+    class Diag(W.FieldSolver):
+        def __init__(self, **kw):
+            W.FieldSolver.__init__(self, kwdict=kw)
+
+        def gtlchg(self): calls.append("gtlchg")
+        def getese(self): calls.append("getese")
+        def rhodia(self): calls.append("rhodia")
+        def sezax(self): calls.append("sezax")
+
+    class Plain(W.FieldSolver):
+        def __init__(self, **kw):
+            W.FieldSolver.__init__(self, kwdict=kw)
+
+    a, b = Diag(nx=8, ny=8, nz=8), Plain(nx=8, ny=8, nz=8)
+    for name in ("rhodia", "gtlchg", "srhoax", "getese", "sphiax", "sezax"):
+        assert getattr(b, name)() is None
+        assert callable(getattr(FS, name + "registered"))
+    W.registersolver(a)
+    W.registersolver(b)
+    try:
+        stepper.field_diagnostics(special=False)
+        assert calls == ["rhodia"]
+        del calls[:]
+        stepper.field_diagnostics(special=True)
+        assert calls == ["rhodia", "gtlchg", "getese", "sezax"]
+    finally:
+        W.unregistersolver(a)
+        W.unregistersolver(b)
+
+
 def test_bad_keyword_rejected():
     import warp_amd as W
     with pytest.raises(AssertionError):
