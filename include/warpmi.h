@@ -96,6 +96,18 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam,
  * (fieldsolver.py:1653-1658); the solve still stops on mgtol. */
 int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream);
 
+/* No counterpart in the reference (a hint for callers that keep phi
+ * between solves, such as a time-step loop): declares that the phi the
+ * next wm_mg3d_solve receives is the same array, unchanged, that the last
+ * wm_mg3d_solve on h returned.  The next solve then starts from its own
+ * red-black copy of that phi instead of converting it again; it does so
+ * only for the FP64 reference form (mgform 1) with six Dirichlet bounds, no
+ * conductor record on an interior point, an up pass, and after a solve of
+ * the same kind that ran at least one V-cycle -- otherwise the hint is
+ * ignored.  The result is the same bits either way.  The hint applies to
+ * the next solve only. */
+int wm_mg3d_reuse_phi(wm_mg3d *h);
+
 /* Event timing of the finest-level FP64 SOR half sweep (the solver's
  * dominant kernel): while on, wm_mg3d_solve launches its kernels one
  * V-cycle at a time instead of replaying its graph, records HIP events

@@ -195,6 +195,38 @@ def test_conductor_solve_pass_counts_bitwise(device, down, up):
         np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
 
 
+def test_solve_starts_from_a_changed_phi(device):
+    """Consecutive solves with the pipe and annulus: after an untouched phi
+    the solver starts from its own copy (wm_mg3d_reuse_phi); after the
+    caller changed phi in place through a view, or through the setter, it
+    starts from the changed phi.  Every solve bitwise equal to the oracle
+    run on the same sequence."""
+    nx, ny, nz = 32, 32, 64
+    s, rho = _c1_like(device, nx, ny, nz)
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    ro = rho.copy()
+
+    def both(tag):
+        s.rho = torch.from_numpy(ro).to(device)
+        s.solve()
+        it, err = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co)
+        assert (s.mgiters, s.mgerror) == (it, err), tag
+        np.testing.assert_array_equal(s.phi.cpu().numpy(), po, err_msg=tag)
+
+    both("cold")
+    ro *= 1.1
+    both("untouched phi")
+    s.phi[10:20, 5:9, 7:30] += 3.0    # in place, through a view of the tensor
+    po[10:20, 5:9, 7:30] += 3.0
+    ro *= 0.95
+    both("changed through a view")
+    po *= 0.5
+    s.phi = torch.from_numpy(po).to(device)
+    both("changed through the setter")
+    both("untouched again")
+
+
 def test_conductor_solve_mgform2_bitwise(device):
     nx, ny, nz = 32, 32, 64
     s, rho = _c1_like(device, nx, ny, nz, mgform=2)

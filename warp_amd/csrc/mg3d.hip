@@ -232,6 +232,12 @@ struct wm_mg3d {
   // level-0 extras
   DevBuf<double> save64, rhosave64;
   DevBuf<double> convparts;  // per-wavefront maxima of the fused convergence test
+  // The last FP64 solve left its phi in phi64 with save64 equal to it and
+  // the pinned points at their values (last_ok); when the caller declares
+  // its phi unchanged since (wm_mg3d_reuse_phi: reuse_next), the next solve
+  // starts from that state.
+  bool last_ok = false, reuse_next = false;
+  const double *last_phi = nullptr;
   DevBuf<SolveState> state;
   DevBuf<double> partials;  // per-block maxima of the convergence test
   SolveState *state_h = nullptr;  // pinned
@@ -1378,6 +1384,7 @@ int wm_mg3d_set_conductors(wm_mg3d *h, const wm_conductor_lists *c) {
       L.c32.reset();
     }
     h->conductors_set = false;
+    h->last_ok = false;
     h->drop_graphs();
     h->gkey = wm_mg3d::GKey();
     if (c == nullptr) return 0;
@@ -1474,18 +1481,30 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     double *rho0 = dc ? h->rhosave64.p : L0.rho64.p;
     hipLaunchKernelGGL(k_rho_in, dim3(blocks(nr)), dim3(256), 0, s, rho, rho0, L0.g, reps0c, rinv);
     WM_LAUNCHED();
-    // the saved phi: written by the conversion itself when the boundary
-    // conditions change nothing (six Dirichlet bounds), else copied after them
+    // The caller's phi is the one the last solve returned (reuse): phi64
+    // and save64 still hold it.  With six Dirichlet bounds no kernel reads
+    // phi64's guard cells and the output's guards are recomputed, so the
+    // result is the same bits as after a fresh conversion.
     bool alldir = true;
     for (int i = 0; i < 6; i++) alldir = alldir && h->bounds[i] == 0;
-    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L0.phi64.p,
-                       L0.g, alldir ? h->save64.p : nullptr);
-    WM_LAUNCHED();
-    launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr, s);
-    if (!alldir) {
-      hipLaunchKernelGGL((k_copy<double, double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
-                         h->save64.p, np, nullptr);
+    const bool fp64plain = prec == WM_PREC_FP64 && P.mgform == 1;
+    const bool reuse = h->reuse_next && h->last_ok && phi == h->last_phi && fp64plain &&
+                       alldir && mgmaxiters > 0 && repin_skippable(h, P);
+    h->reuse_next = false;
+    h->last_ok = false;
+    if (!reuse) {
+      // the saved phi: written by the conversion itself when the boundary
+      // conditions change nothing (six Dirichlet bounds), else copied after
+      hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi,
+                         L0.phi64.p, L0.g, alldir ? h->save64.p : nullptr);
       WM_LAUNCHED();
+      launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr,
+                        s);
+      if (!alldir) {
+        hipLaunchKernelGGL((k_copy<double, double>), dim3(blocks(np)), dim3(256), 0, s,
+                           L0.phi64.p, h->save64.p, np, nullptr);
+        WM_LAUNCHED();
+      }
     }
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, s, h->state.p);
     WM_LAUNCHED();
@@ -1515,7 +1534,7 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       const bool noskip = !repin_skippable(h, P);
       for (int k = 0; k < mgmaxiters; k++) {
         h->prof.used = 0;
-        iteration(h, prec, P, mgtol, (int)mgmaxiters, s, k == 0 || noskip);
+        iteration(h, prec, P, mgtol, (int)mgmaxiters, s, (k == 0 && !reuse) || noskip);
         WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         WM_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < h->prof.used; i++) {
@@ -1572,7 +1591,8 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       int launched = 0;
       for (;;) {
         for (int b = 0; b < batch && launched < mgmaxiters; b++, launched++)
-          WM_HIP(hipGraphLaunch(launched > 0 && h->gexec_rest ? h->gexec_rest : h->gexec, s));
+          WM_HIP(hipGraphLaunch((launched > 0 || reuse) && h->gexec_rest ? h->gexec_rest : h->gexec,
+                                s));
         WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         WM_HIP(hipStreamSynchronize(s));
         if (h->state_h->done || launched >= mgmaxiters) break;
@@ -1594,6 +1614,15 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     WM_HIP(hipStreamSynchronize(s));
     *mgiters = iters;
     *mgerror = err;
+    h->last_ok = fp64plain && alldir && iters > 0 && repin_skippable(h, P);
+    h->last_phi = phi;
+  });
+}
+
+int wm_mg3d_reuse_phi(wm_mg3d *h) {
+  WM_ENTRY({
+    WM_REQUIRE(h != nullptr, "wm_mg3d_reuse_phi: NULL handle");
+    h->reuse_next = true;
   });
 }
 

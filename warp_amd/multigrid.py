@@ -74,6 +74,7 @@ class MultiGrid3D(FieldSolver):
         self._cf = None            # conductor field data for f3d.lcorrectede
         self._cf_key = None
         self._cond_gen = 0         # counts uploads of conductor lists
+        self._phi_token = None     # (version, address) of phi after the last solve
         self._planes = None        # plane charges of gtlchg
         self._esework = None       # column sums of getese, then ese itself
         b = (C.c_int64 * 6)(*self.bounds)
@@ -349,11 +350,17 @@ class MultiGrid3D(FieldSolver):
             self._extrapolate_guess()
         it = C.c_int64(0)
         err = C.c_double(0.)
+        # phi untouched since the last solve returned it (torch's version
+        # counter counts every in-place change made through the tensor or
+        # its views): the solve may start from its own copy
+        if self._phi_token == (self._phi._version, self._phi.data_ptr()):
+            N.call("wm_mg3d_reuse_phi", self._h)
         N.call("wm_mg3d_solve", self._h, self._phi.data_ptr(), self._rho.data_ptr(),
                self.mgparam, self.mgform, self.mgmaxiters, self.mgtol, self.downpasses,
                self.uppasses, int(self.lcndbndy), self.icndbndy,
                PRECISIONS[self.precision], C.byref(it), C.byref(err),
                N.stream_of(self.device))
+        self._phi_token = (self._phi._version, self._phi.data_ptr())
         self.mgiters = it.value
         self.mgiters_history.append(self.mgiters)
         self.mgerror = err.value
@@ -383,6 +390,7 @@ class MultiGrid3D(FieldSolver):
             return
         N.call("wm_phi_extrapolate", self._phi.data_ptr(), self._phiprev.data_ptr(),
                self._phi.numel(), N.stream_of(self.device))
+        self._phi_token = None  # phi changed outside torch's view
 
     # ------------------------------------------------------------------
     # field diagnostics of special steps (multigrid.py:433-467)
