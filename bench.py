@@ -27,14 +27,18 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--config", default="c2")
     p.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--guess", default="previous", choices=["previous", "linear"],
                    help="initial guess of each solve: the previous phi (the reference's) or "
                         "its linear extrapolation in time")
+    p.add_argument("--mixed-precision", default="bf16", choices=["fp32", "bf16", "none"],
+                   help="precision of the second run reported in the 'mixed' block "
+                        "(none: no second run)")
+    p.add_argument("--mixed-guess", default="linear", choices=["previous", "linear"])
     p.add_argument("--sort-interval", type=int, default=25,
                    help="steps between particle sorts by cell (0: only at set-up)")
     p.add_argument("--cpu-particles", type=int, default=4_000_000,
@@ -46,40 +50,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only (the solver has no multi-GPU path); "
-              "got --gpus %d" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
+def run(args, cfg, precision, guess, dev, keep=False):
+    """One complete run of the workload: set-up, initial solve, warm-up
+    steps, then the timed step(K) call.  Returns a dict of the figures and,
+    with keep, the solver and particles."""
     import torch
-    from warp_amd import configs, registersolver, unregistersolver, top, _native as N
+    from warp_amd import configs, registersolver, unregistersolver, top
+    from warp_amd import stepper
     from warp_amd.stepper import step, generate
 
-    dev = torch.device("cuda", 0)
-    cfg = configs.CONFIGS[args.config]
+    top.it, top.time, top.lspecial, top.nspecial = 0, 0., False, 0
     t0 = time.time()
-    solver = configs.make_solver(cfg, precision=args.precision, device=dev, mgverbose=-1,
-                                 guess=args.guess)
+    solver = configs.make_solver(cfg, precision=precision, device=dev, mgverbose=-1,
+                                 guess=guess)
     pg = configs.load_beam(cfg, device=dev)
     top.pgroup = pg
     top.dt = configs.DT
     registersolver(solver)
-    npart0 = pg.npmax
+    npart0 = pg.nplive
     # particles ordered by cell once (particlesortxyzwithcopy) and again
     # every particlesortinterval steps; the beam drifts as a whole, so the
     # order stays nearly sorted in between
     solver.sortparticles(pg)
     top.particlesortinterval = args.sort_interval
-    log("set-up %.1f s: %d particles, mesh %s" % (time.time() - t0, npart0, cfg["n"]))
+    log("[%s/%s] set-up %.1f s: %d particles, mesh %s"
+        % (precision, guess, time.time() - t0, npart0, cfg["n"]))
     t0 = time.time()
     generate()
     torch.cuda.synchronize()
     cold_iters = solver.mgiters
-    log("first solve (cold start) %d V-cycles, %.2f s incl. conductor set-up"
-        % (cold_iters, time.time() - t0))
-
+    log("[%s/%s] first solve (cold start) %d V-cycles, %.2f s incl. conductor set-up"
+        % (precision, guess, cold_iters, time.time() - t0))
     if args.warmup > 0:
         step(args.warmup)
     torch.cuda.synchronize()
@@ -90,17 +91,52 @@ def main():
     solver.timers = dict(deposit=0., solve=0., fetche=0.)
     nspecial0 = top.nspecial
     nsolve0 = len(solver.mgiters_history)
+    counts = []
+    rec = lambda: counts.append(pg.nplive)
+    stepper.installafterstep(rec)
+    it0 = top.it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step(args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    stepper.uninstallafterstep(rec)
     iters = solver.mgiters_history[nsolve0:]
     nspecial = top.nspecial - nspecial0
     ms_per_step = 1e3 * elapsed / args.steps
     ms_solve = 1e3 * solver.timers["solve"] / args.steps
-    log("timed %d steps (%d special): %.3f ms/step, %.3f ms/solve, V-cycles %s"
-        % (args.steps, nspecial, ms_per_step, ms_solve, iters))
+    log("[%s/%s] timed %d steps (it %d..%d, %d special): %.3f ms/step, %.3f ms/solve, "
+        "V-cycles %s, particles %d -> %d" % (precision, guess, args.steps, it0 + 1, top.it,
+                                             nspecial, ms_per_step, ms_solve, iters,
+                                             counts[0], counts[-1]))
+    out = dict(ms_per_step=ms_per_step, ms_per_solve=ms_solve, iters=iters,
+               cold_iters=cold_iters, nspecial=nspecial, npart0=npart0,
+               particles_mean=float(sum(counts)) / len(counts), particles_at_end=counts[-1],
+               steps_timed=(it0 + 1, top.it))
+    if keep:
+        out["solver"], out["pg"] = solver, pg
+    else:
+        unregistersolver(solver)
+        top.pgroup = None
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (the solver has no multi-GPU path); "
+              "got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from warp_amd import configs, unregistersolver, top, _native as N
+    from warp_amd.stepper import step
+
+    dev = torch.device("cuda", 0)
+    cfg = configs.CONFIGS[args.config]
+    r = run(args, cfg, args.precision, args.guess, dev, keep=True)
+    solver, pg = r.pop("solver"), r.pop("pg")
+    ms_per_step, iters = r["ms_per_step"], r["iters"]
 
     # --- dominant kernel: finest-level SOR half sweep, HIP events on the
     # solver's stream around each launch (one more step, kernels launched
@@ -135,9 +171,38 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, cfg, solver, pg, phi_before, iters, nspecial / args.steps)
+        cpu = cpu_baseline(args, cfg, solver, pg, phi_before, iters,
+                           r["nspecial"] / args.steps)
 
     mgtol = solver.mgtol
+    mixed = None
+    if args.mixed_precision != "none":
+        # the same workload with the mixed-precision solve and the better
+        # initial guess; phi and the grid E of its last step against the
+        # FP64 run's
+        phi64 = solver.phi.clone()
+        e64 = solver.getselfe().clone()
+        nstep64 = top.it
+        unregistersolver(solver)
+        top.pgroup = None
+        del solver, pg
+        torch.cuda.empty_cache()
+        m = run(args, cfg, args.mixed_precision, args.mixed_guess, dev, keep=True)
+        step(nstep64 - top.it)  # the FP64 run's extra timing step
+        torch.cuda.synchronize()
+        ph, e = m["solver"].phi, m["solver"].getselfe()
+        rel = lambda u, v: float(torch.linalg.vector_norm(u - v) / torch.linalg.vector_norm(v))
+        mixed = {"precision": args.mixed_precision, "initial_guess": args.mixed_guess,
+                 "ms_per_step": m["ms_per_step"], "steps_per_s": 1e3 / m["ms_per_step"],
+                 "ms_per_solve": m["ms_per_solve"],
+                 "vcycles_per_solve": float(np.mean(m["iters"])),
+                 "speedup_vs_fp64": ms_per_step / m["ms_per_step"],
+                 "phi_rel_l2_vs_fp64": rel(ph, phi64), "e_rel_l2_vs_fp64": rel(e, e64),
+                 "particles_mean": m["particles_mean"]}
+        log("mixed: %s" % json.dumps(mixed))
+        unregistersolver(m["solver"])
+        top.pgroup = None
+
     out = {
         "metric": "PIC steps/s at 256^2x512, 32M ptcls; V-cycles & ms per solve to mgtol=1e-6",
         "value": 1e3 / ms_per_step,
@@ -154,22 +219,37 @@ def main():
                   }[args.precision],
         "data": "synthetic (seeded KV beam, pipe + 10 kV annulus)",
         "config": {"workload": "ES PIC step %dx%dx%d, %d particles, subgrid conductors"
-                               % (nx, ny, nz, npart0),
-                   "mesh": [nx, ny, nz], "particles": npart0, "particles_at_end": pg.npmax,
+                               % (nx, ny, nz, r["npart0"]),
+                   "mesh": [nx, ny, nz], "particles": r["npart0"],
+                   "particles_mean_timed": r["particles_mean"],
+                   "particles_at_end": r["particles_at_end"],
+                   "timed_steps_it": list(r["steps_timed"]),
                    "mgtol": mgtol, "particle_sort_interval": args.sort_interval,
                    "precision": args.precision, "initial_guess": args.guess,
                    "parallelism": "1 GPU"},
         "vcycles_per_solve": float(np.mean(iters)),
-        "vcycles_cold_solve": cold_iters,
-        "ms_per_solve": ms_solve,
+        "vcycles_cold_solve": r["cold_iters"],
+        "ms_per_solve": r["ms_per_solve"],
         "roofline": {"kernel": "k_sor_zmarch (finest-level SOR half sweep)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
-    unregistersolver(solver)
     print(json.dumps(out), flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_threads():
@@ -198,12 +278,13 @@ def cpu_baseline(args, cfg, solver, pg, phi_before, iters, fspecial):
     nx, ny, nz = cfg["n"]
     x0, x1, y0, y1, z0, z1 = configs.DOMAIN
     dx, dy, dz = (x1 - x0) / nx, (y1 - y0) / ny, (z1 - z0) / nz
-    npart = pg.npmax
+    live = pg.species_slice(0)
+    npart = live.stop - live.start
     stride = max(1, -(-npart // args.cpu_particles))
-    sl = slice(0, npart, stride)  # every stride-th particle: the whole beam
-    ns = len(range(npart)[sl])
+    sl = slice(live.start, live.stop, stride)  # every stride-th particle: the whole beam
     cols0 = [getattr(pg, c)[sl].cpu().numpy().copy() for c in ("xp", "yp", "zp", "uxp", "uyp",
                                                                 "uzp")]
+    ns = len(cols0[0])
     q, m, w = pg.sq[0], pg.sm[0], pg.sw[0]
     co = O.install_conductors(configs.conductors(cfg["conductors"], O), nx, ny, nz, x0, x1,
                               y0, y1, z0, z1)
@@ -224,12 +305,14 @@ def cpu_baseline(args, cfg, solver, pg, phi_before, iters, fspecial):
         O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
         O.epush3d(ux, uy, uz, ex, ey, ez, q, m, 0.5 * configs.DT)
         O.xpush3d(x, y, z, ux, uy, uz, gi, configs.DT)
+        # particleboundaries3d + clearpart (fill method 1)
+        O.particleboundaries3d(x, y, z, gi, dx, dy, dz, nx, ny, nz, x0, y0, z0)
+        nk = O.clearpart([x, y, z, ux, uy, uz, gi], 1)
         t_push = time.perf_counter() - t
         t = time.perf_counter()
-        k = (x >= x0) & (x < x1) & (y >= y0) & (y < y1) & (z >= z0) & (z < z1)
-        O.setrho3d(rho, x[k], y[k], z[k], q, w, dx, dy, dz, x0, y0, z0)
+        O.setrho3d(rho, x[:nk], y[:nk], z[:nk], q, w, dx, dy, dz, x0, y0, z0)
         t_dep = time.perf_counter() - t
-        xs, ys, zs, uxs, uys, uzs = x[k], y[k], z[k], ux[k], uy[k], uz[k]
+        xs, ys, zs, uxs, uys, uzs = x[:nk], y[:nk], z[:nk], ux[:nk], uy[:nk], uz[:nk]
         t = time.perf_counter()
         es = [np.zeros(len(xs)) for _ in range(3)]
         O.sete3d(phi, xs, ys, zs, *es, dx, dy, dz, x0, y0, z0)
@@ -251,6 +334,7 @@ def cpu_baseline(args, cfg, solver, pg, phi_before, iters, fspecial):
     step_1, solve_1, _ = one_step(1)
     scale = npart / ns
     return {"value": 1. / step_n, "unit": "steps/s", "cores": nthr, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "one step: gather+push and deposit of every %d-th particle (%d of %d, times scaled "
                       "by %.2f; the deposit serial), synchv kick + moments weighted %.2f, full "
                       "%dx%dx%d solve warm-started from the GPU's phi (%d V-cycles, %.2f s on %d "

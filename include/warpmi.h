@@ -105,8 +105,15 @@ int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream);
  * conductor record on an interior point, an up pass, and after a solve of
  * the same kind that ran at least one V-cycle -- otherwise the hint is
  * ignored.  The result is the same bits either way.  The hint applies to
- * the next solve only. */
+ * the next solve only.  The per-level entry points (wm_mg3d_sorhalfpass,
+ * wm_mg3d_sorpass, wm_mg3d_residual) overwrite the solver's level buffers,
+ * so after any of them the next solve converts phi again. */
 int wm_mg3d_reuse_phi(wm_mg3d *h);
+
+/* No counterpart in the reference: *reused = 1 when the last wm_mg3d_solve
+ * on h started from its own copy of phi (the hint above was honoured),
+ * else 0. */
+int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused);
 
 /* Event timing of the finest-level FP64 SOR half sweep (the solver's
  * dominant kernel): while on, wm_mg3d_solve launches its kernels one
@@ -201,9 +208,9 @@ int wm_xpush3d(int64_t np, double *xp, double *yp, double *zp,
  * E zeroed and fetched (sete3d), epush3d(dt/2) twice, xpush3d(dt), in one
  * pass.  ex, ey, ez may be NULL; when given they receive E.  nlost (device
  * memory, may be NULL) is incremented by the number of particles whose new
- * position fails particleboundaries3d's absorbing test
- * x in [xmmin, xmmax), y in [ymmin, ymmax), z in [zmmin+zgrid, zmmax+zgrid)
- * (w3d.F:11292-). */
+ * position particleboundaries3d would absorb: x <= xmmin or x >= xmmax, the
+ * same in y, z-zgrid <= zmmin or z-zgrid >= zmmax (top.F:3550-3560,
+ * 3779-3795; the reference passes xmmax = xmmin+nx*dx, ..., w3d.F:11344-11356). */
 int wm_gather_push3d(const double *phi, int64_t np, double *xp, double *yp,
                      double *zp, double *uxp, double *uyp, double *uzp,
                      const double *gaminv, double *ex, double *ey,
@@ -345,6 +352,46 @@ int64_t wm_getese3d_worksize(int64_t nx, int64_t ny);
 int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny,
                 int64_t nz, double dx, double dy, double dz, int64_t l4symtry,
                 int64_t l2symtry, void *work, double *ese, void *stream);
+
+/* particleboundaries3d (w3d.F:11292-11450) for absorbing boundaries on
+ * all six faces (pboundxy = pbound0 = pboundnz = absorb), followed by
+ * processlostpart -> clearpart (top.F:4936-5070, 4730-4893) on one species
+ * block of np particles.  cols (a host array of ncol device pointers, 7 <=
+ * ncol <= 16) starts xp, yp, zp, uxp, uyp, uzp, gaminv; further columns
+ * (ex, ey, ez, ...) move along.  A particle is flagged lost (gaminv = 0) when
+ * x <= xmmin or x >= xmmin+nx*dx, the same in y, or z-zgrid >= zmmin+nz*dz
+ * or z-zgrid <= zmmin (xparticleboundaries top.F:3550-3560, its y twin,
+ * zparticleboundaries top.F:3779-3795).  Then every particle with gaminv
+ * <= 0 is removed with clearpart's fillmethod (top.v clearlostpart):
+ *   1: in place, holes filled with live particles taken from the end (the
+ *      reference's default; dstcols NULL);
+ *   2: live particles shifted down, order kept; written to dstcols (ncol
+ *      other device columns), the input columns are left as they were.
+ * *nps_out (device memory) receives the new particle count.  work (device)
+ * must hold wm_clearpart_worksize(np) bytes. */
+int64_t wm_clearpart_worksize(int64_t np);
+int wm_particleboundaries3d(int64_t np, int64_t ncol, double *const *cols, double xmmin,
+                            double ymmin, double zmmin, double dx, double dy, double dz,
+                            int64_t nx, int64_t ny, int64_t nz, double zgrid,
+                            int64_t fillmethod, double *const *dstcols, int64_t *nps_out,
+                            void *work, void *stream);
+
+/* clearpart (top.F:4730-4893) alone: removes the particles already flagged
+ * with gaminv <= 0; arguments as wm_particleboundaries3d. */
+int wm_clearpart(int64_t np, int64_t ncol, double *const *cols, int64_t fillmethod,
+                 double *const *dstcols, int64_t *nps_out, void *work, void *stream);
+
+/* getgrid3d (dtop.F:2818-2920) over a node-centred grid (0:nx,0:ny,0:nz)
+ * spanning [xmin,xmax]x[ymin,ymax]x[zmin,zmax]: f at each position inside
+ * the grid is the trilinear interpolation, in the reference's order of
+ * operations; f is left unchanged outside.  With l4symtry x and y, with
+ * l2symtry y, are reflected about 0 first.  MultiGrid3D.fetchphi calls it
+ * with phi including its guard cells (multigrid.py:560-575): nx+2 cells
+ * from xmmin-dx to xmmax+dx, ... */
+int wm_getgrid3d(int64_t np, const double *x, const double *y, const double *z,
+                 double *f, int64_t nx, int64_t ny, int64_t nz, const double *grid,
+                 double xmin, double xmax, double ymin, double ymax, double zmin,
+                 double zmax, int64_t l2symtry, int64_t l4symtry, void *stream);
 
 /* particlesortxyzwithcopy (top.F:5095-5138) with particlesortbyindex
  * (top.F:5140-5203): the permutation that orders np particles by the block

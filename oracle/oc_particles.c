@@ -303,3 +303,118 @@ void oc_getzmmnt(int64_t np, const double *xp, const double *yp, const double *z
     }
   }
 }
+
+/* The absorbing particle boundaries of particleboundaries3d
+ * (w3d.F:11292-11450) with pboundxy = pbound0 = pboundnz = absorb and
+ * lcountaslost: xparticleboundaries (top.F:3499-3580) and its y twin flag a
+ * particle with gaminv = 0 when xp <= xmin or xp >= xmax; zparticleboundaries
+ * (top.F:3735-3800) when zp-zg >= zmax or zp-zg <= zmin.  The extents are
+ * those getparticleextent and particleboundaries3d pass: xmax = xmmin+nx*dx,
+ * zmax = zmmin+nz*dz (w3d.F:11344-11356). */
+void oc_particleboundaries3d(int64_t np, const double *xp, const double *yp,
+                             const double *zp, double *gaminv, double xmmin,
+                             double ymmin, double zmmin, double dx, double dy,
+                             double dz, int64_t nx, int64_t ny, int64_t nz,
+                             double zgrid) {
+  double xmax = xmmin + nx * dx, ymax = ymmin + ny * dy, zmax = zmmin + nz * dz;
+  int64_t ip;
+  for (ip = 0; ip < np; ip++)
+    if (xp[ip] <= xmmin || xp[ip] >= xmax) gaminv[ip] = 0.;
+  for (ip = 0; ip < np; ip++)
+    if (yp[ip] <= ymmin || yp[ip] >= ymax) gaminv[ip] = 0.;
+  for (ip = 0; ip < np; ip++) {
+    if ((zp[ip] - zgrid) >= zmax)
+      gaminv[ip] = 0.;
+    else if ((zp[ip] - zgrid) <= zmmin)
+      gaminv[ip] = 0.;
+  }
+}
+
+/* clearpart (top.F:4730-4893) on one species block of np particles whose
+ * columns are cols[0..ncol); cols[iuz] is uzp and cols[igam] gaminv (the
+ * lost flag: gaminv <= 0).  fillmethod 1 moves live particles from the end
+ * into the holes, 2 shifts the live particles down keeping their order.
+ * Returns the new nps. */
+int64_t oc_clearpart(int64_t np, int64_t ncol, double **cols, int64_t iuz,
+                     int64_t igam, int64_t fillmethod) {
+  double *g = cols[igam];
+  int64_t nps = 0, c;
+  if (fillmethod == 1) {
+    int64_t ips = np - 1, ip = 0;
+    while (ip <= ips) {
+      if (g[ip] <= 0.) {
+        while (g[ips] <= 0. && ips > ip) ips--;
+        if (ips > ip) {
+          for (c = 0; c < ncol; c++) cols[c][ip] = cols[c][ips];
+          cols[iuz][ips] = 0.;
+          g[ips] = 0.;
+          ips--;
+          nps++;
+        }
+      } else {
+        nps++;
+      }
+      ip++;
+    }
+  } else if (fillmethod == 2) {
+    int64_t ip = 0, ips;
+    for (ips = 0; ips < np; ips++) {
+      if (g[ips] > 0.) {
+        if (ips > ip) {
+          for (c = 0; c < ncol; c++) cols[c][ip] = cols[c][ips];
+          cols[iuz][ips] = 0.;
+          g[ips] = 0.;
+        }
+        ip++;
+        nps++;
+      }
+    }
+  } else {
+    nps = np;
+  }
+  return nps;
+}
+
+/* getgrid3d (dtop.F:2818-2920), ny > 0 branch: linear interpolation of the
+ * node-centred grid (0:nx,0:ny,0:nz) over [xmin,xmax]x[ymin,ymax]x[zmin,zmax]
+ * at the positions; f is left unchanged outside the grid.  The fetchphi
+ * path passes phi with its guard cells (multigrid.py:560-575). */
+void oc_getgrid3d(int64_t np, const double *x, const double *y, const double *z,
+                  double *f, int64_t nx, int64_t ny, int64_t nz, const double *grid,
+                  double xmin, double xmax, double ymin, double ymax, double zmin,
+                  double zmax, int64_t l2symtry, int64_t l4symtry) {
+  double dxi = nx / (xmax - xmin);
+  double dyi = (ymax == ymin) ? 1. : ny / (ymax - ymin);
+  double dzi = nz / (zmax - zmin);
+  int64_t ip;
+#define G(i, j, k) grid[(i) + (nx + 1) * ((j) + (ny + 1) * (int64_t)(k))]
+  for (ip = 0; ip < np; ip++) {
+    double xx, yy, gx = 0., gy = 0., gz = 0.;
+    if (l4symtry) {
+      xx = fabs(x[ip] - 0.) + 0.;
+      yy = fabs(y[ip] - 0.) + 0.;
+    } else if (l2symtry) {
+      xx = x[ip];
+      yy = fabs(y[ip] - 0.) + 0.;
+    } else {
+      xx = x[ip];
+      yy = y[ip];
+    }
+    if (nx > 0) gx = (xx - xmin) * dxi;
+    if (ny > 0) gy = (yy - ymin) * dyi;
+    if (nz > 0) gz = (z[ip] - zmin) * dzi;
+    if (0. <= gx && gx < nx && 0. <= gy && gy < ny && 0. <= gz && gz < nz) {
+      int64_t ix = (int64_t)gx, iy = (int64_t)gy, iz = (int64_t)gz;
+      double wx = gx - ix, wy = gy - iy, wz = gz - iz;
+      f[ip] = G(ix, iy, iz) * (1. - wx) * (1. - wy) * (1. - wz) +
+              G(ix + 1, iy, iz) * wx * (1. - wy) * (1. - wz) +
+              G(ix, iy + 1, iz) * (1. - wx) * wy * (1. - wz) +
+              G(ix + 1, iy + 1, iz) * wx * wy * (1. - wz) +
+              G(ix, iy, iz + 1) * (1. - wx) * (1. - wy) * wz +
+              G(ix + 1, iy, iz + 1) * wx * (1. - wy) * wz +
+              G(ix, iy + 1, iz + 1) * (1. - wx) * wy * wz +
+              G(ix + 1, iy + 1, iz + 1) * wx * wy * wz;
+    }
+  }
+#undef G
+}

@@ -110,6 +110,12 @@ def _declare(L):
     L.oc_sete_from_e3d.argtypes = [P, _i64, P, P, P, _d, _d, _d, _d, _d, _d, _d, _i64,
                                    _i64, _i64, P, P, P]
     L.oc_getzmmnt.argtypes = [_i64, P, P, P, P, P, P, P, _i64, P]
+    L.oc_particleboundaries3d.argtypes = [_i64, P, P, P, P, _d, _d, _d, _d, _d, _d, _i64,
+                                          _i64, _i64, _d]
+    L.oc_clearpart.restype = _i64
+    L.oc_clearpart.argtypes = [_i64, _i64, P, _i64, _i64, _i64]
+    L.oc_getgrid3d.argtypes = [_i64, P, P, P, P, _i64, _i64, _i64, P, _d, _d, _d, _d, _d,
+                               _d, _i64, _i64]
     L.oc_setup_icgrid.argtypes = [_i64, _i64, _i64, _i64, P, _i64, P, _i64, P, P]
     L.oc_efield_subgrid.argtypes = [P, _i64, _i64, _i64, _d, _d, _d, P, _i64, P, P, P, P, P]
     L.oc_fixefield.argtypes = [P, _i64, _i64, _i64, _i64, P]
@@ -510,6 +516,36 @@ def getzmmnt(xp, yp, zp, uxp, uyp, uzp, gaminv, usegaminv=False):
     a = [_f64(v) for v in (xp, yp, zp, uxp, uyp, uzp, gaminv)]
     lib().oc_getzmmnt(len(a[0]), *[_p(v) for v in a], int(usegaminv), _p(mm))
     return mm
+
+
+def particleboundaries3d(xp, yp, zp, gaminv, dx, dy, dz, nx, ny, nz, xmmin, ymmin, zmmin,
+                         zgrid=0.0):
+    """Flags particles outside the absorbing mesh boundaries with gaminv = 0
+    (particleboundaries3d, w3d.F:11292-11450)."""
+    lib().oc_particleboundaries3d(len(xp), _p(xp), _p(yp), _p(zp), _p(gaminv), xmmin, ymmin,
+                                  zmmin, dx, dy, dz, nx, ny, nz, zgrid)
+
+
+def clearpart(cols, fillmethod=1):
+    """clearpart (top.F:4730-4893) in place on the columns
+    [xp, yp, zp, uxp, uyp, uzp, gaminv, ...] (float64 arrays of one length);
+    returns the number of live particles now at the front."""
+    cols = list(cols)
+    for c in cols:
+        assert c.dtype == np.float64 and c.flags.c_contiguous and len(c) == len(cols[0])
+    arr = (C.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    return int(lib().oc_clearpart(len(cols[0]), len(cols), arr, 5, 6, fillmethod))
+
+
+def getgrid3d(x, y, z, f, grid, xmin, xmax, ymin, ymax, zmin, zmax, l2symtry=False,
+              l4symtry=False):
+    """getgrid3d (dtop.F:2818-2920): f at the positions inside the grid
+    (shape (nz+1, ny+1, nx+1)) by linear interpolation; f unchanged elsewhere."""
+    nz, ny, nx = grid.shape[0] - 1, grid.shape[1] - 1, grid.shape[2] - 1
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    lib().oc_getgrid3d(len(x), _p(x), _p(y), _p(z), _p(f), nx, ny, nz, _p(g), xmin, xmax,
+                       ymin, ymax, zmin, zmax, int(l2symtry), int(l4symtry))
 
 
 def plane_sums(rho, l2symtry=False, l4symtry=False):

@@ -207,24 +207,53 @@ def test_solve_starts_from_a_changed_phi(device):
     po = np.zeros((nz + 3, ny + 3, nx + 3))
     ro = rho.copy()
 
-    def both(tag):
+    def both(tag, reused):
         s.rho = torch.from_numpy(ro).to(device)
         s.solve()
         it, err = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co)
         assert (s.mgiters, s.mgerror) == (it, err), tag
         np.testing.assert_array_equal(s.phi.cpu().numpy(), po, err_msg=tag)
+        assert s.last_solve_reused_phi() == reused, tag
 
-    both("cold")
+    both("cold", False)
     ro *= 1.1
-    both("untouched phi")
+    both("untouched phi", True)
     s.phi[10:20, 5:9, 7:30] += 3.0    # in place, through a view of the tensor
     po[10:20, 5:9, 7:30] += 3.0
     ro *= 0.95
-    both("changed through a view")
+    both("changed through a view", False)
     po *= 0.5
     s.phi = torch.from_numpy(po).to(device)
-    both("changed through the setter")
-    both("untouched again")
+    both("changed through the setter", False)
+    both("untouched again", True)
+
+
+def test_level_entry_points_end_phi_reuse(device):
+    """A per-level entry point between two solves (here the residual of
+    another phi, which runs through the solver's level-0 buffers) must not
+    let the next solve start from the stale copy: it converts the caller's
+    phi again and stays bitwise equal to the oracle."""
+    nx, ny, nz = 32, 32, 64
+    s, rho = _c1_like(device, nx, ny, nz)
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    ro = rho.copy()
+    s.rho = torch.from_numpy(ro).to(device)
+    s.solve()
+    O.multigrid3dsolve(po, ro.copy(), s.dx, s.dy, s.dz, conductors=co)
+    keep = s.phi.clone()
+    # the residual of a different phi, through the level-0 buffers
+    s.phi = keep * 0.5
+    s.getresidual()
+    s.phi = keep          # phi now holds the last solve's result again ...
+    s._phi_token = (s._phi._version, s._phi.data_ptr())   # ... and says so
+    ro *= 1.1
+    s.rho = torch.from_numpy(ro).to(device)
+    s.solve()
+    it, err = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co)
+    assert not s.last_solve_reused_phi()
+    assert (s.mgiters, s.mgerror) == (it, err)
+    np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
 
 
 def test_conductor_solve_mgform2_bitwise(device):

@@ -57,10 +57,13 @@ def _oracle_run(x, y, z, u, q, m, w, dt, nsteps, nhist, efetch=1, lcorrectede=Fa
         centers.append(center)
         for _ in range(1 if special else 2):
             O.epush3d(*u, *e, q, m, 0.5 * (dt * 1 * 1.))
-        O.xpush3d(x, y, z, *u, np.ones(len(x)), dt * 1 * 1.)
-        k = (x >= DOM[0]) & (x < DOM[1]) & (y >= DOM[2]) & (y < DOM[3]) & (z >= DOM[4]) & \
-            (z < DOM[5])
-        x, y, z, u = x[k], y[k], z[k], [a[k] for a in u]
+        gi = np.ones(len(x))
+        O.xpush3d(x, y, z, *u, gi, dt * 1 * 1.)
+        # particleboundaries3d: flag, then clearpart with fillmethod 1
+        O.particleboundaries3d(x, y, z, gi, dx, dy, dz, NX, NY, NZ, *lo)
+        cols = [x, y, z] + list(u) + [gi]
+        n = O.clearpart(cols, 1)
+        x, y, z, u = x[:n].copy(), y[:n].copy(), z[:n].copy(), [a[:n].copy() for a in u]
         solve(x, y, z)
         special = it % nhist == 0 or it == nsteps
         if special:
@@ -113,7 +116,9 @@ def test_step_driver_matches_oracle(device, efetch, lcorrectede):
         assert calls == centers
         assert centers == ["halfv", "fullv", "fullv", "fullv", "fullv", "synchv", "halfv",
                            "fullv", "synchv"]
-        got = [getattr(pg, c).cpu().numpy() for c in ("xp", "yp", "zp", "uxp", "uyp", "uzp")]
+        nlive = pg.nps[0]
+        got = [getattr(pg, c)[:nlive].cpu().numpy() for c in ("xp", "yp", "zp", "uxp", "uyp",
+                                                               "uzp")]
         assert len(got[0]) == len(xo)
         for a, b in zip(got, [xo, yo, zo] + uo):
             scale = np.abs(b).max()
@@ -175,7 +180,7 @@ def test_linear_guess_reaches_mgtol(device):
             stepper.generate()
             stepper.step(8)
             errs = s.mgerror
-            out[guess] = ([getattr(pg, c).cpu().numpy() for c in ("xp", "zp", "uzp")],
+            out[guess] = ([getattr(pg, c)[:pg.nps[0]].cpu().numpy() for c in ("xp", "zp", "uzp")],
                           list(s.mgiters_history), errs)
         finally:
             warp_amd.unregistersolver(s)

@@ -884,3 +884,131 @@ def test_line_charge_interpolation_is_linear_in_the_beam_frame(zbeam, nzzarr, dz
     np.testing.assert_allclose(lc, want, rtol=0, atol=1e-14 * (planes * dx * dy).max())
     if zbeam == 0.9:
         assert not lc.any()
+
+
+# ---------------------------------------------------------------------------
+# particle boundaries, clearpart (top.F:3499-3800, 4730-4893) and getgrid3d
+# (dtop.F:2818-2920)
+
+def _ids_columns(alive):
+    """Columns whose values name the particle (xp = index) with gaminv set
+    from alive, uzp = index + 0.5."""
+    n = len(alive)
+    idx = np.arange(n, dtype=np.float64)
+    cols = [idx.copy(), idx * 2., idx * 3., idx * 4., idx * 5., idx + 0.5,
+            np.where(alive, 1.0, 0.0), idx * 7.]
+    return cols
+
+
+def test_clearpart_hand_example():
+    """The worked example of clearpart's fill methods: live, lost, live,
+    lost, live, live.  Method 1 fills hole 1 with the last live particle
+    (5) and hole 3 with the one before it (4); method 2 keeps the order."""
+    alive = np.array([1, 0, 1, 0, 1, 1], bool)
+    c = _ids_columns(alive)
+    assert O.clearpart(c, 1) == 4
+    assert list(c[0][:4]) == [0., 5., 2., 4.]
+    # moved-from slots: uzp and gaminv cleared
+    assert c[5][5] == 0. and c[6][5] == 0. and c[5][4] == 0. and c[6][4] == 0.
+    c = _ids_columns(alive)
+    assert O.clearpart(c, 2) == 4
+    assert list(c[0][:4]) == [0., 2., 4., 5.]
+    c = _ids_columns(np.zeros(5, bool))
+    assert O.clearpart(c, 1) == 0
+    c = _ids_columns(np.ones(5, bool))
+    assert O.clearpart(c, 1) == 5 and list(c[0]) == [0., 1., 2., 3., 4.]
+
+
+@pytest.mark.parametrize("seed,frac", [(0, 0.0), (1, 0.1), (2, 0.5), (3, 0.93), (4, 1.0)])
+def test_clearpart_fill_rules(seed, frac):
+    """Method 1 characterised without its loop: the live set is kept, live
+    particles below the new end stay where they are, and the k-th hole
+    below the end receives the k-th live particle counted from the back.
+    Method 2: the live particles in their original order.  Every column
+    moves together."""
+    rng = np.random.default_rng(seed)
+    n = 997
+    alive = rng.random(n) >= frac
+    L = int(alive.sum())
+    c = _ids_columns(alive)
+    assert O.clearpart(c, 1) == L
+    ids = c[0][:L].astype(int)
+    assert sorted(ids) == list(np.nonzero(alive)[0])
+    holes = [i for i in range(L) if not alive[i]]
+    donors = [i for i in range(n - 1, L - 1, -1) if alive[i]]
+    assert len(holes) == len(donors)
+    for i in range(L):
+        if alive[i]:
+            assert ids[i] == i
+    for h, d in zip(holes, donors):
+        assert ids[h] == d
+    for k, m in ((1, 2.), (2, 3.), (3, 4.), (4, 5.), (7, 7.)):
+        np.testing.assert_array_equal(c[k][:L], ids * m)
+    np.testing.assert_array_equal(c[5][:L], ids + 0.5)
+    assert (c[6][:L] == 1.).all()
+    c = _ids_columns(alive)
+    assert O.clearpart(c, 2) == L
+    np.testing.assert_array_equal(c[0][:L], np.nonzero(alive)[0].astype(float))
+
+
+def test_particleboundaries_faces():
+    """The absorbing tests: a particle exactly on a lower or upper face is
+    lost (x <= xmin, x >= xmax), one a rounding step inside is kept; z is
+    compared after subtracting zgrid; the upper extent is xmmin+nx*dx."""
+    nx, ny, nz, dx, dy, dz = 4, 4, 8, 0.25, 0.25, 0.5
+    x0, y0, z0, zg = -0.5, -0.5, 0.0, 0.75
+    xs = [x0, np.nextafter(x0, 1), x0 + nx * dx, np.nextafter(x0 + nx * dx, -1), 0., 0., 0., 0.,
+          0., np.nan]
+    ys = [0., 0., 0., 0., y0, y0 + ny * dy, 0., 0., 0., 0.]
+    zs = [1., 1., 1., 1., 1., 1., z0 + zg, nz * dz + zg, np.nextafter(nz * dz + zg, 0), 1.]
+    x, y, z = (np.array(v, float) for v in (xs, ys, zs))
+    g = np.ones(len(x))
+    O.particleboundaries3d(x, y, z, g, dx, dy, dz, nx, ny, nz, x0, y0, z0, zg)
+    assert list(g) == [0., 1., 0., 1., 0., 0., 0., 0., 1., 1.]
+
+
+def test_getgrid3d_reproduces_linear_fields():
+    """Trilinear interpolation is exact for a linear field (to rounding),
+    gives node values on nodes exactly, and leaves f outside the grid
+    (including the upper faces, gx < nx) untouched."""
+    rng = np.random.default_rng(2)
+    nx, ny, nz = 7, 5, 9
+    lo, hi = (-0.3, -0.2, 0.1), (0.4, 0.3, 1.0)
+    xs = [np.linspace(l, h, n + 1) for l, h, n in zip(lo, hi, (nx, ny, nz))]
+    Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+    grid = 1.5 + 2.0 * X - 3.0 * Y + 0.5 * Z
+    n = 3000
+    x = rng.uniform(lo[0] - 0.1, hi[0] + 0.1, n)
+    y = rng.uniform(lo[1] - 0.1, hi[1] + 0.1, n)
+    z = rng.uniform(lo[2] - 0.1, hi[2] + 0.1, n)
+    f = np.full(n, -7.0)
+    O.getgrid3d(x, y, z, f, grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+    inside = ((x - lo[0]) * (nx / (hi[0] - lo[0])) >= 0) & \
+        ((x - lo[0]) * (nx / (hi[0] - lo[0])) < nx) & \
+        ((y - lo[1]) * (ny / (hi[1] - lo[1])) >= 0) & \
+        ((y - lo[1]) * (ny / (hi[1] - lo[1])) < ny) & \
+        ((z - lo[2]) * (nz / (hi[2] - lo[2])) >= 0) & \
+        ((z - lo[2]) * (nz / (hi[2] - lo[2])) < nz)
+    assert inside.sum() > n // 3 and (~inside).sum() > 0
+    np.testing.assert_allclose(f[inside], (1.5 + 2.0 * x - 3.0 * y + 0.5 * z)[inside],
+                               rtol=0, atol=1e-13)
+    assert (f[~inside] == -7.0).all()
+    # nodes (interior ones, chosen by index) get the node value exactly
+    i, j, k = 3, 2, 4
+    fx = np.array([xs[0][i]]), np.array([xs[1][j]]), np.array([xs[2][k]])
+    g1 = np.zeros(1)
+    O.getgrid3d(*fx, g1, grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+    gx = (fx[0][0] - lo[0]) * (nx / (hi[0] - lo[0]))
+    if gx == i:  # the scaled position lands on the node
+        assert g1[0] == grid[k, j, i] or abs(g1[0] - grid[k, j, i]) < 1e-14
+    # l4symtry reflects x and y about 0: f(-x, -y) = f(x, y)
+    f1, f2 = np.zeros(200), np.zeros(200)
+    xa, ya, za = rng.uniform(0, 0.3, 200), rng.uniform(0, 0.2, 200), rng.uniform(0.2, 0.9, 200)
+    O.getgrid3d(xa, ya, za, f1, grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], l4symtry=True)
+    O.getgrid3d(-xa, -ya, za, f2, grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2],
+                l4symtry=True)
+    np.testing.assert_array_equal(f1, f2)
+    # l2symtry reflects y only
+    O.getgrid3d(xa, -ya, za, f2, grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2],
+                l2symtry=True)
+    np.testing.assert_array_equal(f1, f2)

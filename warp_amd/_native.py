@@ -93,6 +93,7 @@ SIGNATURES = {
     "wm_getzmmnt_worksize": (_i64, [_i64]),
     "wm_getzmmnt": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _P, _i64, _P, _P, _P]),
     "wm_mg3d_reuse_phi": (C.c_int, [_P]),
+    "wm_mg3d_last_reused": (C.c_int, [_P, _P]),
     "wm_gtlchg3d": (C.c_int, [_P, _i64, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _d, _i64, _i64,
                               _i64, _P, _P, _P]),
     "wm_getese3d_worksize": (_i64, [_i64, _i64]),
@@ -114,6 +115,12 @@ SIGNATURES = {
                                    _P, _P, _P]),
     "wm_permute_columns": (C.c_int, [_i64, _i64, _P, _P, _P, _P]),
     "wm_getselfe3d": (C.c_int, [_P, _i64, _i64, _i64, _P, _d, _d, _d, _i64, _P]),
+    "wm_getgrid3d": (C.c_int, [_i64, _P, _P, _P, _P, _i64, _i64, _i64, _P, _d, _d, _d, _d, _d,
+                               _d, _i64, _i64, _P]),
+    "wm_clearpart_worksize": (_i64, [_i64]),
+    "wm_particleboundaries3d": (C.c_int, [_i64, _i64, _P, _d, _d, _d, _d, _d, _d, _i64, _i64,
+                                          _i64, _d, _i64, _P, _P, _P, _P]),
+    "wm_clearpart": (C.c_int, [_i64, _i64, _P, _i64, _P, _P, _P, _P]),
     "wm_zcylinder_intercepts": (C.c_int, [_d, _d, _d, _d, _d, _P, _d, _i64]),
     "wm_zsrfrv_intercepts": (C.c_int, [_i64, _P, _P, _d, _d, _d, _P]),
     "wm_condlist_create": (C.c_int, [_P]),

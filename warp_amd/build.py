@@ -16,7 +16,8 @@ LIB = os.path.join(LIBDIR, "libwarpmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["mg3d.hip", "particles.hip", "condfield.hip", "fielddiag.hip", "unet.hip"]
+HIP_SOURCES = ["mg3d.hip", "particles.hip", "lostpart.hip", "condfield.hip", "fielddiag.hip",
+               "unet.hip"]
 CXX_SOURCES = ["conductors.cpp", "abi.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",

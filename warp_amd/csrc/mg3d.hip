@@ -237,6 +237,7 @@ struct wm_mg3d {
   // its phi unchanged since (wm_mg3d_reuse_phi: reuse_next), the next solve
   // starts from that state.
   bool last_ok = false, reuse_next = false;
+  bool last_reused = false;  // the last solve started from that state
   const double *last_phi = nullptr;
   DevBuf<SolveState> state;
   DevBuf<double> partials;  // per-block maxima of the convergence test
@@ -1492,6 +1493,7 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
                        alldir && mgmaxiters > 0 && repin_skippable(h, P);
     h->reuse_next = false;
     h->last_ok = false;
+    h->last_reused = reuse;
     if (!reuse) {
       // the saved phi: written by the conversion itself when the boundary
       // conditions change nothing (six Dirichlet bounds), else copied after
@@ -1626,6 +1628,13 @@ int wm_mg3d_reuse_phi(wm_mg3d *h) {
   });
 }
 
+int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused) {
+  WM_ENTRY({
+    WM_REQUIRE(h && reused, "wm_mg3d_last_reused: NULL argument");
+    *reused = h->last_reused ? 1 : 0;
+  });
+}
+
 int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream) {
   WM_ENTRY({
     WM_REQUIRE(n >= 0, "wm_phi_extrapolate: n < 0");
@@ -1661,6 +1670,9 @@ int wm_mg3d_sorhalfpass(wm_mg3d *h, int64_t level, int64_t parity, double *phi, 
     WM_REQUIRE(h && phi && rho, "wm_mg3d_sorhalfpass: NULL argument");
     WM_REQUIRE(level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_sorhalfpass: bad level");
     WM_REQUIRE(parity == 0 || parity == 1, "wm_mg3d_sorhalfpass: parity must be 0 or 1");
+    // this call writes the level's red-black buffers (phi64 among them):
+    // the next solve cannot start from the last one's state
+    h->last_ok = false;
     hipStream_t s = wm_stream(stream);
     Level &L = *h->lv[level];
     Params P{mgparam, (int)mgform, 1, 1, (int)lcndbndy, (int)icndbndy};
@@ -1683,6 +1695,9 @@ int wm_mg3d_sorpass(wm_mg3d *h, int64_t level, double *phi, const double *rho, d
   WM_ENTRY({
     WM_REQUIRE(h && phi && rho, "wm_mg3d_sorpass: NULL argument");
     WM_REQUIRE(level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_sorpass: bad level");
+    // this call writes the level's red-black buffers (phi64 among them):
+    // the next solve cannot start from the last one's state
+    h->last_ok = false;
     hipStream_t s = wm_stream(stream);
     Level &L = *h->lv[level];
     Params P{mgparam, (int)mgform, 1, 1, (int)lcndbndy, (int)icndbndy};
@@ -1706,6 +1721,9 @@ int wm_mg3d_residual(wm_mg3d *h, int64_t level, const double *phi, const double 
   WM_ENTRY({
     WM_REQUIRE(h && phi && rho && res, "wm_mg3d_residual: NULL argument");
     WM_REQUIRE(level >= 0 && level < (int64_t)h->lv.size(), "wm_mg3d_residual: bad level");
+    // this call writes the level's red-black buffers (phi64 among them):
+    // the next solve cannot start from the last one's state
+    h->last_ok = false;
     hipStream_t s = wm_stream(stream);
     Level &L = *h->lv[level];
     Params P{mgparam, (int)mgform, 1, 1, (int)lcndbndy, (int)icndbndy};

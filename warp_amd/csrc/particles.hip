@@ -403,9 +403,10 @@ __global__ __launch_bounds__(256) void k_xpush(long long np, double *__restrict_
 }
 
 // padvnc3d "fullv" for B = 0 (w3d.F:1816, 1956-1986): E = 0 + gather,
-// two half kicks, one drift.  Particles whose new position leaves
-// [xmmin,xmmax) x [ymmin,ymmax) x [zmmin+zgrid,zmmax+zgrid) (the absorbing
-// test of particleboundaries3d) are counted into *nlost.
+// two half kicks, one drift.  Particles whose new position fails the
+// absorbing tests of particleboundaries3d (x <= xmmin or x >= xmmax, the
+// same in y, z-zgrid <= zmmin or z-zgrid >= zmmax) are counted into
+// *nlost.
 // CENTER selects padvnc3d's centring (w3d.F:1848-2330, B = 0, no lattice):
 // 0 "fullv": two half kicks with E(x), then the drift; 1 "halfv": one half
 // kick, then the drift; 2 "synchv": one half kick, no drift.  EGRID: E
@@ -459,8 +460,11 @@ __global__ __launch_bounds__(256) void k_gather_push(
       x[ip] = xn;
       y[ip] = yn;
       z[ip] = zn;
-      lost = !(xn >= m.xmmin && xn < xmmax && yn >= m.ymmin && yn < ymmax &&
-               zn >= m.zmmin + m.zgrid && zn < zmmax + m.zgrid);
+      // the absorbing tests of particleboundaries3d (top.F:3550-3560,
+      // 3779-3795): x <= xmin or x >= xmax, ..., z-zgrid >= zmax or <= zmin
+      double zz = zn - m.zgrid;
+      lost = xn <= m.xmmin || xn >= xmmax || yn <= m.ymmin || yn >= ymmax || zz >= zmmax ||
+             zz <= m.zmmin;
     }
   }
   if (nlost) {
@@ -468,6 +472,42 @@ __global__ __launch_bounds__(256) void k_gather_push(
     if (b && (threadIdx.x & 63) == (__ffsll((long long)b) - 1))
       atomicAdd(nlost, (unsigned long long)__popcll(b));
   }
+}
+
+// getgrid3d (dtop.F:2818-2920), ny > 0: trilinear interpolation of a
+// node-centred grid at each position inside it, in the reference's order of
+// operations; f untouched outside the grid.
+__global__ __launch_bounds__(256) void k_getgrid3d(long long np, const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   const double *__restrict__ z,
+                                                   double *__restrict__ f, int nx, int ny, int nz,
+                                                   const double *__restrict__ grid, double xmin,
+                                                   double ymin, double zmin, double dxi,
+                                                   double dyi, double dzi, int l2symtry,
+                                                   int l4symtry) {
+  long long ip = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (ip >= np) return;
+  double xx, yy;
+  if (l4symtry) {
+    xx = fabs(x[ip] - 0.) + 0.;
+    yy = fabs(y[ip] - 0.) + 0.;
+  } else if (l2symtry) {
+    xx = x[ip];
+    yy = fabs(y[ip] - 0.) + 0.;
+  } else {
+    xx = x[ip];
+    yy = y[ip];
+  }
+  double gx = (xx - xmin) * dxi, gy = (yy - ymin) * dyi, gz = (z[ip] - zmin) * dzi;
+  if (!(0. <= gx && gx < nx && 0. <= gy && gy < ny && 0. <= gz && gz < nz)) return;
+  long long ix = (long long)gx, iy = (long long)gy, iz = (long long)gz;
+  double wx = gx - ix, wy = gy - iy, wz = gz - iz;
+  const long long sx = nx + 1, sxy = (long long)(nx + 1) * (ny + 1);
+  const double *g = grid + ix + sx * iy + sxy * iz;
+  f[ip] = g[0] * (1. - wx) * (1. - wy) * (1. - wz) + g[1] * wx * (1. - wy) * (1. - wz) +
+          g[sx] * (1. - wx) * wy * (1. - wz) + g[sx + 1] * wx * wy * (1. - wz) +
+          g[sxy] * (1. - wx) * (1. - wy) * wz + g[sxy + 1] * wx * (1. - wy) * wz +
+          g[sxy + sx] * (1. - wx) * wy * wz + g[sxy + sx + 1] * wx * wy * wz;
 }
 
 // getselfe3d (w3d.F:6023-6109), nxguarde = 0.
@@ -974,6 +1014,27 @@ int wm_permute_columns(int64_t np, int64_t ncol, const double *const *src, doubl
     }
     hipLaunchKernelGGL(k_permute, dim3(nblk(np)), dim3(256), 0, wm_stream(stream), (long long)np, c,
                        (const unsigned int *)perm);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_getgrid3d(int64_t np, const double *x, const double *y, const double *z, double *f,
+                 int64_t nx, int64_t ny, int64_t nz, const double *grid, double xmin, double xmax,
+                 double ymin, double ymax, double zmin, double zmax, int64_t l2symtry,
+                 int64_t l4symtry, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(np >= 0, "wm_getgrid3d: np < 0");
+    WM_REQUIRE(nx > 0 && ny > 0 && nz > 0, "wm_getgrid3d: needs nx, ny, nz > 0");
+    // getgrid3d's kaboom (dtop.F:2842-2845)
+    WM_REQUIRE(xmax != xmin && zmax != zmin, "getgrid3d: the grid extent is zero, xmin==xmax or zmin==zmax");
+    if (np == 0) return 0;
+    WM_REQUIRE(x && y && z && f && grid, "wm_getgrid3d: NULL array");
+    double dxi = nx / (xmax - xmin);
+    double dyi = (ymax == ymin) ? 1. : ny / (ymax - ymin);
+    double dzi = nz / (zmax - zmin);
+    hipLaunchKernelGGL(k_getgrid3d, dim3(nblk(np)), dim3(256), 0, wm_stream(stream),
+                       (long long)np, x, y, z, f, (int)nx, (int)ny, (int)nz, grid, xmin, ymin,
+                       zmin, dxi, dyi, dzi, (int)(l2symtry != 0), (int)(l4symtry != 0));
     WM_LAUNCHED();
   });
 }

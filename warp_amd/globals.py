@@ -45,6 +45,9 @@ top = Top(dt=0., time=0., it=0, fstype=-1, pgroup=None, zbeam=0., zgrid=0.,
           fstime=0., lrtime=0.,
           # steps between particlesortxyzwithcopy calls in step(); 0 = never
           particlesortinterval=0,
+          # clearpart's fill method for lost particles (top.v:1260): 1 fills
+          # holes with particles from the end, 2 keeps the order, 0 leaves them
+          clearlostpart=1,
           # step controls of w3dexe/step3d (top.v): "special" steps end with
           # the synchv half kick and the beam moments
           nt=20000000, tstop=1.0e36, zstop=1.0e36, itmomnts=[0] * 50, nhist=5, itdump=500,

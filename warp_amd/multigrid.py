@@ -353,14 +353,15 @@ class MultiGrid3D(FieldSolver):
         # phi untouched since the last solve returned it (torch's version
         # counter counts every in-place change made through the tensor or
         # its views): the solve may start from its own copy
-        if self._phi_token == (self._phi._version, self._phi.data_ptr()):
+        tok = self._phi_version()
+        if tok is not None and self._phi_token == tok:
             N.call("wm_mg3d_reuse_phi", self._h)
         N.call("wm_mg3d_solve", self._h, self._phi.data_ptr(), self._rho.data_ptr(),
                self.mgparam, self.mgform, self.mgmaxiters, self.mgtol, self.downpasses,
                self.uppasses, int(self.lcndbndy), self.icndbndy,
                PRECISIONS[self.precision], C.byref(it), C.byref(err),
                N.stream_of(self.device))
-        self._phi_token = (self._phi._version, self._phi.data_ptr())
+        self._phi_token = self._phi_version()
         self.mgiters = it.value
         self.mgiters_history.append(self.mgiters)
         self.mgerror = err.value
@@ -378,6 +379,21 @@ class MultiGrid3D(FieldSolver):
             print("Multigrid: Error converged to %11.3e in %5d v-cycles"
                   % (self.mgerror, self.mgiters), file=sys.stderr)
         self.timers["solve"] += time.perf_counter() - t0
+
+    def last_solve_reused_phi(self):
+        """True when the last solve started from the solver's own copy of
+        the phi the solve before it returned (phi untouched in between)."""
+        r = C.c_int64(0)
+        N.call("wm_mg3d_last_reused", self._h, C.byref(r))
+        return bool(r.value)
+
+    def _phi_version(self):
+        """(version, address) of phi, or None where torch keeps no version
+        counter (tensors made under torch.inference_mode)."""
+        try:
+            return (self._phi._version, self._phi.data_ptr())
+        except RuntimeError:
+            return None
 
     def _extrapolate_guess(self):
         """phi <- 2 phi_(n-1) - phi_(n-2) before a solve (the previous phi
@@ -508,36 +524,26 @@ class MultiGrid3D(FieldSolver):
         self.fetchpotentialfrompositions(w3d.xfsapi, w3d.yfsapi, w3d.zfsapi, w3d.phifsapi)
 
     def fetchpotentialfrompositions(self, x, y, z, phi):
-        """getgrid3d (pywarp90/dtop.F:2818-) over the potential including
-        its guard cells (multigrid.py:560-575): trilinear interpolation."""
+        """getgrid3d (pywarp90/dtop.F:2818-2920) over the potential
+        including its guard cells (multigrid.py:560-575): trilinear
+        interpolation into phi at the positions inside the guarded mesh,
+        phi unchanged elsewhere."""
+        n = int(x.numel())
+        if n == 0:
+            return
+        for t in (x, y, z, phi):
+            if not (isinstance(t, torch.Tensor) and t.is_contiguous() and t.device == self.device
+                    and t.dtype == torch.float64):
+                raise ValueError("fetchphi arrays must be contiguous float64 tensors on %s"
+                                 % self.device)
         dx, dy, dz = self.dx, self.dy, self.dz
         nxp, nyp, nzp = self.nx + 2, self.ny + 2, self.nz + 2
         xmin, xmax = self.xmmin - dx, self.xmmax + dx
         ymin, ymax = self.ymmin - dy, self.ymmax + dy
         zmin, zmax = self.zmmin - dz + top.zgrid, self.zmmax + dz + top.zgrid
-        dxi = nxp / (xmax - xmin)
-        dyi = nyp / (ymax - ymin)
-        dzi = nzp / (zmax - zmin)
-        gx = (x - xmin) * dxi
-        gy = (y - ymin) * dyi
-        gz = (z - zmin) * dzi
-        inside = ((gx >= 0.) & (gx < nxp) & (gy >= 0.) & (gy < nyp) & (gz >= 0.) &
-                  (gz < nzp))
-        ix = gx.to(torch.int64).clamp(0, nxp - 1)
-        iy = gy.to(torch.int64).clamp(0, nyp - 1)
-        iz = gz.to(torch.int64).clamp(0, nzp - 1)
-        wx, wy, wz = gx - ix, gy - iy, gz - iz
-        p = self._phi
-        f = (p[iz, iy, ix] * (1. - wx) * (1. - wy) * (1. - wz) +
-             p[iz, iy, ix + 1] * wx * (1. - wy) * (1. - wz) +
-             p[iz, iy + 1, ix] * (1. - wx) * wy * (1. - wz) +
-             p[iz, iy + 1, ix + 1] * wx * wy * (1. - wz) +
-             p[iz + 1, iy, ix] * (1. - wx) * (1. - wy) * wz +
-             p[iz + 1, iy, ix + 1] * wx * (1. - wy) * wz +
-             p[iz + 1, iy + 1, ix] * (1. - wx) * wy * wz +
-             p[iz + 1, iy + 1, ix + 1] * wx * wy * wz)
-        f = torch.where(inside, f, phi)
-        phi.copy_(f)
+        N.call("wm_getgrid3d", n, x.data_ptr(), y.data_ptr(), z.data_ptr(), phi.data_ptr(),
+               nxp, nyp, nzp, self._phi.data_ptr(), xmin, xmax, ymin, ymax, zmin, zmax,
+               int(self.l2symtry), int(self.l4symtry), N.stream_of(self.device))
 
     # fused fullv advance used by the stepper
     _CENTERS = {"fullv": 0, "halfv": 1, "synchv": 2}
@@ -566,8 +572,8 @@ class MultiGrid3D(FieldSolver):
                    col("uxp").data_ptr(), col("uyp").data_ptr(), col("uzp").data_ptr(),
                    col("gaminv").data_ptr(), col("ex").data_ptr(), col("ey").data_ptr(),
                    col("ez").data_ptr(), top.zgrid, self.xmmin, self.ymmin, self.zmmin, self.dx,
-                   self.dy, self.dz, pg.sq[js], pg.sm[js], dt, self.xmmax, self.ymmax,
-                   self.zmmax, self._nlost.data_ptr(), self._CENTERS[center],
+                   self.dy, self.dz, pg.sq[js], pg.sm[js], dt, *self._particle_extent(),
+                   self._nlost.data_ptr(), self._CENTERS[center],
                    N.stream_of(self.device))
             self.timers["fetche"] += time.perf_counter() - t0
             return
@@ -580,9 +586,15 @@ class MultiGrid3D(FieldSolver):
                col("uyp").data_ptr(), col("uzp").data_ptr(), col("gaminv").data_ptr(),
                col("ex").data_ptr(), col("ey").data_ptr(), col("ez").data_ptr(), top.zgrid,
                self.xmmin, self.ymmin, self.zmmin, self.dx, self.dy, self.dz, self.nx,
-               self.ny, self.nz, pg.sq[js], pg.sm[js], dt, self.xmmax, self.ymmax, self.zmmax,
+               self.ny, self.nz, pg.sq[js], pg.sm[js], dt, *self._particle_extent(),
                self._nlost.data_ptr(), self._CENTERS[center], N.stream_of(self.device))
         self.timers["fetche"] += time.perf_counter() - t0
+
+    def _particle_extent(self):
+        """Upper extents of the absorbing particle boundaries:
+        getparticleextent's xmmin+nx*dx, ... (w3d.F:11344-11356)."""
+        return (self.xmmin + self.nx * self.dx, self.ymmin + self.ny * self.dy,
+                self.zmmin + self.nz * self.dz)
 
     def take_lost_count(self):
         """Particles counted outside the mesh by gather_push since the last

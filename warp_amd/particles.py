@@ -32,7 +32,15 @@ class ParticleGroup:
 
     @property
     def npmax(self):
+        """Length of the particle arrays (the reference's npmax): live
+        particles of species js are [ins-1, ins-1+nps) of it; clearpart
+        leaves the slots past a species' live block unused."""
         return int(self.xp.numel())
+
+    @property
+    def nplive(self):
+        """Live particles of all species."""
+        return int(sum(self.nps))
 
     def addspecies(self, charge, mass, weight):
         self.sq.append(float(charge))
@@ -55,8 +63,13 @@ class ParticleGroup:
         cols["gaminv"] = gaminv
         for c in ("ex", "ey", "ez"):
             cols[c] = torch.zeros(n, dtype=self.dtype, device=self.device)
-        assert self.ins[js] - 1 + self.nps[js] == self.npmax, \
+        end = self.ins[js] - 1 + self.nps[js]
+        assert all(self.ins[k] <= self.ins[js] for k in range(self.ns)), \
             "particles must be appended to the last species"
+        if end < self.npmax:
+            # slots clearpart left past the species' live block
+            for c in self.COLUMNS:
+                setattr(self, c, getattr(self, c)[:end])
         for c in self.COLUMNS:
             v = torch.as_tensor(cols[c], dtype=self.dtype, device=self.device).reshape(-1)
             setattr(self, c, torch.cat([getattr(self, c), v]))
@@ -66,24 +79,61 @@ class ParticleGroup:
         i = self.ins[js] - 1
         return slice(i, i + self.nps[js])
 
-    def compact(self, keep):
-        """Drop particles where keep is False, species by species
-        (the reference's particle boundary handling removes lost
-        particles)."""
-        new_ins, new_nps, parts = [], [], []
-        start = 0
-        for js in range(self.ns):
-            sl = self.species_slice(js)
-            k = keep[sl]
-            idx = torch.nonzero(k).reshape(-1) + sl.start
-            parts.append(idx)
-            new_ins.append(start + 1)
-            new_nps.append(int(idx.numel()))
-            start += int(idx.numel())
-        idx = torch.cat(parts) if parts else torch.zeros(0, dtype=torch.long, device=self.device)
-        for c in self.COLUMNS:
-            setattr(self, c, getattr(self, c)[idx].contiguous())
-        self.ins, self.nps = new_ins, new_nps
+    def _columns(self, sl):
+        return [getattr(self, c)[sl] for c in self.COLUMNS]
+
+    def _lostwork(self, n):
+        from . import _native as N
+        need = int(N.lib().wm_clearpart_worksize(n))
+        w = getattr(self, "_lpwork", None)
+        if w is None or w.numel() < need:
+            w = self._lpwork = torch.empty(need, dtype=torch.uint8, device=self.device)
+            self._lpcount = torch.zeros(1, dtype=torch.int64, device=self.device)
+        return w
+
+    def _remove_lost(self, js, fillmethod, mesh):
+        """processlostpart -> clearpart for species js on the GPU; with
+        mesh = (xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, zgrid) the
+        absorbing tests of particleboundaries3d flag the lost particles
+        first.  Returns the number removed."""
+        import ctypes as C
+        from . import _native as N
+        sl = self.species_slice(js)
+        n = sl.stop - sl.start
+        if fillmethod not in (1, 2):
+            raise NotImplementedError("clearlostpart = %s: fill methods 1 and 2 only"
+                                      % fillmethod)
+        if n == 0:
+            return 0
+        if self.device.type != "cuda":
+            raise RuntimeError("particle boundaries run on the HIP device")
+        cols = self._columns(sl)
+        work = self._lostwork(n)
+        src = (C.c_void_p * len(cols))(*[t.data_ptr() for t in cols])
+        dst, dcols = None, None
+        if fillmethod == 2:
+            dcols = [torch.empty_like(t) for t in cols]
+            dst = (C.c_void_p * len(cols))(*[t.data_ptr() for t in dcols])
+        s = N.stream_of(self.device)
+        if mesh is None:
+            N.call("wm_clearpart", n, len(cols), src, fillmethod, dst,
+                   self._lpcount.data_ptr(), work.data_ptr(), s)
+        else:
+            N.call("wm_particleboundaries3d", n, len(cols), src, *mesh, fillmethod, dst,
+                   self._lpcount.data_ptr(), work.data_ptr(), s)
+        live = int(self._lpcount.item())
+        if fillmethod == 2:
+            for t, d in zip(cols, dcols):
+                t[:live].copy_(d[:live])
+        self.nps[js] = live
+        return n - live
+
+    def clearpart(self, js=-1, fillmethod=1):
+        """clearpart (top.F:4730-4893): removes the particles flagged lost
+        (gaminv <= 0) of species js (all when js < 0) with the given fill
+        method (1: holes filled from the end, 2: order kept)."""
+        for k in (range(self.ns) if js < 0 else [js]):
+            self._remove_lost(k, fillmethod, None)
 
 
 def particlesortxyzwithcopy(pgroup, dx, dy, dz, xmmin, ymmin, zmmin, nx, ny, nz):
@@ -100,13 +150,11 @@ def particlesortxyzwithcopy(pgroup, dx, dy, dz, xmmin, ymmin, zmmin, nx, ny, nz)
     s = N.stream_of(pgroup.device)
     new = {c: torch.empty_like(getattr(pgroup, c)) for c in pgroup.COLUMNS}
     k = len(pgroup.COLUMNS)
-    covered = 0
     for js in range(pgroup.ns):
         sl = pgroup.species_slice(js)
         n = sl.stop - sl.start
         if n == 0:
             continue
-        covered += n
         need = int(N.lib().wm_particle_sort_worksize(n, nx, ny, nz))
         work = torch.empty(need, dtype=torch.uint8, device=pgroup.device)
         perm = torch.empty(n, dtype=torch.int32, device=pgroup.device)
@@ -117,8 +165,7 @@ def particlesortxyzwithcopy(pgroup, dx, dy, dz, xmmin, ymmin, zmmin, nx, ny, nz)
         dst = (C.c_void_p * k)(*[new[c][sl].data_ptr() for c in pgroup.COLUMNS])
         N.call("wm_permute_columns", n, k, src, dst, perm.data_ptr(), s)
         del work, perm
-    if covered != pgroup.npmax:
-        raise RuntimeError("particle arrays hold particles outside every species")
+    # slots outside every species' live block hold nothing: left unset
     for c in pgroup.COLUMNS:
         setattr(pgroup, c, new[c])
 

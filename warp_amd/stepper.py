@@ -47,7 +47,7 @@ def generate():
     FS.loadrhoregistered()
     FS.fieldsolregistered()
     pg = top.pgroup
-    if pg is not None and pg.npmax > 0:
+    if pg is not None and pg.nplive > 0:
         padvnc3d("gen", pg, top.dt)
         top.moments = diagnostics.getzmmnt(pg)
     field_diagnostics(special=True)
@@ -70,17 +70,18 @@ def field_diagnostics(special):
 
 
 def particleboundaries3d(pg, solver, nlost=None):
-    """Absorbing particle boundaries (particleboundaries3d, w3d.F:11292-):
-    particles leaving the mesh are removed.  nlost, when known (counted by
-    the fused advance), skips the scan if no particle left."""
+    """Absorbing particle boundaries (particleboundaries3d,
+    w3d.F:11292-11450): particles outside the field mesh are flagged lost
+    and removed by processlostpart -> clearpart with top.clearlostpart's
+    fill method, all on the GPU (the host reads back the new count).
+    nlost, when known (counted by the fused advance with the same tests),
+    skips the pass if no particle left."""
     if nlost == 0:
         return
-    x, y, z = pg.xp, pg.yp, pg.zp
-    keep = ((x >= solver.xmmin) & (x < solver.xmmax) & (y >= solver.ymmin) &
-            (y < solver.ymmax) & (z >= solver.zmmin + top.zgrid) &
-            (z < solver.zmmax + top.zgrid))
-    if not bool(keep.all()):
-        pg.compact(keep)
+    mesh = (solver.xmmin, solver.ymmin, solver.zmmin, solver.dx, solver.dy, solver.dz,
+            solver.nx, solver.ny, solver.nz, top.zgrid)
+    for js in range(pg.ns):
+        pg._remove_lost(js, top.clearlostpart, mesh)
 
 
 def padvnc3d(center, pg, dt):
@@ -123,12 +124,38 @@ def padvnc3d_fullv(pg, dt):
     padvnc3d("fullv", pg, dt)
 
 
+_beforestep, _afterstep = [], []
+
+
+def installbeforestep(f):
+    """Calls f() before every step (controllers.installbeforestep,
+    scripts/controllers.py:543-552)."""
+    _beforestep.append(f)
+
+
+def uninstallbeforestep(f):
+    _beforestep.remove(f)
+
+
+def installafterstep(f):
+    """Calls f() after every step (controllers.installafterstep,
+    scripts/controllers.py:557-566)."""
+    _afterstep.append(f)
+
+
+def uninstallafterstep(f):
+    _afterstep.remove(f)
+
+
 def step(n=1, pgroup=None, boundaries=True):
-    """Advance n time steps (scripts/ctl.py:78-104 -> w3dexe -> step3d)."""
+    """Advance n time steps (scripts/ctl.py:78-104 -> w3dexe -> step3d),
+    with the before/after-step functions around each."""
     pg = pgroup if pgroup is not None else top.pgroup
     solvers = FS.getregisteredsolvers()
     for k in range(n):
         t0 = time.perf_counter()
+        for f in list(_beforestep):
+            f()
         top.it += 1
         padvnc3d("halfv" if top.lspecial else "fullv", pg, top.dt)
         top.time += top.dt
@@ -150,4 +177,6 @@ def step(n=1, pgroup=None, boundaries=True):
         field_diagnostics(top.lspecial)
         if top.lspecial and lhist:
             top.hist.append(dict(it=top.it, time=top.time, ese=top.ese, **top.moments))
+        for f in list(_afterstep):
+            f()
         top.steptime += time.perf_counter() - t0
