@@ -361,7 +361,9 @@ int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny,
  * (ex, ey, ez, ...) move along.  A particle is flagged lost (gaminv = 0) when
  * x <= xmmin or x >= xmmin+nx*dx, the same in y, or z-zgrid >= zmmin+nz*dz
  * or z-zgrid <= zmmin (xparticleboundaries top.F:3550-3560, its y twin,
- * zparticleboundaries top.F:3779-3795).  Then every particle with gaminv
+ * zparticleboundaries top.F:3779-3795).  With zbound = 2 (pbound0 = pboundnz
+ * = periodic) z wraps by whole periods instead (top.F:3762-3778) and only x
+ * and y absorb; zbound = 0 absorbs in z.  Then every particle with gaminv
  * <= 0 is removed with clearpart's fillmethod (top.v clearlostpart):
  *   1: in place, holes filled with live particles taken from the end (the
  *      reference's default; dstcols NULL);
@@ -372,7 +374,7 @@ int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny,
 int64_t wm_clearpart_worksize(int64_t np);
 int wm_particleboundaries3d(int64_t np, int64_t ncol, double *const *cols, double xmmin,
                             double ymmin, double zmmin, double dx, double dy, double dz,
-                            int64_t nx, int64_t ny, int64_t nz, double zgrid,
+                            int64_t nx, int64_t ny, int64_t nz, double zgrid, int64_t zbound,
                             int64_t fillmethod, double *const *dstcols, int64_t *nps_out,
                             void *work, void *stream);
 
@@ -477,6 +479,32 @@ int wm_unet_apply(const float *out, const uint8_t *mask, int64_t nx, int64_t ny,
  * *loss += mean, g (16-channel bf16) = its gradient. */
 int wm_unet_mse(const float *out, const float *tgt, int64_t n, uint16_t *g,
                 float *loss, void *stream);
+
+/* The learned initial guess at half resolution (no counterpart in the
+ * reference; it replaces the previous-phi guess of fieldsolver.py:1653-1658
+ * with the linear extrapolation in time plus the network's estimate of its
+ * error).  A coarse voxel is the 2x2x2 block of mesh nodes 2i..2i+1 of the
+ * nodes 0..n-1 (nx, ny, nz even); coarse arrays are fp32 (nz/2, ny/2, nx/2).
+ * wm_unet_half_mean: block means of an FP64 natural node array with guard
+ * (0 or 1) guard cells.  wm_unet_second_diff: d2 = r0 - 2 r1 + r2 and
+ * atomic maxima mx[0] = max|d2|, mx[1] = max|r0| (float bits, mx zeroed
+ * beforehand).  wm_unet_learn_input: the network input d2/mx[0], maskh,
+ * r0/mx[1].  wm_unet_learn_target: block means of p0 - 2 p1 + p2 (FP64
+ * phi arrays with guard cells) over cphys*mx[0].  wm_unet_learn_apply:
+ * phi += cphys*mx[0] * (trilinear value of out's channel 0) on nodes
+ * 1..n-1 where mask (one byte per node 0..n-1, may be NULL) is 0. */
+int wm_unet_half_mean(const double *src, int64_t nx, int64_t ny, int64_t nz,
+                      int64_t guard, float *dst, void *stream);
+int wm_unet_second_diff(const float *r0, const float *r1, const float *r2, int64_t n,
+                        float *d2, uint32_t *mx, void *stream);
+int wm_unet_learn_input(const float *d2, const float *r0, const float *maskh,
+                        const uint32_t *mx, int64_t n, uint16_t *x, void *stream);
+int wm_unet_learn_target(const double *p0, const double *p1, const double *p2,
+                         int64_t nx, int64_t ny, int64_t nz, const uint32_t *mx,
+                         double cphys, float *tgt, void *stream);
+int wm_unet_learn_apply(const float *out, const uint32_t *mx, double cphys, int64_t nx,
+                        int64_t ny, int64_t nz, const uint8_t *mask, double *phi,
+                        void *stream);
 
 /* ---------------------------------------------------------------------
  * Conductor set-up on the host (pywarp90/f3d_conductors.F)

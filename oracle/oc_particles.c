@@ -312,16 +312,27 @@ void oc_getzmmnt(int64_t np, const double *xp, const double *yp, const double *z
  * those getparticleextent and particleboundaries3d pass: xmax = xmmin+nx*dx,
  * zmax = zmmin+nz*dz (w3d.F:11344-11356). */
 void oc_particleboundaries3d(int64_t np, const double *xp, const double *yp,
-                             const double *zp, double *gaminv, double xmmin,
+                             double *zp, double *gaminv, double xmmin,
                              double ymmin, double zmmin, double dx, double dy,
                              double dz, int64_t nx, int64_t ny, int64_t nz,
-                             double zgrid) {
+                             double zgrid, int64_t zperiodic) {
   double xmax = xmmin + nx * dx, ymax = ymmin + ny * dy, zmax = zmmin + nz * dz;
   int64_t ip;
   for (ip = 0; ip < np; ip++)
     if (xp[ip] <= xmmin || xp[ip] >= xmax) gaminv[ip] = 0.;
   for (ip = 0; ip < np; ip++)
     if (yp[ip] <= ymmin || yp[ip] >= ymax) gaminv[ip] = 0.;
+  if (zperiodic) {
+    /* pbound0 = periodic, top.F:3762-3778 */
+    double syslen = zmax - zmmin;
+    for (ip = 0; ip < np; ip++) {
+      if (zp[ip] < zgrid + zmmin)
+        zp[ip] = zp[ip] + syslen * (int64_t)((zgrid + zmax - zp[ip]) / syslen);
+      if (zp[ip] >= zgrid + zmax)
+        zp[ip] = zp[ip] - syslen * (int64_t)((zp[ip] - zgrid - zmmin) / syslen);
+    }
+    return;
+  }
   for (ip = 0; ip < np; ip++) {
     if ((zp[ip] - zgrid) >= zmax)
       gaminv[ip] = 0.;

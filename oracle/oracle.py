@@ -111,7 +111,7 @@ def _declare(L):
                                    _i64, _i64, P, P, P]
     L.oc_getzmmnt.argtypes = [_i64, P, P, P, P, P, P, P, _i64, P]
     L.oc_particleboundaries3d.argtypes = [_i64, P, P, P, P, _d, _d, _d, _d, _d, _d, _i64,
-                                          _i64, _i64, _d]
+                                          _i64, _i64, _d, _i64]
     L.oc_clearpart.restype = _i64
     L.oc_clearpart.argtypes = [_i64, _i64, P, _i64, _i64, _i64]
     L.oc_getgrid3d.argtypes = [_i64, P, P, P, P, _i64, _i64, _i64, P, _d, _d, _d, _d, _d,
@@ -519,11 +519,12 @@ def getzmmnt(xp, yp, zp, uxp, uyp, uzp, gaminv, usegaminv=False):
 
 
 def particleboundaries3d(xp, yp, zp, gaminv, dx, dy, dz, nx, ny, nz, xmmin, ymmin, zmmin,
-                         zgrid=0.0):
+                         zgrid=0.0, zperiodic=False):
     """Flags particles outside the absorbing mesh boundaries with gaminv = 0
-    (particleboundaries3d, w3d.F:11292-11450)."""
+    (particleboundaries3d, w3d.F:11292-11450); with zperiodic z wraps by
+    whole periods instead of absorbing."""
     lib().oc_particleboundaries3d(len(xp), _p(xp), _p(yp), _p(zp), _p(gaminv), xmmin, ymmin,
-                                  zmmin, dx, dy, dz, nx, ny, nz, zgrid)
+                                  zmmin, dx, dy, dz, nx, ny, nz, zgrid, int(zperiodic))
 
 
 def clearpart(cols, fillmethod=1):

@@ -156,10 +156,10 @@ void oc_getzmmnt(int64_t np, const double *xp, const double *yp, const double *z
                  const double *uxp, const double *uyp, const double *uzp,
                  const double *gaminv, int64_t usegaminv, double *mm);
 void oc_particleboundaries3d(int64_t np, const double *xp, const double *yp,
-                             const double *zp, double *gaminv, double xmmin,
+                             double *zp, double *gaminv, double xmmin,
                              double ymmin, double zmmin, double dx, double dy,
                              double dz, int64_t nx, int64_t ny, int64_t nz,
-                             double zgrid);
+                             double zgrid, int64_t zperiodic);
 int64_t oc_clearpart(int64_t np, int64_t ncol, double **cols, int64_t iuz,
                      int64_t igam, int64_t fillmethod);
 void oc_getgrid3d(int64_t np, const double *x, const double *y, const double *z,

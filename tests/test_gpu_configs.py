@@ -53,10 +53,12 @@ def test_config_step_against_oracle(device, name):
     k = (x >= x0) & (x < x1) & (y >= y0) & (y < y1) & (z >= z0) & (z < z1)
     rho = np.zeros((nz + 1, ny + 1, nx + 1))
     O.setrho3d(rho, x[k], y[k], z[k], q, w, dx, dy, dz, x0, y0, z0)
+    bounds = configs.field_bounds(cfg)
+    O.applyrhoboundaryconditions3d(rho, bounds)
     co = O.install_conductors(configs.conductors(cfg["conductors"], O), nx, ny, nz, x0, x1,
                               y0, y1, z0, z1)
     phi_ref = phi0.copy()
-    it, err = O.multigrid3dsolve(phi_ref, rho, dx, dy, dz, conductors=co)
+    it, err = O.multigrid3dsolve(phi_ref, rho, dx, dy, dz, bounds=bounds, conductors=co)
     e_ref = O.getselfe3d(phi_ref, dx, dy, dz)
     report = {"fp64": (_rel_l2(phi_gpu, phi_ref),
                        _rel_l2(O.getselfe3d(phi_gpu, dx, dy, dz), e_ref), s.mgiters)}

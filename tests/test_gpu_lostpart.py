@@ -70,7 +70,7 @@ def test_particleboundaries_and_clearpart_match_oracle(device, n, frac, where, f
     out = torch.full((1,), -1, dtype=torch.int64, device=device)
     dst = [torch.full_like(t, np.nan) for t in g] if fillmethod == 2 else None
     N.call("wm_particleboundaries3d", n, len(g), _ptrs(g), LO[0], LO[1], LO[2], DX, DY, DZ,
-           NX, NY, NZ, 0.0, fillmethod, _ptrs(dst) if dst else None, out.data_ptr(),
+           NX, NY, NZ, 0.0, 0, fillmethod, _ptrs(dst) if dst else None, out.data_ptr(),
            work.data_ptr(), N.stream_of(device))
     live = int(out.item())
     O.particleboundaries3d(*cols[:3], cols[6], DX, DY, DZ, NX, NY, NZ, *LO)
@@ -83,6 +83,34 @@ def test_particleboundaries_and_clearpart_match_oracle(device, n, frac, where, f
         # the moved-from slots are cleared as clearpart clears them
         for k in (5, 6):
             np.testing.assert_array_equal(g[k].cpu().numpy(), cols[k], err_msg=NAMES[k])
+
+
+@pytest.mark.parametrize("zg", [0.0, 0.013])
+def test_periodic_z_wraps_and_absorbs_in_x_y(device, zg):
+    """zbound = 2 (pbound0 = pboundnz = periodic): z wraps by whole periods
+    (particles several periods out included), x and y still absorb; bitwise
+    equal to the oracle."""
+    from warp_amd import _native as N
+    n = 40000
+    cols = _particles(n, 21, 0.2)
+    rng = np.random.default_rng(22)
+    zlen = NZ * DZ
+    k = rng.random(n)
+    cols[2][k < 0.1] -= zlen * rng.integers(1, 4, int((k < 0.1).sum()))
+    cols[2][k > 0.9] += zlen * rng.integers(0, 3, int((k > 0.9).sum()))
+    cols[2] += zg
+    g = _gpu_cols(cols, device)
+    work = torch.empty(int(N.lib().wm_clearpart_worksize(n)), dtype=torch.uint8, device=device)
+    out = torch.zeros(1, dtype=torch.int64, device=device)
+    N.call("wm_particleboundaries3d", n, len(g), _ptrs(g), LO[0], LO[1], LO[2], DX, DY, DZ,
+           NX, NY, NZ, zg, 2, 1, None, out.data_ptr(), work.data_ptr(), N.stream_of(device))
+    O.particleboundaries3d(*cols[:3], cols[6], DX, DY, DZ, NX, NY, NZ, *LO, zg, zperiodic=True)
+    want = O.clearpart(cols, 1)
+    assert int(out.item()) == want
+    for name, a, b in zip(NAMES, g, cols):
+        np.testing.assert_array_equal(a[:want].cpu().numpy(), b[:want], err_msg=name)
+    z = cols[2][:want] - zg
+    assert z.min() >= LO[2] - 1e-12 and z.max() < LO[2] + zlen
 
 
 def test_clearpart_of_flagged_particles(device):
@@ -134,7 +162,12 @@ def test_step_removes_lost_particles_on_the_gpu(device):
     s.gather_push(pg, 0, 1.0e-8, "fullv")
     nlost = s.take_lost_count()
     after = [getattr(pg, c).cpu().numpy().copy() for c in NAMES]
-    stepper.particleboundaries3d(pg, s, nlost)
+    saved = (top.pbound0, top.pboundnz)
+    top.pbound0 = top.pboundnz = 0
+    try:
+        stepper.particleboundaries3d(pg, s, nlost)
+    finally:
+        top.pbound0, top.pboundnz = saved
     assert nlost > 1000
     assert pg.nps[0] == 50000 - nlost
     O.particleboundaries3d(*after[:3], after[6], s.dx, s.dy, s.dz, NX, NY, NZ, s.xmmin,

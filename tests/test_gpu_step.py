@@ -93,7 +93,8 @@ def test_step_driver_matches_oracle(device, efetch, lcorrectede):
     saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time, nhist=top.nhist,
                  lspecial=top.lspecial, particlesortinterval=top.particlesortinterval,
                  efetch=top.efetch, nzzarr=top.nzzarr, zzmin=top.zzmin, zzmax=top.zzmax,
-                 dzz=top.dzz, linechg=top.linechg, ese=top.ese)
+                 dzz=top.dzz, linechg=top.linechg, ese=top.ese, pbound0=top.pbound0,
+                 pboundnz=top.pboundnz)
     calls = []
     orig = s.gather_push
 
@@ -105,6 +106,7 @@ def test_step_driver_matches_oracle(device, efetch, lcorrectede):
     try:
         top.pgroup, top.dt, top.it, top.time, top.nhist = pg, 1.0e-9, 0, 0., 5
         top.particlesortinterval = 0
+        top.pbound0 = top.pboundnz = 0     # absorbing in z, as the oracle run
         top.efetch = efetch
         f3d.lcorrectede = lcorrectede
         warp_amd.registersolver(s)
@@ -172,10 +174,12 @@ def test_linear_guess_reaches_mgtol(device):
         for c in H.pipe_and_annulus(warp_amd):
             s.installconductor(c)
         saved = dict(pgroup=top.pgroup, dt=top.dt, it=top.it, time=top.time,
-                     particlesortinterval=top.particlesortinterval)
+                     particlesortinterval=top.particlesortinterval, pbound0=top.pbound0,
+                     pboundnz=top.pboundnz)
         try:
             top.pgroup, top.dt, top.it, top.time = pg, 1.0e-9, 0, 0.
             top.particlesortinterval = 0
+            top.pbound0 = top.pboundnz = 0
             warp_amd.registersolver(s)
             stepper.generate()
             stepper.step(8)

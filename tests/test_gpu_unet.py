@@ -217,38 +217,50 @@ def test_unet_training_overfits_one_sample(device):
     assert hist[-1] < 0.1 * hist[0], hist[::10]
 
 
-def test_warm_start_gain(device):
-    """Train on pairs the multigrid generates (32x32x64, pipe + annulus,
-    random beams moving 1 mm per step), then solve held-out steps from the
-    previous phi (plain multigrid) and from the chosen guess (the network's
-    unless its residual is larger: the fallback).  Every solve reaches
-    mgtol either way; the trained network fits its training targets better
-    than predicting no change; the V-cycles of both and the fallback rate
-    are reported."""
+def test_learned_guess_trains_and_runs(device):
+    """The learned-guess pipeline at 32x32x64 (pipe + annulus): samples from
+    two short randomised runs of the step driver, a short training through
+    the library's kernels, then the network's error of the extrapolation
+    residual on held-out samples below that of predicting zero, and a
+    step-driver run with guess="learned" whose every solve reaches mgtol."""
     import warp_amd
-    from tests import helpers as H
-    from warp_amd.unet_train import WarmStart, choose_guess, generate_pairs, vcycles_from
-    s = warp_amd.MultiGrid3D(nx=32, ny=32, nz=64, xmmin=-0.02, xmmax=0.02, ymmin=-0.02,
-                             ymmax=0.02, zmmin=0., zmmax=0.08, bounds=[0] * 6, mgverbose=-1,
-                             mgmaxiters=400)
-    for c in H.pipe_and_annulus(warp_amd):
-        s.installconductor(c)
-    train = generate_pairs(s, 32, seed=11)
-    test = generate_pairs(s, 6, seed=12)
-    ws = WarmStart(s, seed=1)
-    ws.fit_scales(train)
-    base = float(np.mean([float((ws._target(p0, p1) ** 2).mean()) for _, p0, p1 in train]))
-    hist = ws.train(train, 600, lr=3e-4)
-    assert np.mean(hist[-50:]) < 0.9 * base
-    plain, learned, fallbacks = [], [], 0
-    for rho, p0, p1 in test:
-        guess, fb = choose_guess(s, ws, rho, p0)
-        fallbacks += fb
-        it0, err0 = vcycles_from(s, rho, p0)
-        it1, err1 = vcycles_from(s, rho, guess)
-        assert err0 <= 1e-6 and err1 <= 1e-6
-        plain.append(it0)
-        learned.append(it1)
-    print("\nwarm start: loss %.4g -> %.4g (no-change %.4g); V-cycles from previous phi %s, "
-          "with the network %s; fallbacks %d of %d"
-          % (np.mean(hist[:50]), np.mean(hist[-50:]), base, plain, learned, fallbacks, len(test)))
+    from warp_amd import learned as LG, stepper, configs, unet_train as T
+    from warp_amd.globals import top
+    from warp_amd.unet import UNet3D
+    n = (32, 32, 64)
+    solvers = {v: T.make_solver(v, device, n=n) for v in T.VOLTAGES}
+    samples, _ = T.collect(device, 4, 10, 200000, 5, 10 ** 9, solvers)
+    hold, _ = T.collect(device, 1, 10, 200000, 6, 10 ** 9, solvers)
+    cphys = T.fit_cphys(samples)
+    for *_, t in hold:
+        t.div_(cphys)
+    lg = LG.LearnedGuess(solvers[0.], UNet3D(device, seed=3), cphys)
+    hist, ratio = T.train(lg, samples, 300, lr=3e-4, holdout=hold, log_every=100)
+    print("\nlearned guess at 32x32x64: %d samples, held-out mean-square error / "
+          "extrapolation's %.3f" % (len(samples), ratio))
+    assert len(samples) == 4 * 9 and ratio < 1.0
+    # the step driver with the learned guess
+    import os
+    import tempfile
+    path = os.path.join(tempfile.mkdtemp(), "w.npz")
+    LG.save_weights(path, lg.net, cphys)
+    s = T.make_solver(10.0e3, device, n=n, guess="learned", learned_weights=path)
+    pg = configs.load_beam(configs.CONFIGS["c1"], device, npart=200000)
+    saved = {a: getattr(top, a) for a in ("pgroup", "dt", "it", "time", "pbound0", "pboundnz",
+                                          "particlesortinterval")}
+    try:
+        top.pgroup, top.dt, top.it, top.time = pg, configs.DT, 0, 0.
+        top.pbound0 = top.pboundnz = 0
+        top.particlesortinterval = 0
+        warp_amd.registersolver(s)
+        stepper.generate()
+        stepper.step(8)
+        st = s.learned_guess_state()
+        assert s.mgerror <= s.mgtol
+        assert st.applied + st.fallbacks == 7  # every solve from the third on
+        print("learned guess in the step driver: applied %d, fell back %d, V-cycles %s"
+              % (st.applied, st.fallbacks, s.mgiters_history))
+    finally:
+        warp_amd.unregistersolver(s)
+        for a, v in saved.items():
+            setattr(top, a, v)

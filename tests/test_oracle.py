@@ -1012,3 +1012,22 @@ def test_getgrid3d_reproduces_linear_fields():
     O.getgrid3d(xa, -ya, za, f2, grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2],
                 l2symtry=True)
     np.testing.assert_array_equal(f1, f2)
+
+
+def test_periodic_z_boundary_wraps_by_periods():
+    """pbound0 = periodic (top.F:3762-3778): z below the mesh moves up and z
+    at or above it moves down by whole periods into [zmin+zg, zmax+zg); x
+    and y still absorb; gaminv is untouched by the wrap."""
+    nx, ny, nz, dx, dy, dz = 4, 4, 8, 0.25, 0.25, 0.5
+    x0, y0, z0, zg = -0.5, -0.5, 0.0, 0.3
+    L = nz * dz
+    z = np.array([z0 + zg - 0.1, z0 + zg + L, z0 + zg + 2.5 * L, z0 + zg - 3.2 * L, 1.0, 1.0])
+    x = np.array([0., 0., 0., 0., 0.5, 0.])
+    y = np.zeros(6)
+    g = np.ones(6)
+    O.particleboundaries3d(x, y, z, g, dx, dy, dz, nx, ny, nz, x0, y0, z0, zg, zperiodic=True)
+    assert list(g) == [1., 1., 1., 1., 0., 1.]
+    # (a particle exactly one period up lands on zmin+zg up to the rounding
+    # of z - L, as in the reference)
+    assert ((z - zg >= z0 - 1e-12) & (z - zg < z0 + L)).all()
+    np.testing.assert_allclose(z[:4] - zg, [L - 0.1, 0.0, 0.5 * L, 0.8 * L], rtol=0, atol=1e-12)

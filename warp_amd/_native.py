@@ -110,6 +110,11 @@ SIGNATURES = {
     "wm_unet_input": (C.c_int, [_P, _P, _P, _i64, _i64, _i64, _d, _d, _P, _P]),
     "wm_unet_apply": (C.c_int, [_P, _P, _i64, _i64, _i64, _d, _P, _P]),
     "wm_unet_mse": (C.c_int, [_P, _P, _i64, _P, _P, _P]),
+    "wm_unet_half_mean": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _P]),
+    "wm_unet_second_diff": (C.c_int, [_P, _P, _P, _i64, _P, _P, _P]),
+    "wm_unet_learn_input": (C.c_int, [_P, _P, _P, _P, _i64, _P, _P]),
+    "wm_unet_learn_target": (C.c_int, [_P, _P, _P, _i64, _i64, _i64, _P, _d, _P, _P]),
+    "wm_unet_learn_apply": (C.c_int, [_P, _P, _d, _i64, _i64, _i64, _P, _P, _P]),
     "wm_particle_sort_worksize": (_i64, [_i64, _i64, _i64, _i64]),
     "wm_particle_sort": (C.c_int, [_i64, _P, _P, _P, _d, _d, _d, _d, _d, _d, _i64, _i64, _i64,
                                    _P, _P, _P]),
@@ -119,7 +124,7 @@ SIGNATURES = {
                                _d, _i64, _i64, _P]),
     "wm_clearpart_worksize": (_i64, [_i64]),
     "wm_particleboundaries3d": (C.c_int, [_i64, _i64, _P, _d, _d, _d, _d, _d, _d, _i64, _i64,
-                                          _i64, _d, _i64, _P, _P, _P, _P]),
+                                          _i64, _d, _i64, _i64, _P, _P, _P, _P]),
     "wm_clearpart": (C.c_int, [_i64, _i64, _P, _i64, _P, _P, _P, _P]),
     "wm_zcylinder_intercepts": (C.c_int, [_d, _d, _d, _d, _d, _P, _d, _i64]),
     "wm_zsrfrv_intercepts": (C.c_int, [_i64, _P, _P, _d, _d, _d, _P]),

@@ -14,7 +14,10 @@ Choices the configuration text leaves open, fixed here:
     [0.5, 6.5] cm (its head reaches z = 8 cm after 15 steps of 1 mm) and
     carries 1 mA;
   * the quadrupole channel's rods are 1.5 cm long, one focusing and one
-    defocusing rod set per 4 cm period.
+    defocusing rod set per 4 cm period; the 8 cm mesh holds two periods and
+    is periodic in z, for the field and the particles (the reference's
+    default particle boundary in z), so a long run circulates the beam
+    through the channel; the other problems absorb particles on every face.
 """
 import math
 
@@ -35,8 +38,14 @@ CONFIGS = {
     "c2": dict(index=2, n=(256, 256, 512), npart=32_000_000, beam="kv", current=1e-3,
                conductors="pipe_annulus"),
     "c4": dict(index=4, n=(256, 256, 512), npart=32_000_000, beam="gaussian", current=5e-3,
-               conductors="quads"),
+               conductors="quads", zperiodic=True),
 }
+
+
+def field_bounds(cfg):
+    """Field boundaries (xm, xp, ym, yp, zm, zp): phi = 0 on every face,
+    periodic in z for the quadrupole channel."""
+    return [0, 0, 0, 0, 2, 2] if cfg.get("zperiodic") else [0] * 6
 
 
 def conductors(kind, mod):
@@ -63,6 +72,15 @@ def conductors(kind, mod):
     raise ValueError(kind)
 
 
+def set_particle_boundaries(cfg):
+    """Particle boundaries of a configuration: absorbing on every face,
+    except for the quadrupole channel, whose 8 cm mesh holds two lattice
+    periods and is periodic in z (fields and particles)."""
+    from .globals import top
+    top.pboundxy = 0
+    top.pbound0 = top.pboundnz = 2 if cfg.get("zperiodic") else 0
+
+
 def load_beam(cfg, device="cuda", npart=None):
     """Seeded particle load of a configuration (returns ParticleGroup)."""
     from .particles import load_gaussian, load_kv_ellipse
@@ -86,7 +104,8 @@ def make_solver(cfg, precision="fp64", device="cuda", **kw):
     nx, ny, nz = cfg["n"]
     x0, x1, y0, y1, z0, z1 = DOMAIN
     s = MultiGrid3D(nx=nx, ny=ny, nz=nz, xmmin=x0, xmmax=x1, ymmin=y0, ymmax=y1, zmmin=z0,
-                    zmmax=z1, bounds=[0] * 6, precision=precision, device=device, **kw)
+                    zmmax=z1, bounds=field_bounds(cfg), precision=precision, device=device,
+                    **kw)
     for c in conductors(cfg["conductors"], warp_amd):
         s.installconductor(c)
     return s

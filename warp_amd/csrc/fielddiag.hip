@@ -11,7 +11,38 @@
 // rho has no guard cells; phi has one (nxguardphi = 1).
 #include "wm_internal.h"
 
+#include <mutex>
+
+// Device scratch of wm_getese3d_noalloc: the column sums of rho*phi, one
+// double per (ix, iy) column, at most ESE_SCRATCH_MAX of them (1 MiB, e.g.
+// (nx+1)(ny+1) up to 361 x 361).  Every GPU the library runs on holds its
+// own copy for the life of the process.  The host never passes this name to
+// a kernel: it passes the address hipGetSymbolAddress returned for the
+// current device (ese_scratch_ptr).
+constexpr int64_t ESE_SCRATCH_MAX = 1 << 17;
+static __device__ __align__(256) double scratch[ESE_SCRATCH_MAX];
+
 namespace {
+
+constexpr int ESE_MAX_DEVICES = 64;
+
+// The scratch's address on the current device, looked up on the first call
+// made with that device current and kept.
+double *ese_scratch_ptr() {
+  static std::mutex mu;
+  static double *ptr[ESE_MAX_DEVICES] = {};
+  int dev = -1;
+  WM_HIP(hipGetDevice(&dev));
+  WM_REQUIRE(dev >= 0 && dev < ESE_MAX_DEVICES, "wm_getese3d_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    WM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    WM_REQUIRE(p != nullptr, "wm_getese3d_noalloc: no device address for the scratch");
+    ptr[dev] = (double *)p;
+  }
+  return ptr[dev];
+}
 
 constexpr int PS_CHUNK = 2048;             // doubles staged per step (16 KB)
 constexpr int PS_PER_LANE = PS_CHUNK / 64;  // registers per lane per chunk
@@ -250,30 +281,51 @@ int64_t wm_getese3d_worksize(int64_t nx, int64_t ny) {
   return (int64_t)sizeof(double) * (nx + 1) * (ny + 1);
 }
 
+static void ese_launch(const double *rho, const double *phi, int64_t nx, int64_t ny,
+                     int64_t nz, double dx, double dy, double dz, int64_t l4symtry,
+                     int64_t l2symtry, double *scr, double *ese, hipStream_t s) {
+  const int64_t ncol = (nx + 1) * (ny + 1);
+  dim3 g((unsigned)cdiv(ncol, 256)), b(256);
+  // l4symtry is tested before l2symtry, as dw3d.F:740-757
+  if (l4symtry)
+    hipLaunchKernelGGL(k_ese_columns<SYM_L4>, g, b, 0, s, rho, phi, nx, ny, nz, scr);
+  else if (l2symtry)
+    hipLaunchKernelGGL(k_ese_columns<SYM_L2>, g, b, 0, s, rho, phi, nx, ny, nz, scr);
+  else
+    hipLaunchKernelGGL(k_ese_columns<SYM_NONE>, g, b, 0, s, rho, phi, nx, ny, nz, scr);
+  WM_LAUNCHED();
+  hipLaunchKernelGGL(k_ese_rows, dim3((unsigned)cdiv(nx + 1, 256)), dim3(256), 0, s, scr, nx,
+                     ny);
+  WM_LAUNCHED();
+  hipLaunchKernelGGL(k_ese_final, dim3(1), dim3(64), 0, s, (const double *)scr, nx, dx, dy, dz,
+                     ese);
+  WM_LAUNCHED();
+}
+
+int wm_getese3d_noalloc(const double *rho, const double *phi, int64_t nx, int64_t ny, int64_t nz,
+                        double dx, double dy, double dz, int64_t l4symtry, int64_t l2symtry,
+                        double *ese, void *stream) {
+  WM_ENTRY({
+    check_mesh("wm_getese3d_noalloc", nx, ny, nz);
+    WM_REQUIRE(rho && phi && ese, "wm_getese3d_noalloc: NULL array");
+    WM_REQUIRE((nx + 1) * (ny + 1) <= ESE_SCRATCH_MAX,
+               "wm_getese3d_noalloc: (nx+1)*(ny+1) = " + std::to_string((nx + 1) * (ny + 1)) +
+                   " columns exceed the scratch's " + std::to_string(ESE_SCRATCH_MAX) +
+                   "; use wm_getese3d with a work buffer");
+    // every column sum is written before it is read: nothing to clear
+    ese_launch(rho, phi, nx, ny, nz, dx, dy, dz, l4symtry, l2symtry, ese_scratch_ptr(), ese,
+               wm_stream(stream));
+  });
+}
+
 int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny, int64_t nz,
                 double dx, double dy, double dz, int64_t l4symtry, int64_t l2symtry, void *work,
                 double *ese, void *stream) {
   WM_ENTRY({
     check_mesh("wm_getese3d", nx, ny, nz);
     WM_REQUIRE(rho && phi && work && ese, "wm_getese3d: NULL array");
-    hipStream_t s = wm_stream(stream);
-    double *scr = (double *)work;
-    const int64_t ncol = (nx + 1) * (ny + 1);
-    dim3 g((unsigned)cdiv(ncol, 256)), b(256);
-    // l4symtry is tested before l2symtry, as dw3d.F:740-757
-    if (l4symtry)
-      hipLaunchKernelGGL(k_ese_columns<SYM_L4>, g, b, 0, s, rho, phi, nx, ny, nz, scr);
-    else if (l2symtry)
-      hipLaunchKernelGGL(k_ese_columns<SYM_L2>, g, b, 0, s, rho, phi, nx, ny, nz, scr);
-    else
-      hipLaunchKernelGGL(k_ese_columns<SYM_NONE>, g, b, 0, s, rho, phi, nx, ny, nz, scr);
-    WM_LAUNCHED();
-    hipLaunchKernelGGL(k_ese_rows, dim3((unsigned)cdiv(nx + 1, 256)), dim3(256), 0, s, scr, nx,
-                       ny);
-    WM_LAUNCHED();
-    hipLaunchKernelGGL(k_ese_final, dim3(1), dim3(64), 0, s, (const double *)scr, nx, dx, dy, dz,
-                       ese);
-    WM_LAUNCHED();
+    ese_launch(rho, phi, nx, ny, nz, dx, dy, dz, l4symtry, l2symtry, (double *)work, ese,
+               wm_stream(stream));
   });
 }
 

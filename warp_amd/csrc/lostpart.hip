@@ -37,6 +37,7 @@ struct LpCols {
 
 struct LpBounds {
   double xmin, xmax, ymin, ymax, zmin, zmax, zgrid;
+  int zperiodic;  // pbound0 = pboundnz = periodic: z wraps instead of absorbing
 };
 
 // Work layout (int32 words unless noted): tile counts, tile offsets, the
@@ -74,8 +75,17 @@ __device__ inline bool lp_outside(const LpBounds &b, double x, double y, double 
   bool lost = x <= b.xmin || x >= b.xmax;
   lost = lost || y <= b.ymin || y >= b.ymax;
   double zz = z - b.zgrid;
-  lost = lost || zz >= b.zmax || zz <= b.zmin;
+  if (!b.zperiodic) lost = lost || zz >= b.zmax || zz <= b.zmin;
   return lost;
+}
+
+// zparticleboundaries with pbound0 = periodic (top.F:3762-3778): a particle
+// below the mesh is moved up, one at or above it down, by whole periods.
+__device__ inline double lp_zwrap(const LpBounds &b, double z) {
+  const double syslen = b.zmax - b.zmin;
+  if (z < b.zgrid + b.zmin) z = z + syslen * (double)(long long)((b.zgrid + b.zmax - z) / syslen);
+  if (z >= b.zgrid + b.zmax) z = z - syslen * (double)(long long)((z - b.zgrid - b.zmin) / syslen);
+  return z;
 }
 
 // Exclusive rank of this lane's flag among the block's 256 flags and the
@@ -102,7 +112,7 @@ __device__ inline int lp_block_rank(bool f, int *wsum, int &total) {
 template <bool FLAG>
 __global__ __launch_bounds__(LP_THREADS) void k_lost_tiles(long long np, const double *__restrict__ x,
                                                            const double *__restrict__ y,
-                                                           const double *__restrict__ z,
+                                                           double *__restrict__ z,
                                                            double *__restrict__ gaminv, LpBounds bd,
                                                            int *__restrict__ tcount) {
   __shared__ int wsum[LP_THREADS / 64];
@@ -113,9 +123,17 @@ __global__ __launch_bounds__(LP_THREADS) void k_lost_tiles(long long np, const d
     long long i = base + k * LP_THREADS + threadIdx.x;
     if (i < np) {
       double g = gaminv[i];
-      if (FLAG && lp_outside(bd, x[i], y[i], z[i])) {
-        g = 0.;
-        gaminv[i] = 0.;
+      if (FLAG) {
+        double zi = z[i];
+        if (bd.zperiodic) {
+          double zw = lp_zwrap(bd, zi);
+          if (zw != zi) z[i] = zw;
+          zi = zw;
+        }
+        if (lp_outside(bd, x[i], y[i], zi)) {
+          g = 0.;
+          gaminv[i] = 0.;
+        }
       }
       cnt += g <= 0. ? 1 : 0;
     }
@@ -309,14 +327,17 @@ int64_t wm_clearpart_worksize(int64_t np) { return (int64_t)lp_worksize(std::max
 
 int wm_particleboundaries3d(int64_t np, int64_t ncol, double *const *cols, double xmmin,
                             double ymmin, double zmmin, double dx, double dy, double dz,
-                            int64_t nx, int64_t ny, int64_t nz, double zgrid,
+                            int64_t nx, int64_t ny, int64_t nz, double zgrid, int64_t zbound,
                             int64_t fillmethod, double *const *dstcols, int64_t *nps_out,
                             void *work, void *stream) {
   WM_ENTRY({
     LpCols c = lp_cols(ncol, cols, "wm_particleboundaries3d");
+    WM_REQUIRE(zbound == 0 || zbound == 2,
+               "wm_particleboundaries3d: zbound must be 0 (absorb) or 2 (periodic)");
     // getparticleextent and the calls of particleboundaries3d
     // (w3d.F:11344-11356): the upper extents are xmmin + nx*dx, ...
-    LpBounds bd{xmmin, xmmin + nx * dx, ymmin, ymmin + ny * dy, zmmin, zmmin + nz * dz, zgrid};
+    LpBounds bd{xmmin, xmmin + nx * dx, ymmin,           ymmin + ny * dy,
+                zmmin, zmmin + nz * dz, zgrid, zbound == 2 ? 1 : 0};
     lp_run(np, c, true, bd, fillmethod, dstcols, nps_out, work, wm_stream(stream),
            "wm_particleboundaries3d");
   });

@@ -530,6 +530,140 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ out,
   if (threadIdx.x == 0) atomicAdd(loss, part[0]);
 }
 
+// ---------------------------------------------------------------------------
+// The learned initial guess works at half resolution: a coarse voxel is the
+// 2x2x2 block of mesh nodes (2i..2i+1, 2j..2j+1, 2k..2k+1) of the nodes
+// 0..n-1 (n even in every axis), so the (nz/2, ny/2, nx/2) voxel grid is
+// divisible by 8 for the network's four levels whenever n is by 16.
+
+// Mean of each 2x2x2 node block of a natural-layout FP64 node array with
+// `guard` guard cells (rho: 0, phi: 1), summed in a fixed order.
+__global__ __launch_bounds__(256) void k_half_mean(const double *__restrict__ src, int nx, int ny,
+                                                   int nz, int guard, float *__restrict__ dst) {
+  const int W = nx / 2, H = ny / 2, D = nz / 2;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)W * H * D) return;
+  int i = (int)(t % W);
+  long long r = t / W;
+  int j = (int)(r % H), k = (int)(r / H);
+  const long long sx = nx + 1 + 2 * guard, sxy = sx * (ny + 1 + 2 * guard);
+  const double *p = src + (2 * i + guard) + sx * (2 * j + guard) + sxy * (2 * k + guard);
+  double a = ((p[0] + p[1]) + (p[sx] + p[sx + 1])) + ((p[sxy] + p[sxy + 1]) +
+                                                      (p[sxy + sx] + p[sxy + sx + 1]));
+  dst[t] = (float)(0.125 * a);
+}
+
+// Largest |v| of a block's values into *mx by an integer atomicMax on the
+// bit pattern (non-negative floats order as their bits); *mx starts at 0.
+__device__ inline void block_absmax(float v, unsigned *mx) {
+  __shared__ float part[4];
+  v = fabsf(v);
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+    atomicMax(mx, __float_as_uint(m));
+  }
+}
+
+// d2 = r0 - 2 r1 + r2 (the second time difference of the coarse source) and
+// the maxima of |d2| (mx[0]) and |r0| (mx[1]).
+__global__ __launch_bounds__(256) void k_second_diff(const float *__restrict__ r0,
+                                                     const float *__restrict__ r1,
+                                                     const float *__restrict__ r2, long long n,
+                                                     float *__restrict__ d2,
+                                                     unsigned *__restrict__ mx) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  float a = 0.f, b = 0.f;
+  if (t < n) {
+    b = r0[t];
+    a = (b - 2.f * r1[t]) + r2[t];
+    d2[t] = a;
+  }
+  block_absmax(a, mx);
+  __syncthreads();
+  block_absmax(b, mx + 1);
+}
+
+// Network input (bf16, 16 channels): d2 / max|d2|, the coarse conductor
+// fraction, r0 / max|r0|; the other channels zero.
+__global__ __launch_bounds__(256) void k_learn_input(const float *__restrict__ d2,
+                                                     const float *__restrict__ r0,
+                                                     const float *__restrict__ maskh,
+                                                     const unsigned *__restrict__ mx,
+                                                     long long n, uint16_t *__restrict__ x) {
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  float m0 = __uint_as_float(mx[0]), m1 = __uint_as_float(mx[1]);
+  uint16_t *o = x + t * 16;
+  o[0] = f2bf(m0 > 0.f ? d2[t] / m0 : 0.f);
+  o[1] = f2bf(maskh ? maskh[t] : 0.f);
+  o[2] = f2bf(m1 > 0.f ? r0[t] / m1 : 0.f);
+  for (int k = 3; k < 16; k++) o[k] = 0;
+}
+
+// Training target: the coarse mean of (phi_n - 2 phi_(n-1) + phi_(n-2)),
+// the error of the linear extrapolation, over cphys * max|d2|.
+__global__ __launch_bounds__(256) void k_learn_target(const double *__restrict__ p0,
+                                                      const double *__restrict__ p1,
+                                                      const double *__restrict__ p2, int nx,
+                                                      int ny, int nz, const unsigned *__restrict__ mx,
+                                                      double cphys, float *__restrict__ tgt) {
+  const int W = nx / 2, H = ny / 2, D = nz / 2;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)W * H * D) return;
+  int i = (int)(t % W);
+  long long r = t / W;
+  int j = (int)(r % H), k = (int)(r / H);
+  const long long sx = nx + 3, sxy = sx * (ny + 3);
+  double a = 0.;
+  for (int c = 0; c < 8; c++) {
+    long long q = (2 * i + (c & 1) + 1) + sx * (2 * j + ((c >> 1) & 1) + 1) +
+                  sxy * (2 * k + (c >> 2) + 1);
+    a += (p0[q] - 2. * p1[q]) + p2[q];
+  }
+  double m = (double)__uint_as_float(mx[0]) * cphys;
+  tgt[t] = m > 0. ? (float)(0.125 * a / m) : 0.f;
+}
+
+// phi(ix,iy,iz) += cphys * max|d2| * (trilinear value of the network's
+// channel 0 at the node) on the nodes 1..n-1 that are not pinned (mask 0,
+// nodes 0..n-1); coarse voxel centres sit at fine coordinate 2i + 1/2.
+__global__ __launch_bounds__(256) void k_learn_apply(const float *__restrict__ out,
+                                                     const unsigned *__restrict__ mx,
+                                                     double cphys, int nx, int ny, int nz,
+                                                     const uint8_t *__restrict__ mask,
+                                                     double *__restrict__ phi) {
+  const int W = nx / 2, H = ny / 2, D = nz / 2;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)(nx - 1) * (ny - 1) * (nz - 1)) return;
+  int ix = 1 + (int)(t % (nx - 1));
+  long long r = t / (nx - 1);
+  int iy = 1 + (int)(r % (ny - 1)), iz = 1 + (int)(r / (ny - 1));
+  if (mask && mask[ix + (long long)nx * (iy + (long long)ny * iz)]) return;
+  auto coord = [](int a, int n, int &i0, float &w) {
+    float u = 0.5f * ((float)a - 0.5f);
+    u = fminf(fmaxf(u, 0.f), (float)(n - 1));
+    i0 = min((int)u, n - 2);
+    w = u - (float)i0;
+  };
+  int i0, j0, k0;
+  float wx, wy, wz;
+  coord(ix, W, i0, wx);
+  coord(iy, H, j0, wy);
+  coord(iz, D, k0, wz);
+  auto v = [&](int a, int b, int c) {
+    return out[(((long long)(k0 + c) * H + (j0 + b)) * W + (i0 + a)) * 16];
+  };
+  float f = (1.f - wz) * ((1.f - wy) * ((1.f - wx) * v(0, 0, 0) + wx * v(1, 0, 0)) +
+                          wy * ((1.f - wx) * v(0, 1, 0) + wx * v(1, 1, 0))) +
+            wz * ((1.f - wy) * ((1.f - wx) * v(0, 0, 1) + wx * v(1, 0, 1)) +
+                  wy * ((1.f - wx) * v(0, 1, 1) + wx * v(1, 1, 1)));
+  long long p = (ix + 1) + (long long)(nx + 3) * ((iy + 1) + (long long)(ny + 3) * (iz + 1));
+  phi[p] = phi[p] + (double)f * ((double)__uint_as_float(mx[0]) * cphys);
+}
+
 inline int nb(long long n) { return (int)std::max<long long>(1, (n + 255) / 256); }
 
 }  // namespace
@@ -708,6 +842,63 @@ int wm_unet_apply(const float *out, const uint8_t *mask, int64_t nx, int64_t ny,
   WM_ENTRY({
     hipLaunchKernelGGL(k_unet_apply, dim3(nb(nx * ny * nz)), dim3(256), 0, wm_stream(stream), out,
                        mask, (int)nx, (int)ny, (int)nz, sphi, phi);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_half_mean(const double *src, int64_t nx, int64_t ny, int64_t nz, int64_t guard,
+                      float *dst, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(src && dst, "wm_unet_half_mean: NULL array");
+    WM_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0,
+               "wm_unet_half_mean: nx, ny, nz must be even");
+    WM_REQUIRE(guard == 0 || guard == 1, "wm_unet_half_mean: guard must be 0 or 1");
+    hipLaunchKernelGGL(k_half_mean, dim3(nb(nx / 2 * (ny / 2) * (nz / 2))), dim3(256), 0,
+                       wm_stream(stream), src, (int)nx, (int)ny, (int)nz, (int)guard, dst);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_second_diff(const float *r0, const float *r1, const float *r2, int64_t n, float *d2,
+                        uint32_t *mx, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(r0 && r1 && r2 && d2 && mx, "wm_unet_second_diff: NULL array");
+    hipLaunchKernelGGL(k_second_diff, dim3(nb(n)), dim3(256), 0, wm_stream(stream), r0, r1, r2,
+                       (long long)n, d2, mx);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_learn_input(const float *d2, const float *r0, const float *maskh, const uint32_t *mx,
+                        int64_t n, uint16_t *x, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(d2 && r0 && mx && x, "wm_unet_learn_input: NULL array");
+    hipLaunchKernelGGL(k_learn_input, dim3(nb(n)), dim3(256), 0, wm_stream(stream), d2, r0,
+                       maskh, mx, (long long)n, x);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_learn_target(const double *p0, const double *p1, const double *p2, int64_t nx,
+                         int64_t ny, int64_t nz, const uint32_t *mx, double cphys, float *tgt,
+                         void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(p0 && p1 && p2 && mx && tgt, "wm_unet_learn_target: NULL array");
+    WM_REQUIRE(nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, "wm_unet_learn_target: odd mesh");
+    hipLaunchKernelGGL(k_learn_target, dim3(nb(nx / 2 * (ny / 2) * (nz / 2))), dim3(256), 0,
+                       wm_stream(stream), p0, p1, p2, (int)nx, (int)ny, (int)nz, mx, cphys, tgt);
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_learn_apply(const float *out, const uint32_t *mx, double cphys, int64_t nx,
+                        int64_t ny, int64_t nz, const uint8_t *mask, double *phi, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(out && mx && phi, "wm_unet_learn_apply: NULL array");
+    WM_REQUIRE(nx >= 4 && ny >= 4 && nz >= 4 && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0,
+               "wm_unet_learn_apply: mesh must be even and >= 4");
+    hipLaunchKernelGGL(k_learn_apply, dim3(nb((nx - 1) * (ny - 1) * (nz - 1))), dim3(256), 0,
+                       wm_stream(stream), out, mx, cphys, (int)nx, (int)ny, (int)nz, mask, phi);
     WM_LAUNCHED();
   });
 }

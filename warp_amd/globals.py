@@ -48,6 +48,9 @@ top = Top(dt=0., time=0., it=0, fstype=-1, pgroup=None, zbeam=0., zgrid=0.,
           # clearpart's fill method for lost particles (top.v:1260): 1 fills
           # holes with particles from the end, 2 keeps the order, 0 leaves them
           clearlostpart=1,
+          # particle boundaries (top.v:1187-1195): at iz = 0, at iz = nz, in x
+          # and y; absorb = 0, reflect = 1, periodic = 2 (top.v:75-78)
+          pbound0=2, pboundnz=2, pboundxy=0,
           # step controls of w3dexe/step3d (top.v): "special" steps end with
           # the synchv half kick and the beam moments
           nt=20000000, tstop=1.0e36, zstop=1.0e36, itmomnts=[0] * 50, nhist=5, itdump=500,

@@ -35,7 +35,7 @@ class MultiGrid3D(FieldSolver):
                      'icndbndy', 'laddconductor', 'lprecalccoeffs']
 
     def __init__(self, precision="fp64", device=None, sorted_deposit=False, guess="previous",
-                 **kw):
+                 learned_weights=None, **kw):
         FieldSolver.__init__(self, kwdict=kw)
         self.processdefaultsfrompackage(MultiGrid3D.__f3dinputs__, f3d, kw)
         assert len(kw.keys()) == 0, "Bad keyword arguemnts %s" % list(kw.keys())
@@ -45,12 +45,16 @@ class MultiGrid3D(FieldSolver):
             raise ValueError("precision must be one of %s" % list(PRECISIONS))
         self.precision = precision
         self.sorted_deposit = sorted_deposit
-        if guess not in ("previous", "linear"):
-            raise ValueError("guess must be 'previous' (the reference's) or 'linear'")
-        # initial guess of each solve: the previous phi (the reference), or
-        # its linear extrapolation from the two previous solves
+        if guess not in ("previous", "linear", "learned"):
+            raise ValueError("guess must be 'previous' (the reference's), 'linear' or 'learned'")
+        # initial guess of each solve: the previous phi (the reference), its
+        # linear extrapolation from the two previous solves, or that plus the
+        # learned correction (warp_amd.learned; falls back to the
+        # extrapolation when the network's residual is not smaller)
         self.guess = guess
         self._phiprev = None
+        self._learned = None
+        self._learned_weights = learned_weights
         if not torch.cuda.is_available():
             raise RuntimeError("MultiGrid3D needs a HIP device (torch.cuda is not available)")
         if device is None:
@@ -348,6 +352,8 @@ class MultiGrid3D(FieldSolver):
             self.getconductorobject()
         if self.guess == "linear":
             self._extrapolate_guess()
+        elif self.guess == "learned":
+            self._learned_guess()
         it = C.c_int64(0)
         err = C.c_double(0.)
         # phi untouched since the last solve returned it (torch's version
@@ -395,18 +401,51 @@ class MultiGrid3D(FieldSolver):
         except RuntimeError:
             return None
 
+    def learned_guess_state(self):
+        """The LearnedGuess of guess="learned" (loaded on first use): its
+        network, and the counts of solves that used the network's guess
+        (applied) and that fell back to the extrapolation (fallbacks)."""
+        if self._learned is None:
+            from . import learned as LG
+            path = self._learned_weights or LG.default_weights()
+            net, cphys, meta = LG.load_weights(path, self.device)
+            self._learned = LG.LearnedGuess(self, net, cphys)
+            self._learned.meta = meta
+        return self._learned
+
+    def _learned_guess(self):
+        """Linear extrapolation plus the network's correction of it, kept
+        only when it lowers the largest residual (else the extrapolation
+        alone: the plain method is the fallback)."""
+        from .learned import residual_max
+        lg = self.learned_guess_state()
+        lg.observe(self._rho)
+        extrapolated = self._extrapolate_guess()
+        if not (extrapolated and lg.ready()):
+            return
+        base = self._phi.clone()
+        r_lin = residual_max(self, base)
+        lg.correct(self._phi)
+        r_net = residual_max(self, self._phi)
+        if r_net < r_lin:
+            lg.applied += 1
+        else:
+            self._phi.copy_(base)
+            lg.fallbacks += 1
+
     def _extrapolate_guess(self):
         """phi <- 2 phi_(n-1) - phi_(n-2) before a solve (the previous phi
         alone while fewer than two solves have run); keeps phi_(n-1) for the
         next one."""
         if not self.mgiters_history:
-            return  # phi is not the result of a solve yet
+            return False  # phi is not the result of a solve yet
         if self._phiprev is None:
             self._phiprev = self._phi.clone()
-            return
+            return False
         N.call("wm_phi_extrapolate", self._phi.data_ptr(), self._phiprev.data_ptr(),
                self._phi.numel(), N.stream_of(self.device))
         self._phi_token = None  # phi changed outside torch's view
+        return True
 
     # ------------------------------------------------------------------
     # field diagnostics of special steps (multigrid.py:433-467)

@@ -93,9 +93,9 @@ class ParticleGroup:
 
     def _remove_lost(self, js, fillmethod, mesh):
         """processlostpart -> clearpart for species js on the GPU; with
-        mesh = (xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, zgrid) the
-        absorbing tests of particleboundaries3d flag the lost particles
-        first.  Returns the number removed."""
+        mesh = (xmmin, ymmin, zmmin, dx, dy, dz, nx, ny, nz, zgrid, zbound)
+        the tests of particleboundaries3d flag the lost particles first
+        (zbound 0: z absorbs, 2: z periodic).  Returns the number removed."""
         import ctypes as C
         from . import _native as N
         sl = self.species_slice(js)
@@ -228,3 +228,61 @@ def load_kv_ellipse(npart, a, b, zmin, zmax, vz, vspread, bounds, seed, charge=e
     js = pg.addspecies(charge, mass, uniform_weight(current, zmax - zmin, vz, charge, npart))
     pg.addparticles(js, x, y, z, ux, uy, uz)
     return pg, js, len(x)
+
+
+def load_c3_beam(rng, npart, bounds, vz, vspread, charge=echarge, mass=39 * amu,
+                 device="cuda", kind=None, **over):
+    """A randomised beam of the learned warm start's training recipe:
+    transverse Gaussian, KV-like (uniform in an ellipse), waterbag (uniform
+    in a 4-D ellipsoid, whose projection has density 1 - r^2) or a mix of
+    two of them; sizes a, b in [2, 6] mm, centroid offsets up to 3 mm,
+    tilts up to 20 mrad (in the velocity and in the position along z), a
+    Gaussian (sigma_z 0.8-1.5 cm) or uniform (3-6 cm long) profile in z
+    centred in [2.5, 5.5] cm, currents 0.1-10 mA (log-uniform).  Keywords
+    in `over` replace drawn parameters.  Returns (ParticleGroup, params)."""
+    p = dict(kind=kind or ["gaussian", "kv", "waterbag", "mix"][int(rng.integers(4))],
+             a=rng.uniform(0.002, 0.006), b=rng.uniform(0.002, 0.006),
+             x0=rng.uniform(-0.003, 0.003), y0=rng.uniform(-0.003, 0.003),
+             tx=rng.uniform(-0.02, 0.02), ty=rng.uniform(-0.02, 0.02),
+             zprof="gauss" if rng.random() < 0.5 else "flat",
+             zc=rng.uniform(0.025, 0.055), sz=rng.uniform(0.008, 0.015),
+             zlen=rng.uniform(0.03, 0.06), current=10 ** rng.uniform(-4, -2))
+    p.update(over)
+
+    def transverse(kind, n):
+        if kind == "gaussian":
+            return rng.normal(0., 1., n), rng.normal(0., 1., n)
+        if kind == "kv":
+            r = np.sqrt(rng.random(n))
+            th = 2 * np.pi * rng.random(n)
+            return r * np.cos(th), r * np.sin(th)
+        # waterbag: uniform in the 4-D unit ball, position part
+        v = rng.normal(0., 1., (n, 4))
+        v *= (rng.random(n) ** 0.25 / np.linalg.norm(v, axis=1))[:, None]
+        return v[:, 0], v[:, 1]
+
+    if p["kind"] == "mix":
+        k1, k2 = rng.choice(["gaussian", "kv", "waterbag"], 2, replace=False)
+        n1 = npart // 2
+        u1, v1 = transverse(k1, n1)
+        u2, v2 = transverse(k2, npart - n1)
+        u, v = np.concatenate([u1, u2]), np.concatenate([v1, v2])
+        p["mix"] = "%s+%s" % (k1, k2)
+    else:
+        u, v = transverse(p["kind"], npart)
+    if p["zprof"] == "gauss":
+        z = p["zc"] + rng.normal(0., p["sz"], npart)
+        zlen = math.sqrt(2 * math.pi) * p["sz"]
+    else:
+        z = p["zc"] + p["zlen"] * (rng.random(npart) - 0.5)
+        zlen = p["zlen"]
+    x = p["x0"] + p["a"] * u + p["tx"] * (z - p["zc"])
+    y = p["y0"] + p["b"] * v + p["ty"] * (z - p["zc"])
+    ux = rng.normal(0., vspread * vz, npart) + p["tx"] * vz
+    uy = rng.normal(0., vspread * vz, npart) + p["ty"] * vz
+    uz = vz + rng.normal(0., vspread * vz, npart)
+    x, y, z, ux, uy, uz = _cull(x, y, z, ux, uy, uz, bounds)
+    pg = ParticleGroup(device)
+    js = pg.addspecies(charge, mass, p["current"] * zlen / (vz * charge * npart))
+    pg.addparticles(js, x, y, z, ux, uy, uz)
+    return pg, p

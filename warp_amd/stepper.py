@@ -71,15 +71,22 @@ def field_diagnostics(special):
 
 def particleboundaries3d(pg, solver, nlost=None):
     """Absorbing particle boundaries (particleboundaries3d,
-    w3d.F:11292-11450): particles outside the field mesh are flagged lost
+    w3d.F:11292-11450): particles outside the field mesh in x or y (and in
+    z unless top.pbound0 is periodic, when z wraps instead) are flagged lost
     and removed by processlostpart -> clearpart with top.clearlostpart's
     fill method, all on the GPU (the host reads back the new count).
-    nlost, when known (counted by the fused advance with the same tests),
-    skips the pass if no particle left."""
+    nlost, when known (the fused advance counts the particles that crossed
+    any face of the mesh), skips the pass if none did."""
     if nlost == 0:
         return
+    zb = top.pbound0
+    if top.pboundnz != zb or zb not in (0, 2) or top.pboundxy != 0:
+        raise NotImplementedError(
+            "particle boundaries: absorbing x/y (pboundxy = absorb) with absorbing or "
+            "periodic z (pbound0 = pboundnz) only; got pbound0=%s pboundnz=%s pboundxy=%s"
+            % (top.pbound0, top.pboundnz, top.pboundxy))
     mesh = (solver.xmmin, solver.ymmin, solver.zmmin, solver.dx, solver.dy, solver.dz,
-            solver.nx, solver.ny, solver.nz, top.zgrid)
+            solver.nx, solver.ny, solver.nz, top.zgrid, zb)
     for js in range(pg.ns):
         pg._remove_lost(js, top.clearlostpart, mesh)
 

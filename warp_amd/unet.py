@@ -226,11 +226,11 @@ def make_input(solver, phi_prev, mask, srho, sphi):
 def conductor_mask(solver):
     """1 on interior conductor nodes (0..n-1), from the installed lists."""
     nx, ny, nz = solver.nx, solver.ny, solver.nz
-    m = torch.zeros((nz, ny, nx), dtype=torch.uint8)
+    m = np.zeros((nz, ny, nx), dtype=np.uint8)
     if solver.hasconductors():
         L = solver.getconductorobject().lists()
         e = L["int_istart"][1] if len(L["int_istart"]) > 1 else 0
-        for ix, iy, iz in L["int_indx"][:e]:
-            if ix < nx and iy < ny and iz < nz:
-                m[iz, iy, ix] = 1
-    return m.reshape(-1).to(solver.device)
+        idx = np.asarray(L["int_indx"][:e], dtype=np.int64).reshape(-1, 3)
+        k = (idx[:, 0] < nx) & (idx[:, 1] < ny) & (idx[:, 2] < nz)
+        m[idx[k, 2], idx[k, 1], idx[k, 0]] = 1
+    return torch.from_numpy(m.reshape(-1)).to(solver.device)

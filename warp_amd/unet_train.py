@@ -1,153 +1,213 @@
-"""Training data, training loop and evaluation of the U-Net warm start.
+"""Training of the learned initial guess (warp_amd.learned) on data the
+multigrid itself generates.
 
-Pairs come from the multigrid itself: for a seeded random beam (Gaussian
-or uniform ellipse, centroid offsets, current) the solver computes phi for
-the beam at z0 (the previous step) and at z0 + vz*dt (this step); the
-network learns phi - phi_prev from (rho, conductor mask, phi_prev).  The
-sources are the beams' charge densities evaluated on the mesh nodes.
+Data: time-stepped runs of randomised beams (particles.load_c3_beam) on the
+128x128x256 mesh with the pipe and the annulus electrode at 0-20 kV, run by
+the step driver with the plain method (the previous phi as the guess, the
+reference's).  After every step the coarse source and phi are recorded;
+from the third step on each step gives one sample: the network input (the
+second time difference of the coarse source, the coarse source, the
+conductor fraction) and the target, the error of the linear extrapolation
+2 phi_(n-1) - phi_(n-2) averaged over coarse voxels, in units of
+cphys * max|d2|.  Training: Adam on the mean-square error, one sample per
+step, through the library's forward and backward kernels.
+
+    python -m warp_amd.unet_train --out warp_amd/data/warmstart_c3.npz
 """
+import argparse
 import math
+import sys
+import time
 
 import numpy as np
 import torch
 
 from . import _native as N
-from .globals import eps0
-from .unet import UNet3D, conductor_mask, make_input
+from .learned import LearnedGuess, save_weights
+from .unet import UNet3D
+
+VOLTAGES = (0., 5.0e3, 10.0e3, 15.0e3, 20.0e3)
 
 
-def beam_rho(solver, rng, zshift=0.0, params=None):
-    """Charge density of a random beam on the solver's nodes, and the
-    parameters drawn (re-used with zshift for the next step)."""
-    if params is None:
-        kind = "gauss" if rng.random() < 0.5 else "kv"
-        params = dict(kind=kind, a=rng.uniform(0.002, 0.006), b=rng.uniform(0.002, 0.006),
-                      x0=rng.uniform(-0.003, 0.003), y0=rng.uniform(-0.003, 0.003),
-                      z0=rng.uniform(0.025, 0.055), sz=rng.uniform(0.008, 0.015),
-                      tilt=rng.uniform(-0.02, 0.02), current=10 ** rng.uniform(-4, -2))
-    p = params
-    dev = solver.device
-    z = solver.zmmin + solver.dz * torch.arange(solver.nz + 1, device=dev, dtype=torch.float64)
-    y = solver.ymmin + solver.dy * torch.arange(solver.ny + 1, device=dev, dtype=torch.float64)
-    x = solver.xmmin + solver.dx * torch.arange(solver.nx + 1, device=dev, dtype=torch.float64)
-    Z, Y, X = torch.meshgrid(z, y, x, indexing="ij")
-    zc = p["z0"] + zshift
-    xs = X - p["x0"] - p["tilt"] * (Z - zc)
-    ys = Y - p["y0"]
-    lam = p["current"] / 1.0e6  # line charge at vz = 1e6 m/s, C/m
-    if p["kind"] == "gauss":
-        tr = torch.exp(-0.5 * (xs / p["a"]) ** 2 - 0.5 * (ys / p["b"]) ** 2) / (2 * math.pi * p["a"] * p["b"])
-    else:
-        tr = ((xs / p["a"]) ** 2 + (ys / p["b"]) ** 2 <= 1.).to(torch.float64) / (math.pi * p["a"] * p["b"])
-    lz = torch.exp(-0.5 * ((Z - zc) / p["sz"]) ** 2)
-    return lam * tr * lz, params
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
-def generate_pairs(solver, n, seed, dz_step=1.0e-3, mgtol=1e-9):
-    """n training pairs: (rho, phi_prev, phi) FP64 tensors on the GPU."""
+def make_solver(voltage, device, n=(128, 128, 256), **kw):
+    """The 128x128x256 problem's solver with the annulus at `voltage`."""
+    import warp_amd
+    from . import configs
+    cfg = dict(configs.CONFIGS["c1"], n=n)
+    s = configs.make_solver(cfg, device=device, mgverbose=-1, **kw)
+    s.uninstallconductor(s.conductordatalist[1][0])
+    ann = warp_amd.ZAnnulus(0.012, 0.018, 0.005, voltage, zcent=0.0325, condid=2)
+    s.installconductor(ann)
+    return s
+
+
+class Recorder:
+    """After-step hook: keeps the last three phi and coarse sources and
+    appends (d2, r0, mx, target) samples."""
+
+    def __init__(self, solver, samples, max_samples):
+        self.s = solver
+        self.lg = LearnedGuess(solver)
+        self.phis = []
+        self.samples = samples
+        self.max_samples = max_samples
+        self.tgt_scale = None
+
+    def __call__(self):
+        s, lg = self.s, self.lg
+        lg.observe(s.rho)
+        self.phis = [s.phi.clone()] + self.phis[:2]
+        if not lg.ready() or len(self.samples) >= self.max_samples:
+            return
+        lg.features()
+        tgt = torch.empty(lg.nh, dtype=torch.float32, device=lg.dev)
+        p0, p1, p2 = self.phis
+        # target in units of max|d2| (cphys = 1); rescaled once cphys is known
+        N.call("wm_unet_learn_target", p0.data_ptr(), p1.data_ptr(), p2.data_ptr(), s.nx, s.ny,
+               s.nz, lg.mx.data_ptr(), 1.0, tgt.data_ptr(), N.stream_of(lg.dev))
+        self.samples.append((lg.d2.clone(), lg.ring[0].clone(), lg.mx.clone(), tgt))
+
+
+def collect(device, ntraj, nsteps, npart, seed, max_samples, solvers=None, log_every=10):
+    """Samples from ntraj randomised runs of nsteps steps each."""
+    from . import registersolver, unregistersolver, stepper, configs
+    from .globals import top
+    from .particles import load_c3_beam
     rng = np.random.default_rng(seed)
-    out = []
-    save_tol = solver.mgtol
-    solver.mgtol = mgtol
-    for _ in range(n):
-        rho0, p = beam_rho(solver, rng)
-        rho1, _ = beam_rho(solver, rng, zshift=dz_step, params=p)
-        solver.phi = 0.
-        solver.rho = rho0
-        solver.solve()
-        phi0 = solver.phi.clone()
-        solver.rho = rho1
-        solver.solve()
-        out.append((rho1, phi0, solver.phi.clone()))
-    solver.mgtol = save_tol
-    return out
+    solvers = solvers if solvers is not None else {}
+    samples = []
+    t0 = time.time()
+    for k in range(ntraj):
+        v = VOLTAGES[int(rng.integers(len(VOLTAGES)))]
+        if v not in solvers:
+            solvers[v] = make_solver(v, device)
+        s = solvers[v]
+        pg, p = load_c3_beam(rng, npart, configs.DOMAIN, configs.VZ, configs.VSPREAD,
+                             device=device)
+        saved = {a: getattr(top, a) for a in ("pgroup", "dt", "it", "time", "nhist",
+                                              "lspecial", "particlesortinterval", "pbound0",
+                                              "pboundnz", "pboundxy")}
+        rec = Recorder(s, samples, max_samples)
+        try:
+            top.pgroup, top.dt, top.it, top.time, top.nhist = pg, configs.DT, 0, 0., 0
+            top.particlesortinterval = 0
+            top.pbound0 = top.pboundnz = top.pboundxy = 0
+            s.phi = 0.
+            registersolver(s)
+            s.sortparticles(pg)
+            stepper.generate()
+            rec()
+            stepper.installafterstep(rec)
+            stepper.step(nsteps)
+        finally:
+            if rec in stepper._afterstep:
+                stepper.uninstallafterstep(rec)
+            unregistersolver(s)
+            for a, val in saved.items():
+                setattr(top, a, val)
+        if (k + 1) % log_every == 0 or k + 1 == ntraj:
+            log("collect: %d/%d runs, %d samples, %.0f s (last: %s, %.1f kV, %.2f mA, "
+                "V-cycles %s)" % (k + 1, ntraj, len(samples), time.time() - t0,
+                                  p.get("mix", p["kind"]), v / 1e3, p["current"] * 1e3,
+                                  s.mgiters_history[-3:]))
+        if len(samples) >= max_samples:
+            break
+    return samples, solvers
 
 
-class WarmStart:
-    """A trained network plus the scales of its inputs and output."""
+def fit_cphys(samples):
+    """cphys: the median of max|target| over the samples, so that the
+    network's targets are of order one; the stored targets are rescaled."""
+    m = [float(t.abs().max()) for _, _, _, t in samples]
+    c = float(np.median(m))
+    for _, _, _, t in samples:
+        t.div_(c)
+    return c
 
-    def __init__(self, solver, seed=0):
-        self.solver = solver
-        self.net = UNet3D(solver.device, seed=seed)
-        self.mask = conductor_mask(solver)
-        self.dims = (solver.nz, solver.ny, solver.nx)
-        self.srho = self.sphi = self.sdphi = 1.0
 
-    def fit_scales(self, pairs):
-        self.srho = 1.0 / max(float(r.abs().max()) for r, _, _ in pairs)
-        self.sphi = 1.0 / max(float(p0.abs().max()) for _, p0, _ in pairs)
-        self.sdphi = 1.0 / max(float((p1 - p0).abs().max()) for _, p0, p1 in pairs)
+def train(lg, samples, steps, lr=3e-4, lr_end=3e-5, seed=0, holdout=None, log_every=250):
+    """Adam with a cosine learning-rate decay; returns the loss history and
+    the held-out ratio of the network's mean-square error to the
+    extrapolation's (predicting zero)."""
+    rng = np.random.default_rng(seed)
+    net, dev = lg.net, lg.dev
+    _, maskh = lg._masks()
+    s = N.stream_of(dev)
+    g = torch.empty(lg.nh * 16, dtype=torch.int16, device=dev)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    x = torch.empty(lg.nh * 16, dtype=torch.int16, device=dev)
+    hist = []
+    t0 = time.time()
 
-    def _target(self, phi_prev, phi):
-        d = (phi - phi_prev)[1:-2, 1:-2, 1:-2] * self.sdphi
-        return d.reshape(-1).to(torch.float32).contiguous()
+    def inp(sample):
+        d2, r0, mx, _ = sample
+        N.call("wm_unet_learn_input", d2.data_ptr(), r0.data_ptr(), maskh.data_ptr(),
+               mx.data_ptr(), lg.nh, x.data_ptr(), s)
+        return x
 
-    def _input(self, rho, phi_prev):
-        self.solver.rho = rho
-        return make_input(self.solver, phi_prev, self.mask, self.srho, self.sphi)
-
-    def train(self, pairs, steps, lr=1e-3, seed=0):
-        """Adam on the mean-square error of the scaled change of phi;
-        returns the loss history (one float per step)."""
-        rng = np.random.default_rng(seed)
-        net, dev = self.net, self.solver.device
-        inputs = [(self._input(r, p0), self._target(p0, p1)) for r, p0, p1 in pairs]
-        nv = self.dims[0] * self.dims[1] * self.dims[2]
-        g = torch.empty(nv * 16, dtype=torch.int16, device=dev)
-        loss = torch.zeros(1, dtype=torch.float32, device=dev)
-        hist = []
-        s = N.stream_of(dev)
-        for _ in range(steps):
-            x, tgt = inputs[int(rng.integers(len(inputs)))]
-            net.zero_grad()
-            out = net.forward(x, self.dims, keep=True)
+    def evaluate(sset):
+        num = den = 0.
+        for smp in sset:
+            out = net.forward(inp(smp), lg.dims)
             loss.zero_()
-            N.call("wm_unet_mse", out.data_ptr(), tgt.data_ptr(), nv, g.data_ptr(),
+            N.call("wm_unet_mse", out.data_ptr(), smp[3].data_ptr(), lg.nh, g.data_ptr(),
                    loss.data_ptr(), s)
-            net.backward(g)
-            net.adam_step(lr)
-            hist.append(float(loss.item()))
-        return hist
+            num += float(loss.item())
+            den += float((smp[3] ** 2).mean())
+        return num / den
 
-    def predict(self, rho, phi_prev):
-        """phi_prev plus the predicted change at the unpinned nodes."""
-        x = self._input(rho, phi_prev)
-        out = self.net.forward(x, self.dims)
-        phi = phi_prev.clone()
-        N.call("wm_unet_apply", out.data_ptr(), self.mask.data_ptr(), self.solver.nx,
-               self.solver.ny, self.solver.nz, self.sdphi, phi.data_ptr(),
-               N.stream_of(self.solver.device))
-        return phi
-
-
-def residual_max(solver, rho, phi):
-    """max |residual| of phi for the source rho on the finest level
-    (residual3d, as the V-cycle computes it; conductors included)."""
-    res = torch.zeros_like(phi)
-    reps0c = solver.mgparam / (eps0 * 2. * (1. / solver.dx ** 2 + 1. / solver.dy ** 2 +
-                                             1. / solver.dz ** 2))
-    scaled = rho * reps0c
-    N.call("wm_mg3d_residual", solver._h, 0, phi.data_ptr(), scaled.contiguous().data_ptr(),
-           res.data_ptr(), solver.mgparam, solver.mgform, 0, int(solver.lcndbndy),
-           solver.icndbndy, N.stream_of(solver.device))
-    return float(res.abs().max())
+    for k in range(steps):
+        net.step_lr = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * k / steps))
+        smp = samples[int(rng.integers(len(samples)))]
+        net.zero_grad()
+        out = net.forward(inp(smp), lg.dims, keep=True)
+        loss.zero_()
+        N.call("wm_unet_mse", out.data_ptr(), smp[3].data_ptr(), lg.nh, g.data_ptr(),
+               loss.data_ptr(), s)
+        net.backward(g)
+        net.adam_step(net.step_lr)
+        hist.append(float(loss.item()) / max(float((smp[3] ** 2).mean()), 1e-30))
+        if (k + 1) % log_every == 0 or k + 1 == steps:
+            msg = "train: step %d/%d, lr %.2e, relative loss (last %d) %.4f, %.0f s" % (
+                k + 1, steps, net.step_lr, log_every, np.mean(hist[-log_every:]),
+                time.time() - t0)
+            if holdout:
+                msg += ", held-out %.4f" % evaluate(holdout)
+            log(msg)
+    return hist, (evaluate(holdout) if holdout else None)
 
 
-def choose_guess(solver, ws, rho, phi_prev):
-    """The network's guess unless its residual is larger than that of the
-    previous phi (then the plain method's guess: a fallback)."""
-    guess = ws.predict(rho, phi_prev)
-    if residual_max(solver, rho, guess) < residual_max(solver, rho, phi_prev):
-        return guess, False
-    return phi_prev, True
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--trajectories", type=int, default=120)
+    ap.add_argument("--steps-per-run", type=int, default=24)
+    ap.add_argument("--particles", type=int, default=2_000_000)
+    ap.add_argument("--max-samples", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--seed", type=int, default=2026)
+    a = ap.parse_args()
+    dev = torch.device("cuda", 0)
+    samples, solvers = collect(dev, a.trajectories, a.steps_per_run, a.particles, a.seed,
+                               a.max_samples)
+    hold, _ = collect(dev, 4, a.steps_per_run, a.particles, a.seed + 1, 10 ** 9, solvers)
+    cphys = fit_cphys(samples)
+    for _, _, _, t in hold:
+        t.div_(cphys)
+    log("cphys %.4e from %d samples (%d held out)" % (cphys, len(samples), len(hold)))
+    lg = LearnedGuess(next(iter(solvers.values())), UNet3D(dev, seed=a.seed), cphys)
+    hist, ratio = train(lg, samples, a.train_steps, lr=a.lr, seed=a.seed, holdout=hold[::4])
+    nsolves = a.trajectories * (a.steps_per_run + 1)
+    save_weights(a.out, lg.net, cphys,
+                 meta=dict(samples=len(samples), solves=nsolves, train_steps=a.train_steps,
+                           heldout_ratio=ratio, mesh=[128, 128, 256]))
+    log("saved %s: held-out error / extrapolation error (mean square) %.4f" % (a.out, ratio))
 
 
-def vcycles_from(solver, rho, phi0, mgtol=1e-6):
-    """V-cycles of one solve to mgtol from the initial guess phi0."""
-    save = solver.mgtol
-    solver.mgtol = mgtol
-    solver.rho = rho
-    solver.phi = phi0
-    solver.solve()
-    solver.mgtol = save
-    return solver.mgiters, solver.mgerror
+if __name__ == "__main__":
+    main()
