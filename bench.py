@@ -66,6 +66,7 @@ def run(args, cfg, precision, guess, dev, keep=False):
     pg = configs.load_beam(cfg, device=dev)
     top.pgroup = pg
     top.dt = configs.DT
+    configs.set_particle_boundaries(cfg)
     registersolver(solver)
     npart0 = pg.nplive
     # particles ordered by cell once (particlesortxyzwithcopy) and again
@@ -145,19 +146,26 @@ def main():
     phi_before = solver.phi.clone()
     step(1)
     torch.cuda.synchronize()
-    ms_tot, nl = C.c_double(0.), C.c_int64(0)
+    ms_tot, nl, fused = C.c_double(0.), C.c_int64(0), C.c_int64(0)
     N.call("wm_mg3d_get_timing", solver._h, C.byref(ms_tot), C.byref(nl))
+    N.call("wm_mg3d_timing_fused", solver._h, C.byref(fused))
     N.call("wm_mg3d_set_timing", solver._h, 0)
     nx, ny, nz = cfg["n"]
     nodes = (nx + 1) * (ny + 1) * (nz + 1)
     s_bytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.precision]  # finest-level storage
-    alg_bytes = 2 * s_bytes * nodes  # read phi (s) + rho/2 + write phi/2 per node
+    if fused.value:
+        # full pass, both colours: read phi (s) + rho (s), write phi (s) per node
+        kname, alg_bytes = "k_sor_fused (finest-level SOR pass, both colours)", 3 * s_bytes * nodes
+        tf = os.path.join(ROOT, "profiles", "sor_fused_256x256x512_traffic.json")
+    else:
+        # half sweep: read phi (s) + rho/2, write phi/2 per node
+        kname, alg_bytes = "k_sor_zmarch (finest-level SOR half sweep)", 2 * s_bytes * nodes
+        tf = os.path.join(ROOT, "profiles", "sor_zmarch_256x256x512_traffic.json")
     avg_ms = ms_tot.value / max(nl.value, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    log("SOR half sweep: %d launches, %.4f ms avg, %.0f GB/s" % (nl.value, avg_ms, achieved))
+    log("%s: %d launches, %.4f ms avg, %.0f GB/s" % (kname, nl.value, avg_ms, achieved))
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "sor_zmarch_256x256x512_traffic.json")
-    # the counters were read for the FP64 sweep of this configuration only
+    # the counters were read for the FP64 kernel of this configuration only
     if args.precision == "fp64" and args.config == "c2" and os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -230,7 +238,7 @@ def main():
         "vcycles_per_solve": float(np.mean(iters)),
         "vcycles_cold_solve": r["cold_iters"],
         "ms_per_solve": r["ms_per_solve"],
-        "roofline": {"kernel": "k_sor_zmarch (finest-level SOR half sweep)", "bound": "hbm",
+        "roofline": {"kernel": kname, "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},

@@ -122,6 +122,10 @@ int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused);
  * elapsed times up.  Setting it resets the totals. */
 int wm_mg3d_set_timing(wm_mg3d *h, int64_t on);
 int wm_mg3d_get_timing(const wm_mg3d *h, double *ms_total, int64_t *launches);
+/* *fused = 1 when the timed launches were full SOR passes (both colours in
+ * one launch, reading one phi buffer and writing another: 3 scalars moved
+ * per node) rather than half sweeps (2 scalars per node). */
+int wm_mg3d_timing_fused(const wm_mg3d *h, int64_t *fused);
 
 /* Per-kernel entry points on one level, natural-layout device arrays of
  * that level's size.  sorhalfpass3d (f3d_mgrid.F:2095-2291), residual3d

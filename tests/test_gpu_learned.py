@@ -1,0 +1,91 @@
+"""The learned warm start on the problem it was trained for: the
+128x128x256 mesh with the grounded pipe and the annulus electrode,
+randomised beams of the training recipe (Gaussian / KV / waterbag / mixed,
+offsets, tilts, 0.1-10 mA, electrode 0-20 kV) with held-out seeds, run by
+the step driver with three initial guesses on the same runs: the previous
+phi (the reference's), its linear extrapolation in time, and the
+extrapolation plus the network's correction (falling back to the
+extrapolation when the network's residual is larger).  Every solve reaches
+mgtol = 1e-6 V; the mean V-cycles per solve of each guess and the
+network's fallback rate are printed and the network must save V-cycles
+against the reference's guess.  The network is the committed
+warp_amd/data/warmstart_c3.npz."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 12
+SEEDS = (90001, 90002, 90003, 90004)
+
+
+def _run(device, solvers, seed, guess, weights):
+    import warp_amd
+    from warp_amd import configs, stepper, unet_train as T
+    from warp_amd.globals import top
+    from warp_amd.particles import load_c3_beam
+    rng = np.random.default_rng(seed)
+    v = T.VOLTAGES[int(rng.integers(len(T.VOLTAGES)))]
+    key = (v, guess)
+    if key not in solvers:
+        kw = dict(guess=guess)
+        if guess == "learned":
+            kw["learned_weights"] = weights
+        solvers[key] = T.make_solver(v, device, **kw)
+    s = solvers[key]
+    pg, p = load_c3_beam(rng, 2_000_000, configs.DOMAIN, configs.VZ, configs.VSPREAD,
+                         device=device)
+    saved = {a: getattr(top, a) for a in ("pgroup", "dt", "it", "time", "nhist", "lspecial",
+                                          "particlesortinterval", "pbound0", "pboundnz")}
+    s.phi = 0.
+    s._phiprev = None
+    if s._learned is not None:
+        s._learned.ring = []
+    n0 = len(s.mgiters_history)
+    st0 = (s._learned.applied, s._learned.fallbacks) if s._learned is not None else (0, 0)
+    try:
+        top.pgroup, top.dt, top.it, top.time, top.nhist = pg, configs.DT, 0, 0., 0
+        top.particlesortinterval = 0
+        top.pbound0 = top.pboundnz = 0
+        warp_amd.registersolver(s)
+        s.sortparticles(pg)
+        stepper.generate()
+        errs = []
+        for _ in range(NSTEPS):
+            stepper.step(1)
+            errs.append(s.mgerror)
+    finally:
+        warp_amd.unregistersolver(s)
+        for a, val in saved.items():
+            setattr(top, a, val)
+    its = s.mgiters_history[n0:]
+    st = s._learned
+    used = (st.applied - st0[0], st.fallbacks - st0[1]) if st is not None else (0, 0)
+    return its[2:], max(errs), used, (p.get("mix", p["kind"]), v)
+
+
+def test_learned_guess_saves_vcycles(device):
+    from warp_amd.learned import default_weights
+    weights = default_weights()
+    assert os.path.exists(weights), "the trained network is not in the tree"
+    solvers = {}
+    res = {g: [] for g in ("previous", "linear", "learned")}
+    applied = fallbacks = 0
+    for seed in SEEDS:
+        for g in res:
+            its, err, used, what = _run(device, solvers, seed, g, weights)
+            assert err <= 1e-6, (seed, g, err)
+            res[g] += its
+            if g == "learned":
+                applied += used[0]
+                fallbacks += used[1]
+    mean = {g: float(np.mean(v)) for g, v in res.items()}
+    print("\n128x128x256, %d held-out runs x %d steps (solves from the 3rd on): mean V-cycles "
+          "per solve: previous phi %.2f, linear extrapolation %.2f, learned %.2f; network "
+          "guess used %d times, fell back %d times (%.0f%%)"
+          % (len(SEEDS), NSTEPS, mean["previous"], mean["linear"], mean["learned"], applied,
+             fallbacks, 100. * fallbacks / max(1, applied + fallbacks)))
+    assert mean["learned"] < mean["previous"]
+    assert applied > fallbacks

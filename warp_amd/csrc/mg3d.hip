@@ -9,6 +9,7 @@
 // V-cycles can be enqueued before the host looks at the result.
 #include "wm_internal.h"
 #include "mg3d_kernels.h"
+#include "mg3d_fused.h"
 #include "../../include/warpmi.h"
 
 #include <hip/hip_bf16.h>
@@ -135,6 +136,9 @@ struct SubDev {
   DevBuf<int> pos;
   DevBuf<uint32_t> modes, zero;
   DevBuf<T> k, denom, fac;
+  // per padded row, the first record of the row, and each record's colour
+  // element index (the fused SOR pass finds a masked point's record there)
+  DevBuf<int> rowptr, rj;
   int n = 0;
   bool anyzero = false;  // some record zeroes a neighbour's residual
   void upload(const SubHost &h) {
@@ -153,6 +157,20 @@ struct SubDev {
   SubList<T> view() const {
     return SubList<T>{n, pos.p, modes.p, k.p, denom.p, fac.p, zero.p};
   }
+  // rowptr/rj of records sorted by red-black index (rows never decrease)
+  void build_rows(const RBGeom &g, const SubHost &h) {
+    const long long nrows = (long long)g.PY * g.PZ;
+    std::vector<int> rp((size_t)nrows + 1, 0), jj((size_t)h.n());
+    for (int i = 0; i < h.n(); i++) {
+      int px = h.pos[3 * i] + 1, py = h.pos[3 * i + 1] + 1, pz = h.pos[3 * i + 2] + 1;
+      rp[(size_t)(py + (long long)g.PY * pz) + 1]++;
+      jj[i] = px >> 1;
+    }
+    for (long long r = 0; r < nrows; r++) rp[r + 1] += rp[r];
+    rowptr.upload(rp);
+    rj.upload(jj.empty() ? std::vector<int>(1, 0) : jj);
+  }
+  FusedRecs<T> fused() const { return FusedRecs<T>{view(), rowptr.p, rj.p}; }
 };
 
 struct RecKey {
@@ -188,6 +206,12 @@ struct Level {
   DevBuf<double> phi64, rho64, res64;
   DevBuf<float> phi32, rho32, res32;
   DevBuf<bf16> phi16, rho16;  // finest level of the BF16 mode
+  // the other buffer of each phi for the fused SOR pass, which reads one and
+  // writes the other; after a V-cycle (an even number of passes on every
+  // level) phi is back in the first.  Boundary values match the first's.
+  DevBuf<double> phi64b;
+  DevBuf<float> phi32b;
+  DevBuf<bf16> phi16b;
   // transfer to the next coarser level
   bool coarser = false;
   double ff = 0.;
@@ -272,6 +296,7 @@ struct wm_mg3d {
     int used = 0;
     double ms = 0.;
     long long count = 0;
+    bool fused = false;  // the timed launches were full fused passes
   } prof;
   ~wm_mg3d() {
     for (hipEvent_t e : prof.ev) (void)hipEventDestroy(e);
@@ -512,6 +537,14 @@ DevBuf<float> &phi_of<float>(Level &L) { return L.phi32; }
 template <>
 DevBuf<bf16> &phi_of<bf16>(Level &L) { return L.phi16; }
 template <typename T>
+DevBuf<T> &phib_of(Level &L);
+template <>
+DevBuf<double> &phib_of<double>(Level &L) { return L.phi64b; }
+template <>
+DevBuf<float> &phib_of<float>(Level &L) { return L.phi32b; }
+template <>
+DevBuf<bf16> &phib_of<bf16>(Level &L) { return L.phi16b; }
+template <typename T>
 DevBuf<T> &rho_of(Level &L);
 template <>
 DevBuf<double> &rho_of<double>(Level &L) { return L.rho64; }
@@ -601,6 +634,7 @@ static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
       SubHost sh;
       if (P.lcnd) sh = build_records(L, level, p, h->bounds, P.mgparam, P.mgform, P.icnd, false, false);
       C.sor[p].upload(sh);
+      C.sor[p].build_rows(L.g, sh);
     }
     C.sorkey = key;
   } else {
@@ -618,6 +652,7 @@ static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
 template <typename T>
 static void ensure_buffers(Level &L, bool with_res) {
   phi_of<T>(L).alloc_rb(2 * L.g.cs);
+  phib_of<T>(L).alloc_rb(2 * L.g.cs);
   rho_of<T>(L).alloc_rb(2 * L.g.cs);
   if (with_res) res_of<T>(L).alloc_rb(std::max<size_t>(2 * L.g.cs, (size_t)(L.nx + 3) * (L.ny + 3) * (L.nz + 3)));
 }
@@ -749,11 +784,121 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   }
 }
 
+// Shape of the fused pass on a level: lanes per row (one per colour element
+// of the row's interior, in whole wavefronts), owned rows per block and the
+// z chunk.
+struct FusedShape {
+  int j0, nj, nl, R, zc, nby, nbz;
+};
+
+static FusedShape fused_shape(const wm_mg3d *h, const Level &L) {
+  LoopRange r = sor_range(L, h->bounds);
+  FusedShape f{};
+  f.j0 = (r.ixmin + 1) / 2;
+  f.nj = (r.ixmax + 1) / 2 - f.j0 + 1;
+  f.nl = (int)(64 * cdiv(std::max(f.nj, 1), 64));
+  int rows = 1024 / f.nl;  // rows per block, halo rows included
+  f.R = std::max(1, rows - 2);
+  int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
+  f.nby = (int)cdiv(std::max(nyr, 1), f.R);
+  static const int zc_env = getenv("WARPMI_FUSED_ZC") ? atoi(getenv("WARPMI_FUSED_ZC")) : 0;
+  // about two blocks per compute unit, at least 8 planes per chunk
+  long long want = 2ll * h->ncu;
+  int zc = (int)std::max<long long>(8, cdiv((long long)nzr * f.nby, want));
+  f.zc = zc_env > 0 ? zc_env : std::min(zc, std::max(nzr, 1));
+  f.nbz = (int)cdiv(std::max(nzr, 1), f.zc);
+  return f;
+}
+
+// Whether a pass on this level can be the fused two-colour pass: six
+// Dirichlet boundaries, no record on an interior point, no Neumann subgrid
+// mirror, rows of at most 256 colour elements (two rows of halo and at
+// least one owned row per 1024-thread block), an even number of passes per
+// V-cycle (so that phi ends each V-cycle in its first buffer).
+static bool fused_ok(const wm_mg3d *h, const Level &L, const Params &P) {
+  static const bool on = getenv("WARPMI_SOR_FUSED") && atoi(getenv("WARPMI_SOR_FUSED")) != 0;
+  if (!on || getenv("WARPMI_SOR_TILED")) return false;
+  for (int i = 0; i < 6; i++)
+    if (h->bounds[i] != 0) return false;
+  if (L.sub_on_interior || (P.icnd == 2 && L.neumann)) return false;
+  if ((P.down + P.up) % 2 != 0) return false;
+  LoopRange r = sor_range(L, h->bounds);
+  int nj = (r.ixmax + 1) / 2 - (r.ixmin + 1) / 2 + 1;
+  return nj >= 1 && nj <= 256 && r.iymax >= r.iymin && r.izmax >= r.izmin;
+}
+
+// The fused pass: phi read from the first buffer, written to the second,
+// then the buffers exchanged.  cpart as in halfpass (FP64 only).
+template <typename S>
+static void fused_pass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
+                       hipStream_t s, double *cpart, double *cpart_end, int *nslots) {
+  typedef typename St<S>::T T;
+  DevBuf<S> &A = phi_of<S>(L), &B = phib_of<S>(L);
+  LoopRange r = sor_range(L, h->bounds);
+  double cst = P.mgparam * 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+  T dxsqic = (T)(L.dxsqi * cst), dysqic = (T)(L.dysqi * cst), dzsqic = (T)(L.dzsqi * cst);
+  T spm1 = (T)(1. - P.mgparam);
+  FusedRecs<T> rec0{}, rec1{};  // no row index: every masked point is pinned
+  if (P.lcnd) {
+    rec0 = cond_of<T>(L).sor[0].fused();
+    rec1 = cond_of<T>(L).sor[1].fused();
+  }
+  FusedShape f = fused_shape(h, L);
+  int nthr = (f.R + 2) * f.nl;
+  int nb = f.nby * f.nbz;
+  size_t lds = sizeof(S) * 3 * (size_t)(f.R + 2) * (size_t)(f.nj + 2);
+  bool timed = h->prof.on && level == 0 && cpart == nullptr;
+  if (timed) {
+    while ((int)h->prof.ev.size() < 2 * (h->prof.used + 1)) {
+      hipEvent_t e;
+      WM_HIP(hipEventCreate(&e));
+      h->prof.ev.push_back(e);
+    }
+    WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used], s));
+    h->prof.fused = true;
+  }
+  const uint32_t *mask = L.hasmask ? L.mask.p : nullptr;
+  if (cpart) {
+    long long ns = (long long)nb * (nthr / 64);
+    const bool fits = std::is_same<S, double>::value && ns <= cpart_end - cpart;
+    WM_REQUIRE(fits, "sorpass3d: the fused convergence test does not fit this pass");
+    if constexpr (std::is_same<S, double>::value)
+      hipLaunchKernelGGL((k_sor_fused<S, true>), dim3(nb), dim3(nthr), lds, s, (const S *)A.p,
+                         B.p, (const S *)rho_of<S>(L).p, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax,
+                         r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, rec0, rec1, f.j0,
+                         f.nj, f.nl, f.R, f.zc, f.nby, nb, h->save64.p, cpart, skip);
+    *nslots = (int)ns;
+  } else {
+    hipLaunchKernelGGL((k_sor_fused<S, false>), dim3(nb), dim3(nthr), lds, s, (const S *)A.p,
+                       B.p, (const S *)rho_of<S>(L).p, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax,
+                       r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, rec0, rec1, f.j0,
+                       f.nj, f.nl, f.R, f.zc, f.nby, nb, (S *)nullptr, (double *)nullptr, skip);
+  }
+  WM_LAUNCHED();
+  if (timed) {
+    WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
+    h->prof.used++;
+  }
+  A.swap(B);
+}
+
 template <typename T>
 static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
                     hipStream_t s, bool pin = true, double *cpart = nullptr,
                     double *cpart_end = nullptr, int *nslots = nullptr) {
   int usevolt = (level == 0 && P.mgform == 1);
+  if (fused_ok(h, L, P) && phib_of<T>(L).p) {
+    // the pin writes both buffers, so that conductor points on the mesh
+    // boundary (which no pass writes) agree in both
+    if (pin) {
+      pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
+      pin_and_mirror<T>(h, L, level, P, phib_of<T>(L).p, usevolt, skip, s);
+    }
+    int ns = 0;
+    fused_pass<T>(h, L, level, P, skip, s, cpart, cpart_end, &ns);
+    if (nslots) *nslots = ns;
+    return;
+  }
   if (pin) pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
   int used = 0;
   for (int c = 0; c <= 1; c++) {
@@ -1216,6 +1361,11 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
       hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p, np,
                          0., skip);
       WM_LAUNCHED();
+      // the error starts at zero in both buffers (their boundary values
+      // must agree for the fused pass)
+      hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64b.p, np,
+                         0., skip);
+      WM_LAUNCHED();
       vcycle<double>(h, 0, Pinit, skip, s, true);
       // phi = phi + phisave (the error plus the saved phi)
       hipLaunchKernelGGL((k_dc_add<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
@@ -1462,6 +1612,7 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       prepare_levels<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd});
     if (prec == WM_PREC_BF16_DC) {
       L0.phi16.alloc_rb(2 * L0.g.cs);
+      L0.phi16b.alloc_rb(2 * L0.g.cs);
       L0.rho16.alloc_rb(2 * L0.g.cs);
     }
     if (dc) {
@@ -1498,7 +1649,7 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       // the saved phi: written by the conversion itself when the boundary
       // conditions change nothing (six Dirichlet bounds), else copied after
       hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi,
-                         L0.phi64.p, L0.g, alldir ? h->save64.p : nullptr);
+                         L0.phi64.p, L0.g, alldir ? h->save64.p : nullptr, L0.phi64b.p);
       WM_LAUNCHED();
       launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr,
                         s);
@@ -1652,6 +1803,14 @@ int wm_mg3d_set_timing(wm_mg3d *h, int64_t on) {
     h->prof.on = on != 0;
     h->prof.ms = 0.;
     h->prof.count = 0;
+    h->prof.fused = false;
+  });
+}
+
+int wm_mg3d_timing_fused(const wm_mg3d *h, int64_t *fused) {
+  WM_ENTRY({
+    WM_REQUIRE(h && fused, "wm_mg3d_timing_fused: NULL argument");
+    *fused = h->prof.fused ? 1 : 0;
   });
 }
 
@@ -1681,7 +1840,8 @@ int wm_mg3d_sorhalfpass(wm_mg3d *h, int64_t level, int64_t parity, double *phi, 
     ensure_records<double>(h, L, (int)level, P, false, 0);
     long long nn = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
     long long nr = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
-    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g,
+                       (double *)nullptr, L.phi64b.p);
     hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
     WM_LAUNCHED();
     halfpass<double>(h, L, (int)level, P, (int)parity, nullptr, s, false);
@@ -1706,12 +1866,18 @@ int wm_mg3d_sorpass(wm_mg3d *h, int64_t level, double *phi, const double *rho, d
     ensure_records<double>(h, L, (int)level, P, false, 0);
     long long nn = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
     long long nr = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
-    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g,
+                       (double *)nullptr, L.phi64b.p);
     hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
     WM_LAUNCHED();
+    double *first = L.phi64.p;
     sorpass<double>(h, L, (int)level, P, nullptr, s);
     hipLaunchKernelGGL((k_rb_to_nat<double>), dim3(blocks(nn)), dim3(256), 0, s, L.phi64.p, phi, L.g);
     WM_LAUNCHED();
+    // a fused pass left phi in the second buffer: restore the assignment
+    // the solver's captured V-cycle graphs were recorded with (the next
+    // solve converts its phi into both buffers anyway)
+    if (L.phi64.p != first) L.phi64.swap(L.phi64b);
   });
 }
 
@@ -1731,7 +1897,8 @@ int wm_mg3d_residual(wm_mg3d *h, int64_t level, const double *phi, const double 
     build_mask(h, L, (int)level, P);
     long long nn = (long long)(L.nx + 3) * (L.ny + 3) * (L.nz + 3);
     long long nr = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
-    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g);
+    hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi, L.phi64.p, L.g,
+                       (double *)nullptr, L.phi64b.p);
     hipLaunchKernelGGL((k_rho_to_rb<double>), dim3(blocks(nr)), dim3(256), 0, s, rho, L.rho64.p, L.g);
     WM_LAUNCHED();
     L.res64.alloc_rb(std::max<size_t>(2 * L.g.cs, (size_t)nn));

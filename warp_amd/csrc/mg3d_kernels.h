@@ -1118,12 +1118,14 @@ __global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, long long n, T 
   if (t < n) a[t] = v;
 }
 
-// natural -> red-black (all elements, guards included); dst2, when given,
-// receives the same values (the solve's saved phi).
+// natural -> red-black (all elements, guards included); dst2 and dst3, when
+// given, receive the same values (the solve's saved phi, the second buffer
+// of the fused SOR pass).
 template <typename T>
 __global__ __launch_bounds__(256) void k_nat_to_rb(const double *__restrict__ src,
                                                    T *__restrict__ dst, RBGeom g,
-                                                   T *__restrict__ dst2 = nullptr) {
+                                                   T *__restrict__ dst2 = nullptr,
+                                                   T *__restrict__ dst3 = nullptr) {
   long long sx = g.nx + 3, sy = g.ny + 3;
   long long n = sx * sy * (g.nz + 3);
   long long t = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -1136,6 +1138,7 @@ __global__ __launch_bounds__(256) void k_nat_to_rb(const double *__restrict__ sr
   const T v = (T)src[t];
   dst[e] = v;
   if (dst2) dst2[e] = v;
+  if (dst3) dst3[e] = v;
 }
 
 template <typename T>

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -74,6 +75,13 @@ struct DevBuf {
   // a red-black array: element 1 of each row 128-byte aligned when the row
   // length is a multiple of 128 bytes
   void alloc_rb(size_t count) { alloc(count, 128 / sizeof(T) - 1); }
+  // exchanges the two arrays (the ping-pong buffers of the fused SOR pass)
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(raw, o.raw);
+    std::swap(n, o.n);
+    std::swap(lead, o.lead);
+  }
   void upload(const std::vector<T> &v) {
     alloc(v.size());
     if (!v.empty()) WM_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
