@@ -115,6 +115,15 @@ int wm_mg3d_reuse_phi(wm_mg3d *h);
  * else 0. */
 int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused);
 
+/* Form of the SOR passes (sorpass3d, f3d_mgrid.F:1949-2093): 0 = two
+ * half sweeps, one launch per colour (the default); 1 = both colours in
+ * one launch on every level where that applies (six Dirichlet bounds, no
+ * subgrid record on an interior point, no Neumann subgrid mirror, at most
+ * 256 colour elements per row, an even number of passes per V-cycle),
+ * reading phi from one red-black buffer and writing the other.  Both
+ * forms give the same bits.  Changing it drops the captured V-cycles. */
+int wm_mg3d_set_sor_form(wm_mg3d *h, int64_t form);
+
 /* Event timing of the finest-level FP64 SOR half sweep (the solver's
  * dominant kernel): while on, wm_mg3d_solve launches its kernels one
  * V-cycle at a time instead of replaying its graph, records HIP events

@@ -52,6 +52,7 @@ SIGNATURES = {
                                 _i64, _i64, _P, _P, _P]),
     "wm_phi_extrapolate": (C.c_int, [_P, _P, _i64, _P]),
     "wm_mg3d_set_timing": (C.c_int, [_P, _i64]),
+    "wm_mg3d_set_sor_form": (C.c_int, [_P, _i64]),
     "wm_mg3d_get_timing": (C.c_int, [_P, _P, _P]),
     "wm_mg3d_timing_fused": (C.c_int, [_P, _P]),
     "wm_mg3d_sorhalfpass": (C.c_int, [_P, _i64, _i64, _P, _P, _d, _i64, _i64, _i64, _P]),

@@ -286,6 +286,9 @@ struct wm_mg3d {
     }
   } gkey;
   int last_iters = 0;
+  // SOR pass form: 0 two half sweeps, 1 the fused two-colour pass where it
+  // applies (wm_mg3d_set_sor_form; WARPMI_SOR_FUSED=1 sets the default)
+  int sor_form = getenv("WARPMI_SOR_FUSED") && atoi(getenv("WARPMI_SOR_FUSED")) != 0 ? 1 : 0;
   bool conductors_set = false;
   int ncu = 256;                 // compute units of the device
   size_t lds_per_cu = 160 * 1024;
@@ -816,8 +819,7 @@ static FusedShape fused_shape(const wm_mg3d *h, const Level &L) {
 // least one owned row per 1024-thread block), an even number of passes per
 // V-cycle (so that phi ends each V-cycle in its first buffer).
 static bool fused_ok(const wm_mg3d *h, const Level &L, const Params &P) {
-  static const bool on = getenv("WARPMI_SOR_FUSED") && atoi(getenv("WARPMI_SOR_FUSED")) != 0;
-  if (!on || getenv("WARPMI_SOR_TILED")) return false;
+  if (h->sor_form != 1 || getenv("WARPMI_SOR_TILED")) return false;
   for (int i = 0; i < 6; i++)
     if (h->bounds[i] != 0) return false;
   if (L.sub_on_interior || (P.icnd == 2 && L.neumann)) return false;
@@ -1794,6 +1796,19 @@ int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream) {
     hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)cdiv(n, 256)), dim3(256), 0,
                        wm_stream(stream), phi, prev, (long long)n);
     WM_LAUNCHED();
+  });
+}
+
+int wm_mg3d_set_sor_form(wm_mg3d *h, int64_t form) {
+  WM_ENTRY({
+    WM_REQUIRE(h != nullptr, "wm_mg3d_set_sor_form: NULL handle");
+    WM_REQUIRE(form == 0 || form == 1, "wm_mg3d_set_sor_form: form must be 0 or 1");
+    if (h->sor_form != (int)form) {
+      h->drop_graphs();  // captured V-cycles hold the other form's launches
+      h->gkey = wm_mg3d::GKey();
+      h->last_ok = false;
+      h->sor_form = (int)form;
+    }
   });
 }
 

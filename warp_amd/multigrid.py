@@ -35,7 +35,7 @@ class MultiGrid3D(FieldSolver):
                      'icndbndy', 'laddconductor', 'lprecalccoeffs']
 
     def __init__(self, precision="fp64", device=None, sorted_deposit=False, guess="previous",
-                 learned_weights=None, **kw):
+                 learned_weights=None, sorform=0, **kw):
         FieldSolver.__init__(self, kwdict=kw)
         self.processdefaultsfrompackage(MultiGrid3D.__f3dinputs__, f3d, kw)
         assert len(kw.keys()) == 0, "Bad keyword arguemnts %s" % list(kw.keys())
@@ -87,6 +87,12 @@ class MultiGrid3D(FieldSolver):
             N.call("wm_mg3d_create", C.byref(h), self.nx, self.ny, self.nz, self.dx, self.dy,
                    self.dz, b, self.mgmaxlevels)
         self._h = h
+        # SOR pass form: 0 two half sweeps (one launch per colour), 1 both
+        # colours in one launch where the level allows it; the same bits
+        if sorform not in (0, 1):
+            raise ValueError("sorform must be 0 (two half sweeps) or 1 (fused pass)")
+        self.sorform = sorform
+        N.call("wm_mg3d_set_sor_form", h, sorform)
         self.initializeconductors()
         self.timers = dict(deposit=0., solve=0., fetche=0.)
 
