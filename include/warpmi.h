@@ -366,6 +366,18 @@ int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny,
                 int64_t nz, double dx, double dy, double dz, int64_t l4symtry,
                 int64_t l2symtry, void *work, double *ese, void *stream);
 
+/* No counterpart in the reference: wm_getese3d for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The column sums live in the library's own device
+ * scratch (1 MiB per GPU, held until the process ends) instead of a work
+ * argument, so (nx+1)*(ny+1) may be at most 131072; larger meshes return
+ * an error.  Same result, bit for bit.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (other streams, other host
+ * threads) race. */
+int wm_getese3d_noalloc(const double *rho, const double *phi, int64_t nx, int64_t ny,
+                        int64_t nz, double dx, double dy, double dz, int64_t l4symtry,
+                        int64_t l2symtry, double *ese, void *stream);
+
 /* particleboundaries3d (w3d.F:11292-11450) for absorbing boundaries on
  * all six faces (pboundxy = pbound0 = pboundnz = absorb), followed by
  * processlostpart -> clearpart (top.F:4936-5070, 4730-4893) on one species
