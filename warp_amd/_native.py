@@ -100,6 +100,7 @@ SIGNATURES = {
                               _i64, _P, _P, _P]),
     "wm_getese3d_worksize": (_i64, [_i64, _i64]),
     "wm_getese3d": (C.c_int, [_P, _P, _i64, _i64, _i64, _d, _d, _d, _i64, _i64, _P, _P, _P]),
+    "wm_getese3d_noalloc": (C.c_int, [_P, _P, _i64, _i64, _i64, _d, _d, _d, _i64, _i64, _P, _P]),
     "wm_unet_conv3": (C.c_int, [_P, _i64, _P, _P, _i64, _i64, _i64, _i64, _P, _P, _d, _P]),
     "wm_unet_conv3_wgrad": (C.c_int, [_P, _i64, _P, _i64, _i64, _i64, _i64, _P, _P, _P]),
     "wm_unet_act_bwd": (C.c_int, [_P, _P, _d, _i64, _P]),
