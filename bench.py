@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--mixed-guess", default="linear", choices=["previous", "linear"])
     p.add_argument("--sort-interval", type=int, default=25,
                    help="steps between particle sorts by cell (0: only at set-up)")
+    p.add_argument("--sorform", type=int, default=0, choices=[0, 1],
+                   help="SOR pass form: 0 two half sweeps per pass, 1 both colours in one "
+                        "launch (the same bits)")
     p.add_argument("--cpu-particles", type=int, default=4_000_000,
                    help="particles in the CPU baseline's sample")
     return p.parse_args()
@@ -62,7 +65,7 @@ def run(args, cfg, precision, guess, dev, keep=False):
     top.it, top.time, top.lspecial, top.nspecial = 0, 0., False, 0
     t0 = time.time()
     solver = configs.make_solver(cfg, precision=precision, device=dev, mgverbose=-1,
-                                 guess=guess)
+                                 guess=guess, sorform=args.sorform)
     pg = configs.load_beam(cfg, device=dev)
     top.pgroup = pg
     top.dt = configs.DT
@@ -234,6 +237,7 @@ def main():
                    "timed_steps_it": list(r["steps_timed"]),
                    "mgtol": mgtol, "particle_sort_interval": args.sort_interval,
                    "precision": args.precision, "initial_guess": args.guess,
+                   "sor_form": ["two half sweeps", "fused pass"][args.sorform],
                    "parallelism": "1 GPU"},
         "vcycles_per_solve": float(np.mean(iters)),
         "vcycles_cold_solve": r["cold_iters"],

@@ -87,5 +87,9 @@ def test_learned_guess_saves_vcycles(device):
           "guess used %d times, fell back %d times (%.0f%%)"
           % (len(SEEDS), NSTEPS, mean["previous"], mean["linear"], mean["learned"], applied,
              fallbacks, 100. * fallbacks / max(1, applied + fallbacks)))
-    assert mean["learned"] < mean["previous"]
+    # the network's own gain: fewer V-cycles than the extrapolation it
+    # corrects (0.39 fewer per solve when the network was committed), and
+    # together with it at least one fewer than the reference's guess
+    assert mean["learned"] <= mean["linear"] - 0.2
+    assert mean["learned"] <= mean["previous"] - 1.0
     assert applied > fallbacks

@@ -136,9 +136,6 @@ struct SubDev {
   DevBuf<int> pos;
   DevBuf<uint32_t> modes, zero;
   DevBuf<T> k, denom, fac;
-  // per padded row, the first record of the row, and each record's colour
-  // element index (the fused SOR pass finds a masked point's record there)
-  DevBuf<int> rowptr, rj;
   int n = 0;
   bool anyzero = false;  // some record zeroes a neighbour's residual
   void upload(const SubHost &h) {
@@ -157,20 +154,6 @@ struct SubDev {
   SubList<T> view() const {
     return SubList<T>{n, pos.p, modes.p, k.p, denom.p, fac.p, zero.p};
   }
-  // rowptr/rj of records sorted by red-black index (rows never decrease)
-  void build_rows(const RBGeom &g, const SubHost &h) {
-    const long long nrows = (long long)g.PY * g.PZ;
-    std::vector<int> rp((size_t)nrows + 1, 0), jj((size_t)h.n());
-    for (int i = 0; i < h.n(); i++) {
-      int px = h.pos[3 * i] + 1, py = h.pos[3 * i + 1] + 1, pz = h.pos[3 * i + 2] + 1;
-      rp[(size_t)(py + (long long)g.PY * pz) + 1]++;
-      jj[i] = px >> 1;
-    }
-    for (long long r = 0; r < nrows; r++) rp[r + 1] += rp[r];
-    rowptr.upload(rp);
-    rj.upload(jj.empty() ? std::vector<int>(1, 0) : jj);
-  }
-  FusedRecs<T> fused() const { return FusedRecs<T>{view(), rowptr.p, rj.p}; }
 };
 
 struct RecKey {
@@ -637,7 +620,6 @@ static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
       SubHost sh;
       if (P.lcnd) sh = build_records(L, level, p, h->bounds, P.mgparam, P.mgform, P.icnd, false, false);
       C.sor[p].upload(sh);
-      C.sor[p].build_rows(L.g, sh);
     }
     C.sorkey = key;
   } else {
@@ -788,28 +770,50 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
 }
 
 // Shape of the fused pass on a level: lanes per row (one per colour element
-// of the row's interior, in whole wavefronts), owned rows per block and the
-// z chunk.
+// of the row's interior, in whole wavefronts), owned rows per block, the z
+// chunk and the number of blocks.
 struct FusedShape {
   int j0, nj, nl, R, zc, nby, nbz;
 };
 
-static FusedShape fused_shape(const wm_mg3d *h, const Level &L) {
+// Resident blocks per compute unit of a k_sor_fused form for a block size
+// and LDS size, from the runtime's occupancy calculator (kept for the last
+// shape asked).
+template <typename S, bool CONV, bool MASK>
+static int fused_occupancy(int nthr, size_t lds) {
+  static int n = -1, nthr_of_n = 0;
+  static size_t lds_of_n = 0;
+  if (n < 0 || lds != lds_of_n || nthr != nthr_of_n) {
+    int k = 0;
+    WM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &k, reinterpret_cast<const void *>(&k_sor_fused<S, CONV, MASK>), nthr, lds));
+    n = std::max(k, 1);
+    lds_of_n = lds;
+    nthr_of_n = nthr;
+  }
+  return n;
+}
+
+static FusedShape fused_shape(const wm_mg3d *h, const Level &L, int per_cu) {
   LoopRange r = sor_range(L, h->bounds);
   FusedShape f{};
   f.j0 = (r.ixmin + 1) / 2;
   f.nj = (r.ixmax + 1) / 2 - f.j0 + 1;
   f.nl = (int)(64 * cdiv(std::max(f.nj, 1), 64));
+  static const int rows_env = getenv("WARPMI_FUSED_ROWS") ? atoi(getenv("WARPMI_FUSED_ROWS")) : 0;
   int rows = 1024 / f.nl;  // rows per block, halo rows included
+  if (rows_env >= 3 && rows_env < rows) rows = rows_env;
   f.R = std::max(1, rows - 2);
-  int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
+  int nyr = r.iymax - r.iymin + 1, nzr = std::max(r.izmax - r.izmin + 1, 1);
   f.nby = (int)cdiv(std::max(nyr, 1), f.R);
+  // one round of resident blocks: as many z chunks as the row tiles leave
+  // room for (at least 4 planes each)
   static const int zc_env = getenv("WARPMI_FUSED_ZC") ? atoi(getenv("WARPMI_FUSED_ZC")) : 0;
-  // about two blocks per compute unit, at least 8 planes per chunk
-  long long want = 2ll * h->ncu;
-  int zc = (int)std::max<long long>(8, cdiv((long long)nzr * f.nby, want));
-  f.zc = zc_env > 0 ? zc_env : std::min(zc, std::max(nzr, 1));
-  f.nbz = (int)cdiv(std::max(nzr, 1), f.zc);
+  long long slots = (long long)per_cu * h->ncu;
+  long long nbz = std::max<long long>(1, slots / f.nby);
+  int zc = (int)std::max<long long>(4, cdiv(nzr, nbz));
+  f.zc = zc_env > 0 ? zc_env : std::min(zc, nzr);
+  f.nbz = (int)cdiv(nzr, f.zc);
   return f;
 }
 
@@ -829,8 +833,40 @@ static bool fused_ok(const wm_mg3d *h, const Level &L, const Params &P) {
   return nj >= 1 && nj <= 256 && r.iymax >= r.iymin && r.izmax >= r.izmin;
 }
 
-// The fused pass: phi read from the first buffer, written to the second,
-// then the buffers exchanged.  cpart as in halfpass (FP64 only).
+// The subgrid records of colour c alone (a record-only k_sor_zmarch) on
+// the red-black array phi; with cpart, the convergence test as well.
+// Returns the number of partial-maximum slots written.
+template <typename S>
+static int records_only(Level &L, const Params &P, int c, S *phi, const LoopRange &r,
+                        typename St<S>::T dxsqic, typename St<S>::T dysqic,
+                        typename St<S>::T dzsqic, typename St<S>::T spm1, const int *skip,
+                        hipStream_t s, S *save, double *cpart) {
+  typedef typename St<S>::T T;
+  if (!P.lcnd) return 0;
+  SubList<T> sub = cond_of<T>(L).sor[c].view();
+  if (sub.n == 0) return 0;
+  const int nthr = 256;
+  int nbsub = (int)cdiv(sub.n, nthr);
+  const uint32_t *mask = L.hasmask ? L.mask.p : nullptr;
+  if (cpart) {
+    if constexpr (std::is_same<S, double>::value)
+      hipLaunchKernelGGL((k_sor_zmarch<S, T, true>), dim3(nbsub), dim3(nthr), 0, s, phi,
+                         (const S *)rho_of<S>(L).p, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax,
+                         r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, sub, 0, 1, 1, 1,
+                         0, nbsub, save, cpart, skip);
+  } else {
+    hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbsub), dim3(nthr), 0, s, phi,
+                       (const S *)rho_of<S>(L).p, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax,
+                       r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, sub, 0, 1, 1, 1, 0,
+                       nbsub, (S *)nullptr, (double *)nullptr, skip);
+  }
+  WM_LAUNCHED();
+  return cpart ? nbsub * (nthr / 64) : 0;
+}
+
+// The fused pass: colour-0 records in place in the first buffer, the dense
+// pass from the first buffer into the second, colour-1 records in the
+// second, then the buffers exchanged.  cpart as in halfpass (FP64 only).
 template <typename S>
 static void fused_pass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
                        hipStream_t s, double *cpart, double *cpart_end, int *nslots) {
@@ -840,15 +876,32 @@ static void fused_pass(wm_mg3d *h, Level &L, int level, const Params &P, const i
   double cst = P.mgparam * 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
   T dxsqic = (T)(L.dxsqi * cst), dysqic = (T)(L.dysqi * cst), dzsqic = (T)(L.dzsqi * cst);
   T spm1 = (T)(1. - P.mgparam);
-  FusedRecs<T> rec0{}, rec1{};  // no row index: every masked point is pinned
-  if (P.lcnd) {
-    rec0 = cond_of<T>(L).sor[0].fused();
-    rec1 = cond_of<T>(L).sor[1].fused();
+  const bool conv = cpart != nullptr;
+  const uint32_t *mask = L.hasmask ? L.mask.p : nullptr;
+  FusedShape f0 = fused_shape(h, L, 1);
+  size_t lds = sizeof(S) * 3 * (size_t)(f0.R + 2) * (size_t)(f0.nj + 2);
+  // a thread per lane of every row of the block, halo rows included
+  const int nthr = (f0.R + 2) * f0.nl;
+  int per_cu = 1;
+  if (conv) {
+    if constexpr (std::is_same<S, double>::value)
+      per_cu = mask ? fused_occupancy<S, true, true>(nthr, lds)
+                    : fused_occupancy<S, true, false>(nthr, lds);
+  } else {
+    per_cu = mask ? fused_occupancy<S, false, true>(nthr, lds)
+                  : fused_occupancy<S, false, false>(nthr, lds);
   }
-  FusedShape f = fused_shape(h, L);
-  int nthr = (f.R + 2) * f.nl;
+  FusedShape f = fused_shape(h, L, per_cu);
+  WM_REQUIRE(f.R == f0.R && (f.R + 2) * f.nl == nthr && nthr <= 1024,
+             "sorpass3d: inconsistent fused-pass tile");
   int nb = f.nby * f.nbz;
-  size_t lds = sizeof(S) * 3 * (size_t)(f.R + 2) * (size_t)(f.nj + 2);
+  if (conv) {
+    const bool fits = std::is_same<S, double>::value &&
+                      (long long)nb * 16 + 8 * (cdiv(cond_of<T>(L).sor[0].n, 256) +
+                                                cdiv(cond_of<T>(L).sor[1].n, 256)) <=
+                          cpart_end - cpart;
+    WM_REQUIRE(fits, "sorpass3d: the fused convergence test does not fit this pass");
+  }
   bool timed = h->prof.on && level == 0 && cpart == nullptr;
   if (timed) {
     while ((int)h->prof.ev.size() < 2 * (h->prof.used + 1)) {
@@ -859,28 +912,36 @@ static void fused_pass(wm_mg3d *h, Level &L, int level, const Params &P, const i
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used], s));
     h->prof.fused = true;
   }
-  const uint32_t *mask = L.hasmask ? L.mask.p : nullptr;
-  if (cpart) {
-    long long ns = (long long)nb * (nthr / 64);
-    const bool fits = std::is_same<S, double>::value && ns <= cpart_end - cpart;
-    WM_REQUIRE(fits, "sorpass3d: the fused convergence test does not fit this pass");
-    if constexpr (std::is_same<S, double>::value)
-      hipLaunchKernelGGL((k_sor_fused<S, true>), dim3(nb), dim3(nthr), lds, s, (const S *)A.p,
-                         B.p, (const S *)rho_of<S>(L).p, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax,
-                         r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, rec0, rec1, f.j0,
-                         f.nj, f.nl, f.R, f.zc, f.nby, nb, h->save64.p, cpart, skip);
-    *nslots = (int)ns;
+  int used = records_only<S>(L, P, 0, A.p, r, dxsqic, dysqic, dzsqic, spm1, skip, s,
+                             conv ? (S *)h->save64.p : nullptr, cpart);
+#define WM_FUSED_LAUNCH(CV, MK)                                                                \
+  hipLaunchKernelGGL((k_sor_fused<S, CV, MK>), dim3(nb), dim3(nthr), lds, s, (const S *)A.p, \
+                     B.p, (const S *)rho_of<S>(L).p, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, \
+                     r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, f.j0, f.nj, f.nl,   \
+                     f.R, f.zc, f.nby, nb, CV ? (S *)h->save64.p : (S *)nullptr,              \
+                     CV ? cpart + used : (double *)nullptr, skip)
+  if (conv) {
+    if constexpr (std::is_same<S, double>::value) {
+      if (mask)
+        WM_FUSED_LAUNCH(true, true);
+      else
+        WM_FUSED_LAUNCH(true, false);
+    }
+    used += nb * (nthr / 64);
+  } else if (mask) {
+    WM_FUSED_LAUNCH(false, true);
   } else {
-    hipLaunchKernelGGL((k_sor_fused<S, false>), dim3(nb), dim3(nthr), lds, s, (const S *)A.p,
-                       B.p, (const S *)rho_of<S>(L).p, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax,
-                       r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, rec0, rec1, f.j0,
-                       f.nj, f.nl, f.R, f.zc, f.nby, nb, (S *)nullptr, (double *)nullptr, skip);
+    WM_FUSED_LAUNCH(false, false);
   }
+#undef WM_FUSED_LAUNCH
   WM_LAUNCHED();
+  used += records_only<S>(L, P, 1, B.p, r, dxsqic, dysqic, dzsqic, spm1, skip, s,
+                          conv ? (S *)h->save64.p : nullptr, conv ? cpart + used : nullptr);
   if (timed) {
     WM_HIP(hipEventRecord(h->prof.ev[2 * h->prof.used + 1], s));
     h->prof.used++;
   }
+  if (nslots) *nslots = used;
   A.swap(B);
 }
 

@@ -90,9 +90,9 @@ class LearnedGuess:
     def _masks(self):
         key = getattr(self.solver, "_cond_gen", None)
         if self.mask is None or key != self._mask_key:
-            self.mask = conductor_mask(self.solver)
+            cond = conductor_mask(self.solver)
             nz, ny, nx = self.solver.nz, self.solver.ny, self.solver.nx
-            m = self.mask.reshape(nz, ny, nx).to(torch.float64)
+            m = cond.reshape(nz, ny, nx).to(torch.float64)
             # coarse conductor fraction, through the same block mean (a node
             # array with the nz+1, ny+1, nx+1 layout the kernel expects)
             full = torch.zeros((nz + 1, ny + 1, nx + 1), dtype=torch.float64, device=self.dev)
@@ -100,6 +100,9 @@ class LearnedGuess:
             self.maskh = torch.empty(self.nh, dtype=torch.float32, device=self.dev)
             N.call("wm_unet_half_mean", full.data_ptr(), nx, ny, nz, 0, self.maskh.data_ptr(),
                    N.stream_of(self.dev))
+            # the correction goes only to the region the beam lives in
+            self.mask = torch.from_numpy(outside_main_region(cond.cpu().numpy().reshape(
+                nz, ny, nx)).reshape(-1)).to(self.dev)
             self._mask_key = key
         return self.mask, self.maskh
 
@@ -135,6 +138,27 @@ class LearnedGuess:
         s = self.solver
         N.call("wm_unet_learn_apply", out.data_ptr(), self.mx.data_ptr(), self.cphys, s.nx, s.ny,
                s.nz, mask.data_ptr(), phi.data_ptr(), N.stream_of(self.dev))
+
+
+def outside_main_region(cond):
+    """uint8 (nz, ny, nx): 1 where the learned correction must not go --
+    conductor interior nodes (cond = 1) and every free node that is not
+    6-connected through free nodes to the centre of the mesh (pockets
+    enclosed by a conductor's interior shell, e.g. the corners outside a
+    pipe).  The multigrid converges slowly on errors in such pockets, whose
+    potential the extrapolation already has exactly; the network's small
+    output there only costs V-cycles.  When the centre is a conductor node
+    the largest free region is kept."""
+    from scipy import ndimage
+    free = cond == 0
+    lab, n = ndimage.label(free)
+    if n == 0:
+        return np.ones_like(cond, dtype=np.uint8)
+    nz, ny, nx = cond.shape
+    c = lab[nz // 2, ny // 2, nx // 2]
+    if c == 0:
+        c = int(np.argmax(np.bincount(lab.ravel())[1:])) + 1
+    return (lab != c).astype(np.uint8)
 
 
 def residual_max(solver, phi):
