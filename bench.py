@@ -41,9 +41,9 @@ def parse():
     p.add_argument("--mixed-guess", default="linear", choices=["previous", "linear"])
     p.add_argument("--sort-interval", type=int, default=25,
                    help="steps between particle sorts by cell (0: only at set-up)")
-    p.add_argument("--sorform", type=int, default=0, choices=[0, 1],
+    p.add_argument("--sorform", type=int, default=0, choices=[0, 1, 2],
                    help="SOR pass form: 0 two half sweeps per pass, 1 both colours in one "
-                        "launch (the same bits)")
+                        "launch, 2 that on the finest level only (the same bits)")
     p.add_argument("--cpu-particles", type=int, default=4_000_000,
                    help="particles in the CPU baseline's sample")
     return p.parse_args()
@@ -237,7 +237,8 @@ def main():
                    "timed_steps_it": list(r["steps_timed"]),
                    "mgtol": mgtol, "particle_sort_interval": args.sort_interval,
                    "precision": args.precision, "initial_guess": args.guess,
-                   "sor_form": ["two half sweeps", "fused pass"][args.sorform],
+                   "sor_form": ["two half sweeps", "fused pass",
+                                "fused pass on the finest level"][args.sorform],
                    "parallelism": "1 GPU"},
         "vcycles_per_solve": float(np.mean(iters)),
         "vcycles_cold_solve": r["cold_iters"],

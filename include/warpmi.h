@@ -116,12 +116,14 @@ int wm_mg3d_reuse_phi(wm_mg3d *h);
 int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused);
 
 /* Form of the SOR passes (sorpass3d, f3d_mgrid.F:1949-2093): 0 = two
- * half sweeps, one launch per colour (the default); 1 = both colours in
- * one launch on every level where that applies (six Dirichlet bounds, no
- * subgrid record on an interior point, no Neumann subgrid mirror, at most
- * 256 colour elements per row, an even number of passes per V-cycle),
- * reading phi from one red-black buffer and writing the other.  Both
- * forms give the same bits.  Changing it drops the captured V-cycles. */
+ * half sweeps, one launch per colour (the default); 1 = both colours'
+ * dense updates in one launch on every level where that applies (six
+ * Dirichlet bounds, no subgrid record on an interior point, no Neumann
+ * subgrid mirror, at most 256 colour elements per row, an even number of
+ * passes per V-cycle), reading phi from one red-black buffer and writing
+ * the other, with the conductor records in launches of their own before
+ * and after it; 2 = that on the finest level only.  Every form gives the
+ * same bits.  Changing it drops the captured V-cycles. */
 int wm_mg3d_set_sor_form(wm_mg3d *h, int64_t form);
 
 /* Event timing of the finest-level FP64 SOR half sweep (the solver's
@@ -456,7 +458,8 @@ int wm_getselfe3d(const double *phi, int64_t nx, int64_t ny, int64_t nz,
  * convolution.
  * ------------------------------------------------------------------- */
 
-/* 3x3x3 convolution, zero padding 1, bf16 MFMA with fp32 accumulation:
+/* 3x3x3 convolution, zero padding 1, bf16 MFMA (K = 32) with fp32
+ * accumulation, input tiles staged in LDS:
  * y = leaky_relu(conv(x) + bias, slope) (slope 1: identity; bias may be
  * NULL).  Output to y (bf16) and/or yf (fp32), either may be NULL.
  * cin and cout multiples of 16, cout <= 128. */
@@ -472,6 +475,12 @@ int wm_unet_conv3_wgrad(const uint16_t *x, int64_t cin, const uint16_t *g,
 /* g *= leaky_relu'(y) elementwise (n values). */
 int wm_unet_act_bwd(uint16_t *g, const uint16_t *y, double slope, int64_t n,
                     void *stream);
+/* Packs the fp32 masters w[co][ci][27] as bf16 into wp (forward) and wt
+ * (backward over the input: taps mirrored, channel roles exchanged), each
+ * of wm_unet_pack_size(cin, cout) = 28*cin*cout values: per 16-channel
+ * block and pair of taps (a 28th, zero tap pads the last pair) the K = 32
+ * operand of one MFMA step. */
+int64_t wm_unet_pack_size(int64_t cin, int64_t cout);
 int wm_unet_pack(const float *w, int64_t cin, int64_t cout, uint16_t *wp,
                  uint16_t *wt, void *stream);
 /* 2x2x2 average pooling of x (D,H,W,c) into y; backward: x is the pooled

@@ -99,6 +99,26 @@ def test_fused_pass_defect_correction_same_bits(device, precision):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("conductors", [False, True])
+def test_fused_finest_level_only(device, conductors):
+    """Form 2 (the fused pass on the finest level, half sweeps below):
+    bitwise equal to the oracle, cold and warm."""
+    n = (64, 64, 128)
+    nx, ny, nz = n
+    s = _solver(n, 2, conductors)
+    rho = _rho(n, 25)
+    co = (O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+          if conductors else None)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    for k, f in enumerate((1.0, 0.8)):
+        ro = rho * f
+        s.rho = torch.from_numpy(ro).to(device)
+        s.solve()
+        it, err = O.multigrid3dsolve(po, ro.copy(), s.dx, s.dy, s.dz, conductors=co)
+        assert (s.mgiters, s.mgerror) == (it, err), k
+        np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+
+
 def test_switching_forms_between_solves(device):
     """wm_mg3d_set_sor_form between solves of one solver: every solve
     still equals the oracle bit for bit."""
@@ -109,7 +129,7 @@ def test_switching_forms_between_solves(device):
     rho = _rho(n, 24)
     co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
     po = np.zeros((nz + 3, ny + 3, nx + 3))
-    for k, form in enumerate((0, 1, 1, 0, 1)):
+    for k, form in enumerate((0, 1, 1, 0, 2, 1)):
         N.call("wm_mg3d_set_sor_form", s._h, form)
         ro = rho * (1. + 0.1 * k)
         s.rho = torch.from_numpy(ro).to(device)

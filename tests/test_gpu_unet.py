@@ -40,7 +40,7 @@ def test_conv3_forward(device, cin, cout, dims):
     w = _bf16(torch.randn(cout, cin, 27, generator=g, dtype=torch.float64) * 0.1)
     b = torch.randn(cout, generator=g, dtype=torch.float64).to(torch.float32)
     wd = w.to(torch.float32).to(device)
-    wp = torch.empty(27 * cin * cout, dtype=torch.int16, device=device)
+    wp = torch.empty(int(N.lib().wm_unet_pack_size(cin, cout)), dtype=torch.int16, device=device)
     wt = torch.empty_like(wp)
     s = N.stream_of(device)
     N.call("wm_unet_pack", wd.data_ptr(), cin, cout, wp.data_ptr(), wt.data_ptr(), s)
@@ -67,7 +67,7 @@ def test_conv3_backward(device):
     (y * gy).sum().backward()
     s = N.stream_of(device)
     wd = w.detach().to(torch.float32).to(device)
-    wp = torch.empty(27 * cin * cout, dtype=torch.int16, device=device)
+    wp = torch.empty(int(N.lib().wm_unet_pack_size(cin, cout)), dtype=torch.int16, device=device)
     wt = torch.empty_like(wp)
     N.call("wm_unet_pack", wd.data_ptr(), cin, cout, wp.data_ptr(), wt.data_ptr(), s)
     xd = _to_dev_bits(x.detach().permute(1, 2, 3, 0), device)

@@ -104,6 +104,7 @@ SIGNATURES = {
     "wm_unet_conv3": (C.c_int, [_P, _i64, _P, _P, _i64, _i64, _i64, _i64, _P, _P, _d, _P]),
     "wm_unet_conv3_wgrad": (C.c_int, [_P, _i64, _P, _i64, _i64, _i64, _i64, _P, _P, _P]),
     "wm_unet_act_bwd": (C.c_int, [_P, _P, _d, _i64, _P]),
+    "wm_unet_pack_size": (_i64, [_i64, _i64]),
     "wm_unet_pack": (C.c_int, [_P, _i64, _i64, _P, _P, _P]),
     "wm_unet_pool": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _P]),
     "wm_unet_up": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _i64, _P]),

@@ -270,8 +270,9 @@ struct wm_mg3d {
   } gkey;
   int last_iters = 0;
   // SOR pass form: 0 two half sweeps, 1 the fused two-colour pass where it
-  // applies (wm_mg3d_set_sor_form; WARPMI_SOR_FUSED=1 sets the default)
-  int sor_form = getenv("WARPMI_SOR_FUSED") && atoi(getenv("WARPMI_SOR_FUSED")) != 0 ? 1 : 0;
+  // applies, 2 the fused pass on the finest level only
+  // (wm_mg3d_set_sor_form; WARPMI_SOR_FUSED=1 or 2 sets the default)
+  int sor_form = getenv("WARPMI_SOR_FUSED") ? std::min(2, std::max(0, atoi(getenv("WARPMI_SOR_FUSED")))) : 0;
   bool conductors_set = false;
   int ncu = 256;                 // compute units of the device
   size_t lds_per_cu = 160 * 1024;
@@ -610,7 +611,9 @@ static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
   CondDev<T> &C = cond_of<T>(L);
   RecKey key;
   key.mgparam = P.mgparam;
-  key.mgform = P.mgform;
+  // voltages enter only on the finest level (voltfac), so the records of
+  // coarser levels are the same for both mgforms and are built once
+  key.mgform = level == 0 ? P.mgform : 0;
   key.icndbndy = P.icnd;
   key.lcndbndy = P.lcnd;
   key.mgform2init = residual ? mgform2init : 0;
@@ -704,7 +707,11 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
   int nby = (int)cdiv(nyr, rows);
   int nbsub = (int)(8 * cdiv(cdiv(sub.n, nthr), 8));
   static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
-  const bool tiled = tiled_env && !std::is_same<S, bf16>::value;
+  static const long long tiled_below =
+      getenv("WARPMI_TILED_BELOW") ? atoll(getenv("WARPMI_TILED_BELOW")) : 0;
+  const long long nodes = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
+  const bool tiled = (tiled_env || (cpart == nullptr && nodes < tiled_below)) &&
+                     !std::is_same<S, bf16>::value;
   // z-marching sweep: colour elements j0..j1 cover ix in [ixmin, ixmax]
   // for both row parities; z chunks sized for about 2048 dense blocks.
   int j0 = (r.ixmin + 1) / 2, j1 = (r.ixmax + 1) / 2;
@@ -739,17 +746,31 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
     long long ns = (long long)(nbdense + nbsub) * (nthr / 64);
     const bool fits = !tiled && std::is_same<S, double>::value && ns <= cpart_end - cpart;
     WM_REQUIRE(fits, "sorhalfpass3d: the fused convergence test does not fit this sweep");
-    if constexpr (std::is_same<S, double>::value)
-      hipLaunchKernelGGL((k_sor_zmarch<S, T, true>), dim3(nbdense + nbsub), dim3(nthr), 0, s,
-                         phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
-                         dxsqic, dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0,
-                         nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip);
+    // the dense sweep reads mask words only when the level has a mask
+    if constexpr (std::is_same<S, double>::value) {
+      if (L.hasmask)
+        hipLaunchKernelGGL((k_sor_zmarch<S, T, true, true>), dim3(nbdense + nbsub), dim3(nthr),
+                           0, s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                           r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)L.mask.p, sub,
+                           j0, nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip);
+      else
+        hipLaunchKernelGGL((k_sor_zmarch<S, T, true, false>), dim3(nbdense + nbsub), dim3(nthr),
+                           0, s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                           r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)nullptr, sub,
+                           j0, nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip);
+    }
     *nslots = (int)ns;
   } else if (!tiled) {
-    hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbdense + nbsub), dim3(nthr), 0, s, phi, rho,
-                       L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
-                       dysqic, dzsqic, spm1, L.hasmask ? L.mask.p : nullptr, sub, j0, nbx, nby,
-                       zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip);
+    if (L.hasmask)
+      hipLaunchKernelGGL((k_sor_zmarch<S, T, false, true>), dim3(nbdense + nbsub), dim3(nthr), 0,
+                         s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                         r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)L.mask.p, sub,
+                         j0, nbx, nby, zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip);
+    else
+      hipLaunchKernelGGL((k_sor_zmarch<S, T, false, false>), dim3(nbdense + nbsub), dim3(nthr),
+                         0, s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                         r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)nullptr, sub,
+                         j0, nbx, nby, zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip);
   }
   WM_LAUNCHED();
   if (timed) {
@@ -822,8 +843,9 @@ static FusedShape fused_shape(const wm_mg3d *h, const Level &L, int per_cu) {
 // mirror, rows of at most 256 colour elements (two rows of halo and at
 // least one owned row per 1024-thread block), an even number of passes per
 // V-cycle (so that phi ends each V-cycle in its first buffer).
-static bool fused_ok(const wm_mg3d *h, const Level &L, const Params &P) {
-  if (h->sor_form != 1 || getenv("WARPMI_SOR_TILED")) return false;
+static bool fused_ok(const wm_mg3d *h, const Level &L, int level, const Params &P) {
+  if (h->sor_form == 0 || (h->sor_form == 2 && level != 0) || getenv("WARPMI_SOR_TILED"))
+    return false;
   for (int i = 0; i < 6; i++)
     if (h->bounds[i] != 0) return false;
   if (L.sub_on_interior || (P.icnd == 2 && L.neumann)) return false;
@@ -950,7 +972,7 @@ static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int 
                     hipStream_t s, bool pin = true, double *cpart = nullptr,
                     double *cpart_end = nullptr, int *nslots = nullptr) {
   int usevolt = (level == 0 && P.mgform == 1);
-  if (fused_ok(h, L, P) && phib_of<T>(L).p) {
+  if (fused_ok(h, L, level, P) && phib_of<T>(L).p) {
     // the pin writes both buffers, so that conductor points on the mesh
     // boundary (which no pass writes) agree in both
     if (pin) {
@@ -1051,7 +1073,7 @@ static void restrict_to(Level &L, const TR *res, FineIdx F, TC *out, OutIdx O, c
   RestrictAxis ay{L.rlo[1].p, L.rhi[1].p, L.rw[1].p, L.rcmin[1], L.rcmax[1]};
   RestrictAxis az{L.rlo[2].p, L.rhi[2].p, L.rw[2].p, L.rcmin[2], L.rcmax[2]};
   long long n = (long long)(nxc + 1) * (nyc + 1) * (nzc + 1);
-  if constexpr (std::is_same<TR, double>::value && std::is_same<TC, double>::value) {
+  if constexpr (std::is_same<TC, double>::value) {
     hipLaunchKernelGGL((k_restrict<TR, TC, OutIdx, FineIdx>), dim3(blocks(n)), dim3(256), 0, s,
                        res, F, out, O, nxc, nyc, nzc, ax, ay, az, L.ff, skip);
   } else {
@@ -1091,10 +1113,9 @@ static bool pin_fusable(const wm_mg3d *h, const Level &L, int level, const Param
   return !usevolt && L.nint > 0 && L.imask.p != nullptr && !(P.icnd == 2 && L.neumann);
 }
 
-template <typename S>
+template <typename S, typename SC = typename Coarse<S>::type>
 static void expand_rb(wm_mg3d *h, Level &L, Level &Lc, int level, const Params &P,
                       const int *skip, hipStream_t s, bool fusepin = false) {
-  typedef typename Coarse<S>::type SC;
   ExpandAxis ax{L.ej[0].p, L.ew[0].p, L.elo[0], L.ehi[0]};
   ExpandAxis ay{L.ej[1].p, L.ew[1].p, L.elo[1], L.ehi[1]};
   ExpandAxis az{L.ej[2].p, L.ew[2].p, L.elo[2], L.ehi[2]};
@@ -1280,8 +1301,45 @@ static void zero_interior_res(wm_mg3d *h, const Params &P, hipStream_t s) {
 template <typename S>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
                    bool zeroed = false, bool repin = true, double *cpart = nullptr,
-                   double *cpart_end = nullptr, int *nslots = nullptr) {
+                   double *cpart_end = nullptr, int *nslots = nullptr);
+
+// The coarse-grid correction of one level (f3d_mgrid.F:1293-1507): the
+// residual restricted to the next level (stored as SC), the V-cycle there
+// from zero, the expansion added back.  Returns whether the expansion also
+// did the pin of the first up pass.
+template <typename S, typename SC>
+static bool coarse_correction(wm_mg3d *h, Level &L, int level, const Params &P,
+                              const int *skip, hipStream_t s) {
   typedef typename St<S>::T T;
+  Level &Lc = *h->lv[level + 1];
+  bool anyzero = false;
+  if (L.hasmask) {
+    ensure_records<T>(h, L, level, P, true, 0);
+    anyzero = cond_of<T>(L).res[0][0].anyzero || cond_of<T>(L).res[0][1].anyzero;
+  }
+  // the coarse phi starts at zero: zeroed by the fused kernel's extra
+  // blocks, or by k_fill after the separate residual and restriction
+  if (fused_restrict_ok(h, L, anyzero)) {
+    resrestrict<S, T, SC>(h, L, Lc, level, P, skip, s, phi_of<SC>(Lc).p, 2 * Lc.g.cs);
+  } else {
+    residual_rb<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
+                      s);
+    restrict_to<T, SC>(L, (const T *)res_of<T>(L).p, RBIdx{L.g}, rho_of<SC>(Lc).p,
+                       RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny, Lc.nz);
+    hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
+                       phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
+    WM_LAUNCHED();
+  }
+  vcycle<SC>(h, level + 1, P, skip, s, true);
+  const bool fusepin = P.up > 0 && pin_fusable(h, L, level, P);
+  expand_rb<S, SC>(h, L, Lc, level, P, skip, s, fusepin);
+  launch_bc<S>(phi_of<S>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+  return fusepin;
+}
+
+template <typename S>
+static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
+                   bool zeroed, bool repin, double *cpart, double *cpart_end, int *nslots) {
   typedef typename Coarse<S>::type SC;
   Level &L = *h->lv[level];
   const bool novolt = !(level == 0 && P.mgform == 1);
@@ -1292,29 +1350,11 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   for (int i = 0; i < P.down; i++)
     sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0) && (repin || i > 0));
   if (L.coarser) {
-    Level &Lc = *h->lv[level + 1];
-    bool anyzero = false;
-    if (L.hasmask) {
-      ensure_records<T>(h, L, level, P, true, 0);
-      anyzero = cond_of<T>(L).res[0][0].anyzero || cond_of<T>(L).res[0][1].anyzero;
-    }
-    // the coarse phi starts at zero: zeroed by the fused kernel's extra
-    // blocks, or by k_fill after the separate residual and restriction
-    if (fused_restrict_ok(h, L, anyzero)) {
-      resrestrict<S, T, SC>(h, L, Lc, level, P, skip, s, phi_of<SC>(Lc).p, 2 * Lc.g.cs);
-    } else {
-      residual_rb<S, T>(h, L, level, P, phi_of<S>(L).p, rho_of<S>(L).p, res_of<T>(L).p, 0, skip,
-                        s);
-      restrict_to<T, SC>(L, (const T *)res_of<T>(L).p, RBIdx{L.g}, rho_of<SC>(Lc).p,
-                         RBIdx{Lc.g}, skip, s, Lc.nx, Lc.ny, Lc.nz);
-      hipLaunchKernelGGL((k_fill<SC>), dim3(blocks(2 * Lc.g.cs)), dim3(256), 0, s,
-                         phi_of<SC>(Lc).p, 2 * Lc.g.cs, (SC)0., skip);
-      WM_LAUNCHED();
-    }
-    vcycle<SC>(h, level + 1, P, skip, s, true);
-    fusepin = P.up > 0 && pin_fusable(h, L, level, P);
-    expand_rb<S>(h, L, Lc, level, P, skip, s, fusepin);
-    launch_bc<S>(phi_of<S>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+    // the coarsest level of a lower-precision V-cycle is solved in FP64
+    if (!std::is_same<S, double>::value && level + 2 == (int)h->lv.size())
+      fusepin = coarse_correction<S, double>(h, L, level, P, skip, s);
+    else
+      fusepin = coarse_correction<S, SC>(h, L, level, P, skip, s);
   }
   // the first up pass's pin already ran inside the expansion when fusepin;
   // the last up pass carries the convergence test when cpart is given
@@ -1863,7 +1903,7 @@ int wm_phi_extrapolate(double *phi, double *prev, int64_t n, void *stream) {
 int wm_mg3d_set_sor_form(wm_mg3d *h, int64_t form) {
   WM_ENTRY({
     WM_REQUIRE(h != nullptr, "wm_mg3d_set_sor_form: NULL handle");
-    WM_REQUIRE(form == 0 || form == 1, "wm_mg3d_set_sor_form: form must be 0 or 1");
+    WM_REQUIRE(form >= 0 && form <= 2, "wm_mg3d_set_sor_form: form must be 0, 1 or 2");
     if (h->sor_form != (int)form) {
       h->drop_graphs();  // captured V-cycles hold the other form's launches
       h->gkey = wm_mg3d::GKey();

@@ -191,17 +191,34 @@ __global__ __launch_bounds__(1024) void k_sor_fused(
     o1m = o1c;
     o1c = c.o1p;
   };
-  // Two sets of loads in turn, so that a step's values are never copied
-  // (a copy would wait for them before the next set is issued); the last
-  // step re-reads its own plane into the idle set (unused).
-  FusedPref<T> pa_, pb_;
-  load(za - 1, pa_);
-  for (int z = za - 1; z <= zb + 1; z += 2) {
-    load(min(z + 1, zb + 1), pb_);
-    step(z, pa_);
-    if (z + 1 > zb + 1) break;
-    load(min(z + 2, zb + 1), pa_);
-    step(z + 1, pb_);
+  // Sets of loads in turn, so that a step's values are never copied (a
+  // copy would wait for them before the next set is issued): two (one step
+  // ahead) with the convergence test or narrower types, three (two steps
+  // ahead) for the plain FP64 pass; the last steps re-read their own plane
+  // into idle sets (unused).
+  FusedPref<T> pa_, pb_, pc_;
+  if (CONV || sizeof(S) < 8) {
+    load(za - 1, pa_);
+    for (int z = za - 1; z <= zb + 1; z += 2) {
+      load(min(z + 1, zb + 1), pb_);
+      step(z, pa_);
+      if (z + 1 > zb + 1) break;
+      load(min(z + 2, zb + 1), pa_);
+      step(z + 1, pb_);
+    }
+  } else {
+    load(za - 1, pa_);
+    load(min(za, zb + 1), pb_);
+    for (int z = za - 1; z <= zb + 1; z += 3) {
+      load(min(z + 2, zb + 1), pc_);
+      step(z, pa_);
+      if (z + 1 > zb + 1) break;
+      load(min(z + 3, zb + 1), pa_);
+      step(z + 1, pb_);
+      if (z + 2 > zb + 1) break;
+      load(min(z + 4, zb + 1), pb_);
+      step(z + 2, pc_);
+    }
   }
   if (CONV) {
     cmax = wave_max(cmax);

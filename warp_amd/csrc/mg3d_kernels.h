@@ -243,7 +243,7 @@ __device__ inline double sor_record_conv(S *__restrict__ phi, const S *__restric
 // compared with save, its value before the V-cycle, and copied there; every
 // wavefront writes the largest |save - new| of its points to cpart[slot],
 // slot = block * (wavefronts per block) + wavefront, 0 when it has none.
-template <typename S, typename T = typename St<S>::T, bool CONV = false>
+template <typename S, typename T = typename St<S>::T, bool CONV = false, bool MASK = true>
 __global__ __launch_bounds__(1024) void k_sor_zmarch(
     S *__restrict__ phi, const S *__restrict__ rho, RBGeom g, int c, int ixmin,
     int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
@@ -282,54 +282,65 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     int jc = min(j, g.W - 1);
     long long rowc = (long long)g.W * py + jc;
     long long meoff = (long long)c * g.cs + rowc;  // mask index within a plane
-    // Loads of one plane (padded index pz), issued a plane ahead of use.
-    auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw,
-                    T &sv) {
+    // The loads of one plane (padded index pz), issued a plane ahead of use
+    // and unconditionally (every lane loads its x-neighbour beyond the
+    // wavefront, at a clamped element, whether or not it is an edge lane),
+    // so that the compiler never has to drain the prefetch to wait for a
+    // load issued on only some paths.
+    struct Pl {
+      T ozp, oym, oyp, own, r, edge, sv;
+      uint32_t mw;
+    };
+    auto load = [&](int pz, Pl &q) {
       long long base = pl * pz;
-      if (CONV) sv = St<S>::ld(save[c * g.cs + rowc + base]);
-      ozp = St<S>::ld(o[rowc + base + pl]);
-      oym = St<S>::ld(o[rowc + base - g.W]);
-      oyp = St<S>::ld(o[rowc + base + g.W]);
-      own = St<S>::ld(me[rowc + base]);
-      r = St<S>::ld(rh[rowc + base]);
+      q.sv = CONV ? St<S>::ld(save[c * g.cs + rowc + base]) : (T)0.;
+      q.ozp = St<S>::ld(o[rowc + base + pl]);
+      q.oym = St<S>::ld(o[rowc + base - g.W]);
+      q.oyp = St<S>::ld(o[rowc + base + g.W]);
+      q.own = St<S>::ld(me[rowc + base]);
+      q.r = St<S>::ld(rh[rowc + base]);
       int p0 = (c + py + pz + 1) & 1;
       int jn = p0 ? j + 1 : j - 1;
-      bool e = p0 ? (lane == 63) : (lane == 0);
-      edge = e ? St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]) : (T)0.;
-      mw = mask ? mask[(meoff + base) >> 5] : 0u;
+      q.edge = St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]);
+      q.mw = MASK ? mask[(meoff + base) >> 5] : 0u;
     };
     T oz_m = St<S>::ld(o[rowc + pl * za]);        // padded plane za: iz = za-1
     T oz_0 = St<S>::ld(o[rowc + pl * (za + 1)]);  // iz = za
-    T ozp, oym, oyp, own, r, eg, sv = 0;
-    uint32_t mw;
-    load(za + 1, ozp, oym, oyp, own, r, eg, mw, sv);
-    for (int iz = za; iz <= zb; iz++) {
+    auto step = [&](int iz, const Pl &q) {
       int pz = iz + 1;
-      T nozp = 0, noym = 0, noyp = 0, nown = 0, nr = 0, neg = 0, nsv = 0;
-      uint32_t nmw = 0;
-      if (iz < zb) load(pz + 1, nozp, noym, noyp, nown, nr, neg, nmw, nsv);
       int p0 = (c + py + pz + 1) & 1;
       // other-colour x-neighbours: (j-1, j) when p0 == 0, (j, j+1) when 1
       T sh = p0 ? __shfl_down(oz_0, 1, 64) : __shfl_up(oz_0, 1, 64);
-      if (p0 ? (lane == 63) : (lane == 0)) sh = eg;
+      if (p0 ? (lane == 63) : (lane == 0)) sh = q.edge;
       int ix = p0 + 2 * j - 1;
       long long ce = meoff + pl * pz;
-      bool active = j < g.W && ix >= ixmin && ix <= ixmax && !((mw >> (ce & 31)) & 1u);
+      bool active = j < g.W && ix >= ixmin && ix <= ixmax && !((q.mw >> (ce & 31)) & 1u);
       if (active) {
         T pxm = p0 ? oz_0 : sh;
         T pxp = p0 ? sh : oz_0;
-        const S nv = St<S>::st(r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic +
-                               (oz_m + ozp) * dzsqic + spm1 * own);
+        const S nv = St<S>::st(q.r + (pxm + pxp) * dxsqic + (q.oym + q.oyp) * dysqic +
+                               (oz_m + q.ozp) * dzsqic + spm1 * q.own);
         me[rowc + pl * pz] = nv;
         if (CONV) {
-          const double d = fabs((double)sv - (double)St<S>::ld(nv));
+          const double d = fabs((double)q.sv - (double)St<S>::ld(nv));
           cmax = d > cmax ? d : cmax;
           save[c * g.cs + rowc + pl * pz] = nv;
         }
       }
       oz_m = oz_0;
-      oz_0 = ozp;
-      ozp = nozp; oym = noym; oyp = noyp; own = nown; r = nr; eg = neg; mw = nmw; sv = nsv;
+      oz_0 = q.ozp;
+    };
+    // two sets of loads in turn (a copy of a set would wait for its loads
+    // before the next set is issued); the last plane re-reads itself into
+    // the idle set (unused)
+    Pl qa, qb;
+    load(za + 1, qa);
+    for (int iz = za; iz <= zb; iz += 2) {
+      load(min(iz + 2, zb + 1), qb);
+      step(iz, qa);
+      if (iz + 1 > zb) break;
+      load(min(iz + 3, zb + 1), qa);
+      step(iz + 1, qb);
     }
     if (CONV) {
       cmax = wave_max(cmax);

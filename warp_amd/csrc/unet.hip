@@ -3,10 +3,15 @@
 // Activations are channels-last bf16, x[(z*H + y)*W + x][C], C a multiple of
 // 16.  A 3x3x3 convolution (zero padding 1) is an implicit GEMM:
 // M = voxels, N = output channels, K = 27 taps x input channels, computed
-// with v_mfma_f32_16x16x16_bf16 (bf16 operands, fp32 accumulation).  Weights
-// are packed per 16-channel input block so that every MFMA operand a lane
-// needs is 8 contiguous bytes:
-//   Wp[((tap*(Cin/16) + cb)*Cout + co)*16 + ci%16].
+// with gfx950's v_mfma_f32_16x16x32_bf16 (bf16 operands, fp32
+// accumulation).  The block stages its output tile plus a one-voxel halo of
+// each 16-channel input block in LDS, so every input value is read from
+// memory once per block rather than once per tap.  One MFMA step takes
+// K = 32 as two taps of one 16-channel block (taps 2q and 2q+1; a 28th,
+// zero tap pads the last step), and the weights are packed so that each
+// lane's B operand is 16 contiguous bytes:
+//   Wq[((cb*14 + q)*Cout + co)*32 + k],  k < 16: tap 2q, channel cb*16 + k,
+//                                        k >= 16: tap 2q+1, channel cb*16 + k-16.
 // The backward pass over the input is the same convolution with the taps
 // mirrored and the channel roles exchanged (a second packing of the same
 // weights); the backward pass over the weights accumulates
@@ -45,82 +50,99 @@ struct Dims {
   int D, H, W;
 };
 
-// Forward convolution.  One wavefront computes VT tiles of 16 consecutive
-// voxels (in x-fastest order; lanes past the last voxel load zeros and
-// store nothing) for all output channels (NT = Cout/16 <= 8 accumulators
-// per tile); each weight operand it loads feeds the MFMAs of all VT tiles.
-// A block is 4 wavefronts.  out = act(conv + bias), act the leaky ReLU of
-// slope `slope` (1 = identity); bias may be NULL.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Forward convolution.  A block (4 wavefronts) owns a tile of 16 x VT x 4
+// output voxels (x, y, z): wavefront w takes z plane z0 + w and its VT
+// rows of 16 voxels, for all NT*16 output channels (VT*NT accumulators of
+// 16 voxels x 16 channels).  Per 16-channel input block the block copies
+// the tile and its one-voxel halo, 18 x (VT+2) x 6 voxels of 16 channels,
+// into LDS (zeros outside the grid), then runs 14 MFMA steps; in step q a
+// lane reads its A operand (voxel m of its row, shifted by tap 2q + kg/2,
+// channels 8*(kg%2)..+7) from LDS and its B operands from the packed
+// weights, each B operand feeding the VT row tiles.  out = act(conv +
+// bias), act the leaky ReLU of slope `slope` (1 = identity); bias may be
+// NULL.
 template <int NT, int VT>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, int cin,
                                                uint16_t *__restrict__ y, float *__restrict__ yf,
-                                               Dims d, const uint16_t *__restrict__ wp,
+                                               Dims d, const uint16_t *__restrict__ wq,
                                                const float *__restrict__ bias, float slope,
-                                               long long nvox) {
-  int lane = threadIdx.x & 63;
-  long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 * VT;
-  if (v0 >= nvox) return;
-  int cout = NT * 16;
-  int m = lane & 15, kg = lane >> 4;
-  // this lane's A-operand voxel of each tile
-  long long vm[VT];
-  int xw[VT], yy[VT], zz[VT];
-#pragma unroll
-  for (int j = 0; j < VT; j++) {
-    vm[j] = v0 + 16 * j + m;
-    xw[j] = (int)(vm[j] % d.W);
-    long long r = vm[j] / d.W;
-    yy[j] = (int)(r % d.H);
-    zz[j] = (int)(r / d.H);
-  }
+                                               int ntx, int nty) {
+  constexpr int HX = 18, HY = VT + 2, HZ = 6, HV = HX * HY * HZ;
+  __shared__ __align__(16) uint16_t tile[HV * 16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  int b = blockIdx.x;
+  const int tx = b % ntx;
+  b /= ntx;
+  const int ty = b % nty, tz = b / nty;
+  const int x0 = tx * 16, y0 = ty * VT, z0 = tz * 4;
+  constexpr int cout = NT * 16;
+  const int ncb = cin >> 4;
   f4 acc[VT][NT];
 #pragma unroll
   for (int j = 0; j < VT; j++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[j][t] = f4{0.f, 0.f, 0.f, 0.f};
-  int ncb = cin >> 4;
-  for (int tap = 0; tap < 27; tap++) {
-    int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const uint16_t *src[VT];
-    bool in[VT];
-#pragma unroll
-    for (int j = 0; j < VT; j++) {
-      int sz = zz[j] + dz, sy = yy[j] + dy, sx = xw[j] + dx;
-      in[j] = vm[j] < nvox && sz >= 0 && sz < d.D && sy >= 0 && sy < d.H && sx >= 0 &&
-              sx < d.W;
-      src[j] = x + (in[j] ? (((long long)sz * d.H + sy) * d.W + sx) * cin : 0) + 4 * kg;
+  const bf16x8 zero8 = {};
+  for (int cb = 0; cb < ncb; cb++) {
+    __syncthreads();  // every wavefront is done with the previous block's tile
+    for (int it = threadIdx.x; it < 2 * HV; it += 256) {
+      const int hv = it >> 1, half = it & 1;
+      const int hx = hv % HX, r = hv / HX, hy = r % HY, hz = r / HY;
+      const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D)
+        v = *(const uint4 *)(x + (((long long)gz * d.H + gy) * d.W + gx) * cin + cb * 16 +
+                             half * 8);
+      *(uint4 *)(tile + hv * 16 + half * 8) = v;
     }
-    const uint16_t *wt = wp + (long long)tap * ncb * cout * 16 + m * 16 + 4 * kg;
-    for (int cb = 0; cb < ncb; cb++) {
-      s4 a[VT];
+    __syncthreads();
+    const uint16_t *wcb = wq + (long long)cb * 14 * cout * 32 + 8 * kg;
+#pragma unroll 2
+    for (int q = 0; q < 14; q++) {
+      const int tap = 2 * q + (kg >> 1);
+      const bool real = tap < 27;
+      const int tt = real ? tap : 26;
+      const int dz = tt / 9, dy = (tt / 3) % 3, dx = tt % 3;  // shifts + 1
+      bf16x8 bq[NT];
 #pragma unroll
-      for (int j = 0; j < VT; j++) a[j] = in[j] ? load4(src[j] + cb * 16) : s4{0, 0, 0, 0};
+      for (int t = 0; t < NT; t++)
+        bq[t] = *(const bf16x8 *)(wcb + ((long long)q * cout + t * 16 + m) * 32);
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        s4 b = load4(wt + ((long long)cb * cout + t * 16) * 16);
+      for (int j = 0; j < VT; j++) {
+        const int hv = ((w + dz) * HY + (j + dy)) * HX + (m + dx);
+        bf16x8 a = *(const bf16x8 *)(tile + hv * 16 + 8 * (kg & 1));
+        if (!real) a = zero8;
 #pragma unroll
-        for (int j = 0; j < VT; j++)
-          acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j], b, acc[j][t], 0, 0, 0);
+        for (int t = 0; t < NT; t++)
+          acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[t], acc[j][t], 0, 0, 0);
       }
     }
   }
-  // lane holds D[4*kg + i][m]: voxel v0 + 16*j + 4*kg + i, channel t*16 + m
+  // lane holds D[4*kg + i][m]: voxel x0 + 4*kg + i of row j, channel t*16 + m
+  const int zz = z0 + w;
 #pragma unroll
-  for (int j = 0; j < VT; j++)
+  for (int j = 0; j < VT; j++) {
+    const int yy = y0 + j;
+    if (zz >= d.D || yy >= d.H) continue;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      int co = t * 16 + m;
-      float bv = bias ? bias[co] : 0.f;
+      const int co = t * 16 + m;
+      const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
+        const int xx = x0 + 4 * kg + i;
+        if (xx >= d.W) continue;
         float v = acc[j][t][i] + bv;
         v = v > 0.f ? v : v * slope;
-        long long vi = v0 + 16 * j + 4 * kg + i;
-        if (vi >= nvox) continue;
+        const long long vi = ((long long)zz * d.H + yy) * d.W + xx;
         if (y) y[vi * cout + co] = f2bf(v);
         if (yf) yf[vi * cout + co] = v;
       }
     }
+  }
 }
 
 // Weight gradient in the master layout: dw[(co*cin + ci)*27 + tap] +=
@@ -348,19 +370,29 @@ __global__ __launch_bounds__(256) void k_act_bwd(uint16_t *__restrict__ g,
 // Pack fp32 master weights w[co][ci][tap] into the forward layout, and the
 // mirrored/transposed layout of the backward-over-input convolution.
 __global__ __launch_bounds__(256) void k_pack(const float *__restrict__ w, int cin, int cout,
-                                              uint16_t *__restrict__ wp,
-                                              uint16_t *__restrict__ wt) {
-  long long n = 27ll * cin * cout;
-  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+                                              uint16_t *__restrict__ wq,
+                                              uint16_t *__restrict__ wqt) {
+  // both layouts hold 28 * cin * cout values (14 tap pairs)
+  const long long n = 28ll * cin * cout;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  int tap = (int)(t % 27);
-  long long u = t / 27;
-  int ci = (int)(u % cin), co = (int)(u / cin);
-  uint16_t b = f2bf(w[t]);
-  wp[(((long long)tap * (cin / 16) + ci / 16) * cout + co) * 16 + ci % 16] = b;
-  // backward over the input: input channels co, output channels ci, tap mirrored
-  int mt = 26 - tap;
-  wt[(((long long)mt * (cout / 16) + co / 16) * cin + ci) * 16 + co % 16] = b;
+  const int k = (int)(t % 32);
+  const long long u = t / 32;
+  const int half = k >> 4, kl = k & 15;
+  {  // forward: u = (cb*14 + q)*cout + co
+    const int co = (int)(u % cout);
+    const long long v = u / cout;
+    const int q = (int)(v % 14), cb = (int)(v / 14);
+    const int tap = 2 * q + half, ci = cb * 16 + kl;
+    wq[t] = tap < 27 ? f2bf(w[((long long)co * cin + ci) * 27 + tap]) : (uint16_t)0;
+  }
+  {  // backward over the input: u = (cob*14 + q)*cin + ci, taps mirrored
+    const int ci = (int)(u % cin);
+    const long long v = u / cin;
+    const int q = (int)(v % 14), cob = (int)(v / 14);
+    const int tap = 2 * q + half, co = cob * 16 + kl;
+    wqt[t] = tap < 27 ? f2bf(w[((long long)co * cin + ci) * 27 + (26 - tap)]) : (uint16_t)0;
+  }
 }
 
 // Average pooling 2x2x2 (forward) and its adjoint.
@@ -678,22 +710,25 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
     WM_REQUIRE(cin % 16 == 0 && cin > 0 && cout % 16 == 0 && cout >= 16 && cout <= 128,
                "wm_unet_conv3: channels must be multiples of 16 (cout <= 128)");
     WM_REQUIRE(D > 0 && H > 0 && W > 0, "wm_unet_conv3: empty grid");
-    long long nvox = D * H * W;
+    WM_REQUIRE(D * H * W < (1ll << 31), "wm_unet_conv3: grid too large");
     Dims d{(int)D, (int)H, (int)W};
-    // 4 voxel tiles per wavefront up to 64 output channels, 2 above
+    // rows of 16 voxels per wavefront: 4 up to 64 output channels, 2 above
     const int vt = cout <= 64 ? 4 : 2;
-    int blocks = (int)((((nvox + 15) / 16 + vt - 1) / vt + 3) / 4);
+    const int ntx = (int)((W + 15) / 16), nty = (int)((H + vt - 1) / vt);
+    const long long nblk = (long long)ntx * nty * ((D + 3) / 4);
+    WM_REQUIRE(nblk < (1ll << 31), "wm_unet_conv3: grid too large");
+    const dim3 g((unsigned)nblk), bl(256);
     hipStream_t s = wm_stream(stream);
     float sl = (float)slope;
     switch (cout / 16) {
-      case 1: hipLaunchKernelGGL((k_conv3<1, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 2: hipLaunchKernelGGL((k_conv3<2, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 3: hipLaunchKernelGGL((k_conv3<3, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 4: hipLaunchKernelGGL((k_conv3<4, 4>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 5: hipLaunchKernelGGL((k_conv3<5, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 6: hipLaunchKernelGGL((k_conv3<6, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 7: hipLaunchKernelGGL((k_conv3<7, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
-      case 8: hipLaunchKernelGGL((k_conv3<8, 2>), dim3(blocks), dim3(256), 0, s, x, (int)cin, y, yf, d, wp, bias, sl, nvox); break;
+      case 1: hipLaunchKernelGGL((k_conv3<1, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 2: hipLaunchKernelGGL((k_conv3<2, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 3: hipLaunchKernelGGL((k_conv3<3, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 4: hipLaunchKernelGGL((k_conv3<4, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 5: hipLaunchKernelGGL((k_conv3<5, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 6: hipLaunchKernelGGL((k_conv3<6, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 7: hipLaunchKernelGGL((k_conv3<7, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 8: hipLaunchKernelGGL((k_conv3<8, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
       default: throw wm_error("wm_unet_conv3: cout > 128");
     }
     WM_LAUNCHED();
@@ -756,11 +791,15 @@ int wm_unet_act_bwd(uint16_t *g, const uint16_t *y, double slope, int64_t n, voi
   });
 }
 
+int64_t wm_unet_pack_size(int64_t cin, int64_t cout) {
+  return cin > 0 && cout > 0 ? 28 * cin * cout : 0;
+}
+
 int wm_unet_pack(const float *w, int64_t cin, int64_t cout, uint16_t *wp, uint16_t *wt,
                  void *stream) {
   WM_ENTRY({
     WM_REQUIRE(cin % 16 == 0 && cout % 16 == 0, "wm_unet_pack: channels must be multiples of 16");
-    long long n = 27ll * cin * cout;
+    long long n = 28ll * cin * cout;
     hipLaunchKernelGGL(k_pack, dim3(nb(n)), dim3(256), 0, wm_stream(stream), w, (int)cin, (int)cout,
                        wp, wt);
     WM_LAUNCHED();

@@ -88,9 +88,11 @@ class MultiGrid3D(FieldSolver):
                    self.dz, b, self.mgmaxlevels)
         self._h = h
         # SOR pass form: 0 two half sweeps (one launch per colour), 1 both
-        # colours in one launch where the level allows it; the same bits
-        if sorform not in (0, 1):
-            raise ValueError("sorform must be 0 (two half sweeps) or 1 (fused pass)")
+        # colours in one launch where the level allows it, 2 that on the
+        # finest level only; the same bits
+        if sorform not in (0, 1, 2):
+            raise ValueError("sorform must be 0 (two half sweeps), 1 (fused pass) or 2 (fused "
+                             "pass on the finest level)")
         self.sorform = sorform
         N.call("wm_mg3d_set_sor_form", h, sorform)
         self.initializeconductors()

@@ -39,7 +39,7 @@ class _Conv:
         self.gb = torch.zeros_like(self.b)
         self.m = [torch.zeros_like(self.w), torch.zeros_like(self.w),
                   torch.zeros_like(self.b), torch.zeros_like(self.b)]
-        n = 27 * cin * cout
+        n = int(N.lib().wm_unet_pack_size(cin, cout))
         self.wp = torch.empty(n, dtype=torch.int16, device=device)
         self.wt = torch.empty(n, dtype=torch.int16, device=device)
         self.pack()
