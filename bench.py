@@ -173,12 +173,14 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    # Copy of known size for scaling the FETCH_SIZE / WRITE_SIZE counters.
-    a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
-    b = torch.empty_like(a)
-    b.copy_(a)
+    # Passes of known size for scaling a profiler's FETCH_SIZE / WRITE_SIZE
+    # counters: 2 GiB read only, then 2 GiB written only (wm_stream_probe).
+    a = torch.zeros(1 << 28, dtype=torch.float64, device=dev)
+    o = torch.zeros(1, dtype=torch.float64, device=dev)
+    N.call("wm_stream_probe", a.data_ptr(), o.data_ptr(), a.numel(), 0, N.stream_of(dev))
+    N.call("wm_stream_probe", a.data_ptr(), o.data_ptr(), a.numel(), 1, N.stream_of(dev))
     torch.cuda.synchronize()
-    del a, b
+    del a, o
 
     cpu = None
     if not args.no_cpu_baseline:

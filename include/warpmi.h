@@ -34,6 +34,13 @@ const char *wm_last_error(void);
  * the current device; returns the number of compute units. */
 int wm_device_info(char *name, int64_t namelen, int64_t *cus);
 
+/* Measurement only (no reference counterpart): streams n doubles of x
+ * through the device, read only (mode 0; out receives nothing for finite
+ * data) or write only (mode 1, x = 1.0), with 8-byte lanes.  bench.py runs
+ * both over 2 GiB so that a profiler's FETCH_SIZE / WRITE_SIZE counters can
+ * be scaled by passes of known size. */
+int wm_stream_probe(double *x, double *out, int64_t n, int64_t mode, void *stream);
+
 /* ---------------------------------------------------------------------
  * Multigrid solver (pywarp90/f3d_mgrid.F)
  * ------------------------------------------------------------------- */

@@ -43,6 +43,7 @@ class WmConductorLists(C.Structure):
 SIGNATURES = {
     "wm_last_error": (C.c_char_p, []),
     "wm_device_info": (C.c_int, [C.c_char_p, _i64, _P]),
+    "wm_stream_probe": (C.c_int, [_P, _P, _i64, _i64, _P]),
     "wm_mg3d_create": (C.c_int, [_P, _i64, _i64, _i64, _d, _d, _d, _P, _i64]),
     "wm_mg3d_destroy": (C.c_int, [_P]),
     "wm_mg3d_nlevels": (C.c_int, [_P, _P]),

@@ -330,3 +330,43 @@ int wm_getese3d(const double *rho, const double *phi, int64_t nx, int64_t ny, in
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Streaming probes for scaling the HBM traffic counters: a read-only pass
+// over n doubles (the sum is written only when it equals a value that zero
+// or finite data never give, so nothing is written) and a write-only pass
+// of n doubles, both with 8-byte lanes like the solver's FP64 kernels.
+namespace {
+
+__global__ __launch_bounds__(256) void k_probe_read(const double *__restrict__ x,
+                                                    double *__restrict__ out, long long n) {
+  double v = 0.;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    v += x[i];
+  if (v == -1.2345678e300) out[0] = v;
+}
+
+__global__ __launch_bounds__(256) void k_probe_write(double *__restrict__ x, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    x[i] = 1.;
+}
+
+}  // namespace
+
+extern "C" int wm_stream_probe(double *x, double *out, int64_t n, int64_t mode, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(x && n > 0, "wm_stream_probe: empty array");
+    WM_REQUIRE(mode == 0 || mode == 1, "wm_stream_probe: mode must be 0 (read) or 1 (write)");
+    hipStream_t s = wm_stream(stream);
+    if (mode == 0) {
+      WM_REQUIRE(out != nullptr, "wm_stream_probe: NULL out");
+      hipLaunchKernelGGL(k_probe_read, dim3(8192), dim3(256), 0, s, (const double *)x, out,
+                         (long long)n);
+    } else {
+      hipLaunchKernelGGL(k_probe_write, dim3(8192), dim3(256), 0, s, x, (long long)n);
+    }
+    WM_LAUNCHED();
+  });
+}
