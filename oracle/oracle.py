@@ -292,12 +292,12 @@ def _getmeshnew(g, lev, extent):
     dy = g["dy"] * lev["ly"]
     dz = g["dz"] * lev["lz"]
     nxl, nyl, nzl = lev["nx"], lev["ny"], lev["nz"]
-    xmin = max(g["xmmin"], g["xmmin"] + 0 * dx)
-    xmax = min(g["xmmax"], g["xmmin"] + (0 + nxl) * dx)
-    ymin = max(g["ymmin"], g["ymmin"] + 0 * dy)
-    ymax = min(g["ymmax"], g["ymmin"] + (0 + nyl) * dy)
-    zmin = max(g["zmmin"], g["zmmin"] + 0 * dz + 0.)
-    zmax = min(g["zmmax"], g["zmmin"] + (0 + nzl) * dz + 0.)
+    xmin = max(g["xmin"], g["xmmin"] + 0 * dx)
+    xmax = min(g["xmax"], g["xmmin"] + (0 + nxl) * dx)
+    ymin = max(g["ymin"], g["ymmin"] + 0 * dy)
+    ymax = min(g["ymax"], g["ymmin"] + (0 + nyl) * dy)
+    zmin = max(g["zmin"], g["zmmin"] + 0 * dz + 0.)
+    zmax = min(g["zmax"], g["zmmin"] + (0 + nzl) * dz + 0.)
     mins, maxs = extent
     if mins[0] - dx > xmin:
         xmin = int((mins[0] - dx - xmin) / dx) * dx + xmin
@@ -335,10 +335,10 @@ def _checkoverlap(g, lev, extent):
     dy = g["dy"] * lev["ly"]
     dz = g["dz"] * lev["lz"]
     mins, maxs = extent
-    xmin = max(mins[0], g["xmmin"]); ymin = max(mins[1], g["ymmin"])
-    zmin = max(mins[2], g["zmmin"])
-    xmax = min(maxs[0], g["xmmax"]); ymax = min(maxs[1], g["ymmax"])
-    zmax = min(maxs[2], g["zmmax"])
+    xmin = max(mins[0], g["xmin"]); ymin = max(mins[1], g["ymin"])
+    zmin = max(mins[2], g["zmin"])
+    xmax = min(maxs[0], g["xmax"]); ymax = min(maxs[1], g["ymax"])
+    zmax = min(maxs[2], g["zmax"])
     if zmin - dz > zmax or xmin - dx > xmax or ymin - dy > ymax:
         return False
     return True
@@ -394,14 +394,20 @@ def _intercepts(geom, mesh, mglevel):
 
 
 def install_conductors(geoms, nx, ny, nz, xmmin, xmmax, ymmin, ymmax,
-                       zmmin, zmmax, dfill=2.0, conductors=None):
+                       zmmin, zmmax, dfill=2.0, conductors=None, crop=None):
     """installconductor + Grid.getdatanew + GridIntercepts.installlist
-    (generateconductors.py:135-215, 2750-2858, 1851-1948), serial."""
+    (generateconductors.py:135-215, 2750-2858, 1851-1948), serial; crop =
+    (xmin, xmax, ymin, ymax, zmin, zmax), entries None for the mesh's own
+    (installconductor's region arguments, generateconductors.py:2421-2426)."""
     if conductors is None:
         conductors = Conductors()
     g = dict(xmmin=xmmin, xmmax=xmmax, ymmin=ymmin, ymmax=ymmax, zmmin=zmmin,
              zmmax=zmmax, dx=(xmmax - xmmin) / nx, dy=(ymmax - ymmin) / ny,
              dz=(zmmax - zmmin) / nz)
+    c = crop if crop is not None else (None,) * 6
+    for k, v, dflt in zip(("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"), c,
+                          (xmmin, xmmax, ymmin, ymmax, zmmin, zmmax)):
+        g[k] = dflt if v is None else v
     levels = getmglevels(nx, ny, nz, g["dx"], g["dy"], g["dz"])
     fuzz = 1.e-13
     for geom in geoms:

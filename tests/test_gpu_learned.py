@@ -88,8 +88,8 @@ def test_learned_guess_saves_vcycles(device):
           % (len(SEEDS), NSTEPS, mean["previous"], mean["linear"], mean["learned"], applied,
              fallbacks, 100. * fallbacks / max(1, applied + fallbacks)))
     # the network's own gain: fewer V-cycles than the extrapolation it
-    # corrects (0.39 fewer per solve when the network was committed), and
-    # together with it at least one fewer than the reference's guess
-    assert mean["learned"] <= mean["linear"] - 0.2
-    assert mean["learned"] <= mean["previous"] - 1.0
+    # corrects (0.91 fewer per solve when the network was committed), and
+    # together with it at least 1.5 fewer than the reference's guess
+    assert mean["learned"] <= mean["linear"] - 0.5
+    assert mean["learned"] <= mean["previous"] - 1.5
     assert applied > fallbacks

@@ -322,3 +322,32 @@ def test_quadrupole_channel_solve_bitwise(device):
     it, err = O.multigrid3dsolve(po, rho.copy(), s.dx, s.dy, s.dz, conductors=co)
     assert s.mgiters == it and s.mgerror == err
     np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+
+
+def test_cropped_conductor_solve_bitwise(device):
+    """installconductor with a region (the annulus generated only for
+    z in [3.1, 5] cm, the pipe everywhere): full FP64 solve bitwise equal to
+    the oracle's with the same cropped lists."""
+    import warp_amd
+    nx, ny, nz = 32, 32, 64
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    s = _solver(nx, ny, nz, lo, hi, (0,) * 6)
+    pipe, ann = H.pipe_and_annulus(warp_amd)
+    s.installconductor(pipe)
+    s.installconductor(ann, zmin=0.031, zmax=0.05)
+    rng = np.random.default_rng(41)
+    x, y = rng.normal(0, 0.004, 20000), rng.normal(0, 0.004, 20000)
+    z = 0.04 + rng.normal(0, 0.015, 20000)
+    k = (np.abs(x) < 0.0199) & (np.abs(y) < 0.0199) & (z > 0) & (z < 0.0799)
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    O.setrho3d(rho, x[k], y[k], z[k], 1.602176565e-19, 1e5, s.dx, s.dy, s.dz, *lo)
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    opipe, oann = H.pipe_and_annulus(O)
+    co = O.install_conductors([opipe], nx, ny, nz, *H.DOMAIN)
+    co = O.install_conductors([oann], nx, ny, nz, *H.DOMAIN, conductors=co,
+                              crop=(None, None, None, None, 0.031, 0.05))
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    it, err = O.multigrid3dsolve(po, rho.copy(), s.dx, s.dy, s.dz, conductors=co)
+    assert s.mgiters == it and s.mgerror == err
+    np.testing.assert_array_equal(s.phi.cpu().numpy(), po)

@@ -109,6 +109,38 @@ def test_deposit_runs_with_outside_particles(device):
     assert np.abs(rho.cpu().numpy() - ro).max() <= 1e-13 * np.abs(ro).max()
 
 
+@pytest.mark.parametrize("mesh", [(8, 8, 16), (64, 64, 128), (256, 256, 512)])
+def test_particle_sort_key_widths(device, mesh):
+    """The hand-written radix sort over keys of 10, 20 and 26 bits (2, 3
+    and 4 passes of 8-bit digits), 1.3M particles with many equal keys
+    (positions drawn on a coarse lattice plus particles outside the mesh):
+    the permutation of a stable sort by the reference's block index."""
+    from warp_amd import _native as N
+    nx, ny, nz = mesh
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    dx, dy, dz = [(b - a) / k for a, b, k in zip(lo, hi, mesh)]
+    rng = np.random.default_rng(31)
+    n = 1_300_000
+    x = lo[0] + np.round(rng.uniform(-0.1, 1.1, n) * 40) / 40 * (hi[0] - lo[0])
+    y = lo[1] + rng.uniform(0, 1, n) * (hi[1] - lo[1])
+    z = lo[2] + rng.uniform(0, 1, n) * (hi[2] - lo[2])
+    fx = np.abs(x - lo[0]) / dx
+    fy = np.abs(y - lo[1]) / dy
+    fz = np.abs(z - lo[2]) / dz
+    ix = np.where(fx < nx, fx.astype(np.int64), nx)
+    iy = np.where(fy < ny, fy.astype(np.int64), ny)
+    iz = np.where(fz < nz, fz.astype(np.int64), nz)
+    key = ix + (nx + 1) * (iy + (ny + 1) * iz)
+    want = np.argsort(key, kind="stable")
+    perm = torch.empty(n, dtype=torch.int32, device=device)
+    work = torch.empty(N.lib().wm_particle_sort_worksize(n, nx, ny, nz), dtype=torch.uint8,
+                       device=device)
+    g = [_t(v, device) for v in (x, y, z)]
+    N.call("wm_particle_sort", n, *[a.data_ptr() for a in g], *lo, dx, dy, dz, nx, ny, nz,
+           perm.data_ptr(), work.data_ptr(), N.stream_of(device))
+    np.testing.assert_array_equal(perm.cpu().numpy(), want)
+
+
 def test_particle_sort_matches_counting_sort(device):
     """particlesortxyzwithcopy: the permutation equals a stable sort by the
     reference's block index int(|x-xmmin|/dx) + (nx+1)*(iy + (ny+1)*iz),

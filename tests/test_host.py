@@ -46,8 +46,8 @@ def _product_lists(geoms, n, dom):
     return cd.lists()
 
 
-def _oracle_lists(geoms, n, dom):
-    co = O.install_conductors(geoms, *n, *dom)
+def _oracle_lists(geoms, n, dom, crop=None):
+    co = O.install_conductors(geoms, *n, *dom, crop=crop)
     d = ((dom[1] - dom[0]) / n[0], (dom[3] - dom[2]) / n[1], (dom[5] - dom[4]) / n[2])
     O.checkconductors(co, *n, *d)
     return co.lists()
@@ -71,6 +71,62 @@ def test_product_conductors_equal_oracle(kind, n):
         for k in ("indx", "dels", "volt", "istart"):
             np.testing.assert_array_equal(a["sub%d_%s" % (p, k)], b["sub%d_%s" % (p, k)])
     assert len(a["sub0_dels"]) > 0 and len(a["int_volt"]) > 0
+
+
+CROPS = [(None, None, None, None, 0.031, 0.05), (-0.005, 0.012, None, 0.0, None, None),
+         (None, None, None, None, 0.0326, 0.0329)]
+
+
+@pytest.mark.parametrize("crop", CROPS)
+def test_cropped_conductors_equal_oracle(crop):
+    """installconductor(..., xmin..zmax): the data generated only within the
+    given region (part of the annulus and of the pipe), identical to the
+    oracle's; finest-level points inside the region."""
+    import warp_amd
+    from warp_amd import configs
+    from warp_amd.generateconductors import ConductorData, Grid, installconductor
+    n = (64, 64, 128)
+    dom = configs.DOMAIN
+    nx, ny, nz = n
+    g = Grid(nx, ny, nz, *dom, xmin=crop[0], xmax=crop[1], ymin=crop[2], ymax=crop[3],
+             zmin=crop[4], zmax=crop[5])
+    cd = ConductorData(nx, ny, nz, g.dx, g.dy, g.dz)
+    for c in configs.conductors("pipe_annulus", warp_amd):
+        installconductor(c, g, cd)
+    a = cd.lists()
+    b = _oracle_lists(configs.conductors("pipe_annulus", O), n, dom, crop=crop)
+    full = _oracle_lists(configs.conductors("pipe_annulus", O), n, dom)
+    np.testing.assert_array_equal(a["int_istart"], b["int_istart"])
+    np.testing.assert_array_equal(a["int_indx"], b["int_indx"])
+    np.testing.assert_array_equal(a["int_volt"], b["int_volt"])
+    for p in (0, 1):
+        for k in ("indx", "dels", "volt", "istart"):
+            np.testing.assert_array_equal(a["sub%d_%s" % (p, k)], b["sub%d_%s" % (p, k)])
+    assert 0 < len(a["int_volt"]) < len(full["int_volt"])
+    # on the finest level every point lies in the region, widened by the
+    # cells getmeshnew adds around the conductor's extent
+    e = np.asarray(a["int_istart"])[1] if len(a["int_istart"]) > 1 else len(a["int_volt"])
+    ii = np.asarray(a["int_indx"]).reshape(-1, 3)[:e]
+    lo = np.array(dom[0::2]); d = (np.array(dom[1::2]) - lo) / np.array(n)
+    pos = lo + ii * d
+    for ax in range(3):
+        if crop[2 * ax] is not None:
+            assert (pos[:, ax] >= crop[2 * ax] - 2 * d[ax] - 1e-12).all()
+        if crop[2 * ax + 1] is not None:
+            assert (pos[:, ax] <= crop[2 * ax + 1] + 2 * d[ax] + 1e-12).all()
+
+
+def test_solver_records_the_crop():
+    """MultiGrid3D.installconductor keeps the region arguments until the
+    data is generated (multigrid.py:743-751)."""
+    import warp_amd
+    from warp_amd import configs
+    from warp_amd.multigrid import MultiGrid3D
+    s = MultiGrid3D.__new__(MultiGrid3D)
+    s.conductordatalist = []
+    c = configs.conductors("pipe_annulus", warp_amd)[1]
+    MultiGrid3D.installconductor(s, c, zmin=0.031, zmax=0.05, dfill=1.5)
+    assert s.conductordatalist == [(c, None, None, None, None, 0.031, 0.05, 1.5)]
 
 
 def test_product_mglevels_equal_oracle():

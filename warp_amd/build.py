@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["mg3d.hip", "particles.hip", "lostpart.hip", "condfield.hip", "fielddiag.hip",
-               "unet.hip"]
+               "unet.hip", "sort.hip"]
 CXX_SOURCES = ["conductors.cpp", "abi.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",

@@ -9,7 +9,7 @@
 #include "condfield.h"
 #include "../../include/warpmi.h"
 
-#include <hipcub/hipcub.hpp>
+#include "sort.h"
 
 #include <algorithm>
 
@@ -686,13 +686,8 @@ struct SortedWork {
 };
 
 static SortedWork sorted_layout(int64_t np, int64_t ncell) {
+  (void)ncell;
   SortedWork w{};
-  size_t temp = 0;
-  int endbit = 1;
-  while ((1ll << endbit) <= ncell) endbit++;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (unsigned int *)nullptr,
-                                           (unsigned int *)nullptr, (unsigned int *)nullptr,
-                                           (unsigned int *)nullptr, (int)np, 0, endbit);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t n4 = al(sizeof(unsigned int) * (size_t)std::max<int64_t>(np, 1));
   w.keys = 0;
@@ -700,8 +695,15 @@ static SortedWork sorted_layout(int64_t np, int64_t ncell) {
   w.keys2 = 2 * n4;
   w.idx2 = 3 * n4;
   w.temp = 4 * n4;
-  w.total = 4 * n4 + al(temp);
+  w.total = 4 * n4 + al(radix_sort_worksize(np));
   return w;
+}
+
+// Bits of the largest key ncell (out-of-grid particles get it).
+static int key_bits(int64_t ncell) {
+  int endbit = 1;
+  while ((1ll << endbit) <= ncell) endbit++;
+  return endbit;
 }
 
 }  // namespace
@@ -763,13 +765,10 @@ int wm_setrho3d_sorted(double *rho, int64_t np, const double *xp, const double *
     hipLaunchKernelGGL(k_keys, dim3(nblk(np)), dim3(256), 0, s, (long long)np, xp, yp, zp, m, keys,
                        idx);
     WM_LAUNCHED();
-    size_t temp = w.total - w.temp;
-    int endbit = 1;
-    while ((1ll << endbit) <= ncell) endbit++;
-    WM_HIP(hipcub::DeviceRadixSort::SortPairs(base + w.temp, temp, keys, keys2, idx, idx2, (int)np,
-                                              0, endbit, s));
+    unsigned int *sorted_idx = nullptr;
+    radix_sort_pairs(keys, idx, keys2, idx2, np, key_bits(ncell), base + w.temp, s, &sorted_idx);
     hipLaunchKernelGGL(k_setrho_sorted, dim3(nblk(np)), dim3(256), 0, s, rho, (long long)np, xp, yp,
-                       zp, m, g, idx2);
+                       zp, m, g, (const unsigned int *)sorted_idx);
     WM_LAUNCHED();
   });
 }
@@ -989,13 +988,14 @@ int wm_particle_sort(int64_t np, const double *xp, const double *yp, const doubl
     hipLaunchKernelGGL(k_sortkeys, dim3(nblk(np)), dim3(256), 0, s, (long long)np, xp, yp, zp,
                        xmmin, ymmin, zmmin, dx, dy, dz, (int)nx, (int)ny, (int)nz, keys, idx);
     WM_LAUNCHED();
-    size_t temp = w.total - w.temp;
-    int endbit = 1;
-    while ((1ll << endbit) <= nkey) endbit++;
-    // LSD radix sort is stable: equal keys keep their order, as in the
-    // reference's counting sort (top.F:5156-5174).
-    WM_HIP(hipcub::DeviceRadixSort::SortPairs(base + w.temp, temp, keys, keys2, idx,
-                                              (unsigned int *)perm, (int)np, 0, endbit, s));
+    // the LSD radix sort is stable: equal keys keep their order, as in the
+    // reference's counting sort (top.F:5156-5174)
+    unsigned int *sorted_idx = nullptr;
+    radix_sort_pairs(keys, idx, keys2, (unsigned int *)perm, np, key_bits(nkey), base + w.temp,
+                     s, &sorted_idx);
+    if (sorted_idx != (unsigned int *)perm)
+      WM_HIP(hipMemcpyAsync(perm, sorted_idx, sizeof(unsigned int) * (size_t)np,
+                            hipMemcpyDeviceToDevice, s));
   });
 }
 

@@ -211,15 +211,19 @@ class Grid:
     """Mesh description used to generate conductor data
     (generateconductors.py:2339-2660), serial, zbeam = 0."""
 
-    def __init__(self, nx, ny, nz, xmmin, xmmax, ymmin, ymmax, zmmin, zmmax, zbeam=0.):
+    def __init__(self, nx, ny, nz, xmmin, xmmax, ymmin, ymmax, zmmin, zmmax, zbeam=0.,
+                 xmin=None, xmax=None, ymin=None, ymax=None, zmin=None, zmax=None):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.xmmin, self.xmmax = xmmin, xmmax
         self.ymmin, self.ymmax = ymmin, ymmax
         self.zmmin, self.zmmax = zmmin, zmmax
         self.zbeam = zbeam
-        self.xmin, self.xmax = xmmin, xmmax
-        self.ymin, self.ymax = ymmin, ymmax
-        self.zmin, self.zmax = zmmin + zbeam, zmmax + zbeam
+        # the region conductor data is generated in: the mesh unless the
+        # caller crops it (generateconductors.py:2421-2426)
+        d = lambda v, default: default if v is None else v
+        self.xmin, self.xmax = d(xmin, xmmin), d(xmax, xmmax)
+        self.ymin, self.ymax = d(ymin, ymmin), d(ymax, ymmax)
+        self.zmin, self.zmax = d(zmin, zmmin + zbeam), d(zmax, zmmax + zbeam)
         self.dx = (xmmax - xmmin) / nx
         self.dy = (ymmax - ymmin) / ny
         self.dz = (zmmax - zmmin) / nz

@@ -221,9 +221,9 @@ class MultiGrid3D(FieldSolver):
     def installconductor(self, conductor, xmin=None, xmax=None, ymin=None, ymax=None,
                          zmin=None, zmax=None, dfill=None):
         'Install the conductor into the field solver'
-        if any(v is not None for v in (xmin, xmax, ymin, ymax, zmin, zmax)):
-            raise NotImplementedError("cropping the conductor extent is not supported")
-        self.conductordatalist.append((conductor, dfill))
+        # as the reference, only recorded here; the data is generated when
+        # it is needed (getconductorobject), within xmin..zmax when given
+        self.conductordatalist.append((conductor, xmin, xmax, ymin, ymax, zmin, zmax, dfill))
 
     def uninstallconductor(self, conductor):
         n = len(self.conductordatalist)
@@ -254,9 +254,10 @@ class MultiGrid3D(FieldSolver):
                                                  self.dz)
                 pending = list(self.conductordatalist)
                 self._installedconductors = []
-            grid = Grid(self.nx, self.ny, self.nz, self.xmmin, self.xmmax, self.ymmin,
-                        self.ymmax, self.zmmin, self.zmmax)
-            for conductor, dfill in pending:
+            for conductor, xmin, xmax, ymin, ymax, zmin, zmax, dfill in pending:
+                grid = Grid(self.nx, self.ny, self.nz, self.xmmin, self.xmmax, self.ymmin,
+                            self.ymmax, self.zmmin, self.zmmax, xmin=xmin, xmax=xmax,
+                            ymin=ymin, ymax=ymax, zmin=zmin, zmax=zmax)
                 _install(conductor, grid, self._conductors, dfill if dfill is not None else 2.0)
                 self._installedconductors.append(conductor)
             view = self._conductors.view()

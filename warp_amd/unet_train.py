@@ -12,7 +12,8 @@ conductor fraction) and the target, the error of the linear extrapolation
 cphys * max|d2|.  Training: Adam on the mean-square error, one sample per
 step, through the library's forward and backward kernels.
 
-    python -m warp_amd.unet_train --out warp_amd/data/warmstart_c3.npz
+    python -m warp_amd.unet_train --out warp_amd/data/warmstart_c3.npz \
+        --trajectories 800 --max-samples 12000 --train-steps 20000
 """
 import argparse
 import math
@@ -202,9 +203,12 @@ def main():
     log("cphys %.4e from %d samples (%d held out)" % (cphys, len(samples), len(hold)))
     lg = LearnedGuess(next(iter(solvers.values())), UNet3D(dev, seed=a.seed), cphys)
     hist, ratio = train(lg, samples, a.train_steps, lr=a.lr, seed=a.seed, holdout=hold[::4])
-    nsolves = a.trajectories * (a.steps_per_run + 1)
+    # runs used: collection stops once max_samples are in (each run gives
+    # steps_per_run - 1 samples, from its third solve on)
+    nruns = min(a.trajectories, -(-len(samples) // max(1, a.steps_per_run - 1)))
     save_weights(a.out, lg.net, cphys,
-                 meta=dict(samples=len(samples), solves=nsolves, train_steps=a.train_steps,
+                 meta=dict(samples=len(samples), runs=nruns,
+                           solves=nruns * (a.steps_per_run + 1), train_steps=a.train_steps,
                            heldout_ratio=ratio, mesh=[128, 128, 256]))
     log("saved %s: held-out error / extrapolation error (mean square) %.4f" % (a.out, ratio))
 
