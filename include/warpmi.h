@@ -122,6 +122,14 @@ int wm_mg3d_reuse_phi(wm_mg3d *h);
  * else 0. */
 int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused);
 
+/* No counterpart in the reference: *level = the first level of the FP64
+ * V-cycle's coarse end that the last solve ran as one single-workgroup
+ * launch with the levels' arrays in LDS (the levels from there to the
+ * coarsest), or -1 when it ran level by level.  The environment variable
+ * WARPMI_TAIL_LDS (bytes; 0 switches the single launch off) caps the LDS
+ * budget that decides the level, read when the solver is created. */
+int wm_mg3d_tail_start(const wm_mg3d *h, int64_t *level);
+
 /* Form of the SOR passes (sorpass3d, f3d_mgrid.F:1949-2093): 0 = two
  * half sweeps, one launch per colour (the default); 1 = both colours'
  * dense updates in one launch on every level where that applies (six

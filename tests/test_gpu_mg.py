@@ -351,3 +351,44 @@ def test_cropped_conductor_solve_bitwise(device):
     it, err = O.multigrid3dsolve(po, rho.copy(), s.dx, s.dy, s.dz, conductors=co)
     assert s.mgiters == it and s.mgerror == err
     np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
+
+
+@pytest.mark.parametrize("case", ["pipe_annulus", "quads_periodic", "neumann_periodic"])
+def test_vcycle_tail_same_bits(device, case, monkeypatch):
+    """The coarse end of the V-cycle as one launch (levels in LDS) against
+    the level-by-level launches: the same phi bit for bit and the same
+    V-cycle counts, cold and warm; the single launch did run."""
+    import warp_amd
+    from warp_amd import configs
+    nx, ny, nz = 64, 64, 128
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    bounds = {"pipe_annulus": (0,) * 6, "quads_periodic": (0, 0, 0, 0, 2, 2),
+              "neumann_periodic": (1, 1, 0, 1, 2, 2)}[case]
+    conds = (H.pipe_and_annulus(warp_amd) if case == "pipe_annulus" else
+             configs.conductors("quads", warp_amd) if case == "quads_periodic" else [])
+    rng = np.random.default_rng(31)
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    x, y = rng.normal(0, 0.004, 30000), rng.normal(0, 0.004, 30000)
+    z = 0.04 + rng.normal(0, 0.015, 30000)
+    k = (np.abs(x) < 0.0199) & (np.abs(y) < 0.0199) & (z > 0) & (z < 0.0799)
+    out = []
+    for budget in ("0", None):
+        if budget is None:
+            monkeypatch.delenv("WARPMI_TAIL_LDS", raising=False)
+        else:
+            monkeypatch.setenv("WARPMI_TAIL_LDS", budget)
+        s = _solver(nx, ny, nz, lo, hi, bounds)
+        for g in conds:
+            s.installconductor(g)
+        if not out:
+            O.setrho3d(rho, x[k], y[k], z[k], 1.602176565e-19, 5e5, s.dx, s.dy, s.dz, *lo)
+        res = []
+        for f in (1.0, 1.1):
+            s.rho = torch.from_numpy(rho * f).to(device)
+            s.solve()
+            res.append((s.mgiters, s.mgerror, s.phi.cpu().numpy().copy(), s.tail_start_level()))
+        out.append(res)
+    for a, b in zip(*out):
+        assert a[3] == -1 and b[3] >= 1, (a[3], b[3])
+        assert (a[0], a[1]) == (b[0], b[1])
+        np.testing.assert_array_equal(a[2], b[2])

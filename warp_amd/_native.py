@@ -97,6 +97,7 @@ SIGNATURES = {
     "wm_getzmmnt": (C.c_int, [_i64, _P, _P, _P, _P, _P, _P, _P, _i64, _P, _P, _P]),
     "wm_mg3d_reuse_phi": (C.c_int, [_P]),
     "wm_mg3d_last_reused": (C.c_int, [_P, _P]),
+    "wm_mg3d_tail_start": (C.c_int, [_P, _P]),
     "wm_gtlchg3d": (C.c_int, [_P, _i64, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _d, _i64, _i64,
                               _i64, _P, _P, _P]),
     "wm_getese3d_worksize": (_i64, [_i64, _i64]),

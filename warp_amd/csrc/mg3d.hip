@@ -10,6 +10,7 @@
 #include "wm_internal.h"
 #include "mg3d_kernels.h"
 #include "mg3d_fused.h"
+#include "mg3d_tail.h"
 #include "../../include/warpmi.h"
 
 #include <hip/hip_bf16.h>
@@ -274,6 +275,25 @@ struct wm_mg3d {
   // (wm_mg3d_set_sor_form; WARPMI_SOR_FUSED=1 or 2 sets the default)
   int sor_form = getenv("WARPMI_SOR_FUSED") ? std::min(2, std::max(0, atoi(getenv("WARPMI_SOR_FUSED")))) : 0;
   bool conductors_set = false;
+  // The coarse end of the FP64 V-cycle in one launch (k_vtail): levels
+  // start..nlevels-1, described by the device table tailtab (fixed address,
+  // contents rewritten between solves when they change).  The levels whose
+  // arrays fit in tail_lds bytes of LDS qualify (WARPMI_TAIL_LDS sets a
+  // smaller budget; 0 switches the tail off).
+  struct Tail {
+    bool on = false;
+    int start = -1;
+    TailRun run{};
+    std::vector<TailLevel> host;  // contents of tailtab
+    std::vector<TailItem> items;  // contents of tailitems
+    std::vector<TailOp> ops;      // contents of tailops
+    int down = -1, up = -1;       // the pass counts the steps are for
+  } tail;
+  DevBuf<TailLevel> tailtab;
+  DevBuf<TailItem> tailitems;
+  DevBuf<TailOp> tailops;
+  bool tail_used = false;  // the last solve was FP64 and ran the tail
+  size_t tail_lds = 0;
   int ncu = 256;                 // compute units of the device
   size_t lds_per_cu = 160 * 1024;
   // Event timing of the finest-level SOR half sweeps (eager mode only).
@@ -1303,6 +1323,210 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
                    bool zeroed = false, bool repin = true, double *cpart = nullptr,
                    double *cpart_end = nullptr, int *nslots = nullptr);
 
+// The FP64 V-cycle's tail (k_vtail) for the parameters P: the first level
+// from 1 on with at most tail_nodes nodes, and every level from there on
+// using the half sweeps (the fused pass swaps buffers, which the table does
+// not follow).  Writes the table (synchronously: no V-cycle of this solver
+// is in flight between solves) when its contents change.
+static void prepare_tail(wm_mg3d *h, const Params &P) {
+  h->tail.on = false;
+  const int nlev = (int)h->lv.size();
+  if (h->tail_lds == 0) return;
+  for (int l = 1; l < nlev; l++)
+    if (fused_ok(h, *h->lv[l], l, P) || !h->lv[l]->phi64.p || !h->lv[l]->rho64.p ||
+        (h->lv[l]->coarser && !h->lv[l]->res64.p))
+      return;
+  const int *b = h->bounds;
+  const bool periodic = b[0] == 2 || b[2] == 2 || b[4] == 2;
+  const SubList<double> none{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // The tables of the levels from `start` on and their gather list; the
+  // start moves up a level at a time until everything fits the LDS budget
+  // and the item list.
+  for (int start = 1; start < nlev; start++) {
+    std::vector<TailLevel> tab;
+    std::vector<TailItem> items;
+    int off = 0, gw = 0;
+    // an LDS region of `words` (even offset); gathered from src when given
+    auto region = [&](const void *src, long long words) -> int {
+      const int o = off;
+      off += (int)((words + 1) & ~1ll);
+      if (src && words > 0) {
+        items.push_back(TailItem{(const uint32_t *)src, o, (int)words, gw, 0});
+        gw += (int)((words + TAIL_GB - 1) / TAIL_GB * TAIL_GB);
+      }
+      return o;
+    };
+    auto sub = [&](const SubList<double> &v, bool residual) {
+      TailSub t{v.n, 0, 0, 0, 0, 0, 0};
+      if (v.n == 0) return t;
+      t.pos = region(v.pos, 3ll * v.n);
+      t.modes = region(v.modes, v.n);
+      t.k = region(v.k, 12ll * v.n);
+      t.denom = region(v.denom, 2ll * v.n);
+      if (residual) {
+        t.fac = region(v.fac, 2ll * v.n);
+        t.zero = region(v.zero, v.n);
+      }
+      return t;
+    };
+    for (int l = start; l < nlev; l++) {
+      Level &L = *h->lv[l];
+      TailLevel t;
+      std::memset(&t, 0, sizeof(t));
+      t.g = L.g;
+      t.phi = L.phi64.p;
+      const long long n2 = 2 * L.g.cs;
+      // level arrays: only the first level's rho comes from HBM (its phi is
+      // zero, the others' phi and rho are written by the restriction, and
+      // the points of res that no residual writes are zero)
+      t.lphi = region(nullptr, 2 * n2);
+      t.lrho = region(l == start ? (const void *)L.rho64.p : nullptr, 2 * n2);
+      if (L.coarser) t.lres = region(nullptr, 2 * n2);
+      LoopRange sr = sor_range(L, b), rr = res_range(L, b);
+      const int srv[6] = {sr.ixmin, sr.ixmax, sr.iymin, sr.iymax, sr.izmin, sr.izmax};
+      const int rrv[6] = {rr.ixmin, rr.ixmax, rr.iymin, rr.iymax, rr.izmin, rr.izmax};
+      for (int i = 0; i < 6; i++) {
+        t.sr[i] = srv[i];
+        t.rr[i] = rrv[i];
+      }
+      // padded element and row counts of the ranges (powers of two)
+      auto lg2 = [](int n) {
+        int k = 0;
+        while ((1 << k) < n) k++;
+        return k;
+      };
+      t.ssh[0] = lg2((sr.ixmax + 1) / 2 - (sr.ixmin + 1) / 2 + 1);
+      t.ssh[1] = lg2(sr.iymax - sr.iymin + 1);
+      t.rsh[0] = lg2((rr.ixmax + 1) / 2 - (rr.ixmin + 1) / 2 + 1);
+      t.rsh[1] = lg2(rr.iymax - rr.iymin + 1);
+      // the coefficients as halfpass and residual_rb form them
+      const double cst = P.mgparam * 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+      t.sc[0] = L.dxsqi * cst;
+      t.sc[1] = L.dysqi * cst;
+      t.sc[2] = L.dzsqi * cst;
+      t.sc[3] = 1. - P.mgparam;
+      const double cr = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
+      t.rc[0] = L.dxsqi * P.mgparam * cr;
+      t.rc[1] = L.dysqi * P.mgparam * cr;
+      t.rc[2] = L.dzsqi * P.mgparam * cr;
+      t.rs[0] = P.mgparam * L.dxsqi * cr;
+      t.rs[1] = P.mgparam * L.dysqi * cr;
+      t.rs[2] = P.mgparam * L.dzsqi * cr;
+      CondDev<double> &C = cond_of<double>(L);
+      int f = 0;
+      const bool mirror = P.icnd == 2 && L.neumann;
+      if (L.sub_on_interior || periodic || mirror) f |= TAIL_PIN_AFTER;
+      if (mirror) f |= TAIL_MIRROR;
+      if (L.sub_on_interior && L.nint > 0) f |= TAIL_SPLIT;
+      if (L.coarser && P.up > 0 && pin_fusable(h, L, l, P)) f |= TAIL_FUSEPIN;
+      if (L.hasmask) f |= TAIL_HASMASK;
+      if (!(L.sub_on_interior && L.nint > 0) && !C.res[0][0].anyzero && !C.res[0][1].anyzero)
+        f |= TAIL_RES1;
+      t.flags = f;
+      if (L.hasmask) t.lmask = region(L.mask.p, (long long)L.mask.n);
+      if (f & TAIL_FUSEPIN) t.limask = region(L.imask.p, (long long)L.imask.n);
+      t.nint = L.nint;
+      if (L.nint > 0) t.lint = region(L.int_pos.p, 3ll * L.nint);
+      for (int c = 0; c < 2; c++) {
+        t.sor[c] = sub(P.lcnd ? C.sor[c].view() : none, false);
+        t.rsub[c] = sub(L.hasmask && L.coarser ? C.res[0][c].view() : none, true);
+      }
+      for (int p = 0; p < 2; p++) {
+        t.nsub[p] = (int)(L.subpos_h[p].size() / 3);
+        t.subpos[p] = L.subpos_d[p].p;
+        t.subdels[p] = L.subdels_d[p].p;
+      }
+      if (L.coarser) {
+        for (int a = 0; a < 3; a++) {
+          t.ra[a] = TailAxisR{region(L.rlo[a].p, (long long)L.rlo[a].n),
+                              region(L.rhi[a].p, (long long)L.rhi[a].n),
+                              region(L.rw[a].p, 2ll * (long long)L.rw[a].n), L.rcmin[a],
+                              L.rcmax[a]};
+          t.ea[a] = TailAxisE{region(L.ej[a].p, (long long)L.ej[a].n),
+                              region(L.ew[a].p, 2ll * (long long)L.ew[a].n), L.elo[a], L.ehi[a]};
+        }
+        t.ff = L.ff;
+      }
+      tab.push_back(t);
+    }
+    if ((size_t)off * 4 > h->tail_lds || (int)items.size() > TAIL_MAX_ITEMS) continue;
+    // the steps, in the order vcycle and its callees launch them for these
+    // levels (phi of the first level starts at zero)
+    std::vector<TailOp> ops;
+    const int nl = (int)tab.size();
+    const bool anybc = b[0] || b[1] || b[2] || b[3] || b[4] || b[5];
+    auto pass = [&](int l, bool pin) {
+      const TailLevel &t = tab[l];
+      const bool sweeps = t.sr[3] >= t.sr[2] && t.sr[5] >= t.sr[4];
+      if (pin) ops.push_back(TailOp{TOP_PIN, l, 0, 0});
+      for (int c = 0; c < 2; c++) {
+        ops.push_back(TailOp{TOP_HALF, l, c, 0});
+        if (sweeps && (t.flags & TAIL_PIN_AFTER)) ops.push_back(TailOp{TOP_PIN, l, 0, 0});
+        if (anybc) ops.push_back(TailOp{TOP_BCPHI, l, 0, 0});
+      }
+    };
+    for (int l = 0; l < nl; l++) {
+      for (int i = 0; i < P.down; i++) pass(l, i > 0);
+      if (l + 1 < nl) {
+        ops.push_back(TailOp{TOP_RESID, l, 0, 0});
+        ops.push_back(TailOp{TOP_RESTRICT, l, 0, 0});
+      }
+    }
+    for (int l = nl - 1; l >= 0; l--) {
+      bool fusepin = false;
+      if (l + 1 < nl) {
+        ops.push_back(TailOp{TOP_EXPAND, l, 0, 0});
+        fusepin = tab[l].flags & TAIL_FUSEPIN;
+      }
+      for (int i = 0; i < P.up; i++) pass(l, !(fusepin && i == 0));
+    }
+    if ((int)ops.size() > TAIL_MAX_OPS) return;
+    TailRun run{};
+    run.nops = (int)ops.size();
+    for (int i = 0; i < 6; i++) run.b[i] = b[i];
+    run.mgparam = P.mgparam;
+    run.lds_words = off;
+    run.nitems = (int)items.size();
+    run.gather_words = gw;
+    if (h->tailtab.n < (size_t)nlev) h->tailtab.alloc((size_t)nlev);
+    if (h->tailitems.n < (size_t)TAIL_MAX_ITEMS) h->tailitems.alloc((size_t)TAIL_MAX_ITEMS);
+    if (h->tailops.n < (size_t)TAIL_MAX_OPS) h->tailops.alloc((size_t)TAIL_MAX_OPS);
+    const bool same =
+        h->tail.host.size() == tab.size() && h->tail.items.size() == items.size() &&
+        h->tail.ops.size() == ops.size() &&
+        std::memcmp(h->tail.host.data(), tab.data(), sizeof(TailLevel) * tab.size()) == 0 &&
+        (items.empty() ||
+         std::memcmp(h->tail.items.data(), items.data(), sizeof(TailItem) * items.size()) == 0) &&
+        (ops.empty() || std::memcmp(h->tail.ops.data(), ops.data(), sizeof(TailOp) * ops.size()) == 0);
+    if (!same) {
+      WM_HIP(hipStreamSynchronize(h->s));
+      WM_HIP(hipMemcpy(h->tailtab.p, tab.data(), sizeof(TailLevel) * tab.size(),
+                       hipMemcpyHostToDevice));
+      if (!items.empty())
+        WM_HIP(hipMemcpy(h->tailitems.p, items.data(), sizeof(TailItem) * items.size(),
+                         hipMemcpyHostToDevice));
+      if (!ops.empty())
+        WM_HIP(hipMemcpy(h->tailops.p, ops.data(), sizeof(TailOp) * ops.size(),
+                         hipMemcpyHostToDevice));
+      h->tail.host = tab;
+      h->tail.items = items;
+      h->tail.ops = ops;
+    }
+    h->tail.down = P.down;
+    h->tail.up = P.up;
+    h->tail.run = run;
+    h->tail.start = start;
+    h->tail.on = true;
+    return;
+  }
+}
+
+// Whether vcycle<double> on this level runs as the tail launch for P.
+static bool tail_applies(const wm_mg3d *h, int level, const Params &P) {
+  return h->tail.on && level == h->tail.start && h->tail.down == P.down &&
+         h->tail.up == P.up && h->tail.run.mgparam == P.mgparam;
+}
+
 // The coarse-grid correction of one level (f3d_mgrid.F:1293-1507): the
 // residual restricted to the next level (stored as SC), the V-cycle there
 // from zero, the expansion added back.  Returns whether the expansion also
@@ -1344,6 +1568,16 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   Level &L = *h->lv[level];
   const bool novolt = !(level == 0 && P.mgform == 1);
   bool fusepin = false;
+  if constexpr (std::is_same<S, double>::value) {
+    // levels from the tail's start on, entered with phi at zero
+    if (zeroed && level > 0 && tail_applies(h, level, P)) {
+      hipLaunchKernelGGL(k_vtail, dim3(1), dim3(1024), sizeof(uint32_t) * h->tail.run.lds_words,
+                         s, (const TailLevel *)h->tailtab.p, (const TailItem *)h->tailitems.p,
+                         (const TailOp *)h->tailops.p, h->tail.run, skip);
+      WM_LAUNCHED();
+      return;
+    }
+  }
   // The first pass pins unless phi starts at zero without voltages, or
   // (repin false) the caller knows the pinned points still hold their
   // values from the previous V-cycle.
@@ -1575,6 +1809,22 @@ int wm_mg3d_create(wm_mg3d **out, int64_t nx, int64_t ny, int64_t nz, double dx,
     WM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     if (ncu > 0) h->ncu = ncu;
     if (lds > 0) h->lds_per_cu = (size_t)lds;
+    // LDS of the V-cycle tail: what one workgroup may hold, less the item
+    // list and 1 KB
+    {
+      int blk = 0;
+      WM_HIP(hipDeviceGetAttribute(&blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+      const size_t fixed = sizeof(TailItem) * TAIL_MAX_ITEMS + 1024;
+      size_t cap = (size_t)blk > fixed ? (size_t)blk - fixed : 0;
+      if (getenv("WARPMI_TAIL_LDS")) cap = std::min<size_t>(cap, (size_t)atoll(getenv("WARPMI_TAIL_LDS")));
+      if (cap > 65536 && hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vtail),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)cap) != hipSuccess) {
+        (void)hipGetLastError();
+        cap = 65536;
+      }
+      h->tail_lds = cap;
+    }
     prepare_occupancy();
     WM_HIP(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
     WM_HIP(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
@@ -1727,6 +1977,8 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       ensure_records<double>(h, L0, 0, P2, true, 1);
     }
     h->save64.alloc_rb(2 * L0.g.cs);
+    prepare_tail(h, P);
+    h->tail_used = h->tail.on && prec == WM_PREC_FP64 && mgmaxiters > 0;
     // Inputs.
     WM_HIP(hipEventRecord(h->ev_in, us));
     WM_HIP(hipStreamWaitEvent(s, h->ev_in, 0));
@@ -1886,6 +2138,13 @@ int wm_mg3d_last_reused(const wm_mg3d *h, int64_t *reused) {
   WM_ENTRY({
     WM_REQUIRE(h && reused, "wm_mg3d_last_reused: NULL argument");
     *reused = h->last_reused ? 1 : 0;
+  });
+}
+
+int wm_mg3d_tail_start(const wm_mg3d *h, int64_t *level) {
+  WM_ENTRY({
+    WM_REQUIRE(h && level, "wm_mg3d_tail_start: NULL argument");
+    *level = h->tail_used ? h->tail.start : -1;
   });
 }
 

@@ -238,6 +238,27 @@ __device__ inline double sor_record_conv(S *__restrict__ phi, const S *__restric
   return d > 0. ? d : 0.;
 }
 
+// One subgrid record of a half sweep (condbndymg, f3d_mgrid.F:2766-2980):
+// the stencil with the record's side forms, divided by its denominator.
+template <typename S, typename T>
+__device__ inline void sor_record(S *__restrict__ phi, const S *__restrict__ rho,
+                                  const RBGeom &g, const SubList<T> &sub, int i, T dxsqic,
+                                  T dysqic, T dzsqic, T spm1) {
+  int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
+  uint32_t m = sub.modes[i];
+  const T *k = sub.k + 6 * i;
+  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(phi[rb_index(g, ix - 1, iy, iz)]));
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(phi[rb_index(g, ix + 1, iy, iz)]));
+  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(phi[rb_index(g, ix, iy - 1, iz)]));
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(phi[rb_index(g, ix, iy + 1, iz)]));
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(phi[rb_index(g, ix, iy, iz - 1)]));
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(phi[rb_index(g, ix, iy, iz + 1)]));
+  long long ce = rb_index(g, ix, iy, iz);
+  T prev = St<S>::ld(phi[ce]);
+  phi[ce] = St<S>::st((St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
+                       (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev);
+}
+
 //
 // CONV (the last half sweeps of a V-cycle, FP64): each point written is also
 // compared with save, its value before the V-cycle, and copied there; every
@@ -358,19 +379,7 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     return;
   }
   if (i >= sub.n) return;
-  int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
-  uint32_t m = sub.modes[i];
-  const T *k = sub.k + 6 * i;
-  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(phi[rb_index(g, ix - 1, iy, iz)]));
-  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(phi[rb_index(g, ix + 1, iy, iz)]));
-  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(phi[rb_index(g, ix, iy - 1, iz)]));
-  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(phi[rb_index(g, ix, iy + 1, iz)]));
-  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(phi[rb_index(g, ix, iy, iz - 1)]));
-  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(phi[rb_index(g, ix, iy, iz + 1)]));
-  long long ce = rb_index(g, ix, iy, iz);
-  T prev = St<S>::ld(phi[ce]);
-  phi[ce] = St<S>::st((St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
-                       (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev);
+  sor_record<S, T>(phi, rho, g, sub, i, dxsqic, dysqic, dzsqic, spm1);
 }
 
 // cond_potmg (f3d_mgrid.F:2426-2472): pin interior conductor points.
@@ -393,15 +402,12 @@ __global__ __launch_bounds__(256) void k_pin(S *__restrict__ phi, RBGeom g, int 
 // so this runs as one sequential thread.  It is launched only when a
 // conductor has Neumann surfaces (no configuration in use has any).
 template <typename T>
-__global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
-                                const int *__restrict__ pos0,
-                                const int *__restrict__ pos1,
-                                const double *__restrict__ dels0,
-                                const double *__restrict__ dels1, int lb0,
-                                int lb1, int lb2, int lb3, int lb4, int lb5,
-                                const int *__restrict__ skip) {
-  WM_SKIP();
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ inline void condbndymgint_seq(T *__restrict__ phi, const RBGeom &g, int n0, int n1,
+                                         const int *__restrict__ pos0,
+                                         const int *__restrict__ pos1,
+                                         const double *__restrict__ dels0,
+                                         const double *__restrict__ dels1, int lb0, int lb1,
+                                         int lb2, int lb3, int lb4, int lb5) {
   int nx = g.nx, ny = g.ny, nz = g.nz;
   for (int p = 0; p < 2; p++) {
     int n = p ? n1 : n0;
@@ -428,6 +434,19 @@ __global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
       }
     }
   }
+}
+
+template <typename T>
+__global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
+                                const int *__restrict__ pos0,
+                                const int *__restrict__ pos1,
+                                const double *__restrict__ dels0,
+                                const double *__restrict__ dels1, int lb0,
+                                int lb1, int lb2, int lb3, int lb4, int lb5,
+                                const int *__restrict__ skip) {
+  WM_SKIP();
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  condbndymgint_seq<T>(phi, g, n0, n1, pos0, pos1, dels0, dels1, lb0, lb1, lb2, lb3, lb4, lb5);
 }
 
 
@@ -510,14 +529,13 @@ __global__ __launch_bounds__(256) void k_residual_zmarch(
 // Zero the points of a red-black residual array outside residual3d's loop
 // range (guard cells and Dirichlet faces).  Threads cover the three pairs
 // of two-point-thick shells; interior points are not touched.
+// Item t of the shells (t < 4 ((ny+3)(nz+3) + (nx+3)(nz+3) + (nx+3)(ny+3))).
 template <typename TR>
-__global__ __launch_bounds__(256) void k_res_shell(TR *__restrict__ res, RBGeom g, int ixmin,
-                                                   int ixmax, int iymin, int iymax, int izmin,
-                                                   int izmax, const int *__restrict__ skip) {
-  WM_SKIP();
+__device__ inline void res_shell_point(TR *__restrict__ res, const RBGeom &g, int ixmin,
+                                       int ixmax, int iymin, int iymax, int izmin, int izmax,
+                                       long long t) {
   int nx3 = g.nx + 3, ny3 = g.ny + 3, nz3 = g.nz + 3;
   long long nxs = 4ll * ny3 * nz3, nys = 4ll * nx3 * nz3, nzs = 4ll * nx3 * ny3;
-  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
   int ix, iy, iz;
   auto pick = [](int k, int n) { return k < 2 ? k - 1 : n - 2 + k; };  // -1, 0, n, n+1
   if (t < nxs) {
@@ -543,6 +561,15 @@ __global__ __launch_bounds__(256) void k_res_shell(TR *__restrict__ res, RBGeom 
   }
   if (ix < ixmin || ix > ixmax || iy < iymin || iy > iymax || iz < izmin || iz > izmax)
     res[rb_index(g, ix, iy, iz)] = St<TR>::st((typename St<TR>::T)0.);
+}
+
+template <typename TR>
+__global__ __launch_bounds__(256) void k_res_shell(TR *__restrict__ res, RBGeom g, int ixmin,
+                                                   int ixmax, int iymin, int iymax, int izmin,
+                                                   int izmax, const int *__restrict__ skip) {
+  WM_SKIP();
+  res_shell_point<TR>(res, g, ixmin, ixmax, iymin, iymax, izmin, izmax,
+                      (long long)blockIdx.x * 256 + threadIdx.x);
 }
 
 // cond_potmgres + condbndyres (f3d_mgrid.F:2474-2506, 3094-3365): interior
@@ -621,18 +648,17 @@ __global__ __launch_bounds__(256) void k_residual_rec2(
 // values the earlier parts wrote.  Each thread owns one grid line normal to
 // the face, so the order of the reference's statements is kept per line.
 // ---------------------------------------------------------------------------
+// Line t of axis `axis` (t < (na+3)(nb+3) over the two other axes).
 template <typename T, typename Idx>
-__global__ void k_bc_axis(T *__restrict__ u, Idx I, int axis, int nx, int ny,
-                          int nz, int bm, int bp, int lwithdirichlet,
-                          int lzerodirichlet, const int *__restrict__ skip) {
-  WM_SKIP();
+__device__ inline void bc_line(T *__restrict__ u, const Idx &I, int axis, int nx, int ny, int nz,
+                               int bm, int bp, int lwithdirichlet, int lzerodirichlet,
+                               long long t) {
   // The line is parametrised by (a, b) over the two other axes, each
   // covering -1..n+1; ranges the reference does not touch are filtered.
   int na, nb, n;
   if (axis == 0) { na = ny; nb = nz; n = nx; }
   else if (axis == 1) { na = nx; nb = nz; n = ny; }
   else { na = nx; nb = ny; n = nz; }
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)(na + 3) * (nb + 3)) return;
   int a = (int)(t % (na + 3)) - 1;
   int b = (int)(t / (na + 3)) - 1;
@@ -669,6 +695,15 @@ __global__ void k_bc_axis(T *__restrict__ u, Idx I, int axis, int nx, int ny,
   }
 }
 
+template <typename T, typename Idx>
+__global__ void k_bc_axis(T *__restrict__ u, Idx I, int axis, int nx, int ny,
+                          int nz, int bm, int bp, int lwithdirichlet,
+                          int lzerodirichlet, const int *__restrict__ skip) {
+  WM_SKIP();
+  bc_line<T, Idx>(u, I, axis, nx, ny, nz, bm, bp, lwithdirichlet, lzerodirichlet,
+                  (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // restrict3d (f3d_mgrid.F:1593-1775): one thread per coarse node.  The
 // per-axis window limits and tent weights are the reference's precomputed
@@ -680,14 +715,13 @@ struct RestrictAxis {
   int cmin, cmax;      // computed coarse range
 };
 
-template <typename TR, typename TC, typename OutIdx, typename FineIdx = NatIdx>
-__global__ __launch_bounds__(256) void k_restrict(
-    const TR *__restrict__ res, FineIdx F, TC *__restrict__ out, OutIdx O,
-    int nxc, int nyc, int nzc, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
-    double ff, const int *__restrict__ skip) {
-  WM_SKIP();
+// Coarse node t (t < (nxc+1)(nyc+1)(nzc+1)).
+template <typename TR, typename TC, typename OutIdx, typename FineIdx>
+__device__ inline void restrict_point(const TR *__restrict__ res, const FineIdx &F,
+                                      TC *__restrict__ out, const OutIdx &O, int nxc, int nyc,
+                                      int nzc, const RestrictAxis &ax, const RestrictAxis &ay,
+                                      const RestrictAxis &az, double ff, int t) {
   int n = (nxc + 1) * (nyc + 1) * (nzc + 1);
-  int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
   int ixc = t % (nxc + 1);
   int u = t / (nxc + 1);
@@ -729,6 +763,16 @@ __global__ __launch_bounds__(256) void k_restrict(
   } else {
     out[oi] = (TC)0.;
   }
+}
+
+template <typename TR, typename TC, typename OutIdx, typename FineIdx = NatIdx>
+__global__ __launch_bounds__(256) void k_restrict(
+    const TR *__restrict__ res, FineIdx F, TC *__restrict__ out, OutIdx O,
+    int nxc, int nyc, int nzc, RestrictAxis ax, RestrictAxis ay, RestrictAxis az,
+    double ff, const int *__restrict__ skip) {
+  WM_SKIP();
+  restrict_point<TR, TC, OutIdx, FineIdx>(res, F, out, O, nxc, nyc, nzc, ax, ay, az, ff,
+                                          (int)(blockIdx.x * 256 + threadIdx.x));
 }
 
 // Single-precision variant: the tent weights and sums are taken in T.

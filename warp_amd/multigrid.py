@@ -402,6 +402,13 @@ class MultiGrid3D(FieldSolver):
         N.call("wm_mg3d_last_reused", self._h, C.byref(r))
         return bool(r.value)
 
+    def tail_start_level(self):
+        """The first level of the V-cycle's coarse end that the last solve
+        ran as one launch (the levels' arrays in LDS), or -1."""
+        r = C.c_int64(0)
+        N.call("wm_mg3d_tail_start", self._h, C.byref(r))
+        return int(r.value)
+
     def _phi_version(self):
         """(version, address) of phi, or None where torch keeps no version
         counter (tensors made under torch.inference_mode)."""
