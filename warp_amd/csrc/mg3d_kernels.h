@@ -913,7 +913,7 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
     r_b = St<S>::ld(rho[eB]);
   };
   // conductor points: mask words two planes ahead, their residuals (from
-  // res, only where the bit is set) one plane ahead
+  // res) one plane ahead
   auto mbit = [&](uint32_t w, int e) { return (w >> (e & 31)) & 1u; };
   auto loadmask = [&](int iz, uint32_t &m_a, uint32_t &m_b) {
     if (MASK) {
@@ -923,29 +923,33 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
       m_a = m_b = 0u;
     }
   };
-  auto loadres = [&](int iz, uint32_t m_a, uint32_t m_b, RA &v_a, RA &v_b) {
-    if (!MASK) {
-      v_a = v_b = (RA)0.;
-      return;
-    }
-    const int eA = elA(iz), eB = elB(iz);
-    const bool ba = mbit(m_a, eA), bb = mbit(m_b, eB);
-    const RA ta = (RA)St<TR>::ld(res[ba ? eA : 0]);
-    const RA tb = (RA)St<TR>::ld(res[bb ? eB : 0]);
-    v_a = ba ? ta : (RA)0.;
-    v_b = bb ? tb : (RA)0.;
-  };
   T am = St<S>::ld(phi[elB(izf) - pl]);
   T bm = St<S>::ld(phi[elA(izf) - pl]);
   T a0 = St<S>::ld(phi[elA(izf)]);
   T b0 = St<S>::ld(phi[elB(izf)]);
-  T ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb;
-  uint32_t wa, wb, wa1 = 0, wb1 = 0;
-  RA sa, sb;
-  load(izf, ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb);
-  loadmask(izf, wa, wb);
-  loadmask(min(izf + 1, izl), wa1, wb1);
-  loadres(izf, wa, wb, sa, sb);
+  // The loads of one plane p: its values (centre pair of p+1, the rest of
+  // p) and the residuals of its conductor points (addressed with p's mask
+  // words, which were loaded two planes ahead, and kept raw: step() only
+  // reads them where the bit is set).  Two sets alternate and the mask
+  // words rotate through four slots, with the loop unrolled by four, so
+  // that the next plane's loads are in flight while the current plane is
+  // consumed and no loaded register is copied (a copy would wait for its
+  // load).
+  struct Set {
+    T ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb;
+    RA sa, sb;
+  };
+  auto issue = [&](int p, Set &d, const uint32_t *mp, uint32_t *mp2) {
+    load(p, d.ap, d.bp, d.yam, d.yap, d.ybm, d.ybp, d.xa, d.xb, d.ra, d.rb);
+    if (MASK) {
+      const int eA = elA(p), eB = elB(p);
+      d.sa = (RA)St<TR>::ld(res[mbit(mp[0], eA) ? eA : 0]);
+      d.sb = (RA)St<TR>::ld(res[mbit(mp[1], eB) ? eB : 0]);
+    } else {
+      d.sa = d.sb = (RA)0.;
+    }
+    loadmask(min(p + 2, izl), mp2[0], mp2[1]);
+  };
   // phase 2: the owned coarse point and its running sum
   const int q = threadIdx.x;
   const bool cown = q < mx * my;
@@ -954,37 +958,32 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
   const int lq = 2 * cyl * nfx + 2 * cxl;
   RA rs = (RA)0.;
   const RA half = (RA)0.5, one = (RA)1.;
-  for (int iz = izf; iz <= izl; iz++) {
+  auto step = [&](int iz, const Set &c, const uint32_t *mw) {
     RA *buf = lres + (iz & 1) * lsz;
-    T nap, nbp, nyam, nyap, nybm, nybp, nxa, nxb, nra, nrb;
-    uint32_t wa2, wb2;
-    RA nsa, nsb;
-    // next plane (the last plane re-reads itself; those values are unused)
-    load(min(iz + 1, izl), nap, nbp, nyam, nyap, nybm, nybp, nxa, nxb, nra, nrb);
-    loadres(min(iz + 1, izl), wa1, wb1, nsa, nsb);
-    loadmask(min(iz + 2, izl), wa2, wb2);
     if (own) {
       RA vA, vB;
-      if (mbit(wa, elA(iz))) {
-        vA = sa;
+      if (mbit(mw[0], elA(iz))) {
+        vA = c.sa;
       } else {
-        T v = ra + (xa + b0) * dxsqic + (yam + yap) * dysqic + (am + ap) * dzsqic - a0 * mgparam;
+        T v = c.ra + (c.xa + b0) * dxsqic + (c.yam + c.yap) * dysqic + (am + c.ap) * dzsqic -
+              a0 * mgparam;
         vA = (RA)(TR)v;
       }
-      if (mbit(wb, elB(iz))) {
-        vB = sb;
+      if (mbit(mw[1], elB(iz))) {
+        vB = c.sb;
       } else {
-        T v = rb + (a0 + xb) * dxsqic + (ybm + ybp) * dysqic + (bm + bp) * dzsqic - b0 * mgparam;
+        T v = c.rb + (a0 + c.xb) * dxsqic + (c.ybm + c.ybp) * dysqic + (bm + c.bp) * dzsqic -
+              b0 * mgparam;
         vB = (RA)(TR)v;
       }
       buf[lidx] = vA;
       buf[lidx + 1] = vB;
     }
     // march: planes iz, iz+1 become iz-1, iz
-    am = a0; bm = b0; a0 = ap; b0 = bp; ap = nap; bp = nbp;
-    yam = nyam; yap = nyap; ybm = nybm; ybp = nybp;
-    xa = nxa; xb = nxb; ra = nra; rb = nrb;
-    wa = wa1; wb = wb1; wa1 = wa2; wb1 = wb2; sa = nsa; sb = nsb;
+    am = a0;
+    bm = b0;
+    a0 = c.ap;
+    b0 = c.bp;
     __syncthreads();
     if (cown) {
       // phase 2: the plane's terms; weight of window position d (0..2) is
@@ -1013,6 +1012,39 @@ __global__ __launch_bounds__(1024) void k_resrestrict(
         add(one);
       }
     }
+  };
+  // plane izf + k uses set k % 2 and mask slot k % 4; the last planes
+  // re-read plane izl (those values are unused)
+  Set A, B;
+  uint32_t M0[2], M1[2], M2[2], M3[2];
+  loadmask(izf, M0[0], M0[1]);
+  loadmask(min(izf + 1, izl), M1[0], M1[1]);
+  issue(izf, A, M0, M2);
+  // whole groups of four planes (no exit inside the body, so the waits the
+  // compiler places at the loop head see one kind of predecessor), then
+  // the last zero to three planes
+  int iz = izf;
+  for (; iz + 3 <= izl; iz += 4) {
+    issue(iz + 1, B, M1, M3);
+    step(iz, A, M0);
+    issue(iz + 2, A, M2, M0);
+    step(iz + 1, B, M1);
+    issue(iz + 3, B, M3, M1);
+    step(iz + 2, A, M2);
+    issue(min(iz + 4, izl), A, M0, M2);
+    step(iz + 3, B, M3);
+  }
+  if (iz <= izl) {
+    issue(min(iz + 1, izl), B, M1, M3);
+    step(iz, A, M0);
+  }
+  if (iz + 1 <= izl) {
+    issue(min(iz + 2, izl), A, M2, M0);
+    step(iz + 1, B, M1);
+  }
+  if (iz + 2 <= izl) {
+    issue(min(iz + 3, izl), B, M3, M1);
+    step(iz + 2, A, M2);
   }
 }
 
