@@ -319,6 +319,16 @@ namespace {
 
 static int blocks(long long n) { return (int)std::max<long long>(1, cdiv(n, 256)); }
 
+// Every element of the array (its leading padding included) set to zero.
+template <typename T>
+static void zero_buf(DevBuf<T> &b, const int *skip, hipStream_t s) {
+  const long long n = (long long)(b.n + b.lead);
+  const long long nvec = (long long)(n * sizeof(T) / 16);
+  hipLaunchKernelGGL((k_zero<T>), dim3(blocks(std::max(nvec, n - nvec * 16 / (long long)sizeof(T)))),
+                     dim3(256), 0, s, b.raw, n, skip);
+  WM_LAUNCHED();
+}
+
 // getnextcoarselevel3d (f3d_mgrid.F:3601-3652), mgcoarsening = 2.
 static void next_coarse(int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
                         double dz, int64_t &nxc, int64_t &nyc, int64_t &nzc,
@@ -1021,9 +1031,16 @@ static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int 
 // loop range are zeroed (k_res_shell), the dense points computed in one
 // z-marching launch for both colours, then cond_potmgres/condbndyres
 // (k_residual_cond, even list then odd list) and the boundary conditions.
+//
+// steady: res already holds this solve's zeros outside the loop range and
+// at the interior conductor points (residual_zeros), which nothing but
+// the boundary conditions below writes between calls; then only the dense
+// points and the records are computed, the records of both colours in one
+// launch unless one of them zeroes a neighbour's residual.
 template <typename S, typename TR>
 static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const S *phi,
-                        const S *rho, TR *res, int mgform2init, const int *skip, hipStream_t s) {
+                        const S *rho, TR *res, int mgform2init, const int *skip, hipStream_t s,
+                        bool steady = false) {
   typedef typename St<S>::T T;
   LoopRange r = res_range(L, h->bounds);
   double cst = 0.5 / (L.dxsqi + L.dysqi + L.dzsqi);
@@ -1031,9 +1048,11 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
     dzsqic = (T)(L.dzsqi * P.mgparam * cst);
   long long nsh = 4ll * ((long long)(L.ny + 3) * (L.nz + 3) + (long long)(L.nx + 3) * (L.nz + 3) +
                          (long long)(L.nx + 3) * (L.ny + 3));
-  hipLaunchKernelGGL((k_res_shell<TR>), dim3(blocks(nsh)), dim3(256), 0, s, res, L.g, r.ixmin,
-                     r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, skip);
-  WM_LAUNCHED();
+  if (!steady) {
+    hipLaunchKernelGGL((k_res_shell<TR>), dim3(blocks(nsh)), dim3(256), 0, s, res, L.g, r.ixmin,
+                       r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, skip);
+    WM_LAUNCHED();
+  }
   int nyr = r.iymax - r.iymin + 1, nzr = r.izmax - r.izmin + 1;
   if (nyr > 0 && nzr > 0 && r.ixmax >= r.ixmin) {
     int nby = (int)cdiv(nyr, 4);
@@ -1043,10 +1062,17 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
     int zc = (int)std::min<long long>(64, std::max<long long>(2, cdiv((long long)nzr * cols, 2048)));
     int nbz = (int)cdiv(nzr, zc);
     int nbdense = nbx * nby * nbz;
-    hipLaunchKernelGGL((k_residual_zmarch<S, TR>), dim3(2 * nbdense), dim3(256), 0, s, phi, rho,
-                       res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, dxsqic,
-                       dysqic, dzsqic, (T)P.mgparam, L.hasmask ? L.mask.p : nullptr, j0, nbx,
-                       nby, zc, nbdense, skip);
+    WM_REQUIRE(2 * L.g.cs < (1ll << 31), "residual3d: level too large for 32-bit indexing");
+    if (L.hasmask)
+      hipLaunchKernelGGL((k_residual_pairs<S, TR, T, true>), dim3(nbdense), dim3(256), 0, s,
+                         phi, rho, res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                         r.izmax, dxsqic, dysqic, dzsqic, (T)P.mgparam,
+                         (const uint32_t *)L.mask.p, j0, j1, nbx, nby, zc, nbdense, skip);
+    else
+      hipLaunchKernelGGL((k_residual_pairs<S, TR, T, false>), dim3(nbdense), dim3(256), 0, s,
+                         phi, rho, res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                         r.izmax, dxsqic, dysqic, dzsqic, (T)P.mgparam,
+                         (const uint32_t *)nullptr, j0, j1, nbx, nby, zc, nbdense, skip);
     WM_LAUNCHED();
   }
   if (L.hasmask) {
@@ -1054,6 +1080,29 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
     CondDev<T> &C = cond_of<T>(L);
     T dxsqics = (T)(P.mgparam * L.dxsqi * cst), dysqics = (T)(P.mgparam * L.dysqi * cst),
       dzsqics = (T)(P.mgparam * L.dzsqi * cst);
+    if (steady) {
+      SubList<T> s0 = C.res[mgform2init][0].view(), s1 = C.res[mgform2init][1].view();
+      if (!C.res[mgform2init][0].anyzero && !C.res[mgform2init][1].anyzero) {
+        if (s0.n + s1.n > 0) {
+          hipLaunchKernelGGL((k_residual_rec2<S, TR, T, RBIdx>), dim3(blocks(s0.n + s1.n)),
+                             dim3(256), 0, s, phi, rho, res, RBIdx{L.g}, L.g, s0, s1, dxsqics,
+                             dysqics, dzsqics, (T)P.mgparam, skip);
+          WM_LAUNCHED();
+        }
+      } else {
+        const int *none = nullptr;
+        for (int p = 0; p < 2; p++) {
+          SubList<T> sub = p ? s1 : s0;
+          if (sub.n == 0) continue;
+          hipLaunchKernelGGL((k_residual_cond<S, TR, T, RBIdx>), dim3(blocks(sub.n)), dim3(256), 0,
+                             s, phi, rho, res, RBIdx{L.g}, L.g, 0, none, sub, dxsqics, dysqics,
+                             dzsqics, (T)P.mgparam, skip);
+          WM_LAUNCHED();
+        }
+      }
+      launch_bc<TR>(res, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+      return;
+    }
     // interior points get 0 before any record is evaluated (cond_potmgres
     // then condbndyres): in one launch unless a record sits on an interior
     // point
@@ -1077,6 +1126,27 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
     }
   }
   launch_bc<TR>(res, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
+}
+
+// The points of res that residual_rb's steady form leaves alone: zeros
+// outside the loop range (k_res_shell) and at the interior conductor points
+// (cond_potmgres).  Once per solve, before the iterations.
+template <typename TR>
+static void residual_zeros(wm_mg3d *h, Level &L, TR *res, hipStream_t s) {
+  LoopRange r = res_range(L, h->bounds);
+  long long nsh = 4ll * ((long long)(L.ny + 3) * (L.nz + 3) + (long long)(L.nx + 3) * (L.nz + 3) +
+                         (long long)(L.nx + 3) * (L.ny + 3));
+  hipLaunchKernelGGL((k_res_shell<TR>), dim3(blocks(nsh)), dim3(256), 0, s, res, L.g, r.ixmin,
+                     r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax, (const int *)nullptr);
+  WM_LAUNCHED();
+  if (L.hasmask && L.nint > 0) {
+    SubList<double> none{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL((k_residual_cond<double, TR, double, RBIdx>), dim3(blocks(L.nint)),
+                       dim3(256), 0, s, (const double *)nullptr, (const double *)nullptr, res,
+                       RBIdx{L.g}, L.g, L.nint, L.int_pos.p, none, 0., 0., 0., 0.,
+                       (const int *)nullptr);
+    WM_LAUNCHED();
+  }
 }
 
 struct RhoIdx {
@@ -1620,12 +1690,10 @@ template <typename S>
 static void dc_lowp(wm_mg3d *h, const Params &Pinit, const double *base, long long np,
                     bool fused, double mgtol, int mgmaxiters, const int *skip, hipStream_t s) {
   Level &L0 = *h->lv[0];
-  residual_rb<double, S>(h, L0, 0, Pinit, base, h->rhosave64.p, rho_of<S>(L0).p, 1, skip, s);
-  S zero;
-  std::memset(&zero, 0, sizeof(S));
-  hipLaunchKernelGGL((k_fill<S>), dim3(blocks(np)), dim3(256), 0, s, phi_of<S>(L0).p, np, zero,
-                     skip);
-  WM_LAUNCHED();
+  residual_rb<double, S>(h, L0, 0, Pinit, base, h->rhosave64.p, rho_of<S>(L0).p, 1, skip, s,
+                         true);
+  WM_REQUIRE(phi_of<S>(L0).n >= (size_t)np, "defect correction: error array too small");
+  zero_buf<S>(phi_of<S>(L0), skip, s);
   vcycle<S>(h, 0, Pinit, skip, s, true);
   if (fused) {
     WM_KW_DISPATCH((L0.nx + 4) / 2, hipLaunchKernelGGL((k_dc_update<S, KW>), dim3(nconv_blocks(L0.g)),
@@ -1694,15 +1762,12 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
     pin_and_mirror<double>(h, L0, 0, Pinit, base, 1, skip, s);
     long long np = 2 * L0.g.cs;
     if (prec == WM_PREC_FP64) {
-      residual_rb<double, double>(h, L0, 0, Pinit, base, h->rhosave64.p, L0.rho64.p, 1, skip, s);
-      hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p, np,
-                         0., skip);
-      WM_LAUNCHED();
+      residual_rb<double, double>(h, L0, 0, Pinit, base, h->rhosave64.p, L0.rho64.p, 1, skip, s,
+                                  true);
+      zero_buf<double>(L0.phi64, skip, s);
       // the error starts at zero in both buffers (their boundary values
       // must agree for the fused pass)
-      hipLaunchKernelGGL((k_fill<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64b.p, np,
-                         0., skip);
-      WM_LAUNCHED();
+      zero_buf<double>(L0.phi64b, skip, s);
       vcycle<double>(h, 0, Pinit, skip, s, true);
       // phi = phi + phisave (the error plus the saved phi)
       hipLaunchKernelGGL((k_dc_add<double>), dim3(blocks(np)), dim3(256), 0, s, L0.phi64.p,
@@ -2003,8 +2068,11 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     if (!reuse) {
       // the saved phi: written by the conversion itself when the boundary
       // conditions change nothing (six Dirichlet bounds), else copied after
+      // the fused pass's second buffer needs the boundary values only when
+      // the plain FP64 V-cycle may run that pass on the finest level
       hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi,
-                         L0.phi64.p, L0.g, alldir ? h->save64.p : nullptr, L0.phi64b.p);
+                         L0.phi64.p, L0.g, alldir ? h->save64.p : nullptr,
+                         fp64plain && h->sor_form != 0 ? L0.phi64b.p : nullptr);
       WM_LAUNCHED();
       launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr,
                         s);
@@ -2020,6 +2088,13 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       zero_interior_res<double>(h, dc ? P2of(P) : P, s);
     else
       zero_interior_res<float>(h, Params{P.mgparam, 2, P.down, P.up, P.lcnd, P.icnd}, s);
+    // the fixed zeros of the defect-correction source (its residual runs
+    // in the steady form)
+    if (dc) {
+      if (prec == WM_PREC_FP64) residual_zeros<double>(h, L0, L0.rho64.p, s);
+      else if (prec == WM_PREC_FP32_DC) residual_zeros<float>(h, L0, L0.rho32.p, s);
+      else residual_zeros<bf16>(h, L0, L0.rho16.p, s);
+    }
     if (prec == WM_PREC_FP64 && P.mgform == 1 && P.up >= 1 && repin_skippable(h, P)) {
       // slots of the two fused half sweeps: at most one per wavefront of
       // rows x z chunks of the dense part plus one per 64 records, with room

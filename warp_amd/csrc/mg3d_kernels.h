@@ -351,17 +351,19 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
       oz_m = oz_0;
       oz_0 = q.ozp;
     };
-    // two sets of loads in turn (a copy of a set would wait for its loads
-    // before the next set is issued); the last plane re-reads itself into
-    // the idle set (unused)
+    // two sets of loads in turn (a copy of a set would wait for its
+    // loads), each refilled right after its plane is consumed, so the set
+    // of the next plane is in flight meanwhile; the last planes re-read
+    // plane zb+1 into the idle set (unused)
     Pl qa, qb;
     load(za + 1, qa);
+    load(min(za + 2, zb + 1), qb);
     for (int iz = za; iz <= zb; iz += 2) {
-      load(min(iz + 2, zb + 1), qb);
       step(iz, qa);
-      if (iz + 1 > zb) break;
       load(min(iz + 3, zb + 1), qa);
+      if (iz + 1 > zb) break;
       step(iz + 1, qb);
+      load(min(iz + 4, zb + 1), qb);
     }
     if (CONV) {
       cmax = wave_max(cmax);
@@ -456,7 +458,7 @@ __global__ void k_condbndymgint(T *__restrict__ phi, RBGeom g, int n0, int n1,
 // rho + sum(neighbours * d2c) - phi * mgparam (f3d_mgrid.F:3542-3546) in
 // the red-black array res (same geometry as phi).  Masked points are left
 // to k_residual_cond, points outside the range to k_res_shell.
-template <typename S, typename TR, typename T = typename St<S>::T>
+template <typename S, typename TR, typename T = typename St<S>::T, bool MASK = true>
 __global__ __launch_bounds__(256) void k_residual_zmarch(
     const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RBGeom g,
     int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
@@ -481,48 +483,155 @@ __global__ __launch_bounds__(256) void k_residual_zmarch(
   const S *me = phi + c * g.cs;
   const S *rh = rho + c * g.cs;
   TR *out = res + c * g.cs;
+  // Loads use jc, clamped into the padded row, so they stay in bounds for
+  // lanes past the end of the row; such lanes never store.
   int jc = min(j, g.W - 1);
   long long rowc = (long long)g.W * py + jc;
   long long meoff = (long long)c * g.cs + rowc;
-  auto load = [&](int pz, T &ozp, T &oym, T &oyp, T &own, T &r, T &edge, uint32_t &mw) {
+  // The loads of one plane, issued a plane ahead and unconditionally (as
+  // in k_sor_zmarch: every lane loads its x-neighbour beyond the wavefront
+  // at a clamped element), in two alternating sets, so that the compiler
+  // never waits for the plane being prefetched.
+  struct Pl {
+    T ozp, oym, oyp, own, r, edge;
+    uint32_t mw;
+  };
+  auto load = [&](int pz, Pl &q) {
     long long base = pl * pz;
-    ozp = St<S>::ld(o[rowc + base + pl]);
-    oym = St<S>::ld(o[rowc + base - g.W]);
-    oyp = St<S>::ld(o[rowc + base + g.W]);
-    own = St<S>::ld(me[rowc + base]);
-    r = St<S>::ld(rh[rowc + base]);
+    q.ozp = St<S>::ld(o[rowc + base + pl]);
+    q.oym = St<S>::ld(o[rowc + base - g.W]);
+    q.oyp = St<S>::ld(o[rowc + base + g.W]);
+    q.own = St<S>::ld(me[rowc + base]);
+    q.r = St<S>::ld(rh[rowc + base]);
     int p0 = (c + py + pz + 1) & 1;
     int jn = p0 ? j + 1 : j - 1;
-    bool e = p0 ? (lane == 63) : (lane == 0);
-    edge = e ? St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]) : (T)0.;
-    mw = mask ? mask[(meoff + base) >> 5] : 0u;
+    q.edge = St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]);
+    q.mw = MASK ? mask[(meoff + base) >> 5] : 0u;
   };
   T oz_m = St<S>::ld(o[rowc + pl * za]);
   T oz_0 = St<S>::ld(o[rowc + pl * (za + 1)]);
-  T ozp, oym, oyp, own, r, eg;
-  uint32_t mw;
-  load(za + 1, ozp, oym, oyp, own, r, eg, mw);
-  for (int iz = za; iz <= zb; iz++) {
+  auto step = [&](int iz, const Pl &q) {
     int pz = iz + 1;
-    T nozp = 0, noym = 0, noyp = 0, nown = 0, nr = 0, neg = 0;
-    uint32_t nmw = 0;
-    if (iz < zb) load(pz + 1, nozp, noym, noyp, nown, nr, neg, nmw);
     int p0 = (c + py + pz + 1) & 1;
     T sh = p0 ? __shfl_down(oz_0, 1, 64) : __shfl_up(oz_0, 1, 64);
-    if (p0 ? (lane == 63) : (lane == 0)) sh = eg;
+    if (p0 ? (lane == 63) : (lane == 0)) sh = q.edge;
     int ix = p0 + 2 * j - 1;
     long long ce = meoff + pl * pz;
-    bool active = j < g.W && ix >= ixmin && ix <= ixmax && !((mw >> (ce & 31)) & 1u);
+    bool active = j < g.W && ix >= ixmin && ix <= ixmax && !((q.mw >> (ce & 31)) & 1u);
     if (active) {
       T pxm = p0 ? oz_0 : sh;
       T pxp = p0 ? sh : oz_0;
-      T v = r + (pxm + pxp) * dxsqic + (oym + oyp) * dysqic + (oz_m + ozp) * dzsqic -
-            own * mgparam;
+      T v = q.r + (pxm + pxp) * dxsqic + (q.oym + q.oyp) * dysqic + (oz_m + q.ozp) * dzsqic -
+            q.own * mgparam;
       out[rowc + pl * pz] = St<TR>::st((typename St<TR>::T)v);
     }
     oz_m = oz_0;
-    oz_0 = ozp;
-    ozp = nozp; oym = noym; oyp = noyp; own = nown; r = nr; eg = neg; mw = nmw;
+    oz_0 = q.ozp;
+  };
+  // two sets in turn, each refilled right after its plane is consumed (as
+  // in k_sor_zmarch); the last planes re-read plane zb+1 (unused)
+  Pl qa, qb;
+  load(za + 1, qa);
+  load(min(za + 2, zb + 1), qb);
+  for (int iz = za; iz <= zb; iz += 2) {
+    step(iz, qa);
+    load(min(iz + 3, zb + 1), qa);
+    if (iz + 1 > zb) break;
+    step(iz + 1, qb);
+    load(min(iz + 4, zb + 1), qb);
+  }
+}
+
+// residual3d's dense part, both colours in one pass: each thread owns
+// element j of one row in both colour arrays -- the fine points 2j-1 (A,
+// the colour whose element j is its left point) and 2j (B) -- and walks
+// them through zc planes with their z-neighbours in registers, so each
+// array is read once (k_residual_zmarch reads each colour's array in both
+// colours' passes).  The same expression per point as k_residual_zmarch;
+// masked points and points outside the range are left alone.  Loads are
+// issued a plane ahead in two alternating sets, unconditionally (clamped
+// elements for lanes past the row).
+template <typename S, typename TR, typename T = typename St<S>::T, bool MASK = true>
+__global__ __launch_bounds__(256) void k_residual_pairs(
+    const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RBGeom g,
+    int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
+    T dzsqic, T mgparam, const uint32_t *__restrict__ mask, int j0, int j1, int nbx, int nby,
+    int zc, int nbdense, const int *__restrict__ skip) {
+  WM_SKIP();
+  const int lb = xcd_remap(blockIdx.x, nbdense);
+  const int bx = lb % nbx;
+  const int t = lb / nbx;
+  const int by = t % nby;
+  const int bz = t / nby;
+  const int lane = threadIdx.x & 63;
+  const int j = j0 + bx * 64 + lane;
+  const int iy = iymin + by * 4 + (threadIdx.x >> 6);
+  if (iy > iymax) return;  // whole wave
+  const int za = izmin + bz * zc;
+  const int zb = min(za + zc - 1, izmax);
+  const int py = iy + 1;
+  const int W = g.W, cs = (int)g.cs, pl = g.W * g.PY;
+  const int jc = min(j, W - 1);
+  const bool inj = j <= j1;
+  const int ixA = 2 * j - 1, ixB = 2 * j;
+  const bool okA = inj && ixA >= ixmin && ixA <= ixmax;
+  const bool okB = inj && ixB >= ixmin && ixB <= ixmax;
+  // elements of plane pz (padded index): A's array is the colour cA
+  auto eA = [&](int pz) { return ((py + pz + 1) & 1) * cs + W * (py + g.PY * pz) + jc; };
+  auto eB = [&](int pz) { return (((py + pz + 1) & 1) ^ 1) * cs + W * (py + g.PY * pz) + jc; };
+  struct Pl {
+    T ap, bp, yam, yap, ybm, ybp, xa, xb, ra, rb;
+    uint32_t ma, mb;
+  };
+  auto load = [&](int pz, Pl &q) {
+    const int a = eA(pz), b = eB(pz);
+    q.ap = St<S>::ld(phi[b + pl]);
+    q.bp = St<S>::ld(phi[a + pl]);
+    q.yam = St<S>::ld(phi[b - W]);
+    q.yap = St<S>::ld(phi[b + W]);
+    q.ybm = St<S>::ld(phi[a - W]);
+    q.ybp = St<S>::ld(phi[a + W]);
+    q.xa = St<S>::ld(phi[b - 1]);
+    q.xb = St<S>::ld(phi[a + 1]);
+    q.ra = St<S>::ld(rho[a]);
+    q.rb = St<S>::ld(rho[b]);
+    q.ma = MASK ? mask[a >> 5] : 0u;
+    q.mb = MASK ? mask[b >> 5] : 0u;
+  };
+  const int pza = za + 1;
+  T am = St<S>::ld(phi[eB(pza) - pl]);
+  T bm = St<S>::ld(phi[eA(pza) - pl]);
+  T a0 = St<S>::ld(phi[eA(pza)]);
+  T b0 = St<S>::ld(phi[eB(pza)]);
+  auto step = [&](int iz, const Pl &q) {
+    const int pz = iz + 1;
+    const int a = eA(pz), b = eB(pz);
+    if (okA && !((q.ma >> (a & 31)) & 1u)) {
+      T v = q.ra + (q.xa + b0) * dxsqic + (q.yam + q.yap) * dysqic + (am + q.ap) * dzsqic -
+            a0 * mgparam;
+      res[a] = St<TR>::st((typename St<TR>::T)v);
+    }
+    if (okB && !((q.mb >> (b & 31)) & 1u)) {
+      T v = q.rb + (a0 + q.xb) * dxsqic + (q.ybm + q.ybp) * dysqic + (bm + q.bp) * dzsqic -
+            b0 * mgparam;
+      res[b] = St<TR>::st((typename St<TR>::T)v);
+    }
+    am = a0;
+    bm = b0;
+    a0 = q.ap;
+    b0 = q.bp;
+  };
+  // two sets in turn, each refilled right after its plane is consumed (as
+  // in k_sor_zmarch); the last planes re-read plane zb+1 (unused)
+  Pl qa, qb;
+  load(za + 1, qa);
+  load(min(za + 2, zb + 1), qb);
+  for (int iz = za; iz <= zb; iz += 2) {
+    step(iz, qa);
+    load(min(iz + 3, zb + 1), qa);
+    if (iz + 1 > zb) break;
+    step(iz + 1, qb);
+    load(min(iz + 4, zb + 1), qb);
   }
 }
 
@@ -1203,6 +1312,23 @@ __global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, long long n, T 
   WM_SKIP();
   long long t = (long long)blockIdx.x * 256 + threadIdx.x;
   if (t < n) a[t] = v;
+}
+
+// Zeros over a whole allocation of n elements from its (256-byte aligned)
+// start: 16-byte stores, then the last elements one by one.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(T *__restrict__ raw, long long n,
+                                              const int *__restrict__ skip) {
+  WM_SKIP();
+  const long long nvec = (long long)(n * sizeof(T) / 16);
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < nvec) reinterpret_cast<uint4 *>(raw)[t] = make_uint4(0u, 0u, 0u, 0u);
+  const long long tail0 = nvec * 16 / (long long)sizeof(T);
+  if (t < n - tail0) {
+    T z;
+    __builtin_memset(&z, 0, sizeof(T));
+    raw[tail0 + t] = z;
+  }
 }
 
 // natural -> red-black (all elements, guards included); dst2 and dst3, when
