@@ -172,12 +172,17 @@ struct CondDev {
   RecKey sorkey, reskey[2];
   SubDev<T> sor[2];
   SubDev<T> res[2][2];  // [mgform2init][colour]
+  // both colours' residual records in natural order (z, y, x): entry
+  // (colour << 31) | index, so that neighbouring threads of k_residual_rec2
+  // read neighbouring points of phi
+  DevBuf<uint32_t> resord[2];
   void reset() {
     sorkey = reskey[0] = reskey[1] = RecKey();
     for (int p = 0; p < 2; p++) {
       sor[p].upload(SubHost());
       res[0][p].upload(SubHost());
       res[1][p].upload(SubHost());
+      resord[p].release();
     }
   }
 };
@@ -318,6 +323,14 @@ struct wm_mg3d {
 namespace {
 
 static int blocks(long long n) { return (int)std::max<long long>(1, cdiv(n, 256)); }
+
+// The natural-order index of a level's residual records, for lists long
+// enough that the locality pays for the extra indirection (measured: 55 ->
+// 47 us for 324k records; a few thousand records are latency-bound and
+// slower with it).
+static const uint32_t *rec_order(const DevBuf<uint32_t> &ord, long long n) {
+  return n >= 16384 ? ord.p : nullptr;
+}
 
 // Every element of the array (its leading padding included) set to zero.
 template <typename T>
@@ -657,12 +670,23 @@ static void ensure_records(wm_mg3d *h, Level &L, int level, const Params &P,
     C.sorkey = key;
   } else {
     if (C.reskey[mgform2init] == key) return;
+    std::vector<std::pair<long long, uint32_t>> ord;
     for (int p = 0; p < 2; p++) {
       SubHost sh;
       if (P.lcnd)
         sh = build_records(L, level, p, h->bounds, P.mgparam, P.mgform, P.icnd, true, mgform2init != 0);
+      for (int i = 0; i < sh.n(); i++) {
+        const long long nat = (long long)sh.pos[3 * i] + 1 +
+                              (long long)(L.nx + 3) * ((long long)sh.pos[3 * i + 1] + 1 +
+                                                       (long long)(L.ny + 3) * (sh.pos[3 * i + 2] + 1));
+        ord.push_back({nat, ((uint32_t)p << 31) | (uint32_t)i});
+      }
       C.res[mgform2init][p].upload(sh);
     }
+    std::sort(ord.begin(), ord.end());
+    std::vector<uint32_t> o(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) o[i] = ord[i].second;
+    C.resord[mgform2init].upload(o);
     C.reskey[mgform2init] = key;
   }
 }
@@ -1086,7 +1110,8 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
         if (s0.n + s1.n > 0) {
           hipLaunchKernelGGL((k_residual_rec2<S, TR, T, RBIdx>), dim3(blocks(s0.n + s1.n)),
                              dim3(256), 0, s, phi, rho, res, RBIdx{L.g}, L.g, s0, s1, dxsqics,
-                             dysqics, dzsqics, (T)P.mgparam, skip);
+                             dysqics, dzsqics, (T)P.mgparam, skip,
+                             rec_order(C.resord[mgform2init], s0.n + s1.n));
           WM_LAUNCHED();
         }
       } else {
@@ -1311,7 +1336,7 @@ static void resrestrict(wm_mg3d *h, Level &L, Level &Lc, int level, const Params
     if (s0.n + s1.n > 0) {
       hipLaunchKernelGGL((k_residual_rec2<S, TR, T, RBIdx>), dim3(blocks(s0.n + s1.n)), dim3(256),
                          0, s, phi, rho, res, RBIdx{L.g}, L.g, s0, s1, dxsqic, dysqic, dzsqic,
-                         (T)P.mgparam, skip);
+                         (T)P.mgparam, skip, rec_order(C.resord[0], s0.n + s1.n));
       WM_LAUNCHED();
     }
   }
@@ -2070,8 +2095,10 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       // conditions change nothing (six Dirichlet bounds), else copied after
       // the fused pass's second buffer needs the boundary values only when
       // the plain FP64 V-cycle may run that pass on the finest level
+      // (the FP32/BF16 defect correction with six Dirichlet bounds updates
+      // phi in place and tests the change itself: no saved phi)
       hipLaunchKernelGGL((k_nat_to_rb<double>), dim3(blocks(nn)), dim3(256), 0, s, phi,
-                         L0.phi64.p, L0.g, alldir ? h->save64.p : nullptr,
+                         L0.phi64.p, L0.g, alldir && prec == WM_PREC_FP64 ? h->save64.p : nullptr,
                          fp64plain && h->sor_form != 0 ? L0.phi64b.p : nullptr);
       WM_LAUNCHED();
       launch_bc<double>(L0.phi64.p, RBIdx{L0.g}, L0.nx, L0.ny, L0.nz, h->bounds, 0, 0, nullptr,

@@ -736,13 +736,28 @@ __global__ __launch_bounds__(256) void k_residual_cond(
 
 // Both colours' records in one launch, for record lists that zero no
 // neighbour's residual (then the order of the lists does not matter).
+// With ord, thread i takes record ord[i] ((colour << 31) | index: both
+// lists merged in natural order, so neighbouring threads read neighbouring
+// points); without it, list 0 then list 1.
 template <typename S, typename TR, typename T = typename St<S>::T, typename RIdx = NatIdx>
 __global__ __launch_bounds__(256) void k_residual_rec2(
     const S *__restrict__ phi, const S *__restrict__ rho, TR *__restrict__ res, RIdx RI,
     RBGeom g, SubList<T> sub0, SubList<T> sub1, T dxsqics, T dysqics, T dzsqics, T mgparam,
-    const int *__restrict__ skip) {
+    const int *__restrict__ skip, const uint32_t *__restrict__ ord = nullptr) {
   WM_SKIP();
   int i = blockIdx.x * 256 + threadIdx.x;
+  if (ord) {
+    if (i >= sub0.n + sub1.n) return;
+    const uint32_t o = ord[i];
+    const int k = (int)(o & 0x7fffffffu);
+    if (o >> 31)
+      residual_record<S, TR, T, RIdx>(phi, rho, res, RI, g, sub1, k, dxsqics, dysqics, dzsqics,
+                                      mgparam);
+    else
+      residual_record<S, TR, T, RIdx>(phi, rho, res, RI, g, sub0, k, dxsqics, dysqics, dzsqics,
+                                      mgparam);
+    return;
+  }
   if (i < sub0.n)
     residual_record<S, TR, T, RIdx>(phi, rho, res, RI, g, sub0, i, dxsqics, dysqics, dzsqics,
                                     mgparam);
