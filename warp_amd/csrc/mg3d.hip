@@ -297,6 +297,7 @@ struct wm_mg3d {
   DevBuf<TailLevel> tailtab;
   DevBuf<TailItem> tailitems;
   DevBuf<TailOp> tailops;
+  int dc_double_from = -1;  // levels from here on run in FP64 in the FP32/BF16 V-cycles
   bool tail_used = false;  // the last solve was FP64 and ran the tail
   size_t tail_lds = 0;
   int ncu = 256;                 // compute units of the device
@@ -1426,7 +1427,21 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
 static void prepare_tail(wm_mg3d *h, const Params &P) {
   h->tail.on = false;
   const int nlev = (int)h->lv.size();
+  // the first level from which the levels' FP64 phi, rho and res fit the
+  // budget (the tail starts there or below, once its tables fit too); the
+  // FP32/BF16 V-cycles run these levels in FP64, tail or not, so that
+  // every SOR form gives the same bits
+  h->dc_double_from = -1;
   if (h->tail_lds == 0) return;
+  {
+    long long need = 0;
+    for (int l = nlev - 1; l >= 1; l--) {
+      const Level &L = *h->lv[l];
+      need += (L.coarser ? 3 : 2) * 2 * L.g.cs;
+      if ((size_t)need * sizeof(double) > h->tail_lds) break;
+      h->dc_double_from = l;
+    }
+  }
   for (int l = 1; l < nlev; l++)
     if (fused_ok(h, *h->lv[l], l, P) || !h->lv[l]->phi64.p || !h->lv[l]->rho64.p ||
         (h->lv[l]->coarser && !h->lv[l]->res64.p))
@@ -1679,8 +1694,10 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   for (int i = 0; i < P.down; i++)
     sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0) && (repin || i > 0));
   if (L.coarser) {
-    // the coarsest level of a lower-precision V-cycle is solved in FP64
-    if (!std::is_same<S, double>::value && level + 2 == (int)h->lv.size())
+    // the coarsest level of a lower-precision V-cycle is solved in FP64,
+    // and so are the levels the single-launch tail can hold
+    if (!std::is_same<S, double>::value &&
+        (level + 2 == (int)h->lv.size() || level + 1 == h->dc_double_from))
       fusepin = coarse_correction<S, double>(h, L, level, P, skip, s);
     else
       fusepin = coarse_correction<S, SC>(h, L, level, P, skip, s);
