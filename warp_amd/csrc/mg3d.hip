@@ -1801,7 +1801,12 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
     double *base = fused ? L0.phi64.p : h->save64.p;
     Params Pinit = P;
     Pinit.mgform = 2;
-    pin_and_mirror<double>(h, L0, 0, Pinit, base, 1, skip, s);
+    // the pin changes phi only in the first iteration (the error is zero at
+    // the interior points, so the update leaves their voltages); later
+    // iterations skip it (its skip flag is the iteration count), unless
+    // Neumann mirrors come with it
+    const int *pinskip = (P.icnd == 2 && L0.neumann) ? skip : &h->state.p->iters;
+    pin_and_mirror<double>(h, L0, 0, Pinit, base, 1, pinskip, s);
     long long np = 2 * L0.g.cs;
     if (prec == WM_PREC_FP64) {
       residual_rb<double, double>(h, L0, 0, Pinit, base, h->rhosave64.p, L0.rho64.p, 1, skip, s,
