@@ -392,3 +392,32 @@ def test_vcycle_tail_same_bits(device, case, monkeypatch):
         assert a[3] == -1 and b[3] >= 1, (a[3], b[3])
         assert (a[0], a[1]) == (b[0], b[1])
         np.testing.assert_array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("bounds", [(0,) * 6, (1, 1, 0, 1, 2, 2), (2, 2, 1, 0, 0, 1)])
+def test_residual_wide_rows_match_oracle(device, bounds):
+    """residual3d on a level with padded rows (nx = 128: 80 elements per
+    half row), where the FP64 residual takes two points of each colour per
+    thread with 16-byte loads: bitwise equal to the oracle, with the pipe
+    and annulus, both initial-residual forms."""
+    import warp_amd
+    nx, ny, nz = 128, 16, 24
+    lo, hi = (-0.02, -0.02, 0.0), (0.02, 0.02, 0.08)
+    s = _solver(nx, ny, nz, lo, hi, bounds)
+    for g in H.pipe_and_annulus(warp_amd):
+        s.installconductor(g)
+    s.getconductorobject()
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    O.checkconductors(co, nx, ny, nz, s.dx, s.dy, s.dz)
+    from warp_amd import _native as N
+    st = N.stream_of(device)
+    phi, rho = _random_level_case(nx, ny, nz, 41)
+    for mgform2init in (0, 1):
+        pg = torch.from_numpy(phi).to(device)
+        rg = torch.from_numpy(rho).to(device)
+        res = torch.zeros_like(pg)
+        N.call("wm_mg3d_residual", s._h, 0, pg.data_ptr(), rg.data_ptr(), res.data_ptr(),
+               1.2, 1, mgform2init, 1, 2, st)
+        ro = O.residual3d(phi, rho, s.dx, s.dy, s.dz, bounds, mgparam=1.2, mgform=1,
+                          mglevel=0, conductors=co, mgform2init=bool(mgform2init))
+        np.testing.assert_array_equal(res.cpu().numpy(), ro)

@@ -1088,17 +1088,44 @@ static void residual_rb(wm_mg3d *h, Level &L, int level, const Params &P, const 
     int nbz = (int)cdiv(nzr, zc);
     int nbdense = nbx * nby * nbz;
     WM_REQUIRE(2 * L.g.cs < (1ll << 31), "residual3d: level too large for 32-bit indexing");
-    if (L.hasmask)
-      hipLaunchKernelGGL((k_residual_pairs<S, TR, T, true>), dim3(nbdense), dim3(256), 0, s,
-                         phi, rho, res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
-                         r.izmax, dxsqic, dysqic, dzsqic, (T)P.mgparam,
-                         (const uint32_t *)L.mask.p, j0, j1, nbx, nby, zc, nbdense, skip);
-    else
-      hipLaunchKernelGGL((k_residual_pairs<S, TR, T, false>), dim3(nbdense), dim3(256), 0, s,
-                         phi, rho, res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
-                         r.izmax, dxsqic, dysqic, dzsqic, (T)P.mgparam,
-                         (const uint32_t *)nullptr, j0, j1, nbx, nby, zc, nbdense, skip);
-    WM_LAUNCHED();
+    static const bool one_env = getenv("WARPMI_RESIDUAL_ONE_PAIR") != nullptr;
+    bool launched = false;
+    if constexpr (std::is_same<S, double>::value) {
+      if (!one_env && L.g.W >= 64 && L.g.W % 16 == 0) {
+        // two pairs per thread from an odd element (16-byte aligned)
+        const int jb = (j0 & 1) ? j0 : j0 - 1;
+        const int nbx2 = (int)cdiv(j1 - jb + 1, 128);
+        const long long cols2 = (long long)nbx2 * nby;
+        const int zc2 = (int)std::min<long long>(
+            64, std::max<long long>(2, cdiv((long long)nzr * cols2, 2048)));
+        const int nb2 = nbx2 * nby * (int)cdiv(nzr, zc2);
+        if (L.hasmask)
+          hipLaunchKernelGGL((k_residual_pairs2<TR, true>), dim3(nb2), dim3(256), 0, s, phi, rho,
+                             res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
+                             (double)dxsqic, (double)dysqic, (double)dzsqic, P.mgparam,
+                             (const uint32_t *)L.mask.p, jb, j1, nbx2, nby, zc2, nb2, skip);
+        else
+          hipLaunchKernelGGL((k_residual_pairs2<TR, false>), dim3(nb2), dim3(256), 0, s, phi, rho,
+                             res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin, r.izmax,
+                             (double)dxsqic, (double)dysqic, (double)dzsqic, P.mgparam,
+                             (const uint32_t *)nullptr, jb, j1, nbx2, nby, zc2, nb2, skip);
+        WM_LAUNCHED();
+        launched = true;
+      }
+    }
+    if (!launched) {
+      if (L.hasmask)
+        hipLaunchKernelGGL((k_residual_pairs<S, TR, T, true>), dim3(nbdense), dim3(256), 0, s,
+                           phi, rho, res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                           r.izmax, dxsqic, dysqic, dzsqic, (T)P.mgparam,
+                           (const uint32_t *)L.mask.p, j0, j1, nbx, nby, zc, nbdense, skip);
+      else
+        hipLaunchKernelGGL((k_residual_pairs<S, TR, T, false>), dim3(nbdense), dim3(256), 0, s,
+                           phi, rho, res, L.g, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
+                           r.izmax, dxsqic, dysqic, dzsqic, (T)P.mgparam,
+                           (const uint32_t *)nullptr, j0, j1, nbx, nby, zc, nbdense, skip);
+      WM_LAUNCHED();
+    }
   }
   if (L.hasmask) {
     ensure_records<T>(h, L, level, P, true, mgform2init);

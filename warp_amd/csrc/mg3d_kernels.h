@@ -635,6 +635,105 @@ __global__ __launch_bounds__(256) void k_residual_pairs(
   }
 }
 
+// The same for FP64 phi with two elements per thread (j, j+1, j odd): the
+// loads are 16-byte (element 1 of every padded row starts a 128-byte
+// line), so a thread issues 14 loads for four points instead of 12 for two
+// -- the texture-address units, not HBM, set the pace of these z-marches.
+// Each point's residual is the same expression, with the same operands in
+// the same order, as in k_residual_pairs.  Rows must be padded (W a
+// multiple of 16).
+template <typename TR, bool MASK = true>
+__global__ __launch_bounds__(256) void k_residual_pairs2(
+    const double *__restrict__ phi, const double *__restrict__ rho, TR *__restrict__ res,
+    RBGeom g, int ixmin, int ixmax, int iymin, int iymax, int izmin, int izmax, double dxsqic,
+    double dysqic, double dzsqic, double mgparam, const uint32_t *__restrict__ mask, int jb,
+    int j1, int nbx, int nby, int zc, int nbdense, const int *__restrict__ skip) {
+  WM_SKIP();
+  const int lb = xcd_remap(blockIdx.x, nbdense);
+  const int bx = lb % nbx;
+  const int t = lb / nbx;
+  const int by = t % nby;
+  const int bz = t / nby;
+  const int lane = threadIdx.x & 63;
+  const int j = jb + 2 * (bx * 64 + lane);  // odd: elements j and j+1
+  const int iy = iymin + by * 4 + (threadIdx.x >> 6);
+  if (iy > iymax) return;  // whole wave
+  const int za = izmin + bz * zc;
+  const int zb = min(za + zc - 1, izmax);
+  const int py = iy + 1;
+  const int W = g.W, cs = (int)g.cs, pl = g.W * g.PY;
+  const int jc = min(j, W - 3);  // loads stay in the row: j+2 <= W-1
+  bool ok[4];                    // A(j), A(j+1), B(j), B(j+1)
+  {
+    const int ix[4] = {2 * j - 1, 2 * j + 1, 2 * j, 2 * j + 2};
+    const int jj[4] = {j, j + 1, j, j + 1};
+#pragma unroll
+    for (int q = 0; q < 4; q++) ok[q] = jj[q] <= j1 && ix[q] >= ixmin && ix[q] <= ixmax;
+  }
+  auto eA = [&](int pz) { return ((py + pz + 1) & 1) * cs + W * (py + g.PY * pz) + jc; };
+  auto eB = [&](int pz) { return (((py + pz + 1) & 1) ^ 1) * cs + W * (py + g.PY * pz) + jc; };
+  auto ld2 = [&](int e) { return *reinterpret_cast<const double2 *>(phi + e); };
+  struct Pl {
+    double2 ap, bp, yam, yap, ybm, ybp, ra, rb;
+    double xa, xb;
+    uint32_t ma0, ma1, mb0, mb1;
+  };
+  auto load = [&](int pz, Pl &q) {
+    const int a = eA(pz), b = eB(pz);
+    q.ap = ld2(b + pl);
+    q.bp = ld2(a + pl);
+    q.yam = ld2(b - W);
+    q.yap = ld2(b + W);
+    q.ybm = ld2(a - W);
+    q.ybp = ld2(a + W);
+    q.ra = *reinterpret_cast<const double2 *>(rho + a);
+    q.rb = *reinterpret_cast<const double2 *>(rho + b);
+    q.xa = phi[b - 1];
+    q.xb = phi[a + 2];
+    q.ma0 = MASK ? mask[a >> 5] : 0u;
+    q.ma1 = MASK ? mask[(a + 1) >> 5] : 0u;
+    q.mb0 = MASK ? mask[b >> 5] : 0u;
+    q.mb1 = MASK ? mask[(b + 1) >> 5] : 0u;
+  };
+  const int pza = za + 1;
+  double2 am = ld2(eB(pza) - pl);
+  double2 bm = ld2(eA(pza) - pl);
+  double2 a0 = ld2(eA(pza));
+  double2 b0 = ld2(eB(pza));
+  auto bit = [](uint32_t w, int e) { return (w >> (e & 31)) & 1u; };
+  auto st = [&](int e, double v) { res[e] = St<TR>::st((typename St<TR>::T)v); };
+  auto step = [&](int iz, const Pl &q) {
+    const int pz = iz + 1;
+    const int a = eA(pz), b = eB(pz);
+    if (ok[0] && !bit(q.ma0, a))
+      st(a, q.ra.x + (q.xa + b0.x) * dxsqic + (q.yam.x + q.yap.x) * dysqic +
+                (am.x + q.ap.x) * dzsqic - a0.x * mgparam);
+    if (ok[1] && !bit(q.ma1, a + 1))
+      st(a + 1, q.ra.y + (b0.x + b0.y) * dxsqic + (q.yam.y + q.yap.y) * dysqic +
+                    (am.y + q.ap.y) * dzsqic - a0.y * mgparam);
+    if (ok[2] && !bit(q.mb0, b))
+      st(b, q.rb.x + (a0.x + a0.y) * dxsqic + (q.ybm.x + q.ybp.x) * dysqic +
+                (bm.x + q.bp.x) * dzsqic - b0.x * mgparam);
+    if (ok[3] && !bit(q.mb1, b + 1))
+      st(b + 1, q.rb.y + (a0.y + q.xb) * dxsqic + (q.ybm.y + q.ybp.y) * dysqic +
+                    (bm.y + q.bp.y) * dzsqic - b0.y * mgparam);
+    am = a0;
+    bm = b0;
+    a0 = q.ap;
+    b0 = q.bp;
+  };
+  Pl qa, qb;
+  load(za + 1, qa);
+  load(min(za + 2, zb + 1), qb);
+  for (int iz = za; iz <= zb; iz += 2) {
+    step(iz, qa);
+    load(min(iz + 3, zb + 1), qa);
+    if (iz + 1 > zb) break;
+    step(iz + 1, qb);
+    load(min(iz + 4, zb + 1), qb);
+  }
+}
+
 // Zero the points of a red-black residual array outside residual3d's loop
 // range (guard cells and Dirichlet faces).  Threads cover the three pairs
 // of two-point-thick shells; interior points are not touched.
