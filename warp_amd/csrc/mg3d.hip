@@ -1636,6 +1636,9 @@ static void prepare_tail(wm_mg3d *h, const Params &P) {
          std::memcmp(h->tail.items.data(), items.data(), sizeof(TailItem) * items.size()) == 0) &&
         (ops.empty() || std::memcmp(h->tail.ops.data(), ops.data(), sizeof(TailOp) * ops.size()) == 0);
     if (!same) {
+      // captured V-cycles hold the old step count and gather sizes by value
+      h->drop_graphs();
+      h->gkey = wm_mg3d::GKey();
       WM_HIP(hipStreamSynchronize(h->s));
       WM_HIP(hipMemcpy(h->tailtab.p, tab.data(), sizeof(TailLevel) * tab.size(),
                        hipMemcpyHostToDevice));
