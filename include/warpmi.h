@@ -492,9 +492,9 @@ int wm_unet_act_bwd(uint16_t *g, const uint16_t *y, double slope, int64_t n,
                     void *stream);
 /* Packs the fp32 masters w[co][ci][27] as bf16 into wp (forward) and wt
  * (backward over the input: taps mirrored, channel roles exchanged), each
- * of wm_unet_pack_size(cin, cout) = 28*cin*cout values: per 16-channel
- * block and pair of taps (a 28th, zero tap pads the last pair) the K = 32
- * operand of one MFMA step. */
+ * of wm_unet_pack_size(cin, cout) = 30*cin*cout values: per 16-channel
+ * block, z offset of the tap and pair of its 9 (y, x) taps (a zero tenth
+ * tap pads the last pair) the K = 32 operand of one MFMA step. */
 int64_t wm_unet_pack_size(int64_t cin, int64_t cout);
 int wm_unet_pack(const float *w, int64_t cin, int64_t cout, uint16_t *wp,
                  uint16_t *wt, void *stream);
@@ -506,6 +506,10 @@ int wm_unet_pool(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t c,
  * channels); backward sums the eight children into x-sized y. */
 int wm_unet_up(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx,
                uint16_t *y, int64_t cy, int64_t backward, void *stream);
+/* The decoder's concatenated input: y (2D,2H,2W, cx+cs) = [nearest x2
+ * upsampling of x (D,H,W,cx) | skip (2D,2H,2W,cs)]. */
+int wm_unet_upcat(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx,
+                  const uint16_t *skip, int64_t cs, uint16_t *y, void *stream);
 int wm_unet_chcopy(const uint16_t *src, int64_t sc, int64_t soff,
                    uint16_t *dst, int64_t dc, int64_t doff, int64_t c,
                    int64_t nvox, void *stream);

@@ -31,8 +31,11 @@ def _ref_conv(x, w, b, slope):
 
 
 @pytest.mark.parametrize("cin,cout,dims", [(16, 16, (4, 6, 8)), (32, 48, (4, 4, 4)),
-                                           (112, 48, (2, 4, 8))])
+                                           (112, 48, (2, 4, 8)), (16, 32, (256, 64, 16)),
+                                           (16, 80, (256, 32, 16))])
 def test_conv3_forward(device, cin, cout, dims):
+    """Small grids run one row of 16 voxels per wavefront, the two large
+    ones 4 (32 output channels) and 2 (80) rows."""
     from warp_amd import _native as N
     D, H, W = dims
     g = torch.Generator().manual_seed(cin * 7 + cout)
@@ -52,6 +55,106 @@ def test_conv3_forward(device, cin, cout, dims):
     ref = _ref_conv(x, w, b.to(torch.float64), 0.1).permute(1, 2, 3, 0)
     got = yf.cpu().to(torch.float64).reshape(D, H, W, cout)
     assert (got - ref).abs().max() <= 2e-5 * ref.abs().max() + 1e-6
+
+
+@pytest.mark.parametrize("cin,dims", [(16, (19, 22, 40)), (32, (9, 13, 33)), (48, (11, 9, 20)),
+                                      (16, (1, 3, 5))])
+def test_conv3_zmarch_forward(device, cin, dims):
+    """The z-marching kernel (16 output channels, up to 48 input channels):
+    several blocks in x, y and z with partial tiles and chunks, both output
+    forms; the bf16 output is the fp32 output rounded to nearest even."""
+    from warp_amd import _native as N
+    D, H, W = dims
+    cout = 16
+    g = torch.Generator().manual_seed(cin + D)
+    x = _bf16(torch.randn(cin, D, H, W, generator=g, dtype=torch.float64))
+    w = _bf16(torch.randn(cout, cin, 27, generator=g, dtype=torch.float64) * 0.1)
+    b = torch.randn(cout, generator=g, dtype=torch.float64).to(torch.float32)
+    wd = w.to(torch.float32).to(device)
+    wp = torch.empty(int(N.lib().wm_unet_pack_size(cin, cout)), dtype=torch.int16, device=device)
+    wt = torch.empty_like(wp)
+    s = N.stream_of(device)
+    N.call("wm_unet_pack", wd.data_ptr(), cin, cout, wp.data_ptr(), wt.data_ptr(), s)
+    xd = _to_dev_bits(x.permute(1, 2, 3, 0), device)
+    yf = torch.full((D * H * W * cout,), float("nan"), dtype=torch.float32, device=device)
+    yb = torch.zeros(D * H * W * cout, dtype=torch.int16, device=device)
+    bd = b.to(device)
+    N.call("wm_unet_conv3", xd.data_ptr(), cin, yb.data_ptr(), yf.data_ptr(), cout, D, H, W,
+           wp.data_ptr(), bd.data_ptr(), 0.1, s)
+    ref = _ref_conv(x, w, b.to(torch.float64), 0.1).permute(1, 2, 3, 0)
+    got = yf.cpu().reshape(D, H, W, cout)
+    assert torch.isfinite(got).all()
+    assert (got.to(torch.float64) - ref).abs().max() <= 2e-5 * ref.abs().max() + 1e-6
+    assert torch.equal(yb.cpu().view(torch.bfloat16).reshape(D, H, W, cout),
+                       got.to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("c", [16, 12])
+def test_pool_up_copy_add_act_bitwise(device, c):
+    """Pooling, upsampling, channel copy, add and the activation's backward
+    pass, in their 16-byte form (c = 16) and their one-value form (c = 12),
+    against the same fp32 arithmetic in the same order on the CPU, bit for
+    bit (bf16 rounding to nearest even)."""
+    from warp_amd import _native as N
+    s = N.stream_of(device)
+    D, H, W = 6, 4, 10
+    g = torch.Generator().manual_seed(c)
+    bits = lambda t: t.to(torch.bfloat16).view(torch.int16)
+    val = lambda b: b.view(torch.bfloat16).to(torch.float32)
+    x = bits(torch.randn(D, H, W, c, generator=g))
+    xd = x.reshape(-1).to(device)
+    # average pooling and its adjoint
+    y = torch.zeros(D // 2 * H // 2 * W // 2 * c, dtype=torch.int16, device=device)
+    N.call("wm_unet_pool", xd.data_ptr(), D, H, W, c, y.data_ptr(), 0, s)
+    xv = val(x)
+    acc = torch.zeros(D // 2, H // 2, W // 2, c)
+    for k in range(8):
+        acc = acc + xv[(k >> 2)::2, ((k >> 1) & 1)::2, (k & 1)::2]
+    assert torch.equal(y.cpu(), bits(0.125 * acc).reshape(-1))
+    gb = torch.zeros(D * H * W * c, dtype=torch.int16, device=device)
+    N.call("wm_unet_pool", y.data_ptr(), D, H, W, c, gb.data_ptr(), 1, s)
+    up = val(y.cpu().reshape(D // 2, H // 2, W // 2, c))
+    up = up.repeat_interleave(2, 0).repeat_interleave(2, 1).repeat_interleave(2, 2)
+    assert torch.equal(gb.cpu(), bits(0.125 * up).reshape(-1))
+    # nearest upsampling into channels [0, c) of a 2c-channel tensor, and its adjoint
+    cy = 2 * c
+    yu = torch.zeros(8 * D * H * W * cy, dtype=torch.int16, device=device)
+    N.call("wm_unet_up", xd.data_ptr(), D, H, W, c, yu.data_ptr(), cy, 0, s)
+    ref = x.repeat_interleave(2, 0).repeat_interleave(2, 1).repeat_interleave(2, 2)
+    got = yu.cpu().reshape(2 * D, 2 * H, 2 * W, cy)
+    assert torch.equal(got[..., :c], ref) and not got[..., c:].any()
+    gu = bits(torch.randn(2 * D, 2 * H, 2 * W, cy, generator=g))
+    gx = torch.zeros(D * H * W * c, dtype=torch.int16, device=device)
+    N.call("wm_unet_up", gu.reshape(-1).to(device).data_ptr(), D, H, W, c, gx.data_ptr(), cy, 1, s)
+    gv = val(gu)[..., :c]
+    acc = torch.zeros(D, H, W, c)
+    for k in range(8):
+        acc = acc + gv[(k >> 2)::2, ((k >> 1) & 1)::2, (k & 1)::2]
+    assert torch.equal(gx.cpu(), bits(acc).reshape(-1))
+    # the decoder's concatenated input [up(x) | skip]
+    sk = bits(torch.randn(2 * D, 2 * H, 2 * W, c, generator=g))
+    cat = torch.zeros(8 * D * H * W * cy, dtype=torch.int16, device=device)
+    N.call("wm_unet_upcat", xd.data_ptr(), D, H, W, c, sk.reshape(-1).to(device).data_ptr(), c,
+           cat.data_ptr(), s)
+    assert torch.equal(cat.cpu().reshape(2 * D, 2 * H, 2 * W, cy), torch.cat([ref, sk], dim=3))
+    # channel copy (aligned offsets, then an offset of 4 channels)
+    for soff, doff in ((0, c), (4, 0)):
+        dst = torch.zeros(D * H * W * cy, dtype=torch.int16, device=device)
+        cc = c - soff
+        N.call("wm_unet_chcopy", xd.data_ptr(), c, soff, dst.data_ptr(), cy, doff, cc, D * H * W, s)
+        got = dst.cpu().reshape(D, H, W, cy)
+        assert torch.equal(got[..., doff:doff + cc], x[..., soff:])
+        assert got.count_nonzero() == (x[..., soff:] != 0).sum()
+    # a += b and the leaky ReLU's backward pass
+    b = bits(torch.randn(D, H, W, c, generator=g))
+    a = xd.clone()
+    N.call("wm_unet_add", a.data_ptr(), b.reshape(-1).to(device).data_ptr(), a.numel(), s)
+    assert torch.equal(a.cpu(), bits(val(x) + val(b)).reshape(-1))
+    gr = a.clone()
+    N.call("wm_unet_act_bwd", gr.data_ptr(), xd.data_ptr(), 0.1, gr.numel(), s)
+    av = val(a.cpu())
+    want = torch.where(val(x.reshape(-1)) > 0, a.cpu(), bits(av * torch.tensor(0.1, dtype=torch.float32)))
+    assert torch.equal(gr.cpu(), want)
 
 
 def test_conv3_backward(device):

@@ -111,6 +111,7 @@ SIGNATURES = {
     "wm_unet_pool": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _P]),
     "wm_unet_up": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _i64, _P]),
     "wm_unet_chcopy": (C.c_int, [_P, _i64, _i64, _P, _i64, _i64, _i64, _i64, _P]),
+    "wm_unet_upcat": (C.c_int, [_P, _i64, _i64, _i64, _i64, _P, _i64, _P, _P]),
     "wm_unet_add": (C.c_int, [_P, _P, _i64, _P]),
     "wm_unet_adam": (C.c_int, [_P, _P, _P, _P, _i64, _d, _d, _d, _d, _i64, _P]),
     "wm_unet_input": (C.c_int, [_P, _P, _P, _i64, _i64, _i64, _d, _d, _P, _P]),

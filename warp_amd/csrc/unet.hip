@@ -7,11 +7,13 @@
 // accumulation).  The block stages its output tile plus a one-voxel halo of
 // each 16-channel input block in LDS, so every input value is read from
 // memory once per block rather than once per tap.  One MFMA step takes
-// K = 32 as two taps of one 16-channel block (taps 2q and 2q+1; a 28th,
-// zero tap pads the last step), and the weights are packed so that each
-// lane's B operand is 16 contiguous bytes:
-//   Wq[((cb*14 + q)*Cout + co)*32 + k],  k < 16: tap 2q, channel cb*16 + k,
-//                                        k >= 16: tap 2q+1, channel cb*16 + k-16.
+// K = 32 as two taps of one 16-channel block.  The taps are grouped by
+// their z offset dz: the 9 (dy, dx) taps of one dz, numbered t9 = 3*dy +
+// dx, form the pairs (2r, 2r+1), r = 0..4, a zero tenth tap padding the
+// last, so 15 MFMA steps s = 5*dz + r cover the 27 taps; the weights are
+// packed so that each lane's weight operand is 16 contiguous bytes:
+//   Wq[((cb*15 + s)*Cout + co)*32 + k],  k < 16: tap (dz, 2r), channel cb*16 + k,
+//                                        k >= 16: tap (dz, 2r+1), channel cb*16 + k-16.
 // The backward pass over the input is the same convolution with the taps
 // mirrored and the channel roles exchanged (a second packing of the same
 // weights); the backward pass over the weights accumulates
@@ -21,6 +23,7 @@
 #include "../../include/warpmi.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -51,16 +54,25 @@ struct Dims {
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+
+// Two values to bf16 (round to nearest even) in one instruction
+// (v_cvt_pk_bf16_f32), a in the low half.
+__device__ inline uint32_t pk_bf16(float a, float b) {
+  const f2v v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2v));
+}
 
 // Forward convolution.  A block (4 wavefronts) owns a tile of 16 x VT x 4
 // output voxels (x, y, z): wavefront w takes z plane z0 + w and its VT
 // rows of 16 voxels, for all NT*16 output channels (VT*NT accumulators of
 // 16 voxels x 16 channels).  Per 16-channel input block the block copies
 // the tile and its one-voxel halo, 18 x (VT+2) x 6 voxels of 16 channels,
-// into LDS (zeros outside the grid), then runs 14 MFMA steps; in step q a
-// lane reads its A operand (voxel m of its row, shifted by tap 2q + kg/2,
-// channels 8*(kg%2)..+7) from LDS and its B operands from the packed
-// weights, each B operand feeding the VT row tiles.  out = act(conv +
+// into LDS (zeros outside the grid), then runs 15 MFMA steps; in step s a
+// lane reads its A operand (voxel m of its row, shifted by the tap of step
+// s and kg/2, channels 8*(kg%2)..+7) from LDS and its B operands from the
+// packed weights, each B operand feeding the VT row tiles.  out = act(conv +
 // bias), act the leaky ReLU of slope `slope` (1 = identity); bias may be
 // NULL.
 template <int NT, int VT>
@@ -99,13 +111,13 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
       *(uint4 *)(tile + hv * 16 + half * 8) = v;
     }
     __syncthreads();
-    const uint16_t *wcb = wq + (long long)cb * 14 * cout * 32 + 8 * kg;
-#pragma unroll 2
-    for (int q = 0; q < 14; q++) {
-      const int tap = 2 * q + (kg >> 1);
-      const bool real = tap < 27;
-      const int tt = real ? tap : 26;
-      const int dz = tt / 9, dy = (tt / 3) % 3, dx = tt % 3;  // shifts + 1
+    const uint16_t *wcb = wq + (long long)cb * 15 * cout * 32 + 8 * kg;
+#pragma unroll 3
+    for (int q = 0; q < 15; q++) {
+      const int dz = q / 5, t9 = 2 * (q % 5) + (kg >> 1);
+      const bool real = t9 < 9;
+      const int tt = real ? t9 : 8;
+      const int dy = tt / 3, dx = tt % 3;  // shifts + 1
       bf16x8 bq[NT];
 #pragma unroll
       for (int t = 0; t < NT; t++)
@@ -142,6 +154,246 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
         if (yf) yf[vi * cout + co] = v;
       }
     }
+  }
+}
+
+// Forward convolution marching in z, for 16 output channels and up to 48
+// input channels (the finest level of the network).  A block (4
+// wavefronts) owns a column of 16 x 4VT voxels (x, y) over zc output
+// planes; wavefront w takes the rows y0 + w*VT .. +VT-1.  In the packed
+// order the 5 steps of one z offset dz use the same (dy, dx) shifts, so one
+// LDS read of an input row in halo plane p (16 voxels x 8 channels per
+// lane, shifted by (dy, dx)) feeds the MFMAs of the output planes p+1, p
+// and p-1 (dz = 0, 1, 2): a third of the LDS reads of a tile that is
+// staged per output plane.  Per halo plane the block stages 18 x (4VT+2)
+// voxels of all input channels in LDS (zeros outside the grid; a voxel
+// takes 2*NCB + 1 16-byte chunks, the odd stride keeping the lanes'
+// 16-byte reads on distinct banks; two plane buffers where two blocks
+// still fit a CU, so that one barrier per plane suffices), and the loads of
+// the next three planes are in flight during the current plane's MFMAs;
+// after plane p, output plane p-1 is complete and is written.  The MFMA
+// takes the weights as its A operand (rows: output channels) and the
+// voxels as its B operand (columns), so a lane holds 4 consecutive
+// channels of one voxel and writes them with one 8-byte (bf16) or 16-byte
+// (fp32) store.  The
+// weights stay in registers with one input block (NCB = 1), else in LDS
+// (rows of 32 values padded to 5 chunks).  Accumulators of three output
+// planes rotate through slots 0-2; partial sums of planes outside the
+// chunk fall into slots that are cleared before reuse and never written.
+template <int VT, int NCB>
+struct Conv3zShape {
+  static constexpr int NCH = 2 * NCB, STR = NCH + 1;  // chunks per voxel, voxel stride
+  static constexpr int HX = 18, HY = 4 * VT + 2, HV = HX * HY;
+  static constexpr int NCHK = HV * NCH;               // chunks staged per plane
+  static constexpr int PF = (NCHK + 255) / 256;       // staged chunks per thread
+  static constexpr bool BREG = NCB == 1;
+  static constexpr int BCH = BREG ? 0 : 15 * NCB * 16 * 5;
+  // a plane buffer: the plane and a spare chunk; two buffers (one barrier
+  // per plane) when two blocks still fit a CU
+  static constexpr int PB = HV * STR + 1, SPARE = HV * STR;
+  static constexpr bool DB = (2 * PB + BCH) * 16 <= 72 * 1024;
+  static constexpr int LDSCH = (DB ? 2 : 1) * PB + BCH;
+};
+
+template <int V>
+using ic = std::integral_constant<int, V>;
+
+template <int VT, int NCB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *__restrict__ yf,
+              Dims d, const uint16_t *__restrict__ wq, const float *__restrict__ bias,
+              float slope, int ntx, int nty, int zc) {
+  using C = Conv3zShape<VT, NCB>;
+  constexpr int CIN = NCB * 16;
+  __shared__ __align__(16) uint4 lds[C::LDSCH];
+  uint4 *const pl = lds;                                   // staged plane(s)
+  uint4 *const bl = lds + (C::DB ? 2 : 1) * C::PB;  // weights (NCB > 1)
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  int b = blockIdx.x;
+  const int tx = b % ntx;
+  b /= ntx;
+  const int ty = b % nty, tzc = b / nty;
+  const int x0 = tx * 16, y0 = ty * (4 * VT), z0 = tzc * zc;
+  const int z1 = min(z0 + zc, d.D);
+  const long long plane = (long long)d.H * d.W * CIN;
+
+  // this thread's staged chunks: offset in a plane of x (clamped into the
+  // grid; gin false: the chunk is zero) and in LDS (threads without a k-th
+  // chunk store to a spare chunk).  Every load is issued unconditionally, so
+  // that waiting for one plane never waits for the planes behind it.
+  int goff[C::PF], loff[C::PF];
+  bool gin[C::PF];
+#pragma unroll
+  for (int k = 0; k < C::PF; k++) {
+    const int c = tid + 256 * k;
+    goff[k] = 0;
+    loff[k] = C::SPARE;
+    gin[k] = false;
+    if (c < C::NCHK) {
+      const int hv = c / C::NCH, ch = c - hv * C::NCH;
+      const int gx = x0 - 1 + hv % C::HX, gy = y0 - 1 + hv / C::HX;
+      loff[k] = hv * C::STR + ch;
+      gin[k] = gx >= 0 && gx < d.W && gy >= 0 && gy < d.H;
+      if (gin[k]) goff[k] = (gy * d.W + gx) * CIN + ch * 8;
+    }
+  }
+  // planes in flight: halo plane q is loaded into ring entry (q - z0 + 1) % 3,
+  // three planes ahead of the one being computed; planes beyond the chunk's
+  // last (z1) reload plane z1, planes outside the grid load plane 0 or D-1
+  // and are staged as zeros
+  uint4 pf[3][C::PF];
+  bool pin[3];
+  auto fetch = [&](int q, auto e) __attribute__((always_inline)) {
+    constexpr int E = decltype(e)::value;
+    pin[E] = q >= 0 && q < d.D;
+    const int qc = max(0, min(min(q, z1), d.D - 1));
+    const uint16_t *xp = x + (long long)qc * plane;
+#pragma unroll
+    for (int k = 0; k < C::PF; k++) pf[E][k] = *(const uint4 *)(xp + goff[k]);
+  };
+  // LDS buffer of halo plane q
+  auto buf = [&](int q) __attribute__((always_inline)) {
+    return C::DB ? pl + ((q - z0 + 1) & 1) * C::PB : pl;
+  };
+  // ring entry e (halo plane q) into LDS; with one buffer, every wavefront
+  // must first be done with the previous plane, with two, with the plane
+  // before that, which the barrier after the previous plane's store ensures
+  auto stage = [&](auto e, int q) __attribute__((always_inline)) {
+    constexpr int E = decltype(e)::value;
+    if (!C::DB) __syncthreads();
+    uint4 *dst = buf(q);
+#pragma unroll
+    for (int k = 0; k < C::PF; k++) {
+      const uint32_t mk = gin[k] && pin[E] ? 0xffffffffu : 0u;  // values, not addresses
+      const uint4 v = pf[E][k];
+      dst[loff[k]] = make_uint4(v.x & mk, v.y & mk, v.z & mk, v.w & mk);
+    }
+    __syncthreads();
+  };
+
+  bf16x8 breg[C::BREG ? 15 : 1];
+  if constexpr (C::BREG) {
+#pragma unroll
+    for (int s = 0; s < 15; s++) breg[s] = *(const bf16x8 *)(wq + (s * 16 + m) * 32 + 8 * kg);
+  } else {
+    const uint4 *wsrc = (const uint4 *)wq;
+    for (int i = tid; i < 15 * NCB * 16 * 4; i += 256) bl[(i >> 2) * 5 + (i & 3)] = wsrc[i];
+  }
+  float bv[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) bv[i] = bias ? bias[4 * kg + i] : 0.f;
+
+  // A operand offsets: pair r, this lane's tap t9 = 2r + kg/2 (the tenth,
+  // zero tap reads tap 8's voxel and is zeroed), channel half kg%2
+  int aoff[5];
+#pragma unroll
+  for (int r = 0; r < 5; r++) {
+    const int t9 = min(2 * r + (kg >> 1), 8);
+    aoff[r] = ((w * VT + t9 / 3) * C::HX + m + t9 % 3) * C::STR + (kg & 1);
+  }
+  const bool fake = (kg >> 1) == 1;
+  const bf16x8 zero8 = {};
+
+  f4 acc[3][VT];
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+#pragma unroll
+    for (int j = 0; j < VT; j++) acc[q][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // dz = 0 adds to the slot of output plane p+1 (S2), dz = 1 to p (S1),
+  // dz = 2 to p-1 (S0); DZM selects which of the three run
+  auto compute = [&](const uint4 *cur, auto dzm, auto s0, auto s1, auto s2)
+                     __attribute__((always_inline)) {
+    constexpr int DZM = decltype(dzm)::value, S0 = decltype(s0)::value;
+    constexpr int S1 = decltype(s1)::value, S2 = decltype(s2)::value;
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+      // the block's A operands of this input block first, then the MFMAs
+      bf16x8 a[5][VT];
+#pragma unroll
+      for (int r = 0; r < 5; r++)
+#pragma unroll
+        for (int j = 0; j < VT; j++) {
+          a[r][j] = *(const bf16x8 *)&cur[aoff[r] + j * C::HX * C::STR + 2 * cb];
+          if (r == 4 && fake) a[r][j] = zero8;
+        }
+#pragma unroll
+      for (int r = 0; r < 5; r++) {
+        bf16x8 bw[3];
+#pragma unroll
+        for (int dz = 0; dz < 3; dz++) {
+          if (!((DZM >> dz) & 1)) continue;
+          if constexpr (C::BREG)
+            bw[dz] = breg[dz * 5 + r];
+          else
+            bw[dz] = *(const bf16x8 *)&bl[((cb * 15 + dz * 5 + r) * 16 + m) * 5 + kg];
+        }
+#pragma unroll
+        for (int j = 0; j < VT; j++) {
+          if (DZM & 1)
+            acc[S2][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[0], a[r][j], acc[S2][j], 0, 0, 0);
+          if (DZM & 2)
+            acc[S1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[1], a[r][j], acc[S1][j], 0, 0, 0);
+          if (DZM & 4)
+            acc[S0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[2], a[r][j], acc[S0][j], 0, 0, 0);
+        }
+      }
+    }
+  };
+  // lane holds D[4*kg + i][m]: channel 4*kg + i of voxel x0 + m, row j
+  auto write = [&](int o, auto s) __attribute__((always_inline)) {
+    constexpr int S = decltype(s)::value;
+    const int xx = x0 + m;
+#pragma unroll
+    for (int j = 0; j < VT; j++) {
+      const int yy = y0 + w * VT + j;
+      if (yy < d.H && xx < d.W) {
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          v[i] = acc[S][j][i] + bv[i];
+          v[i] = v[i] > 0.f ? v[i] : v[i] * slope;
+        }
+        const long long vi = (((long long)o * d.H + yy) * d.W + xx) * 16 + 4 * kg;
+        if (y) *(uint2 *)(y + vi) = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
+        if (yf) *(float4 *)(yf + vi) = make_float4(v[0], v[1], v[2], v[3]);
+      }
+      acc[S][j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+
+  fetch(z0 - 1, ic<0>());
+  fetch(z0, ic<1>());
+  fetch(z0 + 1, ic<2>());  // z0 + 1 <= z1
+  // halo plane z0-1: only its dz = 0 terms (output plane z0, slot 2) count
+  stage(ic<0>(), z0 - 1);
+  fetch(z0 + 2, ic<0>());
+  if (z0 >= 1) compute(buf(z0 - 1), ic<1>(), ic<0>(), ic<1>(), ic<2>());
+  // plane z0: no dz = 2 terms (they would belong to plane z0-1)
+  stage(ic<1>(), z0);
+  fetch(z0 + 3, ic<1>());
+  compute(buf(z0), ic<3>(), ic<1>(), ic<2>(), ic<0>());
+  int p = z0 + 1;
+  // one halo plane p > z0; slots (S0, S1, S2) hold output planes (p-1, p, p+1);
+  // the plane is in ring entry S0
+  auto step = [&](auto s0, auto s1, auto s2) __attribute__((always_inline)) -> bool {
+    stage(s0, p);
+    if (p == z1) {  // last halo plane: only its dz = 2 terms (plane z1-1)
+      if (p < d.D) compute(buf(p), ic<4>(), s0, s1, s2);
+      write(p - 1, s0);
+      return true;
+    }
+    fetch(p + 3, s0);
+    compute(buf(p), ic<7>(), s0, s1, s2);
+    write(p - 1, s0);
+    p++;
+    return false;
+  };
+  while (true) {
+    if (step(ic<2>(), ic<0>(), ic<1>())) break;
+    if (step(ic<0>(), ic<1>(), ic<2>())) break;
+    if (step(ic<1>(), ic<2>(), ic<0>())) break;
   }
 }
 
@@ -372,26 +624,26 @@ __global__ __launch_bounds__(256) void k_act_bwd(uint16_t *__restrict__ g,
 __global__ __launch_bounds__(256) void k_pack(const float *__restrict__ w, int cin, int cout,
                                               uint16_t *__restrict__ wq,
                                               uint16_t *__restrict__ wqt) {
-  // both layouts hold 28 * cin * cout values (14 tap pairs)
-  const long long n = 28ll * cin * cout;
+  // both layouts hold 30 * cin * cout values (15 steps of two taps)
+  const long long n = 30ll * cin * cout;
   const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
   const int k = (int)(t % 32);
   const long long u = t / 32;
   const int half = k >> 4, kl = k & 15;
-  {  // forward: u = (cb*14 + q)*cout + co
+  {  // forward: u = (cb*15 + s)*cout + co
     const int co = (int)(u % cout);
     const long long v = u / cout;
-    const int q = (int)(v % 14), cb = (int)(v / 14);
-    const int tap = 2 * q + half, ci = cb * 16 + kl;
-    wq[t] = tap < 27 ? f2bf(w[((long long)co * cin + ci) * 27 + tap]) : (uint16_t)0;
+    const int s = (int)(v % 15), cb = (int)(v / 15);
+    const int t9 = 2 * (s % 5) + half, tap = 9 * (s / 5) + t9, ci = cb * 16 + kl;
+    wq[t] = t9 < 9 ? f2bf(w[((long long)co * cin + ci) * 27 + tap]) : (uint16_t)0;
   }
-  {  // backward over the input: u = (cob*14 + q)*cin + ci, taps mirrored
+  {  // backward over the input: u = (cob*15 + s)*cin + ci, taps mirrored
     const int ci = (int)(u % cin);
     const long long v = u / cin;
-    const int q = (int)(v % 14), cob = (int)(v / 14);
-    const int tap = 2 * q + half, co = cob * 16 + kl;
-    wqt[t] = tap < 27 ? f2bf(w[((long long)co * cin + ci) * 27 + (26 - tap)]) : (uint16_t)0;
+    const int s = (int)(v % 15), cob = (int)(v / 15);
+    const int t9 = 2 * (s % 5) + half, tap = 9 * (s / 5) + t9, co = cob * 16 + kl;
+    wqt[t] = t9 < 9 ? f2bf(w[((long long)co * cin + ci) * 27 + (26 - tap)]) : (uint16_t)0;
   }
 }
 
@@ -475,6 +727,148 @@ __global__ __launch_bounds__(256) void k_chcopy(const uint16_t *__restrict__ src
   int ch = (int)(t % c);
   long long v = t / c;
   dst[v * dc + doff + ch] = src[v * sc + soff + ch];
+}
+
+// The same operations on 8 channels (16 bytes) per thread, for channel
+// counts and offsets that are multiples of 8 and fewer than 2^31 groups of
+// 8: each channel's arithmetic, and its order, is that of the kernels above.
+__device__ inline void unpack8(uint4 u, float *f) {
+  f[0] = bf2f((uint16_t)(u.x & 0xffffu)); f[1] = bf2f((uint16_t)(u.x >> 16));
+  f[2] = bf2f((uint16_t)(u.y & 0xffffu)); f[3] = bf2f((uint16_t)(u.y >> 16));
+  f[4] = bf2f((uint16_t)(u.z & 0xffffu)); f[5] = bf2f((uint16_t)(u.z >> 16));
+  f[6] = bf2f((uint16_t)(u.w & 0xffffu)); f[7] = bf2f((uint16_t)(u.w >> 16));
+}
+__device__ inline uint4 pack8(const float *f) {
+  auto two = [](float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); };
+  return make_uint4(two(f[0], f[1]), two(f[2], f[3]), two(f[4], f[5]), two(f[6], f[7]));
+}
+
+__global__ __launch_bounds__(256) void k_pool8(const uint16_t *__restrict__ x, Dims d, int c,
+                                               uint16_t *__restrict__ y) {
+  const unsigned D2 = d.D / 2, H2 = d.H / 2, W2 = d.W / 2, c8 = c / 8;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= D2 * H2 * W2 * c8) return;
+  const unsigned g = t % c8, v = t / c8;
+  const unsigned xw = v % W2, r = v / W2, yy = r % H2, zz = r / H2;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+  for (int k = 0; k < 8; k++) {
+    const long long sv =
+        ((long long)(2 * zz + (k >> 2)) * d.H + 2 * yy + ((k >> 1) & 1)) * d.W + 2 * xw + (k & 1);
+    unpack8(*(const uint4 *)(x + sv * c + 8 * g), f);
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] += f[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) s[i] = 0.125f * s[i];
+  *(uint4 *)(y + (long long)v * c + 8 * g) = pack8(s);
+}
+
+__global__ __launch_bounds__(256) void k_pool_bwd8(const uint16_t *__restrict__ gy, Dims d,
+                                                   int c, uint16_t *__restrict__ gx) {
+  const unsigned c8 = c / 8, W = d.W, H = d.H;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= (unsigned)d.D * H * W * c8) return;
+  const unsigned g = t % c8, v = t / c8;
+  const unsigned xw = v % W, r = v / W, yy = r % H, zz = r / H;
+  const long long cv = ((long long)(zz / 2) * (H / 2) + yy / 2) * (W / 2) + xw / 2;
+  float f[8];
+  unpack8(*(const uint4 *)(gy + cv * c + 8 * g), f);
+#pragma unroll
+  for (int i = 0; i < 8; i++) f[i] = 0.125f * f[i];
+  *(uint4 *)(gx + (long long)v * c + 8 * g) = pack8(f);
+}
+
+__global__ __launch_bounds__(256) void k_up8(const uint16_t *__restrict__ x, Dims d, int cx,
+                                             uint16_t *__restrict__ y, int cy) {
+  const unsigned c8 = cx / 8, W2 = 2 * d.W, H2 = 2 * d.H;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= 8u * d.D * d.H * d.W * c8) return;
+  const unsigned g = t % c8, v = t / c8;
+  const unsigned xw = v % W2, r = v / W2, yy = r % H2, zz = r / H2;
+  const long long sv = ((long long)(zz / 2) * d.H + yy / 2) * d.W + xw / 2;
+  *(uint4 *)(y + (long long)v * cy + 8 * g) = *(const uint4 *)(x + sv * cx + 8 * g);
+}
+
+// The decoder's input: y (2D, 2H, 2W, cx + cs) = [nearest x2 upsampling of
+// x (D, H, W, cx) | skip (2D, 2H, 2W, cs)], each voxel's channels written
+// as whole 16-byte groups by consecutive threads.
+__global__ __launch_bounds__(256) void k_upcat8(const uint16_t *__restrict__ x, Dims d, int cx,
+                                                const uint16_t *__restrict__ skip, int cs,
+                                                uint16_t *__restrict__ y) {
+  const unsigned cx8 = cx / 8, c8 = (cx + cs) / 8, W2 = 2 * d.W, H2 = 2 * d.H;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= 8u * d.D * d.H * d.W * c8) return;
+  const unsigned g = t % c8, v = t / c8;
+  uint4 val;
+  if (g < cx8) {
+    const unsigned xw = v % W2, r = v / W2, yy = r % H2, zz = r / H2;
+    const long long sv = ((long long)(zz / 2) * d.H + yy / 2) * d.W + xw / 2;
+    val = *(const uint4 *)(x + sv * cx + 8 * g);
+  } else {
+    val = *(const uint4 *)(skip + (long long)v * cs + 8 * (g - cx8));
+  }
+  *(uint4 *)(y + (long long)v * (cx + cs) + 8 * g) = val;
+}
+
+__global__ __launch_bounds__(256) void k_up_bwd8(const uint16_t *__restrict__ gy, Dims d,
+                                                 int cx, int cy, uint16_t *__restrict__ gx) {
+  const unsigned c8 = cx / 8, W = d.W, H = d.H;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= (unsigned)d.D * H * W * c8) return;
+  const unsigned g = t % c8, v = t / c8;
+  const unsigned xw = v % W, r = v / W, yy = r % H, zz = r / H;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+  for (int k = 0; k < 8; k++) {
+    const long long cv = ((long long)(2 * zz + (k >> 2)) * (2 * H) + 2 * yy + ((k >> 1) & 1)) *
+                             (2 * W) + 2 * xw + (k & 1);
+    unpack8(*(const uint4 *)(gy + cv * cy + 8 * g), f);
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] += f[i];
+  }
+  *(uint4 *)(gx + (long long)v * cx + 8 * g) = pack8(s);
+}
+
+__global__ __launch_bounds__(256) void k_chcopy8(const uint16_t *__restrict__ src, int sc,
+                                                 int soff, uint16_t *__restrict__ dst, int dc,
+                                                 int doff, int c, unsigned nvox) {
+  const unsigned c8 = c / 8;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= nvox * c8) return;
+  const unsigned g = t % c8, v = t / c8;
+  *(uint4 *)(dst + (long long)v * dc + doff + 8 * g) =
+      *(const uint4 *)(src + (long long)v * sc + soff + 8 * g);
+}
+
+__global__ __launch_bounds__(256) void k_act_bwd8(uint16_t *__restrict__ g,
+                                                  const uint16_t *__restrict__ y, float slope,
+                                                  unsigned n8) {
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= n8) return;
+  float fg[8], fy[8];
+  uint4 *gp = (uint4 *)g + t;
+  const uint4 gu = *gp;
+  unpack8(gu, fg);
+  unpack8(((const uint4 *)y)[t], fy);
+  uint16_t o[8] = {(uint16_t)(gu.x & 0xffffu), (uint16_t)(gu.x >> 16), (uint16_t)(gu.y & 0xffffu),
+                   (uint16_t)(gu.y >> 16),     (uint16_t)(gu.z & 0xffffu), (uint16_t)(gu.z >> 16),
+                   (uint16_t)(gu.w & 0xffffu), (uint16_t)(gu.w >> 16)};
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (!(fy[i] > 0.f)) o[i] = f2bf(fg[i] * slope);
+  *gp = make_uint4((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16),
+                   (uint32_t)o[4] | ((uint32_t)o[5] << 16), (uint32_t)o[6] | ((uint32_t)o[7] << 16));
+}
+
+__global__ __launch_bounds__(256) void k_add8(uint16_t *__restrict__ a,
+                                              const uint16_t *__restrict__ b, unsigned n8) {
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= n8) return;
+  float fa[8], fb[8];
+  unpack8(((const uint4 *)a)[t], fa);
+  unpack8(((const uint4 *)b)[t], fb);
+#pragma unroll
+  for (int i = 0; i < 8; i++) fa[i] = fa[i] + fb[i];
+  ((uint4 *)a)[t] = pack8(fa);
 }
 
 // a += b elementwise (bf16).
@@ -697,6 +1091,7 @@ __global__ __launch_bounds__(256) void k_learn_apply(const float *__restrict__ o
 }
 
 inline int nb(long long n) { return (int)std::max<long long>(1, (n + 255) / 256); }
+inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
 
@@ -712,23 +1107,52 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
     WM_REQUIRE(D > 0 && H > 0 && W > 0, "wm_unet_conv3: empty grid");
     WM_REQUIRE(D * H * W < (1ll << 31), "wm_unet_conv3: grid too large");
     Dims d{(int)D, (int)H, (int)W};
-    // rows of 16 voxels per wavefront: 4 up to 64 output channels, 2 above
-    const int vt = cout <= 64 ? 4 : 2;
-    const int ntx = (int)((W + 15) / 16), nty = (int)((H + vt - 1) / vt);
-    const long long nblk = (long long)ntx * nty * ((D + 3) / 4);
-    WM_REQUIRE(nblk < (1ll << 31), "wm_unet_conv3: grid too large");
-    const dim3 g((unsigned)nblk), bl(256);
     hipStream_t s = wm_stream(stream);
     float sl = (float)slope;
+    if (cout == 16 && cin <= 48 && H * W * cin < (1ll << 31)) {
+      // z-marching form; chunks of zc planes, about 2048 blocks or more
+      const int ncb = (int)(cin / 16), vt = 2;
+      const int ntx = (int)((W + 15) / 16), nty = (int)((H + 4 * vt - 1) / (4 * vt));
+      const long long nxy = (long long)ntx * nty;
+      const int zc = (int)std::max<long long>(4, std::min<long long>(32, D * nxy / 2048));
+      const long long nblk = nxy * ((D + zc - 1) / zc);
+      WM_REQUIRE(nblk < (1ll << 31), "wm_unet_conv3: grid too large");
+      const dim3 g((unsigned)nblk), bl(256);
+      switch (ncb) {
+        case 1: hipLaunchKernelGGL((k_conv3z<2, 1>), g, bl, 0, s, x, y, yf, d, wp, bias, sl, ntx, nty, zc); break;
+        case 2: hipLaunchKernelGGL((k_conv3z<2, 2>), g, bl, 0, s, x, y, yf, d, wp, bias, sl, ntx, nty, zc); break;
+        default: hipLaunchKernelGGL((k_conv3z<2, 3>), g, bl, 0, s, x, y, yf, d, wp, bias, sl, ntx, nty, zc); break;
+      }
+      WM_LAUNCHED();
+      return 0;
+    }
+    // rows of 16 voxels per wavefront: 4 up to 64 output channels, 2 above;
+    // 1 where that leaves fewer than 1024 blocks (the coarse levels)
+    const int ntx = (int)((W + 15) / 16), nzb = (int)((D + 3) / 4);
+    int vt = cout <= 64 ? 4 : 2;
+    if ((long long)ntx * ((H + vt - 1) / vt) * nzb < 1024) vt = 1;
+    const int nty = (int)((H + vt - 1) / vt);
+    const long long nblk = (long long)ntx * nty * nzb;
+    WM_REQUIRE(nblk < (1ll << 31), "wm_unet_conv3: grid too large");
+    const dim3 g((unsigned)nblk), bl(256);
+    auto run = [&](auto nt) {
+      constexpr int NT = decltype(nt)::value;
+      if (vt == 1)
+        hipLaunchKernelGGL((k_conv3<NT, 1>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx,
+                           nty);
+      else
+        hipLaunchKernelGGL((k_conv3<NT, NT <= 4 ? 4 : 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp,
+                           bias, sl, ntx, nty);
+    };
     switch (cout / 16) {
-      case 1: hipLaunchKernelGGL((k_conv3<1, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 2: hipLaunchKernelGGL((k_conv3<2, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 3: hipLaunchKernelGGL((k_conv3<3, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 4: hipLaunchKernelGGL((k_conv3<4, 4>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 5: hipLaunchKernelGGL((k_conv3<5, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 6: hipLaunchKernelGGL((k_conv3<6, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 7: hipLaunchKernelGGL((k_conv3<7, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
-      case 8: hipLaunchKernelGGL((k_conv3<8, 2>), g, bl, 0, s, x, (int)cin, y, yf, d, wp, bias, sl, ntx, nty); break;
+      case 1: run(ic<1>()); break;
+      case 2: run(ic<2>()); break;
+      case 3: run(ic<3>()); break;
+      case 4: run(ic<4>()); break;
+      case 5: run(ic<5>()); break;
+      case 6: run(ic<6>()); break;
+      case 7: run(ic<7>()); break;
+      case 8: run(ic<8>()); break;
       default: throw wm_error("wm_unet_conv3: cout > 128");
     }
     WM_LAUNCHED();
@@ -785,21 +1209,25 @@ int wm_unet_conv3_wgrad(const uint16_t *x, int64_t cin, const uint16_t *g, int64
 int wm_unet_act_bwd(uint16_t *g, const uint16_t *y, double slope, int64_t n, void *stream) {
   WM_ENTRY({
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_act_bwd, dim3(nb(n)), dim3(256), 0, wm_stream(stream), g, y, (float)slope,
-                       (long long)n);
+    if (n % 8 == 0 && n / 8 < (1ll << 31) && al16(g) && al16(y))
+      hipLaunchKernelGGL(k_act_bwd8, dim3(nb(n / 8)), dim3(256), 0, wm_stream(stream), g, y,
+                         (float)slope, (unsigned)(n / 8));
+    else
+      hipLaunchKernelGGL(k_act_bwd, dim3(nb(n)), dim3(256), 0, wm_stream(stream), g, y,
+                         (float)slope, (long long)n);
     WM_LAUNCHED();
   });
 }
 
 int64_t wm_unet_pack_size(int64_t cin, int64_t cout) {
-  return cin > 0 && cout > 0 ? 28 * cin * cout : 0;
+  return cin > 0 && cout > 0 ? 30 * cin * cout : 0;
 }
 
 int wm_unet_pack(const float *w, int64_t cin, int64_t cout, uint16_t *wp, uint16_t *wt,
                  void *stream) {
   WM_ENTRY({
     WM_REQUIRE(cin % 16 == 0 && cout % 16 == 0, "wm_unet_pack: channels must be multiples of 16");
-    long long n = 28ll * cin * cout;
+    long long n = 30ll * cin * cout;
     hipLaunchKernelGGL(k_pack, dim3(nb(n)), dim3(256), 0, wm_stream(stream), w, (int)cin, (int)cout,
                        wp, wt);
     WM_LAUNCHED();
@@ -812,11 +1240,18 @@ int wm_unet_pool(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t c, 
     WM_REQUIRE(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "wm_unet_pool: odd dimension");
     Dims d{(int)D, (int)H, (int)W};
     hipStream_t s = wm_stream(stream);
+    const bool v8 = c % 8 == 0 && D * H * W * c / 8 < (1ll << 31) && al16(x) && al16(y);
     if (!backward) {
-      hipLaunchKernelGGL(k_pool, dim3(nb(D * H * W * c / 8)), dim3(256), 0, s, x, d, (int)c, y);
+      if (v8)
+        hipLaunchKernelGGL(k_pool8, dim3(nb(D * H * W * c / 64)), dim3(256), 0, s, x, d, (int)c, y);
+      else
+        hipLaunchKernelGGL(k_pool, dim3(nb(D * H * W * c / 8)), dim3(256), 0, s, x, d, (int)c, y);
     } else {
       // x is the gradient at the pooled size, y the gradient at the full size
-      hipLaunchKernelGGL(k_pool_bwd, dim3(nb(D * H * W * c)), dim3(256), 0, s, x, d, (int)c, y);
+      if (v8)
+        hipLaunchKernelGGL(k_pool_bwd8, dim3(nb(D * H * W * c / 8)), dim3(256), 0, s, x, d, (int)c, y);
+      else
+        hipLaunchKernelGGL(k_pool_bwd, dim3(nb(D * H * W * c)), dim3(256), 0, s, x, d, (int)c, y);
     }
     WM_LAUNCHED();
   });
@@ -827,14 +1262,44 @@ int wm_unet_up(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx, u
   WM_ENTRY({
     Dims d{(int)D, (int)H, (int)W};
     hipStream_t s = wm_stream(stream);
+    const bool v8 = cx % 8 == 0 && cy % 8 == 0 && D * H * W * cx < (1ll << 31) && al16(x) && al16(y);
     if (!backward) {
-      hipLaunchKernelGGL(k_up, dim3(nb(8 * D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx, y,
-                         (int)cy);
+      if (v8)
+        hipLaunchKernelGGL(k_up8, dim3(nb(D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx, y,
+                           (int)cy);
+      else
+        hipLaunchKernelGGL(k_up, dim3(nb(8 * D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx, y,
+                           (int)cy);
     } else {
       // x: gradient at the upsampled size (cy channels, first cx used);
       // y: gradient at the coarse size (cx channels)
-      hipLaunchKernelGGL(k_up_bwd, dim3(nb(D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx,
-                         (int)cy, y);
+      if (v8)
+        hipLaunchKernelGGL(k_up_bwd8, dim3(nb(D * H * W * cx / 8)), dim3(256), 0, s, x, d, (int)cx,
+                           (int)cy, y);
+      else
+        hipLaunchKernelGGL(k_up_bwd, dim3(nb(D * H * W * cx)), dim3(256), 0, s, x, d, (int)cx,
+                           (int)cy, y);
+    }
+    WM_LAUNCHED();
+  });
+}
+
+int wm_unet_upcat(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx,
+                  const uint16_t *skip, int64_t cs, uint16_t *y, void *stream) {
+  WM_ENTRY({
+    WM_REQUIRE(x && skip && y, "wm_unet_upcat: NULL argument");
+    Dims d{(int)D, (int)H, (int)W};
+    hipStream_t s = wm_stream(stream);
+    const long long nf = 8 * D * H * W;  // fine voxels
+    if (cx % 8 == 0 && cs % 8 == 0 && nf * (cx + cs) / 8 < (1ll << 31) && al16(x) && al16(skip) &&
+        al16(y)) {
+      hipLaunchKernelGGL(k_upcat8, dim3(nb(nf * (cx + cs) / 8)), dim3(256), 0, s, x, d, (int)cx,
+                         skip, (int)cs, y);
+    } else {
+      hipLaunchKernelGGL(k_up, dim3(nb(nf * cx)), dim3(256), 0, s, x, d, (int)cx, y,
+                         (int)(cx + cs));
+      hipLaunchKernelGGL(k_chcopy, dim3(nb(nf * cs)), dim3(256), 0, s, skip, (int)cs, 0, y,
+                         (int)(cx + cs), (int)cx, (int)cs, nf);
     }
     WM_LAUNCHED();
   });
@@ -843,8 +1308,12 @@ int wm_unet_up(const uint16_t *x, int64_t D, int64_t H, int64_t W, int64_t cx, u
 int wm_unet_chcopy(const uint16_t *src, int64_t sc, int64_t soff, uint16_t *dst, int64_t dc,
                    int64_t doff, int64_t c, int64_t nvox, void *stream) {
   WM_ENTRY({
-    hipLaunchKernelGGL(k_chcopy, dim3(nb(nvox * c)), dim3(256), 0, wm_stream(stream), src, (int)sc,
-                       (int)soff, dst, (int)dc, (int)doff, (int)c, (long long)nvox);
+    if ((sc | soff | dc | doff | c) % 8 == 0 && nvox * c / 8 < (1ll << 31) && al16(src) && al16(dst))
+      hipLaunchKernelGGL(k_chcopy8, dim3(nb(nvox * c / 8)), dim3(256), 0, wm_stream(stream), src,
+                         (int)sc, (int)soff, dst, (int)dc, (int)doff, (int)c, (unsigned)nvox);
+    else
+      hipLaunchKernelGGL(k_chcopy, dim3(nb(nvox * c)), dim3(256), 0, wm_stream(stream), src,
+                         (int)sc, (int)soff, dst, (int)dc, (int)doff, (int)c, (long long)nvox);
     WM_LAUNCHED();
   });
 }
@@ -852,7 +1321,11 @@ int wm_unet_chcopy(const uint16_t *src, int64_t sc, int64_t soff, uint16_t *dst,
 int wm_unet_add(uint16_t *a, const uint16_t *b, int64_t n, void *stream) {
   WM_ENTRY({
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_add, dim3(nb(n)), dim3(256), 0, wm_stream(stream), a, b, (long long)n);
+    if (n % 8 == 0 && n / 8 < (1ll << 31) && al16(a) && al16(b))
+      hipLaunchKernelGGL(k_add8, dim3(nb(n / 8)), dim3(256), 0, wm_stream(stream), a, b,
+                         (unsigned)(n / 8));
+    else
+      hipLaunchKernelGGL(k_add, dim3(nb(n)), dim3(256), 0, wm_stream(stream), a, b, (long long)n);
     WM_LAUNCHED();
   });
 }

@@ -122,9 +122,8 @@ class UNet3D:
             nv = ds[0] * ds[1] * ds[2]
             cup = ca.cin - sk.numel() // nv
             cat = self._t(nv, ca.cin)
-            N.call("wm_unet_up", h.data_ptr(), d[0], d[1], d[2], cup, cat.data_ptr(), ca.cin, 0, s)
-            N.call("wm_unet_chcopy", sk.data_ptr(), ca.cin - cup, 0, cat.data_ptr(), ca.cin, cup,
-                   ca.cin - cup, nv, s)
+            N.call("wm_unet_upcat", h.data_ptr(), d[0], d[1], d[2], cup, sk.data_ptr(),
+                   ca.cin - cup, cat.data_ptr(), s)
             d = ds
             a = self._t(nv, ca.cout)
             self._conv(ca, cat, d, y=a)
