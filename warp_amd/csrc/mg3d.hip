@@ -245,11 +245,13 @@ struct wm_mg3d {
   // level-0 extras
   DevBuf<double> save64, rhosave64;
   DevBuf<double> convparts;  // per-wavefront maxima of the fused convergence test
-  // The last FP64 solve left its phi in phi64 with save64 equal to it and
+  // The last FP64 solve left its phi in phi64 with save64 equal to it (or,
+  // last_pp, equal at every point no sweep writes) and
   // the pinned points at their values (last_ok); when the caller declares
   // its phi unchanged since (wm_mg3d_reuse_phi: reuse_next), the next solve
   // starts from that state.
   bool last_ok = false, reuse_next = false;
+  bool last_pp = false;  // the last solve exchanged the phi and save buffers
   bool last_reused = false;  // the last solve started from that state
   const double *last_phi = nullptr;
   DevBuf<SolveState> state;
@@ -257,13 +259,19 @@ struct wm_mg3d {
   SolveState *state_h = nullptr;  // pinned
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  // captured V-cycle graphs
-  hipGraphExec_t gexec = nullptr;
-  hipGraphExec_t gexec_rest = nullptr;
+  // Which of the two level-0 FP64 buffers holds phi: the V-cycles that
+  // keep their starting phi in the save buffer exchange the two (phi64 and
+  // save64 swap their storage, orient flips).
+  int orient = 0;
+  // captured V-cycle graphs, per orientation at their start
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec_rest[2] = {nullptr, nullptr};
   void drop_graphs() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (gexec_rest) (void)hipGraphExecDestroy(gexec_rest);
-    gexec = gexec_rest = nullptr;
+    for (int o = 0; o < 2; o++) {
+      if (gexec[o]) (void)hipGraphExecDestroy(gexec[o]);
+      if (gexec_rest[o]) (void)hipGraphExecDestroy(gexec_rest[o]);
+      gexec[o] = gexec_rest[o] = nullptr;
+    }
   }
   struct GKey {
     int prec = -1, mgform = -1, down = -1, up = -1, lcnd = -1, icnd = -1, maxit = -1;
@@ -738,11 +746,13 @@ static void pin_and_mirror(wm_mg3d *h, Level &L, int level, const Params &P, T *
 }
 
 // sorhalfpass3d (f3d_mgrid.F:2095-2291) on level data in red-black layout.
+// src (not NULL): read the values before the sweep from src, write phi
+// (k_sor_zmarch); wsave false: the convergence test leaves save unchanged.
 template <typename S>
 static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
                      const int *skip, hipStream_t s, bool inpass = true,
                      double *cpart = nullptr, double *cpart_end = nullptr,
-                     int *nslots = nullptr) {
+                     int *nslots = nullptr, const S *src = nullptr, bool wsave = true) {
   typedef typename St<S>::T T;
   S *phi = phi_of<S>(L).p;
   const S *rho = rho_of<S>(L).p;
@@ -766,7 +776,7 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
       getenv("WARPMI_TILED_BELOW") ? atoll(getenv("WARPMI_TILED_BELOW")) : 0;
   const long long nodes = (long long)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
   const bool tiled = (tiled_env || (cpart == nullptr && nodes < tiled_below)) &&
-                     !std::is_same<S, bf16>::value;
+                     !std::is_same<S, bf16>::value && src == nullptr;
   // z-marching sweep: colour elements j0..j1 cover ix in [ixmin, ixmax]
   // for both row parities; z chunks sized for about 2048 dense blocks.
   int j0 = (r.ixmin + 1) / 2, j1 = (r.ixmax + 1) / 2;
@@ -807,12 +817,14 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
         hipLaunchKernelGGL((k_sor_zmarch<S, T, true, true>), dim3(nbdense + nbsub), dim3(nthr),
                            0, s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
                            r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)L.mask.p, sub,
-                           j0, nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip);
+                           j0, nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip, src,
+                           (int)wsave);
       else
         hipLaunchKernelGGL((k_sor_zmarch<S, T, true, false>), dim3(nbdense + nbsub), dim3(nthr),
                            0, s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
                            r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)nullptr, sub,
-                           j0, nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip);
+                           j0, nbx, nby, zc, nbdense, nbsub, h->save64.p, cpart, skip, src,
+                           (int)wsave);
     }
     *nslots = (int)ns;
   } else if (!tiled) {
@@ -820,12 +832,14 @@ static void halfpass(wm_mg3d *h, Level &L, int level, const Params &P, int c,
       hipLaunchKernelGGL((k_sor_zmarch<S, T, false, true>), dim3(nbdense + nbsub), dim3(nthr), 0,
                          s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
                          r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)L.mask.p, sub,
-                         j0, nbx, nby, zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip);
+                         j0, nbx, nby, zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip,
+                         src, 1);
     else
       hipLaunchKernelGGL((k_sor_zmarch<S, T, false, false>), dim3(nbdense + nbsub), dim3(nthr),
                          0, s, phi, rho, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax, r.izmin,
                          r.izmax, dxsqic, dysqic, dzsqic, spm1, (const uint32_t *)nullptr, sub,
-                         j0, nbx, nby, zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip);
+                         j0, nbx, nby, zc, nbdense, nbsub, (S *)nullptr, (double *)nullptr, skip,
+                         src, 1);
   }
   WM_LAUNCHED();
   if (timed) {
@@ -930,12 +944,12 @@ static int records_only(Level &L, const Params &P, int c, S *phi, const LoopRang
       hipLaunchKernelGGL((k_sor_zmarch<S, T, true>), dim3(nbsub), dim3(nthr), 0, s, phi,
                          (const S *)rho_of<S>(L).p, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax,
                          r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, sub, 0, 1, 1, 1,
-                         0, nbsub, save, cpart, skip);
+                         0, nbsub, save, cpart, skip, (const S *)nullptr, 1);
   } else {
     hipLaunchKernelGGL((k_sor_zmarch<S>), dim3(nbsub), dim3(nthr), 0, s, phi,
                        (const S *)rho_of<S>(L).p, L.g, c, r.ixmin, r.ixmax, r.iymin, r.iymax,
                        r.izmin, r.izmax, dxsqic, dysqic, dzsqic, spm1, mask, sub, 0, 1, 1, 1, 0,
-                       nbsub, (S *)nullptr, (double *)nullptr, skip);
+                       nbsub, (S *)nullptr, (double *)nullptr, skip, (const S *)nullptr, 1);
   }
   WM_LAUNCHED();
   return cpart ? nbsub * (nthr / 64) : 0;
@@ -1022,12 +1036,16 @@ static void fused_pass(wm_mg3d *h, Level &L, int level, const Params &P, const i
   A.swap(B);
 }
 
+// src (not NULL, half sweeps only): the pass starts from the values in src
+// and writes phi; wsave false: the convergence test leaves save unchanged.
 template <typename T>
 static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int *skip,
                     hipStream_t s, bool pin = true, double *cpart = nullptr,
-                    double *cpart_end = nullptr, int *nslots = nullptr) {
+                    double *cpart_end = nullptr, int *nslots = nullptr,
+                    const T *src = nullptr, bool wsave = true) {
   int usevolt = (level == 0 && P.mgform == 1);
   if (fused_ok(h, L, level, P) && phib_of<T>(L).p) {
+    WM_REQUIRE(!src && wsave, "sorpass: the fused pass has no second-buffer form");
     // the pin writes both buffers, so that conductor points on the mesh
     // boundary (which no pass writes) agree in both
     if (pin) {
@@ -1039,12 +1057,13 @@ static void sorpass(wm_mg3d *h, Level &L, int level, const Params &P, const int 
     if (nslots) *nslots = ns;
     return;
   }
+  WM_REQUIRE(!src || !pin, "sorpass: a pass from a second buffer cannot start with a pin");
   if (pin) pin_and_mirror<T>(h, L, level, P, phi_of<T>(L).p, usevolt, skip, s);
   int used = 0;
   for (int c = 0; c <= 1; c++) {
     int ns = 0;
     halfpass<T>(h, L, level, P, c, skip, s, true, cpart ? cpart + used : nullptr, cpart_end,
-                &ns);
+                &ns, src, wsave);
     used += ns;
     launch_bc<T>(phi_of<T>(L).p, RBIdx{L.g}, L.nx, L.ny, L.nz, h->bounds, 0, 0, skip, s);
   }
@@ -1444,7 +1463,23 @@ static void zero_interior_res(wm_mg3d *h, const Params &P, hipStream_t s) {
 template <typename S>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
                    bool zeroed = false, bool repin = true, double *cpart = nullptr,
-                   double *cpart_end = nullptr, int *nslots = nullptr);
+                   double *cpart_end = nullptr, int *nslots = nullptr, bool pp = false);
+
+// Exchanges the storage of the finest level's FP64 phi and save buffers.
+static void pp_swap(wm_mg3d *h) {
+  h->lv[0]->phi64.swap(h->save64);
+  h->orient ^= 1;
+}
+
+// Whether the plain FP64 V-cycles that carry the fused convergence test
+// keep their starting phi in the save buffer (exchanging the buffers)
+// instead of copying every swept point there: the finest level runs half
+// sweeps and at least one down pass (WARPMI_SAVE_SWAP=0 switches it off).
+static bool pp_applies(const wm_mg3d *h, const Params &P) {
+  static const bool off = getenv("WARPMI_SAVE_SWAP") && atoi(getenv("WARPMI_SAVE_SWAP")) == 0;
+  static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
+  return !off && !tiled_env && P.down >= 1 && P.up >= 1 && !fused_ok(h, *h->lv[0], 0, P);
+}
 
 // The FP64 V-cycle's tail (k_vtail) for the parameters P: the first level
 // from 1 on with at most tail_nodes nodes, and every level from there on
@@ -1703,9 +1738,23 @@ static bool coarse_correction(wm_mg3d *h, Level &L, int level, const Params &P,
 
 template <typename S>
 static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipStream_t s,
-                   bool zeroed, bool repin, double *cpart, double *cpart_end, int *nslots) {
+                   bool zeroed, bool repin, double *cpart, double *cpart_end, int *nslots,
+                   bool pp) {
   typedef typename Coarse<S>::type SC;
   Level &L = *h->lv[level];
+  // pp (finest level, FP64): the V-cycle's starting phi moves to the save
+  // buffer by exchanging the two buffers, the first pass sweeps from it
+  // into phi, and the convergence test compares with it without copying
+  const S *src = nullptr;
+  if constexpr (std::is_same<S, double>::value) {
+    if (pp) {
+      WM_REQUIRE(level == 0 && !repin && P.down >= 1 && cpart,
+                 "vcycle: the two-buffer form needs the finest level, no leading pin, a down "
+                 "pass and the fused convergence test");
+      pp_swap(h);
+      src = h->save64.p;
+    }
+  }
   const bool novolt = !(level == 0 && P.mgform == 1);
   bool fusepin = false;
   if constexpr (std::is_same<S, double>::value) {
@@ -1722,7 +1771,8 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   // (repin false) the caller knows the pinned points still hold their
   // values from the previous V-cycle.
   for (int i = 0; i < P.down; i++)
-    sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0) && (repin || i > 0));
+    sorpass<S>(h, L, level, P, skip, s, !(zeroed && novolt && i == 0) && (repin || i > 0),
+               nullptr, nullptr, nullptr, i == 0 ? src : nullptr);
   if (L.coarser) {
     // the coarsest level of a lower-precision V-cycle is solved in FP64,
     // and so are the levels the single-launch tail can hold
@@ -1736,7 +1786,7 @@ static void vcycle(wm_mg3d *h, int level, const Params &P, const int *skip, hipS
   // the last up pass carries the convergence test when cpart is given
   for (int i = 0; i < P.up; i++)
     sorpass<S>(h, L, level, P, skip, s, !(fusepin && i == 0), i == P.up - 1 ? cpart : nullptr,
-               cpart_end, nslots);
+               cpart_end, nslots, nullptr, !pp);
 }
 
 template <typename T>
@@ -1808,12 +1858,17 @@ static void iteration(wm_mg3d *h, int prec, const Params &P, double mgtol, int m
     // up pass's sweeps changes a point between its save at the end of the
     // previous V-cycle and the test: those sweeps compare every point they
     // write with save and copy it there, and k_converge is not needed.  The
-    // first V-cycle of a solve (pin, full copy) keeps k_converge.
+    // first V-cycle of a solve (pin, full copy) keeps k_converge.  Where
+    // pp_applies, the copy is left out instead: the V-cycle exchanges the
+    // buffers, so that save holds its starting phi, and its first pass
+    // sweeps from save into phi (every point a sweep writes, both colours;
+    // the points no sweep writes -- boundaries, pinned interior points --
+    // hold the same values in both buffers).
     static const bool tiled_env = getenv("WARPMI_SOR_TILED") != nullptr;
     const bool fuseconv = !repin && P.up >= 1 && h->convparts.p && !tiled_env;
     int nslots = 0;
     vcycle<double>(h, 0, P, skip, s, false, repin, fuseconv ? h->convparts.p : nullptr,
-                   h->convparts.p + h->convparts.n, &nslots);
+                   h->convparts.p + h->convparts.n, &nslots, fuseconv && pp_applies(h, P));
     if (fuseconv) {
       hipLaunchKernelGGL(k_converge_close, dim3(1), dim3(1024), 0, s, h->convparts.p, nslots,
                          h->state.p, mgtol, mgmaxiters);
@@ -2137,8 +2192,13 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     bool alldir = true;
     for (int i = 0; i < 6; i++) alldir = alldir && h->bounds[i] == 0;
     const bool fp64plain = prec == WM_PREC_FP64 && P.mgform == 1;
+    // the V-cycles after the first keep their starting phi in save64
+    // (exchanging the buffers); then save64 holds a stale phi at the end, so
+    // a solve without that form cannot reuse the state of one with it
+    const bool pp_now = fp64plain && repin_skippable(h, P) && pp_applies(h, P);
     const bool reuse = h->reuse_next && h->last_ok && phi == h->last_phi && fp64plain &&
-                       alldir && mgmaxiters > 0 && repin_skippable(h, P);
+                       alldir && mgmaxiters > 0 && repin_skippable(h, P) &&
+                       !(h->last_pp && !pp_now);
     h->reuse_next = false;
     h->last_ok = false;
     h->last_reused = reuse;
@@ -2192,7 +2252,8 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     static const bool eager_env = getenv("WARPMI_EAGER") != nullptr;
     if (mgmaxiters > 0 && (eager_env || h->prof.on)) {
       // Plain stream launches, one V-cycle at a time (and the event timing
-      // of the finest-level half sweeps when it is switched on).
+      // of the finest-level half sweeps when it is switched on); a V-cycle
+      // that exchanges the level-0 buffers does so as it is issued.
       const bool noskip = !repin_skippable(h, P);
       for (int k = 0; k < mgmaxiters; k++) {
         h->prof.used = 0;
@@ -2223,11 +2284,16 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       key.maxit = (int)mgmaxiters;
       key.mgparam = mgparam;
       key.mgtol = mgtol;
-      if (!h->gexec || !(h->gkey == key)) {
+      const int o0 = h->orient;
+      const bool pp = pp_now;
+      if (!h->gexec[o0] || !(h->gkey == key)) {
         h->drop_graphs();
         // gexec: one V-cycle and the convergence test; gexec_rest: the
         // same without the first pin, for the V-cycles after the first
-        // when repin_skippable holds
+        // when repin_skippable holds.  Capturing a V-cycle that exchanges
+        // the level-0 buffers exchanges them on the host too (the graph
+        // holds the addresses of that orientation), so the graphs of both
+        // orientations are captured, and the host's ends where it began.
         auto capture = [&](hipGraphExec_t *out, bool repin) {
           hipGraph_t graph;
           WM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -2244,17 +2310,27 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
           (void)hipGraphDestroy(graph);
           WM_HIP(e);
         };
-        capture(&h->gexec, true);
-        if (prec == WM_PREC_FP64 && P.mgform == 1 && repin_skippable(h, P))
-          capture(&h->gexec_rest, false);
+        capture(&h->gexec[o0], true);
+        if (prec == WM_PREC_FP64 && P.mgform == 1 && repin_skippable(h, P)) {
+          capture(&h->gexec_rest[o0], false);
+          if (pp) {
+            WM_REQUIRE(h->orient != o0, "mg3d: the two-buffer V-cycle did not exchange");
+            capture(&h->gexec[o0 ^ 1], true);
+            capture(&h->gexec_rest[o0 ^ 1], false);
+          }
+        }
+        WM_REQUIRE(h->orient == o0, "mg3d: graph capture changed the buffer orientation");
         h->gkey = key;
       }
       int batch = std::max(1, std::min<int>(h->last_iters > 0 ? h->last_iters : 4, (int)mgmaxiters));
       int launched = 0;
+      int o = o0;  // orientation each launch starts from, as if every one runs
       for (;;) {
-        for (int b = 0; b < batch && launched < mgmaxiters; b++, launched++)
-          WM_HIP(hipGraphLaunch((launched > 0 || reuse) && h->gexec_rest ? h->gexec_rest : h->gexec,
-                                s));
+        for (int b = 0; b < batch && launched < mgmaxiters; b++, launched++) {
+          const bool rest = (launched > 0 || reuse) && h->gexec_rest[o];
+          WM_HIP(hipGraphLaunch(rest ? h->gexec_rest[o] : h->gexec[o], s));
+          if (rest && pp) o ^= 1;
+        }
         WM_HIP(hipMemcpyAsync(h->state_h, h->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         WM_HIP(hipStreamSynchronize(s));
         if (h->state_h->done || launched >= mgmaxiters) break;
@@ -2263,6 +2339,13 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       iters = h->state_h->iters;
       err = h->state_h->err;
       h->last_iters = iters;
+      // the launches after the converged one changed nothing: the V-cycles
+      // that ran, iters of them, exchanged the buffers once each but the
+      // first (which pins and copies) unless the solve reused phi
+      if (pp && h->gexec_rest[o0]) {
+        const int swaps = std::max(0, iters - (reuse ? 0 : 1));
+        if (swaps & 1) pp_swap(h);
+      }
     }
     // Outputs: phi in natural layout, Dirichlet guards filled
     // (applyboundaryconditions3d with lwithdirichlet, f3d_mgrid.F:441-444).
@@ -2277,6 +2360,7 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
     *mgiters = iters;
     *mgerror = err;
     h->last_ok = fp64plain && alldir && iters > 0 && repin_skippable(h, P);
+    h->last_pp = pp_now;
     h->last_phi = phi;
   });
 }

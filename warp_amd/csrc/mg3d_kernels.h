@@ -218,7 +218,7 @@ template <typename S, typename T>
 __device__ inline double sor_record_conv(S *__restrict__ phi, const S *__restrict__ rho,
                                          const RBGeom &g, const SubList<T> &sub, int i,
                                          T dxsqic, T dysqic, T dzsqic, T spm1,
-                                         S *__restrict__ save) {
+                                         S *__restrict__ save, bool wsave) {
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
   const T *k = sub.k + 6 * i;
@@ -234,43 +234,52 @@ __device__ inline double sor_record_conv(S *__restrict__ phi, const S *__restric
                           (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev);
   phi[ce] = nv;
   const double d = fabs((double)St<S>::ld(save[ce]) - (double)St<S>::ld(nv));
-  save[ce] = nv;
+  if (wsave) save[ce] = nv;
   return d > 0. ? d : 0.;
 }
 
 // One subgrid record of a half sweep (condbndymg, f3d_mgrid.F:2766-2980):
 // the stencil with the record's side forms, divided by its denominator.
+// The neighbours (all of the other colour) are read from ob, the point's
+// value before the sweep from mb; both are phi for a sweep in place.
 template <typename S, typename T>
-__device__ inline void sor_record(S *__restrict__ phi, const S *__restrict__ rho,
-                                  const RBGeom &g, const SubList<T> &sub, int i, T dxsqic,
-                                  T dysqic, T dzsqic, T spm1) {
+__device__ inline void sor_record(S *__restrict__ phi, const S *ob, const S *mb,
+                                  const S *__restrict__ rho, const RBGeom &g,
+                                  const SubList<T> &sub, int i, T dxsqic, T dysqic, T dzsqic,
+                                  T spm1) {
   int ix = sub.pos[3 * i], iy = sub.pos[3 * i + 1], iz = sub.pos[3 * i + 2];
   uint32_t m = sub.modes[i];
   const T *k = sub.k + 6 * i;
-  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(phi[rb_index(g, ix - 1, iy, iz)]));
-  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(phi[rb_index(g, ix + 1, iy, iz)]));
-  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(phi[rb_index(g, ix, iy - 1, iz)]));
-  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(phi[rb_index(g, ix, iy + 1, iz)]));
-  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(phi[rb_index(g, ix, iy, iz - 1)]));
-  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(phi[rb_index(g, ix, iy, iz + 1)]));
+  T pxm = side_value<T>(m & 3u, k[0], St<S>::ld(ob[rb_index(g, ix - 1, iy, iz)]));
+  T pxp = side_value<T>((m >> 2) & 3u, k[1], St<S>::ld(ob[rb_index(g, ix + 1, iy, iz)]));
+  T pym = side_value<T>((m >> 4) & 3u, k[2], St<S>::ld(ob[rb_index(g, ix, iy - 1, iz)]));
+  T pyp = side_value<T>((m >> 6) & 3u, k[3], St<S>::ld(ob[rb_index(g, ix, iy + 1, iz)]));
+  T pzm = side_value<T>((m >> 8) & 3u, k[4], St<S>::ld(ob[rb_index(g, ix, iy, iz - 1)]));
+  T pzp = side_value<T>((m >> 10) & 3u, k[5], St<S>::ld(ob[rb_index(g, ix, iy, iz + 1)]));
   long long ce = rb_index(g, ix, iy, iz);
-  T prev = St<S>::ld(phi[ce]);
+  T prev = St<S>::ld(mb[ce]);
   phi[ce] = St<S>::st((St<S>::ld(rho[ce]) + (pxm + pxp) * dxsqic + (pym + pyp) * dysqic +
                        (pzm + pzp) * dzsqic) / sub.denom[i] + spm1 * prev);
 }
 
 //
 // CONV (the last half sweeps of a V-cycle, FP64): each point written is also
-// compared with save, its value before the V-cycle, and copied there; every
-// wavefront writes the largest |save - new| of its points to cpart[slot],
-// slot = block * (wavefronts per block) + wavefront, 0 when it has none.
+// compared with save, its value before the V-cycle, and copied there unless
+// wsave is 0; every wavefront writes the largest |save - new| of its points
+// to cpart[slot], slot = block * (wavefronts per block) + wavefront, 0 when
+// it has none.
+//
+// src (not NULL): the sweep reads the values before it from src instead of
+// phi and writes phi -- the first half sweeps of a V-cycle whose starting
+// phi stays in src: colour 0 reads both colours from src, colour 1 its own
+// colour from src and colour 0 (just written) from phi.
 template <typename S, typename T = typename St<S>::T, bool CONV = false, bool MASK = true>
 __global__ __launch_bounds__(1024) void k_sor_zmarch(
     S *__restrict__ phi, const S *__restrict__ rho, RBGeom g, int c, int ixmin,
     int ixmax, int iymin, int iymax, int izmin, int izmax, T dxsqic, T dysqic,
     T dzsqic, T spm1, const uint32_t *__restrict__ mask, SubList<T> sub,
     int j0, int nbx, int nby, int zc, int nbdense, int nbsub, S *__restrict__ save,
-    double *__restrict__ cpart, const int *__restrict__ skip) {
+    double *__restrict__ cpart, const int *__restrict__ skip, const S *src, int wsave) {
   WM_SKIP();
   const int slot = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
   double cmax = 0.;
@@ -295,7 +304,8 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     int zb = min(za + zc - 1, izmax);
     int py = iy + 1;
     long long pl = (long long)g.W * g.PY;
-    const S *o = phi + (1 - c) * g.cs;
+    const S *o = ((src && c == 0) ? src : phi) + (1 - c) * g.cs;
+    const S *mr = (src ? src : phi) + c * g.cs;  // own colour before the sweep
     S *me = phi + c * g.cs;
     const S *rh = rho + c * g.cs;
     // Loads use jc, clamped into the padded row, so they stay in bounds for
@@ -318,7 +328,7 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
       q.ozp = St<S>::ld(o[rowc + base + pl]);
       q.oym = St<S>::ld(o[rowc + base - g.W]);
       q.oyp = St<S>::ld(o[rowc + base + g.W]);
-      q.own = St<S>::ld(me[rowc + base]);
+      q.own = St<S>::ld(mr[rowc + base]);
       q.r = St<S>::ld(rh[rowc + base]);
       int p0 = (c + py + pz + 1) & 1;
       int jn = p0 ? j + 1 : j - 1;
@@ -345,7 +355,7 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
         if (CONV) {
           const double d = fabs((double)q.sv - (double)St<S>::ld(nv));
           cmax = d > cmax ? d : cmax;
-          save[c * g.cs + rowc + pl * pz] = nv;
+          if (wsave) save[c * g.cs + rowc + pl * pz] = nv;
         }
       }
       oz_m = oz_0;
@@ -375,13 +385,14 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
   if (CONV) {
     // all lanes reach the reduction; lanes past the list contribute 0
     if (i < sub.n) cmax = sor_record_conv<S, T>(phi, rho, g, sub, i, dxsqic, dysqic, dzsqic,
-                                                spm1, save);
+                                                spm1, save, wsave != 0);
     cmax = wave_max(cmax);
     if ((threadIdx.x & 63) == 0) cpart[slot] = cmax;
     return;
   }
   if (i >= sub.n) return;
-  sor_record<S, T>(phi, rho, g, sub, i, dxsqic, dysqic, dzsqic, spm1);
+  sor_record<S, T>(phi, (src && c == 0) ? src : phi, src ? src : phi, rho, g, sub, i, dxsqic,
+                   dysqic, dzsqic, spm1);
 }
 
 // cond_potmg (f3d_mgrid.F:2426-2472): pin interior conductor points.

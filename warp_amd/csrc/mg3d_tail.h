@@ -202,7 +202,8 @@ __device__ inline bool tail_half(const TailLevel &L, int c, uint32_t *w) {
   const SubList<double> sub = tail_sub(L.sor[c], w);
   TAIL_FOR(t, nd + sub.n) {
     if (t >= nd) {
-      sor_record<double, double>(phi, rho, g, sub, t - nd, dxsqic, dysqic, dzsqic, spm1);
+      sor_record<double, double>(phi, phi, phi, rho, g, sub, t - nd, dxsqic, dysqic, dzsqic,
+                                 spm1);
       continue;
     }
     int j, iy, iz, ix;
