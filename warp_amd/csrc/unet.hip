@@ -1127,10 +1127,10 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
       return 0;
     }
     // rows of 16 voxels per wavefront: 4 up to 64 output channels, 2 above;
-    // 1 where that leaves fewer than 1024 blocks (the coarse levels)
+    // 1 where that leaves fewer than 256 blocks (the coarsest levels)
     const int ntx = (int)((W + 15) / 16), nzb = (int)((D + 3) / 4);
     int vt = cout <= 64 ? 4 : 2;
-    if ((long long)ntx * ((H + vt - 1) / vt) * nzb < 1024) vt = 1;
+    if ((long long)ntx * ((H + vt - 1) / vt) * nzb < 256) vt = 1;
     const int nty = (int)((H + vt - 1) / vt);
     const long long nblk = (long long)ntx * nty * nzb;
     WM_REQUIRE(nblk < (1ll << 31), "wm_unet_conv3: grid too large");
