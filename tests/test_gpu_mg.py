@@ -228,6 +228,34 @@ def test_solve_starts_from_a_changed_phi(device):
     both("untouched again", True)
 
 
+def test_phi_reuse_across_the_two_buffer_form(device):
+    """With one down pass or more, the V-cycles after the first keep their
+    starting phi by exchanging the solver's two finest-level buffers, which
+    leaves the save buffer stale at the end of the solve.  A following
+    solve without down passes (which copies into save instead) must not
+    start from that state; the one after it may again.  Every solve bitwise
+    equal to the oracle run on the same sequence."""
+    nx, ny, nz = 32, 32, 64
+    s, rho = _c1_like(device, nx, ny, nz)
+    co = O.install_conductors(H.pipe_and_annulus(O), nx, ny, nz, *H.DOMAIN)
+    po = np.zeros((nz + 3, ny + 3, nx + 3))
+    ro = rho.copy()
+
+    def both(tag, down, reused):
+        s.downpasses = down
+        s.rho = torch.from_numpy(ro).to(device)
+        s.solve()
+        it, err = O.multigrid3dsolve(po, ro, s.dx, s.dy, s.dz, conductors=co, downpasses=down)
+        assert (s.mgiters, s.mgerror) == (it, err), tag
+        np.testing.assert_array_equal(s.phi.cpu().numpy(), po, err_msg=tag)
+        assert s.last_solve_reused_phi() == reused, tag
+
+    both("cold", 1, False)
+    for k, (down, reused) in enumerate([(1, True), (0, False), (0, True), (1, True), (2, True)]):
+        ro *= 1.05
+        both("solve %d, down %d" % (k, down), down, reused)
+
+
 def test_level_entry_points_end_phi_reuse(device):
     """A per-level entry point between two solves (here the residual of
     another phi, which runs through the solver's level-0 buffers) must not
