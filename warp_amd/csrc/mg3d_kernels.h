@@ -309,8 +309,12 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
     S *me = phi + c * g.cs;
     const S *rh = rho + c * g.cs;
     // Loads use jc, clamped into the padded row, so they stay in bounds for
-    // lanes past the end of the row; such lanes never store.
+    // lanes past the end of the row; such lanes never store.  The
+    // x-neighbour loads stop at the last element a point of the range uses
+    // (ix = 2j <= ixmax reads j+1), so that lanes past the range fetch no
+    // line beyond it.
     int jc = min(j, g.W - 1);
+    const int jhi = min((ixmax >> 1) + 1, g.W - 1);
     long long rowc = (long long)g.W * py + jc;
     long long meoff = (long long)c * g.cs + rowc;  // mask index within a plane
     // The loads of one plane (padded index pz), issued a plane ahead of use
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(1024) void k_sor_zmarch(
       q.r = St<S>::ld(rh[rowc + base]);
       int p0 = (c + py + pz + 1) & 1;
       int jn = p0 ? j + 1 : j - 1;
-      q.edge = St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]);
+      q.edge = St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), jhi)]);
       q.mw = MASK ? mask[(meoff + base) >> 5] : 0u;
     };
     T oz_m = St<S>::ld(o[rowc + pl * za]);        // padded plane za: iz = za-1
@@ -495,8 +499,10 @@ __global__ __launch_bounds__(256) void k_residual_zmarch(
   const S *rh = rho + c * g.cs;
   TR *out = res + c * g.cs;
   // Loads use jc, clamped into the padded row, so they stay in bounds for
-  // lanes past the end of the row; such lanes never store.
+  // lanes past the end of the row; such lanes never store.  The
+  // x-neighbour loads stop at the last element a point of the range uses.
   int jc = min(j, g.W - 1);
+  const int jhi = min((ixmax >> 1) + 1, g.W - 1);
   long long rowc = (long long)g.W * py + jc;
   long long meoff = (long long)c * g.cs + rowc;
   // The loads of one plane, issued a plane ahead and unconditionally (as
@@ -516,7 +522,7 @@ __global__ __launch_bounds__(256) void k_residual_zmarch(
     q.r = St<S>::ld(rh[rowc + base]);
     int p0 = (c + py + pz + 1) & 1;
     int jn = p0 ? j + 1 : j - 1;
-    q.edge = St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), g.W - 1)]);
+    q.edge = St<S>::ld(o[base + (long long)g.W * py + min(max(jn, 0), jhi)]);
     q.mw = MASK ? mask[(meoff + base) >> 5] : 0u;
   };
   T oz_m = St<S>::ld(o[rowc + pl * za]);
