@@ -167,27 +167,31 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
 // and p-1 (dz = 0, 1, 2): a third of the LDS reads of a tile that is
 // staged per output plane.  Per halo plane the block stages 18 x (4VT+2)
 // voxels of all input channels in LDS (zeros outside the grid; a voxel
-// takes 2*NCB + 1 16-byte chunks, the odd stride keeping the lanes'
-// 16-byte reads on distinct banks; two plane buffers where two blocks
-// still fit a CU, so that one barrier per plane suffices), and the loads of
+// takes 2*NCB 16-byte chunks, the two of each 16-channel block swapped in
+// every other run of four voxels, which left the 16-byte reads of the
+// 16-channel form without bank conflicts (SQ_LDS_BANK_CONFLICT 0, against
+// 7.5M cycles with an odd voxel stride of 2*NCB + 1 chunks); two plane
+// buffers where two blocks still fit a CU, so that one barrier per plane
+// suffices), and the loads of
 // the next three planes are in flight during the current plane's MFMAs;
 // after plane p, output plane p-1 is complete and is written.  The MFMA
 // takes the weights as its A operand (rows: output channels) and the
 // voxels as its B operand (columns), so a lane holds 4 consecutive
 // channels of one voxel and writes them with one 8-byte (bf16) or 16-byte
-// (fp32) store.  The
-// weights stay in registers with one input block (NCB = 1), else in LDS
-// (rows of 32 values padded to 5 chunks).  Accumulators of three output
-// planes rotate through slots 0-2; partial sums of planes outside the
-// chunk fall into slots that are cleared before reuse and never written.
+// (fp32) store.  The weights stay in registers with one input block
+// (NCB = 1), else in LDS (rows of 32 values, the four chunks of each row
+// permuted by bits 1-2 of the row index, which left their reads without
+// bank conflicts).  Accumulators of three output planes rotate through
+// slots 0-2; partial sums of planes outside the chunk fall into slots that
+// are cleared before reuse and never written.
 template <int VT, int NCB>
 struct Conv3zShape {
-  static constexpr int NCH = 2 * NCB, STR = NCH + 1;  // chunks per voxel, voxel stride
+  static constexpr int NCH = 2 * NCB, STR = NCH;  // chunks per voxel, voxel stride
   static constexpr int HX = 18, HY = 4 * VT + 2, HV = HX * HY;
   static constexpr int NCHK = HV * NCH;               // chunks staged per plane
   static constexpr int PF = (NCHK + 255) / 256;       // staged chunks per thread
   static constexpr bool BREG = NCB == 1;
-  static constexpr int BCH = BREG ? 0 : 15 * NCB * 16 * 5;
+  static constexpr int BCH = BREG ? 0 : 15 * NCB * 16 * 4;
   // a plane buffer: the plane and a spare chunk; two buffers (one barrier
   // per plane) when two blocks still fit a CU
   static constexpr int PB = HV * STR + 1, SPARE = HV * STR;
@@ -233,7 +237,7 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
     if (c < C::NCHK) {
       const int hv = c / C::NCH, ch = c - hv * C::NCH;
       const int gx = x0 - 1 + hv % C::HX, gy = y0 - 1 + hv / C::HX;
-      loff[k] = hv * C::STR + ch;
+      loff[k] = hv * C::STR + (ch ^ ((hv >> 2) & 1));
       gin[k] = gx >= 0 && gx < d.W && gy >= 0 && gy < d.H;
       if (gin[k]) goff[k] = (gy * d.W + gx) * CIN + ch * 8;
     }
@@ -278,7 +282,10 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
     for (int s = 0; s < 15; s++) breg[s] = *(const bf16x8 *)(wq + (s * 16 + m) * 32 + 8 * kg);
   } else {
     const uint4 *wsrc = (const uint4 *)wq;
-    for (int i = tid; i < 15 * NCB * 16 * 4; i += 256) bl[(i >> 2) * 5 + (i & 3)] = wsrc[i];
+    for (int i = tid; i < 15 * NCB * 16 * 4; i += 256) {
+      const int row = i >> 2;  // (cb * 15 + step) * 16 + output channel
+      bl[row * 4 + ((i & 3) ^ ((row >> 1) & 3))] = wsrc[i];
+    }
   }
   float bv[4];
 #pragma unroll
@@ -290,7 +297,7 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
 #pragma unroll
   for (int r = 0; r < 5; r++) {
     const int t9 = min(2 * r + (kg >> 1), 8);
-    aoff[r] = ((w * VT + t9 / 3) * C::HX + m + t9 % 3) * C::STR + (kg & 1);
+    aoff[r] = (w * VT + t9 / 3) * C::HX + m + t9 % 3;  // voxel of row j = 0
   }
   const bool fake = (kg >> 1) == 1;
   const bf16x8 zero8 = {};
@@ -315,7 +322,8 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
       for (int r = 0; r < 5; r++)
 #pragma unroll
         for (int j = 0; j < VT; j++) {
-          a[r][j] = *(const bf16x8 *)&cur[aoff[r] + j * C::HX * C::STR + 2 * cb];
+          const int hv = aoff[r] + j * C::HX;
+          a[r][j] = *(const bf16x8 *)&cur[hv * C::STR + ((2 * cb + (kg & 1)) ^ ((hv >> 2) & 1))];
           if (r == 4 && fake) a[r][j] = zero8;
         }
 #pragma unroll
@@ -324,10 +332,12 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
 #pragma unroll
         for (int dz = 0; dz < 3; dz++) {
           if (!((DZM >> dz) & 1)) continue;
-          if constexpr (C::BREG)
+          if constexpr (C::BREG) {
             bw[dz] = breg[dz * 5 + r];
-          else
-            bw[dz] = *(const bf16x8 *)&bl[((cb * 15 + dz * 5 + r) * 16 + m) * 5 + kg];
+          } else {
+            const int row = (cb * 15 + dz * 5 + r) * 16 + m;
+            bw[dz] = *(const bf16x8 *)&bl[row * 4 + (kg ^ ((row >> 1) & 3))];
+          }
         }
 #pragma unroll
         for (int j = 0; j < VT; j++) {
