@@ -167,11 +167,10 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
 // and p-1 (dz = 0, 1, 2): a third of the LDS reads of a tile that is
 // staged per output plane.  Per halo plane the block stages 18 x (4VT+2)
 // voxels of all input channels in LDS (zeros outside the grid; a voxel
-// takes 2*NCB 16-byte chunks, the two of each 16-channel block swapped in
-// every other run of four voxels, which left the 16-byte reads of the
-// 16-channel form without bank conflicts (SQ_LDS_BANK_CONFLICT 0, against
-// 7.5M cycles with an odd voxel stride of 2*NCB + 1 chunks); two plane
-// buffers where two blocks still fit a CU, so that one barrier per plane
+// takes 2*NCB 16-byte chunks, with no padding: the 16-byte reads then have
+// no bank conflicts by SQ_LDS_BANK_CONFLICT, where an odd voxel stride of
+// 2*NCB + 1 chunks had 7.5M conflict cycles per launch; two plane buffers
+// where two blocks still fit a CU, so that one barrier per plane
 // suffices), and the loads of
 // the next three planes are in flight during the current plane's MFMAs;
 // after plane p, output plane p-1 is complete and is written.  The MFMA
@@ -180,8 +179,8 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
 // channels of one voxel and writes them with one 8-byte (bf16) or 16-byte
 // (fp32) store.  The weights stay in registers with one input block
 // (NCB = 1), else in LDS (rows of 32 values, the four chunks of each row
-// permuted by bits 1-2 of the row index, which left their reads without
-// bank conflicts).  Accumulators of three output planes rotate through
+// permuted by bits 1-2 of the row index: without the permutation their
+// reads had 24M conflict cycles per 48 -> 16 launch, with it none).  Accumulators of three output planes rotate through
 // slots 0-2; partial sums of planes outside the chunk fall into slots that
 // are cleared before reuse and never written.
 template <int VT, int NCB>
@@ -237,7 +236,7 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
     if (c < C::NCHK) {
       const int hv = c / C::NCH, ch = c - hv * C::NCH;
       const int gx = x0 - 1 + hv % C::HX, gy = y0 - 1 + hv / C::HX;
-      loff[k] = hv * C::STR + (ch ^ ((hv >> 2) & 1));
+      loff[k] = hv * C::STR + ch;
       gin[k] = gx >= 0 && gx < d.W && gy >= 0 && gy < d.H;
       if (gin[k]) goff[k] = (gy * d.W + gx) * CIN + ch * 8;
     }
@@ -322,8 +321,7 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
       for (int r = 0; r < 5; r++)
 #pragma unroll
         for (int j = 0; j < VT; j++) {
-          const int hv = aoff[r] + j * C::HX;
-          a[r][j] = *(const bf16x8 *)&cur[hv * C::STR + ((2 * cb + (kg & 1)) ^ ((hv >> 2) & 1))];
+          a[r][j] = *(const bf16x8 *)&cur[(aoff[r] + j * C::HX) * C::STR + 2 * cb + (kg & 1)];
           if (r == 4 && fake) a[r][j] = zero8;
         }
 #pragma unroll
