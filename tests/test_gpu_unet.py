@@ -57,15 +57,16 @@ def test_conv3_forward(device, cin, cout, dims):
     assert (got - ref).abs().max() <= 2e-5 * ref.abs().max() + 1e-6
 
 
-@pytest.mark.parametrize("cin,dims", [(16, (19, 22, 40)), (32, (9, 13, 33)), (48, (11, 9, 20)),
-                                      (16, (1, 3, 5))])
-def test_conv3_zmarch_forward(device, cin, dims):
-    """The z-marching kernel (16 output channels, up to 48 input channels):
-    several blocks in x, y and z with partial tiles and chunks, both output
-    forms; the bf16 output is the fp32 output rounded to nearest even."""
+@pytest.mark.parametrize("cin,cout,dims", [(16, 16, (19, 22, 40)), (32, 16, (9, 13, 33)),
+                                           (48, 16, (11, 9, 20)), (16, 16, (1, 3, 5)),
+                                           (16, 32, (13, 17, 35)), (32, 32, (10, 9, 20))])
+def test_conv3_zmarch_forward(device, cin, cout, dims):
+    """The z-marching kernel (16 output channels with up to 48 input
+    channels, 32 with up to 32): several blocks in x, y and z with partial
+    tiles and chunks, both output forms; the bf16 output is the fp32 output
+    rounded to nearest even."""
     from warp_amd import _native as N
     D, H, W = dims
-    cout = 16
     g = torch.Generator().manual_seed(cin + D)
     x = _bf16(torch.randn(cin, D, H, W, generator=g, dtype=torch.float64))
     w = _bf16(torch.randn(cout, cin, 27, generator=g, dtype=torch.float64) * 0.1)

@@ -158,7 +158,8 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
 }
 
 // Forward convolution marching in z, for 16 output channels and up to 48
-// input channels (the finest level of the network).  A block (4
+// input channels (the finest level of the network) and 32 output channels
+// and up to 32 input channels (the next level), NT tiles of 16.  A block (4
 // wavefronts) owns a column of 16 x 4VT voxels (x, y) over zc output
 // planes; wavefront w takes the rows y0 + w*VT .. +VT-1.  In the packed
 // order the 5 steps of one z offset dz use the same (dy, dx) shifts, so one
@@ -183,14 +184,15 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ x, i
 // reads had 24M conflict cycles per 48 -> 16 launch, with it none).  Accumulators of three output planes rotate through
 // slots 0-2; partial sums of planes outside the chunk fall into slots that
 // are cleared before reuse and never written.
-template <int VT, int NCB>
+template <int VT, int NCB, int NT>
 struct Conv3zShape {
   static constexpr int NCH = 2 * NCB, STR = NCH;  // chunks per voxel, voxel stride
   static constexpr int HX = 18, HY = 4 * VT + 2, HV = HX * HY;
   static constexpr int NCHK = HV * NCH;               // chunks staged per plane
   static constexpr int PF = (NCHK + 255) / 256;       // staged chunks per thread
-  static constexpr bool BREG = NCB == 1;
-  static constexpr int BCH = BREG ? 0 : 15 * NCB * 16 * 4;
+  static constexpr int COUT = 16 * NT;
+  static constexpr bool BREG = NCB == 1 && NT == 1;
+  static constexpr int BCH = BREG ? 0 : 15 * NCB * COUT * 4;
   // a plane buffer: the plane and a spare chunk; two buffers (one barrier
   // per plane) when two blocks still fit a CU
   static constexpr int PB = HV * STR + 1, SPARE = HV * STR;
@@ -201,12 +203,13 @@ struct Conv3zShape {
 template <int V>
 using ic = std::integral_constant<int, V>;
 
-template <int VT, int NCB>
+template <int VT, int NCB, int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *__restrict__ yf,
               Dims d, const uint16_t *__restrict__ wq, const float *__restrict__ bias,
               float slope, int ntx, int nty, int zc) {
-  using C = Conv3zShape<VT, NCB>;
+  using C = Conv3zShape<VT, NCB, NT>;
+  constexpr int COUT = C::COUT;
   constexpr int CIN = NCB * 16;
   __shared__ __align__(16) uint4 lds[C::LDSCH];
   uint4 *const pl = lds;                                   // staged plane(s)
@@ -281,14 +284,16 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
     for (int s = 0; s < 15; s++) breg[s] = *(const bf16x8 *)(wq + (s * 16 + m) * 32 + 8 * kg);
   } else {
     const uint4 *wsrc = (const uint4 *)wq;
-    for (int i = tid; i < 15 * NCB * 16 * 4; i += 256) {
-      const int row = i >> 2;  // (cb * 15 + step) * 16 + output channel
+    for (int i = tid; i < 15 * NCB * COUT * 4; i += 256) {
+      const int row = i >> 2;  // (cb * 15 + step) * COUT + output channel
       bl[row * 4 + ((i & 3) ^ ((row >> 1) & 3))] = wsrc[i];
     }
   }
-  float bv[4];
+  float bv[NT][4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) bv[i] = bias ? bias[4 * kg + i] : 0.f;
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) bv[t][i] = bias ? bias[16 * t + 4 * kg + i] : 0.f;
 
   // A operand offsets: pair r, this lane's tap t9 = 2r + kg/2 (the tenth,
   // zero tap reads tap 8's voxel and is zeroed), channel half kg%2
@@ -301,11 +306,13 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
   const bool fake = (kg >> 1) == 1;
   const bf16x8 zero8 = {};
 
-  f4 acc[3][VT];
+  f4 acc[3][VT][NT];
 #pragma unroll
   for (int q = 0; q < 3; q++)
 #pragma unroll
-    for (int j = 0; j < VT; j++) acc[q][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < VT; j++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[q][j][t] = f4{0.f, 0.f, 0.f, 0.f};
 
   // dz = 0 adds to the slot of output plane p+1 (S2), dz = 1 to p (S1),
   // dz = 2 to p-1 (S0); DZM selects which of the three run
@@ -326,25 +333,31 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
         }
 #pragma unroll
       for (int r = 0; r < 5; r++) {
-        bf16x8 bw[3];
 #pragma unroll
-        for (int dz = 0; dz < 3; dz++) {
-          if (!((DZM >> dz) & 1)) continue;
-          if constexpr (C::BREG) {
-            bw[dz] = breg[dz * 5 + r];
-          } else {
-            const int row = (cb * 15 + dz * 5 + r) * 16 + m;
-            bw[dz] = *(const bf16x8 *)&bl[row * 4 + (kg ^ ((row >> 1) & 3))];
+        for (int t = 0; t < NT; t++) {
+          bf16x8 bw[3];
+#pragma unroll
+          for (int dz = 0; dz < 3; dz++) {
+            if (!((DZM >> dz) & 1)) continue;
+            if constexpr (C::BREG) {
+              bw[dz] = breg[dz * 5 + r];
+            } else {
+              const int row = (cb * 15 + dz * 5 + r) * COUT + 16 * t + m;
+              bw[dz] = *(const bf16x8 *)&bl[row * 4 + (kg ^ ((row >> 1) & 3))];
+            }
           }
-        }
 #pragma unroll
-        for (int j = 0; j < VT; j++) {
-          if (DZM & 1)
-            acc[S2][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[0], a[r][j], acc[S2][j], 0, 0, 0);
-          if (DZM & 2)
-            acc[S1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[1], a[r][j], acc[S1][j], 0, 0, 0);
-          if (DZM & 4)
-            acc[S0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[2], a[r][j], acc[S0][j], 0, 0, 0);
+          for (int j = 0; j < VT; j++) {
+            if (DZM & 1)
+              acc[S2][j][t] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[0], a[r][j], acc[S2][j][t], 0, 0, 0);
+            if (DZM & 2)
+              acc[S1][j][t] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[1], a[r][j], acc[S1][j][t], 0, 0, 0);
+            if (DZM & 4)
+              acc[S0][j][t] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[2], a[r][j], acc[S0][j][t], 0, 0, 0);
+          }
         }
       }
     }
@@ -357,17 +370,21 @@ void k_conv3z(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, float *_
     for (int j = 0; j < VT; j++) {
       const int yy = y0 + w * VT + j;
       if (yy < d.H && xx < d.W) {
-        float v[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          v[i] = acc[S][j][i] + bv[i];
-          v[i] = v[i] > 0.f ? v[i] : v[i] * slope;
+        for (int t = 0; t < NT; t++) {
+          float v[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            v[i] = acc[S][j][t][i] + bv[t][i];
+            v[i] = v[i] > 0.f ? v[i] : v[i] * slope;
+          }
+          const long long vi = (((long long)o * d.H + yy) * d.W + xx) * COUT + 16 * t + 4 * kg;
+          if (y) *(uint2 *)(y + vi) = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
+          if (yf) *(float4 *)(yf + vi) = make_float4(v[0], v[1], v[2], v[3]);
         }
-        const long long vi = (((long long)o * d.H + yy) * d.W + xx) * 16 + 4 * kg;
-        if (y) *(uint2 *)(y + vi) = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
-        if (yf) *(float4 *)(yf + vi) = make_float4(v[0], v[1], v[2], v[3]);
       }
-      acc[S][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[S][j][t] = f4{0.f, 0.f, 0.f, 0.f};
     }
   };
 
@@ -1117,7 +1134,7 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
     Dims d{(int)D, (int)H, (int)W};
     hipStream_t s = wm_stream(stream);
     float sl = (float)slope;
-    if (cout == 16 && cin <= 48 && H * W * cin < (1ll << 31)) {
+    if (((cout == 16 && cin <= 48) || (cout == 32 && cin <= 32)) && H * W * cin < (1ll << 31)) {
       // z-marching form; chunks of zc planes, about 2048 blocks or more
       const int ncb = (int)(cin / 16), vt = 2;
       const int ntx = (int)((W + 15) / 16), nty = (int)((H + 4 * vt - 1) / (4 * vt));
@@ -1126,10 +1143,16 @@ int wm_unet_conv3(const uint16_t *x, int64_t cin, uint16_t *y, float *yf, int64_
       const long long nblk = nxy * ((D + zc - 1) / zc);
       WM_REQUIRE(nblk < (1ll << 31), "wm_unet_conv3: grid too large");
       const dim3 g((unsigned)nblk), bl(256);
-      switch (ncb) {
-        case 1: hipLaunchKernelGGL((k_conv3z<2, 1>), g, bl, 0, s, x, y, yf, d, wp, bias, sl, ntx, nty, zc); break;
-        case 2: hipLaunchKernelGGL((k_conv3z<2, 2>), g, bl, 0, s, x, y, yf, d, wp, bias, sl, ntx, nty, zc); break;
-        default: hipLaunchKernelGGL((k_conv3z<2, 3>), g, bl, 0, s, x, y, yf, d, wp, bias, sl, ntx, nty, zc); break;
+      auto run = [&](auto ncbc, auto ntc) {
+        hipLaunchKernelGGL((k_conv3z<2, decltype(ncbc)::value, decltype(ntc)::value>), g, bl, 0,
+                           s, x, y, yf, d, wp, bias, sl, ntx, nty, zc);
+      };
+      switch (ncb + 4 * (int)(cout / 16)) {
+        case 1 + 4: run(ic<1>(), ic<1>()); break;
+        case 2 + 4: run(ic<2>(), ic<1>()); break;
+        case 3 + 4: run(ic<3>(), ic<1>()); break;
+        case 1 + 8: run(ic<1>(), ic<2>()); break;
+        default: run(ic<2>(), ic<2>()); break;
       }
       WM_LAUNCHED();
       return 0;
