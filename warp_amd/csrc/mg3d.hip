@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -266,12 +267,17 @@ struct wm_mg3d {
   // captured V-cycle graphs, per orientation at their start
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipGraphExec_t gexec_rest[2] = {nullptr, nullptr};
+  // n V-cycles of the gexec_rest form in one graph, keyed by (orientation
+  // at its start, n): one launch for the V-cycles a solve usually needs
+  std::map<std::pair<int, int>, hipGraphExec_t> gmulti;
   void drop_graphs() {
     for (int o = 0; o < 2; o++) {
       if (gexec[o]) (void)hipGraphExecDestroy(gexec[o]);
       if (gexec_rest[o]) (void)hipGraphExecDestroy(gexec_rest[o]);
       gexec[o] = gexec_rest[o] = nullptr;
     }
+    for (auto &e : gmulti) (void)hipGraphExecDestroy(e.second);
+    gmulti.clear();
   }
   struct GKey {
     int prec = -1, mgform = -1, down = -1, up = -1, lcnd = -1, icnd = -1, maxit = -1;
@@ -2325,6 +2331,47 @@ int wm_mg3d_solve(wm_mg3d *h, double *phi, double *rho, double mgparam, int64_t 
       int batch = std::max(1, std::min<int>(h->last_iters > 0 ? h->last_iters : 4, (int)mgmaxiters));
       int launched = 0;
       int o = o0;  // orientation each launch starts from, as if every one runs
+      // the first batch: its V-cycles of the gexec_rest form as one graph
+      // (captured once per count and orientation: capturing exchanges the
+      // buffers on the host once per V-cycle, so it ends where it began
+      // after one more exchange when the count is odd)
+      const int nmulti = h->gexec_rest[o0] ? batch - (reuse ? 0 : 1) : 0;
+      hipGraphExec_t gm = nullptr;
+      if (nmulti >= 2 && nmulti <= 16) {
+        auto it = h->gmulti.find({o0, nmulti});
+        if (it != h->gmulti.end()) {
+          gm = it->second;
+        } else {
+          hipGraph_t graph;
+          WM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          try {
+            for (int k = 0; k < nmulti; k++)
+              iteration(h, prec, P, mgtol, (int)mgmaxiters, s, false);
+          } catch (...) {
+            hipGraph_t g2;
+            (void)hipStreamEndCapture(s, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            throw;
+          }
+          WM_HIP(hipStreamEndCapture(s, &graph));
+          hipError_t e = hipGraphInstantiate(&gm, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          WM_HIP(e);
+          if (pp && (nmulti & 1)) pp_swap(h);
+          WM_REQUIRE(h->orient == o0, "mg3d: graph capture changed the buffer orientation");
+          h->gmulti[{o0, nmulti}] = gm;
+        }
+      }
+      if (gm) {
+        if (!reuse) {
+          WM_HIP(hipGraphLaunch(h->gexec[o], s));
+          launched++;
+        }
+        WM_HIP(hipGraphLaunch(gm, s));
+        launched += nmulti;
+        if (pp && (nmulti & 1)) o ^= 1;
+        batch = 0;
+      }
       for (;;) {
         for (int b = 0; b < batch && launched < mgmaxiters; b++, launched++) {
           const bool rest = (launched > 0 || reuse) && h->gexec_rest[o];
