@@ -73,6 +73,30 @@ def test_product_conductors_equal_oracle(kind, n):
     assert len(a["sub0_dels"]) > 0 and len(a["int_volt"]) > 0
 
 
+def test_neumann_flag_leaves_intercept_data_unchanged():
+    """Assembly(neumann=1): the reference's default intercept generation
+    passes the flag to GridIntercepts, which keeps nothing of it
+    (generateconductors.py:333-340, 1815-1821), so the installed data equal
+    those of neumann=0, and unions inherit the flag of their left part."""
+    from warp_amd.generateconductors import ZAnnulus, ZCylinderOut
+    from warp_amd import configs
+    n = (32, 32, 64)
+
+    def geoms(flag):
+        return [ZCylinderOut(0.02, 0.1, voltage=0., zcent=0.05, neumann=flag)
+                + ZAnnulus(0.008, 0.012, 0.01, voltage=-1., zcent=0.04, neumann=flag)]
+
+    g1 = geoms(1)
+    assert g1[0].neumann == 1 and g1[0].right.neumann == 1
+    a = _product_lists(geoms(0), n, configs.DOMAIN)
+    b = _product_lists(g1, n, configs.DOMAIN)
+    assert a["levels"] == b["levels"]
+    for k in a:
+        if k != "levels":
+            np.testing.assert_array_equal(a[k], b[k])
+    assert len(a["sub0_dels"]) > 0
+
+
 CROPS = [(None, None, None, None, 0.031, 0.05), (-0.005, 0.012, None, 0.0, None, None),
          (None, None, None, None, 0.0326, 0.0329)]
 

@@ -30,11 +30,20 @@ class Assembly:
     """Base of all conductors (generateconductors.py:230-).
 
     voltage, xcent, ycent, zcent and condid as in the reference; condid
-    'next' takes the next unused integer id.
+    'next' takes the next unused integer id.  neumann is kept as the
+    reference keeps it: its intercept generation, the default one
+    (_lwithnewconductorgeneration = True, generateconductors.py:124, 200-205),
+    hands the flag to GridIntercepts (generateconductors.py:333-340,
+    1815-1821), which stores nothing of it, and conductordelfromintercepts
+    takes no such argument, so the installed data do not depend on it.
+    Only the older distance-based generation (Delta.normalize and
+    setparity, generateconductors.py:1551-1627) reads it; that generation
+    is not provided.
     """
     nextcondid = 1
 
-    def __init__(self, voltage=0., xcent=0., ycent=0., zcent=0., condid='next'):
+    def __init__(self, voltage=0., xcent=0., ycent=0., zcent=0., condid='next', neumann=0):
+        self.neumann = neumann
         self.voltage = voltage
         self.xcent = xcent
         self.ycent = ycent
@@ -74,6 +83,7 @@ class AssemblyPlus(Assembly):
         self.right = right
         self.condid = left.condid
         self.voltage = left.voltage
+        self.neumann = left.neumann
         el, er = left.getextent(), right.getextent()
         self.extent = ConductorExtent([min(a, b) for a, b in zip(el.mins, er.mins)],
                                       [max(a, b) for a, b in zip(el.maxs, er.maxs)])
@@ -83,12 +93,12 @@ class ZCylinder(Assembly):
     """Cylinder aligned with z (generateconductors.py:3305-)."""
 
     def __init__(self, radius, length=None, voltage=0., xcent=0., ycent=0., zcent=0.,
-                 condid='next', zlower=None, zupper=None):
+                 condid='next', zlower=None, zupper=None, neumann=0):
         if zlower is not None and zupper is not None:
             length = zupper - zlower
             zcent = 0.5 * (zlower + zupper)
         assert length is not None, "ZCylinder: either length or both zlower and zupper must be specified"
-        Assembly.__init__(self, voltage, xcent, ycent, zcent, condid)
+        Assembly.__init__(self, voltage, xcent, ycent, zcent, condid, neumann)
         self.radius = radius
         self.length = length
         self.createextent([-radius, -radius, -length / 2.], [+radius, +radius, +length / 2.])
@@ -105,12 +115,12 @@ class ZCylinderOut(Assembly):
     """Outside of a cylinder aligned with z (generateconductors.py:3494-3553)."""
 
     def __init__(self, radius, length=None, voltage=0., xcent=0., ycent=0., zcent=0.,
-                 condid='next', zlower=None, zupper=None):
+                 condid='next', zlower=None, zupper=None, neumann=0):
         if zlower is not None and zupper is not None:
             length = zupper - zlower
             zcent = 0.5 * (zlower + zupper)
         assert length is not None, "ZCylinderOut: either length or both zlower and zupper must be specified"
-        Assembly.__init__(self, voltage, xcent, ycent, zcent, condid)
+        Assembly.__init__(self, voltage, xcent, ycent, zcent, condid, neumann)
         self.radius = radius
         self.length = length
         self.createextent([-LARGEPOS, -LARGEPOS, -length / 2.],
@@ -130,8 +140,8 @@ class ZSrfrvIn(Assembly):
     rmin -> rofzdata -> rmin."""
 
     def __init__(self, zmin, zmax, rmin=0., rofzdata=None, zdata=None, voltage=0.,
-                 xcent=0., ycent=0., zcent=0., condid='next'):
-        Assembly.__init__(self, voltage, xcent, ycent, zcent, condid)
+                 xcent=0., ycent=0., zcent=0., condid='next', neumann=0):
+        Assembly.__init__(self, voltage, xcent, ycent, zcent, condid, neumann)
         self.zmin, self.zmax, self.rmin = zmin, zmax, rmin
         self.rofzdata = list(rofzdata)
         self.zdata = list(zdata)
@@ -156,14 +166,14 @@ class ZAnnulus(ZSrfrvIn):
     z data in the lab frame with zcent = 0."""
 
     def __init__(self, rmin, rmax, length, voltage=0., xcent=0., ycent=0., zcent=0.,
-                 condid='next'):
+                 condid='next', neumann=0):
         self.length = length
         self.rmax = rmax
         zmin = -length / 2. + zcent
         zmax = +length / 2. + zcent
         ZSrfrvIn.__init__(self, zmin, zmax, rmin=rmin, rofzdata=[rmax, rmax],
                           zdata=[zmin, zmax], voltage=voltage, xcent=xcent, ycent=ycent,
-                          zcent=0., condid=condid)
+                          zcent=0., condid=condid, neumann=neumann)
 
 
 def QuadRods(rodradius, rodcenter, rodlength, voltage, zcent=0., xcent=0., ycent=0.,
