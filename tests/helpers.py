@@ -119,6 +119,75 @@ def slab_lists(n3, axis, side, ic, frac, V, dfill=2.0):
     return out
 
 
+def neumann_lists(n3, ic, kind, frac, vfun, levels=1):
+    """Hand-made subgrid lists of a sheet normal to x through the node plane
+    ix = ic (level 0; the coarser levels, up to `levels`, hold none).
+    Every node of the plane is a subgrid point whose x directions are, by
+    kind: "m" Neumann below (dels[mx] = -frac), "p" Neumann above, "mp"
+    Neumann on both sides, "dm" Dirichlet below (dels[mx] = frac at the
+    voltage vfun(ix, iy, iz)) and Neumann above; the other four directions
+    are bulk (dels = 2).  No interior points: a Neumann conductor installs
+    none (Delta.setparity with dfill = 0, generateconductors.py:1623-1626).
+    The dels signs select the branches of handlesubgrid2
+    (f3d_mgrid.F:2661-2762)."""
+    nx, ny, nz = n3
+    nodes = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                                 indexing="ij"), -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    mx, px = {"m": (-frac, 2.), "p": (2., -frac), "mp": (-frac, -frac),
+              "dm": (frac, -frac)}[kind]
+    out = dict(levels=levels, int_indx=np.zeros((0, 3), np.int64), int_volt=np.zeros(0),
+               int_istart=np.zeros(levels + 1, np.int64))
+    for p in range(2):
+        pts = nodes[(nodes[:, 0] == ic) & (nodes.sum(1) % 2 == p)].astype(np.int64)
+        dels = np.full((len(pts), 6), 2.)
+        dels[:, 0], dels[:, 1] = mx, px
+        volt = np.zeros((len(pts), 6))
+        if kind == "dm":
+            volt[:, 0] = [vfun(*q) for q in pts]
+        out["sub%d_indx" % p] = np.ascontiguousarray(pts)
+        out["sub%d_dels" % p] = dels
+        out["sub%d_volt" % p] = volt
+        out["sub%d_istart" % p] = np.array([0] + [len(pts)] * levels, np.int64)
+    return out
+
+
+def oracle_conductors(d):
+    """The oracle's Conductors holding the level-0 lists of a lists() dict
+    (condid 1); multigrid3dsolve sorts them per level (checkconductors)."""
+    from oracle import oracle as O
+    co = O.Conductors()
+    n = [len(d["int_indx"]), len(d["sub0_indx"]), len(d["sub1_indx"])]
+    co.grow(*n)
+    co.int_indx[:] = d["int_indx"]
+    co.int_volt[:] = d["int_volt"]
+    co.int_numb[:] = 1
+    co.int_ilevel[:] = 0
+    co.s.int_n = n[0]
+    for p in range(2):
+        q = co.sub[p]
+        q["indx"][:] = d["sub%d_indx" % p]
+        q["dels"][:] = d["sub%d_dels" % p]
+        q["volt"][:] = d["sub%d_volt" % p]
+        q["numb"][:] = 1
+        q["ilevel"][:] = 0
+        q["prevphi"][:] = 0.
+        co.s.sub_n[p] = n[1 + p]
+    return co
+
+
+def linear_z_case(n3, d3, g, h, seed):
+    """phi = g + h z on every node and guard (the boundary values of six
+    Dirichlet faces), with the interior nodes perturbed by uniform noise in
+    [-1, 1); returns (initial phi, exact phi), both (nz+3, ny+3, nx+3)."""
+    nx, ny, nz = n3
+    z = (np.arange(-1, nz + 2) * d3[2])[:, None, None]
+    exact = np.ascontiguousarray(np.broadcast_to(g + h * z, (nz + 3, ny + 3, nx + 3)))
+    phi = exact.copy()
+    rng = np.random.default_rng(seed)
+    phi[2:nz + 1, 2:ny + 1, 2:nx + 1] += rng.uniform(-1., 1., (nz - 1, ny - 1, nx - 1))
+    return phi, exact
+
+
 def slab_surface(axis, side, ic, frac, d3):
     """Position of the slab's surface along its axis."""
     return (ic + side * frac) * d3[axis]

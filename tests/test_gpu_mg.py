@@ -171,6 +171,46 @@ def test_conductor_solve_fp64_bitwise(device):
     np.testing.assert_array_equal(s.phi.cpu().numpy(), po)
 
 
+@pytest.mark.parametrize("kind", ["m", "p", "mp", "dm"])
+def test_neumann_subgrid_solve_bitwise(device, kind):
+    """Hand-made Neumann subgrid lists (a sheet normal to x, dels in
+    (-1, 0]; no conductor generator makes them): the solve runs the Neumann
+    branches of the record kernels and the ghost mirror (condbndymgint).
+    Its phi must be the closed form of
+    test_oracle.py::test_neumann_sheet_closed_form (same case: the field
+    g + h z, constant across the sheet) and agree with the oracle's to
+    rounding, with the V-cycle count within one.  Unlike the Dirichlet
+    paths this one is not bit for bit: phi differs from the oracle's in the
+    last bits (mgerror 7.4358e-12 against 7.4216e-12 for "m", same count);
+    the cause is not identified."""
+    from warp_amd import _native as N
+    n3, d3 = (12, 8, 16), (0.1, 0.12, 0.08)
+    g, h, dl = 2.0, 25.0, 0.3
+    vf = 1. - dl + dl * dl if kind == "dm" else 1.
+    phi, _ = H.linear_z_case(n3, d3, g, h, seed=3)
+    s = _solver(*n3, (0., 0., 0.), tuple(n * d for n, d in zip(n3, d3)), [0] * 6,
+                mgtol=1e-11, mgmaxiters=200)
+    s.getconductorobject()
+    d = H.neumann_lists(n3, 5, kind, dl, lambda ix, iy, iz: (g + h * iz * d3[2]) * vf,
+                        levels=s.nlevels())
+    view = H.lists_struct(d)
+    N.call("wm_mg3d_set_conductors", s._h, C.addressof(view))
+    rho = np.zeros((n3[2] + 1, n3[1] + 1, n3[0] + 1))
+    s.phi = torch.from_numpy(phi.copy()).to(device)
+    s.rho = torch.from_numpy(rho).to(device)
+    s.solve()
+    po = phi.copy()
+    it, err = O.multigrid3dsolve(po, rho.copy(), *d3, conductors=H.oracle_conductors(d),
+                                 mgtol=1e-11, mgmaxiters=200)
+    assert it < 200 and s.mgerror <= 1e-11
+    assert abs(s.mgiters - it) <= 1
+    pg = s.phi.cpu().numpy()
+    assert np.abs(pg - po).max() <= 1e-9
+    _, exact = H.linear_z_case(n3, d3, g, h, seed=3)
+    inner = (slice(1, n3[2] + 2), slice(1, n3[1] + 2), slice(1, n3[0] + 2))
+    assert np.abs(pg[inner] - exact[inner]).max() < 1e-8
+
+
 @pytest.mark.parametrize("down,up", [(2, 1), (1, 2), (2, 2), (0, 1), (1, 0)])
 def test_conductor_solve_pass_counts_bitwise(device, down, up):
     """downpasses/uppasses other than 1 with the pipe and annulus, cold and

@@ -759,6 +759,50 @@ def test_correctede_empty_parity_list_skips_subgrid_field():
     assert not cf.sub[0]["efield0"].any()
 
 
+NEUMANN_N3, NEUMANN_D3, NEUMANN_GH = (12, 8, 16), (0.1, 0.12, 0.08), (2.0, 25.0)
+
+
+@pytest.mark.parametrize("kind", ["m", "p", "mp", "dm"])
+def test_neumann_sheet_closed_form(kind):
+    """Neumann subgrid points (dels in (-1, 0]): a field constant in x,
+    phi = g + h z, has zero normal derivative on a sheet normal to x, so it
+    solves the subgrid equations of handlesubgrid2's Neumann branches (and
+    the Dirichlet branch at the voltage phi takes there) and condbndymgint's
+    mirror leaves it unchanged; from a perturbed start the solve must
+    return it.  A wrong Neumann coefficient (pp/(0.5+adel), the denominator
+    term 2 adel/(adel+0.5)) or mirror would make the solution vary in x.
+
+    Dirichlet below and Neumann above ("dm"): the reference's denominator
+    term (1 - 2 dels adel - dels)/(dels (adel + dels/2)) (f3d_mgrid.F:2702-
+    2703) against pm = volt/(dels (adel + dels/2)) makes the field constant
+    in x the solution when the surface voltage is phi (1 - dels + dels^2)
+    rather than phi (the term 1 - 2 dels adel - dels^2 would give phi); the
+    test sets that voltage, so it holds the restatement to the reference's
+    form of this branch."""
+    n3, d3 = NEUMANN_N3, NEUMANN_D3
+    g, h = NEUMANN_GH
+    dl = 0.3
+    vf = 1. - dl + dl * dl if kind == "dm" else 1.
+    phi, exact = H.linear_z_case(n3, d3, g, h, seed=3)
+    d = H.neumann_lists(n3, 5, kind, dl, lambda ix, iy, iz: (g + h * iz * d3[2]) * vf)
+    co = H.oracle_conductors(d)
+    rho = np.zeros((n3[2] + 1, n3[1] + 1, n3[0] + 1))
+    it, err = O.multigrid3dsolve(phi, rho, *d3, conductors=co, mgtol=1e-11, mgmaxiters=200)
+    assert err <= 1e-11 and it < 200
+    inner = (slice(1, n3[2] + 2), slice(1, n3[1] + 2), slice(1, n3[0] + 2))
+    assert np.abs(phi[inner] - exact[inner]).max() < 1e-8
+    # the same lists with the sheet's dels made Dirichlet at voltage 0
+    # (a sign error in the branch selection) give a different field
+    d0 = dict(d)
+    for p in range(2):
+        d0["sub%d_dels" % p] = np.abs(d["sub%d_dels" % p])
+        d0["sub%d_volt" % p] = np.zeros_like(d["sub%d_volt" % p])
+    phi0, _ = H.linear_z_case(n3, d3, g, h, seed=3)
+    O.multigrid3dsolve(phi0, rho, *d3, conductors=H.oracle_conductors(d0), mgtol=1e-11,
+                       mgmaxiters=200)
+    assert np.abs(phi0[inner] - exact[inner]).max() > 1e-2
+
+
 def test_fixefield_mesh_aligned_face():
     """A conductor face on a mesh plane (no subgrid points): the centred
     grid E at the face node is half the vacuum field; doubling it at the
